@@ -1,0 +1,106 @@
+/*
+ * mdgpu_c.h -- C ABI of the MI355X multifrontal sparse Cholesky.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (MDGPU_OK == 0) and never exits the process; mdgpu_last_error() returns a
+ * thread-local message for the most recent failure.  Device pointers are
+ * HIP device addresses (hipMalloc or a framework tensor); host pointers are
+ * ordinary memory.  `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Each entry point names the reference interface it replaces.
+ *
+ * Life cycle:
+ *   mdgpu_analyze   -> symbolic handle (host: ordering, trees, front maps)
+ *   mdgpu_numeric_create(sym, precision) -> numeric handle (device arena)
+ *   mdgpu_factorize(num, values)  (repeatable: refactorization, fixed pattern)
+ *   mdgpu_solve / mdgpu_refine / mdgpu_logdet / mdgpu_export_L
+ *   mdgpu_numeric_destroy, mdgpu_symbolic_destroy
+ */
+#ifndef MDGPU_C_H
+#define MDGPU_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDGPU_OK 0
+#define MDGPU_ERR_ARG 1          /* invalid argument or matrix            */
+#define MDGPU_ERR_NOT_SPD 2      /* non-positive pivot (column reported)  */
+#define MDGPU_ERR_DEVICE 3       /* HIP runtime error                     */
+#define MDGPU_ERR_NOMEM 4        /* host or device allocation failed      */
+#define MDGPU_ERR_NOT_CONVERGED 5 /* refinement did not reach tolerance   */
+
+/* Precision of the stored factor and of the dense front kernels. */
+#define MDGPU_FP64 0
+#define MDGPU_FP32 1
+#define MDGPU_BF16 2   /* bf16 MFMA Schur updates, power-of-two front scaling */
+
+/* Node partition: -sn / -rn c / plain flags of the reference driver
+ * (mdgpu.cu:1708-1714). */
+#define MDGPU_PART_PLAIN 0
+#define MDGPU_PART_SUPERNODAL 1
+#define MDGPU_PART_RELAXED 2
+
+typedef struct mdgpu_symbolic mdgpu_symbolic;
+typedef struct mdgpu_numeric mdgpu_numeric;
+
+typedef struct {
+    int use_amd;     /* -amd: cs_amd(1, .) fill-reducing ordering        */
+    int partition;   /* MDGPU_PART_*                                     */
+    int max_zeros;   /* -rn c: relaxed-amalgamation zero budget          */
+} mdgpu_analyze_opts;
+
+/* Most recent error message of the calling thread ("" if none). */
+const char* mdgpu_last_error(void);
+
+/* Symbolic analysis of the lower triangle (diagonal included, rows sorted)
+ * of a symmetric matrix in CSC form.  Replaces the driver's analyze glue
+ * mdgpu.cu:1745-1875 (cs_amd, cs_symperm, cs_etree, cs_post, cs_counts,
+ * GetETree / GetSuperETree / GetRelaxedETree, spL->p) plus the per-node row
+ * merges of MatrizGetFk_3 / VectorMerge (mdgpu.cu:180-291, 535-595). */
+int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,
+                  const mdgpu_analyze_opts* opts, mdgpu_symbolic** sym);
+int mdgpu_symbolic_destroy(mdgpu_symbolic* sym);
+
+/* Scalar facts about an analysis. */
+#define MDGPU_SYM_N 0             /* matrix order                          */
+#define MDGPU_SYM_NNODES 1        /* nodes (fronts) in the partition       */
+#define MDGPU_SYM_LNZ 2           /* entries of L (padded counts)          */
+#define MDGPU_SYM_NNZ_A 3         /* entries of the input lower triangle   */
+#define MDGPU_SYM_QLEN 4          /* total length of all front row lists   */
+#define MDGPU_SYM_MAXFRONT 5      /* largest front order                   */
+#define MDGPU_SYM_NLEVELS 6       /* levels of the front tree              */
+#define MDGPU_SYM_NODES_FLAT 7    /* length of the MDGPU_ARR_NODES array   */
+#define MDGPU_SYM_NSIZES 8
+int mdgpu_symbolic_sizes(const mdgpu_symbolic* sym, int64_t* out /* [MDGPU_SYM_NSIZES] */);
+
+/* Flop counts: [0] = sum_j c_j^2 over the unpadded cs_counts column counts,
+ * [1] = the reference's own sum_j (c_j^2 + 1) over padded counts
+ * (mdgpu.cu:1354-1359). */
+int mdgpu_symbolic_flops(const mdgpu_symbolic* sym, double* out /* [2] */);
+
+/* Integer arrays of the analysis, copied into dst (capacity cap entries). */
+#define MDGPU_ARR_AMD_PERM 0      /* n: cs_amd permutation                 */
+#define MDGPU_ARR_PARENT 1        /* n: cs_etree of the AMD-ordered matrix */
+#define MDGPU_ARR_POST 2          /* n: cs_post                            */
+#define MDGPU_ARR_COUNT 3         /* n: cs_counts                          */
+#define MDGPU_ARR_RELAXED_PERM 4  /* n: GetRelaxedETree perm (relaxed only)*/
+#define MDGPU_ARR_PERM 5          /* n: final permutation (new k = old perm[k]) */
+#define MDGPU_ARR_COUNT_FINAL 6   /* n: padded counts in final order       */
+#define MDGPU_ARR_LP 7            /* n+1: spL->p                           */
+#define MDGPU_ARR_ALOW_P 8        /* n+1: final permuted lower triangle    */
+#define MDGPU_ARR_ALOW_I 9        /* nnz_a                                 */
+#define MDGPU_ARR_NODES 10        /* flat: per node first, ncols, rows,
+                                     zeros, nkids, kid node indices...     */
+#define MDGPU_ARR_FRONT_PARENT 11 /* nnodes: parent node index or -1      */
+#define MDGPU_ARR_Q_OFF 12        /* nnodes+1 (int32; fails past 2^31)     */
+#define MDGPU_ARR_Q 13            /* qlen: sorted row ids of each front    */
+#define MDGPU_ARR_LEVEL 14        /* nnodes: height of each front          */
+int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_t* dst, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDGPU_C_H */

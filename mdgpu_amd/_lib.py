@@ -1,0 +1,83 @@
+"""ctypes binding of the C ABI declared in include/mdgpu_c.h.
+
+The shared library is built in-tree (``mdgpu_amd/libmdgpu_amd.so``) by
+``mdgpu_amd.build``.  Nothing here falls back to another implementation: if
+the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("MDGPU_AMD_LIB", os.path.join(_HERE, "libmdgpu_amd.so"))
+
+MDGPU_OK = 0
+MDGPU_ERR_ARG = 1
+MDGPU_ERR_NOT_SPD = 2
+MDGPU_ERR_DEVICE = 3
+MDGPU_ERR_NOMEM = 4
+MDGPU_ERR_NOT_CONVERGED = 5
+
+FP64, FP32, BF16 = 0, 1, 2
+PART_PLAIN, PART_SUPERNODAL, PART_RELAXED = 0, 1, 2
+
+SYM_N, SYM_NNODES, SYM_LNZ, SYM_NNZ_A, SYM_QLEN, SYM_MAXFRONT, SYM_NLEVELS, SYM_NODES_FLAT = range(8)
+SYM_NSIZES = 8
+
+(ARR_AMD_PERM, ARR_PARENT, ARR_POST, ARR_COUNT, ARR_RELAXED_PERM, ARR_PERM, ARR_COUNT_FINAL,
+ ARR_LP, ARR_ALOW_P, ARR_ALOW_I, ARR_NODES, ARR_FRONT_PARENT, ARR_Q_OFF, ARR_Q, ARR_LEVEL) = range(15)
+
+
+class MdgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[status {code}] {msg}")
+        self.code = code
+
+
+class AnalyzeOpts(C.Structure):
+    _fields_ = [("use_amd", C.c_int), ("partition", C.c_int), ("max_zeros", C.c_int)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); the C ABI surface that Python uses
+_SIGNATURES = {
+    "mdgpu_last_error": (C.c_char_p, []),
+    "mdgpu_analyze": (C.c_int, [C.c_int, _I32P, _I32P, C.POINTER(AnalyzeOpts), C.POINTER(_P)]),
+    "mdgpu_symbolic_destroy": (C.c_int, [_P]),
+    "mdgpu_symbolic_sizes": (C.c_int, [_P, _I64P]),
+    "mdgpu_symbolic_flops": (C.c_int, [_P, _DP]),
+    "mdgpu_symbolic_array": (C.c_int, [_P, C.c_int, _I32P, C.c_int64]),
+}
+
+
+def register(name, restype, argtypes):
+    _SIGNATURES[name] = (restype, argtypes)
+
+
+def lib():
+    """Load the in-tree shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `python -m mdgpu_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != MDGPU_OK:
+        msg = lib().mdgpu_last_error()
+        raise MdgpuError(status, msg.decode() if msg else "")
+    return status

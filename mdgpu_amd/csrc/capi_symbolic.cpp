@@ -1,0 +1,139 @@
+// C ABI: symbolic analysis entry points (see include/mdgpu_c.h).
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "common.h"
+
+namespace mdgpu {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+}  // namespace mdgpu
+
+using mdgpu::fail;
+
+extern "C" const char* mdgpu_last_error(void) { return mdgpu::g_last_error.c_str(); }
+
+extern "C" int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,
+                             const mdgpu_analyze_opts* opts, mdgpu_symbolic** sym) {
+    mdgpu::clear_error();
+    if (!Cp || !Ci || !sym) return fail(MDGPU_ERR_ARG, "mdgpu_analyze: null pointer");
+    *sym = nullptr;
+    mdgpu::AnalyzeOptions o;
+    if (opts) {
+        o.use_amd = opts->use_amd != 0;
+        if (opts->partition < MDGPU_PART_PLAIN || opts->partition > MDGPU_PART_RELAXED)
+            return fail(MDGPU_ERR_ARG, "mdgpu_analyze: unknown partition mode");
+        o.partition = (mdgpu::Partition)opts->partition;
+        o.max_zeros = opts->max_zeros;
+        if (o.max_zeros < 0) return fail(MDGPU_ERR_ARG, "mdgpu_analyze: max_zeros must be >= 0");
+    }
+    mdgpu_symbolic* h = new (std::nothrow) mdgpu_symbolic();
+    if (!h) return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze: out of host memory");
+    try {
+        std::string err = mdgpu::analyze(n, Cp, Ci, o, h->S);
+        if (!err.empty()) {
+            delete h;
+            return fail(MDGPU_ERR_ARG, "mdgpu_analyze: " + err);
+        }
+    } catch (const std::bad_alloc&) {
+        delete h;
+        return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze: out of host memory");
+    }
+    const auto& S = h->S;
+    const int nn = (int)S.nodes.size();
+    h->level.assign(nn, 0);
+    for (int f = 0; f < nn; f++) {
+        int nf = (int)(S.q_off[f + 1] - S.q_off[f]);
+        h->max_front = std::max(h->max_front, nf);
+        for (int kid : S.nodes[f].kids) h->level[f] = std::max(h->level[f], h->level[kid] + 1);
+        h->nlevels = std::max(h->nlevels, h->level[f] + 1);
+    }
+    *sym = h;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_symbolic_destroy(mdgpu_symbolic* sym) {
+    delete sym;
+    return MDGPU_OK;
+}
+
+static int64_t nodes_flat_len(const mdgpu::Symbolic& S) {
+    int64_t len = 0;
+    for (const auto& nd : S.nodes) len += 5 + (int64_t)nd.kids.size();
+    return len;
+}
+
+extern "C" int mdgpu_symbolic_sizes(const mdgpu_symbolic* sym, int64_t* out) {
+    if (!sym || !out) return fail(MDGPU_ERR_ARG, "mdgpu_symbolic_sizes: null pointer");
+    const auto& S = sym->S;
+    out[MDGPU_SYM_N] = S.n;
+    out[MDGPU_SYM_NNODES] = (int64_t)S.nodes.size();
+    out[MDGPU_SYM_LNZ] = S.lnz;
+    out[MDGPU_SYM_NNZ_A] = S.nnz_in;
+    out[MDGPU_SYM_QLEN] = (int64_t)S.q_all.size();
+    out[MDGPU_SYM_MAXFRONT] = sym->max_front;
+    out[MDGPU_SYM_NLEVELS] = sym->nlevels;
+    out[MDGPU_SYM_NODES_FLAT] = nodes_flat_len(S);
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_symbolic_flops(const mdgpu_symbolic* sym, double* out) {
+    if (!sym || !out) return fail(MDGPU_ERR_ARG, "mdgpu_symbolic_flops: null pointer");
+    out[0] = sym->S.flops_counts;
+    out[1] = sym->S.flops_padded;
+    return MDGPU_OK;
+}
+
+template <class V>
+static int copy_out(const V& v, int32_t* dst, int64_t cap, const char* what) {
+    if ((int64_t)v.size() > cap)
+        return fail(MDGPU_ERR_ARG, std::string("mdgpu_symbolic_array: capacity too small for ") + what);
+    for (size_t t = 0; t < v.size(); t++) {
+        if ((int64_t)v[t] > INT32_MAX) return fail(MDGPU_ERR_ARG, std::string(what) + " exceeds int32");
+        dst[t] = (int32_t)v[t];
+    }
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_t* dst, int64_t cap) {
+    if (!sym || !dst) return fail(MDGPU_ERR_ARG, "mdgpu_symbolic_array: null pointer");
+    const auto& S = sym->S;
+    switch (which) {
+        case MDGPU_ARR_AMD_PERM: return copy_out(S.amd_perm, dst, cap, "amd_perm");
+        case MDGPU_ARR_PARENT: return copy_out(S.parent, dst, cap, "parent");
+        case MDGPU_ARR_POST: return copy_out(S.post, dst, cap, "post");
+        case MDGPU_ARR_COUNT: return copy_out(S.count, dst, cap, "count");
+        case MDGPU_ARR_RELAXED_PERM: return copy_out(S.relaxed_perm, dst, cap, "relaxed_perm");
+        case MDGPU_ARR_PERM: return copy_out(S.perm, dst, cap, "perm");
+        case MDGPU_ARR_COUNT_FINAL: return copy_out(S.count_final, dst, cap, "count_final");
+        case MDGPU_ARR_LP: return copy_out(S.lp, dst, cap, "lp");
+        case MDGPU_ARR_ALOW_P: return copy_out(S.a_low.p, dst, cap, "alow_p");
+        case MDGPU_ARR_ALOW_I: return copy_out(S.a_low.i, dst, cap, "alow_i");
+        case MDGPU_ARR_NODES: {
+            std::vector<int> flat;
+            flat.reserve((size_t)nodes_flat_len(S));
+            for (const auto& nd : S.nodes) {
+                flat.push_back(nd.first);
+                flat.push_back(nd.ncols);
+                flat.push_back(nd.rows);
+                flat.push_back(nd.zeros);
+                flat.push_back((int)nd.kids.size());
+                flat.insert(flat.end(), nd.kids.begin(), nd.kids.end());
+            }
+            return copy_out(flat, dst, cap, "nodes");
+        }
+        case MDGPU_ARR_FRONT_PARENT: return copy_out(S.front_parent, dst, cap, "front_parent");
+        case MDGPU_ARR_Q_OFF: return copy_out(S.q_off, dst, cap, "q_off");
+        case MDGPU_ARR_Q: return copy_out(S.q_all, dst, cap, "q");
+        case MDGPU_ARR_LEVEL: return copy_out(sym->level, dst, cap, "level");
+        default: return fail(MDGPU_ERR_ARG, "mdgpu_symbolic_array: unknown array id");
+    }
+}

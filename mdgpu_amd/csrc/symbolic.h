@@ -1,0 +1,87 @@
+// Host symbolic analysis for the multifrontal Cholesky: fill-reducing
+// ordering, elimination tree, postorder, column counts, node partition
+// (plain columns, fundamental supernodes, relaxed supernodes) and the front
+// structure the numeric phase runs on.  Every result matches the reference's
+// symbolic phase bit for bit (CSparse 2.2.3 as vendored in cs.cpp, and
+// etree.cpp); see the per-function citations.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mdgpu {
+
+// Compressed-sparse-column pattern (+ optional values), int32 indices.
+struct Csc {
+    int n = 0;
+    std::vector<int> p;   // n+1
+    std::vector<int> i;   // p[n]
+    int nnz() const { return p.empty() ? 0 : p[n]; }
+};
+
+// One node of a partition tree (a set of consecutive pivot columns after the
+// final permutation).  Mirrors struct ETree (etree.h:6-19) with indices in
+// place of pointers.
+struct PartNode {
+    int first = 0;              // ETree::nodo: first column of the node
+    int ncols = 1;              // ETree::cols
+    int rows = 0;               // ETree::rows
+    int zeros = 0;              // ETree::ceros (relaxed padding)
+    std::vector<int> cols;      // ETree::nodos (relaxed: original column ids)
+    std::vector<int> kids;      // child node indices into the node list
+};
+
+enum Partition { PART_PLAIN = 0, PART_SUPERNODAL = 1, PART_RELAXED = 2 };
+
+struct AnalyzeOptions {
+    bool use_amd = true;
+    Partition partition = PART_PLAIN;
+    int max_zeros = 0;
+};
+
+// Everything the symbolic phase produces.
+struct Symbolic {
+    int n = 0;
+    int nnz_in = 0;                  // entries of the caller's lower triangle
+    AnalyzeOptions opts;
+    // reference-visible intermediates
+    std::vector<int> amd_perm;       // cs_amd result (identity without AMD)
+    std::vector<int> parent, post, count;   // on the AMD-permuted matrix
+    std::vector<int> relaxed_perm;   // GetRelaxedETree perm (relaxed only)
+    std::vector<int> perm;           // final: new column k is original perm[k]
+    std::vector<int> pinv;           // inverse of perm
+    std::vector<int> count_final;    // (padded) L column counts, final order
+    std::vector<int> lp;             // L column pointers = cumsum(count_final)
+    Csc a_low;                       // final permuted lower triangle pattern
+    std::vector<int> a_src;          // a_low entry -> caller entry index
+    std::vector<PartNode> nodes;     // processing (post)order, root last
+    // front structure (one front per node, same indexing)
+    std::vector<int> front_parent;   // -1 for the root(s)
+    std::vector<int64_t> q_off;      // offsets into q_all (nnodes+1)
+    std::vector<int> q_all;          // sorted global row ids of each front
+    double flops_counts = 0;         // sum_j c_j^2 over unpadded cs_counts
+    double flops_padded = 0;         // reference formula sum (c_j^2 + 1), padded
+    int64_t lnz = 0;
+};
+
+// cs_transpose (cs.cpp:407-429) on patterns.
+Csc transpose(const Csc& a);
+
+// cs_amd(1, A) (cs.cpp:645-991) on the upper triangle of a symmetric matrix.
+std::vector<int> amd_order(const Csc& upper);
+
+// cs_etree(A, 0) (cs.cpp:480-507), A = full symmetric pattern.
+std::vector<int> etree(const Csc& full);
+// cs_post (cs.cpp:432-477).
+std::vector<int> postorder(const std::vector<int>& parent);
+// cs_counts(A, parent, post, 0) (cs.cpp:338-404).
+std::vector<int> column_counts(const Csc& full, const std::vector<int>& parent,
+                               const std::vector<int>& post);
+
+// Full analysis (mdgpu.cu:1745-1875 driver glue + front row lists).
+// Returns an empty string on success, else an error message.
+std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts,
+                    Symbolic& out);
+
+}  // namespace mdgpu

@@ -1,0 +1,86 @@
+"""Host mirror of the reference's symbolic analysis, over the C ABI.
+
+``analyze(p, i, amd=True, mode="rn", max_zeros=c)`` corresponds to the
+reference driver's flags ``-amd`` / ``-sn`` / ``-rn c`` (mdgpu.cu:1697-1722)
+and returns a :class:`Symbolic` whose arrays are the reference's
+intermediates (``cs_amd`` permutation, ``cs_etree`` parent, ``cs_post``,
+``cs_counts``, node lists of GetETree / GetSuperETree / GetRelaxedETree,
+``spL->p``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+_MODES = {"plain": L.PART_PLAIN, "sn": L.PART_SUPERNODAL, "rn": L.PART_RELAXED}
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Symbolic:
+    def __init__(self, handle, keep):
+        self._h = handle
+        self._keep = keep
+        sizes = (C.c_int64 * L.SYM_NSIZES)()
+        L.check(L.lib().mdgpu_symbolic_sizes(self._h, sizes))
+        self.sizes = list(sizes)
+        self.n = sizes[L.SYM_N]
+        self.nnodes = sizes[L.SYM_NNODES]
+        self.lnz = sizes[L.SYM_LNZ]
+        self.nnz_a = sizes[L.SYM_NNZ_A]
+        self.max_front = sizes[L.SYM_MAXFRONT]
+        self.nlevels = sizes[L.SYM_NLEVELS]
+        fl = (C.c_double * 2)()
+        L.check(L.lib().mdgpu_symbolic_flops(self._h, fl))
+        self.flops, self.flops_padded = fl[0], fl[1]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def array(self, which):
+        n, nn = self.n, self.nnodes
+        length = {
+            L.ARR_LP: n + 1, L.ARR_ALOW_P: n + 1, L.ARR_ALOW_I: self.nnz_a,
+            L.ARR_NODES: self.sizes[L.SYM_NODES_FLAT], L.ARR_FRONT_PARENT: nn,
+            L.ARR_Q_OFF: nn + 1, L.ARR_Q: self.sizes[L.SYM_QLEN], L.ARR_LEVEL: nn,
+        }.get(which, n)
+        out = np.empty(max(length, 1), dtype=np.int32)
+        L.check(L.lib().mdgpu_symbolic_array(self._h, which, out.ctypes.data_as(L._I32P), length))
+        return out[:length]
+
+    def nodes(self):
+        """List of (first, ncols, rows, zeros, kids) in processing order."""
+        flat = self.array(L.ARR_NODES)
+        out, t = [], 0
+        while t < len(flat):
+            first, ncols, rows, zeros, nk = flat[t:t + 5]
+            out.append((int(first), int(ncols), int(rows), int(zeros), [int(k) for k in flat[t + 5:t + 5 + nk]]))
+            t += 5 + nk
+        return out
+
+    def close(self):
+        if self._h:
+            L.lib().mdgpu_symbolic_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def analyze(p, i, amd=True, mode="plain", max_zeros=0):
+    p, i = _i32(p), _i32(i)
+    n = len(p) - 1
+    opts = L.AnalyzeOpts(int(bool(amd)), _MODES[mode], int(max_zeros))
+    h = C.c_void_p()
+    L.check(L.lib().mdgpu_analyze(n, p.ctypes.data_as(L._I32P), i.ctypes.data_as(L._I32P),
+                                  C.byref(opts), C.byref(h)))
+    return Symbolic(h, (p, i))

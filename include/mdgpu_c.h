@@ -99,6 +99,64 @@ int mdgpu_symbolic_flops(const mdgpu_symbolic* sym, double* out /* [2] */);
 #define MDGPU_ARR_LEVEL 14        /* nnodes: height of each front          */
 int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_t* dst, int64_t cap);
 
+
+/* ------------------------------------------------------------ numeric */
+
+/* Allocate the device arena for a numeric factorization of `sym` (L store,
+ * update-matrix arena, maps and schedule uploaded once).  Replaces the
+ * per-node device allocation and map uploads of the reference
+ * (AllocFrenteGPU / MoverFrenteQlAGPU, mdgpu.cu:597-723).  precision is
+ * MDGPU_FP64 or MDGPU_FP32 (storage and arithmetic of the dense fronts). */
+int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, mdgpu_numeric** num);
+int mdgpu_numeric_destroy(mdgpu_numeric* num);
+
+#define MDGPU_NUM_DEVICE_BYTES 0  /* device memory held by the handle     */
+#define MDGPU_NUM_LSTORE 1        /* elements of the L store               */
+#define MDGPU_NUM_ARENA 2         /* elements of the update arena (peak)   */
+#define MDGPU_NUM_STEPS 3         /* kernel launches of one factorization  */
+#define MDGPU_NUM_LEVELS 4        /* levels of the schedule                */
+#define MDGPU_NUM_PANEL_FRONTS 5  /* fronts factored wholly in LDS         */
+#define MDGPU_NUM_DIAG_JOBS 6     /* diagonal blocks of the blocked path   */
+#define MDGPU_NUM_PRECISION 7
+#define MDGPU_NUM_NINFO 8
+int mdgpu_numeric_info(const mdgpu_numeric* num, int64_t* out /* [MDGPU_NUM_NINFO] */);
+
+/* Host only (no device calls): build the schedule mdgpu_numeric_create
+ * would build for `precision`, check every kernel job against its buffers,
+ * and report the same figures as mdgpu_numeric_info (DEVICE_BYTES is the
+ * planned device footprint).  No counterpart in the reference. */
+int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_t* out /* [MDGPU_NUM_NINFO] */);
+
+/* Numeric factorization.  d_ax: DEVICE pointer to the values of the
+ * caller's lower triangle, in the order of the Ci array given to
+ * mdgpu_analyze (FP64).  Replaces Factor(A, spL, listaNodos, cantNodos),
+ * mdgpu.cu:1394-1608 (front assembly, extend-add, partial factorization of
+ * every front).  Returns MDGPU_ERR_NOT_SPD on a non-positive pivot.
+ * Synchronizes `stream` at the end (pivot check). */
+int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream);
+
+/* x = A^{-1} b with the stored factor (device vectors, FP64, caller's
+ * numbering): permutation, supernodal forward and backward substitution.
+ * Replaces cs_pvec + cs_lsolve + cs_ltsolve + cs_ipvec (cs.cpp:1086-1146,
+ * dead call site mdgpu.cu:1953-1960).  Asynchronous on `stream`. */
+int mdgpu_solve(mdgpu_numeric* num, const double* d_b, double* d_x, void* stream);
+
+/* Iterative refinement in FP64 until the backward error
+ * ||b - A x||_inf / (||A||_1 ||x||_inf + ||b||_inf) (mdgpu.cu:1972) is at
+ * most tol, or maxit corrections were applied.  The residual uses the
+ * values last passed to mdgpu_factorize.  Returns MDGPU_ERR_NOT_CONVERGED
+ * (with eta/iters filled) if tol was not reached. */
+int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol, int maxit,
+                 double* eta, int* iters, void* stream);
+
+/* log det A = 2 sum_j log L_jj (no counterpart in the reference). */
+int mdgpu_logdet(mdgpu_numeric* num, double* out, void* stream);
+
+/* Copy L to host CSC in the reference's spL layout: column j holds the rows
+ * of its front from j down, diagonal first, ascending (mdgpu.cu:1346-1352,
+ * 1859-1873).  Lp has n+1 entries, Li/Lx have lnz; any may be NULL. */
+int mdgpu_export_L(mdgpu_numeric* num, int32_t* Lp, int32_t* Li, double* Lx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,13 @@ def lib():
     """Load the in-tree shared library (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # torch bundles a HIP runtime with the same SONAME as ROCm's; loading
+        # it first makes this library bind to that one runtime, so device
+        # pointers and stream handles from torch are valid here.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} is missing: build it with `python -m mdgpu_amd.build`")
         L = C.CDLL(LIB_PATH)
@@ -81,3 +88,18 @@ def check(status):
         msg = lib().mdgpu_last_error()
         raise MdgpuError(status, msg.decode() if msg else "")
     return status
+
+NUM_DEVICE_BYTES, NUM_LSTORE, NUM_ARENA, NUM_STEPS, NUM_LEVELS, NUM_PANEL_FRONTS, NUM_DIAG_JOBS, NUM_PRECISION = range(8)
+NUM_NINFO = 8
+
+_SIGNATURES.update({
+    "mdgpu_numeric_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "mdgpu_numeric_destroy": (C.c_int, [_P]),
+    "mdgpu_numeric_info": (C.c_int, [_P, _I64P]),
+    "mdgpu_plan_check": (C.c_int, [_P, C.c_int, _I64P]),
+    "mdgpu_factorize": (C.c_int, [_P, _P, _P]),
+    "mdgpu_solve": (C.c_int, [_P, _P, _P, _P]),
+    "mdgpu_refine": (C.c_int, [_P, _P, _P, C.c_double, C.c_int, _DP, C.POINTER(C.c_int), _P]),
+    "mdgpu_logdet": (C.c_int, [_P, _DP, _P]),
+    "mdgpu_export_L": (C.c_int, [_P, _I32P, _I32P, _DP]),
+})

@@ -1,0 +1,40 @@
+// C ABI: host-only planning entry point (no device calls), used to size a
+// factorization and to check its schedule before any kernel runs.
+#include "common.h"
+#include "plan.h"
+
+using mdgpu::fail;
+
+namespace mdgpu {
+PlanOptions plan_options_for(int precision) {
+    PlanOptions po;
+    po.elem_bytes = precision == MDGPU_FP64 ? 8 : 4;
+    po.panel_lds_bytes = 61440;
+    return po;
+}
+}  // namespace mdgpu
+
+extern "C" int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_t* out) {
+    mdgpu::clear_error();
+    if (!sym || !out) return fail(MDGPU_ERR_ARG, "mdgpu_plan_check: null pointer");
+    if (precision != MDGPU_FP64 && precision != MDGPU_FP32)
+        return fail(MDGPU_ERR_ARG, "mdgpu_plan_check: precision must be MDGPU_FP64 or MDGPU_FP32");
+    mdgpu::PlanOptions po = mdgpu::plan_options_for(precision);
+    mdgpu::Plan P;
+    std::string err = mdgpu::build_plan(sym->S, po, P);
+    if (err.empty()) err = mdgpu::validate_plan(P, po);
+    if (!err.empty()) return fail(MDGPU_ERR_ARG, "mdgpu_plan_check: " + err);
+    const int64_t es = po.elem_bytes;
+    int64_t bytes = (P.lstore + P.arena + P.inv_scratch) * es + P.solve_u * 8 +
+                    (int64_t)P.a_dst.size() * 8 + (int64_t)(P.rel.size() + P.q.size()) * 4 +
+                    (int64_t)P.afull_i.size() * 16;
+    out[MDGPU_NUM_DEVICE_BYTES] = bytes;
+    out[MDGPU_NUM_LSTORE] = P.lstore;
+    out[MDGPU_NUM_ARENA] = P.arena;
+    out[MDGPU_NUM_STEPS] = (int64_t)P.steps.size();
+    out[MDGPU_NUM_LEVELS] = P.nlevels;
+    out[MDGPU_NUM_PANEL_FRONTS] = (int64_t)P.panel.size();
+    out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.diag.size();
+    out[MDGPU_NUM_PRECISION] = precision;
+    return MDGPU_OK;
+}

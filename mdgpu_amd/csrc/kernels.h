@@ -1,0 +1,48 @@
+// Launch wrappers of the device kernels (kernels.hip).  All take the stream
+// they run on; none synchronizes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "plan.h"
+
+namespace mdgpu {
+
+template <typename T> void launch_fill(T* p, int64_t n, T v, hipStream_t s);
+template <typename T> void launch_scatter_a(T* L, const double* ax, const int64_t* dst, int nnz, hipStream_t s);
+template <typename T>
+void launch_extend_add(const EaItem* items, int count, const FrontDesc* fronts, const int* rel, T* L, T* U,
+                       hipStream_t s);
+template <typename T>
+void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,
+                  hipStream_t s);
+template <typename T>
+void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s);
+template <typename T>
+void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv, hipStream_t s);
+template <typename T>
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, T* L, T* U, hipStream_t s);
+
+// solves (compute in FP64, factor stored as T)
+void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s);
+void launch_permute_out(const double* xp, const int* perm, double* x, int n, hipStream_t s);
+template <typename T>
+void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
+                const int* q, const T* L, double* xp, double* su, const int64_t* suofs, hipStream_t s);
+template <typename T>
+void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, double* xp,
+                hipStream_t s);
+
+// refinement helpers (FP64)
+void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);
+void launch_residual(const int* rp, const int* ci, const double* av, const double* x, const double* b, double* r,
+                     int n, hipStream_t s);
+void launch_rowabs(const int* rp, const double* av, double* out, int n, hipStream_t s);
+void launch_maxabs(const double* v, int n, unsigned long long* out, hipStream_t s);
+void launch_axpy(double* x, const double* d, int n, hipStream_t s);
+template <typename T>
+void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial, int nparts, hipStream_t s);
+
+}  // namespace mdgpu

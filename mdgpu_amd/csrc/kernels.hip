@@ -1,0 +1,578 @@
+// Device kernels of the multifrontal factorization and solves (gfx950).
+//
+// Conventions: fronts are column-major; the panel P_f of front f has leading
+// dimension n_f; update matrices U_f have leading dimension m_f.  A
+// workgroup is 256 lanes = four 64-wide wavefronts.  No kernel uses atomics
+// on floating-point data: every summation order is fixed by the schedule.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "kernels.h"
+
+namespace mdgpu {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline double to_d(double v) { return v; }
+__device__ inline double to_d(float v) { return (double)v; }
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------- fill / A
+template <typename T>
+__global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// front assembly from A: every caller entry lands at a precomputed panel
+// offset (the value scatter of MatrizGetFk_3, mdgpu.cu:272-282)
+template <typename T>
+__global__ void scatter_a_kernel(T* __restrict__ L, const double* __restrict__ ax,
+                                 const int64_t* __restrict__ dst, int nnz) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz) L[dst[e]] = (T)ax[e];
+}
+
+// -------------------------------------------------------------- extend-add
+// parent(rel[i], rel[j]) += U_child(i, j) over the lower triangle of the
+// columns [j0, j1) of one child (ExtendAdd_CPU / ExtendAdd_GPU_device,
+// mdgpu.cu:748-788).  A column lands either in a pivot column of the
+// parent's panel or in the parent's update matrix.  One wavefront per
+// column, lanes over rows (coalesced reads; writes follow the increasing
+// relative map).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void extend_add_kernel(const EaItem* __restrict__ items,
+                                                              const FrontDesc* __restrict__ fronts,
+                                                              const int* __restrict__ rel, T* __restrict__ L,
+                                                              T* __restrict__ U) {
+    const EaItem it = items[blockIdx.x];
+    const FrontDesc cd = fronts[it.child];
+    const FrontDesc pd = fronts[cd.parent];
+    const int* __restrict__ r = rel + cd.relofs;
+    const T* __restrict__ src = U + cd.uofs;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j = it.j0 + wave; j < it.j1; j += kThreads / 64) {
+        const int pj = r[j];
+        T* dcol;
+        int rowoff;
+        if (pj < pd.c) {
+            dcol = L + pd.lofs + (int64_t)pj * pd.n;
+            rowoff = 0;
+        } else {
+            dcol = U + pd.uofs + (int64_t)(pj - pd.c) * pd.m;
+            rowoff = pd.c;
+        }
+        const T* scol = src + (int64_t)j * cd.m;
+        for (int i = j + lane; i < cd.m; i += 64) dcol[r[i] - rowoff] += scol[i];
+    }
+}
+
+// ------------------------------------------------------ dense: LDS panels
+// Partial Cholesky of a whole panel held in LDS: for each pivot column,
+// sqrt, scale, rank-1 update of the remaining pivot columns (the unblocked
+// scal+syr loop of FactorAux_CPU_3, mdgpu.cu:1318-1326, applied to the full
+// n_f rows, which also performs the trsm of :1330).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void panel_kernel(const int* __restrict__ list,
+                                                         const FrontDesc* __restrict__ fronts,
+                                                         T* __restrict__ L, int* __restrict__ info) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* P = reinterpret_cast<T*>(smem_raw);
+    __shared__ T pivinv;
+    const FrontDesc d = fronts[list[blockIdx.x]];
+    const int n = d.n, c = d.c;
+    T* __restrict__ g = L + d.lofs;
+    const int tot = n * c;
+    for (int t = threadIdx.x; t < tot; t += kThreads) P[t] = g[t];
+    __syncthreads();
+    for (int j = 0; j < c; ++j) {
+        if (threadIdx.x == 0) {
+            T dj = P[j + j * n];
+            if (!(dj > T(0))) atomicMin(info, d.k + j);
+            T rt = sqrt(dj);
+            P[j + j * n] = rt;
+            pivinv = T(1) / rt;
+        }
+        __syncthreads();
+        const T s = pivinv;
+        for (int i = j + 1 + threadIdx.x; i < n; i += kThreads) P[i + j * n] *= s;
+        __syncthreads();
+        for (int i = j + 1 + threadIdx.x; i < n; i += kThreads) {
+            const T lij = P[i + j * n];
+            const int cmax = min(i, c - 1);
+            for (int col = j + 1; col <= cmax; ++col) P[i + col * n] -= lij * P[col + j * n];
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < tot; t += kThreads) g[t] = P[t];
+}
+
+// ----------------------------------------------- dense: blocked panel path
+// Diagonal block b x b: Cholesky in LDS, then its triangular inverse (column
+// j of inv(L_D) by forward substitution on lane j), used by the TRSM below.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
+                                                        T* __restrict__ inv, int* __restrict__ info) {
+    __shared__ T D[64 * 65];
+    __shared__ T Q[64 * 65];
+    __shared__ T pivinv;
+    const DiagJob jb = jobs[blockIdx.x];
+    const int b = jb.b, ld = jb.ld;
+    T* __restrict__ g = L + jb.p_off;
+    for (int t = threadIdx.x; t < b * b; t += kThreads) {
+        int i = t % b, j = t / b;
+        D[i + j * 65] = g[i + (int64_t)j * ld];
+    }
+    __syncthreads();
+    for (int j = 0; j < b; ++j) {
+        if (threadIdx.x == 0) {
+            T dj = D[j + j * 65];
+            if (!(dj > T(0))) atomicMin(info, jb.col0 + j);
+            T rt = sqrt(dj);
+            D[j + j * 65] = rt;
+            pivinv = T(1) / rt;
+        }
+        __syncthreads();
+        const T s = pivinv;
+        for (int i = j + 1 + threadIdx.x; i < b; i += kThreads) D[i + j * 65] *= s;
+        __syncthreads();
+        for (int i = j + 1 + threadIdx.x; i < b; i += kThreads) {
+            const T lij = D[i + j * 65];
+            for (int col = j + 1; col <= i; ++col) D[i + col * 65] -= lij * D[col + j * 65];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < b) {
+        const int j = threadIdx.x;
+        for (int i = 0; i < j; ++i) Q[i + j * 65] = T(0);
+        Q[j + j * 65] = T(1) / D[j + j * 65];
+        for (int i = j + 1; i < b; ++i) {
+            T s = T(0);
+            for (int l = j; l < i; ++l) s += D[i + l * 65] * Q[l + j * 65];
+            Q[i + j * 65] = -s / D[i + i * 65];
+        }
+    }
+    __syncthreads();
+    T* __restrict__ q = inv + jb.inv_off;
+    for (int t = threadIdx.x; t < b * b; t += kThreads) {
+        int i = t % b, j = t / b;
+        if (i >= j) g[i + (int64_t)j * ld] = D[i + j * 65];
+        q[i + j * b] = Q[i + j * 65];
+    }
+}
+
+__device__ inline int find_job_trsm(const TrsmJob* jobs, int njobs, int bid) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// R <- R * inv(L_D)^T for a 64-row tile of the rows below a diagonal block.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmJob* __restrict__ jobs, int njobs,
+                                                        T* __restrict__ L, const T* __restrict__ inv) {
+    __shared__ T R[64 * 64];
+    __shared__ T Q[64 * 65];
+    const TrsmJob jb = jobs[find_job_trsm(jobs, njobs, blockIdx.x)];
+    const int tile = blockIdx.x - jb.tile_base;
+    const int r0 = tile * 64;
+    const int rows = min(64, jb.rows - r0);
+    const int b = jb.b, ld = jb.ld;
+    T* __restrict__ g = L + jb.p_off + r0;
+    const T* __restrict__ q = inv + jb.inv_off;
+    for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
+        int i = t & 63, l = t >> 6;
+        R[i + l * 64] = i < rows ? g[i + (int64_t)l * ld] : T(0);
+    }
+    for (int t = threadIdx.x; t < b * b; t += kThreads) {
+        int i = t % b, j = t / b;
+        Q[i + j * 65] = q[i + j * b];
+    }
+    __syncthreads();
+    const int i = threadIdx.x & 63;
+    for (int j = threadIdx.x >> 6; j < b; j += kThreads / 64) {
+        T acc = T(0);
+        for (int l = 0; l <= j; ++l) acc += R[i + l * 64] * Q[j + l * 65];
+        if (i < rows) g[i + (int64_t)j * ld] = acc;
+    }
+}
+
+// C -= A * B^T on 64x64 tiles (beta0: C = -A * B^T), optionally keeping only
+// the lower triangle; tiles of many jobs (fronts) in one launch.  This is the
+// trailing-matrix gemm of FactorAux_CPU_3 / FactorAux_GPU_2 (mdgpu.cu:1336,
+// :1058) restricted to the lower triangle.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gemm_nt_kernel(const GemmJob* __restrict__ jobs, int njobs,
+                                                           T* __restrict__ L, T* __restrict__ U) {
+    __shared__ T As[16][64 + 1];
+    __shared__ T Bs[16][64 + 1];
+    int lo = 0, hi = njobs - 1;
+    const int bid = blockIdx.x;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
+    }
+    const GemmJob g = jobs[lo];
+    int local = bid - g.tile_base;
+    int ti, tj;
+    if (g.lower) {
+        tj = 0;
+        int cnt = g.tiles_m;
+        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+        ti = tj + local;
+    } else {
+        tj = local / g.tiles_m;
+        ti = local % g.tiles_m;
+    }
+    const T* __restrict__ A = L + g.a_off;
+    const T* __restrict__ B = L + g.b_off;
+    T* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+    const int row0 = ti * 64, col0 = tj * 64;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    T acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc[r][s] = T(0);
+    const int li = threadIdx.x & 63, lk = threadIdx.x >> 6;
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int kk = lk + 4 * s;
+            const int k = k0 + kk;
+            const int ia = row0 + li, jb = col0 + li;
+            As[kk][li] = (k < g.K && ia < g.M) ? A[ia + (int64_t)k * g.lda] : T(0);
+            Bs[kk][li] = (k < g.K && jb < g.N) ? B[jb + (int64_t)k * g.ldb] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            T a[4], bb[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = As[kk][tx + 16 * r];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) bb[s] = Bs[kk][ty + 16 * s];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) acc[r][s] += a[r] * bb[s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int j = col0 + ty + 16 * s;
+        if (j >= g.N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = row0 + tx + 16 * r;
+            if (i >= g.M || (g.lower && i < j)) continue;
+            T* cp = C + i + (int64_t)j * g.ldc;
+            *cp = g.beta0 ? -acc[r][s] : *cp - acc[r][s];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ solves
+__global__ void permute_in_kernel(const double* __restrict__ b, const int* __restrict__ perm,
+                                  double* __restrict__ xp, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) xp[k] = b[perm[k]];
+}
+__global__ void permute_out_kernel(const double* __restrict__ xp, const int* __restrict__ perm,
+                                   double* __restrict__ x, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[perm[k]] = xp[k];
+}
+
+// Forward substitution L y = x over one level, one workgroup per front,
+// multifrontal form of cs_lsolve (cs.cpp:1086-1101): gather the children's
+// update vectors through their relative maps, solve the diagonal blocks
+// 64 columns at a time on one wavefront, update the rows below.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ list,
+                                                       const FrontDesc* __restrict__ fronts,
+                                                       const int* __restrict__ kids, const int* __restrict__ rel,
+                                                       const int* __restrict__ q, const T* __restrict__ L,
+                                                       double* __restrict__ xp, double* __restrict__ su,
+                                                       const int64_t* __restrict__ suofs) {
+    __shared__ double D[64 * 65];
+    __shared__ double yb[64];
+    const int f = list[blockIdx.x];
+    const FrontDesc d = fronts[f];
+    const int n = d.n, c = d.c, m = d.m, k = d.k;
+    const T* __restrict__ P = L + d.lofs;
+    double* __restrict__ u = su + suofs[f];
+    for (int t = threadIdx.x; t < m; t += kThreads) u[t] = 0.0;
+    __syncthreads();
+    for (int t = d.kid_begin; t < d.kid_end; ++t) {
+        const int ch = kids[t];
+        const FrontDesc cd = fronts[ch];
+        const int* __restrict__ r = rel + cd.relofs;
+        const double* __restrict__ cu = su + suofs[ch];
+        for (int i = threadIdx.x; i < cd.m; i += kThreads) {
+            const int pos = r[i];
+            if (pos < c) xp[k + pos] += cu[i];
+            else u[pos - c] += cu[i];
+        }
+        __syncthreads();
+    }
+    for (int j0 = 0; j0 < c; j0 += 64) {
+        const int b = min(64, c - j0);
+        for (int t = threadIdx.x; t < b * b; t += kThreads) {
+            int i = t % b, jj = t / b;
+            D[i + jj * 65] = to_d(P[(j0 + i) + (int64_t)(j0 + jj) * n]);
+        }
+        for (int t = threadIdx.x; t < b; t += kThreads) yb[t] = xp[k + j0 + t];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            double xv = lane < b ? yb[lane] : 0.0;
+            for (int jj = 0; jj < b; ++jj) {
+                const double yj = __shfl(xv, jj, 64) / D[jj + jj * 65];
+                if (lane == jj) xv = yj;
+                else if (lane > jj && lane < b) xv -= D[lane + jj * 65] * yj;
+            }
+            if (lane < b) yb[lane] = xv;
+        }
+        __syncthreads();
+        for (int i = j0 + b + threadIdx.x; i < n; i += kThreads) {
+            double acc = 0.0;
+            for (int jj = 0; jj < b; ++jj) acc += to_d(P[i + (int64_t)(j0 + jj) * n]) * yb[jj];
+            if (i < c) xp[k + i] -= acc;
+            else u[i - c] -= acc;
+        }
+        for (int t = threadIdx.x; t < b; t += kThreads) xp[k + j0 + t] = yb[t];
+        __syncthreads();
+    }
+}
+
+// Backward substitution L^T x = y over one level (cs_ltsolve, cs.cpp:1104-1119),
+// one workgroup per front, diagonal blocks from the last to the first; the
+// rows below each block are read through the front's row list q.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ list,
+                                                       const FrontDesc* __restrict__ fronts,
+                                                       const int* __restrict__ qall, const T* __restrict__ L,
+                                                       double* __restrict__ xp) {
+    __shared__ double D[64 * 65];
+    __shared__ double vb[64];
+    const FrontDesc d = fronts[list[blockIdx.x]];
+    const int n = d.n, c = d.c, k = d.k;
+    const T* __restrict__ P = L + d.lofs;
+    const int* __restrict__ q = qall + d.qofs;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nblk = (c + 63) / 64;
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+        const int j0 = blk * 64, j1 = min(c, j0 + 64), b = j1 - j0;
+        for (int j = j0 + wave; j < j1; j += kThreads / 64) {
+            double s = 0.0;
+            const T* col = P + (int64_t)j * n;
+            for (int i = j1 + lane; i < n; i += 64) s += to_d(col[i]) * xp[q[i]];
+            s = wave_sum(s);
+            if (lane == 0) vb[j - j0] = xp[k + j] - s;
+        }
+        for (int t = threadIdx.x; t < b * b; t += kThreads) {
+            int i = t % b, jj = t / b;
+            D[i + jj * 65] = to_d(P[(j0 + i) + (int64_t)(j0 + jj) * n]);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            double v = lane < b ? vb[lane] : 0.0;
+            for (int jj = b - 1; jj >= 0; --jj) {
+                const double xj = __shfl(v, jj, 64) / D[jj + jj * 65];
+                if (lane == jj) v = xj;
+                else if (lane < jj) v -= D[jj + lane * 65] * xj;
+            }
+            if (lane < b) xp[k + j0 + lane] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------- refinement
+__global__ void gather_kernel(const double* __restrict__ vals, const int* __restrict__ src,
+                              double* __restrict__ dst, int nnz) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nnz) dst[t] = vals[src[t]];
+}
+
+// r = b - A x, one row per thread, FP64 (cs_gaxpy, cs.cpp:1069-1083, on the
+// full symmetric matrix)
+__global__ void residual_kernel(const int* __restrict__ rp, const int* __restrict__ ci,
+                                const double* __restrict__ av, const double* __restrict__ x,
+                                const double* __restrict__ b, double* __restrict__ r, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) s += av[p] * x[ci[p]];
+    r[i] = b[i] - s;
+}
+
+__global__ void rowabs_kernel(const int* __restrict__ rp, const double* __restrict__ av,
+                              double* __restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int p = rp[i]; p < rp[i + 1]; ++p) s += fabs(av[p]);
+    out[i] = s;
+}
+
+// max |v| via integer max on the bit patterns of non-negative doubles
+__global__ __launch_bounds__(kThreads) void maxabs_kernel(const double* __restrict__ v, int n,
+                                                          unsigned long long* __restrict__ out) {
+    __shared__ double red[kThreads / 64];
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a = fabs(v[i]);
+        if (a > m || a != a) m = (a != a) ? __builtin_huge_val() : a;
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, red[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void axpy_kernel(double* __restrict__ x, const double* __restrict__ d, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+// partial sums of log(L_jj), fixed reduction tree per block
+template <typename T>
+__global__ __launch_bounds__(kThreads) void logdiag_kernel(const int64_t* __restrict__ diag_off,
+                                                           const T* __restrict__ L, int n,
+                                                           double* __restrict__ partial) {
+    __shared__ double red[kThreads];
+    double s = 0.0;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        s += log(to_d(L[diag_off[j]]));
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+inline int blocks_for(int64_t n, int per = kThreads) { return (int)((n + per - 1) / per); }
+
+}  // namespace
+
+template <typename T> void launch_fill(T* p, int64_t n, T v, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + kThreads - 1) / kThreads;
+    if (blocks > 65536) blocks = 65536;
+    fill_kernel<T><<<(int)blocks, kThreads, 0, s>>>(p, n, v);
+}
+template <typename T> void launch_scatter_a(T* L, const double* ax, const int64_t* dst, int nnz, hipStream_t s) {
+    if (nnz > 0) scatter_a_kernel<T><<<blocks_for(nnz), kThreads, 0, s>>>(L, ax, dst, nnz);
+}
+template <typename T>
+void launch_extend_add(const EaItem* items, int count, const FrontDesc* fronts, const int* rel, T* L, T* U,
+                       hipStream_t s) {
+    if (count > 0) extend_add_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, rel, L, U);
+}
+template <typename T>
+void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,
+                  hipStream_t s) {
+    if (count <= 0) return;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+        attr_set = true;
+    }
+    panel_kernel<T><<<count, kThreads, lds_bytes, s>>>(list, fronts, L, info);
+}
+template <typename T>
+void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s) {
+    if (count > 0) diag_kernel<T><<<count, kThreads, 0, s>>>(jobs, L, inv, info);
+}
+template <typename T>
+void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv, hipStream_t s) {
+    if (nblocks > 0) trsm_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, inv);
+}
+template <typename T>
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, T* L, T* U, hipStream_t s) {
+    if (nblocks > 0) gemm_nt_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+}
+
+void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s) {
+    permute_in_kernel<<<blocks_for(n), kThreads, 0, s>>>(b, perm, xp, n);
+}
+void launch_permute_out(const double* xp, const int* perm, double* x, int n, hipStream_t s) {
+    permute_out_kernel<<<blocks_for(n), kThreads, 0, s>>>(xp, perm, x, n);
+}
+template <typename T>
+void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
+                const int* q, const T* L, double* xp, double* su, const int64_t* suofs, hipStream_t s) {
+    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, xp, su, suofs);
+}
+template <typename T>
+void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, double* xp,
+                hipStream_t s) {
+    if (count > 0) bwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, q, L, xp);
+}
+
+void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
+    if (nnz > 0) gather_kernel<<<blocks_for(nnz), kThreads, 0, s>>>(src_vals, src, dst, nnz);
+}
+void launch_residual(const int* rp, const int* ci, const double* av, const double* x, const double* b, double* r,
+                     int n, hipStream_t s) {
+    residual_kernel<<<blocks_for(n), kThreads, 0, s>>>(rp, ci, av, x, b, r, n);
+}
+void launch_rowabs(const int* rp, const double* av, double* out, int n, hipStream_t s) {
+    rowabs_kernel<<<blocks_for(n), kThreads, 0, s>>>(rp, av, out, n);
+}
+void launch_maxabs(const double* v, int n, unsigned long long* out, hipStream_t s) {
+    int blocks = blocks_for(n);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    maxabs_kernel<<<blocks, kThreads, 0, s>>>(v, n, out);
+}
+void launch_axpy(double* x, const double* d, int n, hipStream_t s) {
+    axpy_kernel<<<blocks_for(n), kThreads, 0, s>>>(x, d, n);
+}
+template <typename T>
+void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial, int nparts, hipStream_t s) {
+    logdiag_kernel<T><<<nparts, kThreads, 0, s>>>(diag_off, L, n, partial);
+}
+
+#define MDGPU_INSTANTIATE(T)                                                                                   \
+    template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                                 \
+    template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
+    template void launch_extend_add<T>(const EaItem*, int, const FrontDesc*, const int*, T*, T*, hipStream_t); \
+    template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
+    template void launch_diag<T>(const DiagJob*, int, T*, T*, int*, hipStream_t);                             \
+    template void launch_trsm<T>(const TrsmJob*, int, int, T*, const T*, hipStream_t);                         \
+    template void launch_gemm<T>(const GemmJob*, int, int, T*, T*, hipStream_t);                               \
+    template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
+                                const T*, double*, double*, const int64_t*, hipStream_t);                      \
+    template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, double*,              \
+                                hipStream_t);                                                                  \
+    template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);
+
+MDGPU_INSTANTIATE(double)
+MDGPU_INSTANTIATE(float)
+
+}  // namespace mdgpu
+
+namespace mdgpu {
+template void launch_fill<int>(int*, int64_t, int, hipStream_t);
+template void launch_fill<unsigned long long>(unsigned long long*, int64_t, unsigned long long, hipStream_t);
+}  // namespace mdgpu

@@ -1,0 +1,414 @@
+// C ABI: numeric factorization, solves, refinement, log-determinant and
+// L export on one GPU (see include/mdgpu_c.h).  Host code only orchestrates:
+// every arithmetic operation on the factor runs in kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "plan.h"
+
+using mdgpu::fail;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct mdgpu_numeric {
+    const mdgpu_symbolic* sym = nullptr;
+    int precision = MDGPU_FP64;
+    int device = 0;
+    mdgpu::Plan plan;
+    std::vector<DevBuf> bufs;
+    // device views
+    void* dL = nullptr;
+    void* dU = nullptr;
+    void* dInv = nullptr;
+    mdgpu::FrontDesc* dFronts = nullptr;
+    int* dKids = nullptr;
+    int* dRel = nullptr;
+    int* dQ = nullptr;
+    int64_t* dAdst = nullptr;
+    mdgpu::EaItem* dEa = nullptr;
+    mdgpu::GemmJob* dGemm = nullptr;
+    mdgpu::DiagJob* dDiag = nullptr;
+    mdgpu::TrsmJob* dTrsm = nullptr;
+    int* dPanel = nullptr;
+    int* dLevelList = nullptr;
+    std::vector<int> level_off;
+    double* dSu = nullptr;
+    int64_t* dSuofs = nullptr;
+    int* dPerm = nullptr;
+    int* dAfp = nullptr;
+    int* dAfi = nullptr;
+    int* dAfsrc = nullptr;
+    double* dAfx = nullptr;
+    double* dXp = nullptr;
+    double* dR = nullptr;
+    double* dD = nullptr;
+    double* dRowabs = nullptr;
+    int64_t* dDiagOff = nullptr;
+    int* dInfo = nullptr;
+    unsigned long long* dMax = nullptr;
+    double* dPartial = nullptr;
+    int nparts = 0;
+    double anorm = -1;
+    bool factored = false;
+    int64_t device_bytes = 0;
+};
+
+namespace {
+
+const int kLogParts = 256;
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(MDGPU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename V>
+int upload(mdgpu_numeric* N, const V& v, void** out) {
+    using E = typename V::value_type;
+    size_t bytes = std::max<size_t>(v.size() * sizeof(E), 16);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(MDGPU_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    N->bufs.push_back({p, bytes});
+    N->device_bytes += (int64_t)bytes;
+    if (!v.empty()) {
+        e = hipMemcpy(p, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+    }
+    *out = p;
+    return MDGPU_OK;
+}
+
+int alloc(mdgpu_numeric* N, size_t bytes, void** out) {
+    bytes = std::max<size_t>(bytes, 16);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(MDGPU_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    N->bufs.push_back({p, bytes});
+    N->device_bytes += (int64_t)bytes;
+    *out = p;
+    return MDGPU_OK;
+}
+
+void release(mdgpu_numeric* N) {
+    for (auto& b : N->bufs) hipFree(b.p);
+    N->bufs.clear();
+}
+
+#define TRY(x)                        \
+    do {                              \
+        int _st = (x);                \
+        if (_st != MDGPU_OK) return _st; \
+    } while (0)
+
+template <typename T>
+int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s) {
+    const mdgpu::Plan& P = N->plan;
+    T* L = (T*)N->dL;
+    T* U = (T*)N->dU;
+    T* inv = (T*)N->dInv;
+    mdgpu::launch_fill<T>(L, P.lstore, T(0), s);
+    mdgpu::launch_fill<int>(N->dInfo, 1, INT_MAX, s);
+    mdgpu::launch_scatter_a<T>(L, ax, N->dAdst, (int)P.a_dst.size(), s);
+    for (const mdgpu::Step& st : P.steps) {
+        switch (st.kind) {
+            case mdgpu::STEP_EA1:
+            case mdgpu::STEP_EA2:
+                mdgpu::launch_extend_add<T>(N->dEa + st.first, st.count, N->dFronts, N->dRel, L, U, s);
+                break;
+            case mdgpu::STEP_PANEL:
+                mdgpu::launch_panel<T>(N->dPanel + st.first, st.count, N->dFronts, L, N->dInfo, st.arg, s);
+                break;
+            case mdgpu::STEP_DIAG:
+                mdgpu::launch_diag<T>(N->dDiag + st.first, st.count, L, inv, N->dInfo, s);
+                break;
+            case mdgpu::STEP_TRSM:
+                mdgpu::launch_trsm<T>(N->dTrsm + st.first, st.count, st.nblocks, L, inv, s);
+                break;
+            case mdgpu::STEP_GEMM:
+                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, L, U, s);
+                break;
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "factorization launch");
+    return MDGPU_OK;
+}
+
+template <typename T>
+int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
+    const mdgpu::Plan& P = N->plan;
+    const int n = N->sym->S.n;
+    const T* L = (const T*)N->dL;
+    mdgpu::launch_permute_in(b, N->dPerm, N->dXp, n, s);
+    for (int lv = 0; lv < P.nlevels; lv++) {
+        int off = N->level_off[lv], cnt = N->level_off[lv + 1] - off;
+        mdgpu::launch_fwd<T>(N->dLevelList + off, cnt, N->dFronts, N->dKids, N->dRel, N->dQ, L, N->dXp, N->dSu,
+                             N->dSuofs, s);
+    }
+    for (int lv = P.nlevels - 1; lv >= 0; lv--) {
+        int off = N->level_off[lv], cnt = N->level_off[lv + 1] - off;
+        mdgpu::launch_bwd<T>(N->dLevelList + off, cnt, N->dFronts, N->dQ, L, N->dXp, s);
+    }
+    mdgpu::launch_permute_out(N->dXp, N->dPerm, x, n, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "solve launch");
+    return MDGPU_OK;
+}
+
+int solve_any(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
+    if (N->precision == MDGPU_FP64) return run_solve<double>(N, b, x, s);
+    return run_solve<float>(N, b, x, s);
+}
+
+int max_abs(mdgpu_numeric* N, const double* v, int n, hipStream_t s, double* out) {
+    mdgpu::launch_fill<unsigned long long>(N->dMax, 1, 0ULL, s);
+    mdgpu::launch_maxabs(v, n, N->dMax, s);
+    unsigned long long bits = 0;
+    hipError_t e = hipMemcpyAsync(&bits, N->dMax, sizeof(bits), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    double d;
+    memcpy(&d, &bits, sizeof d);
+    *out = d;
+    return MDGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, mdgpu_numeric** out) {
+    mdgpu::clear_error();
+    if (!sym || !out) return fail(MDGPU_ERR_ARG, "mdgpu_numeric_create: null pointer");
+    *out = nullptr;
+    if (precision != MDGPU_FP64 && precision != MDGPU_FP32)
+        return fail(MDGPU_ERR_ARG, "mdgpu_numeric_create: precision must be MDGPU_FP64 or MDGPU_FP32");
+    mdgpu_numeric* N = new (std::nothrow) mdgpu_numeric();
+    if (!N) return fail(MDGPU_ERR_NOMEM, "out of host memory");
+    N->sym = sym;
+    N->precision = precision;
+    hipGetDevice(&N->device);
+    mdgpu::PlanOptions po = mdgpu::plan_options_for(precision);
+    std::string err;
+    try {
+        err = mdgpu::build_plan(sym->S, po, N->plan);
+        if (err.empty()) err = mdgpu::validate_plan(N->plan, po);
+    } catch (const std::bad_alloc&) {
+        delete N;
+        return fail(MDGPU_ERR_NOMEM, "out of host memory building the plan");
+    }
+    if (!err.empty()) {
+        delete N;
+        return fail(MDGPU_ERR_ARG, "mdgpu_numeric_create: " + err);
+    }
+    const mdgpu::Plan& P = N->plan;
+    const mdgpu::Symbolic& S = sym->S;
+    const size_t es = po.elem_bytes;
+    int st = MDGPU_OK;
+    auto guard = [&](int code) {
+        if (code != MDGPU_OK) { release(N); delete N; }
+        return code;
+    };
+    std::vector<int> level_list;
+    N->level_off.assign(P.nlevels + 1, 0);
+    for (int lv = 0; lv < P.nlevels; lv++) {
+        N->level_off[lv] = (int)level_list.size();
+        level_list.insert(level_list.end(), P.level_fronts[lv].begin(), P.level_fronts[lv].end());
+    }
+    N->level_off[P.nlevels] = (int)level_list.size();
+    std::vector<int64_t> diag_off(S.n);
+    for (int f = 0; f < P.nfronts; f++) {
+        const mdgpu::FrontDesc& d = P.fronts[f];
+        for (int j = 0; j < d.c; j++) diag_off[d.k + j] = d.lofs + (int64_t)j * d.n + j;
+    }
+    N->nparts = kLogParts;
+    if ((st = alloc(N, P.lstore * es, &N->dL)) ||
+        (st = alloc(N, P.arena * es, &N->dU)) ||
+        (st = alloc(N, P.inv_scratch * es, &N->dInv)) ||
+        (st = upload(N, P.fronts, (void**)&N->dFronts)) ||
+        (st = upload(N, P.kids, (void**)&N->dKids)) ||
+        (st = upload(N, P.rel, (void**)&N->dRel)) ||
+        (st = upload(N, P.q, (void**)&N->dQ)) ||
+        (st = upload(N, P.a_dst, (void**)&N->dAdst)) ||
+        (st = upload(N, P.ea, (void**)&N->dEa)) ||
+        (st = upload(N, P.gemm, (void**)&N->dGemm)) ||
+        (st = upload(N, P.diag, (void**)&N->dDiag)) ||
+        (st = upload(N, P.trsm, (void**)&N->dTrsm)) ||
+        (st = upload(N, P.panel, (void**)&N->dPanel)) ||
+        (st = upload(N, level_list, (void**)&N->dLevelList)) ||
+        (st = alloc(N, P.solve_u * sizeof(double), (void**)&N->dSu)) ||
+        (st = upload(N, P.solve_uofs, (void**)&N->dSuofs)) ||
+        (st = upload(N, S.perm, (void**)&N->dPerm)) ||
+        (st = upload(N, P.afull_p, (void**)&N->dAfp)) ||
+        (st = upload(N, P.afull_i, (void**)&N->dAfi)) ||
+        (st = upload(N, P.afull_src, (void**)&N->dAfsrc)) ||
+        (st = alloc(N, P.afull_i.size() * sizeof(double), (void**)&N->dAfx)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dXp)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dR)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dD)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dRowabs)) ||
+        (st = upload(N, diag_off, (void**)&N->dDiagOff)) ||
+        (st = alloc(N, sizeof(int), (void**)&N->dInfo)) ||
+        (st = alloc(N, sizeof(unsigned long long), (void**)&N->dMax)) ||
+        (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)))
+        return guard(st);
+    *out = N;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_numeric_destroy(mdgpu_numeric* num) {
+    if (!num) return MDGPU_OK;
+    release(num);
+    delete num;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_numeric_info(const mdgpu_numeric* num, int64_t* out) {
+    if (!num || !out) return fail(MDGPU_ERR_ARG, "mdgpu_numeric_info: null pointer");
+    const mdgpu::Plan& P = num->plan;
+    out[MDGPU_NUM_DEVICE_BYTES] = num->device_bytes;
+    out[MDGPU_NUM_LSTORE] = P.lstore;
+    out[MDGPU_NUM_ARENA] = P.arena;
+    out[MDGPU_NUM_STEPS] = (int64_t)P.steps.size();
+    out[MDGPU_NUM_LEVELS] = P.nlevels;
+    out[MDGPU_NUM_PANEL_FRONTS] = (int64_t)P.panel.size();
+    out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.diag.size();
+    out[MDGPU_NUM_PRECISION] = num->precision;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream) {
+    mdgpu::clear_error();
+    if (!num || !d_ax) return fail(MDGPU_ERR_ARG, "mdgpu_factorize: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    num->factored = false;
+    int st = num->precision == MDGPU_FP64 ? run_factor<double>(num, d_ax, s) : run_factor<float>(num, d_ax, s);
+    if (st != MDGPU_OK) return st;
+    // full symmetric values and ||A||_1 for the residual of refinement
+    const int n = num->sym->S.n;
+    mdgpu::launch_gather(d_ax, num->dAfsrc, num->dAfx, (int)num->plan.afull_i.size(), s);
+    mdgpu::launch_rowabs(num->dAfp, num->dAfx, num->dRowabs, n, s);
+    int info = INT_MAX;
+    hipError_t e = hipMemcpyAsync(&info, num->dInfo, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    double an = 0;
+    st = max_abs(num, num->dRowabs, n, s, &an);   // synchronizes the stream
+    if (st != MDGPU_OK) return st;
+    num->anorm = an;
+    if (info != INT_MAX)
+        return fail(MDGPU_ERR_NOT_SPD, "mdgpu_factorize: non-positive pivot at permuted column " +
+                                           std::to_string(info));
+    num->factored = true;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_solve(mdgpu_numeric* num, const double* d_b, double* d_x, void* stream) {
+    mdgpu::clear_error();
+    if (!num || !d_b || !d_x) return fail(MDGPU_ERR_ARG, "mdgpu_solve: null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_solve: no valid factorization");
+    return solve_any(num, d_b, d_x, (hipStream_t)stream);
+}
+
+extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol, int maxit,
+                            double* eta_out, int* iters_out, void* stream) {
+    mdgpu::clear_error();
+    if (!num || !d_b || !d_x) return fail(MDGPU_ERR_ARG, "mdgpu_refine: null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_refine: no valid factorization");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = num->sym->S.n;
+    TRY(solve_any(num, d_b, d_x, s));
+    double bnorm = 0;
+    TRY(max_abs(num, d_b, n, s, &bnorm));
+    double eta = INFINITY;
+    int it = 0;
+    for (;; it++) {
+        mdgpu::launch_residual(num->dAfp, num->dAfi, num->dAfx, d_x, d_b, num->dR, n, s);
+        double rn = 0, xn = 0;
+        TRY(max_abs(num, num->dR, n, s, &rn));
+        TRY(max_abs(num, d_x, n, s, &xn));
+        // backward error of the reference's check (mdgpu.cu:1972)
+        const double den = num->anorm * xn + bnorm;
+        eta = den > 0 ? rn / den : rn;
+        if (eta <= tol || it >= maxit || !(eta == eta)) break;
+        TRY(solve_any(num, num->dR, num->dD, s));
+        mdgpu::launch_axpy(d_x, num->dD, n, s);
+    }
+    if (eta_out) *eta_out = eta;
+    if (iters_out) *iters_out = it;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "refine launch");
+    if (!(eta <= tol))
+        return fail(MDGPU_ERR_NOT_CONVERGED, "mdgpu_refine: backward error " + std::to_string(eta) +
+                                                 " above tolerance after " + std::to_string(it) + " steps");
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_logdet(mdgpu_numeric* num, double* out, void* stream) {
+    mdgpu::clear_error();
+    if (!num || !out) return fail(MDGPU_ERR_ARG, "mdgpu_logdet: null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_logdet: no valid factorization");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = num->sym->S.n;
+    if (num->precision == MDGPU_FP64)
+        mdgpu::launch_logdiag<double>(num->dDiagOff, (const double*)num->dL, n, num->dPartial, num->nparts, s);
+    else
+        mdgpu::launch_logdiag<float>(num->dDiagOff, (const float*)num->dL, n, num->dPartial, num->nparts, s);
+    std::vector<double> part(num->nparts);
+    hipError_t e = hipMemcpyAsync(part.data(), num->dPartial, part.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "logdet copy");
+    double sum = 0;
+    for (double v : part) sum += v;
+    *out = 2.0 * sum;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_export_L(mdgpu_numeric* num, int32_t* Lp, int32_t* Li, double* Lx) {
+    mdgpu::clear_error();
+    if (!num) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: no valid factorization");
+    const mdgpu::Symbolic& S = num->sym->S;
+    const mdgpu::Plan& P = num->plan;
+    if (S.lnz > INT32_MAX) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: nnz(L) exceeds int32");
+    std::vector<double> h(P.lstore);
+    hipError_t e;
+    if (num->precision == MDGPU_FP64) {
+        e = hipMemcpy(h.data(), num->dL, P.lstore * sizeof(double), hipMemcpyDeviceToHost);
+    } else {
+        std::vector<float> hf(P.lstore);
+        e = hipMemcpy(hf.data(), num->dL, P.lstore * sizeof(float), hipMemcpyDeviceToHost);
+        for (int64_t t = 0; t < P.lstore; t++) h[t] = hf[t];
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy L");
+    // column k+j holds rows q_f[j..n_f) of its front, diagonal first
+    // (the spL write-out of FactorAux_CPU_3, mdgpu.cu:1346-1352)
+    if (Lp)
+        for (int k = 0; k <= S.n; k++) Lp[k] = S.lp[k];
+    for (int f = 0; f < P.nfronts; f++) {
+        const mdgpu::FrontDesc& d = P.fronts[f];
+        for (int j = 0; j < d.c; j++) {
+            int64_t dst = S.lp[d.k + j];
+            for (int i = j; i < d.n; i++, dst++) {
+                if (Li) Li[dst] = S.q_all[d.qofs + i];
+                if (Lx) Lx[dst] = h[d.lofs + (int64_t)j * d.n + i];
+            }
+        }
+    }
+    return MDGPU_OK;
+}

@@ -1,0 +1,458 @@
+// Build the numeric execution plan (see plan.h).
+#include "plan.h"
+
+#include <algorithm>
+#include <map>
+
+namespace mdgpu {
+
+namespace {
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// First-fit allocator over a linear arena, used only to compute static
+// offsets for the update matrices.
+class StaticArena {
+   public:
+    int64_t alloc(int64_t size) {
+        size = align_up(std::max<int64_t>(size, 1), 64);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= size) {
+                int64_t off = it->first;
+                int64_t rest = it->second - size;
+                free_.erase(it);
+                if (rest > 0) free_[off + size] = rest;
+                used_[off] = size;
+                return off;
+            }
+        }
+        int64_t off = top_;
+        top_ += size;
+        used_[off] = size;
+        peak_ = std::max(peak_, top_);
+        return off;
+    }
+    void release(int64_t off) {
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        int64_t size = u->second;
+        used_.erase(u);
+        auto it = free_.emplace(off, size).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+                it = pv;
+            }
+        }
+        if (it->first + it->second == top_) {   // shrink the top
+            top_ = it->first;
+            free_.erase(it);
+        }
+    }
+    int64_t peak() const { return peak_; }
+
+   private:
+    std::map<int64_t, int64_t> free_, used_;
+    int64_t top_ = 0, peak_ = 0;
+};
+
+int pos_in(const int* q, int n, int row) {
+    const int* it = std::lower_bound(q, q + n, row);
+    if (it == q + n || *it != row) return -1;
+    return (int)(it - q);
+}
+
+}  // namespace
+
+std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
+    P = Plan();
+    const int nf = (int)S.nodes.size();
+    const int n = S.n;
+    P.nfronts = nf;
+    P.q = S.q_all;
+    P.fronts.resize(nf);
+    std::vector<int> level(nf, 0);
+    int64_t lofs = 0;
+    for (int f = 0; f < nf; f++) {
+        FrontDesc& d = P.fronts[f];
+        const PartNode& nd = S.nodes[f];
+        d.k = nd.first;
+        d.c = nd.ncols;
+        d.n = (int)(S.q_off[f + 1] - S.q_off[f]);
+        d.m = d.n - d.c;
+        d.parent = S.front_parent[f];
+        d.qofs = S.q_off[f];
+        d.lofs = lofs;
+        lofs += (int64_t)d.n * d.c;
+        d.uofs = -1;
+        d.kid_begin = (int)P.kids.size();
+        for (int kid : nd.kids) {
+            P.kids.push_back(kid);
+            level[f] = std::max(level[f], level[kid] + 1);
+        }
+        d.kid_end = (int)P.kids.size();
+        d.level = level[f];
+        P.nlevels = std::max(P.nlevels, level[f] + 1);
+        P.max_front = std::max(P.max_front, d.n);
+        P.max_kids = std::max(P.max_kids, d.kid_end - d.kid_begin);
+    }
+    P.lstore = lofs;
+
+    // relative maps child update rows -> parent rows (VectorMergeIndices)
+    for (int f = 0; f < nf; f++) {
+        FrontDesc& d = P.fronts[f];
+        d.relofs = (int64_t)P.rel.size();
+        d.split = 0;
+        if (d.parent < 0) {
+            if (d.m != 0) return "root front has update rows";
+            continue;
+        }
+        const FrontDesc& pd = P.fronts[d.parent];
+        const int* qp = S.q_all.data() + pd.qofs;
+        for (int t = 0; t < d.m; t++) {
+            int r = S.q_all[d.qofs + d.c + t];
+            int pos = pos_in(qp, pd.n, r);
+            if (pos < 0) return "update row missing from parent front";
+            P.rel.push_back(pos);
+            if (pos < pd.c) d.split++;
+        }
+    }
+
+    // caller entry -> panel offset (MatrizGetFk_3's A scatter, mdgpu.cu:272-282)
+    std::vector<int> front_of_col(n, -1);
+    for (int f = 0; f < nf; f++)
+        for (int j = P.fronts[f].k; j < P.fronts[f].k + P.fronts[f].c; j++) front_of_col[j] = f;
+    P.a_dst.assign(S.nnz_in, -1);
+    for (int j = 0; j < n; j++) {
+        const int f = front_of_col[j];
+        if (f < 0) return "column not covered by any front";
+        const FrontDesc& d = P.fronts[f];
+        const int* qf = S.q_all.data() + d.qofs;
+        for (int t = S.a_low.p[j]; t < S.a_low.p[j + 1]; t++) {
+            int pos = pos_in(qf, d.n, S.a_low.i[t]);
+            if (pos < 0) return "A entry outside its front";
+            P.a_dst[S.a_src[t]] = d.lofs + (int64_t)(j - d.k) * d.n + pos;
+        }
+    }
+
+    // levels and the static update-arena plan
+    P.level_fronts.assign(P.nlevels, {});
+    for (int f = 0; f < nf; f++) P.level_fronts[level[f]].push_back(f);
+    StaticArena arena;
+    for (int L = 0; L < P.nlevels; L++) {
+        for (int f : P.level_fronts[L])
+            if (P.fronts[f].m > 0) P.fronts[f].uofs = arena.alloc((int64_t)P.fronts[f].m * P.fronts[f].m);
+        for (int f : P.level_fronts[L])
+            for (int t = P.fronts[f].kid_begin; t < P.fronts[f].kid_end; t++) {
+                const FrontDesc& kd = P.fronts[P.kids[t]];
+                if (kd.uofs >= 0) arena.release(kd.uofs);
+            }
+    }
+    P.arena = arena.peak();
+
+    // solve update vectors (one per front, no reuse)
+    P.solve_uofs.resize(nf);
+    int64_t su = 0;
+    for (int f = 0; f < nf; f++) {
+        P.solve_uofs[f] = su;
+        su += align_up(std::max(P.fronts[f].m, 1), 8);
+    }
+    P.solve_u = su;
+
+    // schedule
+    const int T = o.tile;
+    auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles) {
+        g.tiles_m = (g.M + T - 1) / T;
+        g.tiles_n = (g.N + T - 1) / T;
+        if (g.lower) {
+            g.ntiles = 0;
+            for (int tj = 0; tj < g.tiles_n; tj++) g.ntiles += std::max(0, g.tiles_m - tj);
+        } else {
+            g.ntiles = g.tiles_m * g.tiles_n;
+        }
+        if (g.ntiles == 0) return;
+        g.tile_base = tiles;
+        tiles += g.ntiles;
+        batch.push_back(g);
+    };
+    auto flush_gemm = [&](std::vector<GemmJob>& batch, int tiles, int L) {
+        if (batch.empty()) return;
+        Step s{STEP_GEMM, (int)P.gemm.size(), (int)batch.size(), tiles, L, 0};
+        P.gemm.insert(P.gemm.end(), batch.begin(), batch.end());
+        P.steps.push_back(s);
+        batch.clear();
+    };
+    auto add_ea = [&](int L, bool first_pass) {
+        const std::vector<int>& fl = P.level_fronts[L];
+        for (int r = 0; r < P.max_kids; r++) {
+            const int begin = (int)P.ea.size();
+            for (int f : fl) {
+                const FrontDesc& d = P.fronts[f];
+                if (d.kid_begin + r >= d.kid_end) continue;
+                const int ch = P.kids[d.kid_begin + r];
+                const FrontDesc& cd = P.fronts[ch];
+                int j0 = first_pass ? 0 : cd.split;
+                int j1 = first_pass ? cd.split : cd.m;
+                // chunk the lower-triangle columns to ~ea_chunk entries each
+                int j = j0;
+                while (j < j1) {
+                    int64_t acc = 0;
+                    int je = j;
+                    while (je < j1 && (acc == 0 || acc + (cd.m - je) <= o.ea_chunk)) {
+                        acc += cd.m - je;
+                        je++;
+                    }
+                    P.ea.push_back({ch, j, je, 0});
+                    j = je;
+                }
+            }
+            const int count = (int)P.ea.size() - begin;
+            if (count > 0)
+                P.steps.push_back({first_pass ? STEP_EA1 : STEP_EA2, begin, count, count, L, 0});
+        }
+    };
+
+    int64_t max_diag_jobs = 0;
+    for (int L = 0; L < P.nlevels; L++) {
+        const std::vector<int>& fl = P.level_fronts[L];
+        add_ea(L, true);
+        // small panels: whole panel in LDS, one workgroup per front
+        std::vector<int> blocked;
+        {
+            const int begin = (int)P.panel.size();
+            int lds = 0;
+            for (int f : fl) {
+                const FrontDesc& d = P.fronts[f];
+                int64_t bytes = (int64_t)d.n * d.c * o.elem_bytes;
+                if (bytes <= o.panel_lds_bytes) {
+                    P.panel.push_back(f);
+                    lds = std::max<int>(lds, (int)bytes);
+                } else {
+                    blocked.push_back(f);
+                }
+            }
+            const int count = (int)P.panel.size() - begin;
+            if (count > 0) P.steps.push_back({STEP_PANEL, begin, count, count, L, lds});
+        }
+        // large panels: blocked right-looking, nb columns per step
+        int maxc = 0;
+        for (int f : blocked) maxc = std::max(maxc, P.fronts[f].c);
+        for (int r0 = 0; r0 < maxc; r0 += o.nb) {
+            const int dbegin = (int)P.diag.size();
+            for (int f : blocked) {
+                const FrontDesc& d = P.fronts[f];
+                if (r0 >= d.c) continue;
+                DiagJob dj;
+                dj.b = std::min(o.nb, d.c - r0);
+                dj.ld = d.n;
+                dj.p_off = d.lofs + (int64_t)r0 * d.n + r0;
+                dj.inv_off = (int64_t)(P.diag.size() - dbegin) * o.nb * o.nb;
+                dj.col0 = d.k + r0;
+                dj.front = f;
+                P.diag.push_back(dj);
+            }
+            const int dcount = (int)P.diag.size() - dbegin;
+            if (dcount == 0) continue;
+            max_diag_jobs = std::max<int64_t>(max_diag_jobs, dcount);
+            P.steps.push_back({STEP_DIAG, dbegin, dcount, dcount, L, 0});
+            const int tbegin = (int)P.trsm.size();
+            int ttiles = 0;
+            for (int t = dbegin; t < dbegin + dcount; t++) {
+                const DiagJob& dj = P.diag[t];
+                const FrontDesc& d = P.fronts[dj.front];
+                const int rows = d.n - r0 - dj.b;
+                if (rows <= 0) continue;
+                TrsmJob tj;
+                tj.p_off = dj.p_off + dj.b;
+                tj.inv_off = dj.inv_off;
+                tj.ld = d.n;
+                tj.rows = rows;
+                tj.b = dj.b;
+                tj.ntiles = (rows + o.trsm_rows - 1) / o.trsm_rows;
+                tj.tile_base = ttiles;
+                tj.pad = 0;
+                ttiles += tj.ntiles;
+                P.trsm.push_back(tj);
+            }
+            const int tcount = (int)P.trsm.size() - tbegin;
+            if (tcount > 0) P.steps.push_back({STEP_TRSM, tbegin, tcount, ttiles, L, 0});
+            std::vector<GemmJob> batch;
+            int tiles = 0;
+            for (int t = dbegin; t < dbegin + dcount; t++) {
+                const DiagJob& dj = P.diag[t];
+                const FrontDesc& d = P.fronts[dj.front];
+                const int r1 = r0 + dj.b;
+                if (r1 >= d.c) continue;        // no trailing pivot columns
+                GemmJob g{};
+                g.a_off = d.lofs + (int64_t)r0 * d.n + r1;
+                g.b_off = g.a_off;
+                g.c_off = d.lofs + (int64_t)r1 * d.n + r1;
+                g.lda = g.ldb = g.ldc = d.n;
+                g.M = d.n - r1;
+                g.N = d.c - r1;
+                g.K = dj.b;
+                g.c_in_arena = 0;
+                g.beta0 = 0;
+                g.lower = 1;
+                add_gemm(g, batch, tiles);
+            }
+            flush_gemm(batch, tiles, L);
+        }
+        // Schur complements of every front of the level: U_f = -L21 L21^T
+        {
+            std::vector<GemmJob> batch;
+            int tiles = 0;
+            for (int f : fl) {
+                const FrontDesc& d = P.fronts[f];
+                if (d.m == 0) continue;
+                GemmJob g{};
+                g.a_off = d.lofs + d.c;
+                g.b_off = g.a_off;
+                g.c_off = d.uofs;
+                g.lda = g.ldb = d.n;
+                g.ldc = d.m;
+                g.M = g.N = d.m;
+                g.K = d.c;
+                g.c_in_arena = 1;
+                g.beta0 = 1;
+                g.lower = 1;
+                add_gemm(g, batch, tiles);
+            }
+            flush_gemm(batch, tiles, L);
+        }
+        add_ea(L, false);
+    }
+    P.inv_scratch = std::max<int64_t>(max_diag_jobs * o.nb * o.nb, 1);
+
+    // full symmetric A in the caller's numbering (CSR rows sorted), for the
+    // residual r = b - A x of iterative refinement
+    {
+        const int nz = S.a_low.nnz();
+        std::vector<int> rr, cc, ss;
+        rr.reserve((size_t)nz * 2);
+        cc.reserve((size_t)nz * 2);
+        ss.reserve((size_t)nz * 2);
+        for (int j = 0; j < n; j++)
+            for (int t = S.a_low.p[j]; t < S.a_low.p[j + 1]; t++) {
+                int r = S.perm[S.a_low.i[t]], c = S.perm[j];
+                rr.push_back(r); cc.push_back(c); ss.push_back(S.a_src[t]);
+                if (r != c) { rr.push_back(c); cc.push_back(r); ss.push_back(S.a_src[t]); }
+            }
+        const int nnz = (int)rr.size();
+        std::vector<int> bycol(nnz), cp(n + 1, 0);
+        for (int t = 0; t < nnz; t++) cp[cc[t] + 1]++;
+        for (int t = 0; t < n; t++) cp[t + 1] += cp[t];
+        for (int t = 0; t < nnz; t++) bycol[cp[cc[t]]++] = t;
+        P.afull_p.assign(n + 1, 0);
+        for (int t = 0; t < nnz; t++) P.afull_p[rr[t] + 1]++;
+        for (int t = 0; t < n; t++) P.afull_p[t + 1] += P.afull_p[t];
+        std::vector<int> fill(P.afull_p.begin(), P.afull_p.end() - 1);
+        P.afull_i.resize(nnz);
+        P.afull_src.resize(nnz);
+        for (int u = 0; u < nnz; u++) {       // columns ascending within rows
+            int t = bycol[u];
+            int q = fill[rr[t]]++;
+            P.afull_i[q] = cc[t];
+            P.afull_src[q] = ss[t];
+        }
+    }
+    return "";
+}
+
+}  // namespace mdgpu
+
+namespace mdgpu {
+
+std::string validate_plan(const Plan& P, const PlanOptions& o) {
+    const int nf = P.nfronts;
+    auto bad = [](const std::string& s) { return "plan check: " + s; };
+    for (int f = 0; f < nf; f++) {
+        const FrontDesc& d = P.fronts[f];
+        if (d.c < 1 || d.n < d.c || d.m != d.n - d.c) return bad("front shape");
+        if (d.lofs < 0 || d.lofs + (int64_t)d.n * d.c > P.lstore) return bad("panel outside L store");
+        if (d.m > 0 && (d.uofs < 0 || d.uofs + (int64_t)d.m * d.m > P.arena)) return bad("U outside arena");
+        if (d.parent >= 0) {
+            const FrontDesc& pd = P.fronts[d.parent];
+            if (d.parent <= f || pd.level <= d.level) return bad("parent order");
+            int prev = -1;
+            for (int t = 0; t < d.m; t++) {
+                int r = P.rel[d.relofs + t];
+                if (r <= prev || r >= pd.n) return bad("relative map");
+                prev = r;
+            }
+            if (d.split < 0 || d.split > d.m) return bad("split");
+        }
+    }
+    for (int64_t a : P.a_dst)
+        if (a < 0 || a >= P.lstore) return bad("A scatter target");
+    for (const EaItem& it : P.ea) {
+        if (it.child < 0 || it.child >= nf) return bad("EA child");
+        const FrontDesc& cd = P.fronts[it.child];
+        if (cd.parent < 0 || it.j0 < 0 || it.j1 > cd.m || it.j0 >= it.j1) return bad("EA range");
+    }
+    // live update matrices must not overlap: at level L the live set is
+    // {f : level(f) <= L <= level(parent(f))}
+    for (int L = 0; L < P.nlevels; L++) {
+        std::vector<std::pair<int64_t, int64_t>> iv;
+        for (int f = 0; f < nf; f++) {
+            const FrontDesc& d = P.fronts[f];
+            if (d.m == 0) continue;
+            int end = d.parent >= 0 ? P.fronts[d.parent].level : d.level;
+            if (d.level <= L && L <= end) iv.push_back({d.uofs, d.uofs + (int64_t)d.m * d.m});
+        }
+        std::sort(iv.begin(), iv.end());
+        for (size_t t = 1; t < iv.size(); t++)
+            if (iv[t].first < iv[t - 1].second) return bad("overlapping update matrices at level " + std::to_string(L));
+    }
+    for (const GemmJob& g : P.gemm) {
+        if (g.M <= 0 || g.N <= 0 || g.K <= 0 || (g.lower && g.M < g.N)) return bad("gemm shape");
+        int64_t alast = g.a_off + (g.M - 1) + (int64_t)(g.K - 1) * g.lda;
+        int64_t blast = g.b_off + (g.N - 1) + (int64_t)(g.K - 1) * g.ldb;
+        int64_t clast = g.c_off + (g.M - 1) + (int64_t)(g.N - 1) * g.ldc;
+        if (g.a_off < 0 || alast >= P.lstore || g.b_off < 0 || blast >= P.lstore) return bad("gemm A/B bounds");
+        if (g.c_off < 0 || clast >= (g.c_in_arena ? P.arena : P.lstore)) return bad("gemm C bounds");
+    }
+    for (const DiagJob& dj : P.diag) {
+        if (dj.b < 1 || dj.b > o.nb || dj.b > 64) return bad("diag size");
+        if (dj.p_off < 0 || dj.p_off + (dj.b - 1) + (int64_t)(dj.b - 1) * dj.ld >= P.lstore) return bad("diag bounds");
+        if (dj.inv_off < 0 || dj.inv_off + (int64_t)dj.b * dj.b > P.inv_scratch) return bad("diag scratch");
+    }
+    for (const TrsmJob& tj : P.trsm) {
+        if (tj.b < 1 || tj.b > 64 || tj.rows < 1) return bad("trsm shape");
+        if (tj.p_off + (tj.rows - 1) + (int64_t)(tj.b - 1) * tj.ld >= P.lstore) return bad("trsm bounds");
+        if (tj.inv_off + (int64_t)tj.b * tj.b > P.inv_scratch) return bad("trsm scratch");
+    }
+    for (int f : P.panel) {
+        const FrontDesc& d = P.fronts[f];
+        if ((int64_t)d.n * d.c * o.elem_bytes > o.panel_lds_bytes) return bad("panel LDS");
+    }
+    for (const Step& s : P.steps) {
+        if (s.nblocks <= 0 || s.count <= 0) return bad("empty step");
+        if (s.kind == STEP_GEMM) {
+            int tiles = 0;
+            for (int t = s.first; t < s.first + s.count; t++) {
+                if (P.gemm[t].tile_base != tiles) return bad("gemm tile bases");
+                tiles += P.gemm[t].ntiles;
+            }
+            if (tiles != s.nblocks) return bad("gemm tile count");
+        }
+        if (s.kind == STEP_TRSM) {
+            int tiles = 0;
+            for (int t = s.first; t < s.first + s.count; t++) {
+                if (P.trsm[t].tile_base != tiles) return bad("trsm tile bases");
+                tiles += P.trsm[t].ntiles;
+            }
+            if (tiles != s.nblocks) return bad("trsm tile count");
+        }
+    }
+    return "";
+}
+
+}  // namespace mdgpu

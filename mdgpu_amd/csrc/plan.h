@@ -1,0 +1,142 @@
+// Execution plan of the numeric multifrontal factorization on one GPU.
+//
+// Storage (all device-resident, sized once from the symbolic analysis):
+//   * L store: front f keeps its pivot panel P_f, the n_f x c_f column-major
+//     block (leading dimension n_f) of rows q_f and pivot columns k..k+c_f-1.
+//     After factorization P_f holds L(q_f, k:k+c_f) (upper part of its
+//     diagonal block unused).  This replaces the per-front host/device dense
+//     squares of the reference (Matriz, mdgpu.h:7-15) for the L part.
+//   * update arena: front f's Schur complement U_f (m_f x m_f, m_f = n_f -
+//     c_f, leading dimension m_f, lower triangle) lives from f's level until
+//     its parent's level; offsets come from a static first-fit plan so
+//     fronts of different subtrees reuse the same memory.
+//   * relative maps rel_f: position of each update row of f inside its
+//     parent's row list (the reference's `ql`, VectorMergeIndices,
+//     mdgpu.cu:585-595), computed once here instead of per factorization.
+//
+// Per level (fronts whose children are all in lower levels):
+//   1. extend-add pass 1: child update columns that land in the parent's
+//      pivot columns go into the parent's panel (ExtendAdd, mdgpu.cu:748-767);
+//   2. panel factorization (potrf + trsm of FactorAux_CPU_3, :1298-1338);
+//   3. U_f = -L21 L21^T (the gemm of :1336, lower triangle only);
+//   4. extend-add pass 2: remaining child update columns go into U_f.
+// Siblings that update the same parent are split into rounds so no two
+// writers in one launch touch the same entry (deterministic, no atomics).
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "symbolic.h"
+
+namespace mdgpu {
+
+struct FrontDesc {            // mirrored on the device
+    int64_t lofs;             // panel offset in the L store (elements)
+    int64_t uofs;             // U_f offset in the update arena (-1: none)
+    int64_t relofs;           // offset of rel_f in the map buffer
+    int64_t qofs;             // offset of q_f in the row-list buffer
+    int k, c, n, m;           // first column, pivots, rows, update rows
+    int parent;               // parent front or -1
+    int kid_begin, kid_end;   // range in the kids buffer
+    int split;                // # of update columns that land in parent pivots
+    int level;
+};
+
+// One extend-add work item: columns [j0, j1) of child `child`'s update.
+struct EaItem {
+    int child;
+    int j0, j1;
+    int pad;
+};
+
+// One dense job: C -= A * B^T (beta0: C = -A * B^T) on a tile grid.
+struct GemmJob {
+    int64_t a_off, b_off, c_off;   // A, B in the L store; C in L store or arena
+    int lda, ldb, ldc;
+    int M, N, K;
+    int c_in_arena;                // C lives in the update arena
+    int beta0;
+    int lower;                     // keep only entries with row >= col
+    int tiles_m, tiles_n;
+    int tile_base;                 // first global tile of this job
+    int ntiles;
+};
+
+// Diagonal-block job of the blocked panel path: factor the b x b block at
+// (r0, r0) of front f's panel, write its inverse to inv_off.
+struct DiagJob {
+    int64_t p_off;                 // panel offset of the block (elements)
+    int64_t inv_off;               // b*b scratch offset for inv(L_D)
+    int ld, b, col0;               // panel ld, block size, global column of r0
+    int front;
+};
+
+// TRSM job: rows [r0, r1) of a panel block column times inv(L_D)^T.
+struct TrsmJob {
+    int64_t p_off;                 // offset of element (r0, col) of the block column
+    int64_t inv_off;
+    int ld, rows, b;
+    int tile_base, ntiles;
+    int pad;
+};
+
+// A launch of the schedule: kind + a slice of the matching job array.
+enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_DIAG, STEP_TRSM, STEP_GEMM };
+
+struct Step {
+    int kind;
+    int first, count;      // slice of the job array of this kind
+    int nblocks;           // workgroups to launch
+    int level;
+    int arg;               // panel: LDS bytes needed
+};
+
+struct Plan {
+    int nfronts = 0, nlevels = 0;
+    std::vector<FrontDesc> fronts;
+    std::vector<int> kids;             // children of each front (processing order)
+    std::vector<int> rel;              // relative maps, flat
+    std::vector<int> q;                // row lists, flat (final numbering)
+    std::vector<int64_t> a_dst;        // caller entry -> L-store offset
+    std::vector<std::vector<int>> level_fronts;
+    int64_t lstore = 0;                // elements of the L store
+    int64_t arena = 0;                 // elements of the update arena (peak)
+    int64_t inv_scratch = 0;           // elements of the diag-inverse scratch
+    int64_t solve_u = 0;               // elements of the solve update vectors
+    std::vector<int64_t> solve_uofs;   // per front offset (m_f entries)
+    std::vector<EaItem> ea;
+    std::vector<GemmJob> gemm;
+    std::vector<DiagJob> diag;
+    std::vector<TrsmJob> trsm;
+    std::vector<int> panel;            // fronts factored by the LDS panel kernel
+    std::vector<Step> steps;           // the factorization schedule
+    // full symmetric A (original numbering) for the residual: CSR + source
+    std::vector<int> afull_p, afull_i, afull_src;
+    int max_front = 0;
+    int max_kids = 0;
+};
+
+struct PlanOptions {
+    int elem_bytes = 8;          // 8: FP64 store, 4: FP32 store
+    int panel_lds_bytes = 65536; // LDS budget of the one-workgroup panel kernel
+    int nb = 64;                 // block width of the blocked panel path
+    int tile = 64;               // GEMM tile (rows = cols)
+    int trsm_rows = 64;          // rows per TRSM workgroup
+    int ea_chunk = 16384;        // target entries per extend-add workgroup
+};
+
+std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P);
+
+}  // namespace mdgpu
+
+namespace mdgpu {
+// Host-side check that every job of the plan stays inside its buffers and
+// that update matrices live at the same time never overlap.  Returns "" if
+// the plan is sound.
+std::string validate_plan(const Plan& P, const PlanOptions& o);
+// Plan options used for a given MDGPU_FP* precision.
+PlanOptions plan_options_for(int precision);
+}  // namespace mdgpu

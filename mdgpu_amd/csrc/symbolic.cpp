@@ -533,14 +533,17 @@ Csc permute_lower(const Csc& low, const std::vector<int>& pinv, const std::vecto
 }
 
 // GetETree (etree.cpp:12-48): one node per column, children in increasing
-// index order, list in postorder.  A forest is an error (etree.cpp:26-33).
+// index order, list in postorder.  The reference stops on a forest
+// (etree.cpp:26-33); here a forest is accepted unless `single_tree` is set,
+// each tree being factored on its own (reducible matrices, batches).
 std::string plain_tree(const std::vector<int>& parent, const std::vector<int>& post,
-                       const std::vector<int>& count, std::vector<PartNode>& nodes,
+                       const std::vector<int>& count, bool single_tree, std::vector<PartNode>& nodes,
                        std::vector<int>& node_of_col) {
     const int n = (int)parent.size();
     int roots = 0;
     for (int j = 0; j < n; j++) roots += parent[j] == -1;
-    if (roots != 1) return "elimination tree is a forest (" + std::to_string(roots) + " roots)";
+    if (single_tree && roots != 1)
+        return "elimination tree is a forest (" + std::to_string(roots) + " roots)";
     std::vector<int> pos(n);
     for (int k = 0; k < n; k++) pos[post[k]] = k;
     nodes.assign(n, PartNode());
@@ -658,14 +661,16 @@ std::string relaxed_tree(const std::vector<PartNode>& plain, int max_zeros, std:
         stack.push_back((int)pool.size());
         pool.push_back(std::move(root));
     }
-    // depth-first renumbering from the last root (etree.cpp:165-178, 331)
+    // depth-first renumbering from the root (etree.cpp:165-178, 331); for a
+    // forest every remaining stack entry is a root, renumbered in order
     perm.assign(n, -1);
     out.clear();
     std::vector<int> index_of(pool.size(), -1);
     int c = 0;
     struct Frame { int node; size_t next_kid; };
     std::vector<Frame> dfs;
-    dfs.push_back({stack.back(), 0});
+    for (size_t root = 0; root < stack.size(); root++) {
+    dfs.push_back({stack[root], 0});
     while (!dfs.empty()) {
         Frame& f = dfs.back();
         R& r = pool[f.node];
@@ -685,6 +690,7 @@ std::string relaxed_tree(const std::vector<PartNode>& plain, int max_zeros, std:
         index_of[f.node] = (int)out.size();
         out.push_back(std::move(pn));
         dfs.pop_back();
+    }
     }
     if (c != n) return "relaxed tree does not cover every column (forest?)";
     return "";
@@ -743,7 +749,7 @@ std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& o
     // 3. node partition (mdgpu.cu:1803-1853)
     std::vector<PartNode> plain;
     std::vector<int> node_of_col;
-    std::string err = plain_tree(S.parent, S.post, S.count, plain, node_of_col);
+    std::string err = plain_tree(S.parent, S.post, S.count, opts.single_tree, plain, node_of_col);
     if (!err.empty()) return err;
     S.perm = S.amd_perm;
     S.count_final = S.count;

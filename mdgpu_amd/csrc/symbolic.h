@@ -38,6 +38,7 @@ struct AnalyzeOptions {
     bool use_amd = true;
     Partition partition = PART_PLAIN;
     int max_zeros = 0;
+    bool single_tree = false;   // reject forests as the reference does
 };
 
 // Everything the symbolic phase produces.

@@ -1,0 +1,100 @@
+"""Host mirror of the numeric phase over the C ABI (include/mdgpu_c.h).
+
+``Factorization(sym, precision)`` owns a device arena; ``factorize(ax)``
+replaces the reference's ``Factor(A, spL, listaNodos, cantNodos)``
+(mdgpu.cu:1394-1608), ``solve`` its ``cs_lsolve``/``cs_ltsolve`` pair
+(cs.cpp:1086-1119), ``refine`` adds FP64 iterative refinement to the
+reference's backward-error check (mdgpu.cu:1972), and ``export_L`` returns L
+in the reference's ``spL`` CSC layout.  Device vectors are torch tensors on
+the current device; torch is only used to hold memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+PRECISIONS = {"fp64": L.FP64, "fp32": L.FP32, "bf16": L.BF16}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class Factorization:
+    def __init__(self, sym, precision="fp64"):
+        self.sym = sym
+        self.precision = precision
+        h = C.c_void_p()
+        L.check(L.lib().mdgpu_numeric_create(sym.handle, PRECISIONS[precision], C.byref(h)))
+        self._h = h
+        self._ax = None
+
+    def info(self):
+        out = (C.c_int64 * L.NUM_NINFO)()
+        L.check(L.lib().mdgpu_numeric_info(self._h, out))
+        names = ["device_bytes", "lstore", "arena", "steps", "levels", "panel_fronts", "diag_jobs", "precision"]
+        return dict(zip(names, list(out)))
+
+    def factorize(self, ax):
+        """ax: float64 device tensor with the caller's lower-triangle values."""
+        torch = _torch()
+        ax = ax.to(device="cuda", dtype=torch.float64).contiguous()
+        self._ax = ax
+        L.check(L.lib().mdgpu_factorize(self._h, _dptr(ax), _stream()))
+        return self
+
+    def solve(self, b, out=None):
+        torch = _torch()
+        b = b.to(device="cuda", dtype=torch.float64).contiguous()
+        x = out if out is not None else torch.empty_like(b)
+        L.check(L.lib().mdgpu_solve(self._h, _dptr(b), _dptr(x), _stream()))
+        return x
+
+    def refine(self, b, tol=1e-12, maxit=30, out=None, raise_on_fail=True):
+        torch = _torch()
+        b = b.to(device="cuda", dtype=torch.float64).contiguous()
+        x = out if out is not None else torch.empty_like(b)
+        eta = C.c_double()
+        it = C.c_int()
+        st = L.lib().mdgpu_refine(self._h, _dptr(b), _dptr(x), tol, maxit, C.byref(eta), C.byref(it), _stream())
+        if st != L.MDGPU_OK and (raise_on_fail or st != L.MDGPU_ERR_NOT_CONVERGED):
+            L.check(st)
+        return x, eta.value, it.value
+
+    def logdet(self):
+        out = C.c_double()
+        L.check(L.lib().mdgpu_logdet(self._h, C.byref(out), _stream()))
+        return out.value
+
+    def export_L(self):
+        n, lnz = self.sym.n, self.sym.lnz
+        Lp = np.empty(n + 1, dtype=np.int32)
+        Li = np.empty(lnz, dtype=np.int32)
+        Lx = np.empty(lnz, dtype=np.float64)
+        L.check(L.lib().mdgpu_export_L(self._h, Lp.ctypes.data_as(L._I32P), Li.ctypes.data_as(L._I32P),
+                                       Lx.ctypes.data_as(L._DP)))
+        return Lp, Li, Lx
+
+    def close(self):
+        if self._h:
+            L.lib().mdgpu_numeric_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

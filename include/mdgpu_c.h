@@ -135,6 +135,16 @@ int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_t* out /* [
  * Synchronizes `stream` at the end (pivot check). */
 int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream);
 
+/* mdgpu_factorize with per-kernel-class timing: HIP events are recorded on
+ * `stream` around every launch.  out[4*k + {0,1,2,3}] = milliseconds,
+ * launches, algorithmic flops, algorithmic HBM bytes of kernel class k:
+ * 0 assembly (zero fill + A scatter), 1 extend-add, 2 LDS panel
+ * factorization, 3 diagonal blocks, 4 TRSM, 5 tiled GEMM/SYRK.  The event
+ * gaps add launch latency, so use mdgpu_factorize for end-to-end times. */
+#define MDGPU_KIND_N 6
+int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream,
+                            double* out /* [4*MDGPU_KIND_N] */);
+
 /* x = A^{-1} b with the stored factor (device vectors, FP64, caller's
  * numbering): permutation, supernodal forward and backward substitution.
  * Replaces cs_pvec + cs_lsolve + cs_ltsolve + cs_ipvec (cs.cpp:1086-1146,

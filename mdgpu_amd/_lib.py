@@ -91,6 +91,7 @@ def check(status):
 
 NUM_DEVICE_BYTES, NUM_LSTORE, NUM_ARENA, NUM_STEPS, NUM_LEVELS, NUM_PANEL_FRONTS, NUM_DIAG_JOBS, NUM_PRECISION = range(8)
 NUM_NINFO = 8
+KIND_NAMES = ["assembly", "extend_add", "panel", "diag", "trsm", "gemm"]
 
 _SIGNATURES.update({
     "mdgpu_numeric_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
@@ -98,6 +99,7 @@ _SIGNATURES.update({
     "mdgpu_numeric_info": (C.c_int, [_P, _I64P]),
     "mdgpu_plan_check": (C.c_int, [_P, C.c_int, _I64P]),
     "mdgpu_factorize": (C.c_int, [_P, _P, _P]),
+    "mdgpu_factorize_profile": (C.c_int, [_P, _P, _P, _DP]),
     "mdgpu_solve": (C.c_int, [_P, _P, _P, _P]),
     "mdgpu_refine": (C.c_int, [_P, _P, _P, C.c_double, C.c_int, _DP, C.POINTER(C.c_int), _P]),
     "mdgpu_logdet": (C.c_int, [_P, _DP, _P]),

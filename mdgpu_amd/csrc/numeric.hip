@@ -116,16 +116,35 @@ void release(mdgpu_numeric* N) {
         if (_st != MDGPU_OK) return _st; \
     } while (0)
 
+// Kernel classes reported by mdgpu_factorize_profile.
+enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_DIAG, KIND_TRSM, KIND_GEMM, KIND_N };
+
+int kind_of(int step_kind) {
+    switch (step_kind) {
+        case mdgpu::STEP_EA1:
+        case mdgpu::STEP_EA2: return KIND_EXTEND_ADD;
+        case mdgpu::STEP_PANEL: return KIND_PANEL;
+        case mdgpu::STEP_DIAG: return KIND_DIAG;
+        case mdgpu::STEP_TRSM: return KIND_TRSM;
+        default: return KIND_GEMM;
+    }
+}
+
+// One factorization; with `ev` non-null an event is recorded on the launch
+// stream before the first kernel and after every step (ev has steps+2 slots).
 template <typename T>
-int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s) {
+int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev) {
     const mdgpu::Plan& P = N->plan;
     T* L = (T*)N->dL;
     T* U = (T*)N->dU;
     T* inv = (T*)N->dInv;
+    if (ev) hipEventRecord(ev[0], s);
     mdgpu::launch_fill<T>(L, P.lstore, T(0), s);
     mdgpu::launch_fill<int>(N->dInfo, 1, INT_MAX, s);
     mdgpu::launch_scatter_a<T>(L, ax, N->dAdst, (int)P.a_dst.size(), s);
-    for (const mdgpu::Step& st : P.steps) {
+    if (ev) hipEventRecord(ev[1], s);
+    for (size_t t = 0; t < P.steps.size(); t++) {
+        const mdgpu::Step& st = P.steps[t];
         switch (st.kind) {
             case mdgpu::STEP_EA1:
             case mdgpu::STEP_EA2:
@@ -144,6 +163,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s) {
                 mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, L, U, s);
                 break;
         }
+        if (ev) hipEventRecord(ev[t + 2], s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "factorization launch");
@@ -292,13 +312,61 @@ extern "C" int mdgpu_numeric_info(const mdgpu_numeric* num, int64_t* out) {
     return MDGPU_OK;
 }
 
+namespace {
+int finish_factor(mdgpu_numeric* num, const double* d_ax, hipStream_t s);
+}
+
 extern "C" int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream) {
     mdgpu::clear_error();
     if (!num || !d_ax) return fail(MDGPU_ERR_ARG, "mdgpu_factorize: null pointer");
     hipStream_t s = (hipStream_t)stream;
     num->factored = false;
-    int st = num->precision == MDGPU_FP64 ? run_factor<double>(num, d_ax, s) : run_factor<float>(num, d_ax, s);
+    int st = num->precision == MDGPU_FP64 ? run_factor<double>(num, d_ax, s, nullptr)
+                                          : run_factor<float>(num, d_ax, s, nullptr);
     if (st != MDGPU_OK) return st;
+    return finish_factor(num, d_ax, s);
+}
+
+extern "C" int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream, double* out) {
+    mdgpu::clear_error();
+    if (!num || !d_ax || !out) return fail(MDGPU_ERR_ARG, "mdgpu_factorize_profile: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    num->factored = false;
+    const mdgpu::Plan& P = num->plan;
+    std::vector<hipEvent_t> ev(P.steps.size() + 2);
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(MDGPU_ERR_DEVICE, "hipEventCreate failed");
+    int st = num->precision == MDGPU_FP64 ? run_factor<double>(num, d_ax, s, ev.data())
+                                          : run_factor<float>(num, d_ax, s, ev.data());
+    if (st == MDGPU_OK) {
+        hipError_t e = hipEventSynchronize(ev.back());
+        if (e != hipSuccess) st = hip_fail(e, "hipEventSynchronize");
+    }
+    if (st == MDGPU_OK) {
+        for (int k = 0; k < 4 * KIND_N; k++) out[k] = 0;
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev[0], ev[1]);
+        const mdgpu::Symbolic& S = num->sym->S;
+        const double es = num->precision == MDGPU_FP64 ? 8 : 4;
+        out[4 * KIND_ASSEMBLY + 0] += ms;
+        out[4 * KIND_ASSEMBLY + 1] += 3;
+        out[4 * KIND_ASSEMBLY + 3] += P.lstore * es + S.nnz_in * (8.0 + 8.0 + es);
+        for (size_t t = 0; t < P.steps.size(); t++) {
+            hipEventElapsedTime(&ms, ev[t + 1], ev[t + 2]);
+            const int k = kind_of(P.steps[t].kind);
+            out[4 * k + 0] += ms;
+            out[4 * k + 1] += 1;
+            out[4 * k + 2] += P.steps[t].flops;
+            out[4 * k + 3] += P.steps[t].bytes;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    if (st != MDGPU_OK) return st;
+    return finish_factor(num, d_ax, s);
+}
+
+namespace {
+int finish_factor(mdgpu_numeric* num, const double* d_ax, hipStream_t s) {
     // full symmetric values and ||A||_1 for the residual of refinement
     const int n = num->sym->S.n;
     mdgpu::launch_gather(d_ax, num->dAfsrc, num->dAfx, (int)num->plan.afull_i.size(), s);
@@ -307,7 +375,7 @@ extern "C" int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* str
     hipError_t e = hipMemcpyAsync(&info, num->dInfo, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     double an = 0;
-    st = max_abs(num, num->dRowabs, n, s, &an);   // synchronizes the stream
+    int st = max_abs(num, num->dRowabs, n, s, &an);   // synchronizes the stream
     if (st != MDGPU_OK) return st;
     num->anorm = an;
     if (info != INT_MAX)
@@ -316,6 +384,7 @@ extern "C" int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* str
     num->factored = true;
     return MDGPU_OK;
 }
+}  // namespace
 
 extern "C" int mdgpu_solve(mdgpu_numeric* num, const double* d_b, double* d_x, void* stream) {
     mdgpu::clear_error();

@@ -185,6 +185,11 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
     auto flush_gemm = [&](std::vector<GemmJob>& batch, int tiles, int L) {
         if (batch.empty()) return;
         Step s{STEP_GEMM, (int)P.gemm.size(), (int)batch.size(), tiles, L, 0};
+        for (const GemmJob& g : batch) {
+            double entries = g.lower ? (double)g.N * g.M - 0.5 * (double)g.N * (g.N - 1) : (double)g.M * g.N;
+            s.flops += 2.0 * g.K * entries;
+            s.bytes += ((double)g.M + (g.lower ? 0.0 : g.N)) * g.K * o.elem_bytes + entries * o.elem_bytes * (g.beta0 ? 1 : 2);
+        }
         P.gemm.insert(P.gemm.end(), batch.begin(), batch.end());
         P.steps.push_back(s);
         batch.clear();
@@ -214,8 +219,18 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 }
             }
             const int count = (int)P.ea.size() - begin;
-            if (count > 0)
-                P.steps.push_back({first_pass ? STEP_EA1 : STEP_EA2, begin, count, count, L, 0});
+            if (count > 0) {
+                Step st{first_pass ? STEP_EA1 : STEP_EA2, begin, count, count, L, 0};
+                for (int t = begin; t < begin + count; t++) {
+                    const EaItem& it = P.ea[t];
+                    const int m = P.fronts[it.child].m;
+                    // lower-triangle entries of columns [j0, j1): read child, RMW parent
+                    double e = 0;
+                    for (int j = it.j0; j < it.j1; j++) e += m - j;
+                    st.bytes += e * 3.0 * o.elem_bytes + 4.0 * (it.j1 - it.j0) + 4.0 * e;
+                }
+                P.steps.push_back(st);
+            }
         }
     };
 
@@ -239,7 +254,18 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 }
             }
             const int count = (int)P.panel.size() - begin;
-            if (count > 0) P.steps.push_back({STEP_PANEL, begin, count, count, L, lds});
+            if (count > 0) {
+                Step st{STEP_PANEL, begin, count, count, L, lds};
+                for (int t = begin; t < begin + count; t++) {
+                    const FrontDesc& d = P.fronts[P.panel[t]];
+                    for (int j = 0; j < d.c; j++) {
+                        st.flops += (double)(d.n - j - 1);
+                        for (int col = j + 1; col < d.c; col++) st.flops += 2.0 * (d.n - col);
+                    }
+                    st.bytes += 2.0 * d.n * d.c * o.elem_bytes;
+                }
+                P.steps.push_back(st);
+            }
         }
         // large panels: blocked right-looking, nb columns per step
         int maxc = 0;
@@ -261,7 +287,15 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             const int dcount = (int)P.diag.size() - dbegin;
             if (dcount == 0) continue;
             max_diag_jobs = std::max<int64_t>(max_diag_jobs, dcount);
-            P.steps.push_back({STEP_DIAG, dbegin, dcount, dcount, L, 0});
+            {
+                Step st{STEP_DIAG, dbegin, dcount, dcount, L, 0};
+                for (int t = dbegin; t < dbegin + dcount; t++) {
+                    const double b = P.diag[t].b;
+                    st.flops += b * b * b / 3.0 + b * b * b / 3.0;
+                    st.bytes += 2.5 * b * b * o.elem_bytes;
+                }
+                P.steps.push_back(st);
+            }
             const int tbegin = (int)P.trsm.size();
             int ttiles = 0;
             for (int t = dbegin; t < dbegin + dcount; t++) {
@@ -282,7 +316,15 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 P.trsm.push_back(tj);
             }
             const int tcount = (int)P.trsm.size() - tbegin;
-            if (tcount > 0) P.steps.push_back({STEP_TRSM, tbegin, tcount, ttiles, L, 0});
+            if (tcount > 0) {
+                Step st{STEP_TRSM, tbegin, tcount, ttiles, L, 0};
+                for (int t = tbegin; t < tbegin + tcount; t++) {
+                    const TrsmJob& tj = P.trsm[t];
+                    st.flops += (double)tj.rows * tj.b * (tj.b + 1);
+                    st.bytes += 2.0 * tj.rows * tj.b * o.elem_bytes;
+                }
+                P.steps.push_back(st);
+            }
             std::vector<GemmJob> batch;
             int tiles = 0;
             for (int t = dbegin; t < dbegin + dcount; t++) {

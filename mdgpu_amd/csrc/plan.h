@@ -92,6 +92,8 @@ struct Step {
     int nblocks;           // workgroups to launch
     int level;
     int arg;               // panel: LDS bytes needed
+    double flops = 0;      // algorithmic flops of the launch
+    double bytes = 0;      // algorithmic HBM bytes of the launch
 };
 
 struct Plan {

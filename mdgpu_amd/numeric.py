@@ -56,6 +56,17 @@ class Factorization:
         L.check(L.lib().mdgpu_factorize(self._h, _dptr(ax), _stream()))
         return self
 
+    def factorize_profile(self, ax):
+        """Factorize with per-kernel-class HIP-event timing.  Returns
+        {class: {"ms", "launches", "flops", "bytes"}}."""
+        torch = _torch()
+        ax = ax.to(device="cuda", dtype=torch.float64).contiguous()
+        self._ax = ax
+        out = (C.c_double * (4 * len(L.KIND_NAMES)))()
+        L.check(L.lib().mdgpu_factorize_profile(self._h, _dptr(ax), _stream(), out))
+        return {k: {"ms": out[4 * t], "launches": int(out[4 * t + 1]), "flops": out[4 * t + 2],
+                    "bytes": out[4 * t + 3]} for t, k in enumerate(L.KIND_NAMES)}
+
     def solve(self, b, out=None):
         torch = _torch()
         b = b.to(device="cuda", dtype=torch.float64).contiguous()

@@ -1,0 +1,80 @@
+"""Generate the symbolic golden vectors from the reference's own code.
+
+Runs oracle/_ref/ref_symbolic (the reference's cs.cpp + etree.cpp, compiled
+from /root/reference by `make -C oracle ref`) on deterministic inputs and
+writes
+  tests/golden/symbolic_small.npz   full arrays for small cases
+  tests/golden/symbolic_hashes.json sha256 of every array for large cases
+Only data is committed (inputs are regenerated from mdgpu_amd.gen and
+checked against a stored hash of the input pattern).
+
+usage: python tests/golden/make_symbolic_golden.py [--large]
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from tests.golden.golden_cases import LARGE_CASES, SMALL_CASES, input_hash  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+ARRAYS = ["amd_perm", "parent", "post", "count", "relaxed_perm", "perm_final", "count_final",
+          "lp", "alow_p", "alow_i", "nodes"]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
+
+
+def run_case(name, make, mode, mz):
+    p, i, _ = make()
+    with tempfile.TemporaryDirectory() as d:
+        try:
+            out = O.run_ref_symbolic(p, i, d, True, mode, mz)
+        except Exception as e:  # the reference exits(1) on an elimination forest
+            out = {}
+            for fn in os.listdir(d):
+                if fn.endswith(".i32"):
+                    out[fn[:-4]] = np.fromfile(os.path.join(d, fn), dtype=np.int32)
+            out["_error"] = str(e)
+    return p, i, out
+
+
+def main():
+    large = "--large" in sys.argv
+    small = {}
+    meta = {}
+    for name, make, mode, mz in SMALL_CASES:
+        p, i, out = run_case(name, make, mode, mz)
+        meta[name] = {"input": input_hash(p, i), "forest": "_error" in out}
+        for k in ARRAYS:
+            if k in out:
+                small[f"{name}__{k}"] = out[k]
+        print(name, {k: len(v) for k, v in out.items() if k in ARRAYS}, "forest" if "_error" in out else "")
+    np.savez_compressed(os.path.join(HERE, "symbolic_small.npz"), **small)
+    with open(os.path.join(HERE, "symbolic_small_meta.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    if large:
+        hashes = {}
+        for name, make, mode, mz in LARGE_CASES:
+            p, i, out = run_case(name, make, mode, mz)
+            hashes[name] = {"input": input_hash(p, i)}
+            for k in ARRAYS:
+                if k in out:
+                    hashes[name][k] = [int(len(out[k])), sha(out[k])]
+            print(name, "done", flush=True)
+        with open(os.path.join(HERE, "symbolic_hashes.json"), "w") as f:
+            json.dump(hashes, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

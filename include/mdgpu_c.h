@@ -63,6 +63,17 @@ int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,
                   const mdgpu_analyze_opts* opts, mdgpu_symbolic** sym);
 int mdgpu_symbolic_destroy(mdgpu_symbolic* sym);
 
+/* Analysis for a caller-supplied ordering and partition: Cp/Ci is the
+ * lower triangle ALREADY permuted, and the nodes (first column, column
+ * count, children as node indices in kid_ptr/kids CSR form) are in
+ * postorder, children before parents -- exactly what the reference passes
+ * to Factor(A, spL, listaNodos, cantNodos) (mdgpu.cu:1394, 1889).  Children
+ * may be listed in any order.  L's column counts are derived from the front
+ * row lists. */
+int mdgpu_analyze_nodes(int n, const int32_t* Cp, const int32_t* Ci, int nnodes, const int32_t* first,
+                        const int32_t* ncols, const int32_t* kid_ptr, const int32_t* kids,
+                        mdgpu_symbolic** sym);
+
 /* Scalar facts about an analysis. */
 #define MDGPU_SYM_N 0             /* matrix order                          */
 #define MDGPU_SYM_NNODES 1        /* nodes (fronts) in the partition       */

@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libmdgpu_amd.so")
+CLI = os.path.join(HERE, "mdgpu")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
@@ -53,6 +54,9 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
     shutil.move(tmp, OUT)
+    # command-line driver over the drop-in C++ interface
+    _run(["g++", "-O2", "-std=c++17", os.path.join(HERE, "cli", "mdgpu_main.cpp"), "-o", CLI,
+          f"-L{HERE}", "-lmdgpu_amd", "-Wl,-rpath,$ORIGIN"])
     return OUT
 
 

@@ -1,5 +1,7 @@
 // C ABI: host-only planning entry point (no device calls), used to size a
 // factorization and to check its schedule before any kernel runs.
+#include <cstdlib>
+
 #include "common.h"
 #include "plan.h"
 
@@ -10,6 +12,9 @@ PlanOptions plan_options_for(int precision) {
     PlanOptions po;
     po.elem_bytes = precision == MDGPU_FP64 ? 8 : 4;
     po.panel_lds_bytes = 61440;
+    // GEMM tile: 128 selects the MFMA kernel, 64 the FMA kernel
+    if (const char* e = std::getenv("MDGPU_GEMM_TILE")) po.tile = std::atoi(e) == 64 ? 64 : 128;
+    else po.tile = 128;
     return po;
 }
 }  // namespace mdgpu

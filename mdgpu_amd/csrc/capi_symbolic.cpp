@@ -19,6 +19,19 @@ void clear_error() { g_last_error.clear(); }
 
 using mdgpu::fail;
 
+// levels (height above the leaves) and the largest front
+static void finish_handle(mdgpu_symbolic* h) {
+    const auto& S = h->S;
+    const int nn = (int)S.nodes.size();
+    h->level.assign(nn, 0);
+    for (int f = 0; f < nn; f++) {
+        int nf = (int)(S.q_off[f + 1] - S.q_off[f]);
+        h->max_front = std::max(h->max_front, nf);
+        for (int kid : S.nodes[f].kids) h->level[f] = std::max(h->level[f], h->level[kid] + 1);
+        h->nlevels = std::max(h->nlevels, h->level[f] + 1);
+    }
+}
+
 extern "C" const char* mdgpu_last_error(void) { return mdgpu::g_last_error.c_str(); }
 
 extern "C" int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,
@@ -47,15 +60,42 @@ extern "C" int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,
         delete h;
         return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze: out of host memory");
     }
-    const auto& S = h->S;
-    const int nn = (int)S.nodes.size();
-    h->level.assign(nn, 0);
-    for (int f = 0; f < nn; f++) {
-        int nf = (int)(S.q_off[f + 1] - S.q_off[f]);
-        h->max_front = std::max(h->max_front, nf);
-        for (int kid : S.nodes[f].kids) h->level[f] = std::max(h->level[f], h->level[kid] + 1);
-        h->nlevels = std::max(h->nlevels, h->level[f] + 1);
+    finish_handle(h);
+    *sym = h;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_analyze_nodes(int n, const int32_t* Cp, const int32_t* Ci, int nnodes, const int32_t* first,
+                                   const int32_t* ncols, const int32_t* kid_ptr, const int32_t* kids,
+                                   mdgpu_symbolic** sym) {
+    mdgpu::clear_error();
+    if (!Cp || !Ci || !first || !ncols || !kid_ptr || !sym || nnodes <= 0)
+        return fail(MDGPU_ERR_ARG, "mdgpu_analyze_nodes: invalid argument");
+    *sym = nullptr;
+    std::vector<mdgpu::PartNode> nodes(nnodes);
+    for (int f = 0; f < nnodes; f++) {
+        nodes[f].first = first[f];
+        nodes[f].ncols = ncols[f];
+        nodes[f].rows = 0;
+        for (int t = kid_ptr[f]; t < kid_ptr[f + 1]; t++) nodes[f].kids.push_back(kids[t]);
+        std::sort(nodes[f].kids.begin(), nodes[f].kids.end());
+        for (int j = 0; j < ncols[f]; j++) nodes[f].cols.push_back(first[f] + j);
     }
+    mdgpu_symbolic* h = new (std::nothrow) mdgpu_symbolic();
+    if (!h) return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze_nodes: out of host memory");
+    try {
+        std::string err = mdgpu::analyze_with_nodes(n, Cp, Ci, nodes, h->S);
+        if (!err.empty()) {
+            delete h;
+            return fail(MDGPU_ERR_ARG, "mdgpu_analyze_nodes: " + err);
+        }
+    } catch (const std::bad_alloc&) {
+        delete h;
+        return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze_nodes: out of host memory");
+    }
+    for (int f = 0; f < nnodes; f++)
+        h->S.nodes[f].rows = (int)(h->S.q_off[f + 1] - h->S.q_off[f]);
+    finish_handle(h);
     *sym = h;
     return MDGPU_OK;
 }

@@ -22,8 +22,9 @@ template <typename T>
 void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s);
 template <typename T>
 void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv, hipStream_t s);
+// tile 128: MFMA kernel; tile 64: FMA kernel (the plan's tile size decides)
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, T* L, T* U, hipStream_t s);
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s);
 
 // solves (compute in FP64, factor stored as T)
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s);

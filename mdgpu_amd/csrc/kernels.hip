@@ -284,6 +284,113 @@ __global__ __launch_bounds__(kThreads) void gemm_nt_kernel(const GemmJob* __rest
     }
 }
 
+// MFMA version of gemm_nt_kernel for gfx950: 128x128 C tiles, four
+// wavefronts of 64x64, v_mfma_{f32,f64}_16x16x4.  The contraction index of
+// one 16x16x4 instruction is free to relabel as long as both operands agree,
+// so lane group g = lane/16 supplies k = 4g+t in sub-step t: each lane then
+// reads a contiguous 4-vector of k from LDS.  The product is formed as
+// (B A^T) so that each lane's four accumulator rows map to four C columns
+// and a 16-lane group stores 16 consecutive C rows.
+template <typename T> struct MfmaOp;
+template <> struct MfmaOp<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MfmaOp<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
+                                                             T* __restrict__ L, T* __restrict__ U) {
+    constexpr int BT = 128, KT = 16, LDK = KT + 4;
+    typedef typename MfmaOp<T>::acc_t acc_t;
+    __shared__ __align__(16) T As[BT * LDK];
+    __shared__ __align__(16) T Bs[BT * LDK];
+    int lo = 0, hi = njobs - 1;
+    const int bid = blockIdx.x;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
+    }
+    const GemmJob g = jobs[lo];
+    int local = bid - g.tile_base;
+    int ti, tj;
+    if (g.lower) {
+        tj = 0;
+        int cnt = g.tiles_m;
+        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+        ti = tj + local;
+    } else {
+        tj = local / g.tiles_m;
+        ti = local % g.tiles_m;
+    }
+    const T* __restrict__ A = L + g.a_off;
+    const T* __restrict__ B = L + g.b_off;
+    T* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+    const int row0 = ti * BT, col0 = tj * BT;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    const int li = lane & 15, lg = lane >> 4;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
+    for (int k0 = 0; k0 < g.K; k0 += KT) {
+        // stage A rows row0..+127 and B rows col0..+127, k0..k0+15, as [row][k]
+#pragma unroll
+        for (int s = 0; s < (BT * KT) / kThreads; ++s) {
+            const int idx = tid + kThreads * s;
+            const int r = idx & (BT - 1), kk = idx >> 7;
+            const int k = k0 + kk;
+            const int ia = row0 + r, jb = col0 + r;
+            As[r * LDK + kk] = (k < g.K && ia < g.M) ? A[ia + (int64_t)k * g.lda] : T(0);
+            Bs[r * LDK + kk] = (k < g.K && jb < g.N) ? B[jb + (int64_t)k * g.ldb] : T(0);
+        }
+        __syncthreads();
+        T av[4][4], bv[4][4];   // [block][t]
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) av[rb][t] = As[(wr + rb * 16 + li) * LDK + 4 * lg + t];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bv[cb][t] = Bs[(wc + cb * 16 + li) * LDK + 4 * lg + t];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb][t], acc[rb][cb]);
+        __syncthreads();
+    }
+    // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + 4*lg + v)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = col0 + wc + cb * 16 + 4 * lg + v;
+            if (j >= g.N) continue;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                const int i = row0 + wr + rb * 16 + li;
+                if (i >= g.M || (g.lower && i < j)) continue;
+                T* cp = C + i + (int64_t)j * g.ldc;
+                *cp = g.beta0 ? -acc[rb][cb][v] : *cp - acc[rb][cb][v];
+            }
+        }
+}
+
 // ------------------------------------------------------------------ solves
 __global__ void permute_in_kernel(const double* __restrict__ b, const int* __restrict__ perm,
                                   double* __restrict__ xp, int n) {
@@ -508,8 +615,10 @@ void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv
     if (nblocks > 0) trsm_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, inv);
 }
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, T* L, T* U, hipStream_t s) {
-    if (nblocks > 0) gemm_nt_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s) {
+    if (nblocks <= 0) return;
+    if (tile == 128) gemm_mfma_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    else gemm_nt_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
 }
 
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s) {
@@ -560,7 +669,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
     template void launch_diag<T>(const DiagJob*, int, T*, T*, int*, hipStream_t);                             \
     template void launch_trsm<T>(const TrsmJob*, int, int, T*, const T*, hipStream_t);                         \
-    template void launch_gemm<T>(const GemmJob*, int, int, T*, T*, hipStream_t);                               \
+    template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, double*, double*, const int64_t*, hipStream_t);                      \
     template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, double*,              \

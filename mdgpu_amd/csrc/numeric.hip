@@ -64,6 +64,7 @@ struct mdgpu_numeric {
     unsigned long long* dMax = nullptr;
     double* dPartial = nullptr;
     int nparts = 0;
+    int tile = 64;
     double anorm = -1;
     bool factored = false;
     int64_t device_bytes = 0;
@@ -160,7 +161,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_trsm<T>(N->dTrsm + st.first, st.count, st.nblocks, L, inv, s);
                 break;
             case mdgpu::STEP_GEMM:
-                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, L, U, s);
+                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, N->tile, L, U, s);
                 break;
         }
         if (ev) hipEventRecord(ev[t + 2], s);
@@ -257,6 +258,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         for (int j = 0; j < d.c; j++) diag_off[d.k + j] = d.lofs + (int64_t)j * d.n + j;
     }
     N->nparts = kLogParts;
+    N->tile = po.tile;
     if ((st = alloc(N, P.lstore * es, &N->dL)) ||
         (st = alloc(N, P.arena * es, &N->dU)) ||
         (st = alloc(N, P.inv_scratch * es, &N->dInv)) ||

@@ -698,12 +698,74 @@ std::string relaxed_tree(const std::vector<PartNode>& plain, int max_zeros, std:
 
 }  // namespace
 
-std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts, Symbolic& S) {
-    S = Symbolic();
-    S.n = n;
-    S.opts = opts;
+void supernodal_partition(const std::vector<PartNode>& plain, std::vector<PartNode>& out) {
+    supernodal_tree(plain, out);
+}
+
+std::string relaxed_partition(const std::vector<PartNode>& plain, int max_zeros, std::vector<int>& count,
+                              std::vector<int>& perm, std::vector<PartNode>& out) {
+    return relaxed_tree(plain, max_zeros, count, perm, out);
+}
+
+Csc full_pattern(const Csc& low) { return full_from_lower(low); }
+
+// Front row lists: pivot columns, rows of A below them, and the update
+// rows of every child -- the union MatrizGetFk_3 and VectorMerge build per
+// factorization (mdgpu.cu:180-291, 1456-1508).  With check_counts the
+// front sizes must agree with count_final (spL->p); otherwise count_final
+// and lp are derived from the fronts.
+std::string build_fronts(Symbolic& S, bool check_counts) {
+    const int n = S.n;
+    const int nn = (int)S.nodes.size();
+    S.front_parent.assign(nn, -1);
+    for (int f = 0; f < nn; f++)
+        for (int kid : S.nodes[f].kids) {
+            if (kid < 0 || kid >= f) return "node list is not in postorder";
+            S.front_parent[kid] = f;
+        }
+    S.q_off.assign(nn + 1, 0);
+    S.q_all.clear();
+    std::vector<int> stamp(n, -1);
+    std::vector<int> rows;
+    for (int f = 0; f < nn; f++) {
+        const PartNode& nd = S.nodes[f];
+        const int k = nd.first, c = nd.ncols;
+        if (k < 0 || k + c > n) return "node columns out of range";
+        rows.clear();
+        for (int j = k; j < k + c; j++) { stamp[j] = f; rows.push_back(j); }
+        for (int j = k; j < k + c; j++)
+            for (int p = S.a_low.p[j]; p < S.a_low.p[j + 1]; p++) {
+                int r = S.a_low.i[p];
+                if (stamp[r] != f) { stamp[r] = f; rows.push_back(r); }
+            }
+        for (int kid : nd.kids) {
+            const int kc = S.nodes[kid].ncols;
+            for (int64_t t = S.q_off[kid] + kc; t < S.q_off[kid + 1]; t++) {
+                int r = S.q_all[t];
+                if (stamp[r] != f) { stamp[r] = f; rows.push_back(r); }
+            }
+        }
+        std::sort(rows.begin() + c, rows.end());
+        if ((int)rows.size() > c && rows[c] < k + c) return "front rows precede the pivot block";
+        const int nf = (int)rows.size();
+        if (!check_counts)
+            for (int j = 0; j < c; j++) S.count_final[k + j] = nf - j;
+        for (int j = 0; j < c; j++)
+            if (S.count_final[k + j] != nf - j)
+                return "front of node " + std::to_string(f) + " has " + std::to_string(nf) +
+                       " rows but column " + std::to_string(k + j) + " count is " +
+                       std::to_string(S.count_final[k + j]);
+        S.q_all.insert(S.q_all.end(), rows.begin(), rows.end());
+        S.q_off[f + 1] = (int64_t)S.q_all.size();
+    }
+    return "";
+}
+
+namespace {
+
+// Caller's lower triangle: sorted unique rows, diagonal present.
+std::string read_lower(int n, const int* Cp, const int* Ci, Csc& low) {
     if (n <= 0) return "matrix order must be positive";
-    Csc low;
     low.n = n;
     low.p.assign(Cp, Cp + n + 1);
     if (low.p[0] != 0) return "column pointers must start at 0";
@@ -721,6 +783,19 @@ std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& o
         }
         if (!has_diag) return "missing diagonal entry in column " + std::to_string(j);
     }
+    return "";
+}
+
+}  // namespace
+
+std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts, Symbolic& S) {
+    S = Symbolic();
+    S.n = n;
+    S.opts = opts;
+    Csc low;
+    std::string verr = read_lower(n, Cp, Ci, low);
+    if (!verr.empty()) return verr;
+    const int nz = low.p[n];
     S.nnz_in = nz;
     std::vector<int> ident(nz);
     for (int p = 0; p < nz; p++) ident[p] = p;
@@ -783,48 +858,48 @@ std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& o
         S.flops_padded += (double)S.count_final[k] * S.count_final[k] + 1.0;
     }
 
-    // 4. front row lists: pivot columns, rows of A below them, and the
-    //    update rows of every child (the union MatrizGetFk_3 and VectorMerge
-    //    build per factorization, mdgpu.cu:180-291, 1456-1508)
-    const int nn = (int)S.nodes.size();
-    S.front_parent.assign(nn, -1);
-    for (int f = 0; f < nn; f++)
-        for (int kid : S.nodes[f].kids) {
-            if (kid < 0 || kid >= f) return "node list is not in postorder";
-            S.front_parent[kid] = f;
-        }
-    S.q_off.assign(nn + 1, 0);
-    S.q_all.clear();
-    std::vector<int> stamp(n, -1);
-    std::vector<int> rows;
-    for (int f = 0; f < nn; f++) {
-        const PartNode& nd = S.nodes[f];
-        const int k = nd.first, c = nd.ncols;
-        if (k < 0 || k + c > n) return "node columns out of range";
-        rows.clear();
-        for (int j = k; j < k + c; j++) { stamp[j] = f; rows.push_back(j); }
-        for (int j = k; j < k + c; j++)
-            for (int p = S.a_low.p[j]; p < S.a_low.p[j + 1]; p++) {
-                int r = S.a_low.i[p];
-                if (stamp[r] != f) { stamp[r] = f; rows.push_back(r); }
-            }
-        for (int kid : nd.kids) {
-            const int kc = S.nodes[kid].ncols;
-            for (int64_t t = S.q_off[kid] + kc; t < S.q_off[kid + 1]; t++) {
-                int r = S.q_all[t];
-                if (stamp[r] != f) { stamp[r] = f; rows.push_back(r); }
-            }
-        }
-        std::sort(rows.begin() + c, rows.end());
-        if ((int)rows.size() > c && rows[c] < k + c) return "front rows precede the pivot block";
-        const int nf = (int)rows.size();
-        for (int j = 0; j < c; j++)
-            if (S.count_final[k + j] != nf - j)
-                return "front of node " + std::to_string(f) + " has " + std::to_string(nf) +
-                       " rows but column " + std::to_string(k + j) + " count is " +
-                       std::to_string(S.count_final[k + j]);
-        S.q_all.insert(S.q_all.end(), rows.begin(), rows.end());
-        S.q_off[f + 1] = (int64_t)S.q_all.size();
+    return build_fronts(S, true);
+}
+
+std::string analyze_with_nodes(int n, const int* Cp, const int* Ci, const std::vector<PartNode>& nodes,
+                               Symbolic& S) {
+    S = Symbolic();
+    S.n = n;
+    S.opts.use_amd = false;
+    S.opts.partition = PART_SUPERNODAL;
+    Csc low;
+    std::string err = read_lower(n, Cp, Ci, low);
+    if (!err.empty()) return err;
+    const int nz = low.p[n];
+    S.nnz_in = nz;
+    S.a_src.resize(nz);
+    for (int p = 0; p < nz; p++) S.a_src[p] = p;
+    S.amd_perm.resize(n);
+    for (int k = 0; k < n; k++) S.amd_perm[k] = k;
+    S.perm = S.amd_perm;
+    S.pinv = S.amd_perm;
+    Csc full = full_from_lower(low);
+    S.parent = etree(full);
+    S.post = postorder(S.parent);
+    S.count = column_counts(full, S.parent, S.post);
+    S.a_low = std::move(low);
+    std::vector<int> covered(n, 0);
+    for (const PartNode& nd : nodes) {
+        if (nd.ncols < 1 || nd.first < 0 || nd.first + nd.ncols > n) return "node columns out of range";
+        for (int j = nd.first; j < nd.first + nd.ncols; j++) covered[j]++;
+    }
+    for (int j = 0; j < n; j++)
+        if (covered[j] != 1) return "column " + std::to_string(j) + " is not in exactly one node";
+    S.nodes = nodes;
+    S.count_final.assign(n, 0);
+    err = build_fronts(S, false);
+    if (!err.empty()) return err;
+    S.lp.assign(n + 1, 0);
+    for (int k = 0; k < n; k++) S.lp[k + 1] = S.lp[k] + S.count_final[k];
+    S.lnz = S.lp[n];
+    for (int k = 0; k < n; k++) {
+        S.flops_counts += (double)S.count[k] * S.count[k];
+        S.flops_padded += (double)S.count_final[k] * S.count_final[k] + 1.0;
     }
     return "";
 }

@@ -85,4 +85,20 @@ std::vector<int> column_counts(const Csc& full, const std::vector<int>& parent,
 std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts,
                     Symbolic& out);
 
+// Partition builders on a plain (one column per node) postordered tree:
+// fundamental supernodes (GetSuperETree, etree.cpp:50-118) and relaxed
+// supernodes (GetRelaxedETree, etree.cpp:194-342; pads `count`, returns the
+// new permutation).
+void supernodal_partition(const std::vector<PartNode>& plain, std::vector<PartNode>& out);
+std::string relaxed_partition(const std::vector<PartNode>& plain, int max_zeros, std::vector<int>& count,
+                              std::vector<int>& perm, std::vector<PartNode>& out);
+// lower triangle -> full symmetric pattern (make_sym, mdgpu.cu:1630-1638)
+Csc full_pattern(const Csc& low);
+
+// Analysis for a caller-supplied ordering and partition (the reference's
+// Factor() input: A already permuted, postordered node list): identity
+// permutation, front row lists derived, L counts taken from the fronts.
+std::string analyze_with_nodes(int n, const int* Cp, const int* Ci, const std::vector<PartNode>& nodes,
+                               Symbolic& out);
+
 }  // namespace mdgpu

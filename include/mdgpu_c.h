@@ -121,6 +121,12 @@ int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_t* dst, int
 int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, mdgpu_numeric** num);
 int mdgpu_numeric_destroy(mdgpu_numeric* num);
 
+/* Replay factorization and solves from hipGraphs captured on first use
+ * (enable = 1, the default) or launch every kernel eagerly (enable = 0).
+ * The graphs bind internal buffers, so callers' pointers may change between
+ * calls.  No counterpart in the reference. */
+int mdgpu_numeric_set_graph(mdgpu_numeric* num, int enable);
+
 #define MDGPU_NUM_DEVICE_BYTES 0  /* device memory held by the handle     */
 #define MDGPU_NUM_LSTORE 1        /* elements of the L store               */
 #define MDGPU_NUM_ARENA 2         /* elements of the update arena (peak)   */

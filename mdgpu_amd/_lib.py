@@ -96,6 +96,7 @@ KIND_NAMES = ["assembly", "extend_add", "panel", "diag", "trsm", "gemm"]
 _SIGNATURES.update({
     "mdgpu_numeric_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "mdgpu_numeric_destroy": (C.c_int, [_P]),
+    "mdgpu_numeric_set_graph": (C.c_int, [_P, C.c_int]),
     "mdgpu_numeric_info": (C.c_int, [_P, _I64P]),
     "mdgpu_plan_check": (C.c_int, [_P, C.c_int, _I64P]),
     "mdgpu_factorize": (C.c_int, [_P, _P, _P]),

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -65,6 +66,16 @@ struct mdgpu_numeric {
     double* dPartial = nullptr;
     int nparts = 0;
     int tile = 64;
+    // hipGraph replay of the fixed launch sequences (refactorization and
+    // solves on one pattern): captured on cap_stream against the internal
+    // buffers dAx / dB / dX, launched on the caller's stream
+    bool use_graph = true;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t g_factor = nullptr;
+    hipGraphExec_t g_solve = nullptr;
+    double* dAx = nullptr;
+    double* dB = nullptr;
+    double* dX = nullptr;
     double anorm = -1;
     bool factored = false;
     int64_t device_bytes = 0;
@@ -107,8 +118,32 @@ int alloc(mdgpu_numeric* N, size_t bytes, void** out) {
 }
 
 void release(mdgpu_numeric* N) {
-    for (auto& b : N->bufs) hipFree(b.p);
+    if (N->g_factor) (void)hipGraphExecDestroy(N->g_factor);
+    if (N->g_solve) (void)hipGraphExecDestroy(N->g_solve);
+    N->g_factor = N->g_solve = nullptr;
+    if (N->cap_stream) (void)hipStreamDestroy(N->cap_stream);
+    N->cap_stream = nullptr;
+    for (auto& b : N->bufs) (void)hipFree(b.p);
     N->bufs.clear();
+}
+
+// Capture `body(stream)` once into an executable graph.
+template <typename F>
+int capture(mdgpu_numeric* N, hipGraphExec_t* out, F body) {
+    hipError_t e = hipStreamBeginCapture(N->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    int st = body(N->cap_stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(N->cap_stream, &g);
+    if (st != MDGPU_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    return MDGPU_OK;
 }
 
 #define TRY(x)                        \
@@ -192,9 +227,23 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
     return MDGPU_OK;
 }
 
-int solve_any(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
+int solve_direct(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
     if (N->precision == MDGPU_FP64) return run_solve<double>(N, b, x, s);
     return run_solve<float>(N, b, x, s);
+}
+
+int solve_any(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
+    if (!N->use_graph) return solve_direct(N, b, x, s);
+    const size_t bytes = sizeof(double) * N->sym->S.n;
+    if (!N->g_solve) {
+        int st = capture(N, &N->g_solve, [&](hipStream_t cs) { return solve_direct(N, N->dB, N->dX, cs); });
+        if (st != MDGPU_OK) return st;
+    }
+    hipError_t e = hipMemcpyAsync(N->dB, b, bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipGraphLaunch(N->g_solve, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(x, N->dX, bytes, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "solve graph");
+    return MDGPU_OK;
 }
 
 int max_abs(mdgpu_numeric* N, const double* v, int n, hipStream_t s, double* out) {
@@ -259,6 +308,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     }
     N->nparts = kLogParts;
     N->tile = po.tile;
+    if (const char* g = std::getenv("MDGPU_GRAPH")) N->use_graph = std::atoi(g) != 0;
     if ((st = alloc(N, P.lstore * es, &N->dL)) ||
         (st = alloc(N, P.arena * es, &N->dU)) ||
         (st = alloc(N, P.inv_scratch * es, &N->dInv)) ||
@@ -287,9 +337,20 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, diag_off, (void**)&N->dDiagOff)) ||
         (st = alloc(N, sizeof(int), (void**)&N->dInfo)) ||
         (st = alloc(N, sizeof(unsigned long long), (void**)&N->dMax)) ||
-        (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)))
+        (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)) ||
+        (st = alloc(N, std::max<size_t>(P.a_dst.size(), 1) * sizeof(double), (void**)&N->dAx)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dB)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dX)))
         return guard(st);
+    if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess)
+        return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: hipStreamCreate failed"));
     *out = N;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_numeric_set_graph(mdgpu_numeric* num, int enable) {
+    if (!num) return fail(MDGPU_ERR_ARG, "mdgpu_numeric_set_graph: null pointer");
+    num->use_graph = enable != 0;
     return MDGPU_OK;
 }
 
@@ -323,8 +384,24 @@ extern "C" int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* str
     if (!num || !d_ax) return fail(MDGPU_ERR_ARG, "mdgpu_factorize: null pointer");
     hipStream_t s = (hipStream_t)stream;
     num->factored = false;
-    int st = num->precision == MDGPU_FP64 ? run_factor<double>(num, d_ax, s, nullptr)
+    int st;
+    if (num->use_graph) {
+        if (!num->g_factor) {
+            st = capture(num, &num->g_factor, [&](hipStream_t cs) {
+                return num->precision == MDGPU_FP64 ? run_factor<double>(num, num->dAx, cs, nullptr)
+                                                    : run_factor<float>(num, num->dAx, cs, nullptr);
+            });
+            if (st != MDGPU_OK) return st;
+        }
+        hipError_t e = hipMemcpyAsync(num->dAx, d_ax, sizeof(double) * num->plan.a_dst.size(),
+                                      hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipGraphLaunch(num->g_factor, s);
+        if (e != hipSuccess) return hip_fail(e, "factorization graph");
+        st = MDGPU_OK;
+    } else {
+        st = num->precision == MDGPU_FP64 ? run_factor<double>(num, d_ax, s, nullptr)
                                           : run_factor<float>(num, d_ax, s, nullptr);
+    }
     if (st != MDGPU_OK) return st;
     return finish_factor(num, d_ax, s);
 }

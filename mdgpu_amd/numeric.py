@@ -42,6 +42,10 @@ class Factorization:
         self._h = h
         self._ax = None
 
+    def set_graph(self, enable):
+        L.check(L.lib().mdgpu_numeric_set_graph(self._h, int(bool(enable))))
+        return self
+
     def info(self):
         out = (C.c_int64 * L.NUM_NINFO)()
         L.check(L.lib().mdgpu_numeric_info(self._h, out))

@@ -49,3 +49,28 @@ def test_fp32_refined_to_1e12(device, name, make, mode, mz):
     from oracle import oracle as O
     eta_cpu = O.backward_error(p, i, x, xs.cpu().numpy(), b)
     assert eta <= 1e-12 and eta_cpu <= 1e-12, (eta, eta_cpu, iters)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_graph_replays_match_eager(device, prec):
+    """The captured factorization/solve graphs are replayed several times with
+    eager work and synchronizations in between; every replay must equal the
+    eager launch sequence bit for bit."""
+    import torch
+    p, i, x = gen.laplace3d_7pt(14)
+    xt, b = gen.rhs_from_xtrue(p, i, x, seed=4)
+    S = symbolic.analyze(p, i, True, "rn", 2000)
+    ax = torch.from_numpy(x).to(device)
+    bd = torch.from_numpy(b).to(device)
+    E = Factorization(S, prec).set_graph(False).factorize(ax)
+    _, _, lx_eager = E.export_L()
+    xe = E.solve(bd).cpu().numpy()
+    G = Factorization(S, prec).set_graph(True)
+    for rep in range(3):
+        G.factorize(ax)
+        _, _, lx = G.export_L()
+        assert np.array_equal(lx, lx_eager), rep
+        xs = G.solve(bd).cpu().numpy()
+        assert np.array_equal(xs, xe), rep
+        E.solve(bd)                      # eager work between replays
+        torch.cuda.synchronize()

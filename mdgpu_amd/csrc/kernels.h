@@ -36,6 +36,22 @@ template <typename T>
 void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, double* xp,
                 hipStream_t s);
 
+// solves of fronts with more than kSolveBW pivots (see SolveStep)
+void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
+                       double* xp, double* su, const int64_t* suofs, hipStream_t s);
+template <typename T>
+void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
+                    hipStream_t s);
+template <typename T>
+void launch_fwd_gemv(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
+                     double* su, const int64_t* suofs, hipStream_t s);
+template <typename T>
+void launch_bwd_dot(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
+                    const double* xp, double* partial, hipStream_t s);
+template <typename T>
+void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
+                    const double* partial, hipStream_t s);
+
 // refinement helpers (FP64)
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);
 void launch_residual(const int* rp, const int* ci, const double* av, const double* x, const double* b, double* r,

@@ -508,6 +508,181 @@ __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ l
     }
 }
 
+// ------------------------------------------------- solves of large fronts
+// Fronts with more than kSolveBW pivots: forward / backward substitution 256
+// pivot columns at a time, the rows below each block handled by many
+// workgroups (row chunks of 256).
+
+// Forward, step 0: clear the front's update vector and gather its
+// children's update vectors (the first phase of fwd_kernel).
+__global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restrict__ list,
+                                                              const FrontDesc* __restrict__ fronts,
+                                                              const int* __restrict__ kids,
+                                                              const int* __restrict__ rel,
+                                                              double* __restrict__ xp, double* __restrict__ su,
+                                                              const int64_t* __restrict__ suofs) {
+    const int f = list[blockIdx.x];
+    const FrontDesc d = fronts[f];
+    double* __restrict__ u = su + suofs[f];
+    for (int t = threadIdx.x; t < d.m; t += kThreads) u[t] = 0.0;
+    __syncthreads();
+    for (int t = d.kid_begin; t < d.kid_end; ++t) {
+        const int ch = kids[t];
+        const FrontDesc cd = fronts[ch];
+        const int* __restrict__ r = rel + cd.relofs;
+        const double* __restrict__ cu = su + suofs[ch];
+        for (int i = threadIdx.x; i < cd.m; i += kThreads) {
+            const int pos = r[i];
+            if (pos < d.c) xp[d.k + pos] += cu[i];
+            else u[pos - d.c] += cu[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Forward: triangle of one 256-column block (64-column sub-blocks, each
+// solved on one wavefront, then applied to the block's remaining rows).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fwd_tri_kernel(const SolveItem* __restrict__ items,
+                                                           const FrontDesc* __restrict__ fronts,
+                                                           const T* __restrict__ L, double* __restrict__ xp) {
+    __shared__ double D[64 * 65];
+    __shared__ double yb[64];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDesc d = fronts[it.front];
+    const int n = d.n, k = d.k;
+    const int j1 = min(it.j0 + kSolveBW, d.c);
+    const T* __restrict__ P = L + d.lofs;
+    for (int s0 = it.j0; s0 < j1; s0 += 64) {
+        const int b = min(64, j1 - s0);
+        for (int t = threadIdx.x; t < b * b; t += kThreads) {
+            int i = t % b, jj = t / b;
+            D[i + jj * 65] = to_d(P[(s0 + i) + (int64_t)(s0 + jj) * n]);
+        }
+        for (int t = threadIdx.x; t < b; t += kThreads) yb[t] = xp[k + s0 + t];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            double xv = lane < b ? yb[lane] : 0.0;
+            for (int jj = 0; jj < b; ++jj) {
+                const double yj = __shfl(xv, jj, 64) / D[jj + jj * 65];
+                if (lane == jj) xv = yj;
+                else if (lane > jj && lane < b) xv -= D[lane + jj * 65] * yj;
+            }
+            if (lane < b) yb[lane] = xv;
+        }
+        __syncthreads();
+        for (int i = s0 + b + threadIdx.x; i < j1; i += kThreads) {
+            double acc = 0.0;
+            for (int jj = 0; jj < b; ++jj) acc += to_d(P[i + (int64_t)(s0 + jj) * n]) * yb[jj];
+            xp[k + i] -= acc;
+        }
+        for (int t = threadIdx.x; t < b; t += kThreads) xp[k + s0 + t] = yb[t];
+        __syncthreads();
+    }
+}
+
+// Forward: rows [a, a+256) below a solved block -= L(rows, block) * y(block).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fwd_gemv_kernel(const SolveItem* __restrict__ items,
+                                                            const FrontDesc* __restrict__ fronts,
+                                                            const T* __restrict__ L, double* __restrict__ xp,
+                                                            double* __restrict__ su,
+                                                            const int64_t* __restrict__ suofs) {
+    __shared__ double yb[kSolveBW];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDesc d = fronts[it.front];
+    const int j1 = min(it.j0 + kSolveBW, d.c), bw = j1 - it.j0;
+    for (int t = threadIdx.x; t < bw; t += kThreads) yb[t] = xp[d.k + it.j0 + t];
+    __syncthreads();
+    const int i = it.a + threadIdx.x;
+    if (i >= d.n) return;
+    const T* __restrict__ P = L + d.lofs + i + (int64_t)it.j0 * d.n;
+    double acc = 0.0;
+    for (int jj = 0; jj < bw; ++jj) acc += to_d(P[(int64_t)jj * d.n]) * yb[jj];
+    if (i < d.c) xp[d.k + i] -= acc;
+    else su[suofs[it.front] + i - d.c] -= acc;
+}
+
+// Backward: partial dot products of the block's columns with x over 256
+// rows below the block; partial[slot][j] (fixed-order reduction later).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __restrict__ items,
+                                                           const FrontDesc* __restrict__ fronts,
+                                                           const int* __restrict__ qall, const T* __restrict__ L,
+                                                           const double* __restrict__ xp,
+                                                           double* __restrict__ partial) {
+    __shared__ double red[kThreads / 64][kSolveBW];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDesc d = fronts[it.front];
+    const int j1 = min(it.j0 + kSolveBW, d.c), bw = j1 - it.j0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = it.a + threadIdx.x;
+    const bool live = i < d.n;
+    const double xq = live ? xp[qall[d.qofs + i]] : 0.0;
+    const T* __restrict__ P = L + d.lofs + (live ? i : 0) + (int64_t)it.j0 * d.n;
+    for (int jj = 0; jj < bw; ++jj) {
+        double v = live ? to_d(P[(int64_t)jj * d.n]) * xq : 0.0;
+        v = wave_sum(v);
+        if (lane == 0) red[wave][jj] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < bw; t += kThreads) {
+        double sum = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) sum += red[w][t];
+        partial[(int64_t)it.b * kSolveBW + t] = sum;
+    }
+}
+
+// Backward: v = y(block) - sum of partials, then L(block)^T x = v in
+// 64-column sub-blocks from the last.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void bwd_tri_kernel(const SolveItem* __restrict__ items,
+                                                           const FrontDesc* __restrict__ fronts,
+                                                           const T* __restrict__ L, double* __restrict__ xp,
+                                                           const double* __restrict__ partial) {
+    __shared__ double D[64 * 65];
+    __shared__ double vb[kSolveBW];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDesc d = fronts[it.front];
+    const int n = d.n, k = d.k, j0 = it.j0;
+    const int j1 = min(j0 + kSolveBW, d.c), bw = j1 - j0;
+    const T* __restrict__ P = L + d.lofs;
+    for (int t = threadIdx.x; t < bw; t += kThreads) {
+        double v = xp[k + j0 + t];
+        for (int sl = 0; sl < it.b; ++sl) v -= partial[(int64_t)(it.a + sl) * kSolveBW + t];
+        vb[t] = v;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nsub = (bw + 63) / 64;
+    for (int sb = nsub - 1; sb >= 0; --sb) {
+        const int s0 = j0 + sb * 64, s1 = min(s0 + 64, j1), b = s1 - s0;
+        for (int j = s0 + wave; j < s1; j += kThreads / 64) {
+            double acc = 0.0;
+            const T* col = P + (int64_t)j * n;
+            for (int i = s1 + lane; i < j1; i += 64) acc += to_d(col[i]) * xp[k + i];
+            acc = wave_sum(acc);
+            if (lane == 0) vb[j - j0] -= acc;
+        }
+        for (int t = threadIdx.x; t < b * b; t += kThreads) {
+            int i = t % b, jj = t / b;
+            D[i + jj * 65] = to_d(P[(s0 + i) + (int64_t)(s0 + jj) * n]);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            double v = lane < b ? vb[s0 - j0 + lane] : 0.0;
+            for (int jj = b - 1; jj >= 0; --jj) {
+                const double xj = __shfl(v, jj, 64) / D[jj + jj * 65];
+                if (lane == jj) v = xj;
+                else if (lane < jj) v -= D[jj + lane * 65] * xj;
+            }
+            if (lane < b) xp[k + s0 + lane] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // -------------------------------------------------------------- refinement
 __global__ void gather_kernel(const double* __restrict__ vals, const int* __restrict__ src,
                               double* __restrict__ dst, int nnz) {
@@ -638,6 +813,31 @@ void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* 
     if (count > 0) bwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, q, L, xp);
 }
 
+void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
+                       double* xp, double* su, const int64_t* suofs, hipStream_t s) {
+    if (count > 0) fwd_gather_kernel<<<count, kThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
+}
+template <typename T>
+void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
+                    hipStream_t s) {
+    if (count > 0) fwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, xp);
+}
+template <typename T>
+void launch_fwd_gemv(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
+                     double* su, const int64_t* suofs, hipStream_t s) {
+    if (count > 0) fwd_gemv_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, xp, su, suofs);
+}
+template <typename T>
+void launch_bwd_dot(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
+                    const double* xp, double* partial, hipStream_t s) {
+    if (count > 0) bwd_dot_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, xp, partial);
+}
+template <typename T>
+void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
+                    const double* partial, hipStream_t s) {
+    if (count > 0) bwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, xp, partial);
+}
+
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
     if (nnz > 0) gather_kernel<<<blocks_for(nnz), kThreads, 0, s>>>(src_vals, src, dst, nnz);
 }
@@ -674,7 +874,14 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
                                 const T*, double*, double*, const int64_t*, hipStream_t);                      \
     template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, double*,              \
                                 hipStream_t);                                                                  \
-    template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);
+    template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);               \
+    template void launch_fwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, double*, hipStream_t); \
+    template void launch_fwd_gemv<T>(const SolveItem*, int, const FrontDesc*, const T*, double*, double*,    \
+                                     const int64_t*, hipStream_t);                                           \
+    template void launch_bwd_dot<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,          \
+                                    const double*, double*, hipStream_t);                                    \
+    template void launch_bwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, double*,             \
+                                    const double*, hipStream_t);
 
 MDGPU_INSTANTIATE(double)
 MDGPU_INSTANTIATE(float)

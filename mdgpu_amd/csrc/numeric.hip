@@ -76,6 +76,9 @@ struct mdgpu_numeric {
     double* dAx = nullptr;
     double* dB = nullptr;
     double* dX = nullptr;
+    int* dSvFronts = nullptr;
+    mdgpu::SolveItem* dSvItems = nullptr;
+    double* dSvPartial = nullptr;
     double anorm = -1;
     bool factored = false;
     int64_t device_bytes = 0;
@@ -212,14 +215,40 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
     const int n = N->sym->S.n;
     const T* L = (const T*)N->dL;
     mdgpu::launch_permute_in(b, N->dPerm, N->dXp, n, s);
-    for (int lv = 0; lv < P.nlevels; lv++) {
-        int off = N->level_off[lv], cnt = N->level_off[lv + 1] - off;
-        mdgpu::launch_fwd<T>(N->dLevelList + off, cnt, N->dFronts, N->dKids, N->dRel, N->dQ, L, N->dXp, N->dSu,
-                             N->dSuofs, s);
+    for (const mdgpu::SolveStep& st : P.sv_fwd) {
+        const int* list = N->dSvFronts + st.first;
+        const mdgpu::SolveItem* items = N->dSvItems + st.first;
+        switch (st.kind) {
+            case mdgpu::SV_FWD_SMALL:
+                mdgpu::launch_fwd<T>(list, st.count, N->dFronts, N->dKids, N->dRel, N->dQ, L, N->dXp, N->dSu,
+                                     N->dSuofs, s);
+                break;
+            case mdgpu::SV_FWD_GATHER:
+                mdgpu::launch_fwd_gather(list, st.count, N->dFronts, N->dKids, N->dRel, N->dXp, N->dSu, N->dSuofs,
+                                         s);
+                break;
+            case mdgpu::SV_FWD_TRI:
+                mdgpu::launch_fwd_tri<T>(items, st.count, N->dFronts, L, N->dXp, s);
+                break;
+            case mdgpu::SV_FWD_GEMV:
+                mdgpu::launch_fwd_gemv<T>(items, st.count, N->dFronts, L, N->dXp, N->dSu, N->dSuofs, s);
+                break;
+        }
     }
-    for (int lv = P.nlevels - 1; lv >= 0; lv--) {
-        int off = N->level_off[lv], cnt = N->level_off[lv + 1] - off;
-        mdgpu::launch_bwd<T>(N->dLevelList + off, cnt, N->dFronts, N->dQ, L, N->dXp, s);
+    for (const mdgpu::SolveStep& st : P.sv_bwd) {
+        const int* list = N->dSvFronts + st.first;
+        const mdgpu::SolveItem* items = N->dSvItems + st.first;
+        switch (st.kind) {
+            case mdgpu::SV_BWD_SMALL:
+                mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, N->dXp, s);
+                break;
+            case mdgpu::SV_BWD_DOT:
+                mdgpu::launch_bwd_dot<T>(items, st.count, N->dFronts, N->dQ, L, N->dXp, N->dSvPartial, s);
+                break;
+            case mdgpu::SV_BWD_TRI:
+                mdgpu::launch_bwd_tri<T>(items, st.count, N->dFronts, L, N->dXp, N->dSvPartial, s);
+                break;
+        }
     }
     mdgpu::launch_permute_out(N->dXp, N->dPerm, x, n, s);
     hipError_t e = hipGetLastError();
@@ -340,7 +369,10 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)) ||
         (st = alloc(N, std::max<size_t>(P.a_dst.size(), 1) * sizeof(double), (void**)&N->dAx)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dB)) ||
-        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dX)))
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dX)) ||
+        (st = upload(N, P.sv_fronts, (void**)&N->dSvFronts)) ||
+        (st = upload(N, P.sv_items, (void**)&N->dSvItems)) ||
+        (st = alloc(N, P.sv_partial * sizeof(double), (void**)&N->dSvPartial)))
         return guard(st);
     if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess)
         return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: hipStreamCreate failed"));

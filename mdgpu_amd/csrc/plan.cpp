@@ -166,6 +166,90 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
     }
     P.solve_u = su;
 
+    // solve schedule (see SolveStep)
+    {
+        const int BW = kSolveBW, RC = 256;
+        for (int L = 0; L < P.nlevels; L++) {
+            std::vector<int> small, big;
+            for (int f : P.level_fronts[L]) (P.fronts[f].c > BW ? big : small).push_back(f);
+            if (!small.empty()) {
+                SolveStep st{SV_FWD_SMALL, (int)P.sv_fronts.size(), (int)small.size(), 0};
+                for (int f : small) st.bytes += (double)P.fronts[f].n * P.fronts[f].c * o.elem_bytes;
+                P.sv_fronts.insert(P.sv_fronts.end(), small.begin(), small.end());
+                P.sv_fwd.push_back(st);
+            }
+            if (big.empty()) continue;
+            P.sv_fwd.push_back({SV_FWD_GATHER, (int)P.sv_fronts.size(), (int)big.size(), 0});
+            P.sv_fronts.insert(P.sv_fronts.end(), big.begin(), big.end());
+            int maxc = 0;
+            for (int f : big) maxc = std::max(maxc, P.fronts[f].c);
+            for (int j0 = 0; j0 < maxc; j0 += BW) {
+                SolveStep tri{SV_FWD_TRI, (int)P.sv_items.size(), 0, 0};
+                for (int f : big) {
+                    const FrontDesc& d = P.fronts[f];
+                    if (j0 >= d.c) continue;
+                    const int b = std::min(BW, d.c - j0);
+                    P.sv_items.push_back({f, 0, 0, j0});
+                    tri.bytes += 0.5 * b * b * o.elem_bytes;
+                }
+                tri.count = (int)P.sv_items.size() - tri.first;
+                P.sv_fwd.push_back(tri);
+                SolveStep gv{SV_FWD_GEMV, (int)P.sv_items.size(), 0, 0};
+                for (int f : big) {
+                    const FrontDesc& d = P.fronts[f];
+                    if (j0 >= d.c) continue;
+                    const int j1 = std::min(j0 + BW, d.c);
+                    for (int i0 = j1; i0 < d.n; i0 += RC) P.sv_items.push_back({f, i0, 0, j0});
+                    gv.bytes += (double)(d.n - j1) * (j1 - j0) * o.elem_bytes + 16.0 * (d.n - j1);
+                }
+                gv.count = (int)P.sv_items.size() - gv.first;
+                if (gv.count > 0) P.sv_fwd.push_back(gv);
+            }
+        }
+        for (int L = P.nlevels - 1; L >= 0; L--) {
+            std::vector<int> small, big;
+            for (int f : P.level_fronts[L]) (P.fronts[f].c > BW ? big : small).push_back(f);
+            if (!small.empty()) {
+                SolveStep st{SV_BWD_SMALL, (int)P.sv_fronts.size(), (int)small.size(), 0};
+                for (int f : small) st.bytes += (double)P.fronts[f].n * P.fronts[f].c * o.elem_bytes;
+                P.sv_fronts.insert(P.sv_fronts.end(), small.begin(), small.end());
+                P.sv_bwd.push_back(st);
+            }
+            if (big.empty()) continue;
+            int maxc = 0;
+            for (int f : big) maxc = std::max(maxc, P.fronts[f].c);
+            const int nblk = (maxc + BW - 1) / BW;
+            for (int blk = nblk - 1; blk >= 0; blk--) {
+                const int j0 = blk * BW;
+                SolveStep dot{SV_BWD_DOT, (int)P.sv_items.size(), 0, 0};
+                std::vector<std::pair<int, int>> slots;   // per front: first slot, count
+                int slot = 0;
+                for (int f : big) {
+                    const FrontDesc& d = P.fronts[f];
+                    if (j0 >= d.c) { slots.push_back({0, 0}); continue; }
+                    const int j1 = std::min(j0 + BW, d.c);
+                    const int first = slot;
+                    for (int i0 = j1; i0 < d.n; i0 += RC) P.sv_items.push_back({f, i0, slot++, j0});
+                    slots.push_back({first, slot - first});
+                    dot.bytes += (double)(d.n - j1) * (j1 - j0) * o.elem_bytes + 12.0 * (d.n - j1);
+                }
+                dot.count = (int)P.sv_items.size() - dot.first;
+                if (dot.count > 0) P.sv_bwd.push_back(dot);
+                P.sv_partial = std::max<int64_t>(P.sv_partial, (int64_t)slot * BW);
+                SolveStep tri{SV_BWD_TRI, (int)P.sv_items.size(), 0, 0};
+                for (size_t t = 0; t < big.size(); t++) {
+                    const FrontDesc& d = P.fronts[big[t]];
+                    if (j0 >= d.c) continue;
+                    const int b = std::min(BW, d.c - j0);
+                    P.sv_items.push_back({big[t], slots[t].first, slots[t].second, j0});
+                    tri.bytes += 0.5 * b * b * o.elem_bytes;
+                }
+                tri.count = (int)P.sv_items.size() - tri.first;
+                P.sv_bwd.push_back(tri);
+            }
+        }
+    }
+
     // schedule
     const int T = o.tile;
     auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles) {
@@ -471,6 +555,35 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         if (tj.p_off + (tj.rows - 1) + (int64_t)(tj.b - 1) * tj.ld >= P.lstore) return bad("trsm bounds");
         if (tj.inv_off + (int64_t)tj.b * tj.b > P.inv_scratch) return bad("trsm scratch");
     }
+    auto check_solve = [&](const std::vector<SolveStep>& steps) -> std::string {
+        for (const SolveStep& st : steps) {
+            const bool uses_fronts = st.kind == SV_FWD_SMALL || st.kind == SV_BWD_SMALL || st.kind == SV_FWD_GATHER;
+            const int64_t cap = uses_fronts ? (int64_t)P.sv_fronts.size() : (int64_t)P.sv_items.size();
+            if (st.count <= 0 || st.first < 0 || st.first + st.count > cap) return "solve step slice";
+            if (uses_fronts) {
+                for (int t = st.first; t < st.first + st.count; t++)
+                    if (P.sv_fronts[t] < 0 || P.sv_fronts[t] >= nf) return "solve front";
+                continue;
+            }
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const SolveItem& it = P.sv_items[t];
+                if (it.front < 0 || it.front >= nf) return "solve item front";
+                const FrontDesc& d = P.fronts[it.front];
+                if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW) return "solve item block";
+                const int j1 = std::min(it.j0 + kSolveBW, d.c);
+                if ((st.kind == SV_FWD_GEMV || st.kind == SV_BWD_DOT) && (it.a < j1 || it.a >= d.n))
+                    return "solve item rows";
+                if (st.kind == SV_BWD_DOT && (it.b < 0 || (int64_t)(it.b + 1) * kSolveBW > P.sv_partial))
+                    return "solve partial slot";
+                if (st.kind == SV_BWD_TRI && (it.a < 0 || (int64_t)(it.a + it.b) * kSolveBW > P.sv_partial))
+                    return "solve partial range";
+            }
+        }
+        return "";
+    };
+    std::string se = check_solve(P.sv_fwd);
+    if (se.empty()) se = check_solve(P.sv_bwd);
+    if (!se.empty()) return bad(se);
     for (int f : P.panel) {
         const FrontDesc& d = P.fronts[f];
         if ((int64_t)d.n * d.c * o.elem_bytes > o.panel_lds_bytes) return bad("panel LDS");

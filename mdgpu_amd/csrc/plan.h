@@ -96,6 +96,27 @@ struct Step {
     double bytes = 0;      // algorithmic HBM bytes of the launch
 };
 
+// Solve schedule.  Fronts with at most kSolveBW pivots are solved by one
+// workgroup each (FWD_SMALL / BWD_SMALL); larger fronts go 256 columns at a
+// time: a one-workgroup triangle solve (TRI) and a grid-wide update of the
+// rows below (FWD_GEMV) or dot products against them (BWD_DOT, partial sums
+// per 256-row chunk, reduced in a fixed order by BWD_TRI).
+constexpr int kSolveBW = 256;
+enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_TRI, SV_FWD_GEMV, SV_BWD_SMALL, SV_BWD_DOT, SV_BWD_TRI };
+
+struct SolveItem {
+    int front;
+    int a;      // GEMV/DOT: first row (front-local); BWD_TRI: first partial slot
+    int b;      // DOT: partial slot; BWD_TRI: number of slots
+    int j0;     // first pivot column of the 256-column block (front-local)
+};
+
+struct SolveStep {
+    int kind;
+    int first, count;   // slice of sv_items (or of sv_fronts for *_SMALL / GATHER)
+    double bytes;       // algorithmic bytes moved
+};
+
 struct Plan {
     int nfronts = 0, nlevels = 0;
     std::vector<FrontDesc> fronts;
@@ -109,6 +130,10 @@ struct Plan {
     int64_t inv_scratch = 0;           // elements of the diag-inverse scratch
     int64_t solve_u = 0;               // elements of the solve update vectors
     std::vector<int64_t> solve_uofs;   // per front offset (m_f entries)
+    std::vector<int> sv_fronts;        // front lists of SMALL / GATHER steps
+    std::vector<SolveItem> sv_items;
+    std::vector<SolveStep> sv_fwd, sv_bwd;
+    int64_t sv_partial = 1;            // doubles of BWD_DOT partial sums
     std::vector<EaItem> ea;
     std::vector<GemmJob> gemm;
     std::vector<DiagJob> diag;

@@ -1,0 +1,124 @@
+// TEST INFRASTRUCTURE: executes the numeric plan's steps sequentially on the
+// CPU with the same loops as the device kernels, so that the schedule
+// (extend-add rounds, splits, arena offsets, blocked steps) can be checked
+// against the oracle without a GPU.  Built and loaded by the tests only.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../mdgpu_amd/csrc/plan.h"
+#include "../../mdgpu_amd/csrc/symbolic.h"
+
+using namespace mdgpu;
+
+extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
+                              int max_zeros, int nb, double* Lx_out, int* Li_out) {
+    AnalyzeOptions ao;
+    ao.use_amd = use_amd;
+    ao.partition = (Partition)mode;
+    ao.max_zeros = max_zeros;
+    Symbolic S;
+    if (!analyze(n, Cp, Ci, ao, S).empty()) return 1;
+    PlanOptions po;
+    po.elem_bytes = 8;
+    po.panel_lds_bytes = 61440;
+    po.nb = nb;
+    po.tile = 64;
+    Plan P;
+    if (!build_plan(S, po, P).empty()) return 2;
+    if (!validate_plan(P, po).empty()) return 3;
+    std::vector<double> L(P.lstore, 0.0), U(std::max<int64_t>(P.arena, 1), std::nan("")),
+        inv(P.inv_scratch, 0.0);
+    for (size_t e = 0; e < P.a_dst.size(); e++) L[P.a_dst[e]] = ax[e];
+    for (const Step& st : P.steps) {
+        if (st.kind == STEP_EA1 || st.kind == STEP_EA2) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const EaItem& it = P.ea[t];
+                const FrontDesc& cd = P.fronts[it.child];
+                const FrontDesc& pd = P.fronts[cd.parent];
+                const int* r = P.rel.data() + cd.relofs;
+                for (int j = it.j0; j < it.j1; j++) {
+                    int pj = r[j];
+                    double* dcol = pj < pd.c ? &L[pd.lofs + (int64_t)pj * pd.n] : &U[pd.uofs + (int64_t)(pj - pd.c) * pd.m];
+                    int off = pj < pd.c ? 0 : pd.c;
+                    for (int i = j; i < cd.m; i++) dcol[r[i] - off] += U[cd.uofs + i + (int64_t)j * cd.m];
+                }
+            }
+        } else if (st.kind == STEP_PANEL) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const FrontDesc& d = P.fronts[P.panel[t]];
+                double* g = &L[d.lofs];
+                for (int j = 0; j < d.c; j++) {
+                    double rt = std::sqrt(g[j + j * d.n]);
+                    g[j + j * d.n] = rt;
+                    for (int i = j + 1; i < d.n; i++) g[i + j * d.n] /= rt;
+                    for (int i = j + 1; i < d.n; i++)
+                        for (int col = j + 1; col <= std::min(i, d.c - 1); col++)
+                            g[i + col * d.n] -= g[i + j * d.n] * g[col + j * d.n];
+                }
+            }
+        } else if (st.kind == STEP_DIAG) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const DiagJob& dj = P.diag[t];
+                double* g = &L[dj.p_off];
+                const int b = dj.b, ld = dj.ld;
+                for (int j = 0; j < b; j++) {
+                    double rt = std::sqrt(g[j + j * ld]);
+                    g[j + j * ld] = rt;
+                    for (int i = j + 1; i < b; i++) g[i + j * ld] /= rt;
+                    for (int i = j + 1; i < b; i++)
+                        for (int col = j + 1; col <= i; col++) g[i + col * ld] -= g[i + j * ld] * g[col + j * ld];
+                }
+                double* q = &inv[dj.inv_off];
+                for (int j = 0; j < b; j++) {
+                    for (int i = 0; i < j; i++) q[i + j * b] = 0;
+                    q[j + j * b] = 1.0 / g[j + j * ld];
+                    for (int i = j + 1; i < b; i++) {
+                        double s = 0;
+                        for (int l = j; l < i; l++) s += g[i + l * ld] * q[l + j * b];
+                        q[i + j * b] = -s / g[i + i * ld];
+                    }
+                }
+            }
+        } else if (st.kind == STEP_TRSM) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const TrsmJob& tj = P.trsm[t];
+                const double* q = &inv[tj.inv_off];
+                for (int i = 0; i < tj.rows; i++) {
+                    double* row = &L[tj.p_off + i];
+                    std::vector<double> r(tj.b), out(tj.b);
+                    for (int l = 0; l < tj.b; l++) r[l] = row[(int64_t)l * tj.ld];
+                    for (int j = 0; j < tj.b; j++) {
+                        double acc = 0;
+                        for (int l = 0; l <= j; l++) acc += r[l] * q[j + l * tj.b];
+                        out[j] = acc;
+                    }
+                    for (int j = 0; j < tj.b; j++) row[(int64_t)j * tj.ld] = out[j];
+                }
+            }
+        } else if (st.kind == STEP_GEMM) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const GemmJob& g = P.gemm[t];
+                double* C = g.c_in_arena ? &U[g.c_off] : &L[g.c_off];
+                for (int j = 0; j < g.N; j++)
+                    for (int i = (g.lower ? j : 0); i < g.M; i++) {
+                        double acc = 0;
+                        for (int k = 0; k < g.K; k++) acc += L[g.a_off + i + (int64_t)k * g.lda] * L[g.b_off + j + (int64_t)k * g.ldb];
+                        double* cp = &C[i + (int64_t)j * g.ldc];
+                        *cp = g.beta0 ? -acc : *cp - acc;
+                    }
+            }
+        }
+    }
+    for (int f = 0; f < P.nfronts; f++) {
+        const FrontDesc& d = P.fronts[f];
+        for (int j = 0; j < d.c; j++) {
+            int64_t dst = S.lp[d.k + j];
+            for (int i = j; i < d.n; i++, dst++) {
+                Lx_out[dst] = L[d.lofs + (int64_t)j * d.n + i];
+                Li_out[dst] = S.q_all[d.qofs + i];
+            }
+        }
+    }
+    return 0;
+}

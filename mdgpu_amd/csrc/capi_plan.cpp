@@ -30,7 +30,7 @@ extern "C" int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_
     if (err.empty()) err = mdgpu::validate_plan(P, po);
     if (!err.empty()) return fail(MDGPU_ERR_ARG, "mdgpu_plan_check: " + err);
     const int64_t es = po.elem_bytes;
-    int64_t bytes = (P.lstore + P.arena + P.inv_scratch) * es + P.solve_u * 8 +
+    int64_t bytes = (P.lstore + P.arena + P.dinv) * es + P.solve_u * 8 +
                     (int64_t)P.a_dst.size() * 8 + (int64_t)(P.rel.size() + P.q.size()) * 4 +
                     (int64_t)P.afull_i.size() * 16;
     out[MDGPU_NUM_DEVICE_BYTES] = bytes;

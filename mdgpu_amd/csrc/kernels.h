@@ -21,6 +21,8 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 template <typename T>
 void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s);
 template <typename T>
+void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
+template <typename T>
 void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv, hipStream_t s);
 // tile 128: MFMA kernel; tile 64: FMA kernel (the plan's tile size decides)
 template <typename T>
@@ -31,17 +33,18 @@ void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipS
 void launch_permute_out(const double* xp, const int* perm, double* x, int n, hipStream_t s);
 template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
-                const int* q, const T* L, double* xp, double* su, const int64_t* suofs, hipStream_t s);
-template <typename T>
-void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, double* xp,
+                const int* q, const T* L, const T* inv, double* xp, double* su, const int64_t* suofs,
                 hipStream_t s);
+template <typename T>
+void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
+                double* xp, hipStream_t s);
 
 // solves of fronts with more than kSolveBW pivots (see SolveStep)
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                        double* xp, double* su, const int64_t* suofs, hipStream_t s);
 template <typename T>
-void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
-                    hipStream_t s);
+void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+                    double* xp, hipStream_t s);
 template <typename T>
 void launch_fwd_gemv(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
                      double* su, const int64_t* suofs, hipStream_t s);
@@ -49,8 +52,8 @@ template <typename T>
 void launch_bwd_dot(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
                     const double* xp, double* partial, hipStream_t s);
 template <typename T>
-void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
-                    const double* partial, hipStream_t s);
+void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+                    double* xp, const double* partial, hipStream_t s);
 
 // refinement helpers (FP64)
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);

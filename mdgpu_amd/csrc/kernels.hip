@@ -116,56 +116,98 @@ __global__ __launch_bounds__(kThreads) void panel_kernel(const int* __restrict__
 }
 
 // ----------------------------------------------- dense: blocked panel path
-// Diagonal block b x b: Cholesky in LDS, then its triangular inverse (column
-// j of inv(L_D) by forward substitution on lane j), used by the TRSM below.
+// Wavefront-level dense kernels on a 64x64 diagonal block held in registers:
+// lane i keeps row i (a[0..63]); column entries of other lanes are read with
+// v_readlane broadcasts, so no LDS or barrier sits on the serial chain.
+__device__ inline float bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ inline double bcast(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Right-looking Cholesky of the block (rows beyond the block's width are
+// identity padding).  Returns the first non-positive pivot or 64.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
-                                                        T* __restrict__ inv, int* __restrict__ info) {
-    __shared__ T D[64 * 65];
-    __shared__ T Q[64 * 65];
-    __shared__ T pivinv;
+__device__ inline int wave_potrf64(T (&a)[64], int lane) {
+    int bad = 64;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        const T djj = bcast(a[j], j);
+        if (!(djj > T(0)) && bad == 64) bad = j;
+        const T sq = sqrt(djj);
+        const T is = T(1) / sq;
+        a[j] = (lane == j) ? sq : a[j] * is;
+#pragma unroll
+        for (int k = j + 1; k < 64; ++k) a[k] -= a[j] * bcast(a[j], k);
+    }
+    return bad;
+}
+
+// x[r] = inv(L)(r, lane): column `lane` of the inverse by forward
+// substitution, L(r, l) broadcast from lane r.
+template <typename T>
+__device__ inline void wave_trinv64(const T (&a)[64], T (&x)[64], int lane) {
+#pragma unroll
+    for (int r = 0; r < 64; ++r) {
+        T acc = (lane == r) ? T(1) : T(0);
+#pragma unroll
+        for (int l = 0; l < r; ++l) acc -= bcast(a[l], r) * x[l];
+        x[r] = acc * (T(1) / bcast(a[r], r));
+    }
+}
+
+// Diagonal block b x b of the blocked path: Cholesky, write L_D back, and
+// its inverse W(i + 64 r) = inv(L_D)(r, i) for the TRSM and the solves.
+template <typename T>
+__global__ __launch_bounds__(64) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
+                                                  T* __restrict__ inv, int* __restrict__ info) {
     const DiagJob jb = jobs[blockIdx.x];
-    const int b = jb.b, ld = jb.ld;
+    const int b = jb.b, ld = jb.ld, lane = threadIdx.x;
     T* __restrict__ g = L + jb.p_off;
-    for (int t = threadIdx.x; t < b * b; t += kThreads) {
-        int i = t % b, j = t / b;
-        D[i + j * 65] = g[i + (int64_t)j * ld];
+    T a[64];
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+        a[c] = (lane < b && c < b) ? g[lane + (int64_t)c * ld] : (lane == c ? T(1) : T(0));
+    const int bad = wave_potrf64(a, lane);
+    if (bad < b && lane == 0) atomicMin(info, jb.col0 + bad);
+    if (lane < b) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+            if (c <= lane && c < b) g[lane + (int64_t)c * ld] = a[c];
     }
-    __syncthreads();
-    for (int j = 0; j < b; ++j) {
-        if (threadIdx.x == 0) {
-            T dj = D[j + j * 65];
-            if (!(dj > T(0))) atomicMin(info, jb.col0 + j);
-            T rt = sqrt(dj);
-            D[j + j * 65] = rt;
-            pivinv = T(1) / rt;
-        }
-        __syncthreads();
-        const T s = pivinv;
-        for (int i = j + 1 + threadIdx.x; i < b; i += kThreads) D[i + j * 65] *= s;
-        __syncthreads();
-        for (int i = j + 1 + threadIdx.x; i < b; i += kThreads) {
-            const T lij = D[i + j * 65];
-            for (int col = j + 1; col <= i; ++col) D[i + col * 65] -= lij * D[col + j * 65];
-        }
-        __syncthreads();
+    T x[64];
+    wave_trinv64(a, x, lane);
+    T* __restrict__ w = inv + jb.inv_off;
+    if (lane < b) {
+#pragma unroll
+        for (int r = 0; r < 64; ++r)
+            if (r < b) w[lane + 64 * r] = x[r];
     }
-    if (threadIdx.x < b) {
-        const int j = threadIdx.x;
-        for (int i = 0; i < j; ++i) Q[i + j * 65] = T(0);
-        Q[j + j * 65] = T(1) / D[j + j * 65];
-        for (int i = j + 1; i < b; ++i) {
-            T s = T(0);
-            for (int l = j; l < i; ++l) s += D[i + l * 65] * Q[l + j * 65];
-            Q[i + j * 65] = -s / D[i + i * 65];
-        }
-    }
-    __syncthreads();
-    T* __restrict__ q = inv + jb.inv_off;
-    for (int t = threadIdx.x; t < b * b; t += kThreads) {
-        int i = t % b, j = t / b;
-        if (i >= j) g[i + (int64_t)j * ld] = D[i + j * 65];
-        q[i + j * b] = Q[i + j * 65];
+}
+
+// Inverse of one 64-column diagonal block of an already factored panel.
+template <typename T>
+__global__ __launch_bounds__(64) void inv_kernel(const InvJob* __restrict__ jobs, const FrontDesc* __restrict__ fronts,
+                                                 const T* __restrict__ L, T* __restrict__ inv) {
+    const InvJob jb = jobs[blockIdx.x];
+    const FrontDesc d = fronts[jb.front];
+    const int b = min(64, d.c - jb.j0), lane = threadIdx.x;
+    const T* __restrict__ g = L + d.lofs + jb.j0 + (int64_t)jb.j0 * d.n;
+    T a[64];
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+        a[c] = (lane < b && c < b && c <= lane) ? g[lane + (int64_t)c * d.n] : (lane == c ? T(1) : T(0));
+    T x[64];
+    wave_trinv64(a, x, lane);
+    T* __restrict__ w = inv + d.dinv + (int64_t)(jb.j0 / 64) * 4096;
+    if (lane < b) {
+#pragma unroll
+        for (int r = 0; r < 64; ++r)
+            if (r < b) w[lane + 64 * r] = x[r];
     }
 }
 
@@ -196,8 +238,8 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmJob* __restric
         R[i + l * 64] = i < rows ? g[i + (int64_t)l * ld] : T(0);
     }
     for (int t = threadIdx.x; t < b * b; t += kThreads) {
-        int i = t % b, j = t / b;
-        Q[i + j * 65] = q[i + j * b];
+        int l = t % b, j = t / b;
+        Q[j + l * 65] = q[l + 64 * j];          // inv(L_D)(j, l)
     }
     __syncthreads();
     const int i = threadIdx.x & 63;
@@ -292,17 +334,22 @@ __global__ __launch_bounds__(kThreads) void gemm_nt_kernel(const GemmJob* __rest
 // (B A^T) so that each lane's four accumulator rows map to four C columns
 // and a 16-lane group stores 16 consecutive C rows.
 template <typename T> struct MfmaOp;
+// Accumulator register v of lane (16*lg + li) holds D(row(lg, v), li): the
+// 32-bit 16x16 forms put rows 4*lg..4*lg+3 in one lane, the f64 form
+// interleaves them (row lg + 4*v) -- both measured on gfx950.
 template <> struct MfmaOp<float> {
     typedef float acc_t __attribute__((ext_vector_type(4)));
     __device__ static acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static int row(int lg, int v) { return 4 * lg + v; }
 };
 template <> struct MfmaOp<double> {
     typedef double acc_t __attribute__((ext_vector_type(4)));
     __device__ static acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static int row(int lg, int v) { return lg + 4 * v; }
 };
 
 template <typename T>
@@ -374,12 +421,12 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
                 for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb][t], acc[rb][cb]);
         __syncthreads();
     }
-    // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + 4*lg + v)
+    // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + row(lg, v))
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int j = col0 + wc + cb * 16 + 4 * lg + v;
+            const int j = col0 + wc + cb * 16 + MfmaOp<T>::row(lg, v);
             if (j >= g.N) continue;
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb) {
@@ -405,16 +452,19 @@ __global__ void permute_out_kernel(const double* __restrict__ xp, const int* __r
 
 // Forward substitution L y = x over one level, one workgroup per front,
 // multifrontal form of cs_lsolve (cs.cpp:1086-1101): gather the children's
-// update vectors through their relative maps, solve the diagonal blocks
-// 64 columns at a time on one wavefront, update the rows below.
+// update vectors through their relative maps, then per 64-column diagonal
+// block y = inv(L_D) x (stored inverse, a parallel 64x64 product) and update
+// the rows below.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ list,
                                                        const FrontDesc* __restrict__ fronts,
                                                        const int* __restrict__ kids, const int* __restrict__ rel,
                                                        const int* __restrict__ q, const T* __restrict__ L,
-                                                       double* __restrict__ xp, double* __restrict__ su,
+                                                       const T* __restrict__ inv, double* __restrict__ xp,
+                                                       double* __restrict__ su,
                                                        const int64_t* __restrict__ suofs) {
-    __shared__ double D[64 * 65];
+    __shared__ double W[64 * 65];
+    __shared__ double vb[64];
     __shared__ double yb[64];
     const int f = list[blockIdx.x];
     const FrontDesc d = fronts[f];
@@ -437,21 +487,19 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
     }
     for (int j0 = 0; j0 < c; j0 += 64) {
         const int b = min(64, c - j0);
-        for (int t = threadIdx.x; t < b * b; t += kThreads) {
-            int i = t % b, jj = t / b;
-            D[i + jj * 65] = to_d(P[(j0 + i) + (int64_t)(j0 + jj) * n]);
+        const T* __restrict__ w = inv + d.dinv + (int64_t)(j0 / 64) * 4096;
+        for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
+            int l = t & 63, r = t >> 6;
+            if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
         }
-        for (int t = threadIdx.x; t < b; t += kThreads) yb[t] = xp[k + j0 + t];
+        for (int t = threadIdx.x; t < b; t += kThreads) vb[t] = xp[k + j0 + t];
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            double xv = lane < b ? yb[lane] : 0.0;
-            for (int jj = 0; jj < b; ++jj) {
-                const double yj = __shfl(xv, jj, 64) / D[jj + jj * 65];
-                if (lane == jj) xv = yj;
-                else if (lane > jj && lane < b) xv -= D[lane + jj * 65] * yj;
-            }
-            if (lane < b) yb[lane] = xv;
+        if (threadIdx.x < b) {
+            const int r = threadIdx.x;
+            double y = 0.0;
+            for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
+            yb[r] = y;
+            xp[k + j0 + r] = y;
         }
         __syncthreads();
         for (int i = j0 + b + threadIdx.x; i < n; i += kThreads) {
@@ -460,20 +508,19 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
             if (i < c) xp[k + i] -= acc;
             else u[i - c] -= acc;
         }
-        for (int t = threadIdx.x; t < b; t += kThreads) xp[k + j0 + t] = yb[t];
         __syncthreads();
     }
 }
 
 // Backward substitution L^T x = y over one level (cs_ltsolve, cs.cpp:1104-1119),
-// one workgroup per front, diagonal blocks from the last to the first; the
-// rows below each block are read through the front's row list q.
+// one workgroup per front, diagonal blocks from the last to the first: dot
+// products with the already solved rows below (read through the row list
+// q), then x = inv(L_D)^T v with the stored inverse.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ list,
                                                        const FrontDesc* __restrict__ fronts,
                                                        const int* __restrict__ qall, const T* __restrict__ L,
-                                                       double* __restrict__ xp) {
-    __shared__ double D[64 * 65];
+                                                       const T* __restrict__ inv, double* __restrict__ xp) {
     __shared__ double vb[64];
     const FrontDesc d = fronts[list[blockIdx.x]];
     const int n = d.n, c = d.c, k = d.k;
@@ -490,19 +537,13 @@ __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ l
             s = wave_sum(s);
             if (lane == 0) vb[j - j0] = xp[k + j] - s;
         }
-        for (int t = threadIdx.x; t < b * b; t += kThreads) {
-            int i = t % b, jj = t / b;
-            D[i + jj * 65] = to_d(P[(j0 + i) + (int64_t)(j0 + jj) * n]);
-        }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            double v = lane < b ? vb[lane] : 0.0;
-            for (int jj = b - 1; jj >= 0; --jj) {
-                const double xj = __shfl(v, jj, 64) / D[jj + jj * 65];
-                if (lane == jj) v = xj;
-                else if (lane < jj) v -= D[jj + lane * 65] * xj;
-            }
-            if (lane < b) xp[k + j0 + lane] = v;
+        if (threadIdx.x < b) {
+            const int i = threadIdx.x;
+            const T* __restrict__ w = inv + d.dinv + (int64_t)blk * 4096;
+            double x = 0.0;
+            for (int r = i; r < b; ++r) x += to_d(w[i + 64 * r]) * vb[r];
+            xp[k + j0 + i] = x;
         }
         __syncthreads();
     }
@@ -540,13 +581,15 @@ __global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restr
     }
 }
 
-// Forward: triangle of one 256-column block (64-column sub-blocks, each
-// solved on one wavefront, then applied to the block's remaining rows).
+// Forward: the 256-column block's own triangle, 64 columns at a time with
+// the stored diagonal inverses, updates applied within the block.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_tri_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
-                                                           const T* __restrict__ L, double* __restrict__ xp) {
-    __shared__ double D[64 * 65];
+                                                           const T* __restrict__ L, const T* __restrict__ inv,
+                                                           double* __restrict__ xp) {
+    __shared__ double W[64 * 65];
+    __shared__ double vb[64];
     __shared__ double yb[64];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
@@ -555,21 +598,19 @@ __global__ __launch_bounds__(kThreads) void fwd_tri_kernel(const SolveItem* __re
     const T* __restrict__ P = L + d.lofs;
     for (int s0 = it.j0; s0 < j1; s0 += 64) {
         const int b = min(64, j1 - s0);
-        for (int t = threadIdx.x; t < b * b; t += kThreads) {
-            int i = t % b, jj = t / b;
-            D[i + jj * 65] = to_d(P[(s0 + i) + (int64_t)(s0 + jj) * n]);
+        const T* __restrict__ w = inv + d.dinv + (int64_t)(s0 / 64) * 4096;
+        for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
+            int l = t & 63, r = t >> 6;
+            if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
         }
-        for (int t = threadIdx.x; t < b; t += kThreads) yb[t] = xp[k + s0 + t];
+        for (int t = threadIdx.x; t < b; t += kThreads) vb[t] = xp[k + s0 + t];
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            double xv = lane < b ? yb[lane] : 0.0;
-            for (int jj = 0; jj < b; ++jj) {
-                const double yj = __shfl(xv, jj, 64) / D[jj + jj * 65];
-                if (lane == jj) xv = yj;
-                else if (lane > jj && lane < b) xv -= D[lane + jj * 65] * yj;
-            }
-            if (lane < b) yb[lane] = xv;
+        if (threadIdx.x < b) {
+            const int r = threadIdx.x;
+            double y = 0.0;
+            for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
+            yb[r] = y;
+            xp[k + s0 + r] = y;
         }
         __syncthreads();
         for (int i = s0 + b + threadIdx.x; i < j1; i += kThreads) {
@@ -577,12 +618,12 @@ __global__ __launch_bounds__(kThreads) void fwd_tri_kernel(const SolveItem* __re
             for (int jj = 0; jj < b; ++jj) acc += to_d(P[i + (int64_t)(s0 + jj) * n]) * yb[jj];
             xp[k + i] -= acc;
         }
-        for (int t = threadIdx.x; t < b; t += kThreads) xp[k + s0 + t] = yb[t];
         __syncthreads();
     }
 }
 
-// Forward: rows [a, a+256) below a solved block -= L(rows, block) * y(block).
+// Forward: 64 rows [a, a+64) below a solved block -= L(rows, block) y(block);
+// each wavefront takes 64 of the block's columns, lanes over rows.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_gemv_kernel(const SolveItem* __restrict__ items,
                                                             const FrontDesc* __restrict__ fronts,
@@ -590,58 +631,73 @@ __global__ __launch_bounds__(kThreads) void fwd_gemv_kernel(const SolveItem* __r
                                                             double* __restrict__ su,
                                                             const int64_t* __restrict__ suofs) {
     __shared__ double yb[kSolveBW];
+    __shared__ double red[kThreads / 64][64];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
     const int j1 = min(it.j0 + kSolveBW, d.c), bw = j1 - it.j0;
     for (int t = threadIdx.x; t < bw; t += kThreads) yb[t] = xp[d.k + it.j0 + t];
     __syncthreads();
-    const int i = it.a + threadIdx.x;
-    if (i >= d.n) return;
-    const T* __restrict__ P = L + d.lofs + i + (int64_t)it.j0 * d.n;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = it.a + lane;
     double acc = 0.0;
-    for (int jj = 0; jj < bw; ++jj) acc += to_d(P[(int64_t)jj * d.n]) * yb[jj];
-    if (i < d.c) xp[d.k + i] -= acc;
-    else su[suofs[it.front] + i - d.c] -= acc;
+    if (i < d.n) {
+        const T* __restrict__ P = L + d.lofs + i + (int64_t)it.j0 * d.n;
+        for (int jj = wave * 64; jj < min(bw, wave * 64 + 64); ++jj) acc += to_d(P[(int64_t)jj * d.n]) * yb[jj];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64 && i < d.n) {
+        const double tot = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
+        if (i < d.c) xp[d.k + i] -= tot;
+        else su[suofs[it.front] + i - d.c] -= tot;
+    }
 }
 
-// Backward: partial dot products of the block's columns with x over 256
-// rows below the block; partial[slot][j] (fixed-order reduction later).
+// Backward: partial dot products of the block's columns with x over the 64
+// rows [a, a+64) below the block, staged through LDS 64 columns at a time;
+// partial[slot][j] is reduced in a fixed order by bwd_tri_kernel.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
                                                            const int* __restrict__ qall, const T* __restrict__ L,
                                                            const double* __restrict__ xp,
                                                            double* __restrict__ partial) {
-    __shared__ double red[kThreads / 64][kSolveBW];
+    __shared__ T tile[64 * 65];
+    __shared__ double xq[64];
+    __shared__ double red[4][64];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
     const int j1 = min(it.j0 + kSolveBW, d.c), bw = j1 - it.j0;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = it.a + threadIdx.x;
-    const bool live = i < d.n;
-    const double xq = live ? xp[qall[d.qofs + i]] : 0.0;
-    const T* __restrict__ P = L + d.lofs + (live ? i : 0) + (int64_t)it.j0 * d.n;
-    for (int jj = 0; jj < bw; ++jj) {
-        double v = live ? to_d(P[(int64_t)jj * d.n]) * xq : 0.0;
-        v = wave_sum(v);
-        if (lane == 0) red[wave][jj] = v;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < bw; t += kThreads) {
-        double sum = 0.0;
-        for (int w = 0; w < kThreads / 64; ++w) sum += red[w][t];
-        partial[(int64_t)it.b * kSolveBW + t] = sum;
+    const int rows = min(64, d.n - it.a);
+    if (threadIdx.x < 64) xq[threadIdx.x] = threadIdx.x < rows ? xp[qall[d.qofs + it.a + threadIdx.x]] : 0.0;
+    const T* __restrict__ P = L + d.lofs + it.a + (int64_t)it.j0 * d.n;
+    const int r = threadIdx.x & 63, cq = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < bw; c0 += 64) {
+        const int cw = min(64, bw - c0);
+        __syncthreads();
+        for (int cc = cq; cc < 64; cc += 4)
+            tile[cc * 65 + r] = (r < rows && cc < cw) ? P[r + (int64_t)(c0 + cc) * d.n] : T(0);
+        __syncthreads();
+        // thread (quarter cq, column r): rows cq*16 .. cq*16+15 of column r
+        double s = 0.0;
+        for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) s += to_d(tile[r * 65 + rr]) * xq[rr];
+        red[cq][r] = s;
+        __syncthreads();
+        if (threadIdx.x < cw)
+            partial[(int64_t)it.b * kSolveBW + c0 + threadIdx.x] =
+                red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
     }
 }
 
-// Backward: v = y(block) - sum of partials, then L(block)^T x = v in
-// 64-column sub-blocks from the last.
+// Backward: v = y(block) - sum of partials (fixed order), then per 64-column
+// sub-block from the last: dots with the block rows already solved and
+// x = inv(L_D)^T v.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void bwd_tri_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
-                                                           const T* __restrict__ L, double* __restrict__ xp,
+                                                           const T* __restrict__ L, const T* __restrict__ inv,
+                                                           double* __restrict__ xp,
                                                            const double* __restrict__ partial) {
-    __shared__ double D[64 * 65];
     __shared__ double vb[kSolveBW];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
@@ -665,19 +721,13 @@ __global__ __launch_bounds__(kThreads) void bwd_tri_kernel(const SolveItem* __re
             acc = wave_sum(acc);
             if (lane == 0) vb[j - j0] -= acc;
         }
-        for (int t = threadIdx.x; t < b * b; t += kThreads) {
-            int i = t % b, jj = t / b;
-            D[i + jj * 65] = to_d(P[(s0 + i) + (int64_t)(s0 + jj) * n]);
-        }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            double v = lane < b ? vb[s0 - j0 + lane] : 0.0;
-            for (int jj = b - 1; jj >= 0; --jj) {
-                const double xj = __shfl(v, jj, 64) / D[jj + jj * 65];
-                if (lane == jj) v = xj;
-                else if (lane < jj) v -= D[jj + lane * 65] * xj;
-            }
-            if (lane < b) xp[k + s0 + lane] = v;
+        if (threadIdx.x < b) {
+            const int i = threadIdx.x;
+            const T* __restrict__ w = inv + d.dinv + (int64_t)(s0 / 64) * 4096;
+            double x = 0.0;
+            for (int rr = i; rr < b; ++rr) x += to_d(w[i + 64 * rr]) * vb[s0 - j0 + rr];
+            xp[k + s0 + i] = x;
         }
         __syncthreads();
     }
@@ -783,7 +833,11 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 }
 template <typename T>
 void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s) {
-    if (count > 0) diag_kernel<T><<<count, kThreads, 0, s>>>(jobs, L, inv, info);
+    if (count > 0) diag_kernel<T><<<count, 64, 0, s>>>(jobs, L, inv, info);
+}
+template <typename T>
+void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {
+    if (count > 0) inv_kernel<T><<<count, 64, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
 void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv, hipStream_t s) {
@@ -804,13 +858,14 @@ void launch_permute_out(const double* xp, const int* perm, double* x, int n, hip
 }
 template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
-                const int* q, const T* L, double* xp, double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, xp, su, suofs);
+                const int* q, const T* L, const T* inv, double* xp, double* su, const int64_t* suofs,
+                hipStream_t s) {
+    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, su, suofs);
 }
 template <typename T>
-void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, double* xp,
-                hipStream_t s) {
-    if (count > 0) bwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, q, L, xp);
+void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
+                double* xp, hipStream_t s) {
+    if (count > 0) bwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, q, L, inv, xp);
 }
 
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
@@ -818,9 +873,9 @@ void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, cons
     if (count > 0) fwd_gather_kernel<<<count, kThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
 }
 template <typename T>
-void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
-                    hipStream_t s) {
-    if (count > 0) fwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, xp);
+void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+                    double* xp, hipStream_t s) {
+    if (count > 0) fwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, inv, xp);
 }
 template <typename T>
 void launch_fwd_gemv(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
@@ -833,9 +888,9 @@ void launch_bwd_dot(const SolveItem* items, int count, const FrontDesc* fronts, 
     if (count > 0) bwd_dot_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, xp, partial);
 }
 template <typename T>
-void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
-                    const double* partial, hipStream_t s) {
-    if (count > 0) bwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, xp, partial);
+void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+                    double* xp, const double* partial, hipStream_t s) {
+    if (count > 0) bwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, inv, xp, partial);
 }
 
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
@@ -868,19 +923,21 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_extend_add<T>(const EaItem*, int, const FrontDesc*, const int*, T*, T*, hipStream_t); \
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
     template void launch_diag<T>(const DiagJob*, int, T*, T*, int*, hipStream_t);                             \
+    template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_trsm<T>(const TrsmJob*, int, int, T*, const T*, hipStream_t);                         \
     template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
-                                const T*, double*, double*, const int64_t*, hipStream_t);                      \
-    template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, double*,              \
+                                const T*, const T*, double*, double*, const int64_t*, hipStream_t);            \
+    template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, const T*, double*,    \
                                 hipStream_t);                                                                  \
     template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);               \
-    template void launch_fwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, double*, hipStream_t); \
+    template void launch_fwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,    \
+                                    hipStream_t);                                                            \
     template void launch_fwd_gemv<T>(const SolveItem*, int, const FrontDesc*, const T*, double*, double*,    \
                                      const int64_t*, hipStream_t);                                           \
     template void launch_bwd_dot<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,          \
                                     const double*, double*, hipStream_t);                                    \
-    template void launch_bwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, double*,             \
+    template void launch_bwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
                                     const double*, hipStream_t);
 
 MDGPU_INSTANTIATE(double)

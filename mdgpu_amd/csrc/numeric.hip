@@ -46,6 +46,7 @@ struct mdgpu_numeric {
     mdgpu::GemmJob* dGemm = nullptr;
     mdgpu::DiagJob* dDiag = nullptr;
     mdgpu::TrsmJob* dTrsm = nullptr;
+    mdgpu::InvJob* dInvJobs = nullptr;
     int* dPanel = nullptr;
     int* dLevelList = nullptr;
     std::vector<int> level_off;
@@ -165,6 +166,7 @@ int kind_of(int step_kind) {
         case mdgpu::STEP_PANEL: return KIND_PANEL;
         case mdgpu::STEP_DIAG: return KIND_DIAG;
         case mdgpu::STEP_TRSM: return KIND_TRSM;
+        case mdgpu::STEP_INV: return KIND_DIAG;
         default: return KIND_GEMM;
     }
 }
@@ -201,6 +203,9 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
             case mdgpu::STEP_GEMM:
                 mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, N->tile, L, U, s);
                 break;
+            case mdgpu::STEP_INV:
+                mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
+                break;
         }
         if (ev) hipEventRecord(ev[t + 2], s);
     }
@@ -214,21 +219,22 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
     const mdgpu::Plan& P = N->plan;
     const int n = N->sym->S.n;
     const T* L = (const T*)N->dL;
+    const T* inv = (const T*)N->dInv;
     mdgpu::launch_permute_in(b, N->dPerm, N->dXp, n, s);
     for (const mdgpu::SolveStep& st : P.sv_fwd) {
         const int* list = N->dSvFronts + st.first;
         const mdgpu::SolveItem* items = N->dSvItems + st.first;
         switch (st.kind) {
             case mdgpu::SV_FWD_SMALL:
-                mdgpu::launch_fwd<T>(list, st.count, N->dFronts, N->dKids, N->dRel, N->dQ, L, N->dXp, N->dSu,
-                                     N->dSuofs, s);
+                mdgpu::launch_fwd<T>(list, st.count, N->dFronts, N->dKids, N->dRel, N->dQ, L, inv, N->dXp,
+                                     N->dSu, N->dSuofs, s);
                 break;
             case mdgpu::SV_FWD_GATHER:
                 mdgpu::launch_fwd_gather(list, st.count, N->dFronts, N->dKids, N->dRel, N->dXp, N->dSu, N->dSuofs,
                                          s);
                 break;
             case mdgpu::SV_FWD_TRI:
-                mdgpu::launch_fwd_tri<T>(items, st.count, N->dFronts, L, N->dXp, s);
+                mdgpu::launch_fwd_tri<T>(items, st.count, N->dFronts, L, inv, N->dXp, s);
                 break;
             case mdgpu::SV_FWD_GEMV:
                 mdgpu::launch_fwd_gemv<T>(items, st.count, N->dFronts, L, N->dXp, N->dSu, N->dSuofs, s);
@@ -240,13 +246,13 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
         const mdgpu::SolveItem* items = N->dSvItems + st.first;
         switch (st.kind) {
             case mdgpu::SV_BWD_SMALL:
-                mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, N->dXp, s);
+                mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, inv, N->dXp, s);
                 break;
             case mdgpu::SV_BWD_DOT:
                 mdgpu::launch_bwd_dot<T>(items, st.count, N->dFronts, N->dQ, L, N->dXp, N->dSvPartial, s);
                 break;
             case mdgpu::SV_BWD_TRI:
-                mdgpu::launch_bwd_tri<T>(items, st.count, N->dFronts, L, N->dXp, N->dSvPartial, s);
+                mdgpu::launch_bwd_tri<T>(items, st.count, N->dFronts, L, inv, N->dXp, N->dSvPartial, s);
                 break;
         }
     }
@@ -340,7 +346,8 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     if (const char* g = std::getenv("MDGPU_GRAPH")) N->use_graph = std::atoi(g) != 0;
     if ((st = alloc(N, P.lstore * es, &N->dL)) ||
         (st = alloc(N, P.arena * es, &N->dU)) ||
-        (st = alloc(N, P.inv_scratch * es, &N->dInv)) ||
+        (st = alloc(N, P.dinv * es, &N->dInv)) ||
+        (st = upload(N, P.inv, (void**)&N->dInvJobs)) ||
         (st = upload(N, P.fronts, (void**)&N->dFronts)) ||
         (st = upload(N, P.kids, (void**)&N->dKids)) ||
         (st = upload(N, P.rel, (void**)&N->dRel)) ||

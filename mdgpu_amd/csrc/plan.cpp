@@ -104,6 +104,16 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         P.max_kids = std::max(P.max_kids, d.kid_end - d.kid_begin);
     }
     P.lstore = lofs;
+    {
+        int64_t off = 0;
+        for (int f = 0; f < nf; f++) {
+            FrontDesc& d = P.fronts[f];
+            d.dinv = off;
+            const int nblk = (d.c + 63) / 64;
+            off += (int64_t)(nblk - 1) * 4096 + (int64_t)(d.c - (nblk - 1) * 64) * 64;
+        }
+        P.dinv = std::max<int64_t>(off, 1);
+    }
 
     // relative maps child update rows -> parent rows (VectorMergeIndices)
     for (int f = 0; f < nf; f++) {
@@ -168,7 +178,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
 
     // solve schedule (see SolveStep)
     {
-        const int BW = kSolveBW, RC = 256;
+        const int BW = kSolveBW, RC = 64;
         for (int L = 0; L < P.nlevels; L++) {
             std::vector<int> small, big;
             for (int f : P.level_fronts[L]) (P.fronts[f].c > BW ? big : small).push_back(f);
@@ -318,7 +328,6 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
     };
 
-    int64_t max_diag_jobs = 0;
     for (int L = 0; L < P.nlevels; L++) {
         const std::vector<int>& fl = P.level_fronts[L];
         add_ea(L, true);
@@ -363,14 +372,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 dj.b = std::min(o.nb, d.c - r0);
                 dj.ld = d.n;
                 dj.p_off = d.lofs + (int64_t)r0 * d.n + r0;
-                dj.inv_off = (int64_t)(P.diag.size() - dbegin) * o.nb * o.nb;
+                dj.inv_off = d.dinv + (int64_t)(r0 / 64) * 4096;
                 dj.col0 = d.k + r0;
                 dj.front = f;
                 P.diag.push_back(dj);
             }
             const int dcount = (int)P.diag.size() - dbegin;
             if (dcount == 0) continue;
-            max_diag_jobs = std::max<int64_t>(max_diag_jobs, dcount);
             {
                 Step st{STEP_DIAG, dbegin, dcount, dcount, L, 0};
                 for (int t = dbegin; t < dbegin + dcount; t++) {
@@ -455,7 +463,22 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
         add_ea(L, false);
     }
-    P.inv_scratch = std::max<int64_t>(max_diag_jobs * o.nb * o.nb, 1);
+    // inverses of the diagonal blocks of the LDS-panel fronts, for the solves
+    {
+        const int begin = (int)P.inv.size();
+        for (int f : P.panel)
+            for (int j0 = 0; j0 < P.fronts[f].c; j0 += 64) P.inv.push_back({f, j0});
+        const int count = (int)P.inv.size() - begin;
+        if (count > 0) {
+            Step st{STEP_INV, begin, count, count, P.nlevels - 1, 0};
+            for (int t = begin; t < begin + count; t++) {
+                const double b = std::min(64, P.fronts[P.inv[t].front].c - P.inv[t].j0);
+                st.flops += b * b * b / 3.0;
+                st.bytes += 1.5 * b * b * o.elem_bytes;
+            }
+            P.steps.push_back(st);
+        }
+    }
 
     // full symmetric A in the caller's numbering (CSR rows sorted), for the
     // residual r = b - A x of iterative refinement
@@ -548,12 +571,12 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
     for (const DiagJob& dj : P.diag) {
         if (dj.b < 1 || dj.b > o.nb || dj.b > 64) return bad("diag size");
         if (dj.p_off < 0 || dj.p_off + (dj.b - 1) + (int64_t)(dj.b - 1) * dj.ld >= P.lstore) return bad("diag bounds");
-        if (dj.inv_off < 0 || dj.inv_off + (int64_t)dj.b * dj.b > P.inv_scratch) return bad("diag scratch");
+        if (dj.inv_off < 0 || dj.inv_off + (int64_t)dj.b * 64 > P.dinv) return bad("diag inverse store");
     }
     for (const TrsmJob& tj : P.trsm) {
         if (tj.b < 1 || tj.b > 64 || tj.rows < 1) return bad("trsm shape");
         if (tj.p_off + (tj.rows - 1) + (int64_t)(tj.b - 1) * tj.ld >= P.lstore) return bad("trsm bounds");
-        if (tj.inv_off + (int64_t)tj.b * tj.b > P.inv_scratch) return bad("trsm scratch");
+        if (tj.inv_off + (int64_t)tj.b * 64 > P.dinv) return bad("trsm inverse store");
     }
     auto check_solve = [&](const std::vector<SolveStep>& steps) -> std::string {
         for (const SolveStep& st : steps) {
@@ -584,6 +607,20 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
     std::string se = check_solve(P.sv_fwd);
     if (se.empty()) se = check_solve(P.sv_bwd);
     if (!se.empty()) return bad(se);
+    for (const InvJob& ij : P.inv) {
+        if (ij.front < 0 || ij.front >= nf) return bad("inverse job front");
+        const FrontDesc& d = P.fronts[ij.front];
+        if (ij.j0 < 0 || ij.j0 >= d.c || ij.j0 % 64) return bad("inverse job block");
+        if (d.dinv + (int64_t)(ij.j0 / 64) * 4096 + (int64_t)std::min(64, d.c - ij.j0) * 64 > P.dinv)
+            return bad("inverse job store");
+    }
+    for (int f = 0; f < nf; f++) {
+        const FrontDesc& d = P.fronts[f];
+        const int nblk = (d.c + 63) / 64;
+        if (d.dinv < 0 || d.dinv + (int64_t)(nblk - 1) * 4096 + (int64_t)(d.c - (nblk - 1) * 64) * 64 > P.dinv)
+            return bad("front inverse store");
+    }
+    if (o.nb != 64) return bad("blocked path needs 64-column diagonal blocks");
     for (int f : P.panel) {
         const FrontDesc& d = P.fronts[f];
         if ((int64_t)d.n * d.c * o.elem_bytes > o.panel_lds_bytes) return bad("panel LDS");

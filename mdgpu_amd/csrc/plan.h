@@ -43,6 +43,7 @@ struct FrontDesc {            // mirrored on the device
     int kid_begin, kid_end;   // range in the kids buffer
     int split;                // # of update columns that land in parent pivots
     int level;
+    int64_t dinv;             // offset of the diagonal-block inverses (see Plan::dinv)
 };
 
 // One extend-add work item: columns [j0, j1) of child `child`'s update.
@@ -84,7 +85,14 @@ struct TrsmJob {
 };
 
 // A launch of the schedule: kind + a slice of the matching job array.
-enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_DIAG, STEP_TRSM, STEP_GEMM };
+enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_DIAG, STEP_TRSM, STEP_GEMM, STEP_INV };
+
+// Diagonal-block inverse job (fronts factored by the LDS panel kernel):
+// block of 64 pivot columns starting at j0 of front `front`.
+struct InvJob {
+    int front;
+    int j0;
+};
 
 struct Step {
     int kind;
@@ -127,7 +135,6 @@ struct Plan {
     std::vector<std::vector<int>> level_fronts;
     int64_t lstore = 0;                // elements of the L store
     int64_t arena = 0;                 // elements of the update arena (peak)
-    int64_t inv_scratch = 0;           // elements of the diag-inverse scratch
     int64_t solve_u = 0;               // elements of the solve update vectors
     std::vector<int64_t> solve_uofs;   // per front offset (m_f entries)
     std::vector<int> sv_fronts;        // front lists of SMALL / GATHER steps
@@ -139,6 +146,11 @@ struct Plan {
     std::vector<DiagJob> diag;
     std::vector<TrsmJob> trsm;
     std::vector<int> panel;            // fronts factored by the LDS panel kernel
+    std::vector<InvJob> inv;           // diagonal-block inverses of panel fronts
+    // Inverses of the 64x64 diagonal blocks of every front, block t of front
+    // f at fronts[f].dinv + 4096 t, stored as W(i + 64 r) = inv(L_D)(r, i)
+    // for i, r < b (b = block width, last block may be narrower).
+    int64_t dinv = 0;
     std::vector<Step> steps;           // the factorization schedule
     // full symmetric A (original numbering) for the residual: CSR + source
     std::vector<int> afull_p, afull_i, afull_src;

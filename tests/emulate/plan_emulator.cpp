@@ -11,6 +11,21 @@
 
 using namespace mdgpu;
 
+// W(i + 64 r) = inv(L)(r, i) for the lower-triangular b x b block g (ld)
+static void invert_block(const double* g, int ld, int b, double* w) {
+    std::vector<double> X((size_t)b * b, 0.0);   // X(r, i) at r + i*b
+    for (int i = 0; i < b; i++) {
+        X[i + i * b] = 1.0 / g[i + i * ld];
+        for (int r = i + 1; r < b; r++) {
+            double s = 0;
+            for (int l = i; l < r; l++) s += g[r + l * ld] * X[l + i * b];
+            X[r + i * b] = -s / g[r + r * ld];
+        }
+    }
+    for (int i = 0; i < b; i++)
+        for (int r = 0; r < b; r++) w[i + 64 * r] = X[r + i * b];
+}
+
 extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                               int max_zeros, int nb, double* Lx_out, int* Li_out) {
     AnalyzeOptions ao;
@@ -28,7 +43,7 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
     if (!build_plan(S, po, P).empty()) return 2;
     if (!validate_plan(P, po).empty()) return 3;
     std::vector<double> L(P.lstore, 0.0), U(std::max<int64_t>(P.arena, 1), std::nan("")),
-        inv(P.inv_scratch, 0.0);
+        inv(P.dinv, 0.0);
     for (size_t e = 0; e < P.a_dst.size(); e++) L[P.a_dst[e]] = ax[e];
     for (const Step& st : P.steps) {
         if (st.kind == STEP_EA1 || st.kind == STEP_EA2) {
@@ -69,16 +84,7 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
                     for (int i = j + 1; i < b; i++)
                         for (int col = j + 1; col <= i; col++) g[i + col * ld] -= g[i + j * ld] * g[col + j * ld];
                 }
-                double* q = &inv[dj.inv_off];
-                for (int j = 0; j < b; j++) {
-                    for (int i = 0; i < j; i++) q[i + j * b] = 0;
-                    q[j + j * b] = 1.0 / g[j + j * ld];
-                    for (int i = j + 1; i < b; i++) {
-                        double s = 0;
-                        for (int l = j; l < i; l++) s += g[i + l * ld] * q[l + j * b];
-                        q[i + j * b] = -s / g[i + i * ld];
-                    }
-                }
+                invert_block(g, ld, b, &inv[dj.inv_off]);
             }
         } else if (st.kind == STEP_TRSM) {
             for (int t = st.first; t < st.first + st.count; t++) {
@@ -90,11 +96,18 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
                     for (int l = 0; l < tj.b; l++) r[l] = row[(int64_t)l * tj.ld];
                     for (int j = 0; j < tj.b; j++) {
                         double acc = 0;
-                        for (int l = 0; l <= j; l++) acc += r[l] * q[j + l * tj.b];
+                        for (int l = 0; l <= j; l++) acc += r[l] * q[l + 64 * j];
                         out[j] = acc;
                     }
                     for (int j = 0; j < tj.b; j++) row[(int64_t)j * tj.ld] = out[j];
                 }
+            }
+        } else if (st.kind == STEP_INV) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const FrontDesc& d = P.fronts[P.inv[t].front];
+                const int j0 = P.inv[t].j0;
+                invert_block(&L[d.lofs + j0 + (int64_t)j0 * d.n], d.n, std::min(64, d.c - j0),
+                             &inv[d.dinv + (int64_t)(j0 / 64) * 4096]);
             }
         } else if (st.kind == STEP_GEMM) {
             for (int t = st.first; t < st.first + st.count; t++) {

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstdlib>
+#include <string>
 
 #include "kernels.h"
 
@@ -130,9 +132,12 @@ __device__ inline double bcast(double v, int src) {
 }
 
 // Right-looking Cholesky of the block (rows beyond the block's width are
-// identity padding).  Returns the first non-positive pivot or 64.
+// identity padding).  The pivot is broadcast with one v_readlane per
+// column; the scaled column goes through LDS (double-buffered `col`,
+// 2 x 64 entries), so the rank-1 update reads uniform LDS addresses.
+// Returns the first non-positive pivot or 64.  One wavefront per block.
 template <typename T>
-__device__ inline int wave_potrf64(T (&a)[64], int lane) {
+__device__ inline int wave_potrf64(T (&a)[64], int lane, T* col) {
     int bad = 64;
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
@@ -141,23 +146,36 @@ __device__ inline int wave_potrf64(T (&a)[64], int lane) {
         const T sq = sqrt(djj);
         const T is = T(1) / sq;
         a[j] = (lane == j) ? sq : a[j] * is;
+        T* c = col + (j & 1) * 64;
+        c[lane] = a[j];
+        __syncthreads();
 #pragma unroll
-        for (int k = j + 1; k < 64; ++k) a[k] -= a[j] * bcast(a[j], k);
+        for (int k = j + 1; k < 64; ++k) a[k] -= a[j] * c[k];
+        __syncthreads();   // keeps the next column's loads below this step
     }
     return bad;
 }
 
-// x[r] = inv(L)(r, lane): column `lane` of the inverse by forward
-// substitution, L(r, l) broadcast from lane r.
+// Column `lane` of inv(L) by forward substitution, in LDS:
+// Xs[r*65 + lane] = inv(L)(r, lane), reading L rows Ls(r, l) = Ls[r*65 + l]
+// (uniform addresses) and rd[r] = 1 / L(r, r).  Each lane only reads its
+// own column of Xs, so no barrier is needed between rows.
 template <typename T>
-__device__ inline void wave_trinv64(const T (&a)[64], T (&x)[64], int lane) {
-#pragma unroll
+__device__ inline void wave_trinv64(const T* Ls, const T* rd, T* Xs, int lane) {
     for (int r = 0; r < 64; ++r) {
         T acc = (lane == r) ? T(1) : T(0);
-#pragma unroll
-        for (int l = 0; l < r; ++l) acc -= bcast(a[l], r) * x[l];
-        x[r] = acc * (T(1) / bcast(a[r], r));
+        for (int l = lane; l < r; ++l) acc -= Ls[r * 65 + l] * Xs[l * 65 + lane];
+        Xs[r * 65 + lane] = acc * rd[r];
     }
+}
+
+// Stage the factor rows (lane i: a[0..i]) into LDS for wave_trinv64.
+template <typename T>
+__device__ inline void stage_rows(const T (&a)[64], T* Ls, T* rd, int lane) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c) Ls[lane * 65 + c] = c <= lane ? a[c] : T(0);
+    rd[lane] = T(1) / Ls[lane * 65 + lane];
+    __syncthreads();
 }
 
 // Diagonal block b x b of the blocked path: Cholesky, write L_D back, and
@@ -165,6 +183,10 @@ __device__ inline void wave_trinv64(const T (&a)[64], T (&x)[64], int lane) {
 template <typename T>
 __global__ __launch_bounds__(64) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
                                                   T* __restrict__ inv, int* __restrict__ info) {
+    __shared__ T col[128];
+    __shared__ T Ls[64 * 65];
+    __shared__ T Xs[64 * 65];
+    __shared__ T rd[64];
     const DiagJob jb = jobs[blockIdx.x];
     const int b = jb.b, ld = jb.ld, lane = threadIdx.x;
     T* __restrict__ g = L + jb.p_off;
@@ -172,27 +194,68 @@ __global__ __launch_bounds__(64) void diag_kernel(const DiagJob* __restrict__ jo
 #pragma unroll
     for (int c = 0; c < 64; ++c)
         a[c] = (lane < b && c < b) ? g[lane + (int64_t)c * ld] : (lane == c ? T(1) : T(0));
-    const int bad = wave_potrf64(a, lane);
+    const int bad = wave_potrf64(a, lane, col);
     if (bad < b && lane == 0) atomicMin(info, jb.col0 + bad);
     if (lane < b) {
 #pragma unroll
         for (int c = 0; c < 64; ++c)
             if (c <= lane && c < b) g[lane + (int64_t)c * ld] = a[c];
     }
-    T x[64];
-    wave_trinv64(a, x, lane);
+    stage_rows(a, Ls, rd, lane);
+    wave_trinv64(Ls, rd, Xs, lane);
     T* __restrict__ w = inv + jb.inv_off;
-    if (lane < b) {
-#pragma unroll
-        for (int r = 0; r < 64; ++r)
-            if (r < b) w[lane + 64 * r] = x[r];
+    if (lane < b)
+        for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
+}
+
+// Same job as diag_kernel with the block in LDS and rolled loops: lane i
+// owns row i (As[i*65 + c]); pivots and column entries of other rows are
+// uniform LDS reads, row updates hit distinct banks.
+template <typename T>
+__global__ __launch_bounds__(64) void diag_lds_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
+                                                      T* __restrict__ inv, int* __restrict__ info) {
+    __shared__ T As[64 * 65];
+    __shared__ T Xs[64 * 65];
+    __shared__ T rd[64];
+    const DiagJob jb = jobs[blockIdx.x];
+    const int b = jb.b, ld = jb.ld, lane = threadIdx.x;
+    T* __restrict__ g = L + jb.p_off;
+    for (int c = 0; c < 64; ++c)
+        As[lane * 65 + c] = (lane < b && c < b) ? g[lane + (int64_t)c * ld] : (lane == c ? T(1) : T(0));
+    __syncthreads();
+    int bad = 64;
+    T* row = As + lane * 65;
+    for (int j = 0; j < 64; ++j) {
+        const T djj = As[j * 65 + j];
+        if (!(djj > T(0)) && bad == 64) bad = j;
+        const T sq = sqrt(djj);
+        const T is = T(1) / sq;
+        __syncthreads();
+        if (lane == j) row[j] = sq;
+        else if (lane > j) row[j] *= is;
+        __syncthreads();
+        const T lij = row[j];
+        for (int k = j + 1; k <= lane; ++k) row[k] -= lij * As[k * 65 + j];
+        __syncthreads();
     }
+    if (bad < b && lane == 0) atomicMin(info, jb.col0 + bad);
+    if (lane < b)
+        for (int c = 0; c < b && c <= lane; ++c) g[lane + (int64_t)c * ld] = row[c];
+    rd[lane] = T(1) / row[lane];
+    __syncthreads();
+    wave_trinv64(As, rd, Xs, lane);
+    T* __restrict__ w = inv + jb.inv_off;
+    if (lane < b)
+        for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
 }
 
 // Inverse of one 64-column diagonal block of an already factored panel.
 template <typename T>
 __global__ __launch_bounds__(64) void inv_kernel(const InvJob* __restrict__ jobs, const FrontDesc* __restrict__ fronts,
                                                  const T* __restrict__ L, T* __restrict__ inv) {
+    __shared__ T Ls[64 * 65];
+    __shared__ T Xs[64 * 65];
+    __shared__ T rd[64];
     const InvJob jb = jobs[blockIdx.x];
     const FrontDesc d = fronts[jb.front];
     const int b = min(64, d.c - jb.j0), lane = threadIdx.x;
@@ -201,14 +264,11 @@ __global__ __launch_bounds__(64) void inv_kernel(const InvJob* __restrict__ jobs
 #pragma unroll
     for (int c = 0; c < 64; ++c)
         a[c] = (lane < b && c < b && c <= lane) ? g[lane + (int64_t)c * d.n] : (lane == c ? T(1) : T(0));
-    T x[64];
-    wave_trinv64(a, x, lane);
+    stage_rows(a, Ls, rd, lane);
+    wave_trinv64(Ls, rd, Xs, lane);
     T* __restrict__ w = inv + d.dinv + (int64_t)(jb.j0 / 64) * 4096;
-    if (lane < b) {
-#pragma unroll
-        for (int r = 0; r < 64; ++r)
-            if (r < b) w[lane + 64 * r] = x[r];
-    }
+    if (lane < b)
+        for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
 }
 
 __device__ inline int find_job_trsm(const TrsmJob* jobs, int njobs, int bid) {
@@ -355,7 +415,11 @@ template <> struct MfmaOp<double> {
 template <typename T>
 __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                              T* __restrict__ L, T* __restrict__ U) {
-    constexpr int BT = 128, KT = 16, LDK = KT + 4;
+    // k-tile depth: 32 for FP32, 16 for FP64 (register budget of the
+    // prefetch); the next k-tile is loaded into registers while the current
+    // one is multiplied out of LDS.
+    constexpr int BT = 128, KT = sizeof(T) == 4 ? 32 : 16, LDK = KT + 4;
+    constexpr int PER = (BT * KT) / kThreads;      // staged elements per thread and operand
     typedef typename MfmaOp<T>::acc_t acc_t;
     __shared__ __align__(16) T As[BT * LDK];
     __shared__ __align__(16) T Bs[BT * LDK];
@@ -385,6 +449,22 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     const int wave = tid >> 6, lane = tid & 63;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
     const int li = lane & 15, lg = lane >> 4;
+    // staging coordinates: element s of this thread is row r = (tid + 256 s) % 128,
+    // k-offset kk = (tid + 256 s) / 128
+    const int r_st = tid & (BT - 1);
+    const int ia = row0 + r_st, jb = col0 + r_st;
+    const bool a_ok = ia < g.M, b_ok = jb < g.N;
+    const T* __restrict__ Ap = A + ia;
+    const T* __restrict__ Bp = B + jb;
+    T ra[PER], rb[PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < PER; ++s) {
+            const int k = k0 + (tid >> 7) + 2 * s;
+            ra[s] = (a_ok && k < g.K) ? Ap[(int64_t)k * g.lda] : T(0);
+            rb[s] = (b_ok && k < g.K) ? Bp[(int64_t)k * g.ldb] : T(0);
+        }
+    };
     acc_t acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -392,33 +472,35 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
+    fetch(0);
     for (int k0 = 0; k0 < g.K; k0 += KT) {
-        // stage A rows row0..+127 and B rows col0..+127, k0..k0+15, as [row][k]
 #pragma unroll
-        for (int s = 0; s < (BT * KT) / kThreads; ++s) {
-            const int idx = tid + kThreads * s;
-            const int r = idx & (BT - 1), kk = idx >> 7;
-            const int k = k0 + kk;
-            const int ia = row0 + r, jb = col0 + r;
-            As[r * LDK + kk] = (k < g.K && ia < g.M) ? A[ia + (int64_t)k * g.lda] : T(0);
-            Bs[r * LDK + kk] = (k < g.K && jb < g.N) ? B[jb + (int64_t)k * g.ldb] : T(0);
+        for (int s = 0; s < PER; ++s) {
+            const int kk = (tid >> 7) + 2 * s;
+            As[r_st * LDK + kk] = ra[s];
+            Bs[r_st * LDK + kk] = rb[s];
         }
         __syncthreads();
-        T av[4][4], bv[4][4];   // [block][t]
+        if (k0 + KT < g.K) fetch(k0 + KT);
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
+        for (int kq = 0; kq < KT; kq += 16) {
+            T av[4][4], bv[4][4];   // [block][t]
 #pragma unroll
-            for (int t = 0; t < 4; ++t) av[rb][t] = As[(wr + rb * 16 + li) * LDK + 4 * lg + t];
+            for (int rb2 = 0; rb2 < 4; ++rb2)
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
+                for (int t = 0; t < 4; ++t) av[rb2][t] = As[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) bv[cb][t] = Bs[(wc + cb * 16 + li) * LDK + 4 * lg + t];
+            for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+                for (int t = 0; t < 4; ++t) bv[cb][t] = Bs[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb][t], acc[rb][cb]);
+                for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+                    for (int cb = 0; cb < 4; ++cb)
+                        acc[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], acc[rb2][cb]);
+        }
         __syncthreads();
     }
     // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + row(lg, v))
@@ -429,11 +511,11 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
             const int j = col0 + wc + cb * 16 + MfmaOp<T>::row(lg, v);
             if (j >= g.N) continue;
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-                const int i = row0 + wr + rb * 16 + li;
+            for (int rb2 = 0; rb2 < 4; ++rb2) {
+                const int i = row0 + wr + rb2 * 16 + li;
                 if (i >= g.M || (g.lower && i < j)) continue;
                 T* cp = C + i + (int64_t)j * g.ldc;
-                *cp = g.beta0 ? -acc[rb][cb][v] : *cp - acc[rb][cb][v];
+                *cp = g.beta0 ? -acc[rb2][cb][v] : *cp - acc[rb2][cb][v];
             }
         }
 }
@@ -833,7 +915,13 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 }
 template <typename T>
 void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s) {
-    if (count > 0) diag_kernel<T><<<count, 64, 0, s>>>(jobs, L, inv, info);
+    static const bool lds = [] {
+        const char* e = std::getenv("MDGPU_DIAG");
+        return !(e && std::string(e) == "reg");
+    }();
+    if (count <= 0) return;
+    if (lds) diag_lds_kernel<T><<<count, 64, 0, s>>>(jobs, L, inv, info);
+    else diag_kernel<T><<<count, 64, 0, s>>>(jobs, L, inv, info);
 }
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {

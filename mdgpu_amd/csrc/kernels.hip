@@ -131,31 +131,6 @@ __device__ inline double bcast(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Right-looking Cholesky of the block (rows beyond the block's width are
-// identity padding).  The pivot is broadcast with one v_readlane per
-// column; the scaled column goes through LDS (double-buffered `col`,
-// 2 x 64 entries), so the rank-1 update reads uniform LDS addresses.
-// Returns the first non-positive pivot or 64.  One wavefront per block.
-template <typename T>
-__device__ inline int wave_potrf64(T (&a)[64], int lane, T* col) {
-    int bad = 64;
-#pragma unroll
-    for (int j = 0; j < 64; ++j) {
-        const T djj = bcast(a[j], j);
-        if (!(djj > T(0)) && bad == 64) bad = j;
-        const T sq = sqrt(djj);
-        const T is = T(1) / sq;
-        a[j] = (lane == j) ? sq : a[j] * is;
-        T* c = col + (j & 1) * 64;
-        c[lane] = a[j];
-        __syncthreads();
-#pragma unroll
-        for (int k = j + 1; k < 64; ++k) a[k] -= a[j] * c[k];
-        __syncthreads();   // keeps the next column's loads below this step
-    }
-    return bad;
-}
-
 // Column `lane` of inv(L) by forward substitution, in LDS:
 // Xs[r*65 + lane] = inv(L)(r, lane), reading L rows Ls(r, l) = Ls[r*65 + l]
 // (uniform addresses) and rd[r] = 1 / L(r, r).  Each lane only reads its
@@ -178,75 +153,83 @@ __device__ inline void stage_rows(const T (&a)[64], T* Ls, T* rd, int lane) {
     __syncthreads();
 }
 
-// Diagonal block b x b of the blocked path: Cholesky, write L_D back, and
-// its inverse W(i + 64 r) = inv(L_D)(r, i) for the TRSM and the solves.
+// Diagonal block b x b of the blocked path: Cholesky in four 16-column
+// panels -- each panel factored by wavefront 0 with its rows in registers
+// (pivot and column entries broadcast by v_readlane within the panel), the
+// trailing part of the block updated by all four wavefronts from LDS --
+// then the block's inverse W(i + 64 r) = inv(L_D)(r, i) for the TRSM and
+// the solves.  Rows and columns beyond b are identity padding.
 template <typename T>
-__global__ __launch_bounds__(64) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
-                                                  T* __restrict__ inv, int* __restrict__ info) {
-    __shared__ T col[128];
-    __shared__ T Ls[64 * 65];
-    __shared__ T Xs[64 * 65];
+__global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
+                                                        T* __restrict__ inv, int* __restrict__ info) {
+    __shared__ T As[64 * 65];   // column-major, As[r + 65 c] = A(r, c)
+    __shared__ T Xs[64 * 65];   // Xs[r * 65 + i] = inv(L_D)(r, i)
     __shared__ T rd[64];
     const DiagJob jb = jobs[blockIdx.x];
-    const int b = jb.b, ld = jb.ld, lane = threadIdx.x;
+    const int b = jb.b, ld = jb.ld;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     T* __restrict__ g = L + jb.p_off;
-    T a[64];
-#pragma unroll
-    for (int c = 0; c < 64; ++c)
-        a[c] = (lane < b && c < b) ? g[lane + (int64_t)c * ld] : (lane == c ? T(1) : T(0));
-    const int bad = wave_potrf64(a, lane, col);
-    if (bad < b && lane == 0) atomicMin(info, jb.col0 + bad);
-    if (lane < b) {
-#pragma unroll
-        for (int c = 0; c < 64; ++c)
-            if (c <= lane && c < b) g[lane + (int64_t)c * ld] = a[c];
+    for (int t = tid; t < 64 * 64; t += kThreads) {
+        const int r = t & 63, c = t >> 6;
+        As[r + 65 * c] = (r < b && c < b) ? g[r + (int64_t)c * ld] : (r == c ? T(1) : T(0));
     }
-    stage_rows(a, Ls, rd, lane);
-    wave_trinv64(Ls, rd, Xs, lane);
-    T* __restrict__ w = inv + jb.inv_off;
-    if (lane < b)
-        for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
-}
-
-// Same job as diag_kernel with the block in LDS and rolled loops: lane i
-// owns row i (As[i*65 + c]); pivots and column entries of other rows are
-// uniform LDS reads, row updates hit distinct banks.
-template <typename T>
-__global__ __launch_bounds__(64) void diag_lds_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
-                                                      T* __restrict__ inv, int* __restrict__ info) {
-    __shared__ T As[64 * 65];
-    __shared__ T Xs[64 * 65];
-    __shared__ T rd[64];
-    const DiagJob jb = jobs[blockIdx.x];
-    const int b = jb.b, ld = jb.ld, lane = threadIdx.x;
-    T* __restrict__ g = L + jb.p_off;
-    for (int c = 0; c < 64; ++c)
-        As[lane * 65 + c] = (lane < b && c < b) ? g[lane + (int64_t)c * ld] : (lane == c ? T(1) : T(0));
     __syncthreads();
     int bad = 64;
-    T* row = As + lane * 65;
-    for (int j = 0; j < 64; ++j) {
-        const T djj = As[j * 65 + j];
-        if (!(djj > T(0)) && bad == 64) bad = j;
-        const T sq = sqrt(djj);
-        const T is = T(1) / sq;
+    for (int p = 0; p < 4; ++p) {
+        const int c0 = 16 * p;
+        if (wave == 0) {
+            const int r = c0 + lane;
+            T a[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = r < 64 ? As[r + 65 * (c0 + j)] : T(0);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const T djj = bcast(a[j], j);
+                if (!(djj > T(0)) && bad == 64) bad = c0 + j;
+                const T sq = sqrt(djj);
+                const T is = T(1) / sq;
+                a[j] = (lane == j) ? sq : a[j] * is;
+#pragma unroll
+                for (int k = j + 1; k < 16; ++k) a[k] -= a[j] * bcast(a[j], k);
+            }
+            if (r < 64) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (lane >= j) As[r + 65 * (c0 + j)] = a[j];
+            }
+        }
         __syncthreads();
-        if (lane == j) row[j] = sq;
-        else if (lane > j) row[j] *= is;
-        __syncthreads();
-        const T lij = row[j];
-        for (int k = j + 1; k <= lane; ++k) row[k] -= lij * As[k * 65 + j];
+        const int w = 48 - c0;          // trailing rows/cols [c0+16, 64)
+        for (int idx = tid; idx < w * w; idx += kThreads) {
+            const int i = c0 + 16 + idx % w, j = c0 + 16 + idx / w;
+            if (i < j) continue;
+            T acc = T(0);
+#pragma unroll
+            for (int t = 0; t < 16; ++t) acc += As[i + 65 * (c0 + t)] * As[j + 65 * (c0 + t)];
+            As[i + 65 * j] -= acc;
+        }
         __syncthreads();
     }
-    if (bad < b && lane == 0) atomicMin(info, jb.col0 + bad);
-    if (lane < b)
-        for (int c = 0; c < b && c <= lane; ++c) g[lane + (int64_t)c * ld] = row[c];
-    rd[lane] = T(1) / row[lane];
+    if (wave == 0 && lane == 0 && bad < b) atomicMin(info, jb.col0 + bad);
+    for (int t = tid; t < 64 * 64; t += kThreads) {
+        const int r = t & 63, c = t >> 6;
+        if (r < b && c < b && c <= r) g[r + (int64_t)c * ld] = As[r + 65 * c];
+    }
+    if (wave == 0) rd[lane] = T(1) / As[lane + 65 * lane];
     __syncthreads();
-    wave_trinv64(As, rd, Xs, lane);
-    T* __restrict__ w = inv + jb.inv_off;
-    if (lane < b)
-        for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
+    if (wave == 0) {
+        for (int r = 0; r < 64; ++r) {
+            T acc = (lane == r) ? T(1) : T(0);
+            for (int l = lane; l < r; ++l) acc -= As[r + 65 * l] * Xs[l * 65 + lane];
+            Xs[r * 65 + lane] = acc * rd[r];
+        }
+    }
+    __syncthreads();
+    T* __restrict__ wv = inv + jb.inv_off;
+    for (int t = tid; t < 64 * 64; t += kThreads) {
+        const int i = t & 63, r = t >> 6;
+        if (i < b && r < b) wv[i + 64 * r] = Xs[r * 65 + i];
+    }
 }
 
 // Inverse of one 64-column diagonal block of an already factored panel.
@@ -915,13 +898,7 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 }
 template <typename T>
 void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s) {
-    static const bool lds = [] {
-        const char* e = std::getenv("MDGPU_DIAG");
-        return !(e && std::string(e) == "reg");
-    }();
-    if (count <= 0) return;
-    if (lds) diag_lds_kernel<T><<<count, 64, 0, s>>>(jobs, L, inv, info);
-    else diag_kernel<T><<<count, 64, 0, s>>>(jobs, L, inv, info);
+    if (count > 0) diag_kernel<T><<<count, kThreads, 0, s>>>(jobs, L, inv, info);
 }
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {

@@ -73,6 +73,7 @@ __global__ __launch_bounds__(kThreads) void extend_add_kernel(const EaItem* __re
             rowoff = pd.c;
         }
         const T* scol = src + (int64_t)j * cd.m;
+#pragma unroll 4
         for (int i = j + lane; i < cd.m; i += 64) dcol[r[i] - rowoff] += scol[i];
     }
 }
@@ -110,6 +111,7 @@ __global__ __launch_bounds__(kThreads) void panel_kernel(const int* __restrict__
         for (int i = j + 1 + threadIdx.x; i < n; i += kThreads) {
             const T lij = P[i + j * n];
             const int cmax = min(i, c - 1);
+#pragma unroll 4
             for (int col = j + 1; col <= cmax; ++col) P[i + col * n] -= lij * P[col + j * n];
         }
         __syncthreads();
@@ -139,6 +141,7 @@ template <typename T>
 __device__ inline void wave_trinv64(const T* Ls, const T* rd, T* Xs, int lane) {
     for (int r = 0; r < 64; ++r) {
         T acc = (lane == r) ? T(1) : T(0);
+#pragma unroll 8
         for (int l = lane; l < r; ++l) acc -= Ls[r * 65 + l] * Xs[l * 65 + lane];
         Xs[r * 65 + lane] = acc * rd[r];
     }
@@ -220,6 +223,7 @@ __global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restric
     if (wave == 0) {
         for (int r = 0; r < 64; ++r) {
             T acc = (lane == r) ? T(1) : T(0);
+#pragma unroll 8
             for (int l = lane; l < r; ++l) acc -= As[r + 65 * l] * Xs[l * 65 + lane];
             Xs[r * 65 + lane] = acc * rd[r];
         }
@@ -288,6 +292,7 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmJob* __restric
     const int i = threadIdx.x & 63;
     for (int j = threadIdx.x >> 6; j < b; j += kThreads / 64) {
         T acc = T(0);
+#pragma unroll 8
         for (int l = 0; l <= j; ++l) acc += R[i + l * 64] * Q[j + l * 65];
         if (i < rows) g[i + (int64_t)j * ld] = acc;
     }
@@ -562,6 +567,7 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
         if (threadIdx.x < b) {
             const int r = threadIdx.x;
             double y = 0.0;
+#pragma unroll 8
             for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
             yb[r] = y;
             xp[k + j0 + r] = y;
@@ -569,6 +575,7 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
         __syncthreads();
         for (int i = j0 + b + threadIdx.x; i < n; i += kThreads) {
             double acc = 0.0;
+#pragma unroll 16
             for (int jj = 0; jj < b; ++jj) acc += to_d(P[i + (int64_t)(j0 + jj) * n]) * yb[jj];
             if (i < c) xp[k + i] -= acc;
             else u[i - c] -= acc;
@@ -598,6 +605,7 @@ __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ l
         for (int j = j0 + wave; j < j1; j += kThreads / 64) {
             double s = 0.0;
             const T* col = P + (int64_t)j * n;
+#pragma unroll 4
             for (int i = j1 + lane; i < n; i += 64) s += to_d(col[i]) * xp[q[i]];
             s = wave_sum(s);
             if (lane == 0) vb[j - j0] = xp[k + j] - s;
@@ -607,6 +615,7 @@ __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ l
             const int i = threadIdx.x;
             const T* __restrict__ w = inv + d.dinv + (int64_t)blk * 4096;
             double x = 0.0;
+#pragma unroll 8
             for (int r = i; r < b; ++r) x += to_d(w[i + 64 * r]) * vb[r];
             xp[k + j0 + i] = x;
         }
@@ -673,6 +682,7 @@ __global__ __launch_bounds__(kThreads) void fwd_tri_kernel(const SolveItem* __re
         if (threadIdx.x < b) {
             const int r = threadIdx.x;
             double y = 0.0;
+#pragma unroll 8
             for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
             yb[r] = y;
             xp[k + s0 + r] = y;
@@ -680,6 +690,7 @@ __global__ __launch_bounds__(kThreads) void fwd_tri_kernel(const SolveItem* __re
         __syncthreads();
         for (int i = s0 + b + threadIdx.x; i < j1; i += kThreads) {
             double acc = 0.0;
+#pragma unroll 16
             for (int jj = 0; jj < b; ++jj) acc += to_d(P[i + (int64_t)(s0 + jj) * n]) * yb[jj];
             xp[k + i] -= acc;
         }
@@ -707,6 +718,7 @@ __global__ __launch_bounds__(kThreads) void fwd_gemv_kernel(const SolveItem* __r
     double acc = 0.0;
     if (i < d.n) {
         const T* __restrict__ P = L + d.lofs + i + (int64_t)it.j0 * d.n;
+#pragma unroll 16
         for (int jj = wave * 64; jj < min(bw, wave * 64 + 64); ++jj) acc += to_d(P[(int64_t)jj * d.n]) * yb[jj];
     }
     red[wave][lane] = acc;
@@ -740,11 +752,13 @@ __global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __re
     for (int c0 = 0; c0 < bw; c0 += 64) {
         const int cw = min(64, bw - c0);
         __syncthreads();
+#pragma unroll
         for (int cc = cq; cc < 64; cc += 4)
             tile[cc * 65 + r] = (r < rows && cc < cw) ? P[r + (int64_t)(c0 + cc) * d.n] : T(0);
         __syncthreads();
         // thread (quarter cq, column r): rows cq*16 .. cq*16+15 of column r
         double s = 0.0;
+#pragma unroll
         for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) s += to_d(tile[r * 65 + rr]) * xq[rr];
         red[cq][r] = s;
         __syncthreads();
@@ -771,6 +785,7 @@ __global__ __launch_bounds__(kThreads) void bwd_tri_kernel(const SolveItem* __re
     const T* __restrict__ P = L + d.lofs;
     for (int t = threadIdx.x; t < bw; t += kThreads) {
         double v = xp[k + j0 + t];
+#pragma unroll 8
         for (int sl = 0; sl < it.b; ++sl) v -= partial[(int64_t)(it.a + sl) * kSolveBW + t];
         vb[t] = v;
     }
@@ -782,6 +797,7 @@ __global__ __launch_bounds__(kThreads) void bwd_tri_kernel(const SolveItem* __re
         for (int j = s0 + wave; j < s1; j += kThreads / 64) {
             double acc = 0.0;
             const T* col = P + (int64_t)j * n;
+#pragma unroll 4
             for (int i = s1 + lane; i < j1; i += 64) acc += to_d(col[i]) * xp[k + i];
             acc = wave_sum(acc);
             if (lane == 0) vb[j - j0] -= acc;
@@ -791,6 +807,7 @@ __global__ __launch_bounds__(kThreads) void bwd_tri_kernel(const SolveItem* __re
             const int i = threadIdx.x;
             const T* __restrict__ w = inv + d.dinv + (int64_t)(s0 / 64) * 4096;
             double x = 0.0;
+#pragma unroll 8
             for (int rr = i; rr < b; ++rr) x += to_d(w[i + 64 * rr]) * vb[s0 - j0 + rr];
             xp[k + s0 + i] = x;
         }

@@ -23,7 +23,7 @@ void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStr
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
 template <typename T>
-void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv, hipStream_t s);
+void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, hipStream_t s);
 // tile 128: MFMA kernel; tile 64: FMA kernel (the plan's tile size decides)
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s);

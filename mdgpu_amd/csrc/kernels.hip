@@ -159,15 +159,13 @@ __device__ inline void stage_rows(const T (&a)[64], T* Ls, T* rd, int lane) {
 // Diagonal block b x b of the blocked path: Cholesky in four 16-column
 // panels -- each panel factored by wavefront 0 with its rows in registers
 // (pivot and column entries broadcast by v_readlane within the panel), the
-// trailing part of the block updated by all four wavefronts from LDS --
-// then the block's inverse W(i + 64 r) = inv(L_D)(r, i) for the TRSM and
-// the solves.  Rows and columns beyond b are identity padding.
+// trailing part of the block updated by all four wavefronts from LDS.
+// Rows and columns beyond b are identity padding.  (The block's inverse,
+// needed only by the solves, is formed later by inv_kernel.)
 template <typename T>
 __global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
                                                         T* __restrict__ inv, int* __restrict__ info) {
     __shared__ T As[64 * 65];   // column-major, As[r + 65 c] = A(r, c)
-    __shared__ T Xs[64 * 65];   // Xs[r * 65 + i] = inv(L_D)(r, i)
-    __shared__ T rd[64];
     const DiagJob jb = jobs[blockIdx.x];
     const int b = jb.b, ld = jb.ld;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -218,22 +216,6 @@ __global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restric
         const int r = t & 63, c = t >> 6;
         if (r < b && c < b && c <= r) g[r + (int64_t)c * ld] = As[r + 65 * c];
     }
-    if (wave == 0) rd[lane] = T(1) / As[lane + 65 * lane];
-    __syncthreads();
-    if (wave == 0) {
-        for (int r = 0; r < 64; ++r) {
-            T acc = (lane == r) ? T(1) : T(0);
-#pragma unroll 8
-            for (int l = lane; l < r; ++l) acc -= As[r + 65 * l] * Xs[l * 65 + lane];
-            Xs[r * 65 + lane] = acc * rd[r];
-        }
-    }
-    __syncthreads();
-    T* __restrict__ wv = inv + jb.inv_off;
-    for (int t = tid; t < 64 * 64; t += kThreads) {
-        const int i = t & 63, r = t >> 6;
-        if (i < b && r < b) wv[i + 64 * r] = Xs[r * 65 + i];
-    }
 }
 
 // Inverse of one 64-column diagonal block of an already factored panel.
@@ -267,34 +249,45 @@ __device__ inline int find_job_trsm(const TrsmJob* jobs, int njobs, int bid) {
     return lo;
 }
 
-// R <- R * inv(L_D)^T for a 64-row tile of the rows below a diagonal block.
+// R <- R * L_D^{-T} for a 64-row tile of the rows below a diagonal block
+// (the trsm of FactorAux_CPU_3, mdgpu.cu:1330): each of 64 threads solves
+// x L_D^T = r for its row by forward substitution; L_D (read from the
+// panel) and the rows sit in LDS, L_D entries are uniform broadcasts.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmJob* __restrict__ jobs, int njobs,
-                                                        T* __restrict__ L, const T* __restrict__ inv) {
-    __shared__ T R[64 * 64];
-    __shared__ T Q[64 * 65];
+                                                        T* __restrict__ L) {
+    __shared__ T R[64 * 65];    // R[i * 65 + l]: row i of the tile
+    __shared__ T Ld[64 * 65];   // Ld[j * 65 + l] = L_D(j, l)
+    __shared__ T rd[64];
     const TrsmJob jb = jobs[find_job_trsm(jobs, njobs, blockIdx.x)];
     const int tile = blockIdx.x - jb.tile_base;
     const int r0 = tile * 64;
     const int rows = min(64, jb.rows - r0);
     const int b = jb.b, ld = jb.ld;
     T* __restrict__ g = L + jb.p_off + r0;
-    const T* __restrict__ q = inv + jb.inv_off;
+    const T* __restrict__ dg = L + jb.p_off - b;     // the diagonal block (b rows above)
     for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
-        int i = t & 63, l = t >> 6;
-        R[i + l * 64] = i < rows ? g[i + (int64_t)l * ld] : T(0);
-    }
-    for (int t = threadIdx.x; t < b * b; t += kThreads) {
-        int l = t % b, j = t / b;
-        Q[j + l * 65] = q[l + 64 * j];          // inv(L_D)(j, l)
+        const int i = t & 63, l = t >> 6;
+        R[i * 65 + l] = i < rows ? g[i + (int64_t)l * ld] : T(0);
+        if (i < b) Ld[i * 65 + l] = l <= i ? dg[i + (int64_t)l * ld] : T(0);
     }
     __syncthreads();
-    const int i = threadIdx.x & 63;
-    for (int j = threadIdx.x >> 6; j < b; j += kThreads / 64) {
-        T acc = T(0);
+    if (threadIdx.x < b) rd[threadIdx.x] = T(1) / Ld[threadIdx.x * 66];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        T* row = R + threadIdx.x * 65;
+        for (int j = 0; j < b; ++j) {
+            T acc = row[j];
+            const T* lj = Ld + j * 65;
 #pragma unroll 8
-        for (int l = 0; l <= j; ++l) acc += R[i + l * 64] * Q[j + l * 65];
-        if (i < rows) g[i + (int64_t)j * ld] = acc;
+            for (int l = 0; l < j; ++l) acc -= row[l] * lj[l];
+            row[j] = acc * rd[j];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
+        const int i = t & 63, l = t >> 6;
+        if (i < rows) g[i + (int64_t)l * ld] = R[i * 65 + l];
     }
 }
 
@@ -922,8 +915,8 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
     if (count > 0) inv_kernel<T><<<count, 64, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
-void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, const T* inv, hipStream_t s) {
-    if (nblocks > 0) trsm_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, inv);
+void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, hipStream_t s) {
+    if (nblocks > 0) trsm_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L);
 }
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s) {
@@ -1006,7 +999,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
     template void launch_diag<T>(const DiagJob*, int, T*, T*, int*, hipStream_t);                             \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
-    template void launch_trsm<T>(const TrsmJob*, int, int, T*, const T*, hipStream_t);                         \
+    template void launch_trsm<T>(const TrsmJob*, int, int, T*, hipStream_t);                                   \
     template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, const int64_t*, hipStream_t);            \

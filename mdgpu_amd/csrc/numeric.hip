@@ -198,7 +198,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_diag<T>(N->dDiag + st.first, st.count, L, inv, N->dInfo, s);
                 break;
             case mdgpu::STEP_TRSM:
-                mdgpu::launch_trsm<T>(N->dTrsm + st.first, st.count, st.nblocks, L, inv, s);
+                mdgpu::launch_trsm<T>(N->dTrsm + st.first, st.count, st.nblocks, L, s);
                 break;
             case mdgpu::STEP_GEMM:
                 mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, N->tile, L, U, s);

@@ -463,10 +463,11 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
         add_ea(L, false);
     }
-    // inverses of the diagonal blocks of the LDS-panel fronts, for the solves
+    // inverses of the diagonal blocks of every front, for the solves (off
+    // the factorization's critical path: one launch at the end)
     {
         const int begin = (int)P.inv.size();
-        for (int f : P.panel)
+        for (int f = 0; f < nf; f++)
             for (int j0 = 0; j0 < P.fronts[f].c; j0 += 64) P.inv.push_back({f, j0});
         const int count = (int)P.inv.size() - begin;
         if (count > 0) {

@@ -146,7 +146,7 @@ struct Plan {
     std::vector<DiagJob> diag;
     std::vector<TrsmJob> trsm;
     std::vector<int> panel;            // fronts factored by the LDS panel kernel
-    std::vector<InvJob> inv;           // diagonal-block inverses of panel fronts
+    std::vector<InvJob> inv;           // diagonal-block inverses of every front
     // Inverses of the 64x64 diagonal blocks of every front, block t of front
     // f at fronts[f].dinv + 4096 t, stored as W(i + 64 r) = inv(L_D)(r, i)
     // for i, r < b (b = block width, last block may be narrower).

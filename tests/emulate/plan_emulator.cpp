@@ -84,22 +84,19 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
                     for (int i = j + 1; i < b; i++)
                         for (int col = j + 1; col <= i; col++) g[i + col * ld] -= g[i + j * ld] * g[col + j * ld];
                 }
-                invert_block(g, ld, b, &inv[dj.inv_off]);
+                (void)inv;
             }
         } else if (st.kind == STEP_TRSM) {
             for (int t = st.first; t < st.first + st.count; t++) {
                 const TrsmJob& tj = P.trsm[t];
-                const double* q = &inv[tj.inv_off];
+                const double* dg = &L[tj.p_off - tj.b];
                 for (int i = 0; i < tj.rows; i++) {
                     double* row = &L[tj.p_off + i];
-                    std::vector<double> r(tj.b), out(tj.b);
-                    for (int l = 0; l < tj.b; l++) r[l] = row[(int64_t)l * tj.ld];
                     for (int j = 0; j < tj.b; j++) {
-                        double acc = 0;
-                        for (int l = 0; l <= j; l++) acc += r[l] * q[l + 64 * j];
-                        out[j] = acc;
+                        double acc = row[(int64_t)j * tj.ld];
+                        for (int l = 0; l < j; l++) acc -= row[(int64_t)l * tj.ld] * dg[j + (int64_t)l * tj.ld];
+                        row[(int64_t)j * tj.ld] = acc / dg[j + (int64_t)j * tj.ld];
                     }
-                    for (int j = 0; j < tj.b; j++) row[(int64_t)j * tj.ld] = out[j];
                 }
             }
         } else if (st.kind == STEP_INV) {

@@ -28,32 +28,25 @@ void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, hipStream_t 
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s);
 
-// solves (compute in FP64, factor stored as T)
+// solves (compute in FP64, factor stored as T); the forward result goes to
+// Y, the backward result to xp
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s);
 void launch_permute_out(const double* xp, const int* perm, double* x, int n, hipStream_t s);
 template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
-                const int* q, const T* L, const T* inv, double* xp, double* su, const int64_t* suofs,
+                const int* q, const T* L, const T* inv, double* xp, double* Y, double* su, const int64_t* suofs,
                 hipStream_t s);
 template <typename T>
 void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
-                double* xp, hipStream_t s);
-
-// solves of fronts with more than kSolveBW pivots (see SolveStep)
+                double* xp, const double* Y, hipStream_t s);
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                        double* xp, double* su, const int64_t* suofs, hipStream_t s);
 template <typename T>
-void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
-                    double* xp, hipStream_t s);
+void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+                    double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s);
 template <typename T>
-void launch_fwd_gemv(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
-                     double* su, const int64_t* suofs, hipStream_t s);
-template <typename T>
-void launch_bwd_dot(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
-                    const double* xp, double* partial, hipStream_t s);
-template <typename T>
-void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
-                    double* xp, const double* partial, hipStream_t s);
+void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
+                    const T* inv, double* xp, const double* Y, double* partial, int* counters, hipStream_t s);
 
 // refinement helpers (FP64)
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);

@@ -274,17 +274,32 @@ __global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmJob* __restric
     __syncthreads();
     if (threadIdx.x < b) rd[threadIdx.x] = T(1) / Ld[threadIdx.x * 66];
     __syncthreads();
-    if (threadIdx.x < 64) {
-        T* row = R + threadIdx.x * 65;
-        for (int j = 0; j < b; ++j) {
-            T acc = row[j];
-            const T* lj = Ld + j * 65;
-#pragma unroll 8
-            for (int l = 0; l < j; ++l) acc -= row[l] * lj[l];
-            row[j] = acc * rd[j];
+    // blocked substitution: 16 columns at a time on the 64 row threads, then
+    // all threads apply those columns to the remaining ones
+    for (int c0 = 0; c0 < b; c0 += 16) {
+        const int cw = min(16, b - c0);
+        if (threadIdx.x < 64) {
+            T* row = R + threadIdx.x * 65;
+            for (int j = c0; j < c0 + cw; ++j) {
+                T acc = row[j];
+                const T* lj = Ld + j * 65;
+                for (int l = c0; l < j; ++l) acc -= row[l] * lj[l];
+                row[j] = acc * rd[j];
+            }
         }
+        __syncthreads();
+        const int rest = b - c0 - cw;
+        for (int idx = threadIdx.x; idx < 64 * rest; idx += kThreads) {
+            const int i = idx & 63, j = c0 + cw + (idx >> 6);
+            T acc = T(0);
+            const T* ri = R + i * 65 + c0;
+            const T* lj = Ld + j * 65 + c0;
+#pragma unroll 16
+            for (int t = 0; t < cw; ++t) acc += ri[t] * lj[t];
+            R[i * 65 + j] -= acc;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
         const int i = t & 63, l = t >> 6;
         if (i < rows) g[i + (int64_t)l * ld] = R[i * 65 + l];
@@ -524,7 +539,7 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
                                                        const int* __restrict__ kids, const int* __restrict__ rel,
                                                        const int* __restrict__ q, const T* __restrict__ L,
                                                        const T* __restrict__ inv, double* __restrict__ xp,
-                                                       double* __restrict__ su,
+                                                       double* __restrict__ Y, double* __restrict__ su,
                                                        const int64_t* __restrict__ suofs) {
     __shared__ double W[64 * 65];
     __shared__ double vb[64];
@@ -563,7 +578,7 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
 #pragma unroll 8
             for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
             yb[r] = y;
-            xp[k + j0 + r] = y;
+            Y[k + j0 + r] = y;
         }
         __syncthreads();
         for (int i = j0 + b + threadIdx.x; i < n; i += kThreads) {
@@ -585,7 +600,8 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ list,
                                                        const FrontDesc* __restrict__ fronts,
                                                        const int* __restrict__ qall, const T* __restrict__ L,
-                                                       const T* __restrict__ inv, double* __restrict__ xp) {
+                                                       const T* __restrict__ inv, double* __restrict__ xp,
+                                                       const double* __restrict__ Y) {
     __shared__ double vb[64];
     const FrontDesc d = fronts[list[blockIdx.x]];
     const int n = d.n, c = d.c, k = d.k;
@@ -601,7 +617,7 @@ __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ l
 #pragma unroll 4
             for (int i = j1 + lane; i < n; i += 64) s += to_d(col[i]) * xp[q[i]];
             s = wave_sum(s);
-            if (lane == 0) vb[j - j0] = xp[k + j] - s;
+            if (lane == 0) vb[j - j0] = Y[k + j] - s;
         }
         __syncthreads();
         if (threadIdx.x < b) {
@@ -648,71 +664,51 @@ __global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restr
     }
 }
 
-// Forward: the 256-column block's own triangle, 64 columns at a time with
-// the stored diagonal inverses, updates applied within the block.
+// Forward, one 64-column diagonal block of a large front per launch: every
+// workgroup recomputes y = inv(L_D) v for the block (v is final: earlier
+// blocks were applied by earlier launches), the `first` one stores y in Y,
+// and each updates its 64 rows [a, a+64) below the block; the four
+// wavefronts split the block's columns.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void fwd_tri_kernel(const SolveItem* __restrict__ items,
+__global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
                                                            const T* __restrict__ L, const T* __restrict__ inv,
-                                                           double* __restrict__ xp) {
+                                                           double* __restrict__ xp, double* __restrict__ Y,
+                                                           double* __restrict__ su,
+                                                           const int64_t* __restrict__ suofs) {
     __shared__ double W[64 * 65];
     __shared__ double vb[64];
     __shared__ double yb[64];
-    const SolveItem it = items[blockIdx.x];
-    const FrontDesc d = fronts[it.front];
-    const int n = d.n, k = d.k;
-    const int j1 = min(it.j0 + kSolveBW, d.c);
-    const T* __restrict__ P = L + d.lofs;
-    for (int s0 = it.j0; s0 < j1; s0 += 64) {
-        const int b = min(64, j1 - s0);
-        const T* __restrict__ w = inv + d.dinv + (int64_t)(s0 / 64) * 4096;
-        for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
-            int l = t & 63, r = t >> 6;
-            if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
-        }
-        for (int t = threadIdx.x; t < b; t += kThreads) vb[t] = xp[k + s0 + t];
-        __syncthreads();
-        if (threadIdx.x < b) {
-            const int r = threadIdx.x;
-            double y = 0.0;
-#pragma unroll 8
-            for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
-            yb[r] = y;
-            xp[k + s0 + r] = y;
-        }
-        __syncthreads();
-        for (int i = s0 + b + threadIdx.x; i < j1; i += kThreads) {
-            double acc = 0.0;
-#pragma unroll 16
-            for (int jj = 0; jj < b; ++jj) acc += to_d(P[i + (int64_t)(s0 + jj) * n]) * yb[jj];
-            xp[k + i] -= acc;
-        }
-        __syncthreads();
-    }
-}
-
-// Forward: 64 rows [a, a+64) below a solved block -= L(rows, block) y(block);
-// each wavefront takes 64 of the block's columns, lanes over rows.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void fwd_gemv_kernel(const SolveItem* __restrict__ items,
-                                                            const FrontDesc* __restrict__ fronts,
-                                                            const T* __restrict__ L, double* __restrict__ xp,
-                                                            double* __restrict__ su,
-                                                            const int64_t* __restrict__ suofs) {
-    __shared__ double yb[kSolveBW];
     __shared__ double red[kThreads / 64][64];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
-    const int j1 = min(it.j0 + kSolveBW, d.c), bw = j1 - it.j0;
-    for (int t = threadIdx.x; t < bw; t += kThreads) yb[t] = xp[d.k + it.j0 + t];
+    const int j0 = it.j0, b = min(64, d.c - j0);
+    const T* __restrict__ w = inv + d.dinv + (int64_t)(j0 / 64) * 4096;
+    for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
+        const int l = t & 63, r = t >> 6;
+        if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
+    }
+    if (threadIdx.x < b) vb[threadIdx.x] = xp[d.k + j0 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < b) {
+        const int r = threadIdx.x;
+        double y = 0.0;
+#pragma unroll 8
+        for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
+        yb[r] = y;
+        if (it.first) Y[d.k + j0 + r] = y;
+    }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = it.a + lane;
     double acc = 0.0;
     if (i < d.n) {
-        const T* __restrict__ P = L + d.lofs + i + (int64_t)it.j0 * d.n;
-#pragma unroll 16
-        for (int jj = wave * 64; jj < min(bw, wave * 64 + 64); ++jj) acc += to_d(P[(int64_t)jj * d.n]) * yb[jj];
+        const T* __restrict__ P = L + d.lofs + i + (int64_t)j0 * d.n;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int jj = wave * 16 + t;
+            if (jj < b) acc += to_d(P[(int64_t)jj * d.n]) * yb[jj];
+        }
     }
     red[wave][lane] = acc;
     __syncthreads();
@@ -723,89 +719,61 @@ __global__ __launch_bounds__(kThreads) void fwd_gemv_kernel(const SolveItem* __r
     }
 }
 
-// Backward: partial dot products of the block's columns with x over the 64
-// rows [a, a+64) below the block, staged through LDS 64 columns at a time;
-// partial[slot][j] is reduced in a fixed order by bwd_tri_kernel.
+// Backward, one 64-column diagonal block of a large front per launch: each
+// workgroup forms partial dot products of the block's columns with x over
+// its 64 rows below the block (tile staged through LDS); the last workgroup
+// of the group to arrive (atomic counter; none waits) sums the partials in
+// slot order, applies inv(L_D)^T to Y(block) - sum, stores x, and resets
+// the counter for the next launch.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __restrict__ items,
+__global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
                                                            const int* __restrict__ qall, const T* __restrict__ L,
-                                                           const double* __restrict__ xp,
-                                                           double* __restrict__ partial) {
+                                                           const T* __restrict__ inv, double* __restrict__ xp,
+                                                           const double* __restrict__ Y,
+                                                           double* __restrict__ partial,
+                                                           int* __restrict__ counters) {
     __shared__ T tile[64 * 65];
     __shared__ double xq[64];
     __shared__ double red[4][64];
+    __shared__ double vb[64];
+    __shared__ int last;
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
-    const int j1 = min(it.j0 + kSolveBW, d.c), bw = j1 - it.j0;
-    const int rows = min(64, d.n - it.a);
-    if (threadIdx.x < 64) xq[threadIdx.x] = threadIdx.x < rows ? xp[qall[d.qofs + it.a + threadIdx.x]] : 0.0;
-    const T* __restrict__ P = L + d.lofs + it.a + (int64_t)it.j0 * d.n;
+    const int j0 = it.j0, b = min(64, d.c - j0);
+    const int rows = max(0, min(64, d.n - it.a));
     const int r = threadIdx.x & 63, cq = threadIdx.x >> 6;
-    for (int c0 = 0; c0 < bw; c0 += 64) {
-        const int cw = min(64, bw - c0);
-        __syncthreads();
+    if (threadIdx.x < 64) xq[r] = r < rows ? xp[qall[d.qofs + it.a + r]] : 0.0;
+    const T* __restrict__ P = L + d.lofs + it.a + (int64_t)j0 * d.n;
 #pragma unroll
-        for (int cc = cq; cc < 64; cc += 4)
-            tile[cc * 65 + r] = (r < rows && cc < cw) ? P[r + (int64_t)(c0 + cc) * d.n] : T(0);
-        __syncthreads();
-        // thread (quarter cq, column r): rows cq*16 .. cq*16+15 of column r
-        double s = 0.0;
+    for (int cc = cq; cc < 64; cc += 4) tile[cc * 65 + r] = (r < rows && cc < b) ? P[r + (int64_t)cc * d.n] : T(0);
+    __syncthreads();
+    double sdot = 0.0;
 #pragma unroll
-        for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) s += to_d(tile[r * 65 + rr]) * xq[rr];
-        red[cq][r] = s;
-        __syncthreads();
-        if (threadIdx.x < cw)
-            partial[(int64_t)it.b * kSolveBW + c0 + threadIdx.x] =
-                red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-    }
-}
-
-// Backward: v = y(block) - sum of partials (fixed order), then per 64-column
-// sub-block from the last: dots with the block rows already solved and
-// x = inv(L_D)^T v.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void bwd_tri_kernel(const SolveItem* __restrict__ items,
-                                                           const FrontDesc* __restrict__ fronts,
-                                                           const T* __restrict__ L, const T* __restrict__ inv,
-                                                           double* __restrict__ xp,
-                                                           const double* __restrict__ partial) {
-    __shared__ double vb[kSolveBW];
-    const SolveItem it = items[blockIdx.x];
-    const FrontDesc d = fronts[it.front];
-    const int n = d.n, k = d.k, j0 = it.j0;
-    const int j1 = min(j0 + kSolveBW, d.c), bw = j1 - j0;
-    const T* __restrict__ P = L + d.lofs;
-    for (int t = threadIdx.x; t < bw; t += kThreads) {
-        double v = xp[k + j0 + t];
-#pragma unroll 8
-        for (int sl = 0; sl < it.b; ++sl) v -= partial[(int64_t)(it.a + sl) * kSolveBW + t];
-        vb[t] = v;
+    for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) sdot += to_d(tile[r * 65 + rr]) * xq[rr];
+    red[cq][r] = sdot;
+    __syncthreads();
+    if (threadIdx.x < 64) partial[(int64_t)it.slot * 64 + r] = red[0][r] + red[1][r] + red[2][r] + red[3][r];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&counters[it.group], 1) == it.gcount - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < b) {
+        double v = Y[d.k + j0 + r];
+        for (int sl = 0; sl < it.gcount; ++sl) v -= partial[(int64_t)(it.gfirst + sl) * 64 + r];
+        vb[r] = v;
     }
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nsub = (bw + 63) / 64;
-    for (int sb = nsub - 1; sb >= 0; --sb) {
-        const int s0 = j0 + sb * 64, s1 = min(s0 + 64, j1), b = s1 - s0;
-        for (int j = s0 + wave; j < s1; j += kThreads / 64) {
-            double acc = 0.0;
-            const T* col = P + (int64_t)j * n;
-#pragma unroll 4
-            for (int i = s1 + lane; i < j1; i += 64) acc += to_d(col[i]) * xp[k + i];
-            acc = wave_sum(acc);
-            if (lane == 0) vb[j - j0] -= acc;
-        }
-        __syncthreads();
-        if (threadIdx.x < b) {
-            const int i = threadIdx.x;
-            const T* __restrict__ w = inv + d.dinv + (int64_t)(s0 / 64) * 4096;
-            double x = 0.0;
+    if (threadIdx.x < b) {
+        const T* __restrict__ w = inv + d.dinv + (int64_t)(j0 / 64) * 4096;
+        double x = 0.0;
 #pragma unroll 8
-            for (int rr = i; rr < b; ++rr) x += to_d(w[i + 64 * rr]) * vb[s0 - j0 + rr];
-            xp[k + s0 + i] = x;
-        }
-        __syncthreads();
+        for (int rr = r; rr < b; ++rr) x += to_d(w[r + 64 * rr]) * vb[rr];
+        xp[d.k + j0 + r] = x;
     }
+    if (threadIdx.x == 0) counters[it.group] = 0;
 }
 
 // -------------------------------------------------------------- refinement
@@ -933,14 +901,14 @@ void launch_permute_out(const double* xp, const int* perm, double* x, int n, hip
 }
 template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
-                const int* q, const T* L, const T* inv, double* xp, double* su, const int64_t* suofs,
+                const int* q, const T* L, const T* inv, double* xp, double* Y, double* su, const int64_t* suofs,
                 hipStream_t s) {
-    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, su, suofs);
+    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, Y, su, suofs);
 }
 template <typename T>
 void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
-                double* xp, hipStream_t s) {
-    if (count > 0) bwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, q, L, inv, xp);
+                double* xp, const double* Y, hipStream_t s) {
+    if (count > 0) bwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, q, L, inv, xp, Y);
 }
 
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
@@ -948,24 +916,15 @@ void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, cons
     if (count > 0) fwd_gather_kernel<<<count, kThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
 }
 template <typename T>
-void launch_fwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
-                    double* xp, hipStream_t s) {
-    if (count > 0) fwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, inv, xp);
+void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+                    double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s) {
+    if (count > 0) fwd_blk_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, inv, xp, Y, su, suofs);
 }
 template <typename T>
-void launch_fwd_gemv(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, double* xp,
-                     double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_gemv_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, xp, su, suofs);
-}
-template <typename T>
-void launch_bwd_dot(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
-                    const double* xp, double* partial, hipStream_t s) {
-    if (count > 0) bwd_dot_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, xp, partial);
-}
-template <typename T>
-void launch_bwd_tri(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
-                    double* xp, const double* partial, hipStream_t s) {
-    if (count > 0) bwd_tri_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, inv, xp, partial);
+void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
+                    const T* inv, double* xp, const double* Y, double* partial, int* counters, hipStream_t s) {
+    if (count > 0)
+        bwd_blk_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, inv, xp, Y, partial, counters);
 }
 
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
@@ -1002,18 +961,14 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_trsm<T>(const TrsmJob*, int, int, T*, hipStream_t);                                   \
     template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
-                                const T*, const T*, double*, double*, const int64_t*, hipStream_t);            \
+                                const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \
     template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, const T*, double*,    \
-                                hipStream_t);                                                                  \
+                                const double*, hipStream_t);                                                   \
     template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);               \
-    template void launch_fwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,    \
-                                    hipStream_t);                                                            \
-    template void launch_fwd_gemv<T>(const SolveItem*, int, const FrontDesc*, const T*, double*, double*,    \
-                                     const int64_t*, hipStream_t);                                           \
-    template void launch_bwd_dot<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,          \
-                                    const double*, double*, hipStream_t);                                    \
-    template void launch_bwd_tri<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
-                                    const double*, hipStream_t);
+    template void launch_fwd_blk<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
+                                    double*, double*, const int64_t*, hipStream_t);                          \
+    template void launch_bwd_blk<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,          \
+                                    const T*, double*, const double*, double*, int*, hipStream_t);
 
 MDGPU_INSTANTIATE(double)
 MDGPU_INSTANTIATE(float)

@@ -80,6 +80,8 @@ struct mdgpu_numeric {
     int* dSvFronts = nullptr;
     mdgpu::SolveItem* dSvItems = nullptr;
     double* dSvPartial = nullptr;
+    int* dSvCounters = nullptr;
+    double* dY = nullptr;
     double anorm = -1;
     bool factored = false;
     int64_t device_bytes = 0;
@@ -226,18 +228,15 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
         const mdgpu::SolveItem* items = N->dSvItems + st.first;
         switch (st.kind) {
             case mdgpu::SV_FWD_SMALL:
-                mdgpu::launch_fwd<T>(list, st.count, N->dFronts, N->dKids, N->dRel, N->dQ, L, inv, N->dXp,
+                mdgpu::launch_fwd<T>(list, st.count, N->dFronts, N->dKids, N->dRel, N->dQ, L, inv, N->dXp, N->dY,
                                      N->dSu, N->dSuofs, s);
                 break;
             case mdgpu::SV_FWD_GATHER:
                 mdgpu::launch_fwd_gather(list, st.count, N->dFronts, N->dKids, N->dRel, N->dXp, N->dSu, N->dSuofs,
                                          s);
                 break;
-            case mdgpu::SV_FWD_TRI:
-                mdgpu::launch_fwd_tri<T>(items, st.count, N->dFronts, L, inv, N->dXp, s);
-                break;
-            case mdgpu::SV_FWD_GEMV:
-                mdgpu::launch_fwd_gemv<T>(items, st.count, N->dFronts, L, N->dXp, N->dSu, N->dSuofs, s);
+            case mdgpu::SV_FWD_BLK:
+                mdgpu::launch_fwd_blk<T>(items, st.count, N->dFronts, L, inv, N->dXp, N->dY, N->dSu, N->dSuofs, s);
                 break;
         }
     }
@@ -246,13 +245,11 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
         const mdgpu::SolveItem* items = N->dSvItems + st.first;
         switch (st.kind) {
             case mdgpu::SV_BWD_SMALL:
-                mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, inv, N->dXp, s);
+                mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, inv, N->dXp, N->dY, s);
                 break;
-            case mdgpu::SV_BWD_DOT:
-                mdgpu::launch_bwd_dot<T>(items, st.count, N->dFronts, N->dQ, L, N->dXp, N->dSvPartial, s);
-                break;
-            case mdgpu::SV_BWD_TRI:
-                mdgpu::launch_bwd_tri<T>(items, st.count, N->dFronts, L, inv, N->dXp, N->dSvPartial, s);
+            case mdgpu::SV_BWD_BLK:
+                mdgpu::launch_bwd_blk<T>(items, st.count, N->dFronts, N->dQ, L, inv, N->dXp, N->dY, N->dSvPartial,
+                                         N->dSvCounters, s);
                 break;
         }
     }
@@ -379,7 +376,9 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dX)) ||
         (st = upload(N, P.sv_fronts, (void**)&N->dSvFronts)) ||
         (st = upload(N, P.sv_items, (void**)&N->dSvItems)) ||
-        (st = alloc(N, P.sv_partial * sizeof(double), (void**)&N->dSvPartial)))
+        (st = alloc(N, P.sv_partial * sizeof(double), (void**)&N->dSvPartial)) ||
+        (st = upload(N, std::vector<int>(P.sv_groups, 0), (void**)&N->dSvCounters)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dY)))
         return guard(st);
     if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess)
         return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: hipStreamCreate failed"));

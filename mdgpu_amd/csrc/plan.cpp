@@ -194,26 +194,17 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             int maxc = 0;
             for (int f : big) maxc = std::max(maxc, P.fronts[f].c);
             for (int j0 = 0; j0 < maxc; j0 += BW) {
-                SolveStep tri{SV_FWD_TRI, (int)P.sv_items.size(), 0, 0};
-                for (int f : big) {
-                    const FrontDesc& d = P.fronts[f];
-                    if (j0 >= d.c) continue;
-                    const int b = std::min(BW, d.c - j0);
-                    P.sv_items.push_back({f, 0, 0, j0});
-                    tri.bytes += 0.5 * b * b * o.elem_bytes;
-                }
-                tri.count = (int)P.sv_items.size() - tri.first;
-                P.sv_fwd.push_back(tri);
-                SolveStep gv{SV_FWD_GEMV, (int)P.sv_items.size(), 0, 0};
+                SolveStep st{SV_FWD_BLK, (int)P.sv_items.size(), 0, 0};
                 for (int f : big) {
                     const FrontDesc& d = P.fronts[f];
                     if (j0 >= d.c) continue;
                     const int j1 = std::min(j0 + BW, d.c);
-                    for (int i0 = j1; i0 < d.n; i0 += RC) P.sv_items.push_back({f, i0, 0, j0});
-                    gv.bytes += (double)(d.n - j1) * (j1 - j0) * o.elem_bytes + 16.0 * (d.n - j1);
+                    P.sv_items.push_back({f, j1, 1, j0, 0, 0, 0, 0});
+                    for (int a = j1 + RC; a < d.n; a += RC) P.sv_items.push_back({f, a, 0, j0, 0, 0, 0, 0});
+                    st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 16.0 * (d.n - j0);
                 }
-                gv.count = (int)P.sv_items.size() - gv.first;
-                if (gv.count > 0) P.sv_fwd.push_back(gv);
+                st.count = (int)P.sv_items.size() - st.first;
+                P.sv_fwd.push_back(st);
             }
         }
         for (int L = P.nlevels - 1; L >= 0; L--) {
@@ -231,31 +222,23 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             const int nblk = (maxc + BW - 1) / BW;
             for (int blk = nblk - 1; blk >= 0; blk--) {
                 const int j0 = blk * BW;
-                SolveStep dot{SV_BWD_DOT, (int)P.sv_items.size(), 0, 0};
-                std::vector<std::pair<int, int>> slots;   // per front: first slot, count
-                int slot = 0;
+                SolveStep st{SV_BWD_BLK, (int)P.sv_items.size(), 0, 0};
+                int slot = 0, group = 0;
                 for (int f : big) {
                     const FrontDesc& d = P.fronts[f];
-                    if (j0 >= d.c) { slots.push_back({0, 0}); continue; }
-                    const int j1 = std::min(j0 + BW, d.c);
-                    const int first = slot;
-                    for (int i0 = j1; i0 < d.n; i0 += RC) P.sv_items.push_back({f, i0, slot++, j0});
-                    slots.push_back({first, slot - first});
-                    dot.bytes += (double)(d.n - j1) * (j1 - j0) * o.elem_bytes + 12.0 * (d.n - j1);
-                }
-                dot.count = (int)P.sv_items.size() - dot.first;
-                if (dot.count > 0) P.sv_bwd.push_back(dot);
-                P.sv_partial = std::max<int64_t>(P.sv_partial, (int64_t)slot * BW);
-                SolveStep tri{SV_BWD_TRI, (int)P.sv_items.size(), 0, 0};
-                for (size_t t = 0; t < big.size(); t++) {
-                    const FrontDesc& d = P.fronts[big[t]];
                     if (j0 >= d.c) continue;
-                    const int b = std::min(BW, d.c - j0);
-                    P.sv_items.push_back({big[t], slots[t].first, slots[t].second, j0});
-                    tri.bytes += 0.5 * b * b * o.elem_bytes;
+                    const int j1 = std::min(j0 + BW, d.c);
+                    const int nch = std::max(1, (d.n - j1 + RC - 1) / RC);
+                    for (int t = 0; t < nch; t++)
+                        P.sv_items.push_back({f, j1 + t * RC, 0, j0, slot + t, group, slot, nch});
+                    slot += nch;
+                    group++;
+                    st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 12.0 * (d.n - j0);
                 }
-                tri.count = (int)P.sv_items.size() - tri.first;
-                P.sv_bwd.push_back(tri);
+                st.count = (int)P.sv_items.size() - st.first;
+                P.sv_bwd.push_back(st);
+                P.sv_partial = std::max<int64_t>(P.sv_partial, (int64_t)slot * BW);
+                P.sv_groups = std::max(P.sv_groups, group);
             }
         }
     }
@@ -595,12 +578,14 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 const FrontDesc& d = P.fronts[it.front];
                 if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW) return "solve item block";
                 const int j1 = std::min(it.j0 + kSolveBW, d.c);
-                if ((st.kind == SV_FWD_GEMV || st.kind == SV_BWD_DOT) && (it.a < j1 || it.a >= d.n))
-                    return "solve item rows";
-                if (st.kind == SV_BWD_DOT && (it.b < 0 || (int64_t)(it.b + 1) * kSolveBW > P.sv_partial))
-                    return "solve partial slot";
-                if (st.kind == SV_BWD_TRI && (it.a < 0 || (int64_t)(it.a + it.b) * kSolveBW > P.sv_partial))
-                    return "solve partial range";
+                if (it.a < j1 || (it.a >= d.n && it.a != j1)) return "solve item rows";
+                if (st.kind == SV_BWD_BLK) {
+                    if (it.slot < 0 || (int64_t)(it.slot + 1) * kSolveBW > P.sv_partial) return "solve partial slot";
+                    if (it.gfirst < 0 || it.gcount < 1 || (int64_t)(it.gfirst + it.gcount) * kSolveBW > P.sv_partial)
+                        return "solve partial group";
+                    if (it.slot < it.gfirst || it.slot >= it.gfirst + it.gcount) return "solve slot outside group";
+                    if (it.group < 0 || it.group >= P.sv_groups) return "solve counter";
+                }
             }
         }
         return "";

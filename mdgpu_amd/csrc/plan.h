@@ -105,18 +105,24 @@ struct Step {
 };
 
 // Solve schedule.  Fronts with at most kSolveBW pivots are solved by one
-// workgroup each (FWD_SMALL / BWD_SMALL); larger fronts go 256 columns at a
-// time: a one-workgroup triangle solve (TRI) and a grid-wide update of the
-// rows below (FWD_GEMV) or dot products against them (BWD_DOT, partial sums
-// per 256-row chunk, reduced in a fixed order by BWD_TRI).
-constexpr int kSolveBW = 256;
-enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_TRI, SV_FWD_GEMV, SV_BWD_SMALL, SV_BWD_DOT, SV_BWD_TRI };
+// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts go one 64-column
+// diagonal block per launch: FWD_BLK workgroups each take 64 rows below
+// the block (and recompute y = inv(L_D) v for it); BWD_BLK workgroups form
+// partial dot products over 64 rows each, and the last workgroup of a
+// block to finish (atomic counter, no waiting) reduces them in slot order
+// and applies inv(L_D)^T.  The forward result goes to a separate vector Y.
+constexpr int kSolveBW = 64;
+enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_BLK };
 
 struct SolveItem {
     int front;
-    int a;      // GEMV/DOT: first row (front-local); BWD_TRI: first partial slot
-    int b;      // DOT: partial slot; BWD_TRI: number of slots
-    int j0;     // first pivot column of the 256-column block (front-local)
+    int a;        // first row (front-local) of this workgroup's 64-row chunk
+    int first;    // FWD_BLK: 1 if this item writes the block's y
+    int j0;       // first pivot column of the 64-column block (front-local)
+    int slot;     // BWD_BLK: partial-sum slot of this item
+    int group;    // BWD_BLK: counter index of the (front, block) group
+    int gfirst;   // BWD_BLK: first slot of the group
+    int gcount;   // BWD_BLK: items (slots) in the group
 };
 
 struct SolveStep {
@@ -140,7 +146,8 @@ struct Plan {
     std::vector<int> sv_fronts;        // front lists of SMALL / GATHER steps
     std::vector<SolveItem> sv_items;
     std::vector<SolveStep> sv_fwd, sv_bwd;
-    int64_t sv_partial = 1;            // doubles of BWD_DOT partial sums
+    int64_t sv_partial = 1;            // doubles of BWD_BLK partial sums
+    int sv_groups = 1;                 // counters needed by one BWD_BLK launch
     std::vector<EaItem> ea;
     std::vector<GemmJob> gemm;
     std::vector<DiagJob> diag;

@@ -664,11 +664,12 @@ __global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restr
     }
 }
 
-// Forward, one 64-column diagonal block of a large front per launch: every
-// workgroup recomputes y = inv(L_D) v for the block (v is final: earlier
-// blocks were applied by earlier launches), the `first` one stores y in Y,
-// and each updates its 64 rows [a, a+64) below the block; the four
-// wavefronts split the block's columns.
+// Forward, one 256-column block of a large front per launch.  Every
+// workgroup solves the block's own triangle (four 64-column sub-blocks:
+// y_s = inv(L_s) v_s with the stored inverse, then the later sub-blocks'
+// right-hand sides are updated), the `first` one stores y in Y, and each
+// updates its 256 rows [a, a+256) below the block.  v is final on entry:
+// earlier blocks were applied by earlier launches.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
@@ -677,54 +678,58 @@ __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __re
                                                            double* __restrict__ su,
                                                            const int64_t* __restrict__ suofs) {
     __shared__ double W[64 * 65];
-    __shared__ double vb[64];
-    __shared__ double yb[64];
-    __shared__ double red[kThreads / 64][64];
+    __shared__ double vb[kSolveBW];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
-    const int j0 = it.j0, b = min(64, d.c - j0);
-    const T* __restrict__ w = inv + d.dinv + (int64_t)(j0 / 64) * 4096;
-    for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
-        const int l = t & 63, r = t >> 6;
-        if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
-    }
-    if (threadIdx.x < b) vb[threadIdx.x] = xp[d.k + j0 + threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x < b) {
-        const int r = threadIdx.x;
-        double y = 0.0;
-#pragma unroll 8
-        for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
-        yb[r] = y;
-        if (it.first) Y[d.k + j0 + r] = y;
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = it.a + lane;
-    double acc = 0.0;
-    if (i < d.n) {
-        const T* __restrict__ P = L + d.lofs + i + (int64_t)j0 * d.n;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const int jj = wave * 16 + t;
-            if (jj < b) acc += to_d(P[(int64_t)jj * d.n]) * yb[jj];
+    const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
+    const T* __restrict__ P = L + d.lofs;
+    const int tid = threadIdx.x;
+    if (tid < bw) vb[tid] = xp[d.k + j0 + tid];
+    for (int s0 = 0; s0 < bw; s0 += 64) {
+        const int b = min(64, bw - s0);
+        const T* __restrict__ w = inv + d.dinv + (int64_t)((j0 + s0) / 64) * 4096;
+        for (int t = tid; t < 64 * b; t += kThreads) {
+            const int l = t & 63, r = t >> 6;
+            if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
         }
+        __syncthreads();
+        double y = 0.0;
+        if (tid < b) {
+#pragma unroll 8
+            for (int l = 0; l <= tid; ++l) y += W[l + tid * 65] * vb[s0 + l];
+        }
+        __syncthreads();
+        if (tid < b) vb[s0 + tid] = y;
+        __syncthreads();
+        const int row = s0 + b + tid;            // later rows of this block
+        if (row < bw) {
+            const T* __restrict__ pr = P + (j0 + row) + (int64_t)(j0 + s0) * n;
+            double acc = 0.0;
+#pragma unroll 16
+            for (int l = 0; l < b; ++l) acc += to_d(pr[(int64_t)l * n]) * vb[s0 + l];
+            vb[row] -= acc;
+        }
+        __syncthreads();
     }
-    red[wave][lane] = acc;
-    __syncthreads();
-    if (threadIdx.x < 64 && i < d.n) {
-        const double tot = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
-        if (i < d.c) xp[d.k + i] -= tot;
-        else su[suofs[it.front] + i - d.c] -= tot;
+    if (it.first && tid < bw) Y[d.k + j0 + tid] = vb[tid];
+    const int i = it.a + tid;
+    if (i < n) {
+        const T* __restrict__ pr = P + i + (int64_t)j0 * n;
+        double acc = 0.0;
+#pragma unroll 16
+        for (int j = 0; j < bw; ++j) acc += to_d(pr[(int64_t)j * n]) * vb[j];
+        if (i < d.c) xp[d.k + i] -= acc;
+        else su[suofs[it.front] + i - d.c] -= acc;
     }
 }
 
-// Backward, one 64-column diagonal block of a large front per launch: each
+// Backward, one 256-column block of a large front per launch.  Each
 // workgroup forms partial dot products of the block's columns with x over
-// its 64 rows below the block (tile staged through LDS); the last workgroup
-// of the group to arrive (atomic counter; none waits) sums the partials in
-// slot order, applies inv(L_D)^T to Y(block) - sum, stores x, and resets
-// the counter for the next launch.
+// its 256 rows below the block (64x64 tiles staged through LDS); the last
+// workgroup of the group to arrive (atomic counter; none waits) reduces the
+// partials in slot order, solves the block's own triangle from the last
+// 64-column sub-block (dots with the block rows already solved, then
+// inv(L_s)^T), stores x and resets the counter for the next launch.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
@@ -735,45 +740,79 @@ __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __re
                                                            int* __restrict__ counters) {
     __shared__ T tile[64 * 65];
     __shared__ double xq[64];
-    __shared__ double red[4][64];
-    __shared__ double vb[64];
+    __shared__ double red[4][kSolveBW];
+    __shared__ double vb[kSolveBW];
     __shared__ int last;
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
-    const int j0 = it.j0, b = min(64, d.c - j0);
-    const int rows = max(0, min(64, d.n - it.a));
-    const int r = threadIdx.x & 63, cq = threadIdx.x >> 6;
-    if (threadIdx.x < 64) xq[r] = r < rows ? xp[qall[d.qofs + it.a + r]] : 0.0;
-    const T* __restrict__ P = L + d.lofs + it.a + (int64_t)j0 * d.n;
+    const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
+    const T* __restrict__ P = L + d.lofs;
+    const int tid = threadIdx.x, r = tid & 63, cq = tid >> 6;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r0 = it.a; r0 < min(it.a + kSolveBW, n); r0 += 64) {
+        const int rows = min(64, n - r0);
+        __syncthreads();
+        if (tid < 64) xq[r] = r < rows ? xp[qall[d.qofs + r0 + r]] : 0.0;
+        for (int c4 = 0; c4 < 4; ++c4) {
+            const int cbase = c4 * 64;
+            if (cbase >= bw) break;
+            __syncthreads();
 #pragma unroll
-    for (int cc = cq; cc < 64; cc += 4) tile[cc * 65 + r] = (r < rows && cc < b) ? P[r + (int64_t)cc * d.n] : T(0);
-    __syncthreads();
-    double sdot = 0.0;
+            for (int cc = cq; cc < 64; cc += 4)
+                tile[cc * 65 + r] = (r < rows && cbase + cc < bw) ? P[(r0 + r) + (int64_t)(j0 + cbase + cc) * n] : T(0);
+            __syncthreads();
+            double sdot = 0.0;
 #pragma unroll
-    for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) sdot += to_d(tile[r * 65 + rr]) * xq[rr];
-    red[cq][r] = sdot;
+            for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) sdot += to_d(tile[r * 65 + rr]) * xq[rr];
+            acc[c4] += sdot;
+        }
+    }
     __syncthreads();
-    if (threadIdx.x < 64) partial[(int64_t)it.slot * 64 + r] = red[0][r] + red[1][r] + red[2][r] + red[3][r];
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) red[cq][c4 * 64 + r] = acc[c4];
+    __syncthreads();
+    if (tid < bw) partial[(int64_t)it.slot * kSolveBW + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&counters[it.group], 1) == it.gcount - 1;
+    if (tid == 0) last = atomicAdd(&counters[it.group], 1) == it.gcount - 1;
     __syncthreads();
     if (!last) return;
     __threadfence();
-    if (threadIdx.x < b) {
-        double v = Y[d.k + j0 + r];
-        for (int sl = 0; sl < it.gcount; ++sl) v -= partial[(int64_t)(it.gfirst + sl) * 64 + r];
-        vb[r] = v;
+    if (tid < bw) {
+        double v = Y[d.k + j0 + tid];
+#pragma unroll 8
+        for (int sl = 0; sl < it.gcount; ++sl) v -= partial[(int64_t)(it.gfirst + sl) * kSolveBW + tid];
+        vb[tid] = v;
     }
     __syncthreads();
-    if (threadIdx.x < b) {
-        const T* __restrict__ w = inv + d.dinv + (int64_t)(j0 / 64) * 4096;
+    const int nsub = (bw + 63) / 64;
+    for (int sb = nsub - 1; sb >= 0; --sb) {
+        const int s0 = sb * 64, s1 = min(s0 + 64, bw), b = s1 - s0;
+        // v_j -= sum over solved block rows [s1, bw) of L(row, j) x_row, 4 threads per column
+        double part = 0.0;
+        if (r < b) {
+            const T* __restrict__ col = P + (int64_t)(j0 + s0 + r) * n + j0;
+#pragma unroll 4
+            for (int row = s1 + cq; row < bw; row += 4) part += to_d(col[row]) * vb[row];
+        }
+        red[cq][r] = part;
+        __syncthreads();
+        if (tid < b) vb[s0 + tid] -= red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        __syncthreads();
         double x = 0.0;
+        if (tid < b) {
+            const T* __restrict__ w = inv + d.dinv + (int64_t)((j0 + s0) / 64) * 4096;
 #pragma unroll 8
-        for (int rr = r; rr < b; ++rr) x += to_d(w[r + 64 * rr]) * vb[rr];
-        xp[d.k + j0 + r] = x;
+            for (int rr = tid; rr < b; ++rr) x += to_d(w[tid + 64 * rr]) * vb[s0 + rr];
+        }
+        __syncthreads();
+        if (tid < b) {
+            vb[s0 + tid] = x;
+            xp[d.k + j0 + s0 + tid] = x;
+        }
+        __syncthreads();
     }
-    if (threadIdx.x == 0) counters[it.group] = 0;
+    if (tid == 0) counters[it.group] = 0;
 }
 
 // -------------------------------------------------------------- refinement

@@ -178,10 +178,10 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
 
     // solve schedule (see SolveStep)
     {
-        const int BW = kSolveBW, RC = 64;
+        const int BW = kSolveBW, RC = kSolveBW;
         for (int L = 0; L < P.nlevels; L++) {
             std::vector<int> small, big;
-            for (int f : P.level_fronts[L]) (P.fronts[f].c > BW ? big : small).push_back(f);
+            for (int f : P.level_fronts[L]) (P.fronts[f].c > kSolveSmall ? big : small).push_back(f);
             if (!small.empty()) {
                 SolveStep st{SV_FWD_SMALL, (int)P.sv_fronts.size(), (int)small.size(), 0};
                 for (int f : small) st.bytes += (double)P.fronts[f].n * P.fronts[f].c * o.elem_bytes;
@@ -209,7 +209,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
         for (int L = P.nlevels - 1; L >= 0; L--) {
             std::vector<int> small, big;
-            for (int f : P.level_fronts[L]) (P.fronts[f].c > BW ? big : small).push_back(f);
+            for (int f : P.level_fronts[L]) (P.fronts[f].c > kSolveSmall ? big : small).push_back(f);
             if (!small.empty()) {
                 SolveStep st{SV_BWD_SMALL, (int)P.sv_fronts.size(), (int)small.size(), 0};
                 for (int f : small) st.bytes += (double)P.fronts[f].n * P.fronts[f].c * o.elem_bytes;
@@ -576,7 +576,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 const SolveItem& it = P.sv_items[t];
                 if (it.front < 0 || it.front >= nf) return "solve item front";
                 const FrontDesc& d = P.fronts[it.front];
-                if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW) return "solve item block";
+                if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW || it.j0 % 64) return "solve item block";
                 const int j1 = std::min(it.j0 + kSolveBW, d.c);
                 if (it.a < j1 || (it.a >= d.n && it.a != j1)) return "solve item rows";
                 if (st.kind == SV_BWD_BLK) {

@@ -104,14 +104,15 @@ struct Step {
     double bytes = 0;      // algorithmic HBM bytes of the launch
 };
 
-// Solve schedule.  Fronts with at most kSolveBW pivots are solved by one
-// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts go one 64-column
-// diagonal block per launch: FWD_BLK workgroups each take 64 rows below
-// the block (and recompute y = inv(L_D) v for it); BWD_BLK workgroups form
-// partial dot products over 64 rows each, and the last workgroup of a
-// block to finish (atomic counter, no waiting) reduces them in slot order
-// and applies inv(L_D)^T.  The forward result goes to a separate vector Y.
-constexpr int kSolveBW = 64;
+// Solve schedule.  Fronts with at most kSolveSmall pivots are solved by one
+// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts go one 256-column
+// block per launch: FWD_BLK workgroups each take 256 rows below the block
+// (and recompute the block's own solve); BWD_BLK workgroups form partial
+// dot products over 256 rows each, and the last workgroup of a block to
+// finish (atomic counter, no waiting) reduces them in slot order and
+// solves the block.  The forward result goes to a separate vector Y.
+constexpr int kSolveBW = 256;
+constexpr int kSolveSmall = 64;
 enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_BLK };
 
 struct SolveItem {

@@ -34,7 +34,9 @@ extern "C" {
 /* Precision of the stored factor and of the dense front kernels. */
 #define MDGPU_FP64 0
 #define MDGPU_FP32 1
-#define MDGPU_BF16 2   /* bf16 MFMA Schur updates, power-of-two front scaling */
+#define MDGPU_BF16 2   /* FP32 factor, Schur-update GEMMs on bf16 MFMA (one product) */
+#define MDGPU_BF16X3 3 /* FP32 factor, Schur-update GEMMs as split bf16 products
+                          hi*hi + hi*lo + lo*hi on bf16 MFMA (~16-bit mantissa) */
 
 /* Node partition: -sn / -rn c / plain flags of the reference driver
  * (mdgpu.cu:1708-1714). */
@@ -116,8 +118,10 @@ int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_t* dst, int
 /* Allocate the device arena for a numeric factorization of `sym` (L store,
  * update-matrix arena, maps and schedule uploaded once).  Replaces the
  * per-node device allocation and map uploads of the reference
- * (AllocFrenteGPU / MoverFrenteQlAGPU, mdgpu.cu:597-723).  precision is
- * MDGPU_FP64 or MDGPU_FP32 (storage and arithmetic of the dense fronts). */
+ * (AllocFrenteGPU / MoverFrenteQlAGPU, mdgpu.cu:597-723).  precision:
+ * MDGPU_FP64 or MDGPU_FP32 (storage and arithmetic of the dense fronts), or
+ * MDGPU_BF16 / MDGPU_BF16X3 (FP32 storage; the GEMM/SYRK updates on bf16
+ * MFMA with FP32 accumulation; use mdgpu_refine for an FP64-accurate x). */
 int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, mdgpu_numeric** num);
 int mdgpu_numeric_destroy(mdgpu_numeric* num);
 
@@ -156,7 +160,8 @@ int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream);
  * `stream` around every launch.  out[4*k + {0,1,2,3}] = milliseconds,
  * launches, algorithmic flops, algorithmic HBM bytes of kernel class k:
  * 0 assembly (zero fill + A scatter), 1 extend-add, 2 LDS panel
- * factorization, 3 diagonal blocks, 4 TRSM, 5 tiled GEMM/SYRK.  The event
+ * factorization, 3 blocked diagonal potrf + TRSM, 4 diagonal-block
+ * inverses (used by the solves), 5 tiled GEMM/SYRK.  The event
  * gaps add launch latency, so use mdgpu_factorize for end-to-end times. */
 #define MDGPU_KIND_N 6
 int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream,

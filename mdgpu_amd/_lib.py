@@ -19,7 +19,7 @@ MDGPU_ERR_DEVICE = 3
 MDGPU_ERR_NOMEM = 4
 MDGPU_ERR_NOT_CONVERGED = 5
 
-FP64, FP32, BF16 = 0, 1, 2
+FP64, FP32, BF16, BF16X3 = 0, 1, 2, 3
 PART_PLAIN, PART_SUPERNODAL, PART_RELAXED = 0, 1, 2
 
 SYM_N, SYM_NNODES, SYM_LNZ, SYM_NNZ_A, SYM_QLEN, SYM_MAXFRONT, SYM_NLEVELS, SYM_NODES_FLAT = range(8)
@@ -91,7 +91,7 @@ def check(status):
 
 NUM_DEVICE_BYTES, NUM_LSTORE, NUM_ARENA, NUM_STEPS, NUM_LEVELS, NUM_PANEL_FRONTS, NUM_DIAG_JOBS, NUM_PRECISION = range(8)
 NUM_NINFO = 8
-KIND_NAMES = ["assembly", "extend_add", "panel", "diag", "trsm", "gemm"]
+KIND_NAMES = ["assembly", "extend_add", "panel", "potrf_trsm", "inverse", "gemm"]
 
 _SIGNATURES.update({
     "mdgpu_numeric_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),

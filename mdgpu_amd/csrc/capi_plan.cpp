@@ -15,6 +15,12 @@ PlanOptions plan_options_for(int precision) {
     // GEMM tile: 128 selects the MFMA kernel, 64 the FMA kernel
     if (const char* e = std::getenv("MDGPU_GEMM_TILE")) po.tile = std::atoi(e) == 64 ? 64 : 128;
     else po.tile = 128;
+    if (precision == MDGPU_BF16 || precision == MDGPU_BF16X3) po.tile = 128;   // bf16 kernels: 128-row tiles only
+    // outer block width of the blocked panel path (multiple of 64)
+    if (const char* e = std::getenv("MDGPU_NB_OUTER")) {
+        const int v = std::atoi(e);
+        if (v >= 64 && v % 64 == 0) po.nb_outer = v;
+    }
     return po;
 }
 }  // namespace mdgpu
@@ -22,8 +28,8 @@ PlanOptions plan_options_for(int precision) {
 extern "C" int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_t* out) {
     mdgpu::clear_error();
     if (!sym || !out) return fail(MDGPU_ERR_ARG, "mdgpu_plan_check: null pointer");
-    if (precision != MDGPU_FP64 && precision != MDGPU_FP32)
-        return fail(MDGPU_ERR_ARG, "mdgpu_plan_check: precision must be MDGPU_FP64 or MDGPU_FP32");
+    if (precision < MDGPU_FP64 || precision > MDGPU_BF16X3)
+        return fail(MDGPU_ERR_ARG, "mdgpu_plan_check: precision must be MDGPU_FP64, MDGPU_FP32, MDGPU_BF16 or MDGPU_BF16X3");
     mdgpu::PlanOptions po = mdgpu::plan_options_for(precision);
     mdgpu::Plan P;
     std::string err = mdgpu::build_plan(sym->S, po, P);
@@ -39,7 +45,7 @@ extern "C" int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_
     out[MDGPU_NUM_STEPS] = (int64_t)P.steps.size();
     out[MDGPU_NUM_LEVELS] = P.nlevels;
     out[MDGPU_NUM_PANEL_FRONTS] = (int64_t)P.panel.size();
-    out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.diag.size();
+    out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.blk.size();
     out[MDGPU_NUM_PRECISION] = precision;
     return MDGPU_OK;
 }

@@ -18,15 +18,18 @@ void launch_extend_add(const EaItem* items, int count, const FrontDesc* fronts, 
 template <typename T>
 void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,
                   hipStream_t s);
+// fused diagonal-block Cholesky + TRSM of the rows below; counters: one
+// zero-initialised int per job (returned to zero by the kernel)
 template <typename T>
-void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s);
+void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, hipStream_t s);
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
-template <typename T>
-void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, hipStream_t s);
 // tile 128: MFMA kernel; tile 64: FMA kernel (the plan's tile size decides)
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s);
+// same jobs (128-row tiles), FP32 data, bf16 MFMA with FP32 accumulation;
+// nsplit 1: one bf16 product, 3: split hi/lo products (hi*hi + hi*lo + lo*hi)
+void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s);
 
 // solves (compute in FP64, factor stored as T); the forward result goes to
 // Y, the backward result to xp

@@ -156,25 +156,66 @@ __device__ inline void stage_rows(const T (&a)[64], T* Ls, T* rd, int lane) {
     __syncthreads();
 }
 
-// Diagonal block b x b of the blocked path: Cholesky in four 16-column
-// panels -- each panel factored by wavefront 0 with its rows in registers
-// (pivot and column entries broadcast by v_readlane within the panel), the
-// trailing part of the block updated by all four wavefronts from LDS.
-// Rows and columns beyond b are identity padding.  (The block's inverse,
-// needed only by the solves, is formed later by inv_kernel.)
+// Columns [J0, J0+16) of the row solve x L^T = r held in registers
+// (x[0..63], one matrix row per lane), right-looking: x_j /= L(j,j), then
+// x_k -= x_j L(k,j) for k > j, with Lc(k, j) = Lc[64 j + k] read from LDS at
+// uniform addresses.
+template <int J0, typename T>
+__device__ __forceinline__ void trsm_row16(T (&x)[64], const T* Lc, const T* rd) {
+#pragma unroll
+    for (int j = J0; j < J0 + 16; ++j) {
+        x[j] *= rd[j];
+#pragma unroll
+        for (int k = j + 1; k < 64; ++k) x[k] -= x[j] * Lc[64 * j + k];
+    }
+}
+
+// Block column of the blocked path: Cholesky of the b x b diagonal block
+// (the potrf of FactorAux_GPU_2, mdgpu.cu:1010) and the triangular solve of
+// the rows below it (the trsm of :1047), one launch for all fronts of a
+// level.  Each workgroup owns kBlkRows rows below the block, one row per
+// lane, kept in registers; it factors the diagonal block itself (four
+// 16-column panels: each panel by wavefront 0 with its rows in registers
+// and column entries broadcast by v_readlane, the trailing part of the block
+// by all wavefronts from LDS), then solves x L_D^T = r for its rows with
+// L_D in LDS (uniform broadcast reads).  The workgroups of a job count their
+// arrival after reading the block; the last one writes the factored block
+// back, so no workgroup can read a half-written block.  Rows and columns
+// beyond b are identity padding.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restrict__ jobs, T* __restrict__ L,
-                                                        T* __restrict__ inv, int* __restrict__ info) {
-    __shared__ T As[64 * 65];   // column-major, As[r + 65 c] = A(r, c)
-    const DiagJob jb = jobs[blockIdx.x];
+__global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
+                                                       T* __restrict__ L, int* __restrict__ info,
+                                                       int* __restrict__ counters) {
+    __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]; later Lc
+    __shared__ T rd[64];
+    __shared__ int last;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+    }
+    const BlkJob jb = jobs[lo];
     const int b = jb.b, ld = jb.ld;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     T* __restrict__ g = L + jb.p_off;
-    for (int t = tid; t < 64 * 64; t += kThreads) {
-        const int r = t & 63, c = t >> 6;
-        As[r + 65 * c] = (r < b && c < b) ? g[r + (int64_t)c * ld] : (r == c ? T(1) : T(0));
+    {
+        // 16 elements per thread; loads issued together (see below)
+        T v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+            v[q] = g[min(r, b - 1) + (int64_t)min(c, b - 1) * ld];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) asm volatile("" : "+v"(v[q]));
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+            As[r + 65 * c] = (r < b && c < b) ? v[q] : (r == c ? T(1) : T(0));
+        }
     }
     __syncthreads();
+    if (tid == 0) last = atomicAdd(&counters[jb.cnt], 1) == jb.ntiles - 1;
     int bad = 64;
     for (int p = 0; p < 4; ++p) {
         const int c0 = 16 * p;
@@ -186,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restric
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const T djj = bcast(a[j], j);
-                if (!(djj > T(0)) && bad == 64) bad = c0 + j;
+                bad = (!(djj > T(0)) && bad == 64) ? c0 + j : bad;
                 const T sq = sqrt(djj);
                 const T is = T(1) / sq;
                 a[j] = (lane == j) ? sq : a[j] * is;
@@ -211,10 +252,60 @@ __global__ __launch_bounds__(kThreads) void diag_kernel(const DiagJob* __restric
         }
         __syncthreads();
     }
-    if (wave == 0 && lane == 0 && bad < b) atomicMin(info, jb.col0 + bad);
-    for (int t = tid; t < 64 * 64; t += kThreads) {
-        const int r = t & 63, c = t >> 6;
-        if (r < b && c < b && c <= r) g[r + (int64_t)c * ld] = As[r + 65 * c];
+    // repack L_D column-major with stride 64 (Lc(k, j) = Lc[64 j + k]) for
+    // aligned broadcast reads; zero above the diagonal
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+        v[q] = r >= c ? As[r + 65 * c] : T(0);
+    }
+    __syncthreads();
+    T* Lc = As;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+        Lc[64 * c + r] = v[q];
+        if (r == c) rd[r] = T(1) / v[q];
+    }
+    __syncthreads();
+    if (last) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+            if (r < b && c < b && r >= c) g[r + (int64_t)c * ld] = v[q];
+        }
+        if (tid == 0) {
+            if (bad < b) atomicMin(info, jb.col0 + bad);
+            atomicExch(&counters[jb.cnt], 0);
+        }
+    }
+    const int row = ((int)blockIdx.x - jb.tile_base) * kBlkRows + tid;
+    const bool has = row < jb.rows;
+    T* __restrict__ gr = g + b + row;          // element (b + row, 0) of the block column
+    // the 64 row loads are issued together: the empty asm keeps the compiler
+    // from sinking each load into its own branch and wait
+    T x[64];
+    const T* __restrict__ src = has ? gr : g;    // in-range address for every lane
+#pragma unroll
+    for (int c = 0; c < 64; ++c) x[c] = src[(int64_t)min(c, b - 1) * ld];
+#pragma unroll
+    for (int c = 0; c < 64; ++c) asm volatile("" : "+v"(x[c]));
+#pragma unroll
+    for (int c = 0; c < 64; ++c) x[c] = (has && c < b) ? x[c] : T(0);
+    // x <- x L_D^{-T}: right-looking over the 64 columns, in registers
+    trsm_row16<0>(x, Lc, rd);
+    trsm_row16<16>(x, Lc, rd);
+    trsm_row16<32>(x, Lc, rd);
+    trsm_row16<48>(x, Lc, rd);
+    // keep the solve out of the conditional stores below (sinking it there
+    // splits it into 64 basic blocks and spills)
+#pragma unroll
+    for (int c = 0; c < 64; ++c) asm volatile("" : "+v"(x[c]));
+    if (has) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+            if (c < b) gr[(int64_t)c * ld] = x[c];
     }
 }
 
@@ -238,72 +329,6 @@ __global__ __launch_bounds__(64) void inv_kernel(const InvJob* __restrict__ jobs
     T* __restrict__ w = inv + d.dinv + (int64_t)(jb.j0 / 64) * 4096;
     if (lane < b)
         for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
-}
-
-__device__ inline int find_job_trsm(const TrsmJob* jobs, int njobs, int bid) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-// R <- R * L_D^{-T} for a 64-row tile of the rows below a diagonal block
-// (the trsm of FactorAux_CPU_3, mdgpu.cu:1330): each of 64 threads solves
-// x L_D^T = r for its row by forward substitution; L_D (read from the
-// panel) and the rows sit in LDS, L_D entries are uniform broadcasts.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void trsm_kernel(const TrsmJob* __restrict__ jobs, int njobs,
-                                                        T* __restrict__ L) {
-    __shared__ T R[64 * 65];    // R[i * 65 + l]: row i of the tile
-    __shared__ T Ld[64 * 65];   // Ld[j * 65 + l] = L_D(j, l)
-    __shared__ T rd[64];
-    const TrsmJob jb = jobs[find_job_trsm(jobs, njobs, blockIdx.x)];
-    const int tile = blockIdx.x - jb.tile_base;
-    const int r0 = tile * 64;
-    const int rows = min(64, jb.rows - r0);
-    const int b = jb.b, ld = jb.ld;
-    T* __restrict__ g = L + jb.p_off + r0;
-    const T* __restrict__ dg = L + jb.p_off - b;     // the diagonal block (b rows above)
-    for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
-        const int i = t & 63, l = t >> 6;
-        R[i * 65 + l] = i < rows ? g[i + (int64_t)l * ld] : T(0);
-        if (i < b) Ld[i * 65 + l] = l <= i ? dg[i + (int64_t)l * ld] : T(0);
-    }
-    __syncthreads();
-    if (threadIdx.x < b) rd[threadIdx.x] = T(1) / Ld[threadIdx.x * 66];
-    __syncthreads();
-    // blocked substitution: 16 columns at a time on the 64 row threads, then
-    // all threads apply those columns to the remaining ones
-    for (int c0 = 0; c0 < b; c0 += 16) {
-        const int cw = min(16, b - c0);
-        if (threadIdx.x < 64) {
-            T* row = R + threadIdx.x * 65;
-            for (int j = c0; j < c0 + cw; ++j) {
-                T acc = row[j];
-                const T* lj = Ld + j * 65;
-                for (int l = c0; l < j; ++l) acc -= row[l] * lj[l];
-                row[j] = acc * rd[j];
-            }
-        }
-        __syncthreads();
-        const int rest = b - c0 - cw;
-        for (int idx = threadIdx.x; idx < 64 * rest; idx += kThreads) {
-            const int i = idx & 63, j = c0 + cw + (idx >> 6);
-            T acc = T(0);
-            const T* ri = R + i * 65 + c0;
-            const T* lj = Ld + j * 65 + c0;
-#pragma unroll 16
-            for (int t = 0; t < cw; ++t) acc += ri[t] * lj[t];
-            R[i * 65 + j] -= acc;
-        }
-        __syncthreads();
-    }
-    for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
-        const int i = t & 63, l = t >> 6;
-        if (i < rows) g[i + (int64_t)l * ld] = R[i * 65 + l];
-    }
 }
 
 // C -= A * B^T on 64x64 tiles (beta0: C = -A * B^T), optionally keeping only
@@ -408,6 +433,16 @@ template <> struct MfmaOp<double> {
     __device__ static int row(int lg, int v) { return lg + 4 * v; }
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs
+// on XCD i % 8), each with its own L2.  Renumber them so that each XCD gets
+// one contiguous run of tiles: neighbouring tiles share A or B panels, which
+// then hit the same L2.  A bijection on [0, n).
+__device__ inline int xcd_tile(int bid, int n) {
+    constexpr int kXcd = 8;
+    const int x = bid % kXcd, k = bid / kXcd, q = n / kXcd, r = n % kXcd;
+    return x * q + min(x, r) + k;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                              T* __restrict__ L, T* __restrict__ U) {
@@ -420,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     __shared__ __align__(16) T As[BT * LDK];
     __shared__ __align__(16) T Bs[BT * LDK];
     int lo = 0, hi = njobs - 1;
-    const int bid = blockIdx.x;
+    const int bid = xcd_tile(blockIdx.x, gridDim.x);
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
@@ -512,6 +547,134 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
                 if (i >= g.M || (g.lower && i < j)) continue;
                 T* cp = C + i + (int64_t)j * g.ldc;
                 *cp = g.beta0 ? -acc[rb2][cb][v] : *cp - acc[rb2][cb][v];
+            }
+        }
+}
+
+// BF16 variants of gemm_mfma_kernel: the factor stays FP32 in HBM, the A
+// and B tiles are rounded to bf16 (v_cvt_pk_bf16_f32, round to nearest
+// even) when staged into LDS, and v_mfma_f32_16x16x32_bf16 accumulates in
+// FP32.  NSPLIT = 1: one bf16 product.  NSPLIT = 3: each value is split as
+// hi = bf16(x), lo = bf16(x - hi) and the product is hi*hi + hi*lo + lo*hi
+// (about 16 significant bits).  Each thread stages KT/2 consecutive k of one
+// row per operand as 16-byte vectors; a lane's MFMA operand is 8
+// consecutive k of one row (one 16-byte LDS read).  Accumulator layout as
+// for FP32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int NSPLIT>
+__global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __restrict__ jobs, int njobs,
+                                                             float* __restrict__ L, float* __restrict__ U) {
+    constexpr int BT = 128, KT = NSPLIT == 1 ? 64 : 32, LDK = KT + 8;   // rows 16-byte aligned
+    constexpr int KS = KT / 2;                                        // staged k per thread
+    constexpr int NP = NSPLIT == 1 ? 1 : 2;                           // hi (and lo) planes
+    __shared__ __align__(16) __bf16 As[NP][BT * LDK];
+    __shared__ __align__(16) __bf16 Bs[NP][BT * LDK];
+    int lo = 0, hi = njobs - 1;
+    const int bid = xcd_tile(blockIdx.x, gridDim.x);
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
+    }
+    const GemmJob g = jobs[lo];
+    int local = bid - g.tile_base;
+    int ti, tj;
+    if (g.lower) {
+        tj = 0;
+        int cnt = g.tiles_m;
+        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+        ti = tj + local;
+    } else {
+        tj = local / g.tiles_m;
+        ti = local % g.tiles_m;
+    }
+    const float* __restrict__ A = L + g.a_off;
+    const float* __restrict__ B = L + g.b_off;
+    float* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+    const int row0 = ti * BT, col0 = tj * BT;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    const int li = lane & 15, lg = lane >> 4;
+    const int r_st = tid & (BT - 1), kg = (tid >> 7) * KS;   // staged row, first staged k
+    const int ia = row0 + r_st, jb = col0 + r_st;
+    const bool a_ok = ia < g.M, b_ok = jb < g.N;
+    const float* __restrict__ Ap = A + ia;
+    const float* __restrict__ Bp = B + jb;
+    float ra[KS], rb[KS];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k = k0 + kg + s;
+            ra[s] = (a_ok && k < g.K) ? Ap[(int64_t)k * g.lda] : 0.0f;
+            rb[s] = (b_ok && k < g.K) ? Bp[(int64_t)k * g.ldb] : 0.0f;
+        }
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += KT) {
+#pragma unroll
+        for (int q = 0; q < KS / 8; ++q) {
+            bf16x8 ah, bh, al, bl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                ah[e] = (__bf16)ra[8 * q + e];
+                bh[e] = (__bf16)rb[8 * q + e];
+                if constexpr (NSPLIT == 3) {
+                    al[e] = (__bf16)(ra[8 * q + e] - (float)ah[e]);
+                    bl[e] = (__bf16)(rb[8 * q + e] - (float)bh[e]);
+                }
+            }
+            *reinterpret_cast<bf16x8*>(&As[0][r_st * LDK + kg + 8 * q]) = ah;
+            *reinterpret_cast<bf16x8*>(&Bs[0][r_st * LDK + kg + 8 * q]) = bh;
+            if constexpr (NSPLIT == 3) {
+                *reinterpret_cast<bf16x8*>(&As[NP - 1][r_st * LDK + kg + 8 * q]) = al;
+                *reinterpret_cast<bf16x8*>(&Bs[NP - 1][r_st * LDK + kg + 8 * q]) = bl;
+            }
+        }
+        __syncthreads();
+        if (k0 + KT < g.K) fetch(k0 + KT);
+#pragma unroll
+        for (int kq = 0; kq < KT; kq += 32) {
+            bf16x8 av[NP][4], bv[NP][4];
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    av[p][r] = *reinterpret_cast<const bf16x8*>(&As[p][(wr + r * 16 + li) * LDK + kq + 8 * lg]);
+                    bv[p][r] = *reinterpret_cast<const bf16x8*>(&Bs[p][(wc + r * 16 + li) * LDK + kq + 8 * lg]);
+                }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if constexpr (NSPLIT == 3) {
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[1][c], av[0][r], acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[1][r], acc[r][c], 0, 0, 0);
+                    }
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[0][r], acc[r][c], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    // acc[r][c][v] = C(row0 + wr + r*16 + li, col0 + wc + c*16 + 4*lg + v)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = col0 + wc + c * 16 + 4 * lg + v;
+            if (j >= g.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = row0 + wr + r * 16 + li;
+                if (i >= g.M || (g.lower && i < j)) continue;
+                float* cp = C + i + (int64_t)j * g.ldc;
+                *cp = g.beta0 ? -acc[r][c][v] : *cp - acc[r][c][v];
             }
         }
 }
@@ -633,9 +796,9 @@ __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ l
 }
 
 // ------------------------------------------------- solves of large fronts
-// Fronts with more than kSolveBW pivots: forward / backward substitution 256
-// pivot columns at a time, the rows below each block handled by many
-// workgroups (row chunks of 256).
+// Fronts with more than kSolveSmall pivots: forward / backward substitution
+// kSolveBW pivot columns at a time, the rows below each block handled by
+// many workgroups (row chunks of kSolveFwdRows / kSolveBwdRows).
 
 // Forward, step 0: clear the front's update vector and gather its
 // children's update vectors (the first phase of fwd_kernel).
@@ -668,8 +831,10 @@ __global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restr
 // workgroup solves the block's own triangle (four 64-column sub-blocks:
 // y_s = inv(L_s) v_s with the stored inverse, then the later sub-blocks'
 // right-hand sides are updated), the `first` one stores y in Y, and each
-// updates its 256 rows [a, a+256) below the block.  v is final on entry:
-// earlier blocks were applied by earlier launches.
+// updates its kSolveFwdRows rows [a, a + kSolveFwdRows) below the block,
+// four lanes per row (64 columns each, partial sums added in a fixed
+// order).  v is final on entry: earlier blocks were applied by earlier
+// launches.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
@@ -679,6 +844,7 @@ __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __re
                                                            const int64_t* __restrict__ suofs) {
     __shared__ double W[64 * 65];
     __shared__ double vb[kSolveBW];
+    __shared__ double red[4][64];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
     const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
@@ -712,14 +878,21 @@ __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __re
         __syncthreads();
     }
     if (it.first && tid < bw) Y[d.k + j0 + tid] = vb[tid];
-    const int i = it.a + tid;
-    if (i < n) {
-        const T* __restrict__ pr = P + i + (int64_t)j0 * n;
-        double acc = 0.0;
+    const int r = tid & 63, cq = tid >> 6;
+    const int i = it.a + r;
+    const int c0 = cq * 64, c1 = min(c0 + 64, bw);
+    double acc = 0.0;
+    if (i < n && c0 < c1) {
+        const T* __restrict__ pr = P + i + (int64_t)(j0 + c0) * n;
 #pragma unroll 16
-        for (int j = 0; j < bw; ++j) acc += to_d(pr[(int64_t)j * n]) * vb[j];
-        if (i < d.c) xp[d.k + i] -= acc;
-        else su[suofs[it.front] + i - d.c] -= acc;
+        for (int j = 0; j < c1 - c0; ++j) acc += to_d(pr[(int64_t)j * n]) * vb[c0 + j];
+    }
+    red[cq][r] = acc;
+    __syncthreads();
+    if (tid < 64 && i < n) {
+        const double sum = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+        if (i < d.c) xp[d.k + i] -= sum;
+        else su[suofs[it.front] + i - d.c] -= sum;
     }
 }
 
@@ -749,7 +922,7 @@ __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __re
     const T* __restrict__ P = L + d.lofs;
     const int tid = threadIdx.x, r = tid & 63, cq = tid >> 6;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int r0 = it.a; r0 < min(it.a + kSolveBW, n); r0 += 64) {
+    for (int r0 = it.a; r0 < min(it.a + kSolveBwdRows, n); r0 += 64) {
         const int rows = min(64, n - r0);
         __syncthreads();
         if (tid < 64) xq[r] = r < rows ? xp[qall[d.qofs + r0 + r]] : 0.0;
@@ -788,12 +961,19 @@ __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __re
     const int nsub = (bw + 63) / 64;
     for (int sb = nsub - 1; sb >= 0; --sb) {
         const int s0 = sb * 64, s1 = min(s0 + 64, bw), b = s1 - s0;
-        // v_j -= sum over solved block rows [s1, bw) of L(row, j) x_row, 4 threads per column
+        // v_j -= sum over the solved block rows [s1, bw) of L(row, j) x_row,
+        // through 64x64 tiles staged in LDS (coalesced column reads)
         double part = 0.0;
-        if (r < b) {
-            const T* __restrict__ col = P + (int64_t)(j0 + s0 + r) * n + j0;
-#pragma unroll 4
-            for (int row = s1 + cq; row < bw; row += 4) part += to_d(col[row]) * vb[row];
+        for (int t0 = s1; t0 < bw; t0 += 64) {
+            const int rows = min(64, bw - t0);
+            __syncthreads();
+            if (tid < 64) xq[r] = r < rows ? vb[t0 + r] : 0.0;
+#pragma unroll
+            for (int cc = cq; cc < 64; cc += 4)
+                tile[cc * 65 + r] = (r < rows && cc < b) ? P[(j0 + t0 + r) + (int64_t)(j0 + s0 + cc) * n] : T(0);
+            __syncthreads();
+#pragma unroll
+            for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) part += to_d(tile[r * 65 + rr]) * xq[rr];
         }
         red[cq][r] = part;
         __syncthreads();
@@ -914,22 +1094,23 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
     panel_kernel<T><<<count, kThreads, lds_bytes, s>>>(list, fronts, L, info);
 }
 template <typename T>
-void launch_diag(const DiagJob* jobs, int count, T* L, T* inv, int* info, hipStream_t s) {
-    if (count > 0) diag_kernel<T><<<count, kThreads, 0, s>>>(jobs, L, inv, info);
+void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, hipStream_t s) {
+    if (nblocks > 0) blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters);
 }
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {
     if (count > 0) inv_kernel<T><<<count, 64, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
-void launch_trsm(const TrsmJob* jobs, int njobs, int nblocks, T* L, hipStream_t s) {
-    if (nblocks > 0) trsm_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L);
-}
-template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s) {
     if (nblocks <= 0) return;
     if (tile == 128) gemm_mfma_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
     else gemm_nt_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+}
+void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s) {
+    if (nblocks <= 0) return;
+    if (nsplit == 3) gemm_bf16_kernel<3><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    else gemm_bf16_kernel<1><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
 }
 
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s) {
@@ -995,9 +1176,8 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
     template void launch_extend_add<T>(const EaItem*, int, const FrontDesc*, const int*, T*, T*, hipStream_t); \
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
-    template void launch_diag<T>(const DiagJob*, int, T*, T*, int*, hipStream_t);                             \
+    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, hipStream_t);                         \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
-    template void launch_trsm<T>(const TrsmJob*, int, int, T*, hipStream_t);                                   \
     template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -44,8 +45,8 @@ struct mdgpu_numeric {
     int64_t* dAdst = nullptr;
     mdgpu::EaItem* dEa = nullptr;
     mdgpu::GemmJob* dGemm = nullptr;
-    mdgpu::DiagJob* dDiag = nullptr;
-    mdgpu::TrsmJob* dTrsm = nullptr;
+    mdgpu::BlkJob* dBlk = nullptr;
+    int* dBlkCnt = nullptr;          // arrival counters of the block-column jobs
     mdgpu::InvJob* dInvJobs = nullptr;
     int* dPanel = nullptr;
     int* dLevelList = nullptr;
@@ -159,16 +160,15 @@ int capture(mdgpu_numeric* N, hipGraphExec_t* out, F body) {
     } while (0)
 
 // Kernel classes reported by mdgpu_factorize_profile.
-enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_DIAG, KIND_TRSM, KIND_GEMM, KIND_N };
+enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_POTRF_TRSM, KIND_INVERSE, KIND_GEMM, KIND_N };
 
 int kind_of(int step_kind) {
     switch (step_kind) {
         case mdgpu::STEP_EA1:
         case mdgpu::STEP_EA2: return KIND_EXTEND_ADD;
         case mdgpu::STEP_PANEL: return KIND_PANEL;
-        case mdgpu::STEP_DIAG: return KIND_DIAG;
-        case mdgpu::STEP_TRSM: return KIND_TRSM;
-        case mdgpu::STEP_INV: return KIND_DIAG;
+        case mdgpu::STEP_BLK: return KIND_POTRF_TRSM;
+        case mdgpu::STEP_INV: return KIND_INVERSE;
         default: return KIND_GEMM;
     }
 }
@@ -196,13 +196,17 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
             case mdgpu::STEP_PANEL:
                 mdgpu::launch_panel<T>(N->dPanel + st.first, st.count, N->dFronts, L, N->dInfo, st.arg, s);
                 break;
-            case mdgpu::STEP_DIAG:
-                mdgpu::launch_diag<T>(N->dDiag + st.first, st.count, L, inv, N->dInfo, s);
-                break;
-            case mdgpu::STEP_TRSM:
-                mdgpu::launch_trsm<T>(N->dTrsm + st.first, st.count, st.nblocks, L, s);
+            case mdgpu::STEP_BLK:
+                mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt, s);
                 break;
             case mdgpu::STEP_GEMM:
+                if constexpr (std::is_same<T, float>::value) {
+                    if (N->precision == MDGPU_BF16 || N->precision == MDGPU_BF16X3) {
+                        mdgpu::launch_gemm_bf16(N->dGemm + st.first, st.count, st.nblocks,
+                                                N->precision == MDGPU_BF16X3 ? 3 : 1, L, U, s);
+                        break;
+                    }
+                }
                 mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, N->tile, L, U, s);
                 break;
             case mdgpu::STEP_INV:
@@ -298,8 +302,8 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     mdgpu::clear_error();
     if (!sym || !out) return fail(MDGPU_ERR_ARG, "mdgpu_numeric_create: null pointer");
     *out = nullptr;
-    if (precision != MDGPU_FP64 && precision != MDGPU_FP32)
-        return fail(MDGPU_ERR_ARG, "mdgpu_numeric_create: precision must be MDGPU_FP64 or MDGPU_FP32");
+    if (precision < MDGPU_FP64 || precision > MDGPU_BF16X3)
+        return fail(MDGPU_ERR_ARG, "mdgpu_numeric_create: precision must be MDGPU_FP64, MDGPU_FP32, MDGPU_BF16 or MDGPU_BF16X3");
     mdgpu_numeric* N = new (std::nothrow) mdgpu_numeric();
     if (!N) return fail(MDGPU_ERR_NOMEM, "out of host memory");
     N->sym = sym;
@@ -352,8 +356,8 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.a_dst, (void**)&N->dAdst)) ||
         (st = upload(N, P.ea, (void**)&N->dEa)) ||
         (st = upload(N, P.gemm, (void**)&N->dGemm)) ||
-        (st = upload(N, P.diag, (void**)&N->dDiag)) ||
-        (st = upload(N, P.trsm, (void**)&N->dTrsm)) ||
+        (st = upload(N, P.blk, (void**)&N->dBlk)) ||
+        (st = upload(N, std::vector<int>(P.blk.size(), 0), (void**)&N->dBlkCnt)) ||
         (st = upload(N, P.panel, (void**)&N->dPanel)) ||
         (st = upload(N, level_list, (void**)&N->dLevelList)) ||
         (st = alloc(N, P.solve_u * sizeof(double), (void**)&N->dSu)) ||
@@ -408,7 +412,7 @@ extern "C" int mdgpu_numeric_info(const mdgpu_numeric* num, int64_t* out) {
     out[MDGPU_NUM_STEPS] = (int64_t)P.steps.size();
     out[MDGPU_NUM_LEVELS] = P.nlevels;
     out[MDGPU_NUM_PANEL_FRONTS] = (int64_t)P.panel.size();
-    out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.diag.size();
+    out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.blk.size();
     out[MDGPU_NUM_PRECISION] = num->precision;
     return MDGPU_OK;
 }

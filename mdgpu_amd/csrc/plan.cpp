@@ -178,7 +178,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
 
     // solve schedule (see SolveStep)
     {
-        const int BW = kSolveBW, RC = kSolveBW;
+        const int BW = kSolveBW;
         for (int L = 0; L < P.nlevels; L++) {
             std::vector<int> small, big;
             for (int f : P.level_fronts[L]) (P.fronts[f].c > kSolveSmall ? big : small).push_back(f);
@@ -200,7 +200,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     if (j0 >= d.c) continue;
                     const int j1 = std::min(j0 + BW, d.c);
                     P.sv_items.push_back({f, j1, 1, j0, 0, 0, 0, 0});
-                    for (int a = j1 + RC; a < d.n; a += RC) P.sv_items.push_back({f, a, 0, j0, 0, 0, 0, 0});
+                    for (int a = j1 + kSolveFwdRows; a < d.n; a += kSolveFwdRows) P.sv_items.push_back({f, a, 0, j0, 0, 0, 0, 0});
                     st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 16.0 * (d.n - j0);
                 }
                 st.count = (int)P.sv_items.size() - st.first;
@@ -228,9 +228,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     const FrontDesc& d = P.fronts[f];
                     if (j0 >= d.c) continue;
                     const int j1 = std::min(j0 + BW, d.c);
-                    const int nch = std::max(1, (d.n - j1 + RC - 1) / RC);
+                    const int nch = std::max(1, (d.n - j1 + kSolveBwdRows - 1) / kSolveBwdRows);
                     for (int t = 0; t < nch; t++)
-                        P.sv_items.push_back({f, j1 + t * RC, 0, j0, slot + t, group, slot, nch});
+                        P.sv_items.push_back({f, j1 + t * kSolveBwdRows, 0, j0, slot + t, group, slot, nch});
                     slot += nch;
                     group++;
                     st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 12.0 * (d.n - j0);
@@ -343,78 +343,82 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 P.steps.push_back(st);
             }
         }
-        // large panels: blocked right-looking, nb columns per step
+        // large panels: blocked right-looking in two levels -- nb-column
+        // block columns (BLK + a narrow update of the rest of the outer
+        // block) inside nb_outer-column outer blocks, whose trailing update
+        // is one deep (K = nb_outer) GEMM
         int maxc = 0;
         for (int f : blocked) maxc = std::max(maxc, P.fronts[f].c);
-        for (int r0 = 0; r0 < maxc; r0 += o.nb) {
-            const int dbegin = (int)P.diag.size();
-            for (int f : blocked) {
-                const FrontDesc& d = P.fronts[f];
-                if (r0 >= d.c) continue;
-                DiagJob dj;
-                dj.b = std::min(o.nb, d.c - r0);
-                dj.ld = d.n;
-                dj.p_off = d.lofs + (int64_t)r0 * d.n + r0;
-                dj.inv_off = d.dinv + (int64_t)(r0 / 64) * 4096;
-                dj.col0 = d.k + r0;
-                dj.front = f;
-                P.diag.push_back(dj);
-            }
-            const int dcount = (int)P.diag.size() - dbegin;
-            if (dcount == 0) continue;
-            {
-                Step st{STEP_DIAG, dbegin, dcount, dcount, L, 0};
+        for (int R0 = 0; R0 < maxc; R0 += o.nb_outer) {
+            for (int r0 = R0; r0 < std::min(R0 + o.nb_outer, maxc); r0 += o.nb) {
+                const int dbegin = (int)P.blk.size();
+                int btiles = 0;
+                for (int f : blocked) {
+                    const FrontDesc& d = P.fronts[f];
+                    if (r0 >= d.c) continue;
+                    BlkJob bj;
+                    bj.b = std::min(o.nb, d.c - r0);
+                    bj.ld = d.n;
+                    bj.p_off = d.lofs + (int64_t)r0 * d.n + r0;
+                    bj.col0 = d.k + r0;
+                    bj.rows = d.n - r0 - bj.b;
+                    bj.ntiles = std::max(1, (bj.rows + kBlkRows - 1) / kBlkRows);
+                    bj.tile_base = btiles;
+                    bj.cnt = (int)P.blk.size();
+                    bj.front = f;
+                    btiles += bj.ntiles;
+                    P.blk.push_back(bj);
+                }
+                const int dcount = (int)P.blk.size() - dbegin;
+                if (dcount == 0) continue;
+                {
+                    Step st{STEP_BLK, dbegin, dcount, btiles, L, 0};
+                    for (int t = dbegin; t < dbegin + dcount; t++) {
+                        const double bb = P.blk[t].b, rows = P.blk[t].rows;
+                        st.flops += bb * bb * bb / 3.0 + rows * bb * bb;
+                        st.bytes += (2.0 * bb * bb / 2 + 2.0 * rows * bb) * o.elem_bytes;
+                    }
+                    P.steps.push_back(st);
+                }
+                // update of the outer block's remaining columns [r1, e)
+                std::vector<GemmJob> batch;
+                int tiles = 0;
                 for (int t = dbegin; t < dbegin + dcount; t++) {
-                    const double b = P.diag[t].b;
-                    st.flops += b * b * b / 3.0 + b * b * b / 3.0;
-                    st.bytes += 2.5 * b * b * o.elem_bytes;
+                    const BlkJob& dj = P.blk[t];
+                    const FrontDesc& d = P.fronts[dj.front];
+                    const int r1 = r0 + dj.b, e = std::min(R0 + o.nb_outer, d.c);
+                    if (r1 >= e) continue;
+                    GemmJob g{};
+                    g.a_off = d.lofs + (int64_t)r0 * d.n + r1;
+                    g.b_off = g.a_off;
+                    g.c_off = d.lofs + (int64_t)r1 * d.n + r1;
+                    g.lda = g.ldb = g.ldc = d.n;
+                    g.M = d.n - r1;
+                    g.N = e - r1;
+                    g.K = dj.b;
+                    g.c_in_arena = 0;
+                    g.beta0 = 0;
+                    g.lower = 1;
+                    add_gemm(g, batch, tiles);
                 }
-                P.steps.push_back(st);
+                flush_gemm(batch, tiles, L);
             }
-            const int tbegin = (int)P.trsm.size();
-            int ttiles = 0;
-            for (int t = dbegin; t < dbegin + dcount; t++) {
-                const DiagJob& dj = P.diag[t];
-                const FrontDesc& d = P.fronts[dj.front];
-                const int rows = d.n - r0 - dj.b;
-                if (rows <= 0) continue;
-                TrsmJob tj;
-                tj.p_off = dj.p_off + dj.b;
-                tj.inv_off = dj.inv_off;
-                tj.ld = d.n;
-                tj.rows = rows;
-                tj.b = dj.b;
-                tj.ntiles = (rows + o.trsm_rows - 1) / o.trsm_rows;
-                tj.tile_base = ttiles;
-                tj.pad = 0;
-                ttiles += tj.ntiles;
-                P.trsm.push_back(tj);
-            }
-            const int tcount = (int)P.trsm.size() - tbegin;
-            if (tcount > 0) {
-                Step st{STEP_TRSM, tbegin, tcount, ttiles, L, 0};
-                for (int t = tbegin; t < tbegin + tcount; t++) {
-                    const TrsmJob& tj = P.trsm[t];
-                    st.flops += (double)tj.rows * tj.b * (tj.b + 1);
-                    st.bytes += 2.0 * tj.rows * tj.b * o.elem_bytes;
-                }
-                P.steps.push_back(st);
-            }
+            // trailing update of the pivot columns after the outer block
             std::vector<GemmJob> batch;
             int tiles = 0;
-            for (int t = dbegin; t < dbegin + dcount; t++) {
-                const DiagJob& dj = P.diag[t];
-                const FrontDesc& d = P.fronts[dj.front];
-                const int r1 = r0 + dj.b;
-                if (r1 >= d.c) continue;        // no trailing pivot columns
+            for (int f : blocked) {
+                const FrontDesc& d = P.fronts[f];
+                if (R0 >= d.c) continue;
+                const int E = std::min(R0 + o.nb_outer, d.c);
+                if (E >= d.c) continue;
                 GemmJob g{};
-                g.a_off = d.lofs + (int64_t)r0 * d.n + r1;
+                g.a_off = d.lofs + (int64_t)R0 * d.n + E;
                 g.b_off = g.a_off;
-                g.c_off = d.lofs + (int64_t)r1 * d.n + r1;
+                g.c_off = d.lofs + (int64_t)E * d.n + E;
                 g.lda = g.ldb = g.ldc = d.n;
-                g.M = d.n - r1;
-                g.N = d.c - r1;
-                g.K = dj.b;
+                g.M = d.n - E;
+                g.N = d.c - E;
+                g.K = E - R0;
                 g.c_in_arena = 0;
                 g.beta0 = 0;
                 g.lower = 1;
@@ -552,15 +556,14 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         if (g.a_off < 0 || alast >= P.lstore || g.b_off < 0 || blast >= P.lstore) return bad("gemm A/B bounds");
         if (g.c_off < 0 || clast >= (g.c_in_arena ? P.arena : P.lstore)) return bad("gemm C bounds");
     }
-    for (const DiagJob& dj : P.diag) {
-        if (dj.b < 1 || dj.b > o.nb || dj.b > 64) return bad("diag size");
-        if (dj.p_off < 0 || dj.p_off + (dj.b - 1) + (int64_t)(dj.b - 1) * dj.ld >= P.lstore) return bad("diag bounds");
-        if (dj.inv_off < 0 || dj.inv_off + (int64_t)dj.b * 64 > P.dinv) return bad("diag inverse store");
-    }
-    for (const TrsmJob& tj : P.trsm) {
-        if (tj.b < 1 || tj.b > 64 || tj.rows < 1) return bad("trsm shape");
-        if (tj.p_off + (tj.rows - 1) + (int64_t)(tj.b - 1) * tj.ld >= P.lstore) return bad("trsm bounds");
-        if (tj.inv_off + (int64_t)tj.b * 64 > P.dinv) return bad("trsm inverse store");
+    for (size_t t = 0; t < P.blk.size(); t++) {
+        const BlkJob& bj = P.blk[t];
+        if (bj.b < 1 || bj.b > o.nb || bj.b > 64 || bj.rows < 0) return bad("block shape");
+        if (bj.ntiles != std::max(1, (bj.rows + kBlkRows - 1) / kBlkRows)) return bad("block tiles");
+        if (bj.cnt != (int)t) return bad("block counter");
+        if (bj.p_off < 0 || bj.p_off + (bj.b - 1) + bj.rows + (int64_t)(bj.b - 1) * bj.ld >= P.lstore)
+            return bad("block bounds");
+        if (bj.b + bj.rows > bj.ld) return bad("block rows exceed panel");
     }
     auto check_solve = [&](const std::vector<SolveStep>& steps) -> std::string {
         for (const SolveStep& st : steps) {
@@ -607,6 +610,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
             return bad("front inverse store");
     }
     if (o.nb != 64) return bad("blocked path needs 64-column diagonal blocks");
+    if (o.nb_outer < o.nb || o.nb_outer % o.nb) return bad("outer block must be a multiple of the block column");
     for (int f : P.panel) {
         const FrontDesc& d = P.fronts[f];
         if ((int64_t)d.n * d.c * o.elem_bytes > o.panel_lds_bytes) return bad("panel LDS");
@@ -621,13 +625,13 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
             }
             if (tiles != s.nblocks) return bad("gemm tile count");
         }
-        if (s.kind == STEP_TRSM) {
+        if (s.kind == STEP_BLK) {
             int tiles = 0;
             for (int t = s.first; t < s.first + s.count; t++) {
-                if (P.trsm[t].tile_base != tiles) return bad("trsm tile bases");
-                tiles += P.trsm[t].ntiles;
+                if (P.blk[t].tile_base != tiles) return bad("block tile bases");
+                tiles += P.blk[t].ntiles;
             }
-            if (tiles != s.nblocks) return bad("trsm tile count");
+            if (tiles != s.nblocks) return bad("block tile count");
         }
     }
     return "";

@@ -66,26 +66,21 @@ struct GemmJob {
     int ntiles;
 };
 
-// Diagonal-block job of the blocked panel path: factor the b x b block at
-// (r0, r0) of front f's panel, write its inverse to inv_off.
-struct DiagJob {
-    int64_t p_off;                 // panel offset of the block (elements)
-    int64_t inv_off;               // b*b scratch offset for inv(L_D)
-    int ld, b, col0;               // panel ld, block size, global column of r0
-    int front;
+// Block-column job of the blocked panel path: Cholesky of the b x b
+// diagonal block at p_off and the triangular solve of the `rows` rows
+// below it (L21 <- A21 L_D^{-T}), in ntiles workgroups of kBlkRows rows.
+// Every workgroup factors the diagonal block itself; the last one to have
+// read it (counter `cnt`) writes it back.
+struct BlkJob {
+    int64_t p_off;                 // panel offset of the diagonal block
+    int ld, b, col0, rows;         // panel ld, block size, global column, rows below
+    int tile_base, ntiles;         // first workgroup of this job, workgroups (>= 1)
+    int cnt, front;                // arrival counter index, front
 };
-
-// TRSM job: rows [r0, r1) of a panel block column times inv(L_D)^T.
-struct TrsmJob {
-    int64_t p_off;                 // offset of element (r0, col) of the block column
-    int64_t inv_off;
-    int ld, rows, b;
-    int tile_base, ntiles;
-    int pad;
-};
+constexpr int kBlkRows = 256;
 
 // A launch of the schedule: kind + a slice of the matching job array.
-enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_DIAG, STEP_TRSM, STEP_GEMM, STEP_INV };
+enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV };
 
 // Diagonal-block inverse job (fronts factored by the LDS panel kernel):
 // block of 64 pivot columns starting at j0 of front `front`.
@@ -113,6 +108,8 @@ struct Step {
 // solves the block.  The forward result goes to a separate vector Y.
 constexpr int kSolveBW = 256;
 constexpr int kSolveSmall = 64;
+constexpr int kSolveFwdRows = 64;   // rows per forward workgroup below a block
+constexpr int kSolveBwdRows = 256;  // rows per backward workgroup below a block
 enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_BLK };
 
 struct SolveItem {
@@ -151,8 +148,7 @@ struct Plan {
     int sv_groups = 1;                 // counters needed by one BWD_BLK launch
     std::vector<EaItem> ea;
     std::vector<GemmJob> gemm;
-    std::vector<DiagJob> diag;
-    std::vector<TrsmJob> trsm;
+    std::vector<BlkJob> blk;
     std::vector<int> panel;            // fronts factored by the LDS panel kernel
     std::vector<InvJob> inv;           // diagonal-block inverses of every front
     // Inverses of the 64x64 diagonal blocks of every front, block t of front
@@ -170,8 +166,8 @@ struct PlanOptions {
     int elem_bytes = 8;          // 8: FP64 store, 4: FP32 store
     int panel_lds_bytes = 65536; // LDS budget of the one-workgroup panel kernel
     int nb = 64;                 // block width of the blocked panel path
+    int nb_outer = 256;          // outer block width (trailing updates with K = nb_outer)
     int tile = 64;               // GEMM tile (rows = cols)
-    int trsm_rows = 64;          // rows per TRSM workgroup
     int ea_chunk = 16384;        // target entries per extend-add workgroup
 };
 

@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib as L
 
-PRECISIONS = {"fp64": L.FP64, "fp32": L.FP32, "bf16": L.BF16}
+PRECISIONS = {"fp64": L.FP64, "fp32": L.FP32, "bf16": L.BF16, "bf16x3": L.BF16X3}
 
 
 def _torch():

@@ -27,7 +27,7 @@ static void invert_block(const double* g, int ld, int b, double* w) {
 }
 
 extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
-                              int max_zeros, int nb, double* Lx_out, int* Li_out) {
+                              int max_zeros, int nb_outer, double* Lx_out, int* Li_out) {
     AnalyzeOptions ao;
     ao.use_amd = use_amd;
     ao.partition = (Partition)mode;
@@ -37,7 +37,8 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
     PlanOptions po;
     po.elem_bytes = 8;
     po.panel_lds_bytes = 61440;
-    po.nb = nb;
+    po.nb = 64;
+    po.nb_outer = nb_outer;
     po.tile = 64;
     Plan P;
     if (!build_plan(S, po, P).empty()) return 2;
@@ -72,11 +73,11 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
                             g[i + col * d.n] -= g[i + j * d.n] * g[col + j * d.n];
                 }
             }
-        } else if (st.kind == STEP_DIAG) {
+        } else if (st.kind == STEP_BLK) {
             for (int t = st.first; t < st.first + st.count; t++) {
-                const DiagJob& dj = P.diag[t];
-                double* g = &L[dj.p_off];
-                const int b = dj.b, ld = dj.ld;
+                const BlkJob& bj = P.blk[t];
+                double* g = &L[bj.p_off];
+                const int b = bj.b, ld = bj.ld;
                 for (int j = 0; j < b; j++) {
                     double rt = std::sqrt(g[j + j * ld]);
                     g[j + j * ld] = rt;
@@ -84,18 +85,12 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
                     for (int i = j + 1; i < b; i++)
                         for (int col = j + 1; col <= i; col++) g[i + col * ld] -= g[i + j * ld] * g[col + j * ld];
                 }
-                (void)inv;
-            }
-        } else if (st.kind == STEP_TRSM) {
-            for (int t = st.first; t < st.first + st.count; t++) {
-                const TrsmJob& tj = P.trsm[t];
-                const double* dg = &L[tj.p_off - tj.b];
-                for (int i = 0; i < tj.rows; i++) {
-                    double* row = &L[tj.p_off + i];
-                    for (int j = 0; j < tj.b; j++) {
-                        double acc = row[(int64_t)j * tj.ld];
-                        for (int l = 0; l < j; l++) acc -= row[(int64_t)l * tj.ld] * dg[j + (int64_t)l * tj.ld];
-                        row[(int64_t)j * tj.ld] = acc / dg[j + (int64_t)j * tj.ld];
+                for (int i = 0; i < bj.rows; i++) {
+                    double* row = &g[b + i];
+                    for (int j = 0; j < b; j++) {
+                        double acc = row[(int64_t)j * ld];
+                        for (int l = 0; l < j; l++) acc -= row[(int64_t)l * ld] * g[j + (int64_t)l * ld];
+                        row[(int64_t)j * ld] = acc / g[j + (int64_t)j * ld];
                     }
                 }
             }

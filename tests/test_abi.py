@@ -60,7 +60,7 @@ def test_invalid_inputs_are_rejected_with_status():
 def test_plan_check_on_host():
     p, i, x = gen.laplace3d_7pt(10)
     S = symbolic.analyze(p, i, True, "rn", 500)
-    for prec in (L.FP64, L.FP32):
+    for prec in (L.FP64, L.FP32, L.BF16, L.BF16X3):
         out = (C.c_int64 * L.NUM_NINFO)()
         L.check(L.lib().mdgpu_plan_check(S.handle, prec, out))
         assert out[L.NUM_LSTORE] >= S.lnz and out[L.NUM_STEPS] > 0

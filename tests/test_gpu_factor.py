@@ -51,6 +51,42 @@ def test_fp32_refined_to_1e12(device, name, make, mode, mz):
     assert eta <= 1e-12 and eta_cpu <= 1e-12, (eta, eta_cpu, iters)
 
 
+@pytest.mark.parametrize("name,make,mode,mz", CASES, ids=[c[0] for c in CASES])
+def test_bf16x3_refined_to_1e12(device, name, make, mode, mz):
+    """Split-bf16 Schur updates (hi*hi + hi*lo + lo*hi on bf16 MFMA) + FP64 IR."""
+    import torch
+    p, i, x = make()
+    xt, b = gen.rhs_from_xtrue(p, i, x, seed=5)
+    S = symbolic.analyze(p, i, True, mode, mz)
+    F = Factorization(S, "bf16x3").factorize(torch.from_numpy(x).to(device))
+    xs, eta, iters = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=50)
+    from oracle import oracle as O
+    eta_cpu = O.backward_error(p, i, x, xs.cpu().numpy(), b)
+    assert eta <= 1e-12 and eta_cpu <= 1e-12, (eta, eta_cpu, iters)
+
+
+BF16_CASES = [c for c in CASES if c[0] in ("grid2d_7_plain", "bbt_600_rn1000")]
+
+
+@pytest.mark.parametrize("name,make,mode,mz", BF16_CASES, ids=[c[0] for c in BF16_CASES])
+def test_bf16_refined_to_1e12(device, name, make, mode, mz):
+    """Single bf16 products converge under FP64 IR on well-conditioned cases
+    (kappa ~ 25 and ~ 1); the factor itself is close to the FP32 one."""
+    import torch
+    p, i, x = make()
+    xt, b = gen.rhs_from_xtrue(p, i, x, seed=6)
+    S = symbolic.analyze(p, i, True, mode, mz)
+    ax = torch.from_numpy(x).to(device)
+    F = Factorization(S, "bf16").factorize(ax)
+    xs, eta, iters = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=60)
+    from oracle import oracle as O
+    eta_cpu = O.backward_error(p, i, x, xs.cpu().numpy(), b)
+    assert eta <= 1e-12 and eta_cpu <= 1e-12, (eta, eta_cpu, iters)
+    _, _, l16 = F.export_L()
+    _, _, l32 = Factorization(S, "fp32").factorize(ax).export_L()
+    assert rel_err(l16, l32) < 5e-2
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_graph_replays_match_eager(device, prec):
     """The captured factorization/solve graphs are replayed several times with

@@ -1,5 +1,6 @@
 """The numeric schedule (extend-add rounds and splits, update-arena offsets,
-panel / blocked diagonal / TRSM / GEMM jobs, diagonal-block inverses),
+panel / blocked potrf+TRSM / GEMM jobs at several outer block widths,
+diagonal-block inverses),
 executed step by step on the CPU by tests/emulate/plan_emulator.cpp with the
 device kernels' loops, reproduces the oracle's factor.  This checks the plan
 without a GPU; the kernels themselves are checked by the gpu tests."""
@@ -36,8 +37,9 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("nb_outer", [64, 128, 256])
 @pytest.mark.parametrize("name,make,mode,mz", CASES, ids=[c[0] for c in CASES])
-def test_schedule_reproduces_oracle(emulator, name, make, mode, mz):
+def test_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_outer):
     p, i, x = make()
     S = symbolic.analyze(p, i, True, ["plain", "sn", "rn"][mode], mz)
     lp, li, lx, _, _ = oracle_factor(p, i, x, S)
@@ -45,7 +47,7 @@ def test_schedule_reproduces_oracle(emulator, name, make, mode, mz):
     Lx = np.zeros(S.lnz)
     Li = np.zeros(S.lnz, np.int32)
     st = emulator.emulate_factor(len(p) - 1, p.ctypes.data_as(I), i.ctypes.data_as(I), x.ctypes.data_as(D), 1,
-                                 mode, mz, 64, Lx.ctypes.data_as(D), Li.ctypes.data_as(I))
+                                 mode, mz, nb_outer, Lx.ctypes.data_as(D), Li.ctypes.data_as(I))
     assert st == 0
     assert np.array_equal(Li, li)
     assert rel_err(Lx, lx) < 1e-12

@@ -65,6 +65,19 @@ int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,
                   const mdgpu_analyze_opts* opts, mdgpu_symbolic** sym);
 int mdgpu_symbolic_destroy(mdgpu_symbolic* sym);
 
+/* Batch of independent matrices analysed as one: each matrix k (order n[k],
+ * lower triangle Cp[k]/Ci[k]) gets exactly the analysis mdgpu_analyze
+ * would give it (run on `nthreads` host threads, 0 = all cores), and the
+ * results are joined block-diagonally into one forest, so one numeric
+ * handle factors and solves all of them in one level-batched launch
+ * sequence (one hipGraph).  Values, right-hand sides and solutions are the
+ * concatenation of the matrices' own (matrix k's columns start at
+ * MDGPU_ARR_SEG_COL[k], its values at MDGPU_ARR_SEG_NNZ[k]); mdgpu_refine
+ * stops when every matrix reaches the tolerance.  No counterpart in the
+ * reference (its driver takes one matrix, mdgpu.cu:1666-1939). */
+int mdgpu_analyze_batch(int nmat, const int32_t* n, const int32_t* const* Cp, const int32_t* const* Ci,
+                        const mdgpu_analyze_opts* opts, int nthreads, mdgpu_symbolic** sym);
+
 /* Analysis for a caller-supplied ordering and partition: Cp/Ci is the
  * lower triangle ALREADY permuted, and the nodes (first column, column
  * count, children as node indices in kid_ptr/kids CSR form) are in
@@ -85,7 +98,8 @@ int mdgpu_analyze_nodes(int n, const int32_t* Cp, const int32_t* Ci, int nnodes,
 #define MDGPU_SYM_MAXFRONT 5      /* largest front order                   */
 #define MDGPU_SYM_NLEVELS 6       /* levels of the front tree              */
 #define MDGPU_SYM_NODES_FLAT 7    /* length of the MDGPU_ARR_NODES array   */
-#define MDGPU_SYM_NSIZES 8
+#define MDGPU_SYM_NSEG 8          /* independent matrices (1, or the batch) */
+#define MDGPU_SYM_NSIZES 9
 int mdgpu_symbolic_sizes(const mdgpu_symbolic* sym, int64_t* out /* [MDGPU_SYM_NSIZES] */);
 
 /* Flop counts: [0] = sum_j c_j^2 over the unpadded cs_counts column counts,
@@ -110,6 +124,8 @@ int mdgpu_symbolic_flops(const mdgpu_symbolic* sym, double* out /* [2] */);
 #define MDGPU_ARR_Q_OFF 12        /* nnodes+1 (int32; fails past 2^31)     */
 #define MDGPU_ARR_Q 13            /* qlen: sorted row ids of each front    */
 #define MDGPU_ARR_LEVEL 14        /* nnodes: height of each front          */
+#define MDGPU_ARR_SEG_COL 15      /* nseg+1: first column of each matrix   */
+#define MDGPU_ARR_SEG_NNZ 16      /* nseg+1: first caller entry of each matrix */
 int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_t* dst, int64_t cap);
 
 
@@ -180,6 +196,11 @@ int mdgpu_solve(mdgpu_numeric* num, const double* d_b, double* d_x, void* stream
  * (with eta/iters filled) if tol was not reached. */
 int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol, int maxit,
                  double* eta, int* iters, void* stream);
+
+/* Backward error of each independent matrix (MDGPU_SYM_NSEG of them) after
+ * the last mdgpu_refine; for a batch, *eta of mdgpu_refine is their max.
+ * No counterpart in the reference. */
+int mdgpu_refine_segment_eta(const mdgpu_numeric* num, double* out, int cap);
 
 /* log det A = 2 sum_j log L_jj (no counterpart in the reference). */
 int mdgpu_logdet(mdgpu_numeric* num, double* out, void* stream);

@@ -22,11 +22,12 @@ MDGPU_ERR_NOT_CONVERGED = 5
 FP64, FP32, BF16, BF16X3 = 0, 1, 2, 3
 PART_PLAIN, PART_SUPERNODAL, PART_RELAXED = 0, 1, 2
 
-SYM_N, SYM_NNODES, SYM_LNZ, SYM_NNZ_A, SYM_QLEN, SYM_MAXFRONT, SYM_NLEVELS, SYM_NODES_FLAT = range(8)
-SYM_NSIZES = 8
+SYM_N, SYM_NNODES, SYM_LNZ, SYM_NNZ_A, SYM_QLEN, SYM_MAXFRONT, SYM_NLEVELS, SYM_NODES_FLAT, SYM_NSEG = range(9)
+SYM_NSIZES = 9
 
 (ARR_AMD_PERM, ARR_PARENT, ARR_POST, ARR_COUNT, ARR_RELAXED_PERM, ARR_PERM, ARR_COUNT_FINAL,
- ARR_LP, ARR_ALOW_P, ARR_ALOW_I, ARR_NODES, ARR_FRONT_PARENT, ARR_Q_OFF, ARR_Q, ARR_LEVEL) = range(15)
+ ARR_LP, ARR_ALOW_P, ARR_ALOW_I, ARR_NODES, ARR_FRONT_PARENT, ARR_Q_OFF, ARR_Q, ARR_LEVEL,
+ ARR_SEG_COL, ARR_SEG_NNZ) = range(17)
 
 
 class MdgpuError(RuntimeError):
@@ -50,6 +51,8 @@ _DP = C.POINTER(C.c_double)
 _SIGNATURES = {
     "mdgpu_last_error": (C.c_char_p, []),
     "mdgpu_analyze": (C.c_int, [C.c_int, _I32P, _I32P, C.POINTER(AnalyzeOpts), C.POINTER(_P)]),
+    "mdgpu_analyze_batch": (C.c_int, [C.c_int, _I32P, C.POINTER(_I32P), C.POINTER(_I32P), C.POINTER(AnalyzeOpts),
+                                      C.c_int, C.POINTER(_P)]),
     "mdgpu_symbolic_destroy": (C.c_int, [_P]),
     "mdgpu_symbolic_sizes": (C.c_int, [_P, _I64P]),
     "mdgpu_symbolic_flops": (C.c_int, [_P, _DP]),
@@ -103,6 +106,7 @@ _SIGNATURES.update({
     "mdgpu_factorize_profile": (C.c_int, [_P, _P, _P, _DP]),
     "mdgpu_solve": (C.c_int, [_P, _P, _P, _P]),
     "mdgpu_refine": (C.c_int, [_P, _P, _P, C.c_double, C.c_int, _DP, C.POINTER(C.c_int), _P]),
+    "mdgpu_refine_segment_eta": (C.c_int, [_P, _DP, C.c_int]),
     "mdgpu_logdet": (C.c_int, [_P, _DP, _P]),
     "mdgpu_export_L": (C.c_int, [_P, _I32P, _I32P, _DP]),
 })

@@ -1,7 +1,9 @@
 // C ABI: symbolic analysis entry points (see include/mdgpu_c.h).
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <new>
+#include <thread>
 
 #include "common.h"
 
@@ -22,6 +24,10 @@ using mdgpu::fail;
 // levels (height above the leaves) and the largest front
 static void finish_handle(mdgpu_symbolic* h) {
     const auto& S = h->S;
+    if (h->seg_col.empty()) {
+        h->seg_col = {0, S.n};
+        h->seg_nnz = {0, S.nnz_in};
+    }
     const int nn = (int)S.nodes.size();
     h->level.assign(nn, 0);
     for (int f = 0; f < nn; f++) {
@@ -100,6 +106,73 @@ extern "C" int mdgpu_analyze_nodes(int n, const int32_t* Cp, const int32_t* Ci, 
     return MDGPU_OK;
 }
 
+extern "C" int mdgpu_analyze_batch(int nmat, const int32_t* n, const int32_t* const* Cp, const int32_t* const* Ci,
+                                   const mdgpu_analyze_opts* opts, int nthreads, mdgpu_symbolic** sym) {
+    mdgpu::clear_error();
+    if (!n || !Cp || !Ci || !sym || nmat <= 0) return fail(MDGPU_ERR_ARG, "mdgpu_analyze_batch: invalid argument");
+    *sym = nullptr;
+    mdgpu::AnalyzeOptions o;
+    if (opts) {
+        o.use_amd = opts->use_amd != 0;
+        if (opts->partition < MDGPU_PART_PLAIN || opts->partition > MDGPU_PART_RELAXED)
+            return fail(MDGPU_ERR_ARG, "mdgpu_analyze_batch: unknown partition mode");
+        o.partition = (mdgpu::Partition)opts->partition;
+        o.max_zeros = opts->max_zeros;
+        if (o.max_zeros < 0) return fail(MDGPU_ERR_ARG, "mdgpu_analyze_batch: max_zeros must be >= 0");
+    }
+    int64_t ntot = 0, nnztot = 0;
+    for (int k = 0; k < nmat; k++) {
+        if (!Cp[k] || !Ci[k] || n[k] <= 0) return fail(MDGPU_ERR_ARG, "mdgpu_analyze_batch: invalid matrix");
+        ntot += n[k];
+        nnztot += Cp[k][n[k]];
+    }
+    if (ntot > INT32_MAX || nnztot > INT32_MAX)
+        return fail(MDGPU_ERR_ARG, "mdgpu_analyze_batch: batch exceeds int32 columns or entries");
+    std::vector<mdgpu::Symbolic> parts(nmat);
+    std::vector<std::string> errs(nmat);
+    // independent analyses on a pool of host threads (each is the
+    // single-matrix analysis, so each part is bit-exact on its own)
+    std::atomic<int> next(0);
+    bool oom = false;
+    auto work = [&]() {
+        for (int k; (k = next.fetch_add(1)) < nmat;) {
+            try {
+                errs[k] = mdgpu::analyze(n[k], Cp[k], Ci[k], o, parts[k]);
+            } catch (const std::bad_alloc&) {
+                errs[k] = "out of host memory";
+                oom = true;
+            }
+        }
+    };
+    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min(nt, nmat);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    for (int k = 0; k < nmat; k++)
+        if (!errs[k].empty())
+            return fail(oom ? MDGPU_ERR_NOMEM : MDGPU_ERR_ARG,
+                        "mdgpu_analyze_batch: matrix " + std::to_string(k) + ": " + errs[k]);
+    mdgpu_symbolic* h = new (std::nothrow) mdgpu_symbolic();
+    if (!h) return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze_batch: out of host memory");
+    try {
+        mdgpu::merge_symbolic(parts, h->S);
+    } catch (const std::bad_alloc&) {
+        delete h;
+        return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze_batch: out of host memory");
+    }
+    h->seg_col.assign(1, 0);
+    h->seg_nnz.assign(1, 0);
+    for (int k = 0; k < nmat; k++) {
+        h->seg_col.push_back(h->seg_col.back() + n[k]);
+        h->seg_nnz.push_back(h->seg_nnz.back() + Cp[k][n[k]]);
+    }
+    finish_handle(h);
+    *sym = h;
+    return MDGPU_OK;
+}
+
 extern "C" int mdgpu_symbolic_destroy(mdgpu_symbolic* sym) {
     delete sym;
     return MDGPU_OK;
@@ -122,6 +195,7 @@ extern "C" int mdgpu_symbolic_sizes(const mdgpu_symbolic* sym, int64_t* out) {
     out[MDGPU_SYM_MAXFRONT] = sym->max_front;
     out[MDGPU_SYM_NLEVELS] = sym->nlevels;
     out[MDGPU_SYM_NODES_FLAT] = nodes_flat_len(S);
+    out[MDGPU_SYM_NSEG] = (int64_t)sym->seg_col.size() - 1;
     return MDGPU_OK;
 }
 
@@ -173,6 +247,8 @@ extern "C" int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_
         case MDGPU_ARR_FRONT_PARENT: return copy_out(S.front_parent, dst, cap, "front_parent");
         case MDGPU_ARR_Q_OFF: return copy_out(S.q_off, dst, cap, "q_off");
         case MDGPU_ARR_Q: return copy_out(S.q_all, dst, cap, "q");
+        case MDGPU_ARR_SEG_COL: return copy_out(sym->seg_col, dst, cap, "seg_col");
+        case MDGPU_ARR_SEG_NNZ: return copy_out(sym->seg_nnz, dst, cap, "seg_nnz");
         case MDGPU_ARR_LEVEL: return copy_out(sym->level, dst, cap, "level");
         default: return fail(MDGPU_ERR_ARG, "mdgpu_symbolic_array: unknown array id");
     }

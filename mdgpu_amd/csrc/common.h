@@ -20,4 +20,8 @@ struct mdgpu_symbolic {
     std::vector<int> level;      // height of each front (leaves 0)
     int nlevels = 0;
     int max_front = 0;
+    // independent matrices of a batch (one segment for a single matrix):
+    // columns [seg_col[k], seg_col[k+1]) and caller entries
+    // [seg_nnz[k], seg_nnz[k+1]) belong to matrix k
+    std::vector<int> seg_col, seg_nnz;
 };

@@ -57,6 +57,8 @@ void launch_residual(const int* rp, const int* ci, const double* av, const doubl
                      int n, hipStream_t s);
 void launch_rowabs(const int* rp, const double* av, double* out, int n, hipStream_t s);
 void launch_maxabs(const double* v, int n, unsigned long long* out, hipStream_t s);
+// out[seg[i]] = max |v[i]| per contiguous segment (out cleared by the caller)
+void launch_maxabs_seg(const double* v, const int* seg, int n, unsigned long long* out, hipStream_t s);
 void launch_axpy(double* x, const double* d, int n, hipStream_t s);
 template <typename T>
 void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial, int nparts, hipStream_t s);

@@ -896,13 +896,38 @@ __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __re
     }
 }
 
+// Sum over the 64 lanes of a wavefront of 64 values per lane: on return,
+// acc[0] of lane l holds sum over lanes of acc[l] (column l).  A transpose
+// reduction: at each of 6 steps every lane keeps one half of its remaining
+// values and adds its partner's copy of that half (63 shuffles in all).
+__device__ __forceinline__ void wave_transpose_sum64(double (&acc)[64], int lane) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        const bool upper = (lane & s) != 0;
+#pragma unroll
+        for (int k = 0; k < s; ++k) {
+            double lo = acc[k], hi = acc[k + s];
+            // the empty asm keeps the selects on values: folded into a
+            // select of addresses, they would force acc[] into scratch
+            asm volatile("" : "+v"(lo), "+v"(hi));
+            const double send = upper ? lo : hi;
+            const double keep = upper ? hi : lo;
+            acc[k] = keep + __shfl_xor(send, s, 64);
+        }
+    }
+}
+
 // Backward, one 256-column block of a large front per launch.  Each
-// workgroup forms partial dot products of the block's columns with x over
-// its 256 rows below the block (64x64 tiles staged through LDS); the last
-// workgroup of the group to arrive (atomic counter; none waits) reduces the
-// partials in slot order, solves the block's own triangle from the last
-// 64-column sub-block (dots with the block rows already solved, then
-// inv(L_s)^T), stores x and resets the counter for the next launch.
+// workgroup forms the dot products of the block's columns with x over its
+// kSolveBwdRows rows below the block: wavefront w takes columns
+// [64w, 64w+64), its lanes run down the rows (coalesced column reads, all
+// 64 loads of a row chunk in flight), and a transpose reduction sums each
+// column over the lanes.  The last workgroup of the group to arrive (atomic
+// counter; none waits) reduces the partials in slot order, then solves the
+// block's triangle from its last 64-column sub-block: dot products with the
+// block rows already solved (wavefront 0, same pattern), then
+// x_s = inv(L_s)^T v_s with the inverse staged in LDS; it stores x and
+// resets the counter for the next launch.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
@@ -911,40 +936,39 @@ __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __re
                                                            const double* __restrict__ Y,
                                                            double* __restrict__ partial,
                                                            int* __restrict__ counters) {
-    __shared__ T tile[64 * 65];
-    __shared__ double xq[64];
-    __shared__ double red[4][kSolveBW];
+    __shared__ double W[64 * 65];
     __shared__ double vb[kSolveBW];
     __shared__ int last;
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
     const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
     const T* __restrict__ P = L + d.lofs;
-    const int tid = threadIdx.x, r = tid & 63, cq = tid >> 6;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int r0 = it.a; r0 < min(it.a + kSolveBwdRows, n); r0 += 64) {
-        const int rows = min(64, n - r0);
-        __syncthreads();
-        if (tid < 64) xq[r] = r < rows ? xp[qall[d.qofs + r0 + r]] : 0.0;
-        for (int c4 = 0; c4 < 4; ++c4) {
-            const int cbase = c4 * 64;
-            if (cbase >= bw) break;
-            __syncthreads();
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    {
+        const int c0 = 64 * wave;                         // this wavefront's columns
+        double acc[64];
 #pragma unroll
-            for (int cc = cq; cc < 64; cc += 4)
-                tile[cc * 65 + r] = (r < rows && cbase + cc < bw) ? P[(r0 + r) + (int64_t)(j0 + cbase + cc) * n] : T(0);
-            __syncthreads();
-            double sdot = 0.0;
+        for (int k = 0; k < 64; ++k) acc[k] = 0.0;
+        if (c0 < bw) {
+            const int cmax = bw - c0 - 1;                  // clamp: columns past bw read column bw-1
+            for (int r0 = it.a; r0 < min(it.a + kSolveBwdRows, n); r0 += 64) {
+                const int r = r0 + lane;
+                const bool ok = r < n;
+                const double xv = ok ? xp[qall[d.qofs + r]] : 0.0;
+                const T* __restrict__ col = P + (ok ? r : r0) + (int64_t)(j0 + c0) * n;
 #pragma unroll
-            for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) sdot += to_d(tile[r * 65 + rr]) * xq[rr];
-            acc[c4] += sdot;
+                for (int h = 0; h < 64; h += 32) {     // 32 loads in flight per half
+                    T v[32];
+#pragma unroll
+                    for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, cmax) * n];
+#pragma unroll
+                    for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
+                }
+            }
         }
+        wave_transpose_sum64(acc, lane);
+        if (c0 + lane < bw) partial[(int64_t)it.slot * kSolveBW + c0 + lane] = acc[0];
     }
-    __syncthreads();
-#pragma unroll
-    for (int c4 = 0; c4 < 4; ++c4) red[cq][c4 * 64 + r] = acc[c4];
-    __syncthreads();
-    if (tid < bw) partial[(int64_t)it.slot * kSolveBW + tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
     __threadfence();
     __syncthreads();
     if (tid == 0) last = atomicAdd(&counters[it.group], 1) == it.gcount - 1;
@@ -961,29 +985,39 @@ __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __re
     const int nsub = (bw + 63) / 64;
     for (int sb = nsub - 1; sb >= 0; --sb) {
         const int s0 = sb * 64, s1 = min(s0 + 64, bw), b = s1 - s0;
-        // v_j -= sum over the solved block rows [s1, bw) of L(row, j) x_row,
-        // through 64x64 tiles staged in LDS (coalesced column reads)
-        double part = 0.0;
-        for (int t0 = s1; t0 < bw; t0 += 64) {
-            const int rows = min(64, bw - t0);
-            __syncthreads();
-            if (tid < 64) xq[r] = r < rows ? vb[t0 + r] : 0.0;
-#pragma unroll
-            for (int cc = cq; cc < 64; cc += 4)
-                tile[cc * 65 + r] = (r < rows && cc < b) ? P[(j0 + t0 + r) + (int64_t)(j0 + s0 + cc) * n] : T(0);
-            __syncthreads();
-#pragma unroll
-            for (int rr = cq * 16; rr < cq * 16 + 16; ++rr) part += to_d(tile[r * 65 + rr]) * xq[rr];
+        // stage inv(L_s): W[l + 65 r] = inv(L_s)(r, l)
+        const T* __restrict__ w = inv + d.dinv + (int64_t)((j0 + s0) / 64) * 4096;
+        for (int t = tid; t < 64 * 64; t += kThreads) {
+            const int l = t & 63, r = t >> 6;
+            W[l + 65 * r] = (l < b && r < b) ? to_d(w[l + 64 * r]) : 0.0;
         }
-        red[cq][r] = part;
-        __syncthreads();
-        if (tid < b) vb[s0 + tid] -= red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        // v_j -= sum over the solved block rows [s1, bw) of L(row, j) x_row
+        if (wave == 0 && s1 < bw) {
+            double acc[64];
+#pragma unroll
+            for (int k = 0; k < 64; ++k) acc[k] = 0.0;
+            for (int t0 = s1; t0 < bw; t0 += 64) {
+                const int r = t0 + lane;
+                const bool ok = r < bw;
+                const double xv = ok ? vb[r] : 0.0;
+                const T* __restrict__ col = P + (j0 + (ok ? r : t0)) + (int64_t)(j0 + s0) * n;
+#pragma unroll
+                for (int h = 0; h < 64; h += 32) {
+                    T v[32];
+#pragma unroll
+                    for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, b - 1) * n];
+#pragma unroll
+                    for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
+                }
+            }
+            wave_transpose_sum64(acc, lane);
+            if (lane < b) vb[s0 + lane] -= acc[0];
+        }
         __syncthreads();
         double x = 0.0;
         if (tid < b) {
-            const T* __restrict__ w = inv + d.dinv + (int64_t)((j0 + s0) / 64) * 4096;
 #pragma unroll 8
-            for (int rr = tid; rr < b; ++rr) x += to_d(w[tid + 64 * rr]) * vb[s0 + rr];
+            for (int r = tid; r < b; ++r) x += W[tid + 65 * r] * vb[s0 + r];
         }
         __syncthreads();
         if (tid < b) {
@@ -1021,6 +1055,34 @@ __global__ void rowabs_kernel(const int* __restrict__ rp, const double* __restri
     double s = 0.0;
     for (int p = rp[i]; p < rp[i + 1]; ++p) s += fabs(av[p]);
     out[i] = s;
+}
+
+// per-segment max |v| (seg[i] = segment of entry i, segments contiguous):
+// a wavefront whose entries share one segment reduces first, else each lane
+// goes to its own segment; integer max on the bit patterns of non-negative
+// doubles (NaN counts as +inf)
+__global__ __launch_bounds__(kThreads) void maxabs_seg_kernel(const double* __restrict__ v, const int* __restrict__ seg,
+                                                              int n, unsigned long long* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    for (int base = (blockIdx.x * blockDim.x + threadIdx.x) - lane; base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        double a = 0.0;
+        int sg = -1;
+        if (i < n) {
+            a = fabs(v[i]);
+            if (a != a) a = __builtin_huge_val();
+            sg = seg[i];
+        }
+        const int last = min(base + 63, n - 1);
+        const int s0 = __shfl(sg, 0, 64), s1 = seg[last];
+        if (s0 == s1) {
+            double m = a;
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+            if (lane == 0) atomicMax(&out[s0], (unsigned long long)__double_as_longlong(m));
+        } else if (i < n) {
+            atomicMax(&out[sg], (unsigned long long)__double_as_longlong(a));
+        }
+    }
 }
 
 // max |v| via integer max on the bit patterns of non-negative doubles
@@ -1162,6 +1224,12 @@ void launch_maxabs(const double* v, int n, unsigned long long* out, hipStream_t 
     if (blocks > 1024) blocks = 1024;
     if (blocks < 1) blocks = 1;
     maxabs_kernel<<<blocks, kThreads, 0, s>>>(v, n, out);
+}
+void launch_maxabs_seg(const double* v, const int* seg, int n, unsigned long long* out, hipStream_t s) {
+    int blocks = blocks_for(n);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    maxabs_seg_kernel<<<blocks, kThreads, 0, s>>>(v, seg, n, out);
 }
 void launch_axpy(double* x, const double* d, int n, hipStream_t s) {
     axpy_kernel<<<blocks_for(n), kThreads, 0, s>>>(x, d, n);

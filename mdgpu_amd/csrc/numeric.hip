@@ -64,7 +64,9 @@ struct mdgpu_numeric {
     double* dRowabs = nullptr;
     int64_t* dDiagOff = nullptr;
     int* dInfo = nullptr;
-    unsigned long long* dMax = nullptr;
+    unsigned long long* dMax = nullptr;   // one slot per segment
+    int nseg = 1;
+    int* dSeg = nullptr;                  // row -> segment (batches only)
     double* dPartial = nullptr;
     int nparts = 0;
     int tile = 64;
@@ -84,6 +86,8 @@ struct mdgpu_numeric {
     int* dSvCounters = nullptr;
     double* dY = nullptr;
     double anorm = -1;
+    std::vector<double> anorm_seg;        // ||A_k||_1 per segment
+    std::vector<double> eta_seg;          // backward error per segment, last refine
     bool factored = false;
     int64_t device_bytes = 0;
 };
@@ -282,17 +286,19 @@ int solve_any(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
     return MDGPU_OK;
 }
 
-int max_abs(mdgpu_numeric* N, const double* v, int n, hipStream_t s, double* out) {
-    mdgpu::launch_fill<unsigned long long>(N->dMax, 1, 0ULL, s);
-    mdgpu::launch_maxabs(v, n, N->dMax, s);
-    unsigned long long bits = 0;
-    hipError_t e = hipMemcpyAsync(&bits, N->dMax, sizeof(bits), hipMemcpyDeviceToHost, s);
+// max |v| per segment (synchronizes the stream)
+int max_abs(mdgpu_numeric* N, const double* v, int n, hipStream_t s, std::vector<double>& out) {
+    mdgpu::launch_fill<unsigned long long>(N->dMax, N->nseg, 0ULL, s);
+    if (N->nseg == 1) mdgpu::launch_maxabs(v, n, N->dMax, s);
+    else mdgpu::launch_maxabs_seg(v, N->dSeg, n, N->dMax, s);
+    std::vector<unsigned long long> bits(N->nseg, 0);
+    hipError_t e = hipMemcpyAsync(bits.data(), N->dMax, bits.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    double d;
-    memcpy(&d, &bits, sizeof d);
-    *out = d;
+    out.resize(N->nseg);
+    for (int k = 0; k < N->nseg; k++) memcpy(&out[k], &bits[k], sizeof(double));
     return MDGPU_OK;
 }
 
@@ -343,6 +349,13 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         for (int j = 0; j < d.c; j++) diag_off[d.k + j] = d.lofs + (int64_t)j * d.n + j;
     }
     N->nparts = kLogParts;
+    N->nseg = std::max(1, (int)sym->seg_col.size() - 1);
+    std::vector<int> seg_of_row;
+    if (N->nseg > 1) {
+        seg_of_row.resize(S.n);
+        for (int k = 0; k < N->nseg; k++)
+            for (int r = sym->seg_col[k]; r < sym->seg_col[k + 1]; r++) seg_of_row[r] = k;
+    }
     N->tile = po.tile;
     if (const char* g = std::getenv("MDGPU_GRAPH")) N->use_graph = std::atoi(g) != 0;
     if ((st = alloc(N, P.lstore * es, &N->dL)) ||
@@ -373,7 +386,8 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dRowabs)) ||
         (st = upload(N, diag_off, (void**)&N->dDiagOff)) ||
         (st = alloc(N, sizeof(int), (void**)&N->dInfo)) ||
-        (st = alloc(N, sizeof(unsigned long long), (void**)&N->dMax)) ||
+        (st = alloc(N, N->nseg * sizeof(unsigned long long), (void**)&N->dMax)) ||
+        (N->nseg > 1 && (st = upload(N, seg_of_row, (void**)&N->dSeg))) ||
         (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)) ||
         (st = alloc(N, std::max<size_t>(P.a_dst.size(), 1) * sizeof(double), (void**)&N->dAx)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dB)) ||
@@ -495,10 +509,9 @@ int finish_factor(mdgpu_numeric* num, const double* d_ax, hipStream_t s) {
     int info = INT_MAX;
     hipError_t e = hipMemcpyAsync(&info, num->dInfo, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
-    double an = 0;
-    int st = max_abs(num, num->dRowabs, n, s, &an);   // synchronizes the stream
+    int st = max_abs(num, num->dRowabs, n, s, num->anorm_seg);   // synchronizes the stream
     if (st != MDGPU_OK) return st;
-    num->anorm = an;
+    num->anorm = *std::max_element(num->anorm_seg.begin(), num->anorm_seg.end());
     if (info != INT_MAX)
         return fail(MDGPU_ERR_NOT_SPD, "mdgpu_factorize: non-positive pivot at permuted column " +
                                            std::to_string(info));
@@ -522,18 +535,24 @@ extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, 
     hipStream_t s = (hipStream_t)stream;
     const int n = num->sym->S.n;
     TRY(solve_any(num, d_b, d_x, s));
-    double bnorm = 0;
-    TRY(max_abs(num, d_b, n, s, &bnorm));
+    std::vector<double> bn, rn, xn;
+    TRY(max_abs(num, d_b, n, s, bn));
     double eta = INFINITY;
     int it = 0;
+    num->eta_seg.assign(num->nseg, INFINITY);
     for (;; it++) {
         mdgpu::launch_residual(num->dAfp, num->dAfi, num->dAfx, d_x, d_b, num->dR, n, s);
-        double rn = 0, xn = 0;
-        TRY(max_abs(num, num->dR, n, s, &rn));
-        TRY(max_abs(num, d_x, n, s, &xn));
-        // backward error of the reference's check (mdgpu.cu:1972)
-        const double den = num->anorm * xn + bnorm;
-        eta = den > 0 ? rn / den : rn;
+        TRY(max_abs(num, num->dR, n, s, rn));
+        TRY(max_abs(num, d_x, n, s, xn));
+        // backward error of the reference's check (mdgpu.cu:1972), per
+        // independent matrix of a batch; the worst one decides
+        eta = 0;
+        for (int k = 0; k < num->nseg; k++) {
+            const double den = num->anorm_seg[k] * xn[k] + bn[k];
+            const double e = den > 0 ? rn[k] / den : rn[k];
+            num->eta_seg[k] = e;
+            eta = (e > eta || e != e) ? e : eta;
+        }
         if (eta <= tol || it >= maxit || !(eta == eta)) break;
         TRY(solve_any(num, num->dR, num->dD, s));
         mdgpu::launch_axpy(d_x, num->dD, n, s);
@@ -545,6 +564,16 @@ extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, 
     if (!(eta <= tol))
         return fail(MDGPU_ERR_NOT_CONVERGED, "mdgpu_refine: backward error " + std::to_string(eta) +
                                                  " above tolerance after " + std::to_string(it) + " steps");
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_refine_segment_eta(const mdgpu_numeric* num, double* out, int cap) {
+    mdgpu::clear_error();
+    if (!num || !out) return fail(MDGPU_ERR_ARG, "mdgpu_refine_segment_eta: null pointer");
+    if (cap < num->nseg) return fail(MDGPU_ERR_ARG, "mdgpu_refine_segment_eta: capacity below the segment count");
+    if ((int)num->eta_seg.size() != num->nseg)
+        return fail(MDGPU_ERR_ARG, "mdgpu_refine_segment_eta: no refinement has run");
+    for (int k = 0; k < num->nseg; k++) out[k] = num->eta_seg[k];
     return MDGPU_OK;
 }
 

@@ -904,4 +904,67 @@ std::string analyze_with_nodes(int n, const int* Cp, const int* Ci, const std::v
     return "";
 }
 
+void merge_symbolic(const std::vector<Symbolic>& parts, Symbolic& out) {
+    out = Symbolic();
+    if (parts.empty()) return;
+    out.opts = parts[0].opts;
+    int n = 0, nnz_in = 0, nnodes = 0;
+    int64_t nq = 0;
+    for (const Symbolic& P : parts) {
+        n += P.n;
+        nnz_in += P.nnz_in;
+        nnodes += (int)P.nodes.size();
+        nq += (int64_t)P.q_all.size();
+    }
+    out.n = n;
+    out.nnz_in = nnz_in;
+    const bool relaxed = !parts[0].relaxed_perm.empty();
+    auto shift = [](std::vector<int>& dst, const std::vector<int>& src, int off, bool keep_neg) {
+        for (int v : src) dst.push_back(keep_neg && v < 0 ? v : v + off);
+    };
+    out.a_low.n = n;
+    out.a_low.p.assign(1, 0);
+    out.q_off.assign(1, 0);
+    out.q_all.reserve(nq);
+    out.nodes.reserve(nnodes);
+    int col = 0, ent = 0, node = 0;
+    for (const Symbolic& P : parts) {
+        shift(out.amd_perm, P.amd_perm, col, false);
+        shift(out.parent, P.parent, col, true);
+        shift(out.post, P.post, col, false);
+        out.count.insert(out.count.end(), P.count.begin(), P.count.end());
+        if (relaxed) shift(out.relaxed_perm, P.relaxed_perm, col, false);
+        shift(out.perm, P.perm, col, false);
+        shift(out.pinv, P.pinv, col, false);
+        out.count_final.insert(out.count_final.end(), P.count_final.begin(), P.count_final.end());
+        const int base = out.a_low.p.back();
+        for (int k = 1; k <= P.n; k++) out.a_low.p.push_back(base + P.a_low.p[k]);
+        shift(out.a_low.i, P.a_low.i, col, false);
+        shift(out.a_src, P.a_src, ent, false);
+        for (const PartNode& nd : P.nodes) {
+            PartNode m = nd;
+            m.first += col;
+            for (int& c : m.cols) c += col;
+            for (int& k : m.kids) k += node;
+            out.nodes.push_back(std::move(m));
+        }
+        shift(out.front_parent, P.front_parent, node, true);
+        const int64_t qbase = (int64_t)out.q_all.size();
+        for (size_t f = 1; f < P.q_off.size(); f++) out.q_off.push_back(qbase + P.q_off[f]);
+        shift(out.q_all, P.q_all, col, false);
+        out.flops_counts += P.flops_counts;
+        out.flops_padded += P.flops_padded;
+        col += P.n;
+        ent += P.nnz_in;
+        node += (int)P.nodes.size();
+    }
+    out.lnz = 0;
+    for (const Symbolic& P : parts) out.lnz += P.lnz;
+    // CSparse column pointers are int32: only when the batch's L fits
+    if (out.lnz <= INT32_MAX) {
+        out.lp.assign(n + 1, 0);
+        for (int k = 0; k < n; k++) out.lp[k + 1] = out.lp[k] + out.count_final[k];
+    }
+}
+
 }  // namespace mdgpu

@@ -101,4 +101,12 @@ Csc full_pattern(const Csc& low);
 std::string analyze_with_nodes(int n, const int* Cp, const int* Ci, const std::vector<PartNode>& nodes,
                                Symbolic& out);
 
+// Block-diagonal union of independent analyses (a batch of matrices): the
+// columns, rows, caller entries and nodes of part k are shifted by the
+// totals of parts 0..k-1, so the result is the analysis of the block
+// diagonal matrix diag(A_0, A_1, ...) whose caller values are the
+// concatenation of the parts' values.  Each part keeps its own ordering and
+// partition exactly.  The front tree is a forest.
+void merge_symbolic(const std::vector<Symbolic>& parts, Symbolic& out);
+
 }  // namespace mdgpu

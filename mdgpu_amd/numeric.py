@@ -89,6 +89,12 @@ class Factorization:
             L.check(st)
         return x, eta.value, it.value
 
+    def segment_eta(self):
+        """Backward error of each matrix of a batch after the last refine."""
+        out = np.empty(max(self.sym.nseg, 1), dtype=np.float64)
+        L.check(L.lib().mdgpu_refine_segment_eta(self._h, out.ctypes.data_as(L._DP), len(out)))
+        return out
+
     def logdet(self):
         out = C.c_double()
         L.check(L.lib().mdgpu_logdet(self._h, C.byref(out), _stream()))

@@ -35,6 +35,7 @@ class Symbolic:
         self.nnz_a = sizes[L.SYM_NNZ_A]
         self.max_front = sizes[L.SYM_MAXFRONT]
         self.nlevels = sizes[L.SYM_NLEVELS]
+        self.nseg = sizes[L.SYM_NSEG]
         fl = (C.c_double * 2)()
         L.check(L.lib().mdgpu_symbolic_flops(self._h, fl))
         self.flops, self.flops_padded = fl[0], fl[1]
@@ -49,6 +50,7 @@ class Symbolic:
             L.ARR_LP: n + 1, L.ARR_ALOW_P: n + 1, L.ARR_ALOW_I: self.nnz_a,
             L.ARR_NODES: self.sizes[L.SYM_NODES_FLAT], L.ARR_FRONT_PARENT: nn,
             L.ARR_Q_OFF: nn + 1, L.ARR_Q: self.sizes[L.SYM_QLEN], L.ARR_LEVEL: nn,
+            L.ARR_SEG_COL: self.nseg + 1, L.ARR_SEG_NNZ: self.nseg + 1,
         }.get(which, n)
         out = np.empty(max(length, 1), dtype=np.int32)
         L.check(L.lib().mdgpu_symbolic_array(self._h, which, out.ctypes.data_as(L._I32P), length))
@@ -84,3 +86,23 @@ def analyze(p, i, amd=True, mode="plain", max_zeros=0):
     L.check(L.lib().mdgpu_analyze(n, p.ctypes.data_as(L._I32P), i.ctypes.data_as(L._I32P),
                                   C.byref(opts), C.byref(h)))
     return Symbolic(h, (p, i))
+
+
+def analyze_batch(mats, amd=True, mode="plain", max_zeros=0, nthreads=0):
+    """One analysis for a batch of independent matrices ``mats`` = [(p, i), ...]
+    (lower-triangular CSC patterns): each matrix is analysed exactly as by
+    :func:`analyze`, and the results are joined block-diagonally.  Values,
+    right-hand sides and solutions of the batch are the concatenations of
+    the matrices' own; ``array(ARR_SEG_COL)`` / ``array(ARR_SEG_NNZ)`` give the
+    offsets."""
+    ps = [_i32(p) for p, _ in mats]
+    iis = [_i32(i) for _, i in mats]
+    k = len(mats)
+    ns = _i32([len(p) - 1 for p in ps])
+    PP = (L._I32P * k)(*[p.ctypes.data_as(L._I32P) for p in ps])
+    II = (L._I32P * k)(*[i.ctypes.data_as(L._I32P) for i in iis])
+    o = L.AnalyzeOpts(1 if amd else 0, _MODES[mode], int(max_zeros))
+    h = C.c_void_p()
+    L.check(L.lib().mdgpu_analyze_batch(k, ns.ctypes.data_as(L._I32P), PP, II, C.byref(o), int(nthreads),
+                                        C.byref(h)))
+    return Symbolic(h, (ps, iis, ns))

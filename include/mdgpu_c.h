@@ -202,6 +202,29 @@ int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol,
  * No counterpart in the reference. */
 int mdgpu_refine_segment_eta(const mdgpu_numeric* num, double* out, int cap);
 
+/* ---------------------------------------------------------- reverse mode
+ * None of these has a counterpart in the reference.  All need an
+ * MDGPU_FP64 factorization (mdgpu_factorize done).
+ *
+ * Selected inverse Z = A^{-1} on the pattern of L (permuted numbering),
+ * supernodal Takahashi recurrences top-down over the front tree, with the
+ * children's blocks of Z passed down by a reverse extend-add (update
+ * matrices are recomputed from the stored L, not kept).  Asynchronous. */
+int mdgpu_selinv(mdgpu_numeric* num, void* stream);
+/* Copy Z to the host in the layout of mdgpu_export_L's Lx (synchronous). */
+int mdgpu_selinv_export(mdgpu_numeric* num, double* Zx);
+/* d log det A / d a_e for every entry e of the caller's lower triangle
+ * (device vector, caller entry order): Z_ii on the diagonal, 2 Z_ij off it
+ * (A is symmetric: a_ij stands for both a_ij and a_ji).  Runs mdgpu_selinv
+ * first when Z is not current. */
+int mdgpu_logdet_grad(mdgpu_numeric* num, double* d_g, void* stream);
+/* Adjoint of x = A^{-1} b: given x and xbar = dL/dx (device, caller
+ * numbering), bbar = A^{-1} xbar (dL/db; d_bbar may be NULL) and
+ * d_g[e] = dL/da_e = -(bbar_i x_j + bbar_j x_i) off the diagonal,
+ * -bbar_i x_i on it. */
+int mdgpu_solve_adjoint(mdgpu_numeric* num, const double* d_x, const double* d_xbar, double* d_bbar,
+                        double* d_g, void* stream);
+
 /* log det A = 2 sum_j log L_jj (no counterpart in the reference). */
 int mdgpu_logdet(mdgpu_numeric* num, double* out, void* stream);
 

@@ -109,4 +109,8 @@ _SIGNATURES.update({
     "mdgpu_refine_segment_eta": (C.c_int, [_P, _DP, C.c_int]),
     "mdgpu_logdet": (C.c_int, [_P, _DP, _P]),
     "mdgpu_export_L": (C.c_int, [_P, _I32P, _I32P, _DP]),
+    "mdgpu_selinv": (C.c_int, [_P, _P]),
+    "mdgpu_selinv_export": (C.c_int, [_P, _DP]),
+    "mdgpu_logdet_grad": (C.c_int, [_P, _P, _P]),
+    "mdgpu_solve_adjoint": (C.c_int, [_P, _P, _P, _P, _P, _P]),
 })

@@ -48,8 +48,9 @@ template <typename T>
 void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
                     double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s);
 template <typename T>
-void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
-                    const T* inv, double* xp, const double* Y, double* partial, int* counters, hipStream_t s);
+void launch_bwd_blk(const SolveItem* items, int count, const SolveItem* fin, int nfin, const FrontDesc* fronts,
+                    const int* q, const T* L, const T* inv, double* xp, const double* Y, double* partial,
+                    hipStream_t s);
 
 // refinement helpers (FP64)
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);

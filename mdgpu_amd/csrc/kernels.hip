@@ -917,64 +917,68 @@ __device__ __forceinline__ void wave_transpose_sum64(double (&acc)[64], int lane
     }
 }
 
-// Backward, one 256-column block of a large front per launch.  Each
-// workgroup forms the dot products of the block's columns with x over its
-// kSolveBwdRows rows below the block: wavefront w takes columns
-// [64w, 64w+64), its lanes run down the rows (coalesced column reads, all
-// 64 loads of a row chunk in flight), and a transpose reduction sums each
-// column over the lanes.  The last workgroup of the group to arrive (atomic
-// counter; none waits) reduces the partials in slot order, then solves the
-// block's triangle from its last 64-column sub-block: dot products with the
-// block rows already solved (wavefront 0, same pattern), then
-// x_s = inv(L_s)^T v_s with the inverse staged in LDS; it stores x and
-// resets the counter for the next launch.
+// Backward, one 256-column block of a large front: the partial dot
+// products.  Each workgroup forms the dot products of the block's columns
+// with x over its kSolveBwdRows rows below the block: wavefront w takes
+// columns [64w, 64w+64), its lanes run down the rows (coalesced column
+// reads, 32 loads in flight per lane), and a transpose reduction sums each
+// column over the lanes.  One partial vector per workgroup (slot).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __restrict__ items,
+__global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
                                                            const int* __restrict__ qall, const T* __restrict__ L,
-                                                           const T* __restrict__ inv, double* __restrict__ xp,
-                                                           const double* __restrict__ Y,
-                                                           double* __restrict__ partial,
-                                                           int* __restrict__ counters) {
-    __shared__ double W[64 * 65];
-    __shared__ double vb[kSolveBW];
-    __shared__ int last;
+                                                           const double* __restrict__ xp,
+                                                           double* __restrict__ partial) {
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
     const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
     const T* __restrict__ P = L + d.lofs;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    {
-        const int c0 = 64 * wave;                         // this wavefront's columns
-        double acc[64];
+    const int c0 = 64 * wave;                         // this wavefront's columns
+    if (c0 >= bw) return;
+    double acc[64];
 #pragma unroll
-        for (int k = 0; k < 64; ++k) acc[k] = 0.0;
-        if (c0 < bw) {
-            const int cmax = bw - c0 - 1;                  // clamp: columns past bw read column bw-1
-            for (int r0 = it.a; r0 < min(it.a + kSolveBwdRows, n); r0 += 64) {
-                const int r = r0 + lane;
-                const bool ok = r < n;
-                const double xv = ok ? xp[qall[d.qofs + r]] : 0.0;
-                const T* __restrict__ col = P + (ok ? r : r0) + (int64_t)(j0 + c0) * n;
+    for (int k = 0; k < 64; ++k) acc[k] = 0.0;
+    const int cmax = bw - c0 - 1;                      // clamp: columns past bw read column bw-1
+    for (int r0 = it.a; r0 < min(it.a + kSolveBwdRows, n); r0 += 64) {
+        const int r = r0 + lane;
+        const bool ok = r < n;
+        const double xv = ok ? xp[qall[d.qofs + r]] : 0.0;
+        const T* __restrict__ col = P + (ok ? r : r0) + (int64_t)(j0 + c0) * n;
 #pragma unroll
-                for (int h = 0; h < 64; h += 32) {     // 32 loads in flight per half
-                    T v[32];
+        for (int h = 0; h < 64; h += 32) {
+            T v[32];
 #pragma unroll
-                    for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, cmax) * n];
+            for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, cmax) * n];
 #pragma unroll
-                    for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
-                }
-            }
+            for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
         }
-        wave_transpose_sum64(acc, lane);
-        if (c0 + lane < bw) partial[(int64_t)it.slot * kSolveBW + c0 + lane] = acc[0];
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(&counters[it.group], 1) == it.gcount - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    wave_transpose_sum64(acc, lane);
+    if (c0 + lane < bw) partial[(int64_t)it.slot * kSolveBW + c0 + lane] = acc[0];
+}
+
+// Backward, one 256-column block of a large front: the finish, one
+// workgroup per block.  Adds the partial vectors in slot order to
+// v = y - (dot products with the rows below), then solves the block's
+// triangle from its last 64-column sub-block: the dot products with the
+// block rows already solved (one 64-row chunk per wavefront, transpose
+// reduction, chunks added in order), then x_s = inv(L_s)^T v_s with the
+// inverse staged in LDS.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void bwd_fin_kernel(const SolveItem* __restrict__ items,
+                                                           const FrontDesc* __restrict__ fronts,
+                                                           const T* __restrict__ L, const T* __restrict__ inv,
+                                                           double* __restrict__ xp, const double* __restrict__ Y,
+                                                           const double* __restrict__ partial) {
+    __shared__ double W[64 * 65];
+    __shared__ double vb[kSolveBW];
+    __shared__ double red[4][64];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDesc d = fronts[it.front];
+    const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
+    const T* __restrict__ P = L + d.lofs;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid < bw) {
         double v = Y[d.k + j0 + tid];
 #pragma unroll 8
@@ -991,28 +995,30 @@ __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __re
             const int l = t & 63, r = t >> 6;
             W[l + 65 * r] = (l < b && r < b) ? to_d(w[l + 64 * r]) : 0.0;
         }
-        // v_j -= sum over the solved block rows [s1, bw) of L(row, j) x_row
-        if (wave == 0 && s1 < bw) {
-            double acc[64];
+        // v_j -= sum over the solved block rows [s1, bw) of L(row, j) x_row:
+        // wavefront w takes rows [s1 + 64w, s1 + 64w + 64)
+        const int t0 = s1 + 64 * wave;
+        double acc[64];
 #pragma unroll
-            for (int k = 0; k < 64; ++k) acc[k] = 0.0;
-            for (int t0 = s1; t0 < bw; t0 += 64) {
-                const int r = t0 + lane;
-                const bool ok = r < bw;
-                const double xv = ok ? vb[r] : 0.0;
-                const T* __restrict__ col = P + (j0 + (ok ? r : t0)) + (int64_t)(j0 + s0) * n;
+        for (int k = 0; k < 64; ++k) acc[k] = 0.0;
+        if (t0 < bw) {
+            const int r = t0 + lane;
+            const bool ok = r < bw;
+            const double xv = ok ? vb[r] : 0.0;
+            const T* __restrict__ col = P + (j0 + (ok ? r : t0)) + (int64_t)(j0 + s0) * n;
 #pragma unroll
-                for (int h = 0; h < 64; h += 32) {
-                    T v[32];
+            for (int h = 0; h < 64; h += 32) {
+                T v[32];
 #pragma unroll
-                    for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, b - 1) * n];
+                for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, b - 1) * n];
 #pragma unroll
-                    for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
-                }
+                for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
             }
-            wave_transpose_sum64(acc, lane);
-            if (lane < b) vb[s0 + lane] -= acc[0];
         }
+        wave_transpose_sum64(acc, lane);
+        red[wave][lane] = acc[0];
+        __syncthreads();
+        if (tid < b) vb[s0 + tid] -= ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
         __syncthreads();
         double x = 0.0;
         if (tid < b) {
@@ -1026,7 +1032,6 @@ __global__ __launch_bounds__(kThreads) void bwd_blk_kernel(const SolveItem* __re
         }
         __syncthreads();
     }
-    if (tid == 0) counters[it.group] = 0;
 }
 
 // -------------------------------------------------------------- refinement
@@ -1203,10 +1208,11 @@ void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, 
     if (count > 0) fwd_blk_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, inv, xp, Y, su, suofs);
 }
 template <typename T>
-void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
-                    const T* inv, double* xp, const double* Y, double* partial, int* counters, hipStream_t s) {
-    if (count > 0)
-        bwd_blk_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, inv, xp, Y, partial, counters);
+void launch_bwd_blk(const SolveItem* items, int count, const SolveItem* fin, int nfin, const FrontDesc* fronts,
+                    const int* q, const T* L, const T* inv, double* xp, const double* Y, double* partial,
+                    hipStream_t s) {
+    if (count > 0) bwd_dot_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, xp, partial);
+    if (nfin > 0) bwd_fin_kernel<T><<<nfin, kThreads, 0, s>>>(fin, fronts, L, inv, xp, Y, partial);
 }
 
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
@@ -1254,8 +1260,8 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);               \
     template void launch_fwd_blk<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
                                     double*, double*, const int64_t*, hipStream_t);                          \
-    template void launch_bwd_blk<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,          \
-                                    const T*, double*, const double*, double*, int*, hipStream_t);
+    template void launch_bwd_blk<T>(const SolveItem*, int, const SolveItem*, int, const FrontDesc*, const int*, \
+                                    const T*, const T*, double*, const double*, double*, hipStream_t);
 
 MDGPU_INSTANTIATE(double)
 MDGPU_INSTANTIATE(float)

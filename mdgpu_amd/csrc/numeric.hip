@@ -16,6 +16,8 @@
 #include "common.h"
 #include "kernels.h"
 #include "plan.h"
+#include "selinv.h"
+#include "selinv_kernels.h"
 
 using mdgpu::fail;
 
@@ -83,13 +85,25 @@ struct mdgpu_numeric {
     int* dSvFronts = nullptr;
     mdgpu::SolveItem* dSvItems = nullptr;
     double* dSvPartial = nullptr;
-    int* dSvCounters = nullptr;
+    mdgpu::SolveItem* dSvFin = nullptr;
     double* dY = nullptr;
     double anorm = -1;
     std::vector<double> anorm_seg;        // ||A_k||_1 per segment
     std::vector<double> eta_seg;          // backward error per segment, last refine
     bool factored = false;
     int64_t device_bytes = 0;
+    // reverse mode (built on first use, FP64 only)
+    bool sel_ready = false, z_valid = false;
+    mdgpu::SelPlan sel;
+    double* dZ = nullptr;                 // selected inverse, L-store layout
+    double* dZF = nullptr;                // dense per-level work matrices
+    double* dLH = nullptr;                // L(R, J) inv(L(J, J)) per front
+    mdgpu::SelGemm* dSelGemm = nullptr;
+    mdgpu::SelCopy* dSelCopy = nullptr;
+    int64_t* dZfOfs = nullptr;
+    int* dArow = nullptr;                 // caller entry -> row, column (caller numbering)
+    int* dAcol = nullptr;
+    double* dBbar = nullptr;
 };
 
 namespace {
@@ -256,8 +270,8 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
                 mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, inv, N->dXp, N->dY, s);
                 break;
             case mdgpu::SV_BWD_BLK:
-                mdgpu::launch_bwd_blk<T>(items, st.count, N->dFronts, N->dQ, L, inv, N->dXp, N->dY, N->dSvPartial,
-                                         N->dSvCounters, s);
+                mdgpu::launch_bwd_blk<T>(items, st.count, N->dSvFin + st.first2, st.count2, N->dFronts, N->dQ, L, inv,
+                                         N->dXp, N->dY, N->dSvPartial, s);
                 break;
         }
     }
@@ -395,7 +409,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.sv_fronts, (void**)&N->dSvFronts)) ||
         (st = upload(N, P.sv_items, (void**)&N->dSvItems)) ||
         (st = alloc(N, P.sv_partial * sizeof(double), (void**)&N->dSvPartial)) ||
-        (st = upload(N, std::vector<int>(P.sv_groups, 0), (void**)&N->dSvCounters)) ||
+        (st = upload(N, P.sv_fin, (void**)&N->dSvFin)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dY)))
         return guard(st);
     if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess)
@@ -502,6 +516,7 @@ extern "C" int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, v
 
 namespace {
 int finish_factor(mdgpu_numeric* num, const double* d_ax, hipStream_t s) {
+    num->z_valid = false;
     // full symmetric values and ||A||_1 for the residual of refinement
     const int n = num->sym->S.n;
     mdgpu::launch_gather(d_ax, num->dAfsrc, num->dAfx, (int)num->plan.afull_i.size(), s);
@@ -630,4 +645,114 @@ extern "C" int mdgpu_export_L(mdgpu_numeric* num, int32_t* Lp, int32_t* Li, doub
         }
     }
     return MDGPU_OK;
+}
+
+// ------------------------------------------------------------ reverse mode
+namespace {
+int selinv_setup(mdgpu_numeric* N) {
+    if (N->sel_ready) return MDGPU_OK;
+    const mdgpu::Plan& P = N->plan;
+    const mdgpu::Symbolic& S = N->sym->S;
+    std::string err = mdgpu::build_selinv_plan(P, N->sel);
+    if (err.empty()) err = mdgpu::validate_selinv_plan(P, N->sel);
+    if (!err.empty()) return fail(MDGPU_ERR_ARG, "selected inverse schedule: " + err);
+    // caller entries: row and column in the caller's numbering
+    std::vector<int> arow(S.nnz_in), acol(S.nnz_in);
+    for (int j = 0; j < S.n; j++)
+        for (int t = S.a_low.p[j]; t < S.a_low.p[j + 1]; t++) {
+            arow[S.a_src[t]] = S.perm[S.a_low.i[t]];
+            acol[S.a_src[t]] = S.perm[j];
+        }
+    int st;
+    if ((st = alloc(N, P.lstore * sizeof(double), (void**)&N->dZ)) ||
+        (st = alloc(N, N->sel.zf_size * sizeof(double), (void**)&N->dZF)) ||
+        (st = alloc(N, N->sel.lh_size * sizeof(double), (void**)&N->dLH)) ||
+        (st = upload(N, N->sel.gemm, (void**)&N->dSelGemm)) ||
+        (st = upload(N, N->sel.copy, (void**)&N->dSelCopy)) ||
+        (st = upload(N, N->sel.zf_ofs, (void**)&N->dZfOfs)) ||
+        (st = upload(N, arow, (void**)&N->dArow)) ||
+        (st = upload(N, acol, (void**)&N->dAcol)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dBbar)))
+        return st;
+    N->sel_ready = true;
+    return MDGPU_OK;
+}
+
+int check_fp64_factor(const mdgpu_numeric* num, const char* who) {
+    if (!num) return fail(MDGPU_ERR_ARG, std::string(who) + ": null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, std::string(who) + ": no valid factorization");
+    if (num->precision != MDGPU_FP64)
+        return fail(MDGPU_ERR_ARG, std::string(who) + ": the reverse mode needs an MDGPU_FP64 factorization");
+    return MDGPU_OK;
+}
+}  // namespace
+
+extern "C" int mdgpu_selinv(mdgpu_numeric* num, void* stream) {
+    mdgpu::clear_error();
+    TRY(check_fp64_factor(num, "mdgpu_selinv"));
+    TRY(selinv_setup(num));
+    hipStream_t s = (hipStream_t)stream;
+    mdgpu::SelBuffers b;
+    b.p[mdgpu::SB_L] = (double*)num->dL;
+    b.p[mdgpu::SB_Z] = num->dZ;
+    b.p[mdgpu::SB_ZF] = num->dZF;
+    b.p[mdgpu::SB_LH] = num->dLH;
+    b.p[mdgpu::SB_W] = (double*)num->dInv;
+    b.p[mdgpu::SB_U] = (double*)num->dU;
+    for (const mdgpu::SelStep& st : num->sel.steps) {
+        if (st.kind == mdgpu::SS_GEMM)
+            mdgpu::launch_sel_gemm(num->dSelGemm + st.first, st.count, st.nblocks, b, s);
+        else
+            mdgpu::launch_sel_copy(st.copy_kind, num->dSelCopy + st.first, st.count, num->dFronts, num->dRel,
+                                   num->dZfOfs, b, s);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "selected inverse launch");
+    num->z_valid = true;
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_selinv_export(mdgpu_numeric* num, double* Zx) {
+    mdgpu::clear_error();
+    TRY(check_fp64_factor(num, "mdgpu_selinv_export"));
+    if (!num->z_valid) return fail(MDGPU_ERR_ARG, "mdgpu_selinv_export: run mdgpu_selinv first");
+    if (!Zx) return fail(MDGPU_ERR_ARG, "mdgpu_selinv_export: null pointer");
+    const mdgpu::Symbolic& S = num->sym->S;
+    const mdgpu::Plan& P = num->plan;
+    std::vector<double> h(P.lstore);
+    hipError_t e = hipMemcpy(h.data(), num->dZ, P.lstore * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy Z");
+    for (int f = 0; f < P.nfronts; f++) {
+        const mdgpu::FrontDesc& d = P.fronts[f];
+        for (int j = 0; j < d.c; j++) {
+            int64_t dst = S.lp[d.k + j];
+            for (int i = j; i < d.n; i++, dst++) Zx[dst] = h[d.lofs + (int64_t)j * d.n + i];
+        }
+    }
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_logdet_grad(mdgpu_numeric* num, double* d_g, void* stream) {
+    mdgpu::clear_error();
+    TRY(check_fp64_factor(num, "mdgpu_logdet_grad"));
+    if (!d_g) return fail(MDGPU_ERR_ARG, "mdgpu_logdet_grad: null pointer");
+    if (!num->z_valid) TRY(mdgpu_selinv(num, stream));
+    mdgpu::launch_logdet_grad(num->dZ, num->dAdst, num->dArow, num->dAcol, d_g, num->sym->S.nnz_in,
+                              (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MDGPU_OK : hip_fail(e, "logdet gradient launch");
+}
+
+extern "C" int mdgpu_solve_adjoint(mdgpu_numeric* num, const double* d_x, const double* d_xbar, double* d_bbar,
+                                   double* d_g, void* stream) {
+    mdgpu::clear_error();
+    TRY(check_fp64_factor(num, "mdgpu_solve_adjoint"));
+    if (!d_x || !d_xbar || !d_g) return fail(MDGPU_ERR_ARG, "mdgpu_solve_adjoint: null pointer");
+    TRY(selinv_setup(num));
+    hipStream_t s = (hipStream_t)stream;
+    double* bb = d_bbar ? d_bbar : num->dBbar;
+    TRY(solve_any(num, d_xbar, bb, s));
+    mdgpu::launch_solve_adjoint(d_x, bb, num->dArow, num->dAcol, d_g, num->sym->S.nnz_in, s);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MDGPU_OK : hip_fail(e, "solve adjoint launch");
 }

@@ -223,6 +223,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             for (int blk = nblk - 1; blk >= 0; blk--) {
                 const int j0 = blk * BW;
                 SolveStep st{SV_BWD_BLK, (int)P.sv_items.size(), 0, 0};
+                st.first2 = (int)P.sv_fin.size();
                 int slot = 0, group = 0;
                 for (int f : big) {
                     const FrontDesc& d = P.fronts[f];
@@ -231,11 +232,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     const int nch = std::max(1, (d.n - j1 + kSolveBwdRows - 1) / kSolveBwdRows);
                     for (int t = 0; t < nch; t++)
                         P.sv_items.push_back({f, j1 + t * kSolveBwdRows, 0, j0, slot + t, group, slot, nch});
+                    P.sv_fin.push_back({f, j1, 0, j0, slot, group, slot, nch});
                     slot += nch;
                     group++;
                     st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 12.0 * (d.n - j0);
                 }
                 st.count = (int)P.sv_items.size() - st.first;
+                st.count2 = (int)P.sv_fin.size() - st.first2;
                 P.sv_bwd.push_back(st);
                 P.sv_partial = std::max<int64_t>(P.sv_partial, (int64_t)slot * BW);
                 P.sv_groups = std::max(P.sv_groups, group);
@@ -593,6 +596,16 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         }
         return "";
     };
+    for (const SolveStep& st : P.sv_bwd) {
+        if (st.kind != SV_BWD_BLK) continue;
+        if (st.count2 <= 0 || st.first2 < 0 || st.first2 + st.count2 > (int)P.sv_fin.size()) return bad("solve finish slice");
+        for (int t = st.first2; t < st.first2 + st.count2; t++) {
+            const SolveItem& it = P.sv_fin[t];
+            if (it.front < 0 || it.front >= nf || it.j0 < 0 || it.j0 >= P.fronts[it.front].c) return bad("solve finish item");
+            if (it.gfirst < 0 || it.gcount < 1 || (int64_t)(it.gfirst + it.gcount) * kSolveBW > P.sv_partial)
+                return bad("solve finish partials");
+        }
+    }
     std::string se = check_solve(P.sv_fwd);
     if (se.empty()) se = check_solve(P.sv_bwd);
     if (!se.empty()) return bad(se);

@@ -101,11 +101,12 @@ struct Step {
 
 // Solve schedule.  Fronts with at most kSolveSmall pivots are solved by one
 // workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts go one 256-column
-// block per launch: FWD_BLK workgroups each take 256 rows below the block
-// (and recompute the block's own solve); BWD_BLK workgroups form partial
-// dot products over 256 rows each, and the last workgroup of a block to
-// finish (atomic counter, no waiting) reduces them in slot order and
-// solves the block.  The forward result goes to a separate vector Y.
+// block per launch: FWD_BLK workgroups each take kSolveFwdRows rows below
+// the block (and recompute the block's own solve); a BWD_BLK step is two
+// launches, the partial dot products over kSolveBwdRows rows per workgroup
+// (sv_items), then one workgroup per block (sv_fin) that adds the partials
+// in slot order and solves the block.  The forward result goes to a
+// separate vector Y.
 constexpr int kSolveBW = 256;
 constexpr int kSolveSmall = 64;
 constexpr int kSolveFwdRows = 64;   // rows per forward workgroup below a block
@@ -127,6 +128,7 @@ struct SolveStep {
     int kind;
     int first, count;   // slice of sv_items (or of sv_fronts for *_SMALL / GATHER)
     double bytes;       // algorithmic bytes moved
+    int first2 = 0, count2 = 0;   // BWD_BLK: slice of sv_fin (one item per block)
 };
 
 struct Plan {
@@ -143,6 +145,7 @@ struct Plan {
     std::vector<int64_t> solve_uofs;   // per front offset (m_f entries)
     std::vector<int> sv_fronts;        // front lists of SMALL / GATHER steps
     std::vector<SolveItem> sv_items;
+    std::vector<SolveItem> sv_fin;     // BWD_BLK finishing items (front, j0, gfirst, gcount)
     std::vector<SolveStep> sv_fwd, sv_bwd;
     int64_t sv_partial = 1;            // doubles of BWD_BLK partial sums
     int sv_groups = 1;                 // counters needed by one BWD_BLK launch

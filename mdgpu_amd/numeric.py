@@ -100,6 +100,36 @@ class Factorization:
         L.check(L.lib().mdgpu_logdet(self._h, C.byref(out), _stream()))
         return out.value
 
+    # ------------------------------------------------------- reverse mode
+    def selinv(self):
+        """Selected inverse Z = A^{-1} on the pattern of L (FP64 factor)."""
+        L.check(L.lib().mdgpu_selinv(self._h, _stream()))
+        return self
+
+    def selinv_export(self):
+        """Z on the pattern of L, in export_L's layout (host array)."""
+        _torch().cuda.current_stream().synchronize()
+        Zx = np.empty(self.sym.lnz, dtype=np.float64)
+        L.check(L.lib().mdgpu_selinv_export(self._h, Zx.ctypes.data_as(L._DP)))
+        return Zx
+
+    def logdet_grad(self):
+        """d log det A / d a_e for the caller's lower-triangle entries (device)."""
+        torch = _torch()
+        g = torch.empty(self.sym.nnz_a, dtype=torch.float64, device="cuda")
+        L.check(L.lib().mdgpu_logdet_grad(self._h, _dptr(g), _stream()))
+        return g
+
+    def solve_adjoint(self, x, xbar):
+        """Adjoint of x = A^{-1} b: returns (bbar = A^{-1} xbar, dL/da_e)."""
+        torch = _torch()
+        x = x.to(device="cuda", dtype=torch.float64).contiguous()
+        xbar = xbar.to(device="cuda", dtype=torch.float64).contiguous()
+        bbar = torch.empty_like(xbar)
+        g = torch.empty(self.sym.nnz_a, dtype=torch.float64, device="cuda")
+        L.check(L.lib().mdgpu_solve_adjoint(self._h, _dptr(x), _dptr(xbar), _dptr(bbar), _dptr(g), _stream()))
+        return bbar, g
+
     def export_L(self):
         n, lnz = self.sym.n, self.sym.lnz
         Lp = np.empty(n + 1, dtype=np.int32)

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../mdgpu_amd/csrc/plan.h"
+#include "../../mdgpu_amd/csrc/selinv.h"
 #include "../../mdgpu_amd/csrc/symbolic.h"
 
 using namespace mdgpu;
@@ -26,8 +27,80 @@ static void invert_block(const double* g, int ld, int b, double* w) {
         for (int r = 0; r < b; r++) w[i + 64 * r] = X[r + i * b];
 }
 
+// Executes the selected-inverse schedule (selinv_plan.cpp) with the device
+// kernels' loops, given the factor L, the diagonal-block inverses and the
+// update arena buffer U; writes Z in the L store layout.
+static int run_selinv(const Plan& P, const std::vector<double>& L, const std::vector<double>& inv,
+                      std::vector<double>& U, std::vector<double>& Z) {
+    SelPlan Q;
+    if (!build_selinv_plan(P, Q).empty()) return 4;
+    if (!validate_selinv_plan(P, Q).empty()) return 5;
+    std::vector<double> ZF(Q.zf_size, std::nan("")), LH(Q.lh_size, std::nan(""));
+    Z.assign(P.lstore, 0.0);
+    std::vector<double>* bufs[SB_N] = {const_cast<std::vector<double>*>(&L), &Z, &ZF, &LH,
+                                       const_cast<std::vector<double>*>(&inv), &U};
+    for (const SelStep& st : Q.steps) {
+        if (st.kind == SS_GEMM) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const SelGemm& g = Q.gemm[t];
+                const double* A = bufs[g.a_buf]->data() + g.a_off;
+                const double* B = bufs[g.b_buf]->data() + g.b_off;
+                double* C = bufs[g.c_buf]->data() + g.c_off;
+                std::vector<double> out((size_t)g.M * g.N);
+                for (int j = 0; j < g.N; j++)
+                    for (int i = 0; i < g.M; i++) {
+                        double acc = 0;
+                        for (int k = 0; k < g.K; k++)
+                            acc += (g.ta ? A[k + (int64_t)i * g.lda] : A[i + (int64_t)k * g.lda]) *
+                                   (g.tb ? B[j + (int64_t)k * g.ldb] : B[k + (int64_t)j * g.ldb]);
+                        out[i + (size_t)j * g.M] = acc;
+                    }
+                for (int j = 0; j < g.N; j++)
+                    for (int i = 0; i < g.M; i++) {
+                        double& c = C[i + (int64_t)j * g.ldc];
+                        c = g.beta == 0.0 ? g.alpha * out[i + (size_t)j * g.M] : g.alpha * out[i + (size_t)j * g.M] + g.beta * c;
+                    }
+            }
+            continue;
+        }
+        for (int t = st.first; t < st.first + st.count; t++) {
+            const SelCopy& it = Q.copy[t];
+            const FrontDesc& d = P.fronts[it.front];
+            double* z = ZF.data() + Q.zf_ofs[it.front];
+            if (st.copy_kind == SC_INIT) {
+                for (int j = it.j0; j < it.j1; j++)
+                    for (int i = 0; i < d.m; i++) z[(d.c + i) + (int64_t)(d.c + j) * d.n] = U[d.uofs + i + (int64_t)j * d.m];
+            } else if (st.copy_kind == SC_TRANS) {
+                for (int ii = it.j0; ii < it.j1; ii++)
+                    for (int jj = 0; jj < it.b; jj++) {
+                        const int i = it.b0 + it.b + ii;
+                        z[(it.b0 + jj) + (int64_t)i * d.n] = z[i + (int64_t)(it.b0 + jj) * d.n];
+                    }
+            } else if (st.copy_kind == SC_OUT) {
+                for (int j = it.j0; j < it.j1; j++)
+                    for (int i = 0; i < d.n; i++) Z[d.lofs + i + (int64_t)j * d.n] = z[i + (int64_t)j * d.n];
+            } else {
+                const FrontDesc& pd = P.fronts[d.parent];
+                const double* zp = ZF.data() + Q.zf_ofs[d.parent];
+                const int* r = P.rel.data() + d.relofs;
+                for (int j = it.j0; j < it.j1; j++)
+                    for (int i = 0; i < d.m; i++) U[d.uofs + i + (int64_t)j * d.m] = zp[r[i] + (int64_t)r[j] * pd.n];
+            }
+        }
+    }
+    return 0;
+}
+
+extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
+                                     int max_zeros, int nb_outer, double* Lx_out, int* Li_out, double* Zx_out);
+
 extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                               int max_zeros, int nb_outer, double* Lx_out, int* Li_out) {
+    return emulate_factor_selinv(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, Lx_out, Li_out, nullptr);
+}
+
+extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
+                                     int max_zeros, int nb_outer, double* Lx_out, int* Li_out, double* Zx_out) {
     AnalyzeOptions ao;
     ao.use_amd = use_amd;
     ao.partition = (Partition)mode;
@@ -115,6 +188,11 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
             }
         }
     }
+    std::vector<double> Z;
+    if (Zx_out) {
+        const int st = run_selinv(P, L, inv, U, Z);
+        if (st) return st;
+    }
     for (int f = 0; f < P.nfronts; f++) {
         const FrontDesc& d = P.fronts[f];
         for (int j = 0; j < d.c; j++) {
@@ -122,6 +200,7 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
             for (int i = j; i < d.n; i++, dst++) {
                 Lx_out[dst] = L[d.lofs + (int64_t)j * d.n + i];
                 Li_out[dst] = S.q_all[d.qofs + i];
+                if (Zx_out) Zx_out[dst] = Z[d.lofs + (int64_t)j * d.n + i];
             }
         }
     }

@@ -22,7 +22,8 @@ SO = os.path.join(HERE, "emulate", "libemulator.so")
 @pytest.fixture(scope="module")
 def emulator():
     src = [os.path.join(HERE, "emulate", "plan_emulator.cpp"), os.path.join(ROOT, "mdgpu_amd", "csrc", "plan.cpp"),
-           os.path.join(ROOT, "mdgpu_amd", "csrc", "symbolic.cpp")]
+           os.path.join(ROOT, "mdgpu_amd", "csrc", "symbolic.cpp"),
+           os.path.join(ROOT, "mdgpu_amd", "csrc", "selinv_plan.cpp")]
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", SO, *src], check=True)
     return C.CDLL(SO)
 
@@ -51,3 +52,36 @@ def test_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_outer):
     assert st == 0
     assert np.array_equal(Li, li)
     assert rel_err(Lx, lx) < 1e-12
+
+
+SELINV_CASES = [
+    ("grid2d_7_plain", lambda: gen.laplace2d(7), 0, 0),
+    ("grid2d_30_rn500", lambda: gen.laplace2d(30), 2, 500),
+    ("grid3d_9_sn", lambda: gen.laplace3d_7pt(9), 1, 0),
+    ("st27_9_rn100000", lambda: gen.stencil27(9, seed=2), 2, 100000),
+    ("bbt_300_rn500", lambda: gen.bbt_plus_ni(300, seed=9), 2, 500),
+]
+
+
+@pytest.mark.parametrize("name,make,mode,mz", SELINV_CASES, ids=[c[0] for c in SELINV_CASES])
+def test_selinv_schedule_matches_dense_inverse(emulator, name, make, mode, mz):
+    """The selected-inverse schedule, run on the CPU with the device kernels'
+    loops, equals the dense inverse of P A P^T on the pattern of L (numpy)."""
+    p, i, x = make()
+    S = symbolic.analyze(p, i, True, ["plain", "sn", "rn"][mode], mz)
+    I, D = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    Lx = np.zeros(S.lnz)
+    Li = np.zeros(S.lnz, np.int32)
+    Zx = np.zeros(S.lnz)
+    st = emulator.emulate_factor_selinv(len(p) - 1, p.ctypes.data_as(I), i.ctypes.data_as(I), x.ctypes.data_as(D), 1,
+                                        mode, mz, 128, Lx.ctypes.data_as(D), Li.ctypes.data_as(I),
+                                        Zx.ctypes.data_as(D))
+    assert st == 0
+    from mdgpu_amd import _lib as L
+    perm = S.array(L.ARR_PERM)
+    A = gen.full_from_lower(p, i, x).toarray()
+    Zd = np.linalg.inv(A[np.ix_(perm, perm)])
+    lp = S.array(L.ARR_LP)
+    cols = np.repeat(np.arange(S.n), np.diff(lp))
+    ref = Zd[Li, cols]
+    assert rel_err(Zx, ref) < 1e-11

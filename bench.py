@@ -126,17 +126,63 @@ def cpu_sample(S, p, i, x, budget_flops):
             "subtrees": len(chosen)}
 
 
+def bench_batch(args):
+    """C4: 512 uneven independent matrices (n log-uniform in [500, 20000];
+    half 2D/3D Laplacian stencils with a random diagonal shift, half
+    B B^T + n I) factored in FP32 and refined in FP64 as ONE batch: one
+    analysis (a forest), one level-batched launch sequence replayed from a
+    hipGraph.  Every matrix finishes when the batch does, so its latency is
+    the batch's time to solution."""
+    import torch
+    from mdgpu_amd import gen, symbolic
+    from mdgpu_amd.numeric import Factorization
+    t0 = time.perf_counter()
+    mats = gen.batch_matrices(512, seed=0)
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    S = symbolic.analyze_batch([(p, i) for p, i, _ in mats], True, args.mode, args.max_zeros, nthreads=0)
+    t_an = time.perf_counter() - t0
+    log(f"batch: {len(mats)} matrices, n={S.n} fronts={S.nnodes} lnz={S.lnz} flops={S.flops:.3e} "
+        f"gen {t_gen:.1f}s analyze {t_an:.1f}s")
+    ax = torch.from_numpy(np.concatenate([x for _, _, x in mats])).cuda()
+    bs = [gen.rhs_from_xtrue(p, i, x, seed=k)[1] for k, (p, i, x) in enumerate(mats)]
+    bd = torch.from_numpy(np.concatenate(bs)).cuda()
+    F = Factorization(S, "fp32")
+    r = time_path(F, ax, bd, args.steps, args.warmup, args.tol)
+    etas = F.segment_eta()
+    tts = r["factor_s"] + r["refine_s"]
+    ns = [len(p) - 1 for p, _, _ in mats]
+    out = {
+        "metric": "batched factor+refine throughput (matrices/s) at 1e-12 backward error, FP32+IR",
+        "value": len(mats) / tts, "unit": "matrices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": tts * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "512 matrices, n log-uniform [500, 20000], half stencils with shift U(1e-3,1e-1), "
+                               "half B B^T + nI (5 N(0,1) per column)", "n_total": S.n, "fronts": S.nnodes,
+                   "lnz": S.lnz, "flops_counts": S.flops, "n_min": min(ns), "n_max": max(ns),
+                   "partition": f"{args.mode} {args.max_zeros}"},
+        "factor_ms": r["factor_s"] * 1e3, "refine_ms": r["refine_s"] * 1e3, "ir_iters": r["iters"],
+        "eta_max": float(etas.max()), "eta_p50": float(np.median(etas)),
+        "latency_ms_p50": tts * 1e3, "latency_ms_p99": tts * 1e3,
+        "factor_gflops": S.flops / r["factor_s"] / 1e9, "analyze_s": t_an,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
     ap.add_argument("--mode", default="rn")
     ap.add_argument("--max-zeros", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--cpu-budget-gflop", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="fp32,fp64,bf16x3,bf16",
+                    help="comma-separated precisions to time (fp32 and fp64 are needed for the JSON line)")
+    ap.add_argument("--steps-json", default="", help="write per-launch FP32 timings (class, ms, wgs, flops, bytes, level)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -148,6 +194,8 @@ def main():
     from mdgpu_amd import gen, symbolic
     from mdgpu_amd.numeric import Factorization
 
+    if args.workload == "c4":
+        return bench_batch(args)
     desc, make = WORKLOADS[args.workload]
     p, i, x = make(gen)
     xt, b = gen.rhs_from_xtrue(p, i, x, seed=0)
@@ -161,7 +209,8 @@ def main():
     torch.cuda.synchronize()
 
     res = {}
-    for prec in ("fp32", "fp64", "bf16x3", "bf16"):
+    precs = [q for q in ("fp32", "fp64", "bf16x3", "bf16") if q in args.only.split(",")]
+    for prec in precs:
         F = Factorization(S, prec)
         r = time_path(F, ax, bd, args.steps, args.warmup, args.tol, maxit=50 if prec != "bf16" else 30)
         xs = r.pop("x").cpu().numpy()
@@ -169,6 +218,9 @@ def main():
         r["x"] = xs
         prof = F.factorize_profile(ax)
         r["profile"] = prof
+        if prec == "fp32" and args.steps_json:
+            with open(args.steps_json, "w") as fh:
+                json.dump(F.factorize_steps(ax).tolist(), fh)
         r["info"] = F.info()
         res[prec] = r
         log(f"{prec}: factor {r['factor_s']*1e3:.2f} ms  refine {r['refine_s']*1e3:.2f} ms "
@@ -178,6 +230,9 @@ def main():
         del F
         torch.cuda.empty_cache()
 
+    if "fp32" not in res or "fp64" not in res:
+        log("timed " + ",".join(res) + " only (the JSON line needs fp32 and fp64)")
+        return
     from oracle import oracle as O
     eta32_cpu = O.backward_error(p, i, x, res["fp32"]["x"], b)
     eta64_cpu = O.backward_error(p, i, x, res["fp64"]["x"], b)
@@ -245,7 +300,7 @@ def main():
                    "eta_cpu_check": O.backward_error(p, i, x, res[prec]["x"], b),
                    "tts_ratio_over_fp64": (res[prec]["factor_s"] + res[prec]["refine_s"]) / tts64,
                    "gemm": "split bf16 (hi*hi+hi*lo+lo*hi)" if prec == "bf16x3" else "single bf16 product"}
-            for prec in ("bf16x3", "bf16")},
+            for prec in ("bf16x3", "bf16") if prec in res},
         "kernel_classes_fp32": {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in prof.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

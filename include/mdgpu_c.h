@@ -155,7 +155,11 @@ int mdgpu_numeric_set_graph(mdgpu_numeric* num, int enable);
 #define MDGPU_NUM_PANEL_FRONTS 5  /* fronts factored wholly in LDS         */
 #define MDGPU_NUM_DIAG_JOBS 6     /* diagonal blocks of the blocked path   */
 #define MDGPU_NUM_PRECISION 7
-#define MDGPU_NUM_NINFO 8
+#define MDGPU_NUM_FALLBACKS 8     /* FP32/BF16: diagonal blocks of the last
+                                     factorization refactored in FP64 after
+                                     failing the pivot-growth check
+                                     (L_jj^2 < 2^-12 A_jj or L_jj^2 <= 0)  */
+#define MDGPU_NUM_NINFO 9
 int mdgpu_numeric_info(const mdgpu_numeric* num, int64_t* out /* [MDGPU_NUM_NINFO] */);
 
 /* Host only (no device calls): build the schedule mdgpu_numeric_create
@@ -182,6 +186,12 @@ int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream);
 #define MDGPU_KIND_N 6
 int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream,
                             double* out /* [4*MDGPU_KIND_N] */);
+
+/* mdgpu_factorize with one record per launch of the schedule:
+ * out[6*t + {0..5}] = kernel class (as above), milliseconds, workgroups,
+ * algorithmic flops, algorithmic bytes, level.  cap >= 6 * MDGPU_NUM_STEPS.
+ * No counterpart in the reference (its timers are per phase, mdgpu.cu:1905-1930). */
+int mdgpu_factorize_steps(mdgpu_numeric* num, const double* d_ax, void* stream, double* out, int64_t cap);
 
 /* x = A^{-1} b with the stored factor (device vectors, FP64, caller's
  * numbering): permutation, supernodal forward and backward substitution.

@@ -92,8 +92,9 @@ def check(status):
         raise MdgpuError(status, msg.decode() if msg else "")
     return status
 
-NUM_DEVICE_BYTES, NUM_LSTORE, NUM_ARENA, NUM_STEPS, NUM_LEVELS, NUM_PANEL_FRONTS, NUM_DIAG_JOBS, NUM_PRECISION = range(8)
-NUM_NINFO = 8
+(NUM_DEVICE_BYTES, NUM_LSTORE, NUM_ARENA, NUM_STEPS, NUM_LEVELS, NUM_PANEL_FRONTS, NUM_DIAG_JOBS, NUM_PRECISION,
+ NUM_FALLBACKS) = range(9)
+NUM_NINFO = 9
 KIND_NAMES = ["assembly", "extend_add", "panel", "potrf_trsm", "inverse", "gemm"]
 
 _SIGNATURES.update({
@@ -104,6 +105,7 @@ _SIGNATURES.update({
     "mdgpu_plan_check": (C.c_int, [_P, C.c_int, _I64P]),
     "mdgpu_factorize": (C.c_int, [_P, _P, _P]),
     "mdgpu_factorize_profile": (C.c_int, [_P, _P, _P, _DP]),
+    "mdgpu_factorize_steps": (C.c_int, [_P, _P, _P, _DP, C.c_int64]),
     "mdgpu_solve": (C.c_int, [_P, _P, _P, _P]),
     "mdgpu_refine": (C.c_int, [_P, _P, _P, C.c_double, C.c_int, _DP, C.POINTER(C.c_int), _P]),
     "mdgpu_refine_segment_eta": (C.c_int, [_P, _DP, C.c_int]),

@@ -47,5 +47,6 @@ extern "C" int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_
     out[MDGPU_NUM_PANEL_FRONTS] = (int64_t)P.panel.size();
     out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.blk.size();
     out[MDGPU_NUM_PRECISION] = precision;
+    out[MDGPU_NUM_FALLBACKS] = 0;
     return MDGPU_OK;
 }

@@ -19,9 +19,11 @@ template <typename T>
 void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,
                   hipStream_t s);
 // fused diagonal-block Cholesky + TRSM of the rows below; counters: one
-// zero-initialised int per job (returned to zero by the kernel)
+// zero-initialised int per job (returned to zero by the kernel); fallbacks:
+// incremented once per diagonal block refactored in FP64 (pivot growth)
 template <typename T>
-void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, hipStream_t s);
+void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
+                hipStream_t s);
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
 // tile 128: MFMA kernel; tile 64: FMA kernel (the plan's tile size decides)

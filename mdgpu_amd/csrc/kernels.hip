@@ -9,6 +9,7 @@
 #include <climits>
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -170,66 +171,27 @@ __device__ __forceinline__ void trsm_row16(T (&x)[64], const T* Lc, const T* rd)
     }
 }
 
-// Block column of the blocked path: Cholesky of the b x b diagonal block
-// (the potrf of FactorAux_GPU_2, mdgpu.cu:1010) and the triangular solve of
-// the rows below it (the trsm of :1047), one launch for all fronts of a
-// level.  Each workgroup owns kBlkRows rows below the block, one row per
-// lane, kept in registers; it factors the diagonal block itself (four
-// 16-column panels: each panel by wavefront 0 with its rows in registers
-// and column entries broadcast by v_readlane, the trailing part of the block
-// by all wavefronts from LDS), then solves x L_D^T = r for its rows with
-// L_D in LDS (uniform broadcast reads).  The workgroups of a job count their
-// arrival after reading the block; the last one writes the factored block
-// back, so no workgroup can read a half-written block.  Rows and columns
-// beyond b are identity padding.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
-                                                       T* __restrict__ L, int* __restrict__ info,
-                                                       int* __restrict__ counters) {
-    __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]; later Lc
-    __shared__ T rd[64];
-    __shared__ int last;
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile_base <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
-    }
-    const BlkJob jb = jobs[lo];
-    const int b = jb.b, ld = jb.ld;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    T* __restrict__ g = L + jb.p_off;
-    {
-        // 16 elements per thread; loads issued together (see below)
-        T v[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-            v[q] = g[min(r, b - 1) + (int64_t)min(c, b - 1) * ld];
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) asm volatile("" : "+v"(v[q]));
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-            As[r + 65 * c] = (r < b && c < b) ? v[q] : (r == c ? T(1) : T(0));
-        }
-    }
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(&counters[jb.cnt], 1) == jb.ntiles - 1;
-    int bad = 64;
+// Cholesky of the 64x64 block held column-major in LDS (As[r + 65 c]), in
+// four 16-column panels: each panel factored by wavefront 0 with its rows in
+// registers (pivot and column entries broadcast by v_readlane), the trailing
+// part of the block updated by all four wavefronts from LDS.  bad = first
+// non-positive pivot (64 if none).  Called by all threads of the workgroup.
+template <typename U>
+__device__ void chol64_panels(U* As, int tid, int& bad) {
+    const int wave = tid >> 6, lane = tid & 63;
     for (int p = 0; p < 4; ++p) {
         const int c0 = 16 * p;
         if (wave == 0) {
             const int r = c0 + lane;
-            T a[16];
+            U a[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = r < 64 ? As[r + 65 * (c0 + j)] : T(0);
+            for (int j = 0; j < 16; ++j) a[j] = r < 64 ? As[r + 65 * (c0 + j)] : U(0);
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                const T djj = bcast(a[j], j);
-                bad = (!(djj > T(0)) && bad == 64) ? c0 + j : bad;
-                const T sq = sqrt(djj);
-                const T is = T(1) / sq;
+                const U djj = bcast(a[j], j);
+                bad = (!(djj > U(0)) && bad == 64) ? c0 + j : bad;
+                const U sq = sqrt(djj);
+                const U is = U(1) / sq;
                 a[j] = (lane == j) ? sq : a[j] * is;
 #pragma unroll
                 for (int k = j + 1; k < 16; ++k) a[k] -= a[j] * bcast(a[j], k);
@@ -245,12 +207,93 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
         for (int idx = tid; idx < w * w; idx += kThreads) {
             const int i = c0 + 16 + idx % w, j = c0 + 16 + idx / w;
             if (i < j) continue;
-            T acc = T(0);
+            U acc = U(0);
 #pragma unroll
             for (int t = 0; t < 16; ++t) acc += As[i + 65 * (c0 + t)] * As[j + 65 * (c0 + t)];
             As[i + 65 * j] -= acc;
         }
         __syncthreads();
+    }
+}
+
+// Pivot-growth threshold of the reduced-precision factorizations: a
+// diagonal block whose pivots satisfy L_jj^2 < 2^-12 A_jj (more than 12 of
+// FP32's 24 bits cancelled), or that has a non-positive pivot, is factored
+// again in FP64 from its assembled values.
+constexpr double kPivotGrowthMin = 1.0 / 4096.0;
+
+// Block column of the blocked path: Cholesky of the b x b diagonal block
+// (the potrf of FactorAux_GPU_2, mdgpu.cu:1010) and the triangular solve of
+// the rows below it (the trsm of :1047), one launch for all fronts of a
+// level.  Each workgroup owns kBlkRows rows below the block, one row per
+// lane, kept in registers; it factors the diagonal block itself (four
+// 16-column panels: each panel by wavefront 0 with its rows in registers
+// and column entries broadcast by v_readlane, the trailing part of the block
+// by all wavefronts from LDS), then solves x L_D^T = r for its rows with
+// L_D in LDS (uniform broadcast reads).  The workgroups of a job count their
+// arrival after reading the block; the last one writes the factored block
+// back, so no workgroup can read a half-written block.  Rows and columns
+// beyond b are identity padding.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
+                                                       T* __restrict__ L, int* __restrict__ info,
+                                                       int* __restrict__ counters, int* __restrict__ fallbacks) {
+    __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]; later Lc
+    __shared__ T rd[64];
+    __shared__ T a0[64];
+    __shared__ double Adbl[std::is_same<T, float>::value ? 64 * 65 : 1];   // FP64 fallback
+    __shared__ int last;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+    }
+    const BlkJob jb = jobs[lo];
+    const int b = jb.b, ld = jb.ld;
+    const int tid = threadIdx.x;
+    T* __restrict__ g = L + jb.p_off;
+    // 16 elements per thread; loads issued together (the empty asm keeps the
+    // compiler from sinking each load into its own branch and wait)
+    T vin[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+        vin[q] = g[min(r, b - 1) + (int64_t)min(c, b - 1) * ld];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) asm volatile("" : "+v"(vin[q]));
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+        As[r + 65 * c] = (r < b && c < b) ? vin[q] : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(&counters[jb.cnt], 1) == jb.ntiles - 1;
+    if (tid < 64) a0[tid] = As[tid * 66];            // assembled diagonal
+    int bad = 64;
+    chol64_panels<T>(As, tid, bad);
+    bool fell_back = false;
+    if constexpr (std::is_same<T, float>::value) {
+        // pivot-growth check; on failure factor the block again in FP64
+        const double ljj = tid < 64 ? (double)As[tid * 66] : 1.0;
+        const bool weak = tid < 64 && !(ljj * ljj >= kPivotGrowthMin * (double)a0[tid]);
+        if (__syncthreads_or(weak)) {
+            fell_back = true;
+            double* Ad = Adbl;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+                Ad[r + 65 * c] = (r < b && c < b) ? (double)vin[q] : (r == c ? 1.0 : 0.0);
+            }
+            __syncthreads();
+            bad = 64;
+            chol64_panels<double>(Ad, tid, bad);
+            for (int t = tid; t < 64 * 64; t += kThreads) {
+                const int r = t & 63, c = t >> 6;
+                As[r + 65 * c] = (T)Ad[r + 65 * c];
+            }
+            __syncthreads();
+        }
     }
     // repack L_D column-major with stride 64 (Lc(k, j) = Lc[64 j + k]) for
     // aligned broadcast reads; zero above the diagonal
@@ -277,6 +320,7 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
         }
         if (tid == 0) {
             if (bad < b) atomicMin(info, jb.col0 + bad);
+            if (fell_back) atomicAdd(fallbacks, 1);
             atomicExch(&counters[jb.cnt], 0);
         }
     }
@@ -1161,8 +1205,9 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
     panel_kernel<T><<<count, kThreads, lds_bytes, s>>>(list, fronts, L, info);
 }
 template <typename T>
-void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, hipStream_t s) {
-    if (nblocks > 0) blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters);
+void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
+                hipStream_t s) {
+    if (nblocks > 0) blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters, fallbacks);
 }
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {
@@ -1250,7 +1295,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
     template void launch_extend_add<T>(const EaItem*, int, const FrontDesc*, const int*, T*, T*, hipStream_t); \
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
-    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, hipStream_t);                         \
+    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \

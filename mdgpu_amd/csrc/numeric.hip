@@ -91,6 +91,7 @@ struct mdgpu_numeric {
     std::vector<double> anorm_seg;        // ||A_k||_1 per segment
     std::vector<double> eta_seg;          // backward error per segment, last refine
     bool factored = false;
+    int fallbacks = 0;                    // diagonal blocks refactored in FP64, last factorization
     int64_t device_bytes = 0;
     // reverse mode (built on first use, FP64 only)
     bool sel_ready = false, z_valid = false;
@@ -202,6 +203,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
     if (ev) hipEventRecord(ev[0], s);
     mdgpu::launch_fill<T>(L, P.lstore, T(0), s);
     mdgpu::launch_fill<int>(N->dInfo, 1, INT_MAX, s);
+    mdgpu::launch_fill<int>(N->dInfo + 1, 1, 0, s);     // FP64 diagonal fallbacks
     mdgpu::launch_scatter_a<T>(L, ax, N->dAdst, (int)P.a_dst.size(), s);
     if (ev) hipEventRecord(ev[1], s);
     for (size_t t = 0; t < P.steps.size(); t++) {
@@ -215,7 +217,8 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_panel<T>(N->dPanel + st.first, st.count, N->dFronts, L, N->dInfo, st.arg, s);
                 break;
             case mdgpu::STEP_BLK:
-                mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt, s);
+                mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt,
+                                     N->dInfo + 1, s);
                 break;
             case mdgpu::STEP_GEMM:
                 if constexpr (std::is_same<T, float>::value) {
@@ -399,7 +402,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dD)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dRowabs)) ||
         (st = upload(N, diag_off, (void**)&N->dDiagOff)) ||
-        (st = alloc(N, sizeof(int), (void**)&N->dInfo)) ||
+        (st = alloc(N, 2 * sizeof(int), (void**)&N->dInfo)) ||
         (st = alloc(N, N->nseg * sizeof(unsigned long long), (void**)&N->dMax)) ||
         (N->nseg > 1 && (st = upload(N, seg_of_row, (void**)&N->dSeg))) ||
         (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)) ||
@@ -442,6 +445,7 @@ extern "C" int mdgpu_numeric_info(const mdgpu_numeric* num, int64_t* out) {
     out[MDGPU_NUM_PANEL_FRONTS] = (int64_t)P.panel.size();
     out[MDGPU_NUM_DIAG_JOBS] = (int64_t)P.blk.size();
     out[MDGPU_NUM_PRECISION] = num->precision;
+    out[MDGPU_NUM_FALLBACKS] = num->fallbacks;
     return MDGPU_OK;
 }
 
@@ -476,10 +480,10 @@ extern "C" int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* str
     return finish_factor(num, d_ax, s);
 }
 
-extern "C" int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream, double* out) {
-    mdgpu::clear_error();
-    if (!num || !d_ax || !out) return fail(MDGPU_ERR_ARG, "mdgpu_factorize_profile: null pointer");
-    hipStream_t s = (hipStream_t)stream;
+namespace {
+// One factorization with a HIP event around every step on the launch
+// stream; ms[0] = zero fill + A scatter, ms[t + 1] = step t.
+int factor_timed(mdgpu_numeric* num, const double* d_ax, hipStream_t s, std::vector<float>& ms) {
     num->factored = false;
     const mdgpu::Plan& P = num->plan;
     std::vector<hipEvent_t> ev(P.steps.size() + 2);
@@ -492,26 +496,56 @@ extern "C" int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, v
         if (e != hipSuccess) st = hip_fail(e, "hipEventSynchronize");
     }
     if (st == MDGPU_OK) {
-        for (int k = 0; k < 4 * KIND_N; k++) out[k] = 0;
-        float ms = 0;
-        hipEventElapsedTime(&ms, ev[0], ev[1]);
-        const mdgpu::Symbolic& S = num->sym->S;
-        const double es = num->precision == MDGPU_FP64 ? 8 : 4;
-        out[4 * KIND_ASSEMBLY + 0] += ms;
-        out[4 * KIND_ASSEMBLY + 1] += 3;
-        out[4 * KIND_ASSEMBLY + 3] += P.lstore * es + S.nnz_in * (8.0 + 8.0 + es);
-        for (size_t t = 0; t < P.steps.size(); t++) {
-            hipEventElapsedTime(&ms, ev[t + 1], ev[t + 2]);
-            const int k = kind_of(P.steps[t].kind);
-            out[4 * k + 0] += ms;
-            out[4 * k + 1] += 1;
-            out[4 * k + 2] += P.steps[t].flops;
-            out[4 * k + 3] += P.steps[t].bytes;
-        }
+        ms.assign(P.steps.size() + 1, 0.0f);
+        for (size_t t = 0; t + 1 < ev.size(); t++) hipEventElapsedTime(&ms[t], ev[t], ev[t + 1]);
     }
     for (auto& e : ev) hipEventDestroy(e);
     if (st != MDGPU_OK) return st;
     return finish_factor(num, d_ax, s);
+}
+}  // namespace
+
+extern "C" int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream, double* out) {
+    mdgpu::clear_error();
+    if (!num || !d_ax || !out) return fail(MDGPU_ERR_ARG, "mdgpu_factorize_profile: null pointer");
+    std::vector<float> ms;
+    TRY(factor_timed(num, d_ax, (hipStream_t)stream, ms));
+    const mdgpu::Plan& P = num->plan;
+    const mdgpu::Symbolic& S = num->sym->S;
+    const double es = num->precision == MDGPU_FP64 ? 8 : 4;
+    for (int k = 0; k < 4 * KIND_N; k++) out[k] = 0;
+    out[4 * KIND_ASSEMBLY + 0] += ms[0];
+    out[4 * KIND_ASSEMBLY + 1] += 3;
+    out[4 * KIND_ASSEMBLY + 3] += P.lstore * es + S.nnz_in * (8.0 + 8.0 + es);
+    for (size_t t = 0; t < P.steps.size(); t++) {
+        const int k = kind_of(P.steps[t].kind);
+        out[4 * k + 0] += ms[t + 1];
+        out[4 * k + 1] += 1;
+        out[4 * k + 2] += P.steps[t].flops;
+        out[4 * k + 3] += P.steps[t].bytes;
+    }
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_factorize_steps(mdgpu_numeric* num, const double* d_ax, void* stream, double* out, int64_t cap) {
+    mdgpu::clear_error();
+    if (!num || !d_ax || !out) return fail(MDGPU_ERR_ARG, "mdgpu_factorize_steps: null pointer");
+    const mdgpu::Plan& P = num->plan;
+    if (cap < (int64_t)P.steps.size() * 6)
+        return fail(MDGPU_ERR_ARG, "mdgpu_factorize_steps: capacity below 6 * MDGPU_NUM_STEPS");
+    std::vector<float> ms;
+    TRY(factor_timed(num, d_ax, (hipStream_t)stream, ms));
+    for (size_t t = 0; t < P.steps.size(); t++) {
+        const mdgpu::Step& st = P.steps[t];
+        double* o = out + 6 * t;
+        o[0] = kind_of(st.kind);
+        o[1] = ms[t + 1];
+        o[2] = st.nblocks;
+        o[3] = st.flops;
+        o[4] = st.bytes;
+        o[5] = st.level;
+    }
+    return MDGPU_OK;
 }
 
 namespace {
@@ -521,12 +555,14 @@ int finish_factor(mdgpu_numeric* num, const double* d_ax, hipStream_t s) {
     const int n = num->sym->S.n;
     mdgpu::launch_gather(d_ax, num->dAfsrc, num->dAfx, (int)num->plan.afull_i.size(), s);
     mdgpu::launch_rowabs(num->dAfp, num->dAfx, num->dRowabs, n, s);
-    int info = INT_MAX;
-    hipError_t e = hipMemcpyAsync(&info, num->dInfo, sizeof(int), hipMemcpyDeviceToHost, s);
+    int info2[2] = {INT_MAX, 0};
+    hipError_t e = hipMemcpyAsync(info2, num->dInfo, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     int st = max_abs(num, num->dRowabs, n, s, num->anorm_seg);   // synchronizes the stream
     if (st != MDGPU_OK) return st;
     num->anorm = *std::max_element(num->anorm_seg.begin(), num->anorm_seg.end());
+    const int info = info2[0];
+    num->fallbacks = info2[1];
     if (info != INT_MAX)
         return fail(MDGPU_ERR_NOT_SPD, "mdgpu_factorize: non-positive pivot at permuted column " +
                                            std::to_string(info));

@@ -49,7 +49,8 @@ class Factorization:
     def info(self):
         out = (C.c_int64 * L.NUM_NINFO)()
         L.check(L.lib().mdgpu_numeric_info(self._h, out))
-        names = ["device_bytes", "lstore", "arena", "steps", "levels", "panel_fronts", "diag_jobs", "precision"]
+        names = ["device_bytes", "lstore", "arena", "steps", "levels", "panel_fronts", "diag_jobs", "precision",
+                 "fp64_fallbacks"]
         return dict(zip(names, list(out)))
 
     def factorize(self, ax):
@@ -70,6 +71,17 @@ class Factorization:
         L.check(L.lib().mdgpu_factorize_profile(self._h, _dptr(ax), _stream(), out))
         return {k: {"ms": out[4 * t], "launches": int(out[4 * t + 1]), "flops": out[4 * t + 2],
                     "bytes": out[4 * t + 3]} for t, k in enumerate(L.KIND_NAMES)}
+
+    def factorize_steps(self, ax):
+        """Factorize with a HIP-event time for every launch of the schedule:
+        array of rows (kernel class, ms, workgroups, flops, bytes, level)."""
+        torch = _torch()
+        ax = ax.to(device="cuda", dtype=torch.float64).contiguous()
+        self._ax = ax
+        nsteps = self.info()["steps"]
+        out = np.zeros(6 * max(nsteps, 1))
+        L.check(L.lib().mdgpu_factorize_steps(self._h, _dptr(ax), _stream(), out.ctypes.data_as(L._DP), len(out)))
+        return out[:6 * nsteps].reshape(nsteps, 6)
 
     def solve(self, b, out=None):
         torch = _torch()

@@ -110,3 +110,31 @@ def test_graph_replays_match_eager(device, prec):
         assert np.array_equal(xs, xe), rep
         E.solve(bd)                      # eager work between replays
         torch.cuda.synchronize()
+
+
+def _dense_lower(A):
+    n = A.shape[0]
+    r, c = np.tril_indices(n)
+    return gen.lower_csc_from_coo(n, r, c, A[r, c])
+
+
+def test_pivot_growth_fallback_to_fp64(device):
+    """A diagonal block with heavy cancellation (ones + 1e-4 I: pivots ~1e-4
+    of the assembled diagonal) fails the FP32 pivot-growth check and is
+    refactored in FP64; refinement still reaches 1e-12.  A well-conditioned
+    matrix never falls back."""
+    import torch
+    from oracle import oracle as O
+    n = 130
+    A = np.ones((n, n)) + 1e-4 * np.eye(n)
+    p, i, x = _dense_lower(A)
+    S = symbolic.analyze(p, i, False, "sn", 0)
+    F = Factorization(S, "fp32").factorize(torch.from_numpy(x).to(device))
+    assert F.info()["fp64_fallbacks"] > 0
+    b = A @ np.random.default_rng(0).uniform(-1, 1, n)
+    xs, eta, iters = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=80)
+    assert eta <= 1e-12 and O.backward_error(p, i, x, xs.cpu().numpy(), b) <= 1e-12, (eta, iters)
+    p, i, x = gen.laplace2d(100)
+    S = symbolic.analyze(p, i, True, "rn", 1000)
+    G = Factorization(S, "fp32").factorize(torch.from_numpy(x).to(device))
+    assert G.info()["fp64_fallbacks"] == 0

@@ -77,6 +77,8 @@ struct mdgpu_numeric {
     // buffers dAx / dB / dX, launched on the caller's stream
     bool use_graph = true;
     hipStream_t cap_stream = nullptr;
+    hipStream_t side = nullptr;           // look-ahead stream (default priority)
+    std::vector<hipEvent_t> events;       // look-ahead events of the plan
     hipGraphExec_t g_factor = nullptr;
     hipGraphExec_t g_solve = nullptr;
     double* dAx = nullptr;
@@ -149,6 +151,10 @@ void release(mdgpu_numeric* N) {
     N->g_factor = N->g_solve = nullptr;
     if (N->cap_stream) (void)hipStreamDestroy(N->cap_stream);
     N->cap_stream = nullptr;
+    if (N->side) (void)hipStreamDestroy(N->side);
+    N->side = nullptr;
+    for (hipEvent_t e : N->events) (void)hipEventDestroy(e);
+    N->events.clear();
     for (auto& b : N->bufs) (void)hipFree(b.p);
     N->bufs.clear();
 }
@@ -206,8 +212,13 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
     mdgpu::launch_fill<int>(N->dInfo + 1, 1, 0, s);     // FP64 diagonal fallbacks
     mdgpu::launch_scatter_a<T>(L, ax, N->dAdst, (int)P.a_dst.size(), s);
     if (ev) hipEventRecord(ev[1], s);
+    const hipStream_t main_s = s;
     for (size_t t = 0; t < P.steps.size(); t++) {
         const mdgpu::Step& st = P.steps[t];
+        // look-ahead (not when timing every step): side-stream launches and
+        // their event waits / records
+        s = (!ev && st.stream == 1) ? N->side : main_s;
+        if (!ev && st.wait_ev >= 0) hipStreamWaitEvent(s, N->events[st.wait_ev], 0);
         switch (st.kind) {
             case mdgpu::STEP_EA1:
             case mdgpu::STEP_EA2:
@@ -234,6 +245,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
                 break;
         }
+        if (!ev && st.rec_ev >= 0) hipEventRecord(N->events[st.rec_ev], s);
         if (ev) hipEventRecord(ev[t + 2], s);
     }
     hipError_t e = hipGetLastError();
@@ -415,8 +427,13 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.sv_fin, (void**)&N->dSvFin)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dY)))
         return guard(st);
-    if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&N->side, hipStreamNonBlocking) != hipSuccess)
         return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: hipStreamCreate failed"));
+    N->events.assign(P.nevents, nullptr);
+    for (hipEvent_t& e : N->events)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: hipEventCreate failed"));
     *out = N;
     return MDGPU_OK;
 }

@@ -314,6 +314,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
     };
 
+    std::vector<std::pair<size_t, int>> joins;   // (first step index, side event to wait for)
     for (int L = 0; L < P.nlevels; L++) {
         const std::vector<int>& fl = P.level_fronts[L];
         add_ea(L, true);
@@ -352,6 +353,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         // is one deep (K = nb_outer) GEMM
         int maxc = 0;
         for (int f : blocked) maxc = std::max(maxc, P.fronts[f].c);
+        int side_last = -1, join_ev = -1;
         for (int R0 = 0; R0 < maxc; R0 += o.nb_outer) {
             for (int r0 = R0; r0 < std::min(R0 + o.nb_outer, maxc); r0 += o.nb) {
                 const int dbegin = (int)P.blk.size();
@@ -406,30 +408,62 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 }
                 flush_gemm(batch, tiles, L);
             }
-            // trailing update of the pivot columns after the outer block
-            std::vector<GemmJob> batch;
-            int tiles = 0;
+            // trailing update of the pivot columns after the outer block, in
+            // two parts: (a) the next outer block's columns, on the main
+            // stream (the next block's factorization needs them), and (b) all
+            // later columns, on the side stream, overlapping the next outer
+            // block's BLK/GEMM chain.  (a) of this block waits for (b) of the
+            // previous one (both update the same columns); (b) starts when
+            // this outer block is factored.
+            const int inner_done = P.nevents++;
+            if (!P.steps.empty()) P.steps.back().rec_ev = inner_done;
+            std::vector<GemmJob> batch_a, batch_b;
+            int tiles_a = 0, tiles_b = 0;
             for (int f : blocked) {
                 const FrontDesc& d = P.fronts[f];
                 if (R0 >= d.c) continue;
                 const int E = std::min(R0 + o.nb_outer, d.c);
                 if (E >= d.c) continue;
+                const int E2 = std::min(E + o.nb_outer, d.c);
                 GemmJob g{};
                 g.a_off = d.lofs + (int64_t)R0 * d.n + E;
                 g.b_off = g.a_off;
                 g.c_off = d.lofs + (int64_t)E * d.n + E;
                 g.lda = g.ldb = g.ldc = d.n;
                 g.M = d.n - E;
-                g.N = d.c - E;
+                g.N = E2 - E;
                 g.K = E - R0;
                 g.c_in_arena = 0;
                 g.beta0 = 0;
                 g.lower = 1;
-                add_gemm(g, batch, tiles);
+                add_gemm(g, batch_a, tiles_a);
+                if (E2 < d.c) {
+                    GemmJob h = g;
+                    h.a_off = d.lofs + (int64_t)R0 * d.n + E2;
+                    h.b_off = h.a_off;
+                    h.c_off = d.lofs + (int64_t)E2 * d.n + E2;
+                    h.M = d.n - E2;
+                    h.N = d.c - E2;
+                    add_gemm(h, batch_b, tiles_b);
+                }
             }
-            flush_gemm(batch, tiles, L);
+            const int wait_prev = side_last;
+            flush_gemm(batch_a, tiles_a, L);
+            if (tiles_a > 0 && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
+            if (tiles_b > 0) {
+                flush_gemm(batch_b, tiles_b, L);
+                Step& sb = P.steps.back();
+                sb.stream = 1;
+                sb.wait_ev = inner_done;
+                sb.rec_ev = side_last = P.nevents++;
+            }
         }
+        // join: everything on the side stream is done before the Schur complements
+        if (side_last >= 0) join_ev = side_last;
         // Schur complements of every front of the level: U_f = -L21 L21^T
+        // the first main-stream launch after the blocked part waits for the
+        // side stream (see the join pass after the loop)
+        if (join_ev >= 0) joins.push_back({P.steps.size(), join_ev});
         {
             std::vector<GemmJob> batch;
             int tiles = 0;
@@ -469,6 +503,16 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             }
             P.steps.push_back(st);
         }
+    }
+
+    // join pass: the first main-stream launch at or after each recorded
+    // position waits for the side stream's last event of that level
+    for (const auto& jn : joins) {
+        size_t t = jn.first;
+        while (t < P.steps.size() && P.steps[t].stream != 0) t++;
+        if (t == P.steps.size()) return "look-ahead: no launch to join the side stream";
+        if (P.steps[t].wait_ev >= 0) return "look-ahead: join on a launch that already waits";
+        P.steps[t].wait_ev = jn.second;
     }
 
     // full symmetric A in the caller's numbering (CSR rows sorted), for the
@@ -627,6 +671,31 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
     for (int f : P.panel) {
         const FrontDesc& d = P.fronts[f];
         if ((int64_t)d.n * d.c * o.elem_bytes > o.panel_lds_bytes) return bad("panel LDS");
+    }
+    // look-ahead events: recorded once, waited only after being recorded,
+    // only GEMMs on the side stream, and every side launch joined back to
+    // the main stream by a later wait
+    {
+        std::vector<int> rec_at(std::max(P.nevents, 1), -1);
+        for (size_t t = 0; t < P.steps.size(); t++) {
+            const Step& st = P.steps[t];
+            if (st.stream != 0 && (st.stream != 1 || st.kind != STEP_GEMM)) return bad("side-stream launch kind");
+            if (st.wait_ev >= P.nevents || st.rec_ev >= P.nevents) return bad("event index");
+            if (st.wait_ev >= 0 && rec_at[st.wait_ev] < 0) return bad("wait on an event not yet recorded");
+            if (st.rec_ev >= 0) {
+                if (rec_at[st.rec_ev] >= 0) return bad("event recorded twice");
+                rec_at[st.rec_ev] = (int)t;
+            }
+        }
+        for (size_t t = 0; t < P.steps.size(); t++) {
+            const Step& st = P.steps[t];
+            if (st.stream != 1) continue;
+            if (st.rec_ev < 0) return bad("side launch records no event");
+            bool joined = false;
+            for (size_t u = t + 1; u < P.steps.size() && !joined; u++)
+                joined = P.steps[u].stream == 0 && P.steps[u].wait_ev == st.rec_ev;
+            if (!joined) return bad("side launch never joined");
+        }
     }
     for (const Step& s : P.steps) {
         if (s.nblocks <= 0 || s.count <= 0) return bad("empty step");

@@ -97,6 +97,9 @@ struct Step {
     int arg;               // panel: LDS bytes needed
     double flops = 0;      // algorithmic flops of the launch
     double bytes = 0;      // algorithmic HBM bytes of the launch
+    // look-ahead: stream 0 = main, 1 = side; before the launch the stream
+    // waits for event wait_ev, after it records rec_ev (-1: none)
+    int stream = 0, wait_ev = -1, rec_ev = -1;
 };
 
 // Solve schedule.  Fronts with at most kSolveSmall pivots are solved by one
@@ -159,6 +162,7 @@ struct Plan {
     // for i, r < b (b = block width, last block may be narrower).
     int64_t dinv = 0;
     std::vector<Step> steps;           // the factorization schedule
+    int nevents = 0;                   // events used by the look-ahead
     // full symmetric A (original numbering) for the residual: CSR + source
     std::vector<int> afull_p, afull_i, afull_src;
     int max_front = 0;

@@ -138,3 +138,24 @@ def test_pivot_growth_fallback_to_fp64(device):
     S = symbolic.analyze(p, i, True, "rn", 1000)
     G = Factorization(S, "fp32").factorize(torch.from_numpy(x).to(device))
     assert G.info()["fp64_fallbacks"] == 0
+
+
+@pytest.mark.parametrize("nb_outer", ["64", "128"])
+def test_lookahead_outer_blocks(device, monkeypatch, nb_outer):
+    """Narrow outer blocks put many trailing updates on the look-ahead side
+    stream; the FP64 factor still matches the oracle and graph replays equal
+    the eager run bit for bit."""
+    import torch
+    monkeypatch.setenv("MDGPU_NB_OUTER", nb_outer)
+    for name, make, mode, mz in [c for c in CASES if c[0] in ("grid3d_16_rn4000", "st27_12_rn100000")]:
+        p, i, x = make()
+        S = symbolic.analyze(p, i, True, mode, mz)
+        lp, Li_ref, Lx_ref, _, bad = oracle_factor(p, i, x, S)
+        ax = torch.from_numpy(x).to(device)
+        E = Factorization(S, "fp64").set_graph(False).factorize(ax)
+        _, Li, Lx = E.export_L()
+        assert np.array_equal(Li, Li_ref) and rel_err(Lx, Lx_ref) <= 1e-10, name
+        G = Factorization(S, "fp64").set_graph(True)
+        for rep in range(2):
+            G.factorize(ax)
+            assert np.array_equal(G.export_L()[2], Lx), (name, rep)

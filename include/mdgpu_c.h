@@ -89,6 +89,13 @@ int mdgpu_analyze_nodes(int n, const int32_t* Cp, const int32_t* Ci, int nnodes,
                         const int32_t* ncols, const int32_t* kid_ptr, const int32_t* kids,
                         mdgpu_symbolic** sym);
 
+/* Write an analysis to a file / read it back (a versioned binary format
+ * with a checksum; the load checks the structure before trusting it), so a
+ * fixed pattern can be refactorized later without analysing it again.  No
+ * counterpart in the reference. */
+int mdgpu_symbolic_save(const mdgpu_symbolic* sym, const char* path);
+int mdgpu_symbolic_load(const char* path, mdgpu_symbolic** sym);
+
 /* Scalar facts about an analysis. */
 #define MDGPU_SYM_N 0             /* matrix order                          */
 #define MDGPU_SYM_NNODES 1        /* nodes (fronts) in the partition       */

@@ -54,6 +54,8 @@ _SIGNATURES = {
     "mdgpu_analyze_batch": (C.c_int, [C.c_int, _I32P, C.POINTER(_I32P), C.POINTER(_I32P), C.POINTER(AnalyzeOpts),
                                       C.c_int, C.POINTER(_P)]),
     "mdgpu_symbolic_destroy": (C.c_int, [_P]),
+    "mdgpu_symbolic_save": (C.c_int, [_P, C.c_char_p]),
+    "mdgpu_symbolic_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "mdgpu_symbolic_sizes": (C.c_int, [_P, _I64P]),
     "mdgpu_symbolic_flops": (C.c_int, [_P, _DP]),
     "mdgpu_symbolic_array": (C.c_int, [_P, C.c_int, _I32P, C.c_int64]),

@@ -38,6 +38,8 @@ static void finish_handle(mdgpu_symbolic* h) {
     }
 }
 
+void mdgpu_finish_symbolic_handle(mdgpu_symbolic* h) { finish_handle(h); }
+
 extern "C" const char* mdgpu_last_error(void) { return mdgpu::g_last_error.c_str(); }
 
 extern "C" int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,

@@ -66,6 +66,10 @@ class Symbolic:
             t += 5 + nk
         return out
 
+    def save(self, path):
+        """Write the analysis to `path` (see :func:`load`)."""
+        L.check(L.lib().mdgpu_symbolic_save(self._h, str(path).encode()))
+
     def close(self):
         if self._h:
             L.lib().mdgpu_symbolic_destroy(self._h)
@@ -106,3 +110,10 @@ def analyze_batch(mats, amd=True, mode="plain", max_zeros=0, nthreads=0):
     L.check(L.lib().mdgpu_analyze_batch(k, ns.ctypes.data_as(L._I32P), PP, II, C.byref(o), int(nthreads),
                                         C.byref(h)))
     return Symbolic(h, (ps, iis, ns))
+
+
+def load(path):
+    """Read an analysis written by :meth:`Symbolic.save`."""
+    h = C.c_void_p()
+    L.check(L.lib().mdgpu_symbolic_load(str(path).encode(), C.byref(h)))
+    return Symbolic(h, None)

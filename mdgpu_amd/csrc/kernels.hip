@@ -477,6 +477,49 @@ template <> struct MfmaOp<double> {
     __device__ static int row(int lg, int v) { return lg + 4 * v; }
 };
 
+// Store of a wavefront's 64x64 block of C (accumulators acc[rb][cb][v] =
+// C(i0 + 16 rb, j0 + 16 cb + row(lg, v))): C -= acc, or C = -acc (beta0),
+// lower triangle only if g.lower.  For C -= acc all old values of a column
+// block are loaded first, at clamped in-range addresses, so the loads are in
+// flight together (a load after each possibly aliasing store would wait
+// for each one in turn).
+template <typename T, typename Acc, typename RowFn>
+__device__ __forceinline__ void gemm_epilogue(const Acc (&acc)[4][4], T* __restrict__ C, const GemmJob& g, int i0,
+                                              int j0, int lg, RowFn row) {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        T old[4][4];
+        bool ok[4][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                const int i = i0 + rb * 16, j = j0 + cb * 16 + row(lg, v);
+                ok[v][rb] = i < g.M && j < g.N && !(g.lower && i < j);
+                old[v][rb] = T(0);
+            }
+        if (!g.beta0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    const int i = min(i0 + rb * 16, g.M - 1), j = min(j0 + cb * 16 + row(lg, v), g.N - 1);
+                    old[v][rb] = C[i + (int64_t)j * g.ldc];
+                }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) asm volatile("" : "+v"(old[v][rb]));
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+                if (ok[v][rb])
+                    C[(i0 + rb * 16) + (int64_t)(j0 + cb * 16 + row(lg, v)) * g.ldc] = old[v][rb] - acc[rb][cb][v];
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs
 // on XCD i % 8), each with its own L2.  Renumber them so that each XCD gets
 // one contiguous run of tiles: neighbouring tiles share A or B panels, which
@@ -532,12 +575,24 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     const T* __restrict__ Ap = A + ia;
     const T* __restrict__ Bp = B + jb;
     T ra[PER], rb[PER];
+    // loads at clamped in-range addresses, all in flight, then zeroed
+    // outside the job (no per-element branch and wait)
+    const T* __restrict__ Apc = a_ok ? Ap : A;
+    const T* __restrict__ Bpc = b_ok ? Bp : B;
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
-            const int k = k0 + (tid >> 7) + 2 * s;
-            ra[s] = (a_ok && k < g.K) ? Ap[(int64_t)k * g.lda] : T(0);
-            rb[s] = (b_ok && k < g.K) ? Bp[(int64_t)k * g.ldb] : T(0);
+            const int k = min(k0 + (tid >> 7) + 2 * s, g.K - 1);
+            ra[s] = Apc[(int64_t)k * g.lda];
+            rb[s] = Bpc[(int64_t)k * g.ldb];
+        }
+#pragma unroll
+        for (int s = 0; s < PER; ++s) asm volatile("" : "+v"(ra[s]), "+v"(rb[s]));
+#pragma unroll
+        for (int s = 0; s < PER; ++s) {
+            const bool kin = k0 + (tid >> 7) + 2 * s < g.K;
+            ra[s] = (a_ok && kin) ? ra[s] : T(0);
+            rb[s] = (b_ok && kin) ? rb[s] : T(0);
         }
     };
     acc_t acc[4][4];
@@ -579,20 +634,8 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
         __syncthreads();
     }
     // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + row(lg, v))
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int j = col0 + wc + cb * 16 + MfmaOp<T>::row(lg, v);
-            if (j >= g.N) continue;
-#pragma unroll
-            for (int rb2 = 0; rb2 < 4; ++rb2) {
-                const int i = row0 + wr + rb2 * 16 + li;
-                if (i >= g.M || (g.lower && i < j)) continue;
-                T* cp = C + i + (int64_t)j * g.ldc;
-                *cp = g.beta0 ? -acc[rb2][cb][v] : *cp - acc[rb2][cb][v];
-            }
-        }
+    gemm_epilogue<T>(acc, C, g, row0 + wr + li, col0 + wc, lg,
+                     [](int lg_, int v_) { return MfmaOp<T>::row(lg_, v_); });
 }
 
 // BF16 variants of gemm_mfma_kernel: the factor stays FP32 in HBM, the A
@@ -610,7 +653,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 template <int NSPLIT>
 __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                              float* __restrict__ L, float* __restrict__ U) {
-    constexpr int BT = 128, KT = NSPLIT == 1 ? 64 : 32, LDK = KT + 8;   // rows 16-byte aligned
+    constexpr int BT = 128, KT = 32, LDK = KT + 8;                     // rows 16-byte aligned
     constexpr int KS = KT / 2;                                        // staged k per thread
     constexpr int NP = NSPLIT == 1 ? 1 : 2;                           // hi (and lo) planes
     __shared__ __align__(16) __bf16 As[NP][BT * LDK];
@@ -647,12 +690,24 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
     const float* __restrict__ Ap = A + ia;
     const float* __restrict__ Bp = B + jb;
     float ra[KS], rb[KS];
+    // loads at clamped in-range addresses, all in flight, then zeroed
+    // outside the job (no per-element branch and wait)
+    const float* __restrict__ Apc = a_ok ? Ap : A;
+    const float* __restrict__ Bpc = b_ok ? Bp : B;
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            const int k = k0 + kg + s;
-            ra[s] = (a_ok && k < g.K) ? Ap[(int64_t)k * g.lda] : 0.0f;
-            rb[s] = (b_ok && k < g.K) ? Bp[(int64_t)k * g.ldb] : 0.0f;
+            const int k = min(k0 + kg + s, g.K - 1);
+            ra[s] = Apc[(int64_t)k * g.lda];
+            rb[s] = Bpc[(int64_t)k * g.ldb];
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) asm volatile("" : "+v"(ra[s]), "+v"(rb[s]));
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const bool kin = k0 + kg + s < g.K;
+            ra[s] = (a_ok && kin) ? ra[s] : 0.0f;
+            rb[s] = (b_ok && kin) ? rb[s] : 0.0f;
         }
     };
     f32x4 acc[4][4];
@@ -707,20 +762,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
         __syncthreads();
     }
     // acc[r][c][v] = C(row0 + wr + r*16 + li, col0 + wc + c*16 + 4*lg + v)
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int j = col0 + wc + c * 16 + 4 * lg + v;
-            if (j >= g.N) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = row0 + wr + r * 16 + li;
-                if (i >= g.M || (g.lower && i < j)) continue;
-                float* cp = C + i + (int64_t)j * g.ldc;
-                *cp = g.beta0 ? -acc[r][c][v] : *cp - acc[r][c][v];
-            }
-        }
+    gemm_epilogue<float>(acc, C, g, row0 + wr + li, col0 + wc, lg, [](int lg_, int v_) { return 4 * lg_ + v_; });
 }
 
 // ------------------------------------------------------------------ solves
@@ -940,6 +982,21 @@ __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __re
     }
 }
 
+// v[k] = p[k * stride] for k < 32, all 32 loads in flight before the first
+// use: one moving pointer (no per-element 64-bit addresses to keep), and the
+// empty asm makes every value live at once, so the compiler cannot put a
+// wait after each load.
+template <typename T>
+__device__ __forceinline__ void load32_strided(T (&v)[32], const T* __restrict__ p, int64_t stride) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        v[k] = *p;
+        p += stride;
+    }
+#pragma unroll
+    for (int k = 0; k < 32; ++k) asm volatile("" : "+v"(v[k]));
+}
+
 // Sum over the 64 lanes of a wavefront of 64 values per lane: on return,
 // acc[0] of lane l holds sum over lanes of acc[l] (column l).  A transpose
 // reduction: at each of 6 steps every lane keeps one half of its remaining
@@ -992,8 +1049,12 @@ __global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __re
 #pragma unroll
         for (int h = 0; h < 64; h += 32) {
             T v[32];
+            if (cmax >= 63) {
+                load32_strided(v, col + (int64_t)h * n, n);
+            } else {
 #pragma unroll
-            for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, cmax) * n];
+                for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, cmax) * n];
+            }
 #pragma unroll
             for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
         }
@@ -1053,8 +1114,12 @@ __global__ __launch_bounds__(kThreads) void bwd_fin_kernel(const SolveItem* __re
 #pragma unroll
             for (int h = 0; h < 64; h += 32) {
                 T v[32];
+                if (b == 64) {
+                    load32_strided(v, col + (int64_t)h * n, n);
+                } else {
 #pragma unroll
-                for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, b - 1) * n];
+                    for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, b - 1) * n];
+                }
 #pragma unroll
                 for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
             }

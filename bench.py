@@ -258,6 +258,18 @@ def main():
                 "frac": ach / PEAK_HBM_GBS, "traffic": None,
                 "launches": d["launches"], "avg_launch_ms": d["ms"] / max(d["launches"], 1)}
 
+    # HBM traffic per launch of the dominant kernel, measured by two
+    # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled on a copy of
+    # known size; the committed summary holds the figure
+    kname = {"gemm": "gemm_mfma_kernel<float>"}.get(dom)
+    tpath = os.path.join(ROOT, "profiles", "gemm_blk_extendadd_traffic_c3_fp32.md")
+    if kname and args.workload == "c3" and os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith(f"traffic_bytes_per_launch {kname} ="):
+                roof["traffic"] = float(line.split("=")[1])
+                roof["traffic_unit"] = "bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                roof["algorithmic_bytes_per_launch"] = d["bytes"] / max(d["launches"], 1)
+
     cpu = None
     if not args.no_cpu:
         cs = cpu_sample(S, p, i, x, args.cpu_budget_gflop * 1e9)

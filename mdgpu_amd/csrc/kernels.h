@@ -47,12 +47,16 @@ void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* 
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                        double* xp, double* su, const int64_t* suofs, hipStream_t s);
 template <typename T>
-void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* X256,
                     double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s);
 template <typename T>
 void launch_bwd_blk(const SolveItem* items, int count, const SolveItem* fin, int nfin, const FrontDesc* fronts,
-                    const int* q, const T* L, const T* inv, double* xp, const double* Y, double* partial,
+                    const int* q, const T* L, const T* X256, double* xp, const double* Y, double* partial,
                     hipStream_t s);
+// inverses of the kSolveBW-column diagonal blocks (Plan::inv256), after launch_inv
+template <typename T>
+void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,
+                   hipStream_t s);
 
 // refinement helpers (FP64)
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);

@@ -27,6 +27,21 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// v[k] = p[k * stride] for k < 32, all 32 loads in flight before the first
+// use: one moving pointer (no per-element 64-bit addresses to keep), and the
+// empty asm makes every value live at once, so the compiler cannot put a
+// wait after each load.
+template <typename T>
+__device__ __forceinline__ void load32_strided(T (&v)[32], const T* __restrict__ p, int64_t stride) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        v[k] = *p;
+        p += stride;
+    }
+#pragma unroll
+    for (int k = 0; k < 32; ++k) asm volatile("" : "+v"(v[k]));
+}
+
 // ---------------------------------------------------------------- fill / A
 template <typename T>
 __global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
@@ -392,6 +407,102 @@ __global__ __launch_bounds__(64) void inv_kernel(const InvJob* __restrict__ jobs
     T* __restrict__ w = inv + d.dinv + (int64_t)(jb.j0 / 64) * 4096;
     if (lane < b)
         for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
+}
+
+// Inverse X of a kSolveBW-column diagonal block L_B of a front (for the
+// blocked solves), one workgroup per block, from the 64x64 inverses W_s of
+// its diagonal sub-blocks: X_ss = W_s, and below the diagonal, column block
+// by column block, X_ts = -W_t sum_{u=s}^{t-1} L_tu X_us.  Products of
+// 64x64 blocks go through LDS 32 columns at a time; thread (i, g) owns
+// column i, rows 16g..16g+15.  X is written column-major with leading
+// dimension 256 and is zero above the diagonal and beyond the block.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restrict__ jobs,
+                                                          const FrontDesc* __restrict__ fronts,
+                                                          const T* __restrict__ L, const T* __restrict__ inv,
+                                                          T* __restrict__ X256) {
+    __shared__ T Al[64 * 33];
+    __shared__ T Bl[32 * 65];
+    const InvJob jb = jobs[blockIdx.x];
+    const FrontDesc d = fronts[jb.front];
+    const int n = d.n, j0 = jb.j0, bw = min(kSolveBW, d.c - j0), nsub = (bw + 63) / 64;
+    const T* __restrict__ P = L + d.lofs;
+    T* __restrict__ X = X256 + d.dinv256 + (int64_t)(j0 / kSolveBW) * kSolveBW * kSolveBW;
+    const int tid = threadIdx.x, ci = tid & 63, rg = tid >> 6;
+    for (int t = tid; t < kSolveBW * kSolveBW; t += kThreads) X[t] = T(0);
+    __syncthreads();
+    for (int s = 0; s < nsub; ++s) {
+        const int bs = min(64, bw - 64 * s);
+        const T* __restrict__ w = inv + d.dinv + (int64_t)((j0 + 64 * s) / 64) * 4096;
+        for (int t = tid; t < 64 * 64; t += kThreads) {
+            const int i = t & 63, r = t >> 6;          // W[i + 64 r] = inv(L_s)(r, i)
+            if (i < bs && r < bs && r >= i) X[(64 * s + r) + kSolveBW * (64 * s + i)] = w[i + 64 * r];
+        }
+    }
+    __syncthreads();
+    for (int s = 0; s < nsub; ++s) {
+        const int bs = min(64, bw - 64 * s);
+        for (int t = s + 1; t < nsub; ++t) {
+            const int bt = min(64, bw - 64 * t);
+            T acc[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] = T(0);
+            for (int u = s; u < t; ++u) {
+                const int bu = min(64, bw - 64 * u);
+                for (int k0 = 0; k0 < bu; k0 += 32) {
+                    for (int e = tid; e < 64 * 32; e += kThreads) {
+                        const int r = e & 63, kk = e >> 6;
+                        Al[r * 33 + kk] = (r < bt && k0 + kk < bu)
+                                              ? P[(j0 + 64 * t + r) + (int64_t)(j0 + 64 * u + k0 + kk) * n] : T(0);
+                    }
+                    for (int e = tid; e < 32 * 64; e += kThreads) {
+                        const int kk = e & 31, i = e >> 5;
+                        Bl[kk * 65 + i] = X[(64 * u + k0 + kk) + kSolveBW * (64 * s + i)];
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        T a = T(0);
+#pragma unroll 8
+                        for (int kk = 0; kk < 32; ++kk) a += Al[(rg * 16 + q) * 33 + kk] * Bl[kk * 65 + ci];
+                        acc[q] += a;
+                    }
+                    __syncthreads();
+                }
+            }
+            // X_ts = -W_t acc, W_t(r, m) = wt[m + 64 r] (lower: m <= r)
+            const T* __restrict__ wt = inv + d.dinv + (int64_t)((j0 + 64 * t) / 64) * 4096;
+            T out[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) out[q] = T(0);
+            for (int m0 = 0; m0 < 64; m0 += 32) {
+                for (int e = tid; e < 64 * 32; e += kThreads) {
+                    const int r = e & 63, mm = e >> 6, m = m0 + mm;
+                    Al[r * 33 + mm] = (r < bt && m < bt && m <= r) ? wt[m + 64 * r] : T(0);
+                }
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int m = rg * 16 + q;
+                    if (m >= m0 && m < m0 + 32) Bl[(m - m0) * 65 + ci] = acc[q];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    T a = T(0);
+#pragma unroll 8
+                    for (int mm = 0; mm < 32; ++mm) a += Al[(rg * 16 + q) * 33 + mm] * Bl[mm * 65 + ci];
+                    out[q] += a;
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int r = rg * 16 + q;
+                if (r < bt && ci < bs) X[(64 * t + r) + kSolveBW * (64 * s + ci)] = -out[q];
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // C -= A * B^T on 64x64 tiles (beta0: C = -A * B^T), optionally keeping only
@@ -943,11 +1054,10 @@ __global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restr
 template <typename T>
 __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
-                                                           const T* __restrict__ L, const T* __restrict__ inv,
+                                                           const T* __restrict__ L, const T* __restrict__ X256,
                                                            double* __restrict__ xp, double* __restrict__ Y,
                                                            double* __restrict__ su,
                                                            const int64_t* __restrict__ suofs) {
-    __shared__ double W[64 * 65];
     __shared__ double vb[kSolveBW];
     __shared__ double red[4][64];
     const SolveItem it = items[blockIdx.x];
@@ -955,31 +1065,22 @@ __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __re
     const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
     const T* __restrict__ P = L + d.lofs;
     const int tid = threadIdx.x;
-    if (tid < bw) vb[tid] = xp[d.k + j0 + tid];
-    for (int s0 = 0; s0 < bw; s0 += 64) {
-        const int b = min(64, bw - s0);
-        const T* __restrict__ w = inv + d.dinv + (int64_t)((j0 + s0) / 64) * 4096;
-        for (int t = tid; t < 64 * b; t += kThreads) {
-            const int l = t & 63, r = t >> 6;
-            if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
-        }
-        __syncthreads();
+    vb[tid] = tid < bw ? xp[d.k + j0 + tid] : 0.0;
+    __syncthreads();
+    // y = inv(L_B) v with the block inverse (zero above its diagonal)
+    {
+        const T* __restrict__ X = X256 + d.dinv256 + (int64_t)(j0 / kSolveBW) * kSolveBW * kSolveBW;
         double y = 0.0;
-        if (tid < b) {
-#pragma unroll 8
-            for (int l = 0; l <= tid; ++l) y += W[l + tid * 65] * vb[s0 + l];
+        if (tid < bw) {
+            for (int l0 = 0; l0 <= tid; l0 += 32) {
+                T v[32];
+                load32_strided(v, X + tid + (int64_t)kSolveBW * l0, kSolveBW);
+#pragma unroll
+                for (int k = 0; k < 32; ++k) y += to_d(v[k]) * vb[l0 + k];
+            }
         }
         __syncthreads();
-        if (tid < b) vb[s0 + tid] = y;
-        __syncthreads();
-        const int row = s0 + b + tid;            // later rows of this block
-        if (row < bw) {
-            const T* __restrict__ pr = P + (j0 + row) + (int64_t)(j0 + s0) * n;
-            double acc = 0.0;
-#pragma unroll 16
-            for (int l = 0; l < b; ++l) acc += to_d(pr[(int64_t)l * n]) * vb[s0 + l];
-            vb[row] -= acc;
-        }
+        vb[tid] = y;
         __syncthreads();
     }
     if (it.first && tid < bw) Y[d.k + j0 + tid] = vb[tid];
@@ -999,21 +1100,6 @@ __global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __re
         if (i < d.c) xp[d.k + i] -= sum;
         else su[suofs[it.front] + i - d.c] -= sum;
     }
-}
-
-// v[k] = p[k * stride] for k < 32, all 32 loads in flight before the first
-// use: one moving pointer (no per-element 64-bit addresses to keep), and the
-// empty asm makes every value live at once, so the compiler cannot put a
-// wait after each load.
-template <typename T>
-__device__ __forceinline__ void load32_strided(T (&v)[32], const T* __restrict__ p, int64_t stride) {
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        v[k] = *p;
-        p += stride;
-    }
-#pragma unroll
-    for (int k = 0; k < 32; ++k) asm volatile("" : "+v"(v[k]));
 }
 
 // Sum over the 64 lanes of a wavefront of 64 values per lane: on return,
@@ -1092,73 +1178,33 @@ __global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __re
 template <typename T>
 __global__ __launch_bounds__(kThreads) void bwd_fin_kernel(const SolveItem* __restrict__ items,
                                                            const FrontDesc* __restrict__ fronts,
-                                                           const T* __restrict__ L, const T* __restrict__ inv,
-                                                           double* __restrict__ xp, const double* __restrict__ Y,
+                                                           const T* __restrict__ X256, double* __restrict__ xp,
+                                                           const double* __restrict__ Y,
                                                            const double* __restrict__ partial) {
-    __shared__ double W[64 * 65];
     __shared__ double vb[kSolveBW];
-    __shared__ double red[4][64];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
-    const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
-    const T* __restrict__ P = L + d.lofs;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int j0 = it.j0, bw = min(kSolveBW, d.c - j0);
+    const int tid = threadIdx.x;
+    double v = 0.0;
     if (tid < bw) {
-        double v = Y[d.k + j0 + tid];
+        v = Y[d.k + j0 + tid];
 #pragma unroll 8
         for (int sl = 0; sl < it.gcount; ++sl) v -= partial[(int64_t)(it.gfirst + sl) * kSolveBW + tid];
-        vb[tid] = v;
     }
+    vb[tid] = v;
     __syncthreads();
-    const int nsub = (bw + 63) / 64;
-    for (int sb = nsub - 1; sb >= 0; --sb) {
-        const int s0 = sb * 64, s1 = min(s0 + 64, bw), b = s1 - s0;
-        // stage inv(L_s): W[l + 65 r] = inv(L_s)(r, l)
-        const T* __restrict__ w = inv + d.dinv + (int64_t)((j0 + s0) / 64) * 4096;
-        for (int t = tid; t < 64 * 64; t += kThreads) {
-            const int l = t & 63, r = t >> 6;
-            W[l + 65 * r] = (l < b && r < b) ? to_d(w[l + 64 * r]) : 0.0;
-        }
-        // v_j -= sum over the solved block rows [s1, bw) of L(row, j) x_row:
-        // wavefront w takes rows [s1 + 64w, s1 + 64w + 64)
-        const int t0 = s1 + 64 * wave;
-        double acc[64];
-#pragma unroll
-        for (int k = 0; k < 64; ++k) acc[k] = 0.0;
-        if (t0 < bw) {
-            const int r = t0 + lane;
-            const bool ok = r < bw;
-            const double xv = ok ? vb[r] : 0.0;
-            const T* __restrict__ col = P + (j0 + (ok ? r : t0)) + (int64_t)(j0 + s0) * n;
-#pragma unroll
-            for (int h = 0; h < 64; h += 32) {
-                T v[32];
-                if (b == 64) {
-                    load32_strided(v, col + (int64_t)h * n, n);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, b - 1) * n];
-                }
-#pragma unroll
-                for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
-            }
-        }
-        wave_transpose_sum64(acc, lane);
-        red[wave][lane] = acc[0];
-        __syncthreads();
-        if (tid < b) vb[s0 + tid] -= ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-        __syncthreads();
+    // x = inv(L_B)^T v: column i of the block inverse, read contiguously
+    if (tid < bw) {
+        const T* __restrict__ X = X256 + d.dinv256 + (int64_t)(j0 / kSolveBW) * kSolveBW * kSolveBW;
         double x = 0.0;
-        if (tid < b) {
-#pragma unroll 8
-            for (int r = tid; r < b; ++r) x += W[tid + 65 * r] * vb[s0 + r];
+        for (int l0 = tid & ~31; l0 < kSolveBW; l0 += 32) {
+            T w[32];
+            load32_strided(w, X + (int64_t)kSolveBW * tid + l0, 1);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) x += to_d(w[k]) * vb[l0 + k];
         }
-        __syncthreads();
-        if (tid < b) {
-            vb[s0 + tid] = x;
-            xp[d.k + j0 + s0 + tid] = x;
-        }
-        __syncthreads();
+        xp[d.k + j0 + tid] = x;
     }
 }
 
@@ -1332,16 +1378,21 @@ void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, cons
     if (count > 0) fwd_gather_kernel<<<count, kThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
 }
 template <typename T>
-void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* inv,
+void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* X256,
                     double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_blk_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, inv, xp, Y, su, suofs);
+    if (count > 0) fwd_blk_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, X256, xp, Y, su, suofs);
 }
 template <typename T>
 void launch_bwd_blk(const SolveItem* items, int count, const SolveItem* fin, int nfin, const FrontDesc* fronts,
-                    const int* q, const T* L, const T* inv, double* xp, const double* Y, double* partial,
+                    const int* q, const T* L, const T* X256, double* xp, const double* Y, double* partial,
                     hipStream_t s) {
     if (count > 0) bwd_dot_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, xp, partial);
-    if (nfin > 0) bwd_fin_kernel<T><<<nfin, kThreads, 0, s>>>(fin, fronts, L, inv, xp, Y, partial);
+    if (nfin > 0) bwd_fin_kernel<T><<<nfin, kThreads, 0, s>>>(fin, fronts, X256, xp, Y, partial);
+}
+template <typename T>
+void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,
+                   hipStream_t s) {
+    if (count > 0) inv256_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv, X256);
 }
 
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
@@ -1381,6 +1432,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
+    template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \

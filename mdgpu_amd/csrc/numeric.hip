@@ -50,6 +50,8 @@ struct mdgpu_numeric {
     mdgpu::BlkJob* dBlk = nullptr;
     int* dBlkCnt = nullptr;          // arrival counters of the block-column jobs
     mdgpu::InvJob* dInvJobs = nullptr;
+    void* dInv256 = nullptr;              // kSolveBW-column block inverses (solves)
+    mdgpu::InvJob* dInv256Jobs = nullptr;
     int* dPanel = nullptr;
     int* dLevelList = nullptr;
     std::vector<int> level_off;
@@ -193,7 +195,8 @@ int kind_of(int step_kind) {
         case mdgpu::STEP_EA2: return KIND_EXTEND_ADD;
         case mdgpu::STEP_PANEL: return KIND_PANEL;
         case mdgpu::STEP_BLK: return KIND_POTRF_TRSM;
-        case mdgpu::STEP_INV: return KIND_INVERSE;
+        case mdgpu::STEP_INV:
+        case mdgpu::STEP_INV256: return KIND_INVERSE;
         default: return KIND_GEMM;
     }
 }
@@ -244,6 +247,9 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
             case mdgpu::STEP_INV:
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
                 break;
+            case mdgpu::STEP_INV256:
+                mdgpu::launch_inv256<T>(N->dInv256Jobs + st.first, st.count, N->dFronts, L, inv, (T*)N->dInv256, s);
+                break;
         }
         if (!ev && st.rec_ev >= 0) hipEventRecord(N->events[st.rec_ev], s);
         if (ev) hipEventRecord(ev[t + 2], s);
@@ -273,7 +279,8 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
                                          s);
                 break;
             case mdgpu::SV_FWD_BLK:
-                mdgpu::launch_fwd_blk<T>(items, st.count, N->dFronts, L, inv, N->dXp, N->dY, N->dSu, N->dSuofs, s);
+                mdgpu::launch_fwd_blk<T>(items, st.count, N->dFronts, L, (const T*)N->dInv256, N->dXp, N->dY, N->dSu,
+                                         N->dSuofs, s);
                 break;
         }
     }
@@ -285,8 +292,8 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
                 mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, inv, N->dXp, N->dY, s);
                 break;
             case mdgpu::SV_BWD_BLK:
-                mdgpu::launch_bwd_blk<T>(items, st.count, N->dSvFin + st.first2, st.count2, N->dFronts, N->dQ, L, inv,
-                                         N->dXp, N->dY, N->dSvPartial, s);
+                mdgpu::launch_bwd_blk<T>(items, st.count, N->dSvFin + st.first2, st.count2, N->dFronts, N->dQ, L,
+                                         (const T*)N->dInv256, N->dXp, N->dY, N->dSvPartial, s);
                 break;
         }
     }
@@ -391,6 +398,8 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, P.arena * es, &N->dU)) ||
         (st = alloc(N, P.dinv * es, &N->dInv)) ||
         (st = upload(N, P.inv, (void**)&N->dInvJobs)) ||
+        (st = alloc(N, P.inv256 * es, &N->dInv256)) ||
+        (st = upload(N, P.inv256_jobs, (void**)&N->dInv256Jobs)) ||
         (st = upload(N, P.fronts, (void**)&N->dFronts)) ||
         (st = upload(N, P.kids, (void**)&N->dKids)) ||
         (st = upload(N, P.rel, (void**)&N->dRel)) ||

@@ -113,6 +113,15 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             off += (int64_t)(nblk - 1) * 4096 + (int64_t)(d.c - (nblk - 1) * 64) * 64;
         }
         P.dinv = std::max<int64_t>(off, 1);
+        int64_t off256 = 0;
+        for (int f = 0; f < nf; f++) {
+            FrontDesc& d = P.fronts[f];
+            d.dinv256 = -1;
+            if (d.c <= kSolveSmall) continue;
+            d.dinv256 = off256;
+            off256 += (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW;
+        }
+        P.inv256 = std::max<int64_t>(off256, 1);
     }
 
     // relative maps child update rows -> parent rows (VectorMergeIndices)
@@ -503,6 +512,20 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             }
             P.steps.push_back(st);
         }
+        const int b256 = (int)P.inv256_jobs.size();
+        for (int f = 0; f < nf; f++)
+            if (P.fronts[f].dinv256 >= 0)
+                for (int j0 = 0; j0 < P.fronts[f].c; j0 += kSolveBW) P.inv256_jobs.push_back({f, j0});
+        const int c256 = (int)P.inv256_jobs.size() - b256;
+        if (c256 > 0) {
+            Step st{STEP_INV256, b256, c256, c256, P.nlevels - 1, 0};
+            for (int t = b256; t < b256 + c256; t++) {
+                const double b = std::min(kSolveBW, P.fronts[P.inv256_jobs[t].front].c - P.inv256_jobs[t].j0);
+                st.flops += b * b * b / 3.0;
+                st.bytes += 1.5 * b * b * o.elem_bytes;
+            }
+            P.steps.push_back(st);
+        }
     }
 
     // join pass: the first main-stream launch at or after each recorded
@@ -665,6 +688,14 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         const int nblk = (d.c + 63) / 64;
         if (d.dinv < 0 || d.dinv + (int64_t)(nblk - 1) * 4096 + (int64_t)(d.c - (nblk - 1) * 64) * 64 > P.dinv)
             return bad("front inverse store");
+        if ((d.c > kSolveSmall) != (d.dinv256 >= 0)) return bad("front block-inverse presence");
+        if (d.dinv256 >= 0 &&
+            d.dinv256 + (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW > P.inv256)
+            return bad("front block-inverse store");
+    }
+    for (const InvJob& ij : P.inv256_jobs) {
+        if (ij.front < 0 || ij.front >= nf || P.fronts[ij.front].dinv256 < 0) return bad("block-inverse job front");
+        if (ij.j0 < 0 || ij.j0 >= P.fronts[ij.front].c || ij.j0 % kSolveBW) return bad("block-inverse job block");
     }
     if (o.nb != 64) return bad("blocked path needs 64-column diagonal blocks");
     if (o.nb_outer < o.nb || o.nb_outer % o.nb) return bad("outer block must be a multiple of the block column");

@@ -44,6 +44,7 @@ struct FrontDesc {            // mirrored on the device
     int split;                // # of update columns that land in parent pivots
     int level;
     int64_t dinv;             // offset of the diagonal-block inverses (see Plan::dinv)
+    int64_t dinv256;          // offset of the 256-column block inverses (-1: none; see Plan::inv256)
 };
 
 // One extend-add work item: columns [j0, j1) of child `child`'s update.
@@ -80,7 +81,7 @@ struct BlkJob {
 constexpr int kBlkRows = 256;
 
 // A launch of the schedule: kind + a slice of the matching job array.
-enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV };
+enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV, STEP_INV256 };
 
 // Diagonal-block inverse job (fronts factored by the LDS panel kernel):
 // block of 64 pivot columns starting at j0 of front `front`.
@@ -161,6 +162,13 @@ struct Plan {
     // f at fronts[f].dinv + 4096 t, stored as W(i + 64 r) = inv(L_D)(r, i)
     // for i, r < b (b = block width, last block may be narrower).
     int64_t dinv = 0;
+    // Inverses of the kSolveBW-column diagonal blocks of fronts solved by the
+    // blocked solve path (c > kSolveSmall): block t of front f at
+    // fronts[f].dinv256 + 65536 t, column-major with leading dimension 256,
+    // X(r, i) = inv(L_B)(r, i) for r, i < bw, zero elsewhere.  Formed from
+    // the 64x64 inverses after them (STEP_INV256).
+    int64_t inv256 = 1;
+    std::vector<InvJob> inv256_jobs;   // (front, first pivot column of the block)
     std::vector<Step> steps;           // the factorization schedule
     int nevents = 0;                   // events used by the look-ahead
     // full symmetric A (original numbering) for the residual: CSR + source

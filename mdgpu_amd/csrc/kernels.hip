@@ -199,9 +199,17 @@ template <int J0, typename T>
 __device__ __forceinline__ void trsm_row16(T (&x)[64], const T* Lc, const T* rd) {
 #pragma unroll
     for (int j = J0; j < J0 + 16; ++j) {
-        x[j] *= rd[j];
+        // column j below the diagonal: all reads issued before one wait
+        // (the empty asm), not one wait per read
+        T lc[64];
+        const T rj = rd[j];
 #pragma unroll
-        for (int k = j + 1; k < 64; ++k) x[k] -= x[j] * Lc[64 * j + k];
+        for (int k = j + 1; k < 64; ++k) lc[k] = Lc[64 * j + k];
+#pragma unroll
+        for (int k = j + 1; k < 64; ++k) asm volatile("" : "+v"(lc[k]));
+        x[j] *= rj;
+#pragma unroll
+        for (int k = j + 1; k < 64; ++k) x[k] -= x[j] * lc[k];
     }
 }
 
@@ -362,15 +370,16 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
     const bool has = row < jb.rows;
     T* __restrict__ gr = g + b + row;          // element (b + row, 0) of the block column
     // the 64 row loads are issued together: the empty asm keeps the compiler
-    // from sinking each load into its own branch and wait
+    // from sinking each load into its own branch and wait.  Lanes without a
+    // row, and columns past b, load in-range duplicates instead of zeros:
+    // L_D is the identity beyond b (so those values never reach a column
+    // < b) and they are never stored.
     T x[64];
     const T* __restrict__ src = has ? gr : g;    // in-range address for every lane
 #pragma unroll
     for (int c = 0; c < 64; ++c) x[c] = src[(int64_t)min(c, b - 1) * ld];
 #pragma unroll
     for (int c = 0; c < 64; ++c) asm volatile("" : "+v"(x[c]));
-#pragma unroll
-    for (int c = 0; c < 64; ++c) x[c] = (has && c < b) ? x[c] : T(0);
     // x <- x L_D^{-T}: right-looking over the 64 columns, in registers
     trsm_row16<0>(x, Lc, rd);
     trsm_row16<16>(x, Lc, rd);
@@ -613,17 +622,17 @@ template <> struct MfmaOp<double> {
 // block are loaded first, at clamped in-range addresses, so the loads are in
 // flight together (a load after each possibly aliasing store would wait
 // for each one in turn).
-template <typename T, typename Acc, typename RowFn>
-__device__ __forceinline__ void gemm_epilogue(const Acc (&acc)[4][4], T* __restrict__ C, const GemmJob& g, int i0,
+template <typename T, int NB, typename Acc, typename RowFn>
+__device__ __forceinline__ void gemm_epilogue(const Acc (&acc)[NB][NB], T* __restrict__ C, const GemmJob& g, int i0,
                                               int j0, int lg, RowFn row) {
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-        T old[4][4];
-        bool ok[4][4];
+    for (int cb = 0; cb < NB; ++cb) {
+        T old[4][NB];
+        bool ok[4][NB];
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
+            for (int rb = 0; rb < NB; ++rb) {
                 const int i = i0 + rb * 16, j = j0 + cb * 16 + row(lg, v);
                 ok[v][rb] = i < g.M && j < g.N && !(g.lower && i < j);
                 old[v][rb] = T(0);
@@ -632,19 +641,19 @@ __device__ __forceinline__ void gemm_epilogue(const Acc (&acc)[4][4], T* __restr
 #pragma unroll
             for (int v = 0; v < 4; ++v)
 #pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
+                for (int rb = 0; rb < NB; ++rb) {
                     const int i = min(i0 + rb * 16, g.M - 1), j = min(j0 + cb * 16 + row(lg, v), g.N - 1);
                     old[v][rb] = C[i + (int64_t)j * g.ldc];
                 }
 #pragma unroll
             for (int v = 0; v < 4; ++v)
 #pragma unroll
-                for (int rb = 0; rb < 4; ++rb) asm volatile("" : "+v"(old[v][rb]));
+                for (int rb = 0; rb < NB; ++rb) asm volatile("" : "+v"(old[v][rb]));
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+            for (int rb = 0; rb < NB; ++rb)
                 if (ok[v][rb])
                     C[(i0 + rb * 16) + (int64_t)(j0 + cb * 16 + row(lg, v)) * g.ldc] = old[v][rb] - acc[rb][cb][v];
     }
@@ -660,13 +669,15 @@ __device__ inline int xcd_tile(int bid, int n) {
     return x * q + min(x, r) + k;
 }
 
-template <typename T>
+template <typename T, int BT>
 __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                              T* __restrict__ L, T* __restrict__ U) {
     // k-tile depth: 32 for FP32, 16 for FP64 (register budget of the
     // prefetch); the next k-tile is loaded into registers while the current
     // one is multiplied out of LDS.
-    constexpr int BT = 128, KT = sizeof(T) == 4 ? 32 : 16, LDK = KT + 4;
+    // BT x BT tiles (128, or 64 for narrow updates); each wavefront owns a
+    // (BT/2) x (BT/2) quadrant of NB x NB 16x16 MFMA blocks
+    constexpr int KT = sizeof(T) == 4 ? 32 : 16, LDK = KT + 4, NB = BT / 32, KSTEP = kThreads / BT;
     constexpr int PER = (BT * KT) / kThreads;      // staged elements per thread and operand
     typedef typename MfmaOp<T>::acc_t acc_t;
     __shared__ __align__(16) T As[BT * LDK];
@@ -695,10 +706,10 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     const int row0 = ti * BT, col0 = tj * BT;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
     const int li = lane & 15, lg = lane >> 4;
-    // staging coordinates: element s of this thread is row r = (tid + 256 s) % 128,
-    // k-offset kk = (tid + 256 s) / 128
+    // staging coordinates: element s of this thread is row r = tid % BT,
+    // k-offset kk = tid / BT + KSTEP s
     const int r_st = tid & (BT - 1);
     const int ia = row0 + r_st, jb = col0 + r_st;
     const bool a_ok = ia < g.M, b_ok = jb < g.N;
@@ -712,7 +723,7 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
-            const int k = min(k0 + (tid >> 7) + 2 * s, g.K - 1);
+            const int k = min(k0 + tid / BT + KSTEP * s, g.K - 1);
             ra[s] = Apc[(int64_t)k * g.lda];
             rb[s] = Bpc[(int64_t)k * g.ldb];
         }
@@ -720,23 +731,23 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
         for (int s = 0; s < PER; ++s) asm volatile("" : "+v"(ra[s]), "+v"(rb[s]));
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
-            const bool kin = k0 + (tid >> 7) + 2 * s < g.K;
+            const bool kin = k0 + tid / BT + KSTEP * s < g.K;
             ra[s] = (a_ok && kin) ? ra[s] : T(0);
             rb[s] = (b_ok && kin) ? rb[s] : T(0);
         }
     };
-    acc_t acc[4][4];
+    acc_t acc[NB][NB];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < NB; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
     fetch(0);
     for (int k0 = 0; k0 < g.K; k0 += KT) {
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
-            const int kk = (tid >> 7) + 2 * s;
+            const int kk = tid / BT + KSTEP * s;
             As[r_st * LDK + kk] = ra[s];
             Bs[r_st * LDK + kk] = rb[s];
         }
@@ -744,28 +755,28 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
         if (k0 + KT < g.K) fetch(k0 + KT);
 #pragma unroll
         for (int kq = 0; kq < KT; kq += 16) {
-            T av[4][4], bv[4][4];   // [block][t]
+            T av[NB][4], bv[NB][4];   // [block][t]
 #pragma unroll
-            for (int rb2 = 0; rb2 < 4; ++rb2)
+            for (int rb2 = 0; rb2 < NB; ++rb2)
 #pragma unroll
                 for (int t = 0; t < 4; ++t) av[rb2][t] = As[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
+            for (int cb = 0; cb < NB; ++cb)
 #pragma unroll
                 for (int t = 0; t < 4; ++t) bv[cb][t] = Bs[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int rb2 = 0; rb2 < 4; ++rb2)
+                for (int rb2 = 0; rb2 < NB; ++rb2)
 #pragma unroll
-                    for (int cb = 0; cb < 4; ++cb)
+                    for (int cb = 0; cb < NB; ++cb)
                         acc[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], acc[rb2][cb]);
         }
         __syncthreads();
     }
     // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + row(lg, v))
-    gemm_epilogue<T>(acc, C, g, row0 + wr + li, col0 + wc, lg,
-                     [](int lg_, int v_) { return MfmaOp<T>::row(lg_, v_); });
+    gemm_epilogue<T, NB>(acc, C, g, row0 + wr + li, col0 + wc, lg,
+                         [](int lg_, int v_) { return MfmaOp<T>::row(lg_, v_); });
 }
 
 // BF16 variants of gemm_mfma_kernel: the factor stays FP32 in HBM, the A
@@ -892,7 +903,7 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
         __syncthreads();
     }
     // acc[r][c][v] = C(row0 + wr + r*16 + li, col0 + wc + c*16 + 4*lg + v)
-    gemm_epilogue<float>(acc, C, g, row0 + wr + li, col0 + wc, lg, [](int lg_, int v_) { return 4 * lg_ + v_; });
+    gemm_epilogue<float, 4>(acc, C, g, row0 + wr + li, col0 + wc, lg, [](int lg_, int v_) { return 4 * lg_ + v_; });
 }
 
 // ------------------------------------------------------------------ solves
@@ -1344,10 +1355,11 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
     if (count > 0) inv_kernel<T><<<count, 64, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s) {
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, bool fma, T* L, T* U, hipStream_t s) {
     if (nblocks <= 0) return;
-    if (tile == 128) gemm_mfma_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
-    else gemm_nt_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    if (fma) gemm_nt_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    else if (tile == 128) gemm_mfma_kernel<T, 128><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    else gemm_mfma_kernel<T, 64><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
 }
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s) {
     if (nblocks <= 0) return;
@@ -1433,7 +1445,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
-    template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
+    template void launch_gemm<T>(const GemmJob*, int, int, int, bool, T*, T*, hipStream_t);                         \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \
     template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, const T*, double*,    \

@@ -236,13 +236,13 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 break;
             case mdgpu::STEP_GEMM:
                 if constexpr (std::is_same<T, float>::value) {
-                    if (N->precision == MDGPU_BF16 || N->precision == MDGPU_BF16X3) {
+                    if ((N->precision == MDGPU_BF16 || N->precision == MDGPU_BF16X3) && st.arg == 128) {
                         mdgpu::launch_gemm_bf16(N->dGemm + st.first, st.count, st.nblocks,
                                                 N->precision == MDGPU_BF16X3 ? 3 : 1, L, U, s);
                         break;
                     }
                 }
-                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, N->tile, L, U, s);
+                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, st.arg, N->tile == 64, L, U, s);
                 break;
             case mdgpu::STEP_INV:
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);

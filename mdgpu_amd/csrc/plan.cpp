@@ -256,8 +256,12 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
     }
 
     // schedule
-    const int T = o.tile;
-    auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles) {
+    // GEMM tile size of a launch: o.tile for the deep updates (trailing,
+    // Schur complement); the narrow in-panel updates use 64 when the
+    // MFMA kernels run 128-row tiles (four times the workgroups for the
+    // same work)
+    const int narrow_tile = std::min(o.tile, 64);
+    auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles, int T) {
         g.tiles_m = (g.M + T - 1) / T;
         g.tiles_n = (g.N + T - 1) / T;
         if (g.lower) {
@@ -271,9 +275,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         tiles += g.ntiles;
         batch.push_back(g);
     };
-    auto flush_gemm = [&](std::vector<GemmJob>& batch, int tiles, int L) {
+    auto flush_gemm = [&](std::vector<GemmJob>& batch, int tiles, int L, int T) {
         if (batch.empty()) return;
-        Step s{STEP_GEMM, (int)P.gemm.size(), (int)batch.size(), tiles, L, 0};
+        Step s{STEP_GEMM, (int)P.gemm.size(), (int)batch.size(), tiles, L, T};
         for (const GemmJob& g : batch) {
             double entries = g.lower ? (double)g.N * g.M - 0.5 * (double)g.N * (g.N - 1) : (double)g.M * g.N;
             s.flops += 2.0 * g.K * entries;
@@ -413,9 +417,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     g.c_in_arena = 0;
                     g.beta0 = 0;
                     g.lower = 1;
-                    add_gemm(g, batch, tiles);
+                    add_gemm(g, batch, tiles, narrow_tile);
                 }
-                flush_gemm(batch, tiles, L);
+                flush_gemm(batch, tiles, L, narrow_tile);
             }
             // trailing update of the pivot columns after the outer block, in
             // two parts: (a) the next outer block's columns, on the main
@@ -445,7 +449,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 g.c_in_arena = 0;
                 g.beta0 = 0;
                 g.lower = 1;
-                add_gemm(g, batch_a, tiles_a);
+                add_gemm(g, batch_a, tiles_a, o.tile);
                 if (E2 < d.c) {
                     GemmJob h = g;
                     h.a_off = d.lofs + (int64_t)R0 * d.n + E2;
@@ -453,14 +457,14 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     h.c_off = d.lofs + (int64_t)E2 * d.n + E2;
                     h.M = d.n - E2;
                     h.N = d.c - E2;
-                    add_gemm(h, batch_b, tiles_b);
+                    add_gemm(h, batch_b, tiles_b, o.tile);
                 }
             }
             const int wait_prev = side_last;
-            flush_gemm(batch_a, tiles_a, L);
+            flush_gemm(batch_a, tiles_a, L, o.tile);
             if (tiles_a > 0 && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
             if (tiles_b > 0) {
-                flush_gemm(batch_b, tiles_b, L);
+                flush_gemm(batch_b, tiles_b, L, o.tile);
                 Step& sb = P.steps.back();
                 sb.stream = 1;
                 sb.wait_ev = inner_done;
@@ -490,9 +494,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 g.c_in_arena = 1;
                 g.beta0 = 1;
                 g.lower = 1;
-                add_gemm(g, batch, tiles);
+                add_gemm(g, batch, tiles, o.tile);
             }
-            flush_gemm(batch, tiles, L);
+            flush_gemm(batch, tiles, L, o.tile);
         }
         add_ea(L, false);
     }
@@ -731,8 +735,12 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
     for (const Step& s : P.steps) {
         if (s.nblocks <= 0 || s.count <= 0) return bad("empty step");
         if (s.kind == STEP_GEMM) {
+            if ((s.arg != 64 && s.arg != 128) || s.arg > o.tile) return bad("gemm tile size");
             int tiles = 0;
             for (int t = s.first; t < s.first + s.count; t++) {
+                const GemmJob& g = P.gemm[t];
+                if (g.tiles_m != (g.M + s.arg - 1) / s.arg || g.tiles_n != (g.N + s.arg - 1) / s.arg)
+                    return bad("gemm tiles do not match the launch's tile size");
                 if (P.gemm[t].tile_base != tiles) return bad("gemm tile bases");
                 tiles += P.gemm[t].ntiles;
             }

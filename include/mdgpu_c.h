@@ -194,6 +194,13 @@ int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream);
 int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream,
                             double* out /* [4*MDGPU_KIND_N] */);
 
+/* Autotuner: factorizes d_ax with every GEMM launch of the schedule on 64-
+ * and on 128-row MFMA tiles (HIP-event time per launch), keeps the faster
+ * tile for each launch and installs that schedule (captured graphs are
+ * recaptured on next use).  *changed = launches on 64-row tiles.  Ends with
+ * a valid factorization of d_ax.  No counterpart in the reference. */
+int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, void* stream, int64_t* changed);
+
 /* mdgpu_factorize with one record per launch of the schedule:
  * out[6*t + {0..5}] = kernel class (as above), milliseconds, workgroups,
  * algorithmic flops, algorithmic bytes, level.  cap >= 6 * MDGPU_NUM_STEPS.

@@ -108,6 +108,7 @@ _SIGNATURES.update({
     "mdgpu_factorize": (C.c_int, [_P, _P, _P]),
     "mdgpu_factorize_profile": (C.c_int, [_P, _P, _P, _DP]),
     "mdgpu_factorize_steps": (C.c_int, [_P, _P, _P, _DP, C.c_int64]),
+    "mdgpu_numeric_autotune": (C.c_int, [_P, _P, _P, _I64P]),
     "mdgpu_solve": (C.c_int, [_P, _P, _P, _P]),
     "mdgpu_refine": (C.c_int, [_P, _P, _P, C.c_double, C.c_int, _DP, C.POINTER(C.c_int), _P]),
     "mdgpu_refine_segment_eta": (C.c_int, [_P, _DP, C.c_int]),

@@ -82,6 +82,7 @@ struct mdgpu_numeric {
     hipStream_t side = nullptr;           // look-ahead stream (default priority)
     std::vector<hipEvent_t> events;       // look-ahead events of the plan
     hipGraphExec_t g_factor = nullptr;
+    mdgpu::PlanOptions po;                // options the plan was built with
     hipGraphExec_t g_solve = nullptr;
     double* dAx = nullptr;
     double* dB = nullptr;
@@ -351,7 +352,8 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     N->sym = sym;
     N->precision = precision;
     hipGetDevice(&N->device);
-    mdgpu::PlanOptions po = mdgpu::plan_options_for(precision);
+    mdgpu::PlanOptions& po = N->po;
+    po = mdgpu::plan_options_for(precision);
     std::string err;
     try {
         err = mdgpu::build_plan(sym->S, po, N->plan);
@@ -817,4 +819,65 @@ extern "C" int mdgpu_solve_adjoint(mdgpu_numeric* num, const double* d_x, const 
     mdgpu::launch_solve_adjoint(d_x, bb, num->dArow, num->dAcol, d_g, num->sym->S.nnz_in, s);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? MDGPU_OK : hip_fail(e, "solve adjoint launch");
+}
+
+// ------------------------------------------------------------ autotuning
+namespace {
+// Rebuild the plan with per-launch GEMM tiles `tiles` and install it: only
+// the GEMM job table and the launch records change (the schedule's shape
+// does not depend on tile sizes), so the device job table is overwritten in
+// place.  Captured graphs are dropped (recaptured on next use).
+int install_gemm_tiles(mdgpu_numeric* N, const std::vector<int>& tiles) {
+    mdgpu::PlanOptions po = N->po;
+    po.gemm_tiles = tiles;
+    mdgpu::Plan P;
+    std::string err = mdgpu::build_plan(N->sym->S, po, P);
+    if (err.empty()) err = mdgpu::validate_plan(P, po);
+    if (!err.empty()) return fail(MDGPU_ERR_ARG, "autotune plan: " + err);
+    if (P.gemm.size() != N->plan.gemm.size() || P.steps.size() != N->plan.steps.size())
+        return fail(MDGPU_ERR_ARG, "autotune plan: schedule shape changed");
+    hipError_t e = hipMemcpy(N->dGemm, P.gemm.data(), P.gemm.size() * sizeof(mdgpu::GemmJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "autotune upload");
+    N->plan.gemm.swap(P.gemm);
+    N->plan.steps.swap(P.steps);
+    N->po.gemm_tiles = tiles;
+    if (N->g_factor) (void)hipGraphExecDestroy(N->g_factor);
+    N->g_factor = nullptr;
+    return MDGPU_OK;
+}
+}  // namespace
+
+extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, void* stream, int64_t* changed) {
+    mdgpu::clear_error();
+    if (!num || !d_ax) return fail(MDGPU_ERR_ARG, "mdgpu_numeric_autotune: null pointer");
+    // FMA kernels have one tile size; the bf16 variants keep their 128-row
+    // bf16 kernel (64-row launches would run the FP32 kernel instead)
+    if (num->po.tile != 128 || num->precision == MDGPU_BF16 || num->precision == MDGPU_BF16X3) {
+        if (changed) *changed = 0;
+        return MDGPU_OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int> ords;          // step index of each GEMM launch, in order
+    for (size_t t = 0; t < num->plan.steps.size(); t++)
+        if (num->plan.steps[t].kind == mdgpu::STEP_GEMM) ords.push_back((int)t);
+    const std::vector<int> initial = num->po.gemm_tiles;
+    std::vector<float> ms64, ms128, ms;
+    TRY(install_gemm_tiles(num, std::vector<int>(ords.size(), 64)));
+    TRY(factor_timed(num, d_ax, s, ms));        // warm
+    TRY(factor_timed(num, d_ax, s, ms64));
+    TRY(install_gemm_tiles(num, std::vector<int>(ords.size(), 128)));
+    TRY(factor_timed(num, d_ax, s, ms));        // warm
+    TRY(factor_timed(num, d_ax, s, ms128));
+    std::vector<int> best(ords.size());
+    int64_t n64 = 0;
+    for (size_t g = 0; g < ords.size(); g++) {
+        const int t = ords[g] + 1;              // ms[0] is the assembly
+        best[g] = ms64[t] < ms128[t] ? 64 : 128;
+        n64 += best[g] == 64;
+    }
+    TRY(install_gemm_tiles(num, best));
+    TRY(factor_timed(num, d_ax, s, ms));        // leaves a valid factorization
+    if (changed) *changed = n64;
+    (void)initial;
+    return MDGPU_OK;
 }

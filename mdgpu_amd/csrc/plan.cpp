@@ -260,7 +260,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
     // Schur complement); the narrow in-panel updates use 64 when the
     // MFMA kernels run 128-row tiles (four times the workgroups for the
     // same work)
-    const int narrow_tile = std::min(o.tile, 64);
+    const int default_narrow = std::min(o.tile, 64);
+    int gemm_ordinal = 0;   // GEMM launches emitted so far
+    // tile of the GEMM launch being assembled (the autotuner's choice if any)
+    auto tile_for = [&](int dflt) {
+        if (gemm_ordinal < (int)o.gemm_tiles.size()) return o.gemm_tiles[gemm_ordinal];
+        return dflt;
+    };
     auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles, int T) {
         g.tiles_m = (g.M + T - 1) / T;
         g.tiles_n = (g.N + T - 1) / T;
@@ -277,6 +283,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
     };
     auto flush_gemm = [&](std::vector<GemmJob>& batch, int tiles, int L, int T) {
         if (batch.empty()) return;
+        gemm_ordinal++;
         Step s{STEP_GEMM, (int)P.gemm.size(), (int)batch.size(), tiles, L, T};
         for (const GemmJob& g : batch) {
             double entries = g.lower ? (double)g.N * g.M - 0.5 * (double)g.N * (g.N - 1) : (double)g.M * g.N;
@@ -401,6 +408,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 // update of the outer block's remaining columns [r1, e)
                 std::vector<GemmJob> batch;
                 int tiles = 0;
+                const int narrow_tile = tile_for(default_narrow);
                 for (int t = dbegin; t < dbegin + dcount; t++) {
                     const BlkJob& dj = P.blk[t];
                     const FrontDesc& d = P.fronts[dj.front];
@@ -432,6 +440,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             if (!P.steps.empty()) P.steps.back().rec_ev = inner_done;
             std::vector<GemmJob> batch_a, batch_b;
             int tiles_a = 0, tiles_b = 0;
+            // ordinals: (a) is the next GEMM launch, (b) the one after it
+            const int tile_a = tile_for(o.tile);
+            const int tile_b = gemm_ordinal + 1 < (int)o.gemm_tiles.size() ? o.gemm_tiles[gemm_ordinal + 1] : o.tile;
             for (int f : blocked) {
                 const FrontDesc& d = P.fronts[f];
                 if (R0 >= d.c) continue;
@@ -449,7 +460,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 g.c_in_arena = 0;
                 g.beta0 = 0;
                 g.lower = 1;
-                add_gemm(g, batch_a, tiles_a, o.tile);
+                add_gemm(g, batch_a, tiles_a, tile_a);
                 if (E2 < d.c) {
                     GemmJob h = g;
                     h.a_off = d.lofs + (int64_t)R0 * d.n + E2;
@@ -457,14 +468,14 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     h.c_off = d.lofs + (int64_t)E2 * d.n + E2;
                     h.M = d.n - E2;
                     h.N = d.c - E2;
-                    add_gemm(h, batch_b, tiles_b, o.tile);
+                    add_gemm(h, batch_b, tiles_b, tile_b);
                 }
             }
             const int wait_prev = side_last;
-            flush_gemm(batch_a, tiles_a, L, o.tile);
+            flush_gemm(batch_a, tiles_a, L, tile_a);
             if (tiles_a > 0 && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
             if (tiles_b > 0) {
-                flush_gemm(batch_b, tiles_b, L, o.tile);
+                flush_gemm(batch_b, tiles_b, L, tile_b);
                 Step& sb = P.steps.back();
                 sb.stream = 1;
                 sb.wait_ev = inner_done;
@@ -480,6 +491,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         {
             std::vector<GemmJob> batch;
             int tiles = 0;
+            const int tile_s = tile_for(o.tile);
             for (int f : fl) {
                 const FrontDesc& d = P.fronts[f];
                 if (d.m == 0) continue;
@@ -494,9 +506,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 g.c_in_arena = 1;
                 g.beta0 = 1;
                 g.lower = 1;
-                add_gemm(g, batch, tiles, o.tile);
+                add_gemm(g, batch, tiles, tile_s);
             }
-            flush_gemm(batch, tiles, L, o.tile);
+            flush_gemm(batch, tiles, L, tile_s);
         }
         add_ea(L, false);
     }

@@ -183,6 +183,10 @@ struct PlanOptions {
     int nb = 64;                 // block width of the blocked panel path
     int nb_outer = 256;          // outer block width (trailing updates with K = nb_outer)
     int tile = 64;               // GEMM tile (rows = cols)
+    // per-launch GEMM tile sizes (64 or 128, <= tile) chosen by the
+    // autotuner, indexed by the GEMM launch's ordinal in the schedule;
+    // empty: 64 for the narrow in-panel updates, `tile` for the rest
+    std::vector<int> gemm_tiles;
     int ea_chunk = 16384;        // target entries per extend-add workgroup
 };
 

@@ -72,6 +72,16 @@ class Factorization:
         return {k: {"ms": out[4 * t], "launches": int(out[4 * t + 1]), "flops": out[4 * t + 2],
                     "bytes": out[4 * t + 3]} for t, k in enumerate(L.KIND_NAMES)}
 
+    def autotune(self, ax):
+        """Pick 64- or 128-row GEMM tiles per launch by timing both on `ax`;
+        returns the number of launches on 64-row tiles.  Leaves `ax` factored."""
+        torch = _torch()
+        ax = ax.to(device="cuda", dtype=torch.float64).contiguous()
+        self._ax = ax
+        out = C.c_int64()
+        L.check(L.lib().mdgpu_numeric_autotune(self._h, _dptr(ax), _stream(), C.byref(out)))
+        return out.value
+
     def factorize_steps(self, ax):
         """Factorize with a HIP-event time for every launch of the schedule:
         array of rows (kernel class, ms, workgroups, flops, bytes, level)."""

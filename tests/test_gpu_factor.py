@@ -159,3 +159,21 @@ def test_lookahead_outer_blocks(device, monkeypatch, nb_outer):
         for rep in range(2):
             G.factorize(ax)
             assert np.array_equal(G.export_L()[2], Lx), (name, rep)
+
+
+def test_autotune_keeps_the_factor(device):
+    """The autotuner only changes GEMM tile sizes: the FP64 factor still
+    matches the oracle, and graph replays after tuning reproduce it."""
+    import torch
+    name, make, mode, mz = [c for c in CASES if c[0] == "st27_12_rn100000"][0]
+    p, i, x = make()
+    S = symbolic.analyze(p, i, True, mode, mz)
+    lp, Li_ref, Lx_ref, _, bad = oracle_factor(p, i, x, S)
+    ax = torch.from_numpy(x).to(device)
+    F = Factorization(S, "fp64")
+    n64 = F.autotune(ax)
+    assert 0 <= n64 <= F.info()["steps"]
+    _, Li, Lx = F.export_L()
+    assert np.array_equal(Li, Li_ref) and rel_err(Lx, Lx_ref) <= 1e-10
+    F.factorize(ax)
+    assert np.array_equal(F.export_L()[2], Lx)

@@ -113,6 +113,22 @@ __global__ __launch_bounds__(kThreads) void extend_add_kernel(const EaItem* __re
     }
 }
 
+// 1/sqrt(d) of a pivot d > 0 from the hardware reciprocal square root:
+// v_rsq_f32 is within 1 ulp; v_rsq_f64 gives about 23 correct bits, and two
+// Newton steps y <- y + y (1 - d y^2) / 2 bring it to double precision.  The
+// pivot chain of a column is then a handful of dependent instructions, not a
+// correctly rounded sqrt followed by a correctly rounded division.
+__device__ inline float pivot_rsqrt(float d) { return __builtin_amdgcn_rsqf(d); }
+__device__ inline double pivot_rsqrt(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double r = fma(-d * y, y, 1.0);
+        y = fma(0.5 * y, r, y);
+    }
+    return y;
+}
+
 // ------------------------------------------------------ dense: LDS panels
 // Partial Cholesky of a whole panel held in LDS: for each pivot column,
 // sqrt, scale, rank-1 update of the remaining pivot columns (the unblocked
@@ -135,9 +151,9 @@ __global__ __launch_bounds__(kThreads) void panel_kernel(const int* __restrict__
         if (threadIdx.x == 0) {
             T dj = P[j + j * n];
             if (!(dj > T(0))) atomicMin(info, d.k + j);
-            T rt = sqrt(dj);
-            P[j + j * n] = rt;
-            pivinv = T(1) / rt;
+            const T is = pivot_rsqrt(dj);
+            P[j + j * n] = dj * is;
+            pivinv = is;
         }
         __syncthreads();
         const T s = pivinv;
@@ -218,6 +234,25 @@ __device__ __forceinline__ void trsm_row16(T (&x)[64], const T* Lc, const T* rd)
 // registers (pivot and column entries broadcast by v_readlane), the trailing
 // part of the block updated by all four wavefronts from LDS.  bad = first
 // non-positive pivot (64 if none).  Called by all threads of the workgroup.
+// Trailing update As(i, j) -= sum_t As(i, c0+t) As(j, c0+t) over the lower
+// triangle i >= j of rows/cols [c0 + 16, 64), W = 48 - c0.  The triangle is
+// folded onto a (W + 1) x (W / 2) rectangle so no work item is idle: (r, c)
+// with r > c is (r - 1, c), otherwise (W - 1 - r, W - 1 - c).
+template <typename U, int W>
+__device__ inline void chol64_trailing(U* As, int c0, int tid) {
+    constexpr int H = W / 2;
+    for (int idx = tid; idx < (W + 1) * H; idx += kThreads) {
+        const int r = idx / H, c = idx - r * H;
+        const bool low = r > c;
+        const int i = c0 + 16 + (low ? r - 1 : W - 1 - r);
+        const int j = c0 + 16 + (low ? c : W - 1 - c);
+        U acc = U(0);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) acc += As[i + 65 * (c0 + t)] * As[j + 65 * (c0 + t)];
+        As[i + 65 * j] -= acc;
+    }
+}
+
 template <typename U>
 __device__ void chol64_panels(U* As, int tid, int& bad) {
     const int wave = tid >> 6, lane = tid & 63;
@@ -232,9 +267,8 @@ __device__ void chol64_panels(U* As, int tid, int& bad) {
             for (int j = 0; j < 16; ++j) {
                 const U djj = bcast(a[j], j);
                 bad = (!(djj > U(0)) && bad == 64) ? c0 + j : bad;
-                const U sq = sqrt(djj);
-                const U is = U(1) / sq;
-                a[j] = (lane == j) ? sq : a[j] * is;
+                const U is = pivot_rsqrt(djj);
+                a[j] = (lane == j ? djj : a[j]) * is;
 #pragma unroll
                 for (int k = j + 1; k < 16; ++k) a[k] -= a[j] * bcast(a[j], k);
             }
@@ -245,15 +279,9 @@ __device__ void chol64_panels(U* As, int tid, int& bad) {
             }
         }
         __syncthreads();
-        const int w = 48 - c0;          // trailing rows/cols [c0+16, 64)
-        for (int idx = tid; idx < w * w; idx += kThreads) {
-            const int i = c0 + 16 + idx % w, j = c0 + 16 + idx / w;
-            if (i < j) continue;
-            U acc = U(0);
-#pragma unroll
-            for (int t = 0; t < 16; ++t) acc += As[i + 65 * (c0 + t)] * As[j + 65 * (c0 + t)];
-            As[i + 65 * j] -= acc;
-        }
+        if (p == 0) chol64_trailing<U, 48>(As, c0, tid);
+        else if (p == 1) chol64_trailing<U, 32>(As, c0, tid);
+        else if (p == 2) chol64_trailing<U, 16>(As, c0, tid);
         __syncthreads();
     }
 }

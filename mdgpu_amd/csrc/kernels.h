@@ -35,7 +35,8 @@ void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, bool fma
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s);
 
 // solves (compute in FP64, factor stored as T); the forward result goes to
-// Y, the backward result to xp
+// Y (which the backward solve of large fronts then updates in place), the
+// backward result to xp
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s);
 void launch_permute_out(const double* xp, const int* perm, double* x, int n, hipStream_t s);
 template <typename T>
@@ -48,16 +49,21 @@ void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* 
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                        double* xp, double* su, const int64_t* suofs, hipStream_t s);
 template <typename T>
-void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* X256,
+void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
                     double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s);
+// backward: BWD_UPD subtracts the update rows' dot products from Y, BWD_BLK
+// solves a block into xp and subtracts its rows' dot products from Y
 template <typename T>
-void launch_bwd_blk(const SolveItem* items, int count, const SolveItem* fin, int nfin, const FrontDesc* fronts,
-                    const int* q, const T* L, const T* X256, double* xp, const double* Y, double* partial,
-                    hipStream_t s);
-// inverses of the kSolveBW-column diagonal blocks (Plan::inv256), after launch_inv
+void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
+                    const double* xp, double* Y, hipStream_t s);
+template <typename T>
+void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
+                    double* xp, double* Y, hipStream_t s);
+// inverses of the kSolveBW-column diagonal blocks (Plan::inv256) and their
+// packed forms (Plan::invpk), after launch_inv
 template <typename T>
 void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,
-                   hipStream_t s);
+                   T* Xpk, hipStream_t s);
 
 // refinement helpers (FP64)
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);

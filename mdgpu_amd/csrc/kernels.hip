@@ -42,6 +42,23 @@ __device__ __forceinline__ void load32_strided(T (&v)[32], const T* __restrict__
     for (int k = 0; k < 32; ++k) asm volatile("" : "+v"(v[k]));
 }
 
+// N loads p[0], p[stride], ... with a moving pointer and no wait: the
+// caller makes them wait together (keep_in_flight) after issuing all the
+// loads it wants in flight.
+template <int N, typename T>
+__device__ __forceinline__ void issue_strided(T (&v)[N], const T* __restrict__ p, int64_t stride) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        v[k] = *p;
+        p += stride;
+    }
+}
+template <int N, typename T>
+__device__ __forceinline__ void keep_in_flight(T (&v)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) asm volatile("" : "+v"(v[k]));
+}
+
 // ---------------------------------------------------------------- fill / A
 template <typename T>
 __global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
@@ -457,7 +474,7 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restrict__ jobs,
                                                           const FrontDesc* __restrict__ fronts,
                                                           const T* __restrict__ L, const T* __restrict__ inv,
-                                                          T* __restrict__ X256) {
+                                                          T* __restrict__ X256, T* __restrict__ Xpk) {
     __shared__ T Al[64 * 33];
     __shared__ T Bl[32 * 65];
     const InvJob jb = jobs[blockIdx.x];
@@ -539,6 +556,22 @@ __global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restri
             }
             __syncthreads();
         }
+    }
+    // packed forms for the blocked solves (kPk layout): X, then R X^T R,
+    // whose entry (r, c) is X(255 - c, 255 - r)
+    T* __restrict__ pk = Xpk + (d.dinv256 / (kSolveBW * kSolveBW) + j0 / kSolveBW) * 2 * kPk;
+    for (int idx = tid; idx < kPk; idx += kThreads) {
+        const int k = idx / kSolveThreads, t = idx - k * kSolveThreads;
+        const int p = t >> 2, sg = t & 3, e = pk_first(sg) + k;
+        T vf = T(0), vr = T(0);
+        if (e < pk_first(sg + 1)) {
+            const bool lo = e <= p;
+            const int r = lo ? p : kSolveBW - 1 - p, c = lo ? e : e - p - 1;
+            vf = X[r + kSolveBW * c];
+            vr = X[(kSolveBW - 1 - c) + kSolveBW * (kSolveBW - 1 - r)];
+        }
+        pk[idx] = vf;
+        pk[kPk + idx] = vr;
     }
 }
 
@@ -1052,8 +1085,9 @@ __global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ l
 
 // ------------------------------------------------- solves of large fronts
 // Fronts with more than kSolveSmall pivots: forward / backward substitution
-// kSolveBW pivot columns at a time, the rows below each block handled by
-// many workgroups (row chunks of kSolveFwdRows / kSolveBwdRows).
+// kSolveBW pivot columns (a block) per launch, spread over many workgroups
+// of kSolveThreads threads: row chunks of kSolveFwdRows below the block
+// (forward), column chunks of kSolveCols left of it (backward).
 
 // Forward, step 0: clear the front's update vector and gather its
 // children's update vectors (the first phase of fwd_kernel).
@@ -1082,169 +1116,239 @@ __global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restr
     }
 }
 
-// Forward, one 256-column block of a large front per launch.  Every
-// workgroup solves the block's own triangle (four 64-column sub-blocks:
-// y_s = inv(L_s) v_s with the stored inverse, then the later sub-blocks'
-// right-hand sides are updated), the `first` one stores y in Y, and each
-// updates its kSolveFwdRows rows [a, a + kSolveFwdRows) below the block,
-// four lanes per row (64 columns each, partial sums added in a fixed
-// order).  v is final on entry: earlier blocks were applied by earlier
-// launches.
+// z = M v for a packed lower-triangular kSolveBW x kSolveBW matrix M (kPk
+// layout) of which this thread holds its kPkSlots entries m[]; vb and zb
+// are LDS vectors of kSolveBW entries, red an LDS scratch of 2
+// kSolveThreads.  Each output is the sum of its pair's four partial sums in
+// a fixed order.  Called by all kSolveThreads threads; ends with a barrier.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
-                                                           const FrontDesc* __restrict__ fronts,
-                                                           const T* __restrict__ L, const T* __restrict__ X256,
-                                                           double* __restrict__ xp, double* __restrict__ Y,
-                                                           double* __restrict__ su,
-                                                           const int64_t* __restrict__ suofs) {
-    __shared__ double vb[kSolveBW];
-    __shared__ double red[4][64];
+__device__ __forceinline__ void packed_trmv(const T (&m)[kPkSlots], const double* vb, double* zb, double* red,
+                                            int tid) {
+    const int p = tid >> 2, e0 = pk_first(tid & 3);
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kPkSlots; ++k) {
+        const int e = e0 + k;
+        const bool lo = e <= p;
+        const double t = to_d(m[k]) * vb[lo ? e : min(e - p - 1, kSolveBW - 1)];
+        a0 += lo ? t : 0.0;
+        a1 += lo ? 0.0 : t;
+    }
+    red[tid] = a0;
+    red[kSolveThreads + tid] = a1;
+    __syncthreads();
+    if (tid < kSolveBW) {
+        const bool top = tid < kSolveBW / 2;
+        const double* r = red + (top ? 4 * tid : kSolveThreads + 4 * (kSolveBW - 1 - tid));
+        zb[tid] = (r[0] + r[1]) + (r[2] + r[3]);
+    }
+    __syncthreads();
+}
+
+// 32 entries of one row of a front panel P (n rows, column-major), columns
+// col, col + 1, ... of which only the first `valid` are real: the loads of
+// the others repeat the last real column (in range) and read as zero.
+template <typename T>
+__device__ __forceinline__ void issue_row32(T (&v)[32], const T* __restrict__ P, int row, int col, int valid,
+                                            int64_t n) {
+    if (valid >= 32) {
+        issue_strided<32>(v, P + row + (int64_t)col * n, n);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) v[k] = P[row + (int64_t)(col + min(k, valid - 1)) * n];
+    }
+}
+
+// packed block inverse of block j0 of front d: forward form, backward form at + kPk
+__device__ inline int64_t pk_offset(const FrontDesc& d, int j0) {
+    return (d.dinv256 / (kSolveBW * kSolveBW) + j0 / kSolveBW) * 2 * kPk;
+}
+
+// Forward, one kSolveBW-column block of a large front per launch.  Every
+// workgroup solves the block's triangle, y = X v with the packed block
+// inverse X (v is final on entry: earlier blocks were applied by earlier
+// launches), the `first` one stores y in Y, and each subtracts the block's
+// contribution from its kSolveFwdRows rows [a, a + kSolveFwdRows) below the
+// block: thread (r, cs) takes row a + r and 32 columns, all of its loads
+// (those 32 and the kPkSlots of X) in flight together; the eight column
+// segments of a row are added in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
+                                                                const FrontDesc* __restrict__ fronts,
+                                                                const T* __restrict__ L, const T* __restrict__ Xpk,
+                                                                double* __restrict__ xp, double* __restrict__ Y,
+                                                                double* __restrict__ su,
+                                                                const int64_t* __restrict__ suofs) {
+    __shared__ double vb[kSolveBW], yb[kSolveBW];
+    __shared__ double red[2 * kSolveThreads];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
     const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
     const T* __restrict__ P = L + d.lofs;
-    const int tid = threadIdx.x;
-    vb[tid] = tid < bw ? xp[d.k + j0 + tid] : 0.0;
-    __syncthreads();
-    // y = inv(L_B) v with the block inverse (zero above its diagonal)
-    {
-        const T* __restrict__ X = X256 + d.dinv256 + (int64_t)(j0 / kSolveBW) * kSolveBW * kSolveBW;
-        double y = 0.0;
-        if (tid < bw) {
-            for (int l0 = 0; l0 <= tid; l0 += 32) {
-                T v[32];
-                load32_strided(v, X + tid + (int64_t)kSolveBW * l0, kSolveBW);
-#pragma unroll
-                for (int k = 0; k < 32; ++k) y += to_d(v[k]) * vb[l0 + k];
-            }
-        }
-        __syncthreads();
-        vb[tid] = y;
-        __syncthreads();
-    }
-    if (it.first && tid < bw) Y[d.k + j0 + tid] = vb[tid];
-    const int r = tid & 63, cq = tid >> 6;
+    const int tid = threadIdx.x, r = tid & 63, cs = tid >> 6;
     const int i = it.a + r;
-    const int c0 = cq * 64, c1 = min(c0 + 64, bw);
-    double acc = 0.0;
-    if (i < n && c0 < c1) {
-        const T* __restrict__ pr = P + i + (int64_t)(j0 + c0) * n;
-#pragma unroll 16
-        for (int j = 0; j < c1 - c0; ++j) acc += to_d(pr[(int64_t)j * n]) * vb[c0 + j];
-    }
-    red[cq][r] = acc;
+    const int cvalid = min(32, bw - 32 * cs);          // real columns of this segment
+    T lv[32], m[kPkSlots];
+    if (cvalid > 0) issue_row32(lv, P, min(i, n - 1), j0 + 32 * cs, cvalid, n);
+    issue_strided<kPkSlots>(m, Xpk + pk_offset(d, j0) + tid, kSolveThreads);
+    if (tid < kSolveBW) vb[tid] = tid < bw ? xp[d.k + j0 + tid] : 0.0;
+    keep_in_flight(lv);
+    keep_in_flight(m);
     __syncthreads();
-    if (tid < 64 && i < n) {
-        const double sum = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
-        if (i < d.c) xp[d.k + i] -= sum;
-        else su[suofs[it.front] + i - d.c] -= sum;
+    packed_trmv(m, vb, yb, red, tid);
+    if (it.first && tid < bw) Y[d.k + j0 + tid] = yb[tid];
+    double acc = 0.0;
+    if (cvalid > 0) {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) acc += (k < cvalid ? to_d(lv[k]) : 0.0) * yb[32 * cs + k];
+    }
+    red[tid] = acc;
+    __syncthreads();
+    if (tid < kSolveFwdRows && it.a + tid < n) {
+        const double* rr = red + tid;
+        const double sum = ((rr[0] + rr[64]) + (rr[128] + rr[192])) + ((rr[256] + rr[320]) + (rr[384] + rr[448]));
+        const int ii = it.a + tid;
+        if (ii < d.c) xp[d.k + ii] -= sum;
+        else su[suofs[it.front] + ii - d.c] -= sum;
     }
 }
 
-// Sum over the 64 lanes of a wavefront of 64 values per lane: on return,
-// acc[0] of lane l holds sum over lanes of acc[l] (column l).  A transpose
-// reduction: at each of 6 steps every lane keeps one half of its remaining
-// values and adds its partner's copy of that half (63 shuffles in all).
-__device__ __forceinline__ void wave_transpose_sum64(double (&acc)[64], int lane) {
+// acc[k], k < 32, summed over the 64 lanes of the wavefront: afterwards
+// acc[0] of lanes l and l + 32 holds the sum of acc[l & 31] (transpose
+// reduction within each half, then the halves added).
+__device__ __forceinline__ void wave_transpose_sum32(double (&acc)[32], int lane) {
 #pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) {
+    for (int s = 16; s >= 1; s >>= 1) {
         const bool upper = (lane & s) != 0;
 #pragma unroll
         for (int k = 0; k < s; ++k) {
             double lo = acc[k], hi = acc[k + s];
-            // the empty asm keeps the selects on values: folded into a
-            // select of addresses, they would force acc[] into scratch
             asm volatile("" : "+v"(lo), "+v"(hi));
             const double send = upper ? lo : hi;
             const double keep = upper ? hi : lo;
             acc[k] = keep + __shfl_xor(send, s, 64);
         }
     }
+    acc[0] += __shfl_xor(acc[0], 32, 64);
 }
 
-// Backward, one 256-column block of a large front: the partial dot
-// products.  Each workgroup forms the dot products of the block's columns
-// with x over its kSolveBwdRows rows below the block: wavefront w takes
-// columns [64w, 64w+64), its lanes run down the rows (coalesced column
-// reads, 32 loads in flight per lane), and a transpose reduction sums each
-// column over the lanes.  One partial vector per workgroup (slot).
+// Column sums of a kSolveThreads-thread workgroup laid out as (row r =
+// tid & 255, column half h = tid >> 8), acc[k] = this thread's term of
+// column 32 h + k: reduced over the wavefront, then over the four
+// wavefronts of a half in a fixed order.  Returns the sum of column tid
+// for tid < 64 (0 elsewhere).  redc: LDS scratch of 8 * 32.
+__device__ __forceinline__ double column_sums64(double (&acc)[32], double* redc, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+    wave_transpose_sum32(acc, lane);
+    if (lane < 32) redc[wave * 32 + lane] = acc[0];
+    __syncthreads();
+    double sum = 0.0;
+    if (tid < 64) {
+        const double* rr = redc + 4 * (tid >> 5) * 32 + (tid & 31);
+        sum = (rr[0] + rr[32]) + (rr[64] + rr[96]);
+    }
+    return sum;
+}
+
+// Backward, once per level before the blocks: workgroup (front, 64 pivot
+// columns [a, a + 64)) subtracts from Y the dot products of its columns
+// with the front's update rows [c, n), whose solution entries (final: they
+// belong to ancestors) are read through the row list q.  Rows go 256 at a
+// time, the next chunk's loads issued before the current one is used.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void bwd_dot_kernel(const SolveItem* __restrict__ items,
-                                                           const FrontDesc* __restrict__ fronts,
-                                                           const int* __restrict__ qall, const T* __restrict__ L,
-                                                           const double* __restrict__ xp,
-                                                           double* __restrict__ partial) {
+__global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem* __restrict__ items,
+                                                                const FrontDesc* __restrict__ fronts,
+                                                                const int* __restrict__ qall,
+                                                                const T* __restrict__ L,
+                                                                const double* __restrict__ xp,
+                                                                double* __restrict__ Y) {
+    __shared__ double redc[8 * 32];
+    const SolveItem it = items[blockIdx.x];
+    const FrontDesc d = fronts[it.front];
+    const int n = d.n, c = d.c, c0 = it.a;
+    const T* __restrict__ P = L + d.lofs;
+    const int* __restrict__ q = qall + d.qofs;
+    const int tid = threadIdx.x, r = tid & 255, h = tid >> 8;
+    const int col = c0 + 32 * h, cvalid = min(32, c - col);
+    double acc[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) acc[k] = 0.0;
+    if (cvalid > 0 && sizeof(T) == 4) {
+        T cur[32];
+        int r0 = c;
+        issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, n);
+        double xv = r0 + r < n ? xp[q[r0 + r]] : 0.0;
+        for (; r0 < n; r0 += 256) {
+            T nxt[32];
+            double xn = 0.0;
+            const bool more = r0 + 256 < n;
+            if (more) {
+                issue_row32(nxt, P, min(r0 + 256 + r, n - 1), col, cvalid, n);
+                xn = r0 + 256 + r < n ? xp[q[r0 + 256 + r]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 32; ++k) acc[k] += (k < cvalid ? to_d(cur[k]) : 0.0) * xv;
+            if (more) {
+#pragma unroll
+                for (int k = 0; k < 32; ++k) cur[k] = nxt[k];
+                xv = xn;
+            }
+        }
+    } else if (cvalid > 0) {
+        // FP64: one chunk in flight (two would not fit the register budget)
+        for (int r0 = c; r0 < n; r0 += 256) {
+            T cur[32];
+            issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, n);
+            const double xv = r0 + r < n ? xp[q[r0 + r]] : 0.0;
+            keep_in_flight(cur);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) acc[k] += (k < cvalid ? to_d(cur[k]) : 0.0) * xv;
+        }
+    }
+    const double sum = column_sums64(acc, redc, tid);
+    if (tid < kSolveCols && c0 + tid < c) Y[d.k + c0 + tid] -= sum;
+}
+
+// Backward, one kSolveBW-column block of a large front per launch (blocks
+// from the last): every workgroup solves the block, x = X^T y, as
+// (R X^T R)(R y) with the packed backward form (y final on entry: the
+// update rows and the later blocks were applied by earlier launches).  The
+// `first` workgroup stores x; each other one owns 64 pivot columns
+// [a, a + 64) left of the block and subtracts their dot products with the
+// block's rows from Y.  Its 32 loads per thread of those rows are issued
+// with the inverse's, before x is known.
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void bwd_blk_kernel(const SolveItem* __restrict__ items,
+                                                                const FrontDesc* __restrict__ fronts,
+                                                                const T* __restrict__ L, const T* __restrict__ Xpk,
+                                                                double* __restrict__ xp, double* __restrict__ Y) {
+    __shared__ double vb[kSolveBW], zb[kSolveBW];
+    __shared__ double red[2 * kSolveThreads];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
     const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
     const T* __restrict__ P = L + d.lofs;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int c0 = 64 * wave;                         // this wavefront's columns
-    if (c0 >= bw) return;
-    double acc[64];
-#pragma unroll
-    for (int k = 0; k < 64; ++k) acc[k] = 0.0;
-    const int cmax = bw - c0 - 1;                      // clamp: columns past bw read column bw-1
-    for (int r0 = it.a; r0 < min(it.a + kSolveBwdRows, n); r0 += 64) {
-        const int r = r0 + lane;
-        const bool ok = r < n;
-        const double xv = ok ? xp[qall[d.qofs + r]] : 0.0;
-        const T* __restrict__ col = P + (ok ? r : r0) + (int64_t)(j0 + c0) * n;
-#pragma unroll
-        for (int h = 0; h < 64; h += 32) {
-            T v[32];
-            if (cmax >= 63) {
-                load32_strided(v, col + (int64_t)h * n, n);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 32; ++k) v[k] = col[(int64_t)min(h + k, cmax) * n];
-            }
-#pragma unroll
-            for (int k = 0; k < 32; ++k) acc[h + k] += to_d(v[k]) * xv;
-        }
-    }
-    wave_transpose_sum64(acc, lane);
-    if (c0 + lane < bw) partial[(int64_t)it.slot * kSolveBW + c0 + lane] = acc[0];
-}
-
-// Backward, one 256-column block of a large front: the finish, one
-// workgroup per block.  Adds the partial vectors in slot order to
-// v = y - (dot products with the rows below), then solves the block's
-// triangle from its last 64-column sub-block: the dot products with the
-// block rows already solved (one 64-row chunk per wavefront, transpose
-// reduction, chunks added in order), then x_s = inv(L_s)^T v_s with the
-// inverse staged in LDS.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void bwd_fin_kernel(const SolveItem* __restrict__ items,
-                                                           const FrontDesc* __restrict__ fronts,
-                                                           const T* __restrict__ X256, double* __restrict__ xp,
-                                                           const double* __restrict__ Y,
-                                                           const double* __restrict__ partial) {
-    __shared__ double vb[kSolveBW];
-    const SolveItem it = items[blockIdx.x];
-    const FrontDesc d = fronts[it.front];
-    const int j0 = it.j0, bw = min(kSolveBW, d.c - j0);
-    const int tid = threadIdx.x;
-    double v = 0.0;
-    if (tid < bw) {
-        v = Y[d.k + j0 + tid];
-#pragma unroll 8
-        for (int sl = 0; sl < it.gcount; ++sl) v -= partial[(int64_t)(it.gfirst + sl) * kSolveBW + tid];
-    }
-    vb[tid] = v;
+    const int tid = threadIdx.x, r = tid & 255, h = tid >> 8;
+    const bool own = !it.first;
+    T lv[32], m[kPkSlots];
+    if (own) issue_strided<32>(lv, P + (j0 + min(r, bw - 1)) + (int64_t)(it.a + 32 * h) * n, n);
+    issue_strided<kPkSlots>(m, Xpk + pk_offset(d, j0) + kPk + tid, kSolveThreads);
+    if (tid < kSolveBW) vb[tid] = kSolveBW - 1 - tid < bw ? Y[d.k + j0 + kSolveBW - 1 - tid] : 0.0;
+    if (own) keep_in_flight(lv);
+    keep_in_flight(m);
     __syncthreads();
-    // x = inv(L_B)^T v: column i of the block inverse, read contiguously
-    if (tid < bw) {
-        const T* __restrict__ X = X256 + d.dinv256 + (int64_t)(j0 / kSolveBW) * kSolveBW * kSolveBW;
-        double x = 0.0;
-        for (int l0 = tid & ~31; l0 < kSolveBW; l0 += 32) {
-            T w[32];
-            load32_strided(w, X + (int64_t)kSolveBW * tid + l0, 1);
-#pragma unroll
-            for (int k = 0; k < 32; ++k) x += to_d(w[k]) * vb[l0 + k];
-        }
-        xp[d.k + j0 + tid] = x;
+    packed_trmv(m, vb, zb, red, tid);
+    // x(i) = zb[255 - i]
+    if (!own) {
+        if (tid < bw) xp[d.k + j0 + tid] = zb[kSolveBW - 1 - tid];
+        return;
     }
+    const double xv = r < bw ? zb[kSolveBW - 1 - r] : 0.0;
+    double acc[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) acc[k] = to_d(lv[k]) * xv;
+    const double sum = column_sums64(acc, red, tid);
+    if (tid < kSolveCols) Y[d.k + it.a + tid] -= sum;
 }
 
 // -------------------------------------------------------------- refinement
@@ -1418,21 +1522,24 @@ void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, cons
     if (count > 0) fwd_gather_kernel<<<count, kThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
 }
 template <typename T>
-void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* X256,
+void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
                     double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_blk_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, L, X256, xp, Y, su, suofs);
+    if (count > 0) fwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, L, Xpk, xp, Y, su, suofs);
 }
 template <typename T>
-void launch_bwd_blk(const SolveItem* items, int count, const SolveItem* fin, int nfin, const FrontDesc* fronts,
-                    const int* q, const T* L, const T* X256, double* xp, const double* Y, double* partial,
-                    hipStream_t s) {
-    if (count > 0) bwd_dot_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, q, L, xp, partial);
-    if (nfin > 0) bwd_fin_kernel<T><<<nfin, kThreads, 0, s>>>(fin, fronts, X256, xp, Y, partial);
+void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
+                    const double* xp, double* Y, hipStream_t s) {
+    if (count > 0) bwd_upd_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, q, L, xp, Y);
+}
+template <typename T>
+void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
+                    double* xp, double* Y, hipStream_t s) {
+    if (count > 0) bwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, L, Xpk, xp, Y);
 }
 template <typename T>
 void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,
-                   hipStream_t s) {
-    if (count > 0) inv256_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv, X256);
+                   T* Xpk, hipStream_t s) {
+    if (count > 0) inv256_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv, X256, Xpk);
 }
 
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
@@ -1472,7 +1579,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
-    template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
+    template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmJob*, int, int, int, bool, T*, T*, hipStream_t);                         \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \
@@ -1481,8 +1588,10 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);               \
     template void launch_fwd_blk<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
                                     double*, double*, const int64_t*, hipStream_t);                          \
-    template void launch_bwd_blk<T>(const SolveItem*, int, const SolveItem*, int, const FrontDesc*, const int*, \
-                                    const T*, const T*, double*, const double*, double*, hipStream_t);
+    template void launch_bwd_upd<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,           \
+                                    const double*, double*, hipStream_t);                                    \
+    template void launch_bwd_blk<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
+                                    double*, hipStream_t);
 
 MDGPU_INSTANTIATE(double)
 MDGPU_INSTANTIATE(float)

@@ -51,6 +51,7 @@ struct mdgpu_numeric {
     int* dBlkCnt = nullptr;          // arrival counters of the block-column jobs
     mdgpu::InvJob* dInvJobs = nullptr;
     void* dInv256 = nullptr;              // kSolveBW-column block inverses (solves)
+    void* dInvPk = nullptr;               // their packed forms (Plan::invpk)
     mdgpu::InvJob* dInv256Jobs = nullptr;
     int* dPanel = nullptr;
     int* dLevelList = nullptr;
@@ -89,8 +90,6 @@ struct mdgpu_numeric {
     double* dX = nullptr;
     int* dSvFronts = nullptr;
     mdgpu::SolveItem* dSvItems = nullptr;
-    double* dSvPartial = nullptr;
-    mdgpu::SolveItem* dSvFin = nullptr;
     double* dY = nullptr;
     double anorm = -1;
     std::vector<double> anorm_seg;        // ||A_k||_1 per segment
@@ -249,7 +248,8 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
                 break;
             case mdgpu::STEP_INV256:
-                mdgpu::launch_inv256<T>(N->dInv256Jobs + st.first, st.count, N->dFronts, L, inv, (T*)N->dInv256, s);
+                mdgpu::launch_inv256<T>(N->dInv256Jobs + st.first, st.count, N->dFronts, L, inv, (T*)N->dInv256,
+                                        (T*)N->dInvPk, s);
                 break;
         }
         if (!ev && st.rec_ev >= 0) hipEventRecord(N->events[st.rec_ev], s);
@@ -280,7 +280,7 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
                                          s);
                 break;
             case mdgpu::SV_FWD_BLK:
-                mdgpu::launch_fwd_blk<T>(items, st.count, N->dFronts, L, (const T*)N->dInv256, N->dXp, N->dY, N->dSu,
+                mdgpu::launch_fwd_blk<T>(items, st.count, N->dFronts, L, (const T*)N->dInvPk, N->dXp, N->dY, N->dSu,
                                          N->dSuofs, s);
                 break;
         }
@@ -292,9 +292,11 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
             case mdgpu::SV_BWD_SMALL:
                 mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, inv, N->dXp, N->dY, s);
                 break;
+            case mdgpu::SV_BWD_UPD:
+                mdgpu::launch_bwd_upd<T>(items, st.count, N->dFronts, N->dQ, L, N->dXp, N->dY, s);
+                break;
             case mdgpu::SV_BWD_BLK:
-                mdgpu::launch_bwd_blk<T>(items, st.count, N->dSvFin + st.first2, st.count2, N->dFronts, N->dQ, L,
-                                         (const T*)N->dInv256, N->dXp, N->dY, N->dSvPartial, s);
+                mdgpu::launch_bwd_blk<T>(items, st.count, N->dFronts, L, (const T*)N->dInvPk, N->dXp, N->dY, s);
                 break;
         }
     }
@@ -401,6 +403,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, P.dinv * es, &N->dInv)) ||
         (st = upload(N, P.inv, (void**)&N->dInvJobs)) ||
         (st = alloc(N, P.inv256 * es, &N->dInv256)) ||
+        (st = alloc(N, P.invpk * es, &N->dInvPk)) ||
         (st = upload(N, P.inv256_jobs, (void**)&N->dInv256Jobs)) ||
         (st = upload(N, P.fronts, (void**)&N->dFronts)) ||
         (st = upload(N, P.kids, (void**)&N->dKids)) ||
@@ -434,8 +437,6 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dX)) ||
         (st = upload(N, P.sv_fronts, (void**)&N->dSvFronts)) ||
         (st = upload(N, P.sv_items, (void**)&N->dSvItems)) ||
-        (st = alloc(N, P.sv_partial * sizeof(double), (void**)&N->dSvPartial)) ||
-        (st = upload(N, P.sv_fin, (void**)&N->dSvFin)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dY)))
         return guard(st);
     if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess ||

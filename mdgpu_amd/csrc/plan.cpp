@@ -122,6 +122,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             off256 += (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW;
         }
         P.inv256 = std::max<int64_t>(off256, 1);
+        P.invpk = std::max<int64_t>(off256 / ((int64_t)kSolveBW * kSolveBW) * 2 * kPk, 1);
     }
 
     // relative maps child update rows -> parent rows (VectorMergeIndices)
@@ -208,8 +209,8 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     const FrontDesc& d = P.fronts[f];
                     if (j0 >= d.c) continue;
                     const int j1 = std::min(j0 + BW, d.c);
-                    P.sv_items.push_back({f, j1, 1, j0, 0, 0, 0, 0});
-                    for (int a = j1 + kSolveFwdRows; a < d.n; a += kSolveFwdRows) P.sv_items.push_back({f, a, 0, j0, 0, 0, 0, 0});
+                    P.sv_items.push_back({f, j1, 1, j0});
+                    for (int a = j1 + kSolveFwdRows; a < d.n; a += kSolveFwdRows) P.sv_items.push_back({f, a, 0, j0});
                     st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 16.0 * (d.n - j0);
                 }
                 st.count = (int)P.sv_items.size() - st.first;
@@ -226,31 +227,33 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 P.sv_bwd.push_back(st);
             }
             if (big.empty()) continue;
+            {
+                SolveStep st{SV_BWD_UPD, (int)P.sv_items.size(), 0, 0};
+                for (int f : big) {
+                    const FrontDesc& d = P.fronts[f];
+                    if (d.n == d.c) continue;
+                    for (int c0 = 0; c0 < d.c; c0 += kSolveCols) P.sv_items.push_back({f, c0, 0, 0});
+                    st.bytes += (double)(d.n - d.c) * d.c * o.elem_bytes + 12.0 * (d.n - d.c) + 16.0 * d.c;
+                }
+                st.count = (int)P.sv_items.size() - st.first;
+                if (st.count > 0) P.sv_bwd.push_back(st);
+            }
             int maxc = 0;
             for (int f : big) maxc = std::max(maxc, P.fronts[f].c);
             const int nblk = (maxc + BW - 1) / BW;
             for (int blk = nblk - 1; blk >= 0; blk--) {
                 const int j0 = blk * BW;
                 SolveStep st{SV_BWD_BLK, (int)P.sv_items.size(), 0, 0};
-                st.first2 = (int)P.sv_fin.size();
-                int slot = 0, group = 0;
                 for (int f : big) {
                     const FrontDesc& d = P.fronts[f];
                     if (j0 >= d.c) continue;
                     const int j1 = std::min(j0 + BW, d.c);
-                    const int nch = std::max(1, (d.n - j1 + kSolveBwdRows - 1) / kSolveBwdRows);
-                    for (int t = 0; t < nch; t++)
-                        P.sv_items.push_back({f, j1 + t * kSolveBwdRows, 0, j0, slot + t, group, slot, nch});
-                    P.sv_fin.push_back({f, j1, 0, j0, slot, group, slot, nch});
-                    slot += nch;
-                    group++;
-                    st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 12.0 * (d.n - j0);
+                    P.sv_items.push_back({f, j0, 1, j0});
+                    for (int c0 = 0; c0 < j0; c0 += kSolveCols) P.sv_items.push_back({f, c0, 0, j0});
+                    st.bytes += (double)(j1 - j0) * j1 * o.elem_bytes + 16.0 * j1;
                 }
                 st.count = (int)P.sv_items.size() - st.first;
-                st.count2 = (int)P.sv_fin.size() - st.first2;
                 P.sv_bwd.push_back(st);
-                P.sv_partial = std::max<int64_t>(P.sv_partial, (int64_t)slot * BW);
-                P.sv_groups = std::max(P.sv_groups, group);
             }
         }
     }
@@ -665,30 +668,22 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 const SolveItem& it = P.sv_items[t];
                 if (it.front < 0 || it.front >= nf) return "solve item front";
                 const FrontDesc& d = P.fronts[it.front];
-                if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW || it.j0 % 64) return "solve item block";
+                if (d.c <= kSolveSmall || d.dinv256 < 0) return "solve item front size";
+                if (st.kind == SV_BWD_UPD) {
+                    if (d.n <= d.c || it.a < 0 || it.a >= d.c || it.a % kSolveCols) return "solve update columns";
+                    continue;
+                }
+                if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW) return "solve item block";
                 const int j1 = std::min(it.j0 + kSolveBW, d.c);
-                if (it.a < j1 || (it.a >= d.n && it.a != j1)) return "solve item rows";
-                if (st.kind == SV_BWD_BLK) {
-                    if (it.slot < 0 || (int64_t)(it.slot + 1) * kSolveBW > P.sv_partial) return "solve partial slot";
-                    if (it.gfirst < 0 || it.gcount < 1 || (int64_t)(it.gfirst + it.gcount) * kSolveBW > P.sv_partial)
-                        return "solve partial group";
-                    if (it.slot < it.gfirst || it.slot >= it.gfirst + it.gcount) return "solve slot outside group";
-                    if (it.group < 0 || it.group >= P.sv_groups) return "solve counter";
+                if (st.kind == SV_FWD_BLK) {
+                    if (it.first ? it.a != j1 : (it.a < j1 + kSolveFwdRows || it.a >= d.n)) return "solve item rows";
+                } else if (it.first ? it.a != it.j0 : (it.a < 0 || it.a >= it.j0 || it.a % kSolveCols)) {
+                    return "solve item columns";
                 }
             }
         }
         return "";
     };
-    for (const SolveStep& st : P.sv_bwd) {
-        if (st.kind != SV_BWD_BLK) continue;
-        if (st.count2 <= 0 || st.first2 < 0 || st.first2 + st.count2 > (int)P.sv_fin.size()) return bad("solve finish slice");
-        for (int t = st.first2; t < st.first2 + st.count2; t++) {
-            const SolveItem& it = P.sv_fin[t];
-            if (it.front < 0 || it.front >= nf || it.j0 < 0 || it.j0 >= P.fronts[it.front].c) return bad("solve finish item");
-            if (it.gfirst < 0 || it.gcount < 1 || (int64_t)(it.gfirst + it.gcount) * kSolveBW > P.sv_partial)
-                return bad("solve finish partials");
-        }
-    }
     std::string se = check_solve(P.sv_fwd);
     if (se.empty()) se = check_solve(P.sv_bwd);
     if (!se.empty()) return bad(se);
@@ -708,6 +703,10 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         if (d.dinv256 >= 0 &&
             d.dinv256 + (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW > P.inv256)
             return bad("front block-inverse store");
+        if (d.dinv256 >= 0 && (d.dinv256 % ((int64_t)kSolveBW * kSolveBW) ||
+                               (d.dinv256 / ((int64_t)kSolveBW * kSolveBW) + (d.c + kSolveBW - 1) / kSolveBW) * 2 *
+                                       kPk > P.invpk))
+            return bad("front packed block-inverse store");
     }
     for (const InvJob& ij : P.inv256_jobs) {
         if (ij.front < 0 || ij.front >= nf || P.fronts[ij.front].dinv256 < 0) return bad("block-inverse job front");

@@ -104,35 +104,49 @@ struct Step {
 };
 
 // Solve schedule.  Fronts with at most kSolveSmall pivots are solved by one
-// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts go one 256-column
-// block per launch: FWD_BLK workgroups each take kSolveFwdRows rows below
-// the block (and recompute the block's own solve); a BWD_BLK step is two
-// launches, the partial dot products over kSolveBwdRows rows per workgroup
-// (sv_items), then one workgroup per block (sv_fin) that adds the partials
-// in slot order and solves the block.  The forward result goes to a
-// separate vector Y.
+// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts go kSolveBW pivot
+// columns (a block) per launch, kSolveThreads threads per workgroup, with
+// the block's inverse in packed form (see kPk):
+//  * FWD_BLK: every workgroup solves the block (y = X v) and updates its
+//    kSolveFwdRows rows below it; the `first` one stores y in Y.
+//  * BWD_UPD (once per level, before the blocks): each workgroup owns
+//    kSolveCols pivot columns of a front and subtracts their dot products
+//    with the front's update rows (solved by the ancestors, read through
+//    the row list q) from Y.
+//  * BWD_BLK (blocks from the last to the first): every workgroup solves
+//    the block (x = X^T y), the `first` one stores x; each other one owns
+//    kSolveCols pivot columns left of the block and subtracts the dot
+//    products of those columns with the block's rows from Y.  Right-looking:
+//    every entry of Y has one owner per launch, so no cross-workgroup
+//    reduction is needed and the summation order is fixed.
 constexpr int kSolveBW = 256;
 constexpr int kSolveSmall = 64;
+constexpr int kSolveThreads = 512;
 constexpr int kSolveFwdRows = 64;   // rows per forward workgroup below a block
-constexpr int kSolveBwdRows = 256;  // rows per backward workgroup below a block
-enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_BLK };
+constexpr int kSolveCols = 64;      // pivot columns per backward workgroup
+// Packed lower-triangular kSolveBW x kSolveBW matrix (a block inverse X for
+// the forward solve, R X^T R with R the order reversal for the backward):
+// row pairs (p, 255 - p), p < 128, of 257 entries each, entry e being
+// (p, e) for e <= p and (255 - p, e - p - 1) otherwise; thread t = 4p + s
+// holds entries [pk_first(s), pk_first(s + 1)) of pair p, entry
+// pk_first(s) + k at [k * kSolveThreads + t] (zero where unused).
+constexpr int kPkSlots = 65;
+constexpr int kPk = kPkSlots * kSolveThreads;
+inline constexpr int pk_first(int s) { return s == 0 ? 0 : 1 + 64 * s; }
+enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_UPD, SV_BWD_BLK };
 
 struct SolveItem {
     int front;
-    int a;        // first row (front-local) of this workgroup's 64-row chunk
-    int first;    // FWD_BLK: 1 if this item writes the block's y
-    int j0;       // first pivot column of the 64-column block (front-local)
-    int slot;     // BWD_BLK: partial-sum slot of this item
-    int group;    // BWD_BLK: counter index of the (front, block) group
-    int gfirst;   // BWD_BLK: first slot of the group
-    int gcount;   // BWD_BLK: items (slots) in the group
+    int a;        // FWD_BLK: first row (front-local) of the kSolveFwdRows-row chunk;
+                  // BWD_*: first pivot column of the kSolveCols-column chunk
+    int first;    // *_BLK: 1 if this item stores the block's solution
+    int j0;       // *_BLK: first pivot column of the block
 };
 
 struct SolveStep {
     int kind;
     int first, count;   // slice of sv_items (or of sv_fronts for *_SMALL / GATHER)
     double bytes;       // algorithmic bytes moved
-    int first2 = 0, count2 = 0;   // BWD_BLK: slice of sv_fin (one item per block)
 };
 
 struct Plan {
@@ -149,10 +163,7 @@ struct Plan {
     std::vector<int64_t> solve_uofs;   // per front offset (m_f entries)
     std::vector<int> sv_fronts;        // front lists of SMALL / GATHER steps
     std::vector<SolveItem> sv_items;
-    std::vector<SolveItem> sv_fin;     // BWD_BLK finishing items (front, j0, gfirst, gcount)
     std::vector<SolveStep> sv_fwd, sv_bwd;
-    int64_t sv_partial = 1;            // doubles of BWD_BLK partial sums
-    int sv_groups = 1;                 // counters needed by one BWD_BLK launch
     std::vector<EaItem> ea;
     std::vector<GemmJob> gemm;
     std::vector<BlkJob> blk;
@@ -166,8 +177,11 @@ struct Plan {
     // blocked solve path (c > kSolveSmall): block t of front f at
     // fronts[f].dinv256 + 65536 t, column-major with leading dimension 256,
     // X(r, i) = inv(L_B)(r, i) for r, i < bw, zero elsewhere.  Formed from
-    // the 64x64 inverses after them (STEP_INV256).
+    // the 64x64 inverses after them (STEP_INV256), which also writes the
+    // packed forms: block t of front f at (fronts[f].dinv256 / 65536 + t) * 2
+    // kPk, X then R X^T R (kPk elements each).
     int64_t inv256 = 1;
+    int64_t invpk = 1;                 // elements of the packed forms
     std::vector<InvJob> inv256_jobs;   // (front, first pivot column of the block)
     std::vector<Step> steps;           // the factorization schedule
     int nevents = 0;                   // events used by the look-ahead

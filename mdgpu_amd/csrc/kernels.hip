@@ -740,6 +740,8 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     // (BT/2) x (BT/2) quadrant of NB x NB 16x16 MFMA blocks
     constexpr int KT = sizeof(T) == 4 ? 32 : 16, LDK = KT + 4, NB = BT / 32, KSTEP = kThreads / BT;
     constexpr int PER = (BT * KT) / kThreads;      // staged elements per thread and operand
+    constexpr int G = 16 / sizeof(T), NG = PER / G; // k values per 16-byte LDS store, stores per operand
+    typedef T vec_t __attribute__((ext_vector_type(G)));
     typedef typename MfmaOp<T>::acc_t acc_t;
     __shared__ __align__(16) T As[BT * LDK];
     __shared__ __align__(16) T Bs[BT * LDK];
@@ -769,9 +771,10 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     const int wave = tid >> 6, lane = tid & 63;
     const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
     const int li = lane & 15, lg = lane >> 4;
-    // staging coordinates: element s of this thread is row r = tid % BT,
-    // k-offset kk = tid / BT + KSTEP s
-    const int r_st = tid & (BT - 1);
+    // staging coordinates: element G j + u of this thread is row r = tid % BT,
+    // k-offset kk = G (tid / BT + KSTEP j) + u: G consecutive k of one row go
+    // to LDS in one 16-byte store (rows 4 banks apart: conflict-free)
+    const int r_st = tid & (BT - 1), kgrp = tid / BT;
     const int ia = row0 + r_st, jb = col0 + r_st;
     const bool a_ok = ia < g.M, b_ok = jb < g.N;
     const T* __restrict__ Ap = A + ia;
@@ -781,10 +784,11 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     // outside the job (no per-element branch and wait)
     const T* __restrict__ Apc = a_ok ? Ap : A;
     const T* __restrict__ Bpc = b_ok ? Bp : B;
+    auto kof = [&](int s) { return G * (kgrp + KSTEP * (s / G)) + s % G; };
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
-            const int k = min(k0 + tid / BT + KSTEP * s, g.K - 1);
+            const int k = min(k0 + kof(s), g.K - 1);
             ra[s] = Apc[(int64_t)k * g.lda];
             rb[s] = Bpc[(int64_t)k * g.ldb];
         }
@@ -792,7 +796,7 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
         for (int s = 0; s < PER; ++s) asm volatile("" : "+v"(ra[s]), "+v"(rb[s]));
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
-            const bool kin = k0 + tid / BT + KSTEP * s < g.K;
+            const bool kin = k0 + kof(s) < g.K;
             ra[s] = (a_ok && kin) ? ra[s] : T(0);
             rb[s] = (b_ok && kin) ? rb[s] : T(0);
         }
@@ -807,10 +811,16 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     fetch(0);
     for (int k0 = 0; k0 < g.K; k0 += KT) {
 #pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            const int kk = tid / BT + KSTEP * s;
-            As[r_st * LDK + kk] = ra[s];
-            Bs[r_st * LDK + kk] = rb[s];
+        for (int j = 0; j < NG; ++j) {
+            vec_t va, vb;
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                va[u] = ra[G * j + u];
+                vb[u] = rb[G * j + u];
+            }
+            const int kk = G * (kgrp + KSTEP * j);
+            *reinterpret_cast<vec_t*>(&As[r_st * LDK + kk]) = va;
+            *reinterpret_cast<vec_t*>(&Bs[r_st * LDK + kk]) = vb;
         }
         __syncthreads();
         if (k0 + KT < g.K) fetch(k0 + KT);

@@ -743,8 +743,9 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     constexpr int G = 16 / sizeof(T), NG = PER / G; // k values per 16-byte LDS store, stores per operand
     typedef T vec_t __attribute__((ext_vector_type(G)));
     typedef typename MfmaOp<T>::acc_t acc_t;
-    __shared__ __align__(16) T As[BT * LDK];
-    __shared__ __align__(16) T Bs[BT * LDK];
+    // two LDS buffers: the next k-tile is stored while this one is multiplied
+    __shared__ __align__(16) T As[2][BT * LDK];
+    __shared__ __align__(16) T Bs[2][BT * LDK];
     int lo = 0, hi = njobs - 1;
     const int bid = xcd_tile(blockIdx.x, gridDim.x);
     while (lo < hi) {
@@ -780,8 +781,9 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     const T* __restrict__ Ap = A + ia;
     const T* __restrict__ Bp = B + jb;
     T ra[PER], rb[PER];
-    // loads at clamped in-range addresses, all in flight, then zeroed
-    // outside the job (no per-element branch and wait)
+    // loads at clamped in-range addresses; entries outside the job are
+    // zeroed when stored to LDS, so no wait sits between a fetch and the
+    // multiplications that follow it
     const T* __restrict__ Apc = a_ok ? Ap : A;
     const T* __restrict__ Bpc = b_ok ? Bp : B;
     auto kof = [&](int s) { return G * (kgrp + KSTEP * (s / G)) + s % G; };
@@ -792,14 +794,6 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
             ra[s] = Apc[(int64_t)k * g.lda];
             rb[s] = Bpc[(int64_t)k * g.ldb];
         }
-#pragma unroll
-        for (int s = 0; s < PER; ++s) asm volatile("" : "+v"(ra[s]), "+v"(rb[s]));
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            const bool kin = k0 + kof(s) < g.K;
-            ra[s] = (a_ok && kin) ? ra[s] : T(0);
-            rb[s] = (b_ok && kin) ? rb[s] : T(0);
-        }
     };
     acc_t acc[NB][NB];
 #pragma unroll
@@ -808,33 +802,45 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
-    fetch(0);
-    for (int k0 = 0; k0 < g.K; k0 += KT) {
+    auto store = [&](int buf, int k0) {
 #pragma unroll
         for (int j = 0; j < NG; ++j) {
             vec_t va, vb;
 #pragma unroll
             for (int u = 0; u < G; ++u) {
-                va[u] = ra[G * j + u];
-                vb[u] = rb[G * j + u];
+                const bool kin = k0 + kof(G * j + u) < g.K;
+                va[u] = (a_ok && kin) ? ra[G * j + u] : T(0);
+                vb[u] = (b_ok && kin) ? rb[G * j + u] : T(0);
             }
             const int kk = G * (kgrp + KSTEP * j);
-            *reinterpret_cast<vec_t*>(&As[r_st * LDK + kk]) = va;
-            *reinterpret_cast<vec_t*>(&Bs[r_st * LDK + kk]) = vb;
+            *reinterpret_cast<vec_t*>(&As[buf][r_st * LDK + kk]) = va;
+            *reinterpret_cast<vec_t*>(&Bs[buf][r_st * LDK + kk]) = vb;
         }
-        __syncthreads();
-        if (k0 + KT < g.K) fetch(k0 + KT);
+    };
+    fetch(0);
+    store(0, 0);
+    if (KT < g.K) fetch(KT);
+    __syncthreads();
+    for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+        // tile k0 + KT (in registers) to the other buffer, whose last
+        // readers passed the barrier that ended the previous iteration
+        if (k0 + KT < g.K) {
+            store(cur ^ 1, k0 + KT);
+            if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+        }
+        const T* __restrict__ Ac = As[cur];
+        const T* __restrict__ Bc = Bs[cur];
 #pragma unroll
         for (int kq = 0; kq < KT; kq += 16) {
             T av[NB][4], bv[NB][4];   // [block][t]
 #pragma unroll
             for (int rb2 = 0; rb2 < NB; ++rb2)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) av[rb2][t] = As[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
+                for (int t = 0; t < 4; ++t) av[rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
 #pragma unroll
             for (int cb = 0; cb < NB; ++cb)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) bv[cb][t] = Bs[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
+                for (int t = 0; t < 4; ++t) bv[cb][t] = Bc[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll

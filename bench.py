@@ -10,6 +10,9 @@ value  = algorithmic factorization GFLOP/s of the FP32 path, flops =
          sum_j c_j^2 over the unpadded cs_counts column counts.
 Inputs (matrix values, right-hand side) are resident in HBM before timing;
 the symbolic analysis runs once, untimed (it is reused for refactorization).
+The C3 run also times the z-anisotropic variant (z-edge weights x 1e4, same
+pattern and analysis) and reports, for every workload, the fronts per level,
+the extend-add GB/s and the MFMA utilisation of the large GEMM launches.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1]
 """
@@ -64,6 +67,36 @@ def time_path(F, ax, b, steps, warmup, tol, maxit=50):
     total = time.perf_counter() - t_all
     return {"factor_s": float(np.mean(tf)), "refine_s": float(np.mean(tr)), "total_s": total,
             "eta": float(max(etas)), "iters": int(max(its)), "x": x}
+
+
+def level_report(S):
+    """Fronts per level of the front tree (level = height above the leaves,
+    the schedule's level): [level, fronts, max front rows, median front rows]."""
+    nodes = S.nodes()
+    lev = np.zeros(len(nodes), np.int64)
+    for f, nd in enumerate(nodes):          # postorder: children first
+        for k in nd[4]:
+            lev[f] = max(lev[f], lev[k] + 1)
+    rows = np.array([nd[2] for nd in nodes], np.int64)
+    out = []
+    for L in range(int(lev.max()) + 1):
+        r = rows[lev == L]
+        out.append([L, int(r.size), int(r.max()), int(np.median(r))])
+    return out
+
+
+def kernel_report(prof, steps, prec):
+    """Extend-add HBM GB/s (algorithmic bytes / HIP-event time) and the MFMA
+    utilisation of the large Schur-complement launches (>= 1024 tiles)."""
+    ea = prof["extend_add"]
+    g = steps[steps[:, 0] == 5]
+    big = g[g[:, 2] >= 1024]
+    util = None
+    if len(big):
+        util = float(big[:, 3].sum() / (big[:, 1].sum() * 1e-3) / 1e12 / PEAK_TFLOPS[prec])
+    return {"extend_add_gbs": ea["bytes"] / max(ea["ms"], 1e-9) / 1e6,
+            "extend_add_frac_hbm": ea["bytes"] / max(ea["ms"], 1e-9) / 1e6 / PEAK_HBM_GBS,
+            "gemm_large_launches": int(len(big)), "gemm_large_mfma_util": util}
 
 
 def permuted_lower(p, i, x, perm):
@@ -180,6 +213,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--cpu-budget-gflop", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-aniso", action="store_true", help="skip the z-anisotropic C3 variant")
     ap.add_argument("--only", default="fp32,fp64,bf16x3,bf16",
                     help="comma-separated precisions to time (fp32 and fp64 are needed for the JSON line)")
     ap.add_argument("--steps-json", default="", help="write per-launch FP32 timings (class, ms, wgs, flops, bytes, level)")
@@ -218,9 +252,12 @@ def main():
         r["x"] = xs
         prof = F.factorize_profile(ax)
         r["profile"] = prof
-        if prec == "fp32" and args.steps_json:
-            with open(args.steps_json, "w") as fh:
-                json.dump(F.factorize_steps(ax).tolist(), fh)
+        if prec in ("fp32", "fp64"):
+            steps = F.factorize_steps(ax)
+            r["report"] = kernel_report(prof, steps, prec)
+            if prec == "fp32" and args.steps_json:
+                with open(args.steps_json, "w") as fh:
+                    json.dump(steps.tolist(), fh)
         r["info"] = F.info()
         res[prec] = r
         log(f"{prec}: factor {r['factor_s']*1e3:.2f} ms  refine {r['refine_s']*1e3:.2f} ms "
@@ -233,6 +270,30 @@ def main():
     if "fp32" not in res or "fp64" not in res:
         log("timed " + ",".join(res) + " only (the JSON line needs fp32 and fp64)")
         return
+    aniso = None
+    if args.workload == "c3" and not args.no_aniso:
+        # z-edges scaled by 1e4: same pattern, so the same analysis
+        pa, ia, xa = gen.stencil27(64, seed=0, anisotropic=True)
+        assert np.array_equal(pa, p) and np.array_equal(ia, i)
+        xta, ba = gen.rhs_from_xtrue(pa, ia, xa, seed=0)
+        axa = torch.from_numpy(xa).cuda()
+        bda = torch.from_numpy(ba).cuda()
+        aniso = {}
+        for prec in ("fp32", "fp64"):
+            F = Factorization(S, prec)
+            r = time_path(F, axa, bda, args.steps, args.warmup, args.tol)
+            xs = r.pop("x").cpu().numpy()
+            aniso[prec] = {"factor_ms": r["factor_s"] * 1e3, "refine_ms": r["refine_s"] * 1e3,
+                           "time_to_solve_ms": (r["factor_s"] + r["refine_s"]) * 1e3, "ir_iters": r["iters"],
+                           "eta": r["eta"], "converged": bool(r["eta"] <= args.tol),
+                           "rel_err_xtrue": float(np.abs(xs - xta).max() / np.abs(xta).max()),
+                           "fp64_fallbacks": F.info().get("fp64_fallbacks", 0)}
+            log(f"anisotropic {prec}: factor {r['factor_s']*1e3:.2f} ms refine {r['refine_s']*1e3:.2f} ms "
+                f"(it={r['iters']}, eta={r['eta']:.2e})")
+            F.close()
+            del F
+            torch.cuda.empty_cache()
+        aniso["tts_ratio_fp32ir_over_fp64"] = aniso["fp32"]["time_to_solve_ms"] / aniso["fp64"]["time_to_solve_ms"]
     from oracle import oracle as O
     eta32_cpu = O.backward_error(p, i, x, res["fp32"]["x"], b)
     eta64_cpu = O.backward_error(p, i, x, res["fp64"]["x"], b)
@@ -313,7 +374,10 @@ def main():
                    "tts_ratio_over_fp64": (res[prec]["factor_s"] + res[prec]["refine_s"]) / tts64,
                    "gemm": "split bf16 (hi*hi+hi*lo+lo*hi)" if prec == "bf16x3" else "single bf16 product"}
             for prec in ("bf16x3", "bf16") if prec in res},
+        "anisotropic_variant": aniso,
         "kernel_classes_fp32": {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in prof.items()},
+        "kernel_report": {"fp32": r32["report"], "fp64": r64["report"]},
+        "levels": level_report(S),
         "roofline": roof,
         "cpu_baseline": cpu,
         "analyze_s": t_an,

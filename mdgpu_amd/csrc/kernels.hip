@@ -874,8 +874,9 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
     constexpr int BT = 128, KT = 32, LDK = KT + 8;                     // rows 16-byte aligned
     constexpr int KS = KT / 2;                                        // staged k per thread
     constexpr int NP = NSPLIT == 1 ? 1 : 2;                           // hi (and lo) planes
-    __shared__ __align__(16) __bf16 As[NP][BT * LDK];
-    __shared__ __align__(16) __bf16 Bs[NP][BT * LDK];
+    // two LDS buffers: the next k-tile is stored while this one is multiplied
+    __shared__ __align__(16) __bf16 As[2][NP][BT * LDK];
+    __shared__ __align__(16) __bf16 Bs[2][NP][BT * LDK];
     int lo = 0, hi = njobs - 1;
     const int bid = xcd_tile(blockIdx.x, gridDim.x);
     while (lo < hi) {
@@ -908,8 +909,9 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
     const float* __restrict__ Ap = A + ia;
     const float* __restrict__ Bp = B + jb;
     float ra[KS], rb[KS];
-    // loads at clamped in-range addresses, all in flight, then zeroed
-    // outside the job (no per-element branch and wait)
+    // loads at clamped in-range addresses; entries outside the job are
+    // zeroed when stored to LDS, so no wait sits between a fetch and the
+    // multiplications that follow it
     const float* __restrict__ Apc = a_ok ? Ap : A;
     const float* __restrict__ Bpc = b_ok ? Bp : B;
     auto fetch = [&](int k0) {
@@ -919,13 +921,29 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
             ra[s] = Apc[(int64_t)k * g.lda];
             rb[s] = Bpc[(int64_t)k * g.ldb];
         }
+    };
+    auto store = [&](int buf, int k0) {
 #pragma unroll
-        for (int s = 0; s < KS; ++s) asm volatile("" : "+v"(ra[s]), "+v"(rb[s]));
+        for (int q = 0; q < KS / 8; ++q) {
+            bf16x8 ah, bh, al, bl;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const bool kin = k0 + kg + s < g.K;
-            ra[s] = (a_ok && kin) ? ra[s] : 0.0f;
-            rb[s] = (b_ok && kin) ? rb[s] : 0.0f;
+            for (int e = 0; e < 8; ++e) {
+                const bool kin = k0 + kg + 8 * q + e < g.K;
+                const float av = (a_ok && kin) ? ra[8 * q + e] : 0.0f;
+                const float bv = (b_ok && kin) ? rb[8 * q + e] : 0.0f;
+                ah[e] = (__bf16)av;
+                bh[e] = (__bf16)bv;
+                if constexpr (NSPLIT == 3) {
+                    al[e] = (__bf16)(av - (float)ah[e]);
+                    bl[e] = (__bf16)(bv - (float)bh[e]);
+                }
+            }
+            *reinterpret_cast<bf16x8*>(&As[buf][0][r_st * LDK + kg + 8 * q]) = ah;
+            *reinterpret_cast<bf16x8*>(&Bs[buf][0][r_st * LDK + kg + 8 * q]) = bh;
+            if constexpr (NSPLIT == 3) {
+                *reinterpret_cast<bf16x8*>(&As[buf][NP - 1][r_st * LDK + kg + 8 * q]) = al;
+                *reinterpret_cast<bf16x8*>(&Bs[buf][NP - 1][r_st * LDK + kg + 8 * q]) = bl;
+            }
         }
     };
     f32x4 acc[4][4];
@@ -934,28 +952,16 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     fetch(0);
-    for (int k0 = 0; k0 < g.K; k0 += KT) {
-#pragma unroll
-        for (int q = 0; q < KS / 8; ++q) {
-            bf16x8 ah, bh, al, bl;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                ah[e] = (__bf16)ra[8 * q + e];
-                bh[e] = (__bf16)rb[8 * q + e];
-                if constexpr (NSPLIT == 3) {
-                    al[e] = (__bf16)(ra[8 * q + e] - (float)ah[e]);
-                    bl[e] = (__bf16)(rb[8 * q + e] - (float)bh[e]);
-                }
-            }
-            *reinterpret_cast<bf16x8*>(&As[0][r_st * LDK + kg + 8 * q]) = ah;
-            *reinterpret_cast<bf16x8*>(&Bs[0][r_st * LDK + kg + 8 * q]) = bh;
-            if constexpr (NSPLIT == 3) {
-                *reinterpret_cast<bf16x8*>(&As[NP - 1][r_st * LDK + kg + 8 * q]) = al;
-                *reinterpret_cast<bf16x8*>(&Bs[NP - 1][r_st * LDK + kg + 8 * q]) = bl;
-            }
+    store(0, 0);
+    if (KT < g.K) fetch(KT);
+    __syncthreads();
+    for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+        // tile k0 + KT (in registers) to the other buffer, whose last
+        // readers passed the barrier that ended the previous iteration
+        if (k0 + KT < g.K) {
+            store(cur ^ 1, k0 + KT);
+            if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
         }
-        __syncthreads();
-        if (k0 + KT < g.K) fetch(k0 + KT);
 #pragma unroll
         for (int kq = 0; kq < KT; kq += 32) {
             bf16x8 av[NP][4], bv[NP][4];
@@ -963,8 +969,8 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    av[p][r] = *reinterpret_cast<const bf16x8*>(&As[p][(wr + r * 16 + li) * LDK + kq + 8 * lg]);
-                    bv[p][r] = *reinterpret_cast<const bf16x8*>(&Bs[p][(wc + r * 16 + li) * LDK + kq + 8 * lg]);
+                    av[p][r] = *reinterpret_cast<const bf16x8*>(&As[cur][p][(wr + r * 16 + li) * LDK + kq + 8 * lg]);
+                    bv[p][r] = *reinterpret_cast<const bf16x8*>(&Bs[cur][p][(wc + r * 16 + li) * LDK + kq + 8 * lg]);
                 }
 #pragma unroll
             for (int r = 0; r < 4; ++r)

@@ -44,5 +44,6 @@ def cases():
         ("c1_100x100_rn1000", lambda: gen.laplace2d(100), "rn", 1000),
         ("grid3d_16_rn4000", lambda: gen.laplace3d_7pt(16), "rn", 4000),
         ("st27_12_rn100000", lambda: gen.stencil27(12, seed=3), "rn", 100000),
+        ("st27_16_aniso_rn1000", lambda: gen.stencil27(16, seed=4, anisotropic=True), "rn", 1000),
         ("bbt_600_rn1000", lambda: gen.bbt_plus_ni(600, seed=5), "rn", 1000),
     ]

@@ -187,6 +187,30 @@ __global__ __launch_bounds__(kThreads) void panel_kernel(const int* __restrict__
     for (int t = threadIdx.x; t < tot; t += kThreads) g[t] = P[t];
 }
 
+// MFMA operands of the 16x16x4 products (v_mfma_{f32,f64}_16x16x4):
+// mma(a, b, c) = c + A B with A 16x4 supplied by lane (li, lg) = (lane & 15,
+// lane >> 4) as A(li, lg) and B 4x16 supplied as B(lg, li).  The
+// contraction index of one instruction is free to relabel as long as both
+// operands agree.
+template <typename T> struct MfmaOp;
+// Accumulator register v of lane (16*lg + li) holds D(row(lg, v), li): the
+// 32-bit 16x16 forms put rows 4*lg..4*lg+3 in one lane, the f64 form
+// interleaves them (row lg + 4*v) -- both measured on gfx950.
+template <> struct MfmaOp<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lg, int v) { return 4 * lg + v; }
+};
+template <> struct MfmaOp<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    __device__ static acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static int row(int lg, int v) { return lg + 4 * v; }
+};
+
 // ----------------------------------------------- dense: blocked panel path
 // Wavefront-level dense kernels on a 64x64 diagonal block held in registers:
 // lane i keeps row i (a[0..63]); column entries of other lanes are read with
@@ -224,50 +248,36 @@ __device__ inline void stage_rows(const T (&a)[64], T* Ls, T* rd, int lane) {
     __syncthreads();
 }
 
-// Columns [J0, J0+16) of the row solve x L^T = r held in registers
-// (x[0..63], one matrix row per lane), right-looking: x_j /= L(j,j), then
-// x_k -= x_j L(k,j) for k > j, with Lc(k, j) = Lc[64 j + k] read from LDS at
-// uniform addresses.
-template <int J0, typename T>
-__device__ __forceinline__ void trsm_row16(T (&x)[64], const T* Lc, const T* rd) {
-#pragma unroll
-    for (int j = J0; j < J0 + 16; ++j) {
-        // column j below the diagonal: all reads issued before one wait
-        // (the empty asm), not one wait per read
-        T lc[64];
-        const T rj = rd[j];
-#pragma unroll
-        for (int k = j + 1; k < 64; ++k) lc[k] = Lc[64 * j + k];
-#pragma unroll
-        for (int k = j + 1; k < 64; ++k) asm volatile("" : "+v"(lc[k]));
-        x[j] *= rj;
-#pragma unroll
-        for (int k = j + 1; k < 64; ++k) x[k] -= x[j] * lc[k];
-    }
-}
 
 // Cholesky of the 64x64 block held column-major in LDS (As[r + 65 c]), in
 // four 16-column panels: each panel factored by wavefront 0 with its rows in
 // registers (pivot and column entries broadcast by v_readlane), the trailing
 // part of the block updated by all four wavefronts from LDS.  bad = first
 // non-positive pivot (64 if none).  Called by all threads of the workgroup.
-// Trailing update As(i, j) -= sum_t As(i, c0+t) As(j, c0+t) over the lower
-// triangle i >= j of rows/cols [c0 + 16, 64), W = 48 - c0.  The triangle is
-// folded onto a (W + 1) x (W / 2) rectangle so no work item is idle: (r, c)
-// with r > c is (r - 1, c), otherwise (W - 1 - r, W - 1 - c).
-template <typename U, int W>
-__device__ inline void chol64_trailing(U* As, int c0, int tid) {
-    constexpr int H = W / 2;
-    for (int idx = tid; idx < (W + 1) * H; idx += kThreads) {
-        const int r = idx / H, c = idx - r * H;
-        const bool low = r > c;
-        const int i = c0 + 16 + (low ? r - 1 : W - 1 - r);
-        const int j = c0 + 16 + (low ? c : W - 1 - c);
-        U acc = U(0);
+// Trailing update of the 64x64 Cholesky after the panel of columns
+// [c0, c0 + 16): As(i, j) -= sum_t As(i, c0 + t) As(j, c0 + t) on the 16x16
+// tiles (I, J), I >= J, of rows/cols [c0 + 16, 64), one tile (four 16x16x4
+// MFMA products) per wavefront in turn.  The upper halves of the diagonal
+// tiles are updated too; they are never read.
+template <typename U>
+__device__ inline void chol64_trailing(U* As, int c0, int wave, int lane) {
+    typedef typename MfmaOp<U>::acc_t acc_t;
+    const int nt = (48 - c0) / 16, li = lane & 15, lg = lane >> 4;
+    int t = 0;
+    for (int I = 0; I < nt; ++I)
+        for (int J = 0; J <= I; ++J, ++t) {
+            if ((t & 3) != wave) continue;
+            const int i0 = c0 + 16 + 16 * I, j0 = c0 + 16 + 16 * J;
+            acc_t acc = {U(0), U(0), U(0), U(0)};
 #pragma unroll
-        for (int t = 0; t < 16; ++t) acc += As[i + 65 * (c0 + t)] * As[j + 65 * (c0 + t)];
-        As[i + 65 * j] -= acc;
-    }
+            for (int s = 0; s < 4; ++s) {
+                const int col = c0 + 4 * s + lg;
+                // D(p, q) = sum_k As(i0 + p, col_k) As(j0 + q, col_k)
+                acc = MfmaOp<U>::mma(As[(i0 + li) + 65 * col], As[(j0 + li) + 65 * col], acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) As[(i0 + MfmaOp<U>::row(lg, v)) + 65 * (j0 + li)] -= acc[v];
+        }
 }
 
 template <typename U>
@@ -296,9 +306,67 @@ __device__ void chol64_panels(U* As, int tid, int& bad) {
             }
         }
         __syncthreads();
-        if (p == 0) chol64_trailing<U, 48>(As, c0, tid);
-        else if (p == 1) chol64_trailing<U, 32>(As, c0, tid);
-        else if (p == 2) chol64_trailing<U, 16>(As, c0, tid);
+        if (p < 3) {
+            chol64_trailing<U>(As, c0, wave, lane);
+            __syncthreads();
+        }
+    }
+}
+
+// Z = inv(L) of the 64x64 lower-triangular factor in As (As[r + 65 c],
+// upper part ignored), rd[i] = 1 / L(i, i), into Zs (same layout, zero
+// above the diagonal of the diagonal blocks) by 16x16 blocks: wavefront w
+// inverts diagonal block w (lane j < 16: column j by forward substitution),
+// then, one block diagonal after the other, Z_IJ = -Z_II sum_{K=J}^{I-1}
+// L_IK Z_KJ on MFMA, one block per wavefront, the sum staged in Ts (16 x 17
+// per wavefront).  Called by all threads; ends with a barrier.
+template <typename T>
+__device__ void inv64_blocked(const T* As, const T* rd, T* Zs, T* Ts, int wave, int lane) {
+    typedef typename MfmaOp<T>::acc_t acc_t;
+    const int li = lane & 15, lg = lane >> 4;
+    if (lane < 16) {
+        const int b0 = 16 * wave, j = lane;
+        T z[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            T s = i == j ? T(1) : T(0);
+#pragma unroll
+            for (int k = 0; k < i; ++k) s -= As[(b0 + i) + 65 * (b0 + k)] * z[k];
+            z[i] = s * rd[b0 + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Zs[(b0 + i) + 65 * (b0 + j)] = i >= j ? z[i] : T(0);
+    }
+    __syncthreads();
+    T* __restrict__ tw = Ts + wave * (16 * 17);
+    for (int d = 1; d < 4; ++d) {
+        const int I = d + wave, J = wave;
+        const bool act = I < 4;
+        if (act) {
+            acc_t acc = {T(0), T(0), T(0), T(0)};
+            for (int K = J; K < I; ++K) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int kk = 16 * K + 4 * s + lg;
+                    // D(p, q) = sum_k L(16 I + p, kk) Z(kk, 16 J + q)
+                    acc = MfmaOp<T>::mma(As[(16 * I + li) + 65 * kk], Zs[kk + 65 * (16 * J + li)], acc);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) tw[MfmaOp<T>::row(lg, v) + 17 * li] = acc[v];
+        }
+        __syncthreads();
+        if (act) {
+            acc_t acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int k = 4 * s + lg;
+                // D(p, q) = sum_k Z(16 I + p, 16 I + k) T(k, q)
+                acc = MfmaOp<T>::mma(Zs[(16 * I + li) + 65 * (16 * I + k)], tw[k + 17 * li], acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Zs[(16 * I + MfmaOp<T>::row(lg, v)) + 65 * (16 * J + li)] = -acc[v];
+        }
         __syncthreads();
     }
 }
@@ -309,26 +377,29 @@ __device__ void chol64_panels(U* As, int tid, int& bad) {
 // again in FP64 from its assembled values.
 constexpr double kPivotGrowthMin = 1.0 / 4096.0;
 
-// Block column of the blocked path: Cholesky of the b x b diagonal block
-// (the potrf of FactorAux_GPU_2, mdgpu.cu:1010) and the triangular solve of
-// the rows below it (the trsm of :1047), one launch for all fronts of a
-// level.  Each workgroup owns kBlkRows rows below the block, one row per
-// lane, kept in registers; it factors the diagonal block itself (four
-// 16-column panels: each panel by wavefront 0 with its rows in registers
-// and column entries broadcast by v_readlane, the trailing part of the block
-// by all wavefronts from LDS), then solves x L_D^T = r for its rows with
-// L_D in LDS (uniform broadcast reads).  The workgroups of a job count their
-// arrival after reading the block; the last one writes the factored block
-// back, so no workgroup can read a half-written block.  Rows and columns
-// beyond b are identity padding.
+// Fused diagonal-block Cholesky + TRSM of one 64-column block step (the
+// potrf + trsm pair of FactorAux_GPU_2, mdgpu.cu:903-1216).  Every
+// workgroup of a job factors the 64x64 diagonal block in LDS (identity
+// beyond the block width b), forms Z = inv(L_D) and multiplies its
+// kBlkRows rows below the block by Z^T on MFMA (X = A Z^T: Z is lower
+// triangular, so block column N of X needs the first N + 1 column blocks
+// of A only).  The rows are loaded at the start, in the operand layout of
+// the products, so their latency hides behind the factorization.  The
+// last workgroup of the job to arrive writes L_D back (counters[jb.cnt]
+// returns to zero).  float: a pivot-growth check (L(j,j)^2 against the
+// assembled diagonal) refactors the block in FP64 when it fails.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
                                                        T* __restrict__ L, int* __restrict__ info,
                                                        int* __restrict__ counters, int* __restrict__ fallbacks) {
-    __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]; later Lc
+    typedef typename MfmaOp<T>::acc_t acc_t;
+    constexpr int kZ = 64 * 65 + 4 * 16 * 17;                           // Zs + Ts
+    constexpr int kFb = std::is_same<T, float>::value ? 64 * 65 * 8 : 0;  // FP64 fallback block
+    constexpr int kScr = kZ * (int)sizeof(T) > kFb ? kZ * (int)sizeof(T) : kFb;
+    __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]
+    __shared__ __align__(16) unsigned char scr[kScr];   // Zs, Ts; or the FP64 fallback block
     __shared__ T rd[64];
     __shared__ T a0[64];
-    __shared__ double Adbl[std::is_same<T, float>::value ? 64 * 65 : 1];   // FP64 fallback
     __shared__ int last;
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
@@ -337,18 +408,28 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
     }
     const BlkJob jb = jobs[lo];
     const int b = jb.b, ld = jb.ld;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
     T* __restrict__ g = L + jb.p_off;
-    // 16 elements per thread; loads issued together (the empty asm keeps the
-    // compiler from sinking each load into its own branch and wait)
+    // diagonal block: 16 elements per thread
     T vin[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
         vin[q] = g[min(r, b - 1) + (int64_t)min(c, b - 1) * ld];
     }
+    // this wavefront's 64 rows below the block, xa[M][ks] = A(16 M + li,
+    // 4 ks + lg); rows past the job and columns past b read in-range
+    // duplicates (Z is the identity beyond b, so they never reach a column
+    // < b, and they are never stored)
+    const int rbase = ((int)blockIdx.x - jb.tile_base) * kBlkRows + 64 * wave;
+    T xa[4][16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) asm volatile("" : "+v"(vin[q]));
+    for (int M = 0; M < 4; ++M) {
+        const T* __restrict__ src = g + b + min(rbase + 16 * M + li, jb.rows - 1);
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) xa[M][ks] = src[(int64_t)min(4 * ks + lg, b - 1) * ld];
+    }
+    keep_in_flight(vin);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
@@ -366,7 +447,7 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
         const bool weak = tid < 64 && !(ljj * ljj >= kPivotGrowthMin * (double)a0[tid]);
         if (__syncthreads_or(weak)) {
             fell_back = true;
-            double* Ad = Adbl;
+            double* Ad = reinterpret_cast<double*>(scr);
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
@@ -382,28 +463,11 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
             __syncthreads();
         }
     }
-    // repack L_D column-major with stride 64 (Lc(k, j) = Lc[64 j + k]) for
-    // aligned broadcast reads; zero above the diagonal
-    T v[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-        v[q] = r >= c ? As[r + 65 * c] : T(0);
-    }
-    __syncthreads();
-    T* Lc = As;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-        Lc[64 * c + r] = v[q];
-        if (r == c) rd[r] = T(1) / v[q];
-    }
-    __syncthreads();
+    if (tid < 64) rd[tid] = T(1) / As[tid * 66];
     if (last) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-            if (r < b && c < b && r >= c) g[r + (int64_t)c * ld] = v[q];
+        for (int t = tid; t < 64 * 64; t += kThreads) {
+            const int r = t & 63, c = t >> 6;
+            if (r < b && c < b && r >= c) g[r + (int64_t)c * ld] = As[r + 65 * c];
         }
         if (tid == 0) {
             if (bad < b) atomicMin(info, jb.col0 + bad);
@@ -411,33 +475,36 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
             atomicExch(&counters[jb.cnt], 0);
         }
     }
-    const int row = ((int)blockIdx.x - jb.tile_base) * kBlkRows + tid;
-    const bool has = row < jb.rows;
-    T* __restrict__ gr = g + b + row;          // element (b + row, 0) of the block column
-    // the 64 row loads are issued together: the empty asm keeps the compiler
-    // from sinking each load into its own branch and wait.  Lanes without a
-    // row, and columns past b, load in-range duplicates instead of zeros:
-    // L_D is the identity beyond b (so those values never reach a column
-    // < b) and they are never stored.
-    T x[64];
-    const T* __restrict__ src = has ? gr : g;    // in-range address for every lane
+    __syncthreads();
+    T* Zs = reinterpret_cast<T*>(scr);
+    inv64_blocked<T>(As, rd, Zs, Zs + 64 * 65, wave, lane);
+    // X = A Z^T: D(p, q) = sum_k Z(16 N + p, k) A(16 M + q, k) = X(16 M + q, 16 N + p)
+    acc_t acc[4][4];
 #pragma unroll
-    for (int c = 0; c < 64; ++c) x[c] = src[(int64_t)min(c, b - 1) * ld];
+    for (int M = 0; M < 4; ++M)
 #pragma unroll
-    for (int c = 0; c < 64; ++c) asm volatile("" : "+v"(x[c]));
-    // x <- x L_D^{-T}: right-looking over the 64 columns, in registers
-    trsm_row16<0>(x, Lc, rd);
-    trsm_row16<16>(x, Lc, rd);
-    trsm_row16<32>(x, Lc, rd);
-    trsm_row16<48>(x, Lc, rd);
-    // keep the solve out of the conditional stores below (sinking it there
-    // splits it into 64 basic blocks and spills)
+        for (int N = 0; N < 4; ++N) acc[M][N] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int c = 0; c < 64; ++c) asm volatile("" : "+v"(x[c]));
-    if (has) {
+    for (int N = 0; N < 4; ++N)
 #pragma unroll
-        for (int c = 0; c < 64; ++c)
-            if (c < b) gr[(int64_t)c * ld] = x[c];
+        for (int ks = 0; ks < 4 * N + 4; ++ks) {
+            const T zb = Zs[(16 * N + li) + 65 * (4 * ks + lg)];
+#pragma unroll
+            for (int M = 0; M < 4; ++M) acc[M][N] = MfmaOp<T>::mma(zb, xa[M][ks], acc[M][N]);
+        }
+#pragma unroll
+    for (int M = 0; M < 4; ++M) {
+        const int row = rbase + 16 * M + li;
+        if (row < jb.rows) {
+            T* __restrict__ dst = g + b + row;
+#pragma unroll
+            for (int N = 0; N < 4; ++N)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int col = 16 * N + MfmaOp<T>::row(lg, v);
+                    if (col < b) dst[(int64_t)col * ld] = acc[M][N][v];
+                }
+        }
     }
 }
 
@@ -658,25 +725,6 @@ __global__ __launch_bounds__(kThreads) void gemm_nt_kernel(const GemmJob* __rest
 // reads a contiguous 4-vector of k from LDS.  The product is formed as
 // (B A^T) so that each lane's four accumulator rows map to four C columns
 // and a 16-lane group stores 16 consecutive C rows.
-template <typename T> struct MfmaOp;
-// Accumulator register v of lane (16*lg + li) holds D(row(lg, v), li): the
-// 32-bit 16x16 forms put rows 4*lg..4*lg+3 in one lane, the f64 form
-// interleaves them (row lg + 4*v) -- both measured on gfx950.
-template <> struct MfmaOp<float> {
-    typedef float acc_t __attribute__((ext_vector_type(4)));
-    __device__ static acc_t mma(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    __device__ static int row(int lg, int v) { return 4 * lg + v; }
-};
-template <> struct MfmaOp<double> {
-    typedef double acc_t __attribute__((ext_vector_type(4)));
-    __device__ static acc_t mma(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    __device__ static int row(int lg, int v) { return lg + 4 * v; }
-};
-
 // Store of a wavefront's 64x64 block of C (accumulators acc[rb][cb][v] =
 // C(i0 + 16 rb, j0 + 16 cb + row(lg, v))): C -= acc, or C = -acc (beta0),
 // lower triangle only if g.lower.  For C -= acc all old values of a column

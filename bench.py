@@ -181,6 +181,7 @@ def bench_batch(args):
     bs = [gen.rhs_from_xtrue(p, i, x, seed=k)[1] for k, (p, i, x) in enumerate(mats)]
     bd = torch.from_numpy(np.concatenate(bs)).cuda()
     F = Factorization(S, "fp32")
+    tuned = F.autotune(ax) if not args.no_autotune else None
     r = time_path(F, ax, bd, args.steps, args.warmup, args.tol)
     etas = F.segment_eta()
     tts = r["factor_s"] + r["refine_s"]
@@ -197,7 +198,7 @@ def bench_batch(args):
         "factor_ms": r["factor_s"] * 1e3, "refine_ms": r["refine_s"] * 1e3, "ir_iters": r["iters"],
         "eta_max": float(etas.max()), "eta_p50": float(np.median(etas)),
         "latency_ms_p50": tts * 1e3, "latency_ms_p99": tts * 1e3,
-        "factor_gflops": S.flops / r["factor_s"] / 1e9, "analyze_s": t_an,
+        "factor_gflops": S.flops / r["factor_s"] / 1e9, "analyze_s": t_an, "autotune_64_tile_launches": tuned,
     }
     print(json.dumps(out), flush=True)
 
@@ -214,6 +215,7 @@ def main():
     ap.add_argument("--cpu-budget-gflop", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aniso", action="store_true", help="skip the z-anisotropic C3 variant")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile sizes")
     ap.add_argument("--only", default="fp32,fp64,bf16x3,bf16",
                     help="comma-separated precisions to time (fp32 and fp64 are needed for the JSON line)")
     ap.add_argument("--steps-json", default="", help="write per-launch FP32 timings (class, ms, wgs, flops, bytes, level)")
@@ -246,7 +248,11 @@ def main():
     precs = [q for q in ("fp32", "fp64", "bf16x3", "bf16") if q in args.only.split(",")]
     for prec in precs:
         F = Factorization(S, prec)
+        # per-launch GEMM tile sizes from timed factorizations (untimed setup,
+        # like the analysis: it depends on the pattern only)
+        tuned = F.autotune(ax) if not args.no_autotune else None
         r = time_path(F, ax, bd, args.steps, args.warmup, args.tol, maxit=50 if prec != "bf16" else 30)
+        r["autotune_64_tile_launches"] = tuned
         xs = r.pop("x").cpu().numpy()
         r["rel_err_xtrue"] = float(np.abs(xs - xt).max() / np.abs(xt).max())
         r["x"] = xs
@@ -281,6 +287,8 @@ def main():
         aniso = {}
         for prec in ("fp32", "fp64"):
             F = Factorization(S, prec)
+            if not args.no_autotune:
+                F.autotune(axa)
             r = time_path(F, axa, bda, args.steps, args.warmup, args.tol)
             xs = r.pop("x").cpu().numpy()
             aniso[prec] = {"factor_ms": r["factor_s"] * 1e3, "refine_ms": r["refine_s"] * 1e3,
@@ -356,6 +364,8 @@ def main():
         "config": {"workload": desc, "n": S.n, "nnz_lower": int(len(i)), "ordering": "amd",
                    "partition": f"{args.mode} {args.max_zeros}", "fronts": S.nnodes, "lnz": S.lnz,
                    "max_front": S.max_front, "flops_counts": S.flops, "flops_padded": S.flops_padded},
+        "autotune_64_tile_launches": {"fp32": r32["autotune_64_tile_launches"],
+                                      "fp64": r64["autotune_64_tile_launches"]},
         "fp32_ir": {"factor_ms": r32["factor_s"] * 1e3, "refine_ms": r32["refine_s"] * 1e3,
                     "time_to_solve_ms": tts32 * 1e3, "ir_iters": r32["iters"], "eta": r32["eta"],
                     "eta_cpu_check": eta32_cpu, "rel_err_xtrue": r32["rel_err_xtrue"],

@@ -1049,107 +1049,196 @@ __global__ void permute_out_kernel(const double* __restrict__ xp, const int* __r
     if (k < n) x[perm[k]] = xp[k];
 }
 
-// Forward substitution L y = x over one level, one workgroup per front,
-// multifrontal form of cs_lsolve (cs.cpp:1086-1101): gather the children's
-// update vectors through their relative maps, then per 64-column diagonal
-// block y = inv(L_D) x (stored inverse, a parallel 64x64 product) and update
-// the rows below.
+// 32 entries of one row of a front panel P (n rows, column-major), columns
+// col, col + 1, ... of which only the first `valid` are real: the loads of
+// the others repeat the last real column (in range) and read as zero.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ list,
-                                                       const FrontDesc* __restrict__ fronts,
-                                                       const int* __restrict__ kids, const int* __restrict__ rel,
-                                                       const int* __restrict__ q, const T* __restrict__ L,
-                                                       const T* __restrict__ inv, double* __restrict__ xp,
-                                                       double* __restrict__ Y, double* __restrict__ su,
-                                                       const int64_t* __restrict__ suofs) {
-    __shared__ double W[64 * 65];
-    __shared__ double vb[64];
+__device__ __forceinline__ void issue_row32(T (&v)[32], const T* __restrict__ P, int row, int col, int valid,
+                                            int64_t n) {
+    if (valid >= 32) {
+        issue_strided<32>(v, P + row + (int64_t)col * n, n);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) v[k] = P[row + (int64_t)(col + min(k, valid - 1)) * n];
+    }
+}
+
+// acc[k], k < 32, summed over the 64 lanes of the wavefront: afterwards
+// acc[0] of lanes l and l + 32 holds the sum of acc[l & 31] (transpose
+// reduction within each half, then the halves added).
+__device__ __forceinline__ void wave_transpose_sum32(double (&acc)[32], int lane) {
+#pragma unroll
+    for (int s = 16; s >= 1; s >>= 1) {
+        const bool upper = (lane & s) != 0;
+#pragma unroll
+        for (int k = 0; k < s; ++k) {
+            double lo = acc[k], hi = acc[k + s];
+            asm volatile("" : "+v"(lo), "+v"(hi));
+            const double send = upper ? lo : hi;
+            const double keep = upper ? hi : lo;
+            acc[k] = keep + __shfl_xor(send, s, 64);
+        }
+    }
+    acc[0] += __shfl_xor(acc[0], 32, 64);
+}
+
+// Column sums of a kSolveThreads-thread workgroup laid out as (row r =
+// tid & 255, column half h = tid >> 8), acc[k] = this thread's term of
+// column 32 h + k: reduced over the wavefront, then over the four
+// wavefronts of a half in a fixed order.  Returns the sum of column tid
+// for tid < 64 (0 elsewhere).  redc: LDS scratch of 8 * 32.
+__device__ __forceinline__ double column_sums64(double (&acc)[32], double* redc, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+    wave_transpose_sum32(acc, lane);
+    if (lane < 32) redc[wave * 32 + lane] = acc[0];
+    __syncthreads();
+    double sum = 0.0;
+    if (tid < 64) {
+        const double* rr = redc + 4 * (tid >> 5) * 32 + (tid & 31);
+        sum = (rr[0] + rr[32]) + (rr[64] + rr[96]);
+    }
+    return sum;
+}
+
+// Fronts with at most kSolveSmall pivots (one diagonal block), one
+// workgroup of kSolveThreads threads per front and one launch per level.
+// Front vectors of at most kSmallVec entries live in LDS while the
+// children's update vectors are added.
+constexpr int kSmallVec = 2048;
+
+// Forward substitution L y = x over one level, multifrontal form of
+// cs_lsolve (cs.cpp:1086-1101): the front vector (pivot entries from xp,
+// update entries zero) gathers the children's update vectors through their
+// relative maps, child by child; then y = inv(L_D) v (stored inverse, four
+// threads per row), stored in Y, and the update rows below subtract L y,
+// one row per thread with its c loads in flight together.
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void fwd_kernel(const int* __restrict__ list,
+                                                            const FrontDesc* __restrict__ fronts,
+                                                            const int* __restrict__ kids,
+                                                            const int* __restrict__ rel, const int* __restrict__ q,
+                                                            const T* __restrict__ L, const T* __restrict__ inv,
+                                                            double* __restrict__ xp, double* __restrict__ Y,
+                                                            double* __restrict__ su,
+                                                            const int64_t* __restrict__ suofs) {
+    __shared__ double fv[kSmallVec];
     __shared__ double yb[64];
+    __shared__ double red[4 * 64];
     const int f = list[blockIdx.x];
     const FrontDesc d = fronts[f];
-    const int n = d.n, c = d.c, m = d.m, k = d.k;
+    const int n = d.n, c = d.c, k = d.k, tid = threadIdx.x;
     const T* __restrict__ P = L + d.lofs;
     double* __restrict__ u = su + suofs[f];
-    for (int t = threadIdx.x; t < m; t += kThreads) u[t] = 0.0;
+    const bool inl = n <= kSmallVec;
+    for (int t = tid; t < n; t += kSolveThreads) {
+        if (inl) fv[t] = t < c ? xp[k + t] : 0.0;
+        else if (t >= c) u[t - c] = 0.0;
+    }
     __syncthreads();
     for (int t = d.kid_begin; t < d.kid_end; ++t) {
         const int ch = kids[t];
         const FrontDesc cd = fronts[ch];
         const int* __restrict__ r = rel + cd.relofs;
         const double* __restrict__ cu = su + suofs[ch];
-        for (int i = threadIdx.x; i < cd.m; i += kThreads) {
+        for (int i = tid; i < cd.m; i += kSolveThreads) {
             const int pos = r[i];
-            if (pos < c) xp[k + pos] += cu[i];
-            else u[pos - c] += cu[i];
+            const double v = cu[i];
+            if (inl) fv[pos] += v;
+            else if (pos < c) xp[k + pos] += v;
+            else u[pos - c] += v;
         }
         __syncthreads();
     }
-    for (int j0 = 0; j0 < c; j0 += 64) {
-        const int b = min(64, c - j0);
-        const T* __restrict__ w = inv + d.dinv + (int64_t)(j0 / 64) * 4096;
-        for (int t = threadIdx.x; t < 64 * b; t += kThreads) {
-            int l = t & 63, r = t >> 6;
-            if (l < b) W[l + r * 65] = to_d(w[l + 64 * r]);
+    // y = W v, W(l, r) = w[l + 64 r] = inv(L_D)(r, l): thread (r, part)
+    // sums columns [16 part, 16 part + 16) of row r
+    const T* __restrict__ w = inv + d.dinv;
+    if (tid < 4 * 64) {
+        const int r = tid >> 2, l0 = 16 * (tid & 3);
+        double s = 0.0;
+        if (r < c) {
+            T wv[16];
+#pragma unroll
+            for (int l = 0; l < 16; ++l) wv[l] = w[min(l0 + l, r) + 64 * r];
+            keep_in_flight(wv);
+#pragma unroll
+            for (int l = 0; l < 16; ++l)
+                if (l0 + l <= r) s += to_d(wv[l]) * (inl ? fv[l0 + l] : xp[k + l0 + l]);
         }
-        for (int t = threadIdx.x; t < b; t += kThreads) vb[t] = xp[k + j0 + t];
+        red[tid] = s;
+    }
+    __syncthreads();
+    if (tid < c) {
+        const double* rr = red + 4 * tid;
+        const double y = (rr[0] + rr[1]) + (rr[2] + rr[3]);
+        yb[tid] = y;
+        Y[k + tid] = y;
+    }
+    __syncthreads();
+    for (int i = c + tid; i < n; i += kSolveThreads) {
+        T lv[64];
+#pragma unroll
+        for (int j = 0; j < 64; ++j) lv[j] = P[i + (int64_t)min(j, c - 1) * n];
+        keep_in_flight(lv);
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            if (j < c) acc += to_d(lv[j]) * yb[j];
+        if (inl) fv[i] -= acc;
+        else u[i - c] -= acc;
+    }
+    if (inl) {
         __syncthreads();
-        if (threadIdx.x < b) {
-            const int r = threadIdx.x;
-            double y = 0.0;
-#pragma unroll 8
-            for (int l = 0; l <= r; ++l) y += W[l + r * 65] * vb[l];
-            yb[r] = y;
-            Y[k + j0 + r] = y;
-        }
-        __syncthreads();
-        for (int i = j0 + b + threadIdx.x; i < n; i += kThreads) {
-            double acc = 0.0;
-#pragma unroll 16
-            for (int jj = 0; jj < b; ++jj) acc += to_d(P[i + (int64_t)(j0 + jj) * n]) * yb[jj];
-            if (i < c) xp[k + i] -= acc;
-            else u[i - c] -= acc;
-        }
-        __syncthreads();
+        for (int t = c + tid; t < n; t += kSolveThreads) u[t - c] = fv[t];
     }
 }
 
-// Backward substitution L^T x = y over one level (cs_ltsolve, cs.cpp:1104-1119),
-// one workgroup per front, diagonal blocks from the last to the first: dot
-// products with the already solved rows below (read through the row list
-// q), then x = inv(L_D)^T v with the stored inverse.
+// Backward substitution L^T x = y over one level (cs_ltsolve,
+// cs.cpp:1104-1119), one workgroup per front: the dot products of the c
+// pivot columns with the solved update rows (x read through the row list
+// q), thread (row, column half) with its 32 loads in flight, summed over
+// the rows in a fixed order; then x = inv(L_D)^T v with the stored inverse,
+// one thread per pivot.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void bwd_kernel(const int* __restrict__ list,
-                                                       const FrontDesc* __restrict__ fronts,
-                                                       const int* __restrict__ qall, const T* __restrict__ L,
-                                                       const T* __restrict__ inv, double* __restrict__ xp,
-                                                       const double* __restrict__ Y) {
+__global__ __launch_bounds__(kSolveThreads) void bwd_kernel(const int* __restrict__ list,
+                                                            const FrontDesc* __restrict__ fronts,
+                                                            const int* __restrict__ qall, const T* __restrict__ L,
+                                                            const T* __restrict__ inv, double* __restrict__ xp,
+                                                            const double* __restrict__ Y) {
+    __shared__ double redc[8 * 32];
     __shared__ double vb[64];
     const FrontDesc d = fronts[list[blockIdx.x]];
-    const int n = d.n, c = d.c, k = d.k;
+    const int n = d.n, c = d.c, k = d.k, tid = threadIdx.x;
     const T* __restrict__ P = L + d.lofs;
     const int* __restrict__ q = qall + d.qofs;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nblk = (c + 63) / 64;
-    for (int blk = nblk - 1; blk >= 0; --blk) {
-        const int j0 = blk * 64, j1 = min(c, j0 + 64), b = j1 - j0;
-        for (int j = j0 + wave; j < j1; j += kThreads / 64) {
-            double s = 0.0;
-            const T* col = P + (int64_t)j * n;
-#pragma unroll 4
-            for (int i = j1 + lane; i < n; i += 64) s += to_d(col[i]) * xp[q[i]];
-            s = wave_sum(s);
-            if (lane == 0) vb[j - j0] = Y[k + j] - s;
+    const int r = tid & 255, h = tid >> 8, col = 32 * h, cvalid = min(32, c - col);
+    double acc[32];
+#pragma unroll
+    for (int kk = 0; kk < 32; ++kk) acc[kk] = 0.0;
+    if (cvalid > 0) {
+        for (int r0 = c; r0 < n; r0 += 256) {
+            T lv[32];
+            issue_row32(lv, P, min(r0 + r, n - 1), col, cvalid, n);
+            const double xv = r0 + r < n ? xp[q[r0 + r]] : 0.0;
+            keep_in_flight(lv);
+#pragma unroll
+            for (int kk = 0; kk < 32; ++kk) acc[kk] += (kk < cvalid ? to_d(lv[kk]) : 0.0) * xv;
         }
-        __syncthreads();
-        if (threadIdx.x < b) {
-            const int i = threadIdx.x;
-            const T* __restrict__ w = inv + d.dinv + (int64_t)blk * 4096;
-            double x = 0.0;
-#pragma unroll 8
-            for (int r = i; r < b; ++r) x += to_d(w[i + 64 * r]) * vb[r];
-            xp[k + j0 + i] = x;
-        }
-        __syncthreads();
+    }
+    const double sum = column_sums64(acc, redc, tid);
+    if (tid < c) vb[tid] = Y[k + tid] - sum;
+    __syncthreads();
+    if (tid < c) {
+        // x_i = sum_{r >= i} inv(L_D)(r, i) v_r, W(i, r) = w[i + 64 r]
+        const T* __restrict__ w = inv + d.dinv;
+        T wv[64];
+#pragma unroll
+        for (int rr = 0; rr < 64; ++rr) wv[rr] = w[tid + 64 * min(max(rr, tid), c - 1)];
+        keep_in_flight(wv);
+        double x = 0.0;
+#pragma unroll
+        for (int rr = 0; rr < 64; ++rr)
+            if (rr >= tid && rr < c) x += to_d(wv[rr]) * vb[rr];
+        xp[k + tid] = x;
     }
 }
 
@@ -1215,20 +1304,6 @@ __device__ __forceinline__ void packed_trmv(const T (&m)[kPkSlots], const double
     __syncthreads();
 }
 
-// 32 entries of one row of a front panel P (n rows, column-major), columns
-// col, col + 1, ... of which only the first `valid` are real: the loads of
-// the others repeat the last real column (in range) and read as zero.
-template <typename T>
-__device__ __forceinline__ void issue_row32(T (&v)[32], const T* __restrict__ P, int row, int col, int valid,
-                                            int64_t n) {
-    if (valid >= 32) {
-        issue_strided<32>(v, P + row + (int64_t)col * n, n);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 32; ++k) v[k] = P[row + (int64_t)(col + min(k, valid - 1)) * n];
-    }
-}
-
 // packed block inverse of block j0 of front d: forward form, backward form at + kPk
 __device__ inline int64_t pk_offset(const FrontDesc& d, int j0) {
     return (d.dinv256 / (kSolveBW * kSolveBW) + j0 / kSolveBW) * 2 * kPk;
@@ -1281,43 +1356,6 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem*
         if (ii < d.c) xp[d.k + ii] -= sum;
         else su[suofs[it.front] + ii - d.c] -= sum;
     }
-}
-
-// acc[k], k < 32, summed over the 64 lanes of the wavefront: afterwards
-// acc[0] of lanes l and l + 32 holds the sum of acc[l & 31] (transpose
-// reduction within each half, then the halves added).
-__device__ __forceinline__ void wave_transpose_sum32(double (&acc)[32], int lane) {
-#pragma unroll
-    for (int s = 16; s >= 1; s >>= 1) {
-        const bool upper = (lane & s) != 0;
-#pragma unroll
-        for (int k = 0; k < s; ++k) {
-            double lo = acc[k], hi = acc[k + s];
-            asm volatile("" : "+v"(lo), "+v"(hi));
-            const double send = upper ? lo : hi;
-            const double keep = upper ? hi : lo;
-            acc[k] = keep + __shfl_xor(send, s, 64);
-        }
-    }
-    acc[0] += __shfl_xor(acc[0], 32, 64);
-}
-
-// Column sums of a kSolveThreads-thread workgroup laid out as (row r =
-// tid & 255, column half h = tid >> 8), acc[k] = this thread's term of
-// column 32 h + k: reduced over the wavefront, then over the four
-// wavefronts of a half in a fixed order.  Returns the sum of column tid
-// for tid < 64 (0 elsewhere).  redc: LDS scratch of 8 * 32.
-__device__ __forceinline__ double column_sums64(double (&acc)[32], double* redc, int tid) {
-    const int lane = tid & 63, wave = tid >> 6;
-    wave_transpose_sum32(acc, lane);
-    if (lane < 32) redc[wave * 32 + lane] = acc[0];
-    __syncthreads();
-    double sum = 0.0;
-    if (tid < 64) {
-        const double* rr = redc + 4 * (tid >> 5) * 32 + (tid & 31);
-        sum = (rr[0] + rr[32]) + (rr[64] + rr[96]);
-    }
-    return sum;
 }
 
 // Backward, once per level before the blocks: workgroup (front, 64 pivot
@@ -1579,12 +1617,12 @@ template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                 const int* q, const T* L, const T* inv, double* xp, double* Y, double* su, const int64_t* suofs,
                 hipStream_t s) {
-    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, Y, su, suofs);
+    if (count > 0) fwd_kernel<T><<<count, kSolveThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, Y, su, suofs);
 }
 template <typename T>
 void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
                 double* xp, const double* Y, hipStream_t s) {
-    if (count > 0) bwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, q, L, inv, xp, Y);
+    if (count > 0) bwd_kernel<T><<<count, kSolveThreads, 0, s>>>(list, fronts, q, L, inv, xp, Y);
 }
 
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,

@@ -660,8 +660,10 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
             const int64_t cap = uses_fronts ? (int64_t)P.sv_fronts.size() : (int64_t)P.sv_items.size();
             if (st.count <= 0 || st.first < 0 || st.first + st.count > cap) return "solve step slice";
             if (uses_fronts) {
-                for (int t = st.first; t < st.first + st.count; t++)
+                for (int t = st.first; t < st.first + st.count; t++) {
                     if (P.sv_fronts[t] < 0 || P.sv_fronts[t] >= nf) return "solve front";
+                    if (st.kind != SV_FWD_GATHER && P.fronts[P.sv_fronts[t]].c > kSolveSmall) return "solve small front";
+                }
                 continue;
             }
             for (int t = st.first; t < st.first + st.count; t++) {

@@ -51,11 +51,14 @@ void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, cons
 template <typename T>
 void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
                     double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s);
-// backward: BWD_UPD subtracts the update rows' dot products from Y, BWD_BLK
-// solves a block into xp and subtracts its rows' dot products from Y
+// backward: BWD_UPD writes the update rows' dot products to partial-sum
+// slots, BWD_UPD_SUM subtracts them from Y, BWD_BLK solves a block into xp
+// and subtracts its rows' dot products from Y
 template <typename T>
 void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
-                    const double* xp, double* Y, hipStream_t s);
+                    const double* xp, double* partial, hipStream_t s);
+void launch_bwd_upd_sum(const SolveItem* items, int count, const FrontDesc* fronts, const double* partial,
+                        double* Y, hipStream_t s);
 template <typename T>
 void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
                     double* xp, double* Y, hipStream_t s);

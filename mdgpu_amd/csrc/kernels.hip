@@ -1101,7 +1101,7 @@ __device__ __forceinline__ double column_sums64(double (&acc)[32], double* redc,
 }
 
 // Fronts with at most kSolveSmall pivots (one diagonal block), one
-// workgroup of kSolveThreads threads per front and one launch per level.
+// workgroup per front and one launch per level.
 // Front vectors of at most kSmallVec entries live in LDS while the
 // children's update vectors are added.
 constexpr int kSmallVec = 2048;
@@ -1113,7 +1113,7 @@ constexpr int kSmallVec = 2048;
 // threads per row), stored in Y, and the update rows below subtract L y,
 // one row per thread with its c loads in flight together.
 template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void fwd_kernel(const int* __restrict__ list,
+__global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ list,
                                                             const FrontDesc* __restrict__ fronts,
                                                             const int* __restrict__ kids,
                                                             const int* __restrict__ rel, const int* __restrict__ q,
@@ -1130,7 +1130,7 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_kernel(const int* __restric
     const T* __restrict__ P = L + d.lofs;
     double* __restrict__ u = su + suofs[f];
     const bool inl = n <= kSmallVec;
-    for (int t = tid; t < n; t += kSolveThreads) {
+    for (int t = tid; t < n; t += kThreads) {
         if (inl) fv[t] = t < c ? xp[k + t] : 0.0;
         else if (t >= c) u[t - c] = 0.0;
     }
@@ -1140,7 +1140,7 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_kernel(const int* __restric
         const FrontDesc cd = fronts[ch];
         const int* __restrict__ r = rel + cd.relofs;
         const double* __restrict__ cu = su + suofs[ch];
-        for (int i = tid; i < cd.m; i += kSolveThreads) {
+        for (int i = tid; i < cd.m; i += kThreads) {
             const int pos = r[i];
             const double v = cu[i];
             if (inl) fv[pos] += v;
@@ -1174,7 +1174,7 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_kernel(const int* __restric
         Y[k + tid] = y;
     }
     __syncthreads();
-    for (int i = c + tid; i < n; i += kSolveThreads) {
+    for (int i = c + tid; i < n; i += kThreads) {
         T lv[64];
 #pragma unroll
         for (int j = 0; j < 64; ++j) lv[j] = P[i + (int64_t)min(j, c - 1) * n];
@@ -1188,7 +1188,7 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_kernel(const int* __restric
     }
     if (inl) {
         __syncthreads();
-        for (int t = c + tid; t < n; t += kSolveThreads) u[t - c] = fv[t];
+        for (int t = c + tid; t < n; t += kThreads) u[t - c] = fv[t];
     }
 }
 
@@ -1359,21 +1359,22 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem*
 }
 
 // Backward, once per level before the blocks: workgroup (front, 64 pivot
-// columns [a, a + 64)) subtracts from Y the dot products of its columns
-// with the front's update rows [c, n), whose solution entries (final: they
-// belong to ancestors) are read through the row list q.  Rows go 256 at a
-// time, the next chunk's loads issued before the current one is used.
+// columns [a, a + 64), update rows [j0, j0 + kSolveUpdRows)) forms the dot
+// products of its columns with those rows of the front, whose solution
+// entries (final: they belong to ancestors) are read through the row list
+// q, into partial-sum slot `slot`.  Rows go 256 at a time, the next
+// chunk's loads issued before the current one is used.
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem* __restrict__ items,
                                                                 const FrontDesc* __restrict__ fronts,
                                                                 const int* __restrict__ qall,
                                                                 const T* __restrict__ L,
                                                                 const double* __restrict__ xp,
-                                                                double* __restrict__ Y) {
+                                                                double* __restrict__ partial) {
     __shared__ double redc[8 * 32];
     const SolveItem it = items[blockIdx.x];
     const FrontDesc d = fronts[it.front];
-    const int n = d.n, c = d.c, c0 = it.a;
+    const int c = d.c, c0 = it.a, n = min(d.n, it.j0 + kSolveUpdRows);
     const T* __restrict__ P = L + d.lofs;
     const int* __restrict__ q = qall + d.qofs;
     const int tid = threadIdx.x, r = tid & 255, h = tid >> 8;
@@ -1383,15 +1384,15 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem*
     for (int k = 0; k < 32; ++k) acc[k] = 0.0;
     if (cvalid > 0 && sizeof(T) == 4) {
         T cur[32];
-        int r0 = c;
-        issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, n);
+        int r0 = it.j0;
+        issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, d.n);
         double xv = r0 + r < n ? xp[q[r0 + r]] : 0.0;
         for (; r0 < n; r0 += 256) {
             T nxt[32];
             double xn = 0.0;
             const bool more = r0 + 256 < n;
             if (more) {
-                issue_row32(nxt, P, min(r0 + 256 + r, n - 1), col, cvalid, n);
+                issue_row32(nxt, P, min(r0 + 256 + r, n - 1), col, cvalid, d.n);
                 xn = r0 + 256 + r < n ? xp[q[r0 + 256 + r]] : 0.0;
             }
 #pragma unroll
@@ -1404,9 +1405,9 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem*
         }
     } else if (cvalid > 0) {
         // FP64: one chunk in flight (two would not fit the register budget)
-        for (int r0 = c; r0 < n; r0 += 256) {
+        for (int r0 = it.j0; r0 < n; r0 += 256) {
             T cur[32];
-            issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, n);
+            issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, d.n);
             const double xv = r0 + r < n ? xp[q[r0 + r]] : 0.0;
             keep_in_flight(cur);
 #pragma unroll
@@ -1414,7 +1415,32 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem*
         }
     }
     const double sum = column_sums64(acc, redc, tid);
-    if (tid < kSolveCols && c0 + tid < c) Y[d.k + c0 + tid] -= sum;
+    if (tid < kSolveCols) partial[(int64_t)it.slot * kSolveCols + tid] = sum;
+}
+
+// Backward, after BWD_UPD: Y(c0 + i) -= sum of the chunk's partial-sum
+// slots, added in slot order (one workgroup of kSolveCols threads per
+// column chunk).
+__global__ __launch_bounds__(kSolveCols) void bwd_upd_sum_kernel(const SolveItem* __restrict__ items,
+                                                                 const FrontDesc* __restrict__ fronts,
+                                                                 const double* __restrict__ partial,
+                                                                 double* __restrict__ Y) {
+    const SolveItem it = items[blockIdx.x];
+    const FrontDesc d = fronts[it.front];
+    const int i = threadIdx.x;
+    if (it.a + i >= d.c) return;
+    const double* __restrict__ p = partial + (int64_t)it.slot * kSolveCols + i;
+    double s = 0.0;
+    for (int t0 = 0; t0 < it.nslot; t0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = p[(int64_t)min(t0 + t, it.nslot - 1) * kSolveCols];
+        keep_in_flight(v);
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if (t0 + t < it.nslot) s += v[t];
+    }
+    Y[d.k + it.a + i] -= s;
 }
 
 // Backward, one kSolveBW-column block of a large front per launch (blocks
@@ -1617,7 +1643,7 @@ template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                 const int* q, const T* L, const T* inv, double* xp, double* Y, double* su, const int64_t* suofs,
                 hipStream_t s) {
-    if (count > 0) fwd_kernel<T><<<count, kSolveThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, Y, su, suofs);
+    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, Y, su, suofs);
 }
 template <typename T>
 void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
@@ -1636,8 +1662,12 @@ void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, 
 }
 template <typename T>
 void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
-                    const double* xp, double* Y, hipStream_t s) {
-    if (count > 0) bwd_upd_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, q, L, xp, Y);
+                    const double* xp, double* partial, hipStream_t s) {
+    if (count > 0) bwd_upd_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, q, L, xp, partial);
+}
+void launch_bwd_upd_sum(const SolveItem* items, int count, const FrontDesc* fronts, const double* partial,
+                        double* Y, hipStream_t s) {
+    if (count > 0) bwd_upd_sum_kernel<<<count, kSolveCols, 0, s>>>(items, fronts, partial, Y);
 }
 template <typename T>
 void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,

@@ -52,6 +52,7 @@ struct mdgpu_numeric {
     mdgpu::InvJob* dInvJobs = nullptr;
     void* dInv256 = nullptr;              // kSolveBW-column block inverses (solves)
     void* dInvPk = nullptr;               // their packed forms (Plan::invpk)
+    double* dUpdPartial = nullptr;        // BWD_UPD partial sums
     mdgpu::InvJob* dInv256Jobs = nullptr;
     int* dPanel = nullptr;
     int* dLevelList = nullptr;
@@ -293,7 +294,10 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
                 mdgpu::launch_bwd<T>(list, st.count, N->dFronts, N->dQ, L, inv, N->dXp, N->dY, s);
                 break;
             case mdgpu::SV_BWD_UPD:
-                mdgpu::launch_bwd_upd<T>(items, st.count, N->dFronts, N->dQ, L, N->dXp, N->dY, s);
+                mdgpu::launch_bwd_upd<T>(items, st.count, N->dFronts, N->dQ, L, N->dXp, N->dUpdPartial, s);
+                break;
+            case mdgpu::SV_BWD_UPD_SUM:
+                mdgpu::launch_bwd_upd_sum(items, st.count, N->dFronts, N->dUpdPartial, N->dY, s);
                 break;
             case mdgpu::SV_BWD_BLK:
                 mdgpu::launch_bwd_blk<T>(items, st.count, N->dFronts, L, (const T*)N->dInvPk, N->dXp, N->dY, s);
@@ -404,6 +408,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.inv, (void**)&N->dInvJobs)) ||
         (st = alloc(N, P.inv256 * es, &N->dInv256)) ||
         (st = alloc(N, P.invpk * es, &N->dInvPk)) ||
+        (st = alloc(N, P.sv_upd_partial * sizeof(double), (void**)&N->dUpdPartial)) ||
         (st = upload(N, P.inv256_jobs, (void**)&N->dInv256Jobs)) ||
         (st = upload(N, P.fronts, (void**)&N->dFronts)) ||
         (st = upload(N, P.kids, (void**)&N->dKids)) ||

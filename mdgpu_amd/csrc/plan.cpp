@@ -229,14 +229,26 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             if (big.empty()) continue;
             {
                 SolveStep st{SV_BWD_UPD, (int)P.sv_items.size(), 0, 0};
+                std::vector<SolveItem> sums;
+                int slot = 0;
                 for (int f : big) {
                     const FrontDesc& d = P.fronts[f];
                     if (d.n == d.c) continue;
-                    for (int c0 = 0; c0 < d.c; c0 += kSolveCols) P.sv_items.push_back({f, c0, 0, 0});
+                    for (int c0 = 0; c0 < d.c; c0 += kSolveCols) {
+                        const int first = slot;
+                        for (int r0 = d.c; r0 < d.n; r0 += kSolveUpdRows) P.sv_items.push_back({f, c0, 0, r0, slot++, 0});
+                        sums.push_back({f, c0, 0, 0, first, slot - first});
+                    }
                     st.bytes += (double)(d.n - d.c) * d.c * o.elem_bytes + 12.0 * (d.n - d.c) + 16.0 * d.c;
                 }
                 st.count = (int)P.sv_items.size() - st.first;
-                if (st.count > 0) P.sv_bwd.push_back(st);
+                if (st.count > 0) {
+                    P.sv_bwd.push_back(st);
+                    SolveStep ss{SV_BWD_UPD_SUM, (int)P.sv_items.size(), (int)sums.size(), 8.0 * slot * kSolveCols};
+                    P.sv_items.insert(P.sv_items.end(), sums.begin(), sums.end());
+                    P.sv_bwd.push_back(ss);
+                    P.sv_upd_partial = std::max<int64_t>(P.sv_upd_partial, (int64_t)slot * kSolveCols);
+                }
             }
             int maxc = 0;
             for (int f : big) maxc = std::max(maxc, P.fronts[f].c);
@@ -671,8 +683,14 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 if (it.front < 0 || it.front >= nf) return "solve item front";
                 const FrontDesc& d = P.fronts[it.front];
                 if (d.c <= kSolveSmall || d.dinv256 < 0) return "solve item front size";
-                if (st.kind == SV_BWD_UPD) {
+                if (st.kind == SV_BWD_UPD || st.kind == SV_BWD_UPD_SUM) {
                     if (d.n <= d.c || it.a < 0 || it.a >= d.c || it.a % kSolveCols) return "solve update columns";
+                    if (st.kind == SV_BWD_UPD &&
+                        (it.j0 < d.c || it.j0 >= d.n || (int64_t)(it.slot + 1) * kSolveCols > P.sv_upd_partial))
+                        return "solve update rows";
+                    if (st.kind == SV_BWD_UPD_SUM &&
+                        (it.slot < 0 || it.nslot < 1 || (int64_t)(it.slot + it.nslot) * kSolveCols > P.sv_upd_partial))
+                        return "solve update slots";
                     continue;
                 }
                 if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW) return "solve item block";

@@ -109,10 +109,11 @@ struct Step {
 // the block's inverse in packed form (see kPk):
 //  * FWD_BLK: every workgroup solves the block (y = X v) and updates its
 //    kSolveFwdRows rows below it; the `first` one stores y in Y.
-//  * BWD_UPD (once per level, before the blocks): each workgroup owns
-//    kSolveCols pivot columns of a front and subtracts their dot products
-//    with the front's update rows (solved by the ancestors, read through
-//    the row list q) from Y.
+//  * BWD_UPD (once per level, before the blocks): each workgroup takes
+//    kSolveCols pivot columns of a front and kSolveUpdRows of its update
+//    rows (solved by the ancestors, read through the row list q) and writes
+//    the dot products to its partial-sum slot; BWD_UPD_SUM then subtracts
+//    the slots of each column chunk from Y, in slot order.
 //  * BWD_BLK (blocks from the last to the first): every workgroup solves
 //    the block (x = X^T y), the `first` one stores x; each other one owns
 //    kSolveCols pivot columns left of the block and subtracts the dot
@@ -124,6 +125,7 @@ constexpr int kSolveSmall = 64;
 constexpr int kSolveThreads = 512;
 constexpr int kSolveFwdRows = 64;   // rows per forward workgroup below a block
 constexpr int kSolveCols = 64;      // pivot columns per backward workgroup
+constexpr int kSolveUpdRows = 1024; // update rows per BWD_UPD workgroup
 // Packed lower-triangular kSolveBW x kSolveBW matrix (a block inverse X for
 // the forward solve, R X^T R with R the order reversal for the backward):
 // row pairs (p, 255 - p), p < 128, of 257 entries each, entry e being
@@ -133,14 +135,16 @@ constexpr int kSolveCols = 64;      // pivot columns per backward workgroup
 constexpr int kPkSlots = 65;
 constexpr int kPk = kPkSlots * kSolveThreads;
 inline constexpr int pk_first(int s) { return s == 0 ? 0 : 1 + 64 * s; }
-enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_UPD, SV_BWD_BLK };
+enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_UPD, SV_BWD_UPD_SUM, SV_BWD_BLK };
 
 struct SolveItem {
     int front;
     int a;        // FWD_BLK: first row (front-local) of the kSolveFwdRows-row chunk;
                   // BWD_*: first pivot column of the kSolveCols-column chunk
     int first;    // *_BLK: 1 if this item stores the block's solution
-    int j0;       // *_BLK: first pivot column of the block
+    int j0;       // *_BLK: first pivot column of the block; BWD_UPD: first update row
+    int slot;     // BWD_UPD: partial-sum slot; BWD_UPD_SUM: first slot of the chunk
+    int nslot;    // BWD_UPD_SUM: slots of the chunk
 };
 
 struct SolveStep {
@@ -164,6 +168,7 @@ struct Plan {
     std::vector<int> sv_fronts;        // front lists of SMALL / GATHER steps
     std::vector<SolveItem> sv_items;
     std::vector<SolveStep> sv_fwd, sv_bwd;
+    int64_t sv_upd_partial = 1;        // doubles of BWD_UPD partial sums (one level at a time)
     std::vector<EaItem> ea;
     std::vector<GemmJob> gemm;
     std::vector<BlkJob> blk;

@@ -1248,30 +1248,60 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_kernel(const int* __restric
 // of kSolveThreads threads: row chunks of kSolveFwdRows below the block
 // (forward), column chunks of kSolveCols left of it (backward).
 
-// Forward, step 0: clear the front's update vector and gather its
-// children's update vectors (the first phase of fwd_kernel).
-__global__ __launch_bounds__(kThreads) void fwd_gather_kernel(const int* __restrict__ list,
-                                                              const FrontDesc* __restrict__ fronts,
-                                                              const int* __restrict__ kids,
-                                                              const int* __restrict__ rel,
-                                                              double* __restrict__ xp, double* __restrict__ su,
-                                                              const int64_t* __restrict__ suofs) {
+// Forward, step 0 of a large front: its front vector (pivot entries from
+// xp, update entries zero) gathers the children's update vectors through
+// their relative maps, child by child (the first phase of fwd_kernel).
+// Fronts of at most kGatherVec rows accumulate in LDS; each thread keeps
+// four entries of a child in flight.
+constexpr int kGatherThreads = 1024, kGatherVec = 16384;
+__global__ __launch_bounds__(kGatherThreads) void fwd_gather_kernel(const int* __restrict__ list,
+                                                                    const FrontDesc* __restrict__ fronts,
+                                                                    const int* __restrict__ kids,
+                                                                    const int* __restrict__ rel,
+                                                                    double* __restrict__ xp, double* __restrict__ su,
+                                                                    const int64_t* __restrict__ suofs) {
+    __shared__ double fv[kGatherVec];
     const int f = list[blockIdx.x];
     const FrontDesc d = fronts[f];
+    const int n = d.n, c = d.c, k = d.k, tid = threadIdx.x;
     double* __restrict__ u = su + suofs[f];
-    for (int t = threadIdx.x; t < d.m; t += kThreads) u[t] = 0.0;
+    const bool inl = n <= kGatherVec;
+    for (int t = tid; t < n; t += kGatherThreads) {
+        if (inl) fv[t] = t < c ? xp[k + t] : 0.0;
+        else if (t >= c) u[t - c] = 0.0;
+    }
     __syncthreads();
     for (int t = d.kid_begin; t < d.kid_end; ++t) {
         const int ch = kids[t];
         const FrontDesc cd = fronts[ch];
         const int* __restrict__ r = rel + cd.relofs;
         const double* __restrict__ cu = su + suofs[ch];
-        for (int i = threadIdx.x; i < cd.m; i += kThreads) {
-            const int pos = r[i];
-            if (pos < d.c) xp[d.k + pos] += cu[i];
-            else u[pos - d.c] += cu[i];
+        for (int i0 = tid; i0 < cd.m; i0 += 4 * kGatherThreads) {
+            int pos[4];
+            double v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = min(i0 + e * kGatherThreads, cd.m - 1);
+                pos[e] = r[i];
+                v[e] = cu[i];
+            }
+            keep_in_flight(pos);
+            keep_in_flight(v);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (i0 + e * kGatherThreads >= cd.m) continue;
+                if (inl) fv[pos[e]] += v[e];
+                else if (pos[e] < c) xp[k + pos[e]] += v[e];
+                else u[pos[e] - c] += v[e];
+            }
         }
         __syncthreads();
+    }
+    if (inl) {
+        for (int t = tid; t < n; t += kGatherThreads) {
+            if (t < c) xp[k + t] = fv[t];
+            else u[t - c] = fv[t];
+        }
     }
 }
 
@@ -1304,10 +1334,6 @@ __device__ __forceinline__ void packed_trmv(const T (&m)[kPkSlots], const double
     __syncthreads();
 }
 
-// packed block inverse of block j0 of front d: forward form, backward form at + kPk
-__device__ inline int64_t pk_offset(const FrontDesc& d, int j0) {
-    return (d.dinv256 / (kSolveBW * kSolveBW) + j0 / kSolveBW) * 2 * kPk;
-}
 
 // Forward, one kSolveBW-column block of a large front per launch.  Every
 // workgroup solves the block's triangle, y = X v with the packed block
@@ -1319,29 +1345,26 @@ __device__ inline int64_t pk_offset(const FrontDesc& d, int j0) {
 // segments of a row are added in a fixed order.
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
-                                                                const FrontDesc* __restrict__ fronts,
                                                                 const T* __restrict__ L, const T* __restrict__ Xpk,
                                                                 double* __restrict__ xp, double* __restrict__ Y,
-                                                                double* __restrict__ su,
-                                                                const int64_t* __restrict__ suofs) {
+                                                                double* __restrict__ su) {
     __shared__ double vb[kSolveBW], yb[kSolveBW];
     __shared__ double red[2 * kSolveThreads];
     const SolveItem it = items[blockIdx.x];
-    const FrontDesc d = fronts[it.front];
-    const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
-    const T* __restrict__ P = L + d.lofs;
+    const int n = it.n, j0 = it.j0, bw = min(kSolveBW, it.c - j0);
+    const T* __restrict__ P = L + it.lofs;
     const int tid = threadIdx.x, r = tid & 63, cs = tid >> 6;
     const int i = it.a + r;
     const int cvalid = min(32, bw - 32 * cs);          // real columns of this segment
     T lv[32], m[kPkSlots];
     if (cvalid > 0) issue_row32(lv, P, min(i, n - 1), j0 + 32 * cs, cvalid, n);
-    issue_strided<kPkSlots>(m, Xpk + pk_offset(d, j0) + tid, kSolveThreads);
-    if (tid < kSolveBW) vb[tid] = tid < bw ? xp[d.k + j0 + tid] : 0.0;
+    issue_strided<kPkSlots>(m, Xpk + it.pk + tid, kSolveThreads);
+    if (tid < kSolveBW) vb[tid] = tid < bw ? xp[it.k + j0 + tid] : 0.0;
     keep_in_flight(lv);
     keep_in_flight(m);
     __syncthreads();
     packed_trmv(m, vb, yb, red, tid);
-    if (it.first && tid < bw) Y[d.k + j0 + tid] = yb[tid];
+    if (it.first && tid < bw) Y[it.k + j0 + tid] = yb[tid];
     double acc = 0.0;
     if (cvalid > 0) {
 #pragma unroll
@@ -1353,8 +1376,8 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem*
         const double* rr = red + tid;
         const double sum = ((rr[0] + rr[64]) + (rr[128] + rr[192])) + ((rr[256] + rr[320]) + (rr[384] + rr[448]));
         const int ii = it.a + tid;
-        if (ii < d.c) xp[d.k + ii] -= sum;
-        else su[suofs[it.front] + ii - d.c] -= sum;
+        if (ii < it.c) xp[it.k + ii] -= sum;
+        else su[it.su + ii - it.c] -= sum;
     }
 }
 
@@ -1366,17 +1389,15 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem*
 // chunk's loads issued before the current one is used.
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem* __restrict__ items,
-                                                                const FrontDesc* __restrict__ fronts,
                                                                 const int* __restrict__ qall,
                                                                 const T* __restrict__ L,
                                                                 const double* __restrict__ xp,
                                                                 double* __restrict__ partial) {
     __shared__ double redc[8 * 32];
     const SolveItem it = items[blockIdx.x];
-    const FrontDesc d = fronts[it.front];
-    const int c = d.c, c0 = it.a, n = min(d.n, it.j0 + kSolveUpdRows);
-    const T* __restrict__ P = L + d.lofs;
-    const int* __restrict__ q = qall + d.qofs;
+    const int c = it.c, c0 = it.a, n = min(it.n, it.j0 + kSolveUpdRows);
+    const T* __restrict__ P = L + it.lofs;
+    const int* __restrict__ q = qall + it.qofs;
     const int tid = threadIdx.x, r = tid & 255, h = tid >> 8;
     const int col = c0 + 32 * h, cvalid = min(32, c - col);
     double acc[32];
@@ -1385,14 +1406,14 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem*
     if (cvalid > 0 && sizeof(T) == 4) {
         T cur[32];
         int r0 = it.j0;
-        issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, d.n);
+        issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, it.n);
         double xv = r0 + r < n ? xp[q[r0 + r]] : 0.0;
         for (; r0 < n; r0 += 256) {
             T nxt[32];
             double xn = 0.0;
             const bool more = r0 + 256 < n;
             if (more) {
-                issue_row32(nxt, P, min(r0 + 256 + r, n - 1), col, cvalid, d.n);
+                issue_row32(nxt, P, min(r0 + 256 + r, n - 1), col, cvalid, it.n);
                 xn = r0 + 256 + r < n ? xp[q[r0 + 256 + r]] : 0.0;
             }
 #pragma unroll
@@ -1407,7 +1428,7 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem*
         // FP64: one chunk in flight (two would not fit the register budget)
         for (int r0 = it.j0; r0 < n; r0 += 256) {
             T cur[32];
-            issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, d.n);
+            issue_row32(cur, P, min(r0 + r, n - 1), col, cvalid, it.n);
             const double xv = r0 + r < n ? xp[q[r0 + r]] : 0.0;
             keep_in_flight(cur);
 #pragma unroll
@@ -1422,13 +1443,11 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_upd_kernel(const SolveItem*
 // slots, added in slot order (one workgroup of kSolveCols threads per
 // column chunk).
 __global__ __launch_bounds__(kSolveCols) void bwd_upd_sum_kernel(const SolveItem* __restrict__ items,
-                                                                 const FrontDesc* __restrict__ fronts,
                                                                  const double* __restrict__ partial,
                                                                  double* __restrict__ Y) {
     const SolveItem it = items[blockIdx.x];
-    const FrontDesc d = fronts[it.front];
     const int i = threadIdx.x;
-    if (it.a + i >= d.c) return;
+    if (it.a + i >= it.c) return;
     const double* __restrict__ p = partial + (int64_t)it.slot * kSolveCols + i;
     double s = 0.0;
     for (int t0 = 0; t0 < it.nslot; t0 += 16) {
@@ -1440,7 +1459,7 @@ __global__ __launch_bounds__(kSolveCols) void bwd_upd_sum_kernel(const SolveItem
         for (int t = 0; t < 16; ++t)
             if (t0 + t < it.nslot) s += v[t];
     }
-    Y[d.k + it.a + i] -= s;
+    Y[it.k + it.a + i] -= s;
 }
 
 // Backward, one kSolveBW-column block of a large front per launch (blocks
@@ -1453,28 +1472,26 @@ __global__ __launch_bounds__(kSolveCols) void bwd_upd_sum_kernel(const SolveItem
 // with the inverse's, before x is known.
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void bwd_blk_kernel(const SolveItem* __restrict__ items,
-                                                                const FrontDesc* __restrict__ fronts,
                                                                 const T* __restrict__ L, const T* __restrict__ Xpk,
                                                                 double* __restrict__ xp, double* __restrict__ Y) {
     __shared__ double vb[kSolveBW], zb[kSolveBW];
     __shared__ double red[2 * kSolveThreads];
     const SolveItem it = items[blockIdx.x];
-    const FrontDesc d = fronts[it.front];
-    const int n = d.n, j0 = it.j0, bw = min(kSolveBW, d.c - j0);
-    const T* __restrict__ P = L + d.lofs;
+    const int n = it.n, j0 = it.j0, bw = min(kSolveBW, it.c - j0);
+    const T* __restrict__ P = L + it.lofs;
     const int tid = threadIdx.x, r = tid & 255, h = tid >> 8;
     const bool own = !it.first;
     T lv[32], m[kPkSlots];
     if (own) issue_strided<32>(lv, P + (j0 + min(r, bw - 1)) + (int64_t)(it.a + 32 * h) * n, n);
-    issue_strided<kPkSlots>(m, Xpk + pk_offset(d, j0) + kPk + tid, kSolveThreads);
-    if (tid < kSolveBW) vb[tid] = kSolveBW - 1 - tid < bw ? Y[d.k + j0 + kSolveBW - 1 - tid] : 0.0;
+    issue_strided<kPkSlots>(m, Xpk + it.pk + kPk + tid, kSolveThreads);
+    if (tid < kSolveBW) vb[tid] = kSolveBW - 1 - tid < bw ? Y[it.k + j0 + kSolveBW - 1 - tid] : 0.0;
     if (own) keep_in_flight(lv);
     keep_in_flight(m);
     __syncthreads();
     packed_trmv(m, vb, zb, red, tid);
     // x(i) = zb[255 - i]
     if (!own) {
-        if (tid < bw) xp[d.k + j0 + tid] = zb[kSolveBW - 1 - tid];
+        if (tid < bw) xp[it.k + j0 + tid] = zb[kSolveBW - 1 - tid];
         return;
     }
     const double xv = r < bw ? zb[kSolveBW - 1 - r] : 0.0;
@@ -1482,7 +1499,7 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_blk_kernel(const SolveItem*
 #pragma unroll
     for (int k = 0; k < 32; ++k) acc[k] = to_d(lv[k]) * xv;
     const double sum = column_sums64(acc, red, tid);
-    if (tid < kSolveCols) Y[d.k + it.a + tid] -= sum;
+    if (tid < kSolveCols) Y[it.k + it.a + tid] -= sum;
 }
 
 // -------------------------------------------------------------- refinement
@@ -1653,26 +1670,26 @@ void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* 
 
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                        double* xp, double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_gather_kernel<<<count, kThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
+    if (count > 0) fwd_gather_kernel<<<count, kGatherThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
 }
 template <typename T>
 void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
                     double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, L, Xpk, xp, Y, su, suofs);
+    if (count > 0) fwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, L, Xpk, xp, Y, su);
 }
 template <typename T>
 void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
                     const double* xp, double* partial, hipStream_t s) {
-    if (count > 0) bwd_upd_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, q, L, xp, partial);
+    if (count > 0) bwd_upd_kernel<T><<<count, kSolveThreads, 0, s>>>(items, q, L, xp, partial);
 }
 void launch_bwd_upd_sum(const SolveItem* items, int count, const FrontDesc* fronts, const double* partial,
                         double* Y, hipStream_t s) {
-    if (count > 0) bwd_upd_sum_kernel<<<count, kSolveCols, 0, s>>>(items, fronts, partial, Y);
+    if (count > 0) bwd_upd_sum_kernel<<<count, kSolveCols, 0, s>>>(items, partial, Y);
 }
 template <typename T>
 void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
                     double* xp, double* Y, hipStream_t s) {
-    if (count > 0) bwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, fronts, L, Xpk, xp, Y);
+    if (count > 0) bwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, L, Xpk, xp, Y);
 }
 template <typename T>
 void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,

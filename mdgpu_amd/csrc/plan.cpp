@@ -270,6 +270,25 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
     }
 
+    // front fields copied into the solve items
+    for (const std::vector<SolveStep>* steps : {&P.sv_fwd, &P.sv_bwd})
+        for (const SolveStep& st : *steps) {
+            if (st.kind == SV_FWD_SMALL || st.kind == SV_BWD_SMALL || st.kind == SV_FWD_GATHER) continue;
+            for (int t = st.first; t < st.first + st.count; t++) {
+                SolveItem& it = P.sv_items[t];
+                const FrontDesc& d = P.fronts[it.front];
+                it.n = d.n;
+                it.c = d.c;
+                it.k = d.k;
+                it.qofs = d.qofs;
+                it.lofs = d.lofs;
+                it.su = P.solve_uofs[it.front];
+                it.pk = (st.kind == SV_FWD_BLK || st.kind == SV_BWD_BLK)
+                            ? (d.dinv256 / ((int64_t)kSolveBW * kSolveBW) + it.j0 / kSolveBW) * 2 * kPk
+                            : 0;
+            }
+        }
+
     // schedule
     // GEMM tile size of a launch: o.tile for the deep updates (trailing,
     // Schur complement); the narrow in-panel updates use 64 when the
@@ -682,6 +701,9 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 const SolveItem& it = P.sv_items[t];
                 if (it.front < 0 || it.front >= nf) return "solve item front";
                 const FrontDesc& d = P.fronts[it.front];
+                if (it.n != d.n || it.c != d.c || it.k != d.k || it.qofs != d.qofs || it.lofs != d.lofs ||
+                    it.su != P.solve_uofs[it.front])
+                    return "solve item front fields";
                 if (d.c <= kSolveSmall || d.dinv256 < 0) return "solve item front size";
                 if (st.kind == SV_BWD_UPD || st.kind == SV_BWD_UPD_SUM) {
                     if (d.n <= d.c || it.a < 0 || it.a >= d.c || it.a % kSolveCols) return "solve update columns";
@@ -695,6 +717,9 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 }
                 if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW) return "solve item block";
                 const int j1 = std::min(it.j0 + kSolveBW, d.c);
+                if (it.pk != (d.dinv256 / ((int64_t)kSolveBW * kSolveBW) + it.j0 / kSolveBW) * 2 * kPk ||
+                    it.pk < 0 || it.pk + 2 * kPk > P.invpk)
+                    return "solve item packed inverse";
                 if (st.kind == SV_FWD_BLK) {
                     if (it.first ? it.a != j1 : (it.a < j1 + kSolveFwdRows || it.a >= d.n)) return "solve item rows";
                 } else if (it.first ? it.a != it.j0 : (it.a < 0 || it.a >= it.j0 || it.a % kSolveCols)) {

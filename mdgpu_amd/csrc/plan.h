@@ -145,6 +145,13 @@ struct SolveItem {
     int j0;       // *_BLK: first pivot column of the block; BWD_UPD: first update row
     int slot;     // BWD_UPD: partial-sum slot; BWD_UPD_SUM: first slot of the chunk
     int nslot;    // BWD_UPD_SUM: slots of the chunk
+    // copies of the front's fields, so that a workgroup reads one item and
+    // no front descriptor before its data loads
+    int n, c, k;
+    int64_t qofs;  // offset of the front's row list
+    int64_t lofs;  // panel offset in the L store
+    int64_t pk;    // *_BLK: packed inverse of the block (forward form; backward at + kPk)
+    int64_t su;    // offset of the front's update vector (solve_uofs)
 };
 
 struct SolveStep {

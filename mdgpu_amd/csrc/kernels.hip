@@ -1049,18 +1049,23 @@ __global__ void permute_out_kernel(const double* __restrict__ xp, const int* __r
     if (k < n) x[perm[k]] = xp[k];
 }
 
-// 32 entries of one row of a front panel P (n rows, column-major), columns
+// N entries of one row of a front panel P (n rows, column-major), columns
 // col, col + 1, ... of which only the first `valid` are real: the loads of
 // the others repeat the last real column (in range) and read as zero.
+template <int N, typename T>
+__device__ __forceinline__ void issue_row(T (&v)[N], const T* __restrict__ P, int row, int col, int valid,
+                                          int64_t n) {
+    if (valid >= N) {
+        issue_strided<N>(v, P + row + (int64_t)col * n, n);
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = P[row + (int64_t)(col + min(k, valid - 1)) * n];
+    }
+}
 template <typename T>
 __device__ __forceinline__ void issue_row32(T (&v)[32], const T* __restrict__ P, int row, int col, int valid,
                                             int64_t n) {
-    if (valid >= 32) {
-        issue_strided<32>(v, P + row + (int64_t)col * n, n);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 32; ++k) v[k] = P[row + (int64_t)(col + min(k, valid - 1)) * n];
-    }
+    issue_row<32>(v, P, row, col, valid, n);
 }
 
 // acc[k], k < 32, summed over the 64 lanes of the wavefront: afterwards
@@ -1245,7 +1250,7 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_kernel(const int* __restric
 // ------------------------------------------------- solves of large fronts
 // Fronts with more than kSolveSmall pivots: forward / backward substitution
 // kSolveBW pivot columns (a block) per launch, spread over many workgroups
-// of kSolveThreads threads: row chunks of kSolveFwdRows below the block
+// of kSolveThreads threads: row chunks of solve_fwd_rows() below the block
 // (forward), column chunks of kSolveCols left of it (backward).
 
 // Forward, step 0 of a large front: its front vector (pivot entries from
@@ -1339,27 +1344,34 @@ __device__ __forceinline__ void packed_trmv(const T (&m)[kPkSlots], const double
 // workgroup solves the block's triangle, y = X v with the packed block
 // inverse X (v is final on entry: earlier blocks were applied by earlier
 // launches), the `first` one stores y in Y, and each subtracts the block's
-// contribution from its kSolveFwdRows rows [a, a + kSolveFwdRows) below the
-// block: thread (r, cs) takes row a + r and 32 columns, all of its loads
-// (those 32 and the kPkSlots of X) in flight together; the eight column
-// segments of a row are added in a fixed order.
+// contribution from its solve_fwd_rows(sizeof(T)) rows [a, a + FR) below
+// the block: thread (r, cs) takes row a + r and kSolveBW / NS columns.  All
+// of a thread's loads (its row entries and the kPkSlots of X) are in flight
+// together; the NS column segments of a row are added in a fixed order.
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
                                                                 const T* __restrict__ L, const T* __restrict__ Xpk,
                                                                 double* __restrict__ xp, double* __restrict__ Y,
                                                                 double* __restrict__ su) {
+    constexpr int FR = solve_fwd_rows(sizeof(T)), NS = kSolveThreads / FR, CW = kSolveBW / NS;
     __shared__ double vb[kSolveBW], yb[kSolveBW];
     __shared__ double red[2 * kSolveThreads];
     const SolveItem it = items[blockIdx.x];
     const int n = it.n, j0 = it.j0, bw = min(kSolveBW, it.c - j0);
     const T* __restrict__ P = L + it.lofs;
-    const int tid = threadIdx.x, r = tid & 63, cs = tid >> 6;
+    const int tid = threadIdx.x, r = tid % FR, cs = tid / FR;
     const int i = it.a + r;
-    const int cvalid = min(32, bw - 32 * cs);          // real columns of this segment
-    T lv[32], m[kPkSlots];
-    if (cvalid > 0) issue_row32(lv, P, min(i, n - 1), j0 + 32 * cs, cvalid, n);
+    const int cvalid = min(CW, bw - CW * cs);          // real columns of this segment
+    const bool out = tid < FR && i < n;                // this thread writes row i
+    T lv[CW], m[kPkSlots];
+    if (cvalid > 0) issue_row<CW>(lv, P, min(i, n - 1), j0 + CW * cs, cvalid, n);
     issue_strided<kPkSlots>(m, Xpk + it.pk + tid, kSolveThreads);
     if (tid < kSolveBW) vb[tid] = tid < bw ? xp[it.k + j0 + tid] : 0.0;
+    // the old value of the output row is read with the other loads (FP32;
+    // in FP64 the registers are taken by the loads, so it is read at the end)
+    double old = 0.0;
+    if constexpr (sizeof(T) == 4)
+        if (out) old = i < it.c ? xp[it.k + i] : su[it.su + (i - it.c)];
     keep_in_flight(lv);
     keep_in_flight(m);
     __syncthreads();
@@ -1368,16 +1380,20 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem*
     double acc = 0.0;
     if (cvalid > 0) {
 #pragma unroll
-        for (int k = 0; k < 32; ++k) acc += (k < cvalid ? to_d(lv[k]) : 0.0) * yb[32 * cs + k];
+        for (int k = 0; k < CW; ++k) acc += (k < cvalid ? to_d(lv[k]) : 0.0) * yb[CW * cs + k];
     }
     red[tid] = acc;
     __syncthreads();
-    if (tid < kSolveFwdRows && it.a + tid < n) {
+    if (out) {
         const double* rr = red + tid;
-        const double sum = ((rr[0] + rr[64]) + (rr[128] + rr[192])) + ((rr[256] + rr[320]) + (rr[384] + rr[448]));
-        const int ii = it.a + tid;
-        if (ii < it.c) xp[it.k + ii] -= sum;
-        else su[it.su + ii - it.c] -= sum;
+        double sum;
+        if constexpr (NS == 4)
+            sum = (rr[0] + rr[FR]) + (rr[2 * FR] + rr[3 * FR]);
+        else
+            sum = ((rr[0] + rr[FR]) + (rr[2 * FR] + rr[3 * FR])) + ((rr[4 * FR] + rr[5 * FR]) + (rr[6 * FR] + rr[7 * FR]));
+        double* __restrict__ dst = i < it.c ? xp + it.k + i : su + it.su + (i - it.c);
+        if constexpr (sizeof(T) == 4) *dst = old - sum;
+        else *dst -= sum;
     }
 }
 
@@ -1449,6 +1465,7 @@ __global__ __launch_bounds__(kSolveCols) void bwd_upd_sum_kernel(const SolveItem
     const int i = threadIdx.x;
     if (it.a + i >= it.c) return;
     const double* __restrict__ p = partial + (int64_t)it.slot * kSolveCols + i;
+    const double yold = Y[it.k + it.a + i];
     double s = 0.0;
     for (int t0 = 0; t0 < it.nslot; t0 += 16) {
         double v[16];
@@ -1459,7 +1476,7 @@ __global__ __launch_bounds__(kSolveCols) void bwd_upd_sum_kernel(const SolveItem
         for (int t = 0; t < 16; ++t)
             if (t0 + t < it.nslot) s += v[t];
     }
-    Y[it.k + it.a + i] -= s;
+    Y[it.k + it.a + i] = yold - s;
 }
 
 // Backward, one kSolveBW-column block of a large front per launch (blocks
@@ -1485,6 +1502,7 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_blk_kernel(const SolveItem*
     if (own) issue_strided<32>(lv, P + (j0 + min(r, bw - 1)) + (int64_t)(it.a + 32 * h) * n, n);
     issue_strided<kPkSlots>(m, Xpk + it.pk + kPk + tid, kSolveThreads);
     if (tid < kSolveBW) vb[tid] = kSolveBW - 1 - tid < bw ? Y[it.k + j0 + kSolveBW - 1 - tid] : 0.0;
+    const double yold = own && tid < kSolveCols ? Y[it.k + it.a + tid] : 0.0;
     if (own) keep_in_flight(lv);
     keep_in_flight(m);
     __syncthreads();
@@ -1499,7 +1517,7 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_blk_kernel(const SolveItem*
 #pragma unroll
     for (int k = 0; k < 32; ++k) acc[k] = to_d(lv[k]) * xv;
     const double sum = column_sums64(acc, red, tid);
-    if (tid < kSolveCols) Y[it.k + it.a + tid] -= sum;
+    if (tid < kSolveCols) Y[it.k + it.a + tid] = yold - sum;
 }
 
 // -------------------------------------------------------------- refinement

@@ -210,7 +210,8 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     if (j0 >= d.c) continue;
                     const int j1 = std::min(j0 + BW, d.c);
                     P.sv_items.push_back({f, j1, 1, j0});
-                    for (int a = j1 + kSolveFwdRows; a < d.n; a += kSolveFwdRows) P.sv_items.push_back({f, a, 0, j0});
+                    const int fr = solve_fwd_rows(o.elem_bytes);
+                    for (int a = j1 + fr; a < d.n; a += fr) P.sv_items.push_back({f, a, 0, j0});
                     st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 16.0 * (d.n - j0);
                 }
                 st.count = (int)P.sv_items.size() - st.first;
@@ -721,7 +722,8 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                     it.pk < 0 || it.pk + 2 * kPk > P.invpk)
                     return "solve item packed inverse";
                 if (st.kind == SV_FWD_BLK) {
-                    if (it.first ? it.a != j1 : (it.a < j1 + kSolveFwdRows || it.a >= d.n)) return "solve item rows";
+                    const int fr = solve_fwd_rows(o.elem_bytes);
+                    if (it.first ? it.a != j1 : (it.a < j1 + fr || it.a >= d.n || (it.a - j1) % fr)) return "solve item rows";
                 } else if (it.first ? it.a != it.j0 : (it.a < 0 || it.a >= it.j0 || it.a % kSolveCols)) {
                     return "solve item columns";
                 }

@@ -108,7 +108,7 @@ struct Step {
 // columns (a block) per launch, kSolveThreads threads per workgroup, with
 // the block's inverse in packed form (see kPk):
 //  * FWD_BLK: every workgroup solves the block (y = X v) and updates its
-//    kSolveFwdRows rows below it; the `first` one stores y in Y.
+//    solve_fwd_rows() rows below it; the `first` one stores y in Y.
 //  * BWD_UPD (once per level, before the blocks): each workgroup takes
 //    kSolveCols pivot columns of a front and kSolveUpdRows of its update
 //    rows (solved by the ancestors, read through the row list q) and writes
@@ -123,7 +123,9 @@ struct Step {
 constexpr int kSolveBW = 256;
 constexpr int kSolveSmall = 64;
 constexpr int kSolveThreads = 512;
-constexpr int kSolveFwdRows = 64;   // rows per forward workgroup below a block
+// rows per forward workgroup below a block: 128 when the factor is FP32,
+// 64 when FP64 (register budget of the row loads)
+constexpr int solve_fwd_rows(int elem_bytes) { return elem_bytes == 4 ? 128 : 64; }
 constexpr int kSolveCols = 64;      // pivot columns per backward workgroup
 constexpr int kSolveUpdRows = 1024; // update rows per BWD_UPD workgroup
 // Packed lower-triangular kSolveBW x kSolveBW matrix (a block inverse X for
@@ -139,7 +141,7 @@ enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_B
 
 struct SolveItem {
     int front;
-    int a;        // FWD_BLK: first row (front-local) of the kSolveFwdRows-row chunk;
+    int a;        // FWD_BLK: first row (front-local) of the solve_fwd_rows() chunk;
                   // BWD_*: first pivot column of the kSolveCols-column chunk
     int first;    // *_BLK: 1 if this item stores the block's solution
     int j0;       // *_BLK: first pivot column of the block; BWD_UPD: first update row

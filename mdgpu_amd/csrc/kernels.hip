@@ -106,27 +106,29 @@ __global__ __launch_bounds__(kThreads) void extend_add_kernel(const EaItem* __re
             rowoff = pd.c;
         }
         const T* scol = src + (int64_t)j * cd.m;
-        // eight rows per lane at a time: sources, targets and old target
-        // values are all loaded before any store (the relative map is
-        // injective, so the eight targets are distinct; the empty asm keeps
-        // the loads from waiting one by one behind the previous stores)
-        int i0 = j + lane;
-        for (; i0 + 7 * 64 < cd.m; i0 += 8 * 64) {
+        // eight rows per lane at a time, the last batch of a column too:
+        // sources, targets and old target values are all loaded before any
+        // store (the relative map is injective, so the targets are
+        // distinct; rows past the column end load row m - 1 again and are
+        // not stored; the empty asm keeps the loads from waiting one by one
+        // behind the previous stores)
+        for (int i0 = j + lane; i0 < cd.m; i0 += 8 * 64) {
             int ri[8];
             T sv[8], dv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                ri[u] = r[i0 + 64 * u] - rowoff;
-                sv[u] = scol[i0 + 64 * u];
+                const int i = min(i0 + 64 * u, cd.m - 1);
+                ri[u] = r[i] - rowoff;
+                sv[u] = scol[i];
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) dv[u] = dcol[ri[u]];
 #pragma unroll
             for (int u = 0; u < 8; ++u) asm volatile("" : "+v"(dv[u]), "+v"(sv[u]));
 #pragma unroll
-            for (int u = 0; u < 8; ++u) dcol[ri[u]] = dv[u] + sv[u];
+            for (int u = 0; u < 8; ++u)
+                if (i0 + 64 * u < cd.m) dcol[ri[u]] = dv[u] + sv[u];
         }
-        for (int i = i0; i < cd.m; i += 64) dcol[r[i] - rowoff] += scol[i];
     }
 }
 

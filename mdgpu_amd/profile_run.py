@@ -2,8 +2,9 @@
 
     python -m mdgpu_amd.profile_run --workload c3 --precision fp32 --factorizations 2 --solves 1
 
-Analyses the workload, factorizes it ``--factorizations`` times and solves
-``--solves`` times (eager launches, so every kernel is its own dispatch),
+Analyses the workload (optionally autotunes the GEMM tiles), factorizes it
+``--factorizations`` times and solves ``--solves`` times (eager launches, so
+every kernel is its own dispatch),
 then copies a buffer of ``--copy-mib`` MiB with a plain device copy.  The
 copy moves a known number of bytes, so hardware byte counters
 (FETCH_SIZE / WRITE_SIZE) read on it give the scale for the counters read
@@ -34,6 +35,8 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--mode", default="rn")
     ap.add_argument("--max-zeros", type=int, default=1000)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the per-launch GEMM tiles first (as bench.py does); its own factorizations come first")
     args = ap.parse_args(argv)
     import torch
     p, i, x = WORKLOADS[args.workload]()
@@ -42,6 +45,8 @@ def main(argv=None):
     print(f"analyzed n={S.n} fronts={S.nnodes} flops={S.flops:.3e} in {time.perf_counter() - t0:.1f}s", flush=True)
     ax = torch.from_numpy(x).cuda()
     F = Factorization(S, args.precision).set_graph(False)
+    if args.autotune:
+        print(f"autotune: {F.autotune(ax)} launches on 64-row tiles", flush=True)
     for k in range(args.factorizations):
         F.factorize(ax)
         torch.cuda.synchronize()

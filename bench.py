@@ -14,7 +14,7 @@ The C3 run also times the z-anisotropic variant (z-edge weights x 1e4, same
 pattern and analysis) and reports, for every workload, the fronts per level,
 the extend-add GB/s and the MFMA utilisation of the large GEMM launches.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c4|c5]
 """
 from __future__ import annotations
 
@@ -203,12 +203,83 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_reverse(args):
+    """C5: Gaussian-Markov-field precision Q(theta) = tau (kappa^2 I + L) on a
+    256^2 grid (L the 5-point Dirichlet Laplacian), theta = (log tau, log
+    kappa) = (0, -1), b ~ N(0, 1).  One step = forward (FP64 factorization,
+    log det Q, x = Q^{-1} b) + backward (selected inverse: d log det / dQ on
+    the pattern; solve adjoint with xbar = 1: bbar and -sym(bbar x^T)), with
+    d/d theta by the chain rule (dQ/dlog tau = Q, dQ/dlog kappa = 2 tau
+    kappa^2 I).  Checked against the closed forms d log det / d log tau = n
+    and d log det / d log kappa = sum 2 kappa^2 / (kappa^2 + lambda_ij)."""
+    import torch
+    from mdgpu_amd import gen, symbolic
+    from mdgpu_amd.numeric import Factorization
+    k, log_tau, log_kappa = 256, 0.0, -1.0
+    p, i, x = gen.gmrf_precision(k, log_tau, log_kappa)
+    S = symbolic.analyze(p, i, True, args.mode, args.max_zeros)
+    n = S.n
+    vals = torch.from_numpy(x).cuda()
+    rng = np.random.default_rng(0)
+    b = torch.from_numpy(rng.standard_normal(n)).cuda()
+    xbar = torch.ones(n, dtype=torch.float64, device="cuda")
+    cols = np.repeat(np.arange(n), np.diff(np.asarray(p)))
+    diag = torch.from_numpy((np.asarray(i) == cols).astype(np.float64)).cuda()   # diagonal entries
+    F = Factorization(S, "fp64")
+    tuned = F.autotune(vals) if not args.no_autotune else None
+
+    def step():
+        F.factorize(vals)
+        ld = F.logdet()
+        xs = F.solve(b)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g = F.logdet_grad()
+        bbar, ga = F.solve_adjoint(xs, xbar)
+        torch.cuda.synchronize()
+        return ld, xs, g, bbar, ga, t1
+
+    for _ in range(args.warmup):
+        step()
+    tf, tb = [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ld, xs, g, bbar, ga, t1 = step()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    tau, kappa = np.exp(log_tau), np.exp(log_kappa)
+    # d/dlog tau: dQ = Q (all entries); d/dlog kappa: dQ = 2 tau kappa^2 on the diagonal
+    dld_dlogtau = float((g * vals).sum())
+    dld_dlogkappa = float((g * diag).sum()) * 2.0 * tau * kappa * kappa
+    ii = np.arange(1, k + 1)
+    lam = (4.0 - 2.0 * np.cos(ii * np.pi / (k + 1)))[:, None] - 2.0 * np.cos(ii * np.pi / (k + 1))[None, :]
+    ld_ref = float(np.sum(np.log(tau * (kappa * kappa + lam))))
+    dk_ref = float(np.sum(2.0 * kappa * kappa / (kappa * kappa + lam)))
+    out = {
+        "metric": "reverse-mode step ms: FP64 factor + logdet + solve, then selected-inverse gradient + solve adjoint",
+        "value": (np.mean(tf) + np.mean(tb)) * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": (np.mean(tf) + np.mean(tb)) * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": "GMRF precision tau (kappa^2 I + L), 5-point Laplacian 256^2, "
+                               "theta = (log tau, log kappa) = (0, -1), b ~ N(0,1), xbar = 1",
+                   "n": n, "fronts": S.nnodes, "lnz": S.lnz, "partition": f"{args.mode} {args.max_zeros}"},
+        "forward_ms": np.mean(tf) * 1e3, "backward_ms": np.mean(tb) * 1e3,
+        "logdet": ld, "logdet_rel_err_closed_form": abs(ld - ld_ref) / abs(ld_ref),
+        "dlogdet_dlogtau": dld_dlogtau, "dlogdet_dlogtau_rel_err": abs(dld_dlogtau - n) / n,
+        "dlogdet_dlogkappa": dld_dlogkappa, "dlogdet_dlogkappa_rel_err": abs(dld_dlogkappa - dk_ref) / abs(dk_ref),
+        "autotune_64_tile_launches": tuned,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
     ap.add_argument("--mode", default="rn")
     ap.add_argument("--max-zeros", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-12)
@@ -232,6 +303,8 @@ def main():
 
     if args.workload == "c4":
         return bench_batch(args)
+    if args.workload == "c5":
+        return bench_reverse(args)
     desc, make = WORKLOADS[args.workload]
     p, i, x = make(gen)
     xt, b = gen.rhs_from_xtrue(p, i, x, seed=0)
@@ -256,6 +329,17 @@ def main():
         xs = r.pop("x").cpu().numpy()
         r["rel_err_xtrue"] = float(np.abs(xs - xt).max() / np.abs(xt).max())
         r["x"] = xs
+        if prec == "fp32":
+            # same factorization with the values handed over in host memory (the
+            # C++ drop-in's case): includes the PCIe upload of A's values
+            ax_host = torch.from_numpy(x)
+            tph = []
+            for _ in range(max(args.steps, 1)):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                F.factorize(ax_host)
+                tph.append(time.perf_counter() - t0)
+            r["factor_s_host_values"] = float(np.mean(tph))
         prof = F.factorize_profile(ax)
         r["profile"] = prof
         if prec in ("fp32", "fp64"):
@@ -368,6 +452,7 @@ def main():
                                       "fp64": r64["autotune_64_tile_launches"]},
         "fp32_ir": {"factor_ms": r32["factor_s"] * 1e3, "refine_ms": r32["refine_s"] * 1e3,
                     "time_to_solve_ms": tts32 * 1e3, "ir_iters": r32["iters"], "eta": r32["eta"],
+                    "factor_ms_host_values": r32["factor_s_host_values"] * 1e3,
                     "eta_cpu_check": eta32_cpu, "rel_err_xtrue": r32["rel_err_xtrue"],
                     "rel_diff_vs_fp64_solution": x32_vs_64},
         "fp64": {"factor_ms": r64["factor_s"] * 1e3, "refine_ms": r64["refine_s"] * 1e3,

@@ -251,24 +251,22 @@ __device__ inline void stage_rows(const T (&a)[64], T* Ls, T* rd, int lane) {
 }
 
 
-// Cholesky of the 64x64 block held column-major in LDS (As[r + 65 c]), in
-// four 16-column panels: each panel factored by wavefront 0 with its rows in
-// registers (pivot and column entries broadcast by v_readlane), the trailing
-// part of the block updated by all four wavefronts from LDS.  bad = first
-// non-positive pivot (64 if none).  Called by all threads of the workgroup.
 // Trailing update of the 64x64 Cholesky after the panel of columns
 // [c0, c0 + 16): As(i, j) -= sum_t As(i, c0 + t) As(j, c0 + t) on the 16x16
-// tiles (I, J), I >= J, of rows/cols [c0 + 16, 64), one tile (four 16x16x4
-// MFMA products) per wavefront in turn.  The upper halves of the diagonal
-// tiles are updated too; they are never read.
+// tiles (I, J), I >= J, of rows/cols [c0 + 16, 64) with jlo <= J < jhi,
+// one tile (four 16x16x4 MFMA products) per wavefront of [wfirst, wfirst +
+// wcount) in turn.  The upper halves of the diagonal tiles are updated too;
+// they are never read.
 template <typename U>
-__device__ inline void chol64_trailing(U* As, int c0, int wave, int lane) {
+__device__ inline void chol64_trailing(U* As, int c0, int wave, int lane, int jlo, int jhi, int wfirst,
+                                       int wcount) {
     typedef typename MfmaOp<U>::acc_t acc_t;
     const int nt = (48 - c0) / 16, li = lane & 15, lg = lane >> 4;
+    if (wave < wfirst || wave >= wfirst + wcount) return;
     int t = 0;
     for (int I = 0; I < nt; ++I)
-        for (int J = 0; J <= I; ++J, ++t) {
-            if ((t & 3) != wave) continue;
+        for (int J = jlo; J <= I && J < jhi; ++J, ++t) {
+            if (t % wcount != wave - wfirst) continue;
             const int i0 = c0 + 16 + 16 * I, j0 = c0 + 16 + 16 * J;
             acc_t acc = {U(0), U(0), U(0), U(0)};
 #pragma unroll
@@ -282,6 +280,13 @@ __device__ inline void chol64_trailing(U* As, int c0, int wave, int lane) {
         }
 }
 
+// Cholesky of the 64x64 block held column-major in LDS (As[r + 65 c]), in
+// four 16-column panels: each panel factored by wavefront 0 with its rows in
+// registers (pivot and column entries broadcast by v_readlane).  After a
+// panel, the four wavefronts update the next panel's columns (one MFMA tile
+// each); the rest of that panel's trailing update runs on wavefronts 1-3
+// while wavefront 0 factors the next panel.  bad = first
+// non-positive pivot (64 if none).  Called by all threads of the workgroup.
 template <typename U>
 __device__ void chol64_panels(U* As, int tid, int& bad) {
     const int wave = tid >> 6, lane = tid & 63;
@@ -306,10 +311,14 @@ __device__ void chol64_panels(U* As, int tid, int& bad) {
                 for (int j = 0; j < 16; ++j)
                     if (lane >= j) As[r + 65 * (c0 + j)] = a[j];
             }
+        } else if (p > 0) {
+            // look-ahead: the previous panel's update of the columns right
+            // of this panel, while wavefront 0 factors it
+            chol64_trailing<U>(As, c0 - 16, wave, lane, 1, 4, 1, 3);
         }
         __syncthreads();
         if (p < 3) {
-            chol64_trailing<U>(As, c0, wave, lane);
+            chol64_trailing<U>(As, c0, wave, lane, 0, 1, 0, 4);
             __syncthreads();
         }
     }

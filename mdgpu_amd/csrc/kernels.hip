@@ -303,8 +303,14 @@ __device__ void chol64_panels(U* As, int tid, int& bad) {
                 bad = (!(djj > U(0)) && bad == 64) ? c0 + j : bad;
                 const U is = pivot_rsqrt(djj);
                 a[j] = (lane == j ? djj : a[j]) * is;
+                // all broadcasts of the column first, then the updates: the
+                // scalars land in distinct SGPRs, so no update waits on the
+                // broadcast just before it
+                U l[16];
 #pragma unroll
-                for (int k = j + 1; k < 16; ++k) a[k] -= a[j] * bcast(a[j], k);
+                for (int k = j + 1; k < 16; ++k) l[k] = bcast(a[j], k);
+#pragma unroll
+                for (int k = j + 1; k < 16; ++k) a[k] -= a[j] * l[k];
             }
             if (r < 64) {
 #pragma unroll

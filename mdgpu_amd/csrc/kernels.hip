@@ -285,10 +285,11 @@ __device__ inline void chol64_trailing(U* As, int c0, int wave, int lane, int jl
 // registers (pivot and column entries broadcast by v_readlane).  After a
 // panel, the four wavefronts update the next panel's columns (one MFMA tile
 // each); the rest of that panel's trailing update runs on wavefronts 1-3
-// while wavefront 0 factors the next panel.  bad = first
-// non-positive pivot (64 if none).  Called by all threads of the workgroup.
+// while wavefront 0 factors the next panel.  A non-positive pivot d leaves
+// L(j, j) = d rsq(d) = NaN, so the first one is found afterwards from the
+// diagonal.  Called by all threads of the workgroup.
 template <typename U>
-__device__ void chol64_panels(U* As, int tid, int& bad) {
+__device__ void chol64_panels(U* As, int tid) {
     const int wave = tid >> 6, lane = tid & 63;
     for (int p = 0; p < 4; ++p) {
         const int c0 = 16 * p;
@@ -300,7 +301,6 @@ __device__ void chol64_panels(U* As, int tid, int& bad) {
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const U djj = bcast(a[j], j);
-                bad = (!(djj > U(0)) && bad == 64) ? c0 + j : bad;
                 const U is = pivot_rsqrt(djj);
                 a[j] = (lane == j ? djj : a[j]) * is;
                 // all broadcasts of the column first, then the updates: the
@@ -455,8 +455,7 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
     __syncthreads();
     if (tid == 0) last = atomicAdd(&counters[jb.cnt], 1) == jb.ntiles - 1;
     if (tid < 64) a0[tid] = As[tid * 66];            // assembled diagonal
-    int bad = 64;
-    chol64_panels<T>(As, tid, bad);
+    chol64_panels<T>(As, tid);
     bool fell_back = false;
     if constexpr (std::is_same<T, float>::value) {
         // pivot-growth check; on failure factor the block again in FP64
@@ -471,8 +470,7 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
                 Ad[r + 65 * c] = (r < b && c < b) ? (double)vin[q] : (r == c ? 1.0 : 0.0);
             }
             __syncthreads();
-            bad = 64;
-            chol64_panels<double>(Ad, tid, bad);
+            chol64_panels<double>(Ad, tid);
             for (int t = tid; t < 64 * 64; t += kThreads) {
                 const int r = t & 63, c = t >> 6;
                 As[r + 65 * c] = (T)Ad[r + 65 * c];
@@ -481,6 +479,13 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
         }
     }
     if (tid < 64) rd[tid] = T(1) / As[tid * 66];
+    // first non-positive pivot (64 if none): the first diagonal entry of L
+    // that is not > 0 (NaN included)
+    int bad = 64;
+    if (wave == 0) {
+        const unsigned long long nb = __ballot(!(As[lane * 66] > T(0)));
+        bad = nb ? __builtin_ctzll(nb) : 64;
+    }
     if (last) {
         for (int t = tid; t < 64 * 64; t += kThreads) {
             const int r = t & 63, c = t >> 6;

@@ -177,3 +177,27 @@ def test_autotune_keeps_the_factor(device):
     assert np.array_equal(Li, Li_ref) and rel_err(Lx, Lx_ref) <= 1e-10
     F.factorize(ax)
     assert np.array_equal(F.export_L()[2], Lx)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_indefinite_matrix_is_reported(device, prec):
+    """A - sigma I with sigma above lambda_min: the factorization meets a
+    non-positive pivot late (in the large fronts of the blocked path) and
+    returns MDGPU_ERR_NOT_SPD naming the column; a following factorization
+    of the SPD matrix on the same handle succeeds."""
+    import torch
+    from mdgpu_amd import _lib as L
+    p, i, x = gen.laplace3d_7pt(16)          # lambda_min = 6.001 - 6 cos(pi/17) ~ 0.103
+    n = len(p) - 1
+    cols = np.repeat(np.arange(n), np.diff(p))
+    xs = x.copy()
+    xs[i == cols] -= 0.2
+    S = symbolic.analyze(p, i, True, "rn", 4000)
+    F = Factorization(S, prec)
+    with pytest.raises(L.MdgpuError) as e:
+        F.factorize(torch.from_numpy(xs).to(device))
+    assert e.value.code == L.MDGPU_ERR_NOT_SPD and "non-positive pivot" in str(e.value)
+    F.factorize(torch.from_numpy(x).to(device))
+    xt, b = gen.rhs_from_xtrue(p, i, x, seed=9)
+    _, eta, _ = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=30)
+    assert eta <= 1e-12

@@ -891,35 +891,79 @@ __global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __re
     store(0, 0);
     if (KT < g.K) fetch(KT);
     __syncthreads();
-    for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
-        // tile k0 + KT (in registers) to the other buffer, whose last
-        // readers passed the barrier that ended the previous iteration
-        if (k0 + KT < g.K) {
-            store(cur ^ 1, k0 + KT);
-            if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
-        }
-        const T* __restrict__ Ac = As[cur];
-        const T* __restrict__ Bc = Bs[cur];
+    if constexpr (sizeof(T) == 8) {
+        // FP64: each iteration is one basic block (the store and fetch of
+        // the next tiles run unconditionally: past the end they store zeros
+        // to the unused buffer and load clamped addresses), and scheduling
+        // groups interleave them with the multiplications -- measured
+        // faster for FP64, slower for FP32
+        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+            const T* __restrict__ Ac = As[cur];
+            const T* __restrict__ Bc = Bs[cur];
+            T av[KT / 16][NB][4], bv[KT / 16][NB][4];
 #pragma unroll
-        for (int kq = 0; kq < KT; kq += 16) {
-            T av[NB][4], bv[NB][4];   // [block][t]
-#pragma unroll
-            for (int rb2 = 0; rb2 < NB; ++rb2)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) av[rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
-#pragma unroll
-            for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) bv[cb][t] = Bc[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int q = 0; q < KT / 16; ++q) {
 #pragma unroll
                 for (int rb2 = 0; rb2 < NB; ++rb2)
 #pragma unroll
-                    for (int cb = 0; cb < NB; ++cb)
-                        acc[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], acc[rb2][cb]);
+                    for (int t = 0; t < 4; ++t) av[q][rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + 16 * q + 4 * lg + t];
+#pragma unroll
+                for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) bv[q][cb][t] = Bc[(wc + cb * 16 + li) * LDK + 16 * q + 4 * lg + t];
+            }
+            store(cur ^ 1, k0 + KT);
+            fetch(k0 + 2 * KT);
+#pragma unroll
+            for (int q = 0; q < KT / 16; ++q)
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int rb2 = 0; rb2 < NB; ++rb2)
+#pragma unroll
+                        for (int cb = 0; cb < NB; ++cb)
+                            acc[rb2][cb] = MfmaOp<T>::mma(bv[q][cb][t], av[q][rb2][t], acc[rb2][cb]);
+            // NG rounds of: a share of the products, the two 16-byte LDS
+            // stores of one group, its 2 G global loads
+#pragma unroll
+            for (int i = 0; i < NG; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, (KT / 4) * NB * NB / NG, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x200, 2, 0);                          // DS write
+                __builtin_amdgcn_sched_group_barrier(0x020, 2 * G, 0);                      // VMEM read
+            }
+            __syncthreads();
         }
-        __syncthreads();
+    } else {
+        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+            // tile k0 + KT (in registers) to the other buffer, whose last
+            // readers passed the barrier that ended the previous iteration
+            if (k0 + KT < g.K) {
+                store(cur ^ 1, k0 + KT);
+                if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+            }
+            const T* __restrict__ Ac = As[cur];
+            const T* __restrict__ Bc = Bs[cur];
+    #pragma unroll
+            for (int kq = 0; kq < KT; kq += 16) {
+                T av[NB][4], bv[NB][4];   // [block][t]
+    #pragma unroll
+                for (int rb2 = 0; rb2 < NB; ++rb2)
+    #pragma unroll
+                    for (int t = 0; t < 4; ++t) av[rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
+    #pragma unroll
+                for (int cb = 0; cb < NB; ++cb)
+    #pragma unroll
+                    for (int t = 0; t < 4; ++t) bv[cb][t] = Bc[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
+    #pragma unroll
+                for (int t = 0; t < 4; ++t)
+    #pragma unroll
+                    for (int rb2 = 0; rb2 < NB; ++rb2)
+    #pragma unroll
+                        for (int cb = 0; cb < NB; ++cb)
+                            acc[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], acc[rb2][cb]);
+            }
+            __syncthreads();
+        }
     }
     // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + row(lg, v))
     gemm_epilogue<T, NB>(acc, C, g, row0 + wr + li, col0 + wc, lg,

@@ -800,15 +800,18 @@ __device__ inline int xcd_tile(int bid, int n) {
     return x * q + min(x, r) + k;
 }
 
+// Register budget: at least 3 wavefronts per SIMD in FP32, 2 in FP64
+// (measured: the 128-row FP32 kernel then runs four workgroups per CU, the
+// FP64 one two instead of one).
 template <typename T, int BT>
-__global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
+__global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                              T* __restrict__ L, T* __restrict__ U) {
-    // k-tile depth: 32 for FP32, 16 for FP64 (register budget of the
-    // prefetch); the next k-tile is loaded into registers while the current
-    // one is multiplied out of LDS.
+    // k-tile depth 16 (register budget of the prefetch, and LDS for four
+    // FP32 workgroups per CU); the next k-tile is loaded into registers while
+    // the current one is multiplied out of LDS.
     // BT x BT tiles (128, or 64 for narrow updates); each wavefront owns a
     // (BT/2) x (BT/2) quadrant of NB x NB 16x16 MFMA blocks
-    constexpr int KT = sizeof(T) == 4 ? 32 : 16, LDK = KT + 4, NB = BT / 32, KSTEP = kThreads / BT;
+    constexpr int KT = 16, LDK = KT + 4, NB = BT / 32, KSTEP = kThreads / BT;
     constexpr int PER = (BT * KT) / kThreads;      // staged elements per thread and operand
     constexpr int G = 16 / sizeof(T), NG = PER / G; // k values per 16-byte LDS store, stores per operand
     typedef T vec_t __attribute__((ext_vector_type(G)));

@@ -252,30 +252,6 @@ __device__ inline double bcast(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Column `lane` of inv(L) by forward substitution, in LDS:
-// Xs[r*65 + lane] = inv(L)(r, lane), reading L rows Ls(r, l) = Ls[r*65 + l]
-// (uniform addresses) and rd[r] = 1 / L(r, r).  Each lane only reads its
-// own column of Xs, so no barrier is needed between rows.
-template <typename T>
-__device__ inline void wave_trinv64(const T* Ls, const T* rd, T* Xs, int lane) {
-    for (int r = 0; r < 64; ++r) {
-        T acc = (lane == r) ? T(1) : T(0);
-#pragma unroll 8
-        for (int l = lane; l < r; ++l) acc -= Ls[r * 65 + l] * Xs[l * 65 + lane];
-        Xs[r * 65 + lane] = acc * rd[r];
-    }
-}
-
-// Stage the factor rows (lane i: a[0..i]) into LDS for wave_trinv64.
-template <typename T>
-__device__ inline void stage_rows(const T (&a)[64], T* Ls, T* rd, int lane) {
-#pragma unroll
-    for (int c = 0; c < 64; ++c) Ls[lane * 65 + c] = c <= lane ? a[c] : T(0);
-    rd[lane] = T(1) / Ls[lane * 65 + lane];
-    __syncthreads();
-}
-
-
 // Trailing update of the 64x64 Cholesky after the panel of columns
 // [c0, c0 + 16): As(i, j) -= sum_t As(i, c0 + t) As(j, c0 + t) on the 16x16
 // tiles (I, J), I >= J, of rows/cols [c0 + 16, 64) with jlo <= J < jhi,
@@ -314,7 +290,7 @@ __device__ inline void chol64_trailing(U* As, int c0, int wave, int lane, int jl
 // L(j, j) = d rsq(d) = NaN, so the first one is found afterwards from the
 // diagonal.  Called by all threads of the workgroup.
 template <typename U>
-__device__ void chol64_panels(U* As, int tid) {
+__device__ __forceinline__ void chol64_panels(U* As, int tid) {
     const int wave = tid >> 6, lane = tid & 63;
     for (int p = 0; p < 4; ++p) {
         const int c0 = 16 * p;
@@ -363,7 +339,7 @@ __device__ void chol64_panels(U* As, int tid) {
 // L_IK Z_KJ on MFMA, one block per wavefront, the sum staged in Ts (16 x 17
 // per wavefront).  Called by all threads; ends with a barrier.
 template <typename T>
-__device__ void inv64_blocked(const T* As, const T* rd, T* Zs, T* Ts, int wave, int lane) {
+__device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T* Ts, int wave, int lane) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     const int li = lane & 15, lg = lane >> 4;
     if (lane < 16) {
@@ -555,26 +531,44 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
     }
 }
 
-// Inverse of one 64-column diagonal block of an already factored panel.
+// Inverse of one 64-column diagonal block of an already factored panel
+// (identity beyond the block width b), one 256-thread workgroup per block,
+// by 16x16 blocks on MFMA (inv64_blocked).  Stored as W(i + 64 r) =
+// inv(L_D)(r, i) for i, r < b, zero above the diagonal.
 template <typename T>
-__global__ __launch_bounds__(64) void inv_kernel(const InvJob* __restrict__ jobs, const FrontDesc* __restrict__ fronts,
-                                                 const T* __restrict__ L, T* __restrict__ inv) {
-    __shared__ T Ls[64 * 65];
-    __shared__ T Xs[64 * 65];
+__global__ __launch_bounds__(kThreads) void inv_kernel(const InvJob* __restrict__ jobs,
+                                                       const FrontDesc* __restrict__ fronts,
+                                                       const T* __restrict__ L, T* __restrict__ inv) {
+    __shared__ T As[64 * 65];
+    __shared__ T Zs[64 * 65 + 4 * 16 * 17];
     __shared__ T rd[64];
     const InvJob jb = jobs[blockIdx.x];
     const FrontDesc d = fronts[jb.front];
-    const int b = min(64, d.c - jb.j0), lane = threadIdx.x;
-    const T* __restrict__ g = L + d.lofs + jb.j0 + (int64_t)jb.j0 * d.n;
-    T a[64];
+    const int b = min(64, d.c - jb.j0), n = d.n;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const T* __restrict__ g = L + d.lofs + jb.j0 + (int64_t)jb.j0 * n;
+    T v[16];
 #pragma unroll
-    for (int c = 0; c < 64; ++c)
-        a[c] = (lane < b && c < b && c <= lane) ? g[lane + (int64_t)c * d.n] : (lane == c ? T(1) : T(0));
-    stage_rows(a, Ls, rd, lane);
-    wave_trinv64(Ls, rd, Xs, lane);
+    for (int q = 0; q < 16; ++q) {
+        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+        v[q] = g[min(r, b - 1) + (int64_t)min(c, b - 1) * n];
+    }
+    keep_in_flight(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+        As[r + 65 * c] = (r < b && c < b) ? v[q] : (r == c ? T(1) : T(0));
+    }
+    __syncthreads();
+    if (tid < 64) rd[tid] = T(1) / As[tid * 66];
+    __syncthreads();
+    inv64_blocked<T>(As, rd, Zs, Zs + 64 * 65, wave, lane);
     T* __restrict__ w = inv + d.dinv + (int64_t)(jb.j0 / 64) * 4096;
-    if (lane < b)
-        for (int r = 0; r < b; ++r) w[lane + 64 * r] = Xs[r * 65 + lane];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int t = tid + kThreads * q, i = t & 63, r = t >> 6;
+        if (i < b && r < b) w[i + 64 * r] = r >= i ? Zs[r + 65 * i] : T(0);
+    }
 }
 
 // Inverse X of a kSolveBW-column diagonal block L_B of a front (for the
@@ -1747,7 +1741,7 @@ void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int
 }
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {
-    if (count > 0) inv_kernel<T><<<count, 64, 0, s>>>(jobs, fronts, L, inv);
+    if (count > 0) inv_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, bool fma, T* L, T* U, hipStream_t s) {

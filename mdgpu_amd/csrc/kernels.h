@@ -12,8 +12,9 @@ namespace mdgpu {
 
 template <typename T> void launch_fill(T* p, int64_t n, T v, hipStream_t s);
 template <typename T> void launch_scatter_a(T* L, const double* ax, const int64_t* dst, int nnz, hipStream_t s);
+// one wavefront per parent column item (Plan::ea), contributors Plan::ea_contrib
 template <typename T>
-void launch_extend_add(const EaItem* items, int count, const FrontDesc* fronts, const int* rel, T* L, T* U,
+void launch_extend_add(const EaItem* items, int count, const EaContrib* contrib, const int* rel, T* L, T* U,
                        hipStream_t s);
 template <typename T>
 void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,

@@ -77,83 +77,52 @@ __global__ void scatter_a_kernel(T* __restrict__ L, const double* __restrict__ a
 }
 
 // -------------------------------------------------------------- extend-add
-// parent(rel[i], rel[j]) += U_child(i, j) over the lower triangle of the
-// columns [j0, j1) of one child (ExtendAdd_CPU / ExtendAdd_GPU_device,
-// mdgpu.cu:748-788).  A column lands either in a pivot column of the
-// parent's panel or in the parent's update matrix.  Each wavefront takes two
-// columns at a time in FP32 (one in FP64), lanes over rows (coalesced reads;
-// writes follow the increasing relative map), eight rows per lane and column
-// in flight.
+// parent(rel[i], rel[j]) += U_child(i, j) over the lower triangle
+// (ExtendAdd_CPU / ExtendAdd_GPU_device, mdgpu.cu:748-788), parent side: one
+// wavefront per parent column (EaItem), adding its contributors -- child
+// columns, in child order -- one after the other.  Lanes run over the child
+// rows (coalesced reads; writes follow the increasing relative map), eight
+// rows per lane in flight.  Two contributors can share target rows, so a
+// workgroup-scope fence (every wavefront of the workgroup shares one L1)
+// separates each contributor's stores from the next one's loads.
 template <typename T>
-__device__ __forceinline__ T* ea_target(const FrontDesc& pd, int pj, T* L, T* U, int& rowoff) {
-    if (pj < pd.c) {
-        rowoff = 0;
-        return L + pd.lofs + (int64_t)pj * pd.n;
-    }
-    rowoff = pd.c;
-    return U + pd.uofs + (int64_t)(pj - pd.c) * pd.m;
-}
-
-template <typename T>
-__global__ __launch_bounds__(kThreads) void extend_add_kernel(const EaItem* __restrict__ items,
-                                                              const FrontDesc* __restrict__ fronts,
+__global__ __launch_bounds__(kThreads) void extend_add_kernel(const EaItem* __restrict__ items, int count,
+                                                              const EaContrib* __restrict__ contrib,
                                                               const int* __restrict__ rel, T* __restrict__ L,
                                                               T* __restrict__ U) {
-    constexpr int W = kThreads / 64;
-    const EaItem it = items[blockIdx.x];
-    const FrontDesc cd = fronts[it.child];
-    const FrontDesc pd = fronts[cd.parent];
-    const int* __restrict__ r = rel + cd.relofs;
-    const T* __restrict__ src = U + cd.uofs;
-    const int m = cd.m, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // FP32: two columns per wavefront at a time; FP64: one (register budget)
-    constexpr int NC = sizeof(T) == 4 ? 2 : 1;
-    for (int j = it.j0 + wave; j < it.j1; j += NC * W) {
-        // columns j and jb = j + W (jb shorter: rows [jb, m)); without a
-        // second column, jb's loads repeat column j's and nothing is stored
-        const bool two = NC == 2 && j + W < it.j1;
-        const int jb = two ? j + W : j;
-        int off0, off1;
-        T* dcol0 = ea_target(pd, r[j], L, U, off0);
-        T* dcol1 = ea_target(pd, r[jb], L, U, off1);
-        const T* scol0 = src + (int64_t)j * m;
-        const T* scol1 = src + (int64_t)jb * m;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * (kThreads / 64) + wave;
+    if (k >= count) return;
+    const EaItem it = items[k];
+    T* __restrict__ dcol = (it.in_u ? U : L) + it.dst;
+    for (int t = it.cbeg; t < it.cend; ++t) {
+        const EaContrib e = contrib[t];
+        const int* __restrict__ r = rel + e.rel;
+        const T* __restrict__ scol = U + e.src;
+        const int m = e.m, j = e.j;
         // sources, targets and old target values are all loaded before any
-        // store (the relative map is injective, so the targets of a column
-        // are distinct, and two columns never share one); rows past a
-        // column's end load row m - 1 again and are not stored; the empty
-        // asm keeps the loads from waiting one by one behind the stores
-        for (int e0 = lane; j + e0 < m; e0 += 8 * 64) {
-            int ri0[8], ri1[8];
-            T sv0[8], sv1[8], dv0[8], dv1[8];
+        // store (the relative map is injective, so the targets are distinct);
+        // rows past the column end load row m - 1 again and are not stored;
+        // the empty asm keeps the loads from waiting one by one behind the
+        // previous stores
+        for (int e0 = j + lane; e0 < m; e0 += 8 * 64) {
+            int ri[8];
+            T sv[8], dv[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int i0 = min(j + e0 + 64 * u, m - 1);
-                ri0[u] = r[i0] - off0;
-                sv0[u] = scol0[i0];
-                if constexpr (NC == 2) {
-                    const int i1 = min(jb + e0 + 64 * u, m - 1);
-                    ri1[u] = r[i1] - off1;
-                    sv1[u] = scol1[i1];
-                }
+                const int i = min(e0 + 64 * u, m - 1);
+                ri[u] = r[i] - it.rowoff;
+                sv[u] = scol[i];
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                dv0[u] = dcol0[ri0[u]];
-                if constexpr (NC == 2) dv1[u] = dcol1[ri1[u]];
-            }
+            for (int u = 0; u < 8; ++u) dv[u] = dcol[ri[u]];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                asm volatile("" : "+v"(dv0[u]), "+v"(sv0[u]));
-                if constexpr (NC == 2) asm volatile("" : "+v"(dv1[u]), "+v"(sv1[u]));
-            }
+            for (int u = 0; u < 8; ++u) asm volatile("" : "+v"(dv[u]), "+v"(sv[u]));
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (j + e0 + 64 * u < m) dcol0[ri0[u]] = dv0[u] + sv0[u];
-                if constexpr (NC == 2)
-                    if (two && jb + e0 + 64 * u < m) dcol1[ri1[u]] = dv1[u] + sv1[u];
-            }
+            for (int u = 0; u < 8; ++u)
+                if (e0 + 64 * u < m) dcol[ri[u]] = dv[u] + sv[u];
         }
+        if (t + 1 < it.cend) __threadfence_block();
     }
 }
 
@@ -1719,9 +1688,9 @@ template <typename T> void launch_scatter_a(T* L, const double* ax, const int64_
     if (nnz > 0) scatter_a_kernel<T><<<blocks_for(nnz), kThreads, 0, s>>>(L, ax, dst, nnz);
 }
 template <typename T>
-void launch_extend_add(const EaItem* items, int count, const FrontDesc* fronts, const int* rel, T* L, T* U,
+void launch_extend_add(const EaItem* items, int count, const EaContrib* contrib, const int* rel, T* L, T* U,
                        hipStream_t s) {
-    if (count > 0) extend_add_kernel<T><<<count, kThreads, 0, s>>>(items, fronts, rel, L, U);
+    if (count > 0) extend_add_kernel<T><<<(count + 3) / 4, kThreads, 0, s>>>(items, count, contrib, rel, L, U);
 }
 template <typename T>
 void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,
@@ -1836,7 +1805,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
 #define MDGPU_INSTANTIATE(T)                                                                                   \
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                                 \
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
-    template void launch_extend_add<T>(const EaItem*, int, const FrontDesc*, const int*, T*, T*, hipStream_t); \
+    template void launch_extend_add<T>(const EaItem*, int, const EaContrib*, const int*, T*, T*, hipStream_t); \
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \

@@ -46,6 +46,7 @@ struct mdgpu_numeric {
     int* dQ = nullptr;
     int64_t* dAdst = nullptr;
     mdgpu::EaItem* dEa = nullptr;
+    mdgpu::EaContrib* dEaContrib = nullptr;
     mdgpu::GemmJob* dGemm = nullptr;
     mdgpu::BlkJob* dBlk = nullptr;
     int* dBlkCnt = nullptr;          // arrival counters of the block-column jobs
@@ -226,7 +227,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
         switch (st.kind) {
             case mdgpu::STEP_EA1:
             case mdgpu::STEP_EA2:
-                mdgpu::launch_extend_add<T>(N->dEa + st.first, st.count, N->dFronts, N->dRel, L, U, s);
+                mdgpu::launch_extend_add<T>(N->dEa + st.first, st.count, N->dEaContrib, N->dRel, L, U, s);
                 break;
             case mdgpu::STEP_PANEL:
                 mdgpu::launch_panel<T>(N->dPanel + st.first, st.count, N->dFronts, L, N->dInfo, st.arg, s);
@@ -416,6 +417,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.q, (void**)&N->dQ)) ||
         (st = upload(N, P.a_dst, (void**)&N->dAdst)) ||
         (st = upload(N, P.ea, (void**)&N->dEa)) ||
+        (st = upload(N, P.ea_contrib, (void**)&N->dEaContrib)) ||
         (st = upload(N, P.gemm, (void**)&N->dGemm)) ||
         (st = upload(N, P.blk, (void**)&N->dBlk)) ||
         (st = upload(N, std::vector<int>(P.blk.size(), 0), (void**)&N->dBlkCnt)) ||

@@ -330,44 +330,51 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         batch.clear();
     };
     auto add_ea = [&](int L, bool first_pass) {
-        const std::vector<int>& fl = P.level_fronts[L];
-        for (int r = 0; r < P.max_kids; r++) {
-            const int begin = (int)P.ea.size();
-            for (int f : fl) {
-                const FrontDesc& d = P.fronts[f];
-                if (d.kid_begin + r >= d.kid_end) continue;
-                const int ch = P.kids[d.kid_begin + r];
+        // items of this pass: every column of a level-L front that receives
+        // child columns (first pass: the parent's pivot columns; second: its
+        // update columns), contributors in child order
+        const int begin = (int)P.ea.size();
+        std::vector<std::pair<double, EaItem>> items;
+        for (int f : P.level_fronts[L]) {
+            const FrontDesc& pd = P.fronts[f];
+            std::vector<std::vector<EaContrib>> col(pd.n);
+            for (int t = pd.kid_begin; t < pd.kid_end; t++) {
+                const int ch = P.kids[t];
                 const FrontDesc& cd = P.fronts[ch];
-                int j0 = first_pass ? 0 : cd.split;
-                int j1 = first_pass ? cd.split : cd.m;
-                // chunk the lower-triangle columns to ~ea_chunk entries each
-                int j = j0;
-                while (j < j1) {
-                    int64_t acc = 0;
-                    int je = j;
-                    while (je < j1 && (acc == 0 || acc + (cd.m - je) <= o.ea_chunk)) {
-                        acc += cd.m - je;
-                        je++;
-                    }
-                    P.ea.push_back({ch, j, je, 0});
-                    j = je;
-                }
+                const int j0 = first_pass ? 0 : cd.split, j1 = first_pass ? cd.split : cd.m;
+                for (int j = j0; j < j1; j++)
+                    col[P.rel[cd.relofs + j]].push_back({cd.uofs + (int64_t)j * cd.m, cd.relofs, j, cd.m});
             }
-            const int count = (int)P.ea.size() - begin;
-            if (count > 0) {
-                Step st{first_pass ? STEP_EA1 : STEP_EA2, begin, count, count, L, 0};
-                for (int t = begin; t < begin + count; t++) {
-                    const EaItem& it = P.ea[t];
-                    const int m = P.fronts[it.child].m;
-                    // lower-triangle entries of columns [j0, j1): read child, RMW parent
-                    double e = 0;
-                    for (int j = it.j0; j < it.j1; j++) e += m - j;
-                    st.bytes += e * 3.0 * o.elem_bytes + 4.0 * (it.j1 - it.j0) + 4.0 * e;
+            for (int pc = 0; pc < pd.n; pc++) {
+                if (col[pc].empty()) continue;
+                EaItem it;
+                it.in_u = pc >= pd.c;
+                it.rowoff = it.in_u ? pd.c : 0;
+                it.dst = it.in_u ? pd.uofs + (int64_t)(pc - pd.c) * pd.m : pd.lofs + (int64_t)pc * pd.n;
+                it.cbeg = (int)P.ea_contrib.size();
+                double work = 0;
+                for (const EaContrib& e : col[pc]) {
+                    P.ea_contrib.push_back(e);
+                    work += e.m - e.j;
                 }
-                P.steps.push_back(st);
+                it.cend = (int)P.ea_contrib.size();
+                items.push_back({work, it});
             }
         }
+        if (items.empty()) return;
+        // longest columns first (the order of disjoint targets does not
+        // change any sum)
+        std::stable_sort(items.begin(), items.end(),
+                         [](const std::pair<double, EaItem>& x, const std::pair<double, EaItem>& y) { return x.first > y.first; });
+        Step st{first_pass ? STEP_EA1 : STEP_EA2, begin, (int)items.size(), ((int)items.size() + 3) / 4, L, 0};
+        for (const auto& w : items) {
+            P.ea.push_back(w.second);
+            // entries of the contributors: read child, RMW parent, map
+            st.bytes += w.first * (3.0 * o.elem_bytes + 4.0);
+        }
+        P.steps.push_back(st);
     };
+
 
     std::vector<std::pair<size_t, int>> joins;   // (first step index, side event to wait for)
     for (int L = 0; L < P.nlevels; L++) {
@@ -651,9 +658,21 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
     for (int64_t a : P.a_dst)
         if (a < 0 || a >= P.lstore) return bad("A scatter target");
     for (const EaItem& it : P.ea) {
-        if (it.child < 0 || it.child >= nf) return bad("EA child");
-        const FrontDesc& cd = P.fronts[it.child];
-        if (cd.parent < 0 || it.j0 < 0 || it.j1 > cd.m || it.j0 >= it.j1) return bad("EA range");
+        if (it.cbeg < 0 || it.cend > (int)P.ea_contrib.size() || it.cbeg >= it.cend) return bad("EA contributors");
+        if (it.in_u ? (it.rowoff < 0 || it.dst < 0) : (it.rowoff != 0 || it.dst < 0 || it.dst >= P.lstore))
+            return bad("EA target");
+        for (int t = it.cbeg; t < it.cend; t++) {
+            const EaContrib& e = P.ea_contrib[t];
+            if (e.j < 0 || e.j >= e.m || e.rel < 0 || e.rel + e.m > (int64_t)P.rel.size()) return bad("EA contributor");
+            if (e.src < 0 || e.src + e.m > P.arena) return bad("EA source");
+            // every target row lies in the parent column: rel values are
+            // below n_parent, checked with the relative maps above; the
+            // column itself must hold them
+            const int last = P.rel[e.rel + e.m - 1] - it.rowoff;
+            if (P.rel[e.rel + e.j] - it.rowoff < 0) return bad("EA row below the column");
+            const int64_t cap = it.in_u ? P.arena : P.lstore;
+            if (it.dst + last >= cap) return bad("EA target bounds");
+        }
     }
     // live update matrices must not overlap: at level L the live set is
     // {f : level(f) <= L <= level(parent(f))}

@@ -48,10 +48,22 @@ struct FrontDesc {            // mirrored on the device
 };
 
 // One extend-add work item: columns [j0, j1) of child `child`'s update.
+// Extend-add, parent side ("pull"): one item per parent column that
+// receives child columns in a pass, with its contributors (child columns)
+// in child order.  A wavefront owns the parent column and adds the
+// contributors one after the other, so all of a level's children go in
+// one launch per pass and each parent entry sums its children in a fixed
+// order.
 struct EaItem {
-    int child;
-    int j0, j1;
-    int pad;
+    int64_t dst;                   // offset of the parent column (L store or, in_u, the arena)
+    int in_u;                      // the column is in the parent's update matrix
+    int rowoff;                    // parent row of the column's first entry (0 or c_parent)
+    int cbeg, cend;                // contributors [cbeg, cend) in Plan::ea_contrib
+};
+struct EaContrib {
+    int64_t src;                   // child column in the arena: uofs + j * m
+    int64_t rel;                   // child's relative map (Plan::rel offset)
+    int j, m;                      // child column, child update rows (rows j..m-1 are added)
 };
 
 // One dense job: C -= A * B^T (beta0: C = -A * B^T) on a tile grid.
@@ -179,6 +191,7 @@ struct Plan {
     std::vector<SolveStep> sv_fwd, sv_bwd;
     int64_t sv_upd_partial = 1;        // doubles of BWD_UPD partial sums (one level at a time)
     std::vector<EaItem> ea;
+    std::vector<EaContrib> ea_contrib;
     std::vector<GemmJob> gemm;
     std::vector<BlkJob> blk;
     std::vector<int> panel;            // fronts factored by the LDS panel kernel

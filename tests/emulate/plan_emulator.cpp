@@ -123,14 +123,11 @@ extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const 
         if (st.kind == STEP_EA1 || st.kind == STEP_EA2) {
             for (int t = st.first; t < st.first + st.count; t++) {
                 const EaItem& it = P.ea[t];
-                const FrontDesc& cd = P.fronts[it.child];
-                const FrontDesc& pd = P.fronts[cd.parent];
-                const int* r = P.rel.data() + cd.relofs;
-                for (int j = it.j0; j < it.j1; j++) {
-                    int pj = r[j];
-                    double* dcol = pj < pd.c ? &L[pd.lofs + (int64_t)pj * pd.n] : &U[pd.uofs + (int64_t)(pj - pd.c) * pd.m];
-                    int off = pj < pd.c ? 0 : pd.c;
-                    for (int i = j; i < cd.m; i++) dcol[r[i] - off] += U[cd.uofs + i + (int64_t)j * cd.m];
+                double* dcol = it.in_u ? &U[it.dst] : &L[it.dst];
+                for (int c = it.cbeg; c < it.cend; c++) {
+                    const EaContrib& e = P.ea_contrib[c];
+                    const int* r = P.rel.data() + e.rel;
+                    for (int i = e.j; i < e.m; i++) dcol[r[i] - it.rowoff] += U[e.src + i];
                 }
             }
         } else if (st.kind == STEP_PANEL) {

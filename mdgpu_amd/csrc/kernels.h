@@ -49,25 +49,29 @@ void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* 
                 double* xp, const double* Y, hipStream_t s);
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                        double* xp, double* su, const int64_t* suofs, hipStream_t s);
+// forward chain over a level's large fronts (ctr: the launch's ticket
+// counter, zero; Y holds kChainEmpty where the chain stores y)
 template <typename T>
-void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
-                    double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s);
+void launch_fwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, const double* xp,
+                      double* Y, double* su, hipStream_t s);
 // backward: BWD_UPD writes the update rows' dot products to partial-sum
-// slots, BWD_UPD_SUM subtracts them from Y, BWD_BLK solves a block into xp
-// and subtracts its rows' dot products from Y
+// slots, BWD_UPD_SUM subtracts them from Y, then the chain solves the blocks
+// into xp (which holds kChainEmpty there)
 template <typename T>
 void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
                     const double* xp, double* partial, hipStream_t s);
 void launch_bwd_upd_sum(const SolveItem* items, int count, const FrontDesc* fronts, const double* partial,
                         double* Y, hipStream_t s);
 template <typename T>
-void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
-                    double* xp, double* Y, hipStream_t s);
-// inverses of the kSolveBW-column diagonal blocks (Plan::inv256) and their
-// packed forms (Plan::invpk), after launch_inv
+void launch_bwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, double* xp,
+                      const double* Y, hipStream_t s);
+// inverses of the kSolveBW-column diagonal blocks (Plan::inv256), after
+// launch_inv
 template <typename T>
 void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,
-                   T* Xpk, hipStream_t s);
+                   hipStream_t s);
+// bit pattern of a chained-solve value not yet stored
+constexpr unsigned long long kChainEmptyBits = 0x7ff4dead0000beefULL;
 
 // refinement helpers (FP64)
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s);

@@ -551,7 +551,7 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restrict__ jobs,
                                                           const FrontDesc* __restrict__ fronts,
                                                           const T* __restrict__ L, const T* __restrict__ inv,
-                                                          T* __restrict__ X256, T* __restrict__ Xpk) {
+                                                          T* __restrict__ X256) {
     __shared__ T Al[64 * 33];
     __shared__ T Bl[32 * 65];
     const InvJob jb = jobs[blockIdx.x];
@@ -633,22 +633,6 @@ __global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restri
             }
             __syncthreads();
         }
-    }
-    // packed forms for the blocked solves (kPk layout): X, then R X^T R,
-    // whose entry (r, c) is X(255 - c, 255 - r)
-    T* __restrict__ pk = Xpk + (d.dinv256 / (kSolveBW * kSolveBW) + j0 / kSolveBW) * 2 * kPk;
-    for (int idx = tid; idx < kPk; idx += kThreads) {
-        const int k = idx / kSolveThreads, t = idx - k * kSolveThreads;
-        const int p = t >> 2, sg = t & 3, e = pk_first(sg) + k;
-        T vf = T(0), vr = T(0);
-        if (e < pk_first(sg + 1)) {
-            const bool lo = e <= p;
-            const int r = lo ? p : kSolveBW - 1 - p, c = lo ? e : e - p - 1;
-            vf = X[r + kSolveBW * c];
-            vr = X[(kSolveBW - 1 - c) + kSolveBW * (kSolveBW - 1 - r)];
-        }
-        pk[idx] = vf;
-        pk[kPk + idx] = vr;
     }
 }
 
@@ -1305,10 +1289,11 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_kernel(const int* __restric
 }
 
 // ------------------------------------------------- solves of large fronts
-// Fronts with more than kSolveSmall pivots: forward / backward substitution
-// kSolveBW pivot columns (a block) per launch, spread over many workgroups
-// of kSolveThreads threads: row chunks of solve_fwd_rows() below the block
-// (forward), column chunks of kSolveCols left of it (backward).
+// Fronts with more than kSolveSmall pivots: the children's update vectors
+// are gathered first (forward) or the update rows applied first (backward),
+// then the kChainBW-column blocks of all of a level's large fronts are
+// solved in one launch, each block by its own workgroup, which waits inside
+// the launch for the blocks it depends on (chained solves, below).
 
 // Forward, step 0 of a large front: its front vector (pivot entries from
 // xp, update entries zero) gathers the children's update vectors through
@@ -1367,91 +1352,143 @@ __global__ __launch_bounds__(kGatherThreads) void fwd_gather_kernel(const int* _
     }
 }
 
-// z = M v for a packed lower-triangular kSolveBW x kSolveBW matrix M (kPk
-// layout) of which this thread holds its kPkSlots entries m[]; vb and zb
-// are LDS vectors of kSolveBW entries, red an LDS scratch of 2
-// kSolveThreads.  Each output is the sum of its pair's four partial sums in
-// a fixed order.  Called by all kSolveThreads threads; ends with a barrier.
+// ------------------------------------------------------------ chained solves
+// The values of Y (forward) and xp (backward) that a chain launch waits for
+// hold kChainEmpty until the workgroup of their block stores them: a
+// signalling NaN, which arithmetic never produces (operations on a NaN give
+// a quiet NaN).
+constexpr unsigned long long kChainEmpty = kChainEmptyBits;
+constexpr int kChainSpin = 1 << 20;   // polls of one value before giving up on it
+
+// *p once it no longer holds kChainEmpty, read with agent-scope atomic
+// loads (another workgroup, possibly on another XCD, stores it with
+// chain_store).  After kChainSpin polls the wait ends with a quiet NaN,
+// which then shows in every result that depends on the value.
+__device__ __forceinline__ double chain_wait(const double* p) {
+    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+    unsigned long long v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int s = 0; v == kChainEmpty; ++s) {
+        if (s >= kChainSpin) return __builtin_nan("");
+        __builtin_amdgcn_s_sleep(2);
+        v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __longlong_as_double((long long)v);
+}
+__device__ __forceinline__ void chain_store(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The workgroup's item index: a ticket from the launch's counter, so every
+// item it waits for belongs to a workgroup that has already started.
+__device__ __forceinline__ int chain_ticket(int* ctr, int* sh) {
+    if (threadIdx.x == 0) *sh = atomicAdd(ctr, 1);
+    __syncthreads();
+    return *sh;
+}
+// Rows r0 and r1 of panel P (n rows, c columns), columns col .. col + 15;
+// columns from c on repeat column c - 1 (in range) and are multiplied by
+// zero by the caller.
 template <typename T>
-__device__ __forceinline__ void packed_trmv(const T (&m)[kPkSlots], const double* vb, double* zb, double* red,
-                                            int tid) {
-    const int p = tid >> 2, e0 = pk_first(tid & 3);
-    double a0 = 0.0, a1 = 0.0;
+__device__ __forceinline__ void chain_tile(T (&lo)[16], T (&hi)[16], const T* __restrict__ P, int r0, int r1,
+                                           int col, int c, int64_t n) {
+    // no branch around the loads: a later wait for some of them can then
+    // leave the rest in flight
 #pragma unroll
-    for (int k = 0; k < kPkSlots; ++k) {
-        const int e = e0 + k;
-        const bool lo = e <= p;
-        const double t = to_d(m[k]) * vb[lo ? e : min(e - p - 1, kSolveBW - 1)];
-        a0 += lo ? t : 0.0;
-        a1 += lo ? 0.0 : t;
+    for (int e = 0; e < 16; ++e) {
+        const int64_t off = (int64_t)min(col + e, c - 1) * n;
+        lo[e] = P[off + r0];
+        hi[e] = P[off + r1];
     }
-    red[tid] = a0;
-    red[kSolveThreads + tid] = a1;
-    __syncthreads();
-    if (tid < kSolveBW) {
-        const bool top = tid < kSolveBW / 2;
-        const double* r = red + (top ? 4 * tid : kSolveThreads + 4 * (kSolveBW - 1 - tid));
-        zb[tid] = (r[0] + r[1]) + (r[2] + r[3]);
-    }
-    __syncthreads();
 }
 
-
-// Forward, one kSolveBW-column block of a large front per launch.  Every
-// workgroup solves the block's triangle, y = X v with the packed block
-// inverse X (v is final on entry: earlier blocks were applied by earlier
-// launches), the `first` one stores y in Y, and each subtracts the block's
-// contribution from its solve_fwd_rows(sizeof(T)) rows [a, a + FR) below
-// the block: thread (r, cs) takes row a + r and kSolveBW / NS columns.  All
-// of a thread's loads (its row entries and the kPkSlots of X) are in flight
-// together; the NS column segments of a row are added in a fixed order.
+// Forward over the large fronts of one level: per front a chain of
+// kChainBW-column blocks inside the launch (TRI items), and the update rows
+// below the pivots (RECT items).  An item owns rows [a, a + 128) of its
+// front: thread (lane, wave) holds rows a + lane and a + 64 + lane of each
+// column block j < J, columns 128 j + 16 wave + e, and adds their products
+// with y_j (waited for by the first 128 threads, in LDS) to two row sums;
+// the eight wavefronts' sums of a row are added in a fixed order.  Block
+// j + 1's entries are loaded while block j is waited for and used, and a
+// TRI item loads its block inverse (thread (r, q): X(r, 32 q + e)) first, so
+// all of its data is in registers when the last y_j it needs arrives.
+//   TRI:  y_b = X_b (v_b - sum_{j<b} L_bj y_j), stored in Y (chain_store);
+//   RECT: u -= sum_j L_rj y_j.
 template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void fwd_blk_kernel(const SolveItem* __restrict__ items,
-                                                                const T* __restrict__ L, const T* __restrict__ Xpk,
-                                                                double* __restrict__ xp, double* __restrict__ Y,
-                                                                double* __restrict__ su) {
-    constexpr int FR = solve_fwd_rows(sizeof(T)), NS = kSolveThreads / FR, CW = kSolveBW / NS;
-    __shared__ double vb[kSolveBW], yb[kSolveBW];
-    __shared__ double red[2 * kSolveThreads];
-    const SolveItem it = items[blockIdx.x];
-    const int n = it.n, j0 = it.j0, bw = min(kSolveBW, it.c - j0);
+__global__ __launch_bounds__(kSolveThreads) void fwd_chain_kernel(const SolveItem* __restrict__ items,
+                                                                  int* __restrict__ ctr, const T* __restrict__ L,
+                                                                  const T* __restrict__ X256,
+                                                                  const double* __restrict__ xp,
+                                                                  double* __restrict__ Y, double* __restrict__ su) {
+    constexpr int BW = kChainBW;
+    __shared__ double yl[2][BW];
+    __shared__ double red[8][BW];
+    __shared__ double zb[BW];
+    __shared__ int sh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const SolveItem it = items[chain_ticket(ctr, &sh)];
+    const bool tri = it.first != 0;
+    const int n = it.n, c = it.c, row0 = it.a;
+    const int nrow = min(tri ? c : n, row0 + BW) - row0;
+    const int J = tri ? row0 / BW : (c + BW - 1) / BW;
     const T* __restrict__ P = L + it.lofs;
-    const int tid = threadIdx.x, r = tid % FR, cs = tid / FR;
-    const int i = it.a + r;
-    const int cvalid = min(CW, bw - CW * cs);          // real columns of this segment
-    const bool out = tid < FR && i < n;                // this thread writes row i
-    T lv[CW], m[kPkSlots];
-    if (cvalid > 0) issue_row<CW>(lv, P, min(i, n - 1), j0 + CW * cs, cvalid, n);
-    issue_strided<kPkSlots>(m, Xpk + it.pk + tid, kSolveThreads);
-    if (tid < kSolveBW) vb[tid] = tid < bw ? xp[it.k + j0 + tid] : 0.0;
-    // the old value of the output row is read with the other loads (FP32;
-    // in FP64 the registers are taken by the loads, so it is read at the end)
-    double old = 0.0;
-    if constexpr (sizeof(T) == 4)
-        if (out) old = i < it.c ? xp[it.k + i] : su[it.su + (i - it.c)];
-    keep_in_flight(lv);
-    keep_in_flight(m);
-    __syncthreads();
-    packed_trmv(m, vb, yb, red, tid);
-    if (it.first && tid < bw) Y[it.k + j0 + tid] = yb[tid];
-    double acc = 0.0;
-    if (cvalid > 0) {
+    const int r = tid & (BW - 1), q = tid >> 7;
+    double rhs = 0.0;
+    if (tid < nrow) rhs = tri ? xp[it.k + row0 + tid] : su[it.su + row0 + tid - c];
+    const int ra = min(row0 + lane, n - 1), rb = min(row0 + 64 + lane, n - 1);
+    double acc0 = 0.0, acc1 = 0.0;
+    // block j's products with y_j (tile tl / th), y_j waited for by the
+    // first 128 threads
+    auto apply = [&](int j, const T (&tl)[16], const T (&th)[16]) {
+        double* yb = yl[j & 1];
+        if (tid < BW) yb[tid] = tid < c - BW * j ? chain_wait(Y + it.k + BW * j + tid) : 0.0;
+        __syncthreads();
+        double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-        for (int k = 0; k < CW; ++k) acc += (k < cvalid ? to_d(lv[k]) : 0.0) * yb[CW * cs + k];
+        for (int e = 0; e < 16; ++e) {
+            const double y = yb[16 * wave + e];
+            s0 += to_d(tl[e]) * y;
+            s1 += to_d(th[e]) * y;
+        }
+        acc0 += s0;
+        acc1 += s1;
+    };
+    T cl[16], ch[16];
+    if (J > 0) chain_tile(cl, ch, P, ra, rb, 16 * wave, c, n);
+    for (int j = 0; j + 1 < J; ++j) {
+        T nl[16], nh[16];
+        chain_tile(nl, nh, P, ra, rb, BW * (j + 1) + 16 * wave, c, n);
+        apply(j, cl, ch);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            cl[e] = nl[e];
+            ch[e] = nh[e];
+        }
     }
-    red[tid] = acc;
+    // the block inverse goes in flight before the last wait
+    T xm[32];
+    if (tri) issue_strided<32>(xm, X256 + it.pk + r + (int64_t)(32 * kSolveBW) * q, kSolveBW);
+    if (J > 0) apply(J - 1, cl, ch);
+    red[wave][lane] = acc0;
+    red[wave][64 + lane] = acc1;
     __syncthreads();
-    if (out) {
-        const double* rr = red + tid;
-        double sum;
-        if constexpr (NS == 4)
-            sum = (rr[0] + rr[FR]) + (rr[2 * FR] + rr[3 * FR]);
-        else
-            sum = ((rr[0] + rr[FR]) + (rr[2 * FR] + rr[3 * FR])) + ((rr[4 * FR] + rr[5 * FR]) + (rr[6 * FR] + rr[7 * FR]));
-        double* __restrict__ dst = i < it.c ? xp + it.k + i : su + it.su + (i - it.c);
-        if constexpr (sizeof(T) == 4) *dst = old - sum;
-        else *dst -= sum;
+    double z = 0.0;
+    if (tid < nrow) {
+        const double sum = ((red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid])) +
+                           ((red[4][tid] + red[5][tid]) + (red[6][tid] + red[7][tid]));
+        z = rhs - sum;
     }
+    if (!tri) {
+        if (tid < nrow) su[it.su + row0 + tid - c] = z;
+        return;
+    }
+    if (tid < BW) zb[tid] = z;
+    __syncthreads();
+    double p = 0.0;
+#pragma unroll
+    for (int e = 0; e < 32; ++e) p += to_d(xm[e]) * zb[32 * q + e];
+    red[q][r] = p;
+    __syncthreads();
+    if (tid < nrow) chain_store(Y + it.k + row0 + tid, (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]));
 }
 
 // Backward, once per level before the blocks: workgroup (front, 64 pivot
@@ -1536,45 +1573,99 @@ __global__ __launch_bounds__(kSolveCols) void bwd_upd_sum_kernel(const SolveItem
     Y[it.k + it.a + i] = yold - s;
 }
 
-// Backward, one kSolveBW-column block of a large front per launch (blocks
-// from the last): every workgroup solves the block, x = X^T y, as
-// (R X^T R)(R y) with the packed backward form (y final on entry: the
-// update rows and the later blocks were applied by earlier launches).  The
-// `first` workgroup stores x; each other one owns 64 pivot columns
-// [a, a + 64) left of the block and subtracts their dot products with the
-// block's rows from Y.  Its 32 loads per thread of those rows are issued
-// with the inverse's, before x is known.
-template <typename T>
-__global__ __launch_bounds__(kSolveThreads) void bwd_blk_kernel(const SolveItem* __restrict__ items,
-                                                                const T* __restrict__ L, const T* __restrict__ Xpk,
-                                                                double* __restrict__ xp, double* __restrict__ Y) {
-    __shared__ double vb[kSolveBW], zb[kSolveBW];
-    __shared__ double red[2 * kSolveThreads];
-    const SolveItem it = items[blockIdx.x];
-    const int n = it.n, j0 = it.j0, bw = min(kSolveBW, it.c - j0);
-    const T* __restrict__ P = L + it.lofs;
-    const int tid = threadIdx.x, r = tid & 255, h = tid >> 8;
-    const bool own = !it.first;
-    T lv[32], m[kPkSlots];
-    if (own) issue_strided<32>(lv, P + (j0 + min(r, bw - 1)) + (int64_t)(it.a + 32 * h) * n, n);
-    issue_strided<kPkSlots>(m, Xpk + it.pk + kPk + tid, kSolveThreads);
-    if (tid < kSolveBW) vb[tid] = kSolveBW - 1 - tid < bw ? Y[it.k + j0 + kSolveBW - 1 - tid] : 0.0;
-    const double yold = own && tid < kSolveCols ? Y[it.k + it.a + tid] : 0.0;
-    if (own) keep_in_flight(lv);
-    keep_in_flight(m);
-    __syncthreads();
-    packed_trmv(m, vb, zb, red, tid);
-    // x(i) = zb[255 - i]
-    if (!own) {
-        if (tid < bw) xp[it.k + j0 + tid] = zb[kSolveBW - 1 - tid];
-        return;
-    }
-    const double xv = r < bw ? zb[kSolveBW - 1 - r] : 0.0;
-    double acc[32];
+// acc[k], k < 16, summed over the 64 lanes of the wavefront: afterwards
+// acc[0] of lane l holds the sum of acc[l & 15] (transpose reduction within
+// each 16-lane group, then the groups added).
+__device__ __forceinline__ void wave_transpose_sum16(double (&acc)[16], int lane) {
 #pragma unroll
-    for (int k = 0; k < 32; ++k) acc[k] = to_d(lv[k]) * xv;
-    const double sum = column_sums64(acc, red, tid);
-    if (tid < kSolveCols) Y[it.k + it.a + tid] = yold - sum;
+    for (int s = 8; s >= 1; s >>= 1) {
+        const bool upper = (lane & s) != 0;
+#pragma unroll
+        for (int k = 0; k < s; ++k) {
+            double lo = acc[k], hi = acc[k + s];
+            asm volatile("" : "+v"(lo), "+v"(hi));
+            const double send = upper ? lo : hi;
+            const double keep = upper ? hi : lo;
+            acc[k] = keep + __shfl_xor(send, s, 64);
+        }
+    }
+    acc[0] += __shfl_xor(acc[0], 16, 64);
+    acc[0] += __shfl_xor(acc[0], 32, 64);
+}
+
+// Backward over the large fronts of one level, a chain per front from its
+// last kChainBW-column block (see SolveStep; the update rows were applied
+// by BWD_UPD / BWD_UPD_SUM).  Item = pivot columns [a, a + 128): thread
+// (lane, wave) holds rows 128 j + lane and 128 j + 64 + lane of each later
+// block j, columns a + 16 wave + e, and adds their products with x_j
+// (waited for by the first 128 threads, in LDS) to 16 column sums, reduced
+// over the wavefront at the end.  Then x_b = X_b^T z_b with thread (r, q)
+// holding X(32 q + e, r), a contiguous run of column r of the inverse,
+// stored in xp (chain_store).
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void bwd_chain_kernel(const SolveItem* __restrict__ items,
+                                                                  int* __restrict__ ctr, const T* __restrict__ L,
+                                                                  const T* __restrict__ X256, double* __restrict__ xp,
+                                                                  const double* __restrict__ Y) {
+    constexpr int BW = kChainBW;
+    __shared__ double xl[2][BW];
+    __shared__ double red[4][BW];
+    __shared__ double zb[BW];
+    __shared__ int sh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const SolveItem it = items[chain_ticket(ctr, &sh)];
+    const int n = it.n, c = it.c, col0 = it.a;
+    const int bw = min(BW, c - col0), nb = (c + BW - 1) / BW, jend = col0 / BW;
+    const T* __restrict__ P = L + it.lofs;
+    const int r = tid & (BW - 1), q = tid >> 7;
+    const double yv = tid < bw ? Y[it.k + col0 + tid] : 0.0;
+    const int cb = col0 + 16 * wave;
+    double acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    // block j's products with x_j (tile tl / th), x_j waited for by the
+    // first 128 threads
+    auto apply = [&](int j, const T (&tl)[16], const T (&th)[16]) {
+        double* xb = xl[j & 1];
+        if (tid < BW) xb[tid] = tid < c - BW * j ? chain_wait(xp + it.k + BW * j + tid) : 0.0;
+        __syncthreads();
+        const double x0 = xb[lane], x1 = xb[64 + lane];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            acc[e] += to_d(tl[e]) * x0;
+            acc[e] += to_d(th[e]) * x1;
+        }
+    };
+    // later blocks j = nb - 1 down to jend + 1 (the rows of the last one
+    // clamped to the front, those of the others are below c <= n)
+    T cl[16], ch[16];
+    if (nb - 1 > jend) chain_tile(cl, ch, P, min(BW * (nb - 1) + lane, n - 1), min(BW * (nb - 1) + 64 + lane, n - 1), cb, c,
+                                     n);
+    for (int j = nb - 1; j - 1 > jend; --j) {
+        T nl[16], nh[16];
+        chain_tile(nl, nh, P, BW * (j - 1) + lane, BW * (j - 1) + 64 + lane, cb, c, n);
+        apply(j, cl, ch);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            cl[e] = nl[e];
+            ch[e] = nh[e];
+        }
+    }
+    // the block inverse goes in flight before the last wait
+    T xm[32];
+    issue_strided<32>(xm, X256 + it.pk + 32 * q + (int64_t)kSolveBW * r, 1);
+    if (nb - 1 > jend) apply(jend + 1, cl, ch);
+    wave_transpose_sum16(acc, lane);
+    if (lane < 16) red[0][16 * wave + lane] = acc[0];
+    __syncthreads();
+    if (tid < BW) zb[tid] = tid < bw ? yv - red[0][tid] : 0.0;
+    __syncthreads();
+    double p = 0.0;
+#pragma unroll
+    for (int e = 0; e < 32; ++e) p += to_d(xm[e]) * zb[32 * q + e];
+    red[q][r] = p;
+    __syncthreads();
+    if (tid < bw) chain_store(xp + it.k + col0 + tid, (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]));
 }
 
 // -------------------------------------------------------------- refinement
@@ -1584,25 +1675,34 @@ __global__ void gather_kernel(const double* __restrict__ vals, const int* __rest
     if (t < nnz) dst[t] = vals[src[t]];
 }
 
-// r = b - A x, one row per thread, FP64 (cs_gaxpy, cs.cpp:1069-1083, on the
-// full symmetric matrix)
-__global__ void residual_kernel(const int* __restrict__ rp, const int* __restrict__ ci,
-                                const double* __restrict__ av, const double* __restrict__ x,
-                                const double* __restrict__ b, double* __restrict__ r, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// r = b - A x, FP64 (cs_gaxpy, cs.cpp:1069-1083, on the full symmetric
+// matrix), 16 lanes per row: the row's entries are read coalesced and the
+// lanes' partial sums added by a butterfly in a fixed order
+constexpr int kRowLanes = 16;
+__global__ __launch_bounds__(kThreads) void residual_kernel(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                            const double* __restrict__ av,
+                                                            const double* __restrict__ x,
+                                                            const double* __restrict__ b, double* __restrict__ r,
+                                                            int n) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes, l = threadIdx.x % kRowLanes;
+    if (i >= n) return;   // the whole 16-lane group
     double s = 0.0;
-    for (int p = rp[i]; p < rp[i + 1]; ++p) s += av[p] * x[ci[p]];
-    r[i] = b[i] - s;
+    for (int p = rp[i] + l; p < rp[i + 1]; p += kRowLanes) s += av[p] * x[ci[p]];
+#pragma unroll
+    for (int o = kRowLanes / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o, kRowLanes);
+    if (l == 0) r[i] = b[i] - s;
 }
 
-__global__ void rowabs_kernel(const int* __restrict__ rp, const double* __restrict__ av,
-                              double* __restrict__ out, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kThreads) void rowabs_kernel(const int* __restrict__ rp,
+                                                          const double* __restrict__ av,
+                                                          double* __restrict__ out, int n) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes, l = threadIdx.x % kRowLanes;
     if (i >= n) return;
     double s = 0.0;
-    for (int p = rp[i]; p < rp[i + 1]; ++p) s += fabs(av[p]);
-    out[i] = s;
+    for (int p = rp[i] + l; p < rp[i + 1]; p += kRowLanes) s += fabs(av[p]);
+#pragma unroll
+    for (int o = kRowLanes / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o, kRowLanes);
+    if (l == 0) out[i] = s;
 }
 
 // per-segment max |v| (seg[i] = segment of entry i, segments contiguous):
@@ -1748,9 +1848,9 @@ void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, cons
     if (count > 0) fwd_gather_kernel<<<count, kGatherThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
 }
 template <typename T>
-void launch_fwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
-                    double* xp, double* Y, double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, L, Xpk, xp, Y, su);
+void launch_fwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, const double* xp,
+                      double* Y, double* su, hipStream_t s) {
+    if (count > 0) fwd_chain_kernel<T><<<count, kSolveThreads, 0, s>>>(items, ctr, L, X256, xp, Y, su);
 }
 template <typename T>
 void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
@@ -1762,14 +1862,14 @@ void launch_bwd_upd_sum(const SolveItem* items, int count, const FrontDesc* fron
     if (count > 0) bwd_upd_sum_kernel<<<count, kSolveCols, 0, s>>>(items, partial, Y);
 }
 template <typename T>
-void launch_bwd_blk(const SolveItem* items, int count, const FrontDesc* fronts, const T* L, const T* Xpk,
-                    double* xp, double* Y, hipStream_t s) {
-    if (count > 0) bwd_blk_kernel<T><<<count, kSolveThreads, 0, s>>>(items, L, Xpk, xp, Y);
+void launch_bwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, double* xp,
+                      const double* Y, hipStream_t s) {
+    if (count > 0) bwd_chain_kernel<T><<<count, kSolveThreads, 0, s>>>(items, ctr, L, X256, xp, Y);
 }
 template <typename T>
 void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,
-                   T* Xpk, hipStream_t s) {
-    if (count > 0) inv256_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv, X256, Xpk);
+                   hipStream_t s) {
+    if (count > 0) inv256_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv, X256);
 }
 
 void launch_gather(const double* src_vals, const int* src, double* dst, int nnz, hipStream_t s) {
@@ -1777,10 +1877,10 @@ void launch_gather(const double* src_vals, const int* src, double* dst, int nnz,
 }
 void launch_residual(const int* rp, const int* ci, const double* av, const double* x, const double* b, double* r,
                      int n, hipStream_t s) {
-    residual_kernel<<<blocks_for(n), kThreads, 0, s>>>(rp, ci, av, x, b, r, n);
+    residual_kernel<<<blocks_for((int64_t)n * kRowLanes), kThreads, 0, s>>>(rp, ci, av, x, b, r, n);
 }
 void launch_rowabs(const int* rp, const double* av, double* out, int n, hipStream_t s) {
-    rowabs_kernel<<<blocks_for(n), kThreads, 0, s>>>(rp, av, out, n);
+    rowabs_kernel<<<blocks_for((int64_t)n * kRowLanes), kThreads, 0, s>>>(rp, av, out, n);
 }
 void launch_maxabs(const double* v, int n, unsigned long long* out, hipStream_t s) {
     int blocks = blocks_for(n);
@@ -1809,19 +1909,19 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
-    template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, T*, hipStream_t); \
+    template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmJob*, int, int, int, bool, T*, T*, hipStream_t);                         \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \
     template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, const T*, double*,    \
                                 const double*, hipStream_t);                                                   \
     template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);               \
-    template void launch_fwd_blk<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
-                                    double*, double*, const int64_t*, hipStream_t);                          \
+    template void launch_fwd_chain<T>(const SolveItem*, int, int*, const T*, const T*, const double*,        \
+                                      double*, double*, hipStream_t);                                        \
     template void launch_bwd_upd<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,           \
                                     const double*, double*, hipStream_t);                                    \
-    template void launch_bwd_blk<T>(const SolveItem*, int, const FrontDesc*, const T*, const T*, double*,   \
-                                    double*, hipStream_t);
+    template void launch_bwd_chain<T>(const SolveItem*, int, int*, const T*, const T*, double*,              \
+                                      const double*, hipStream_t);
 
 MDGPU_INSTANTIATE(double)
 MDGPU_INSTANTIATE(float)

@@ -52,7 +52,7 @@ struct mdgpu_numeric {
     int* dBlkCnt = nullptr;          // arrival counters of the block-column jobs
     mdgpu::InvJob* dInvJobs = nullptr;
     void* dInv256 = nullptr;              // kSolveBW-column block inverses (solves)
-    void* dInvPk = nullptr;               // their packed forms (Plan::invpk)
+    int* dChainCtr = nullptr;             // ticket counters of the chained solves
     double* dUpdPartial = nullptr;        // BWD_UPD partial sums
     mdgpu::InvJob* dInv256Jobs = nullptr;
     int* dPanel = nullptr;
@@ -250,8 +250,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
                 break;
             case mdgpu::STEP_INV256:
-                mdgpu::launch_inv256<T>(N->dInv256Jobs + st.first, st.count, N->dFronts, L, inv, (T*)N->dInv256,
-                                        (T*)N->dInvPk, s);
+                mdgpu::launch_inv256<T>(N->dInv256Jobs + st.first, st.count, N->dFronts, L, inv, (T*)N->dInv256, s);
                 break;
         }
         if (!ev && st.rec_ev >= 0) hipEventRecord(N->events[st.rec_ev], s);
@@ -268,6 +267,9 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
     const int n = N->sym->S.n;
     const T* L = (const T*)N->dL;
     const T* inv = (const T*)N->dInv;
+    const T* X256 = (const T*)N->dInv256;
+    mdgpu::launch_fill<int>(N->dChainCtr, P.sv_nctr, 0, s);
+    mdgpu::launch_fill<unsigned long long>((unsigned long long*)N->dY, n, mdgpu::kChainEmptyBits, s);
     mdgpu::launch_permute_in(b, N->dPerm, N->dXp, n, s);
     for (const mdgpu::SolveStep& st : P.sv_fwd) {
         const int* list = N->dSvFronts + st.first;
@@ -281,12 +283,14 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
                 mdgpu::launch_fwd_gather(list, st.count, N->dFronts, N->dKids, N->dRel, N->dXp, N->dSu, N->dSuofs,
                                          s);
                 break;
-            case mdgpu::SV_FWD_BLK:
-                mdgpu::launch_fwd_blk<T>(items, st.count, N->dFronts, L, (const T*)N->dInvPk, N->dXp, N->dY, N->dSu,
-                                         N->dSuofs, s);
+            case mdgpu::SV_FWD_CHAIN:
+                mdgpu::launch_fwd_chain<T>(items, st.count, N->dChainCtr + st.ctr, L, X256, N->dXp, N->dY, N->dSu, s);
                 break;
         }
     }
+    // every entry of xp is stored again by the backward sweep before it is
+    // read; the chained backward waits for kChainEmpty to go
+    mdgpu::launch_fill<unsigned long long>((unsigned long long*)N->dXp, n, mdgpu::kChainEmptyBits, s);
     for (const mdgpu::SolveStep& st : P.sv_bwd) {
         const int* list = N->dSvFronts + st.first;
         const mdgpu::SolveItem* items = N->dSvItems + st.first;
@@ -300,8 +304,8 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
             case mdgpu::SV_BWD_UPD_SUM:
                 mdgpu::launch_bwd_upd_sum(items, st.count, N->dFronts, N->dUpdPartial, N->dY, s);
                 break;
-            case mdgpu::SV_BWD_BLK:
-                mdgpu::launch_bwd_blk<T>(items, st.count, N->dFronts, L, (const T*)N->dInvPk, N->dXp, N->dY, s);
+            case mdgpu::SV_BWD_CHAIN:
+                mdgpu::launch_bwd_chain<T>(items, st.count, N->dChainCtr + st.ctr, L, X256, N->dXp, N->dY, s);
                 break;
         }
     }
@@ -408,7 +412,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, P.dinv * es, &N->dInv)) ||
         (st = upload(N, P.inv, (void**)&N->dInvJobs)) ||
         (st = alloc(N, P.inv256 * es, &N->dInv256)) ||
-        (st = alloc(N, P.invpk * es, &N->dInvPk)) ||
+        (st = alloc(N, P.sv_nctr * sizeof(int), (void**)&N->dChainCtr)) ||
         (st = alloc(N, P.sv_upd_partial * sizeof(double), (void**)&N->dUpdPartial)) ||
         (st = upload(N, P.inv256_jobs, (void**)&N->dInv256Jobs)) ||
         (st = upload(N, P.fronts, (void**)&N->dFronts)) ||

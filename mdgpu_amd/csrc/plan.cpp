@@ -71,6 +71,14 @@ int pos_in(const int* q, int n, int row) {
 
 }  // namespace
 
+// Offset in Plan::inv256 of the inverse of the kChainBW-column block of
+// front d starting at pivot column a: a diagonal 128 x 128 block of the
+// 256-column inverse holding it (leading dimension kSolveBW).
+static int64_t chain_inverse(const FrontDesc& d, int a) {
+    return d.dinv256 + (int64_t)(a / kSolveBW) * kSolveBW * kSolveBW +
+           (int64_t)((a % kSolveBW) / kChainBW) * kChainBW * (kSolveBW + 1);
+}
+
 std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
     P = Plan();
     const int nf = (int)S.nodes.size();
@@ -122,7 +130,6 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             off256 += (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW;
         }
         P.inv256 = std::max<int64_t>(off256, 1);
-        P.invpk = std::max<int64_t>(off256 / ((int64_t)kSolveBW * kSolveBW) * 2 * kPk, 1);
     }
 
     // relative maps child update rows -> parent rows (VectorMergeIndices)
@@ -188,7 +195,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
 
     // solve schedule (see SolveStep)
     {
-        const int BW = kSolveBW;
+        int nctr = 0;
         for (int L = 0; L < P.nlevels; L++) {
             std::vector<int> small, big;
             for (int f : P.level_fronts[L]) (P.fronts[f].c > kSolveSmall ? big : small).push_back(f);
@@ -201,22 +208,19 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             if (big.empty()) continue;
             P.sv_fwd.push_back({SV_FWD_GATHER, (int)P.sv_fronts.size(), (int)big.size(), 0});
             P.sv_fronts.insert(P.sv_fronts.end(), big.begin(), big.end());
-            int maxc = 0;
-            for (int f : big) maxc = std::max(maxc, P.fronts[f].c);
-            for (int j0 = 0; j0 < maxc; j0 += BW) {
-                SolveStep st{SV_FWD_BLK, (int)P.sv_items.size(), 0, 0};
-                for (int f : big) {
-                    const FrontDesc& d = P.fronts[f];
-                    if (j0 >= d.c) continue;
-                    const int j1 = std::min(j0 + BW, d.c);
-                    P.sv_items.push_back({f, j1, 1, j0});
-                    const int fr = solve_fwd_rows(o.elem_bytes);
-                    for (int a = j1 + fr; a < d.n; a += fr) P.sv_items.push_back({f, a, 0, j0});
-                    st.bytes += (double)(d.n - j0) * (j1 - j0) * o.elem_bytes + 16.0 * (d.n - j0);
-                }
-                st.count = (int)P.sv_items.size() - st.first;
-                P.sv_fwd.push_back(st);
+            SolveStep st{SV_FWD_CHAIN, (int)P.sv_items.size(), 0, 0};
+            st.ctr = nctr++;
+            for (int f : big) {
+                const FrontDesc& d = P.fronts[f];
+                for (int a = 0; a < d.c; a += kChainBW) P.sv_items.push_back({f, a, 1});
+                for (int a = d.c; a < d.n; a += kChainBW) P.sv_items.push_back({f, a, 0});
+                // strictly lower panel below the diagonal blocks, the block
+                // inverses (128 x 128 each), the front vector in and out
+                st.bytes += ((double)d.n * d.c - (double)d.c * (d.c + 1) / 2 + (double)kChainBW * d.c) * o.elem_bytes +
+                            16.0 * d.n;
             }
+            st.count = (int)P.sv_items.size() - st.first;
+            P.sv_fwd.push_back(st);
         }
         for (int L = P.nlevels - 1; L >= 0; L--) {
             std::vector<int> small, big;
@@ -251,24 +255,18 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     P.sv_upd_partial = std::max<int64_t>(P.sv_upd_partial, (int64_t)slot * kSolveCols);
                 }
             }
-            int maxc = 0;
-            for (int f : big) maxc = std::max(maxc, P.fronts[f].c);
-            const int nblk = (maxc + BW - 1) / BW;
-            for (int blk = nblk - 1; blk >= 0; blk--) {
-                const int j0 = blk * BW;
-                SolveStep st{SV_BWD_BLK, (int)P.sv_items.size(), 0, 0};
-                for (int f : big) {
-                    const FrontDesc& d = P.fronts[f];
-                    if (j0 >= d.c) continue;
-                    const int j1 = std::min(j0 + BW, d.c);
-                    P.sv_items.push_back({f, j0, 1, j0});
-                    for (int c0 = 0; c0 < j0; c0 += kSolveCols) P.sv_items.push_back({f, c0, 0, j0});
-                    st.bytes += (double)(j1 - j0) * j1 * o.elem_bytes + 16.0 * j1;
-                }
-                st.count = (int)P.sv_items.size() - st.first;
-                P.sv_bwd.push_back(st);
+            SolveStep st{SV_BWD_CHAIN, (int)P.sv_items.size(), 0, 0};
+            st.ctr = nctr++;
+            for (int f : big) {
+                const FrontDesc& d = P.fronts[f];
+                for (int a = (d.c - 1) / kChainBW * kChainBW; a >= 0; a -= kChainBW) P.sv_items.push_back({f, a, 1});
+                // triangle below the diagonal blocks and the block inverses
+                st.bytes += ((double)d.c * (d.c + 1) / 2 + (double)kChainBW * d.c) * o.elem_bytes + 24.0 * d.c;
             }
+            st.count = (int)P.sv_items.size() - st.first;
+            P.sv_bwd.push_back(st);
         }
+        P.sv_nctr = std::max(nctr, 1);
     }
 
     // front fields copied into the solve items
@@ -284,9 +282,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 it.qofs = d.qofs;
                 it.lofs = d.lofs;
                 it.su = P.solve_uofs[it.front];
-                it.pk = (st.kind == SV_FWD_BLK || st.kind == SV_BWD_BLK)
-                            ? (d.dinv256 / ((int64_t)kSolveBW * kSolveBW) + it.j0 / kSolveBW) * 2 * kPk
-                            : 0;
+                it.pk = ((st.kind == SV_FWD_CHAIN || st.kind == SV_BWD_CHAIN) && it.first) ? chain_inverse(d, it.a) : 0;
             }
         }
 
@@ -735,16 +731,35 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                         return "solve update slots";
                     continue;
                 }
-                if (it.j0 < 0 || it.j0 >= d.c || it.j0 % kSolveBW) return "solve item block";
-                const int j1 = std::min(it.j0 + kSolveBW, d.c);
-                if (it.pk != (d.dinv256 / ((int64_t)kSolveBW * kSolveBW) + it.j0 / kSolveBW) * 2 * kPk ||
-                    it.pk < 0 || it.pk + 2 * kPk > P.invpk)
-                    return "solve item packed inverse";
-                if (st.kind == SV_FWD_BLK) {
-                    const int fr = solve_fwd_rows(o.elem_bytes);
-                    if (it.first ? it.a != j1 : (it.a < j1 + fr || it.a >= d.n || (it.a - j1) % fr)) return "solve item rows";
-                } else if (it.first ? it.a != it.j0 : (it.a < 0 || it.a >= it.j0 || it.a % kSolveCols)) {
-                    return "solve item columns";
+                // chain items: TRI blocks of pivots in order (FWD) or reverse
+                // order (BWD), then (FWD) RECT chunks of update rows; every
+                // item of a front in the same launch
+                if (it.first) {
+                    if (it.a < 0 || it.a >= d.c || it.a % kChainBW) return "solve chain block";
+                    if (it.pk != chain_inverse(d, it.a) || it.pk < 0 ||
+                        it.pk + (int64_t)(kChainBW - 1) * (kSolveBW + 1) >= P.inv256)
+                        return "solve chain inverse";
+                } else if (st.kind != SV_FWD_CHAIN || it.a < d.c || it.a >= d.n || (it.a - d.c) % kChainBW) {
+                    return "solve chain rows";
+                }
+                if (st.ctr < 0 || st.ctr >= P.sv_nctr) return "solve chain counter";
+                // the blocks an item waits for come earlier in the launch:
+                // FWD, blocks 0.. of the front at [t0, t0 + nwait); BWD, the
+                // blocks after it, last one first, at [t - nwait, t)
+                const int nb = (d.c + kChainBW - 1) / kChainBW;
+                int t0, nwait;
+                if (st.kind == SV_FWD_CHAIN) {
+                    nwait = it.first ? it.a / kChainBW : nb;
+                    t0 = t - (it.first ? nwait : nb + (it.a - d.c) / kChainBW);
+                } else {
+                    nwait = nb - 1 - it.a / kChainBW;
+                    t0 = t - nwait;
+                }
+                if (t0 < st.first) return "solve chain order";
+                for (int u = t0; u < t0 + nwait; u++) {
+                    const SolveItem& e = P.sv_items[u];
+                    const int want = st.kind == SV_FWD_CHAIN ? (u - t0) * kChainBW : it.a + (t - u) * kChainBW;
+                    if (e.front != it.front || !e.first || e.a != want) return "solve chain order";
                 }
             }
         }
@@ -769,10 +784,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         if (d.dinv256 >= 0 &&
             d.dinv256 + (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW > P.inv256)
             return bad("front block-inverse store");
-        if (d.dinv256 >= 0 && (d.dinv256 % ((int64_t)kSolveBW * kSolveBW) ||
-                               (d.dinv256 / ((int64_t)kSolveBW * kSolveBW) + (d.c + kSolveBW - 1) / kSolveBW) * 2 *
-                                       kPk > P.invpk))
-            return bad("front packed block-inverse store");
+        if (d.dinv256 >= 0 && d.dinv256 % ((int64_t)kSolveBW * kSolveBW)) return bad("front block-inverse alignment");
     }
     for (const InvJob& ij : P.inv256_jobs) {
         if (ij.front < 0 || ij.front >= nf || P.fronts[ij.front].dinv256 < 0) return bad("block-inverse job front");

@@ -116,47 +116,40 @@ struct Step {
 };
 
 // Solve schedule.  Fronts with at most kSolveSmall pivots are solved by one
-// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts go kSolveBW pivot
-// columns (a block) per launch, kSolveThreads threads per workgroup, with
-// the block's inverse in packed form (see kPk):
-//  * FWD_BLK: every workgroup solves the block (y = X v) and updates its
-//    solve_fwd_rows() rows below it; the `first` one stores y in Y.
-//  * BWD_UPD (once per level, before the blocks): each workgroup takes
-//    kSolveCols pivot columns of a front and kSolveUpdRows of its update
-//    rows (solved by the ancestors, read through the row list q) and writes
-//    the dot products to its partial-sum slot; BWD_UPD_SUM then subtracts
-//    the slots of each column chunk from Y, in slot order.
-//  * BWD_BLK (blocks from the last to the first): every workgroup solves
-//    the block (x = X^T y), the `first` one stores x; each other one owns
-//    kSolveCols pivot columns left of the block and subtracts the dot
-//    products of those columns with the block's rows from Y.  Right-looking:
-//    every entry of Y has one owner per launch, so no cross-workgroup
-//    reduction is needed and the summation order is fixed.
-constexpr int kSolveBW = 256;
+// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts are cut into
+// kChainBW-column blocks and solved by one launch per level whose
+// workgroups form a chain inside the launch:
+//  * FWD_CHAIN: one workgroup per block (TRI item: y_b = X_b (v_b -
+//    sum_{j<b} L_bj y_j), X_b the block's inverse) and one per kChainBW
+//    update rows below the pivots (RECT item: u -= sum_j L_rj y_j).  A
+//    workgroup reads y_j from Y as soon as it is there: Y holds the
+//    kChainEmpty pattern until the TRI item of block j stores y_j.
+//  * BWD_UPD (before the chain): each workgroup takes kSolveCols pivot
+//    columns of a front and kSolveUpdRows of its update rows (solved by the
+//    ancestors, read through the row list q) and writes the dot products to
+//    its partial-sum slot; BWD_UPD_SUM then subtracts the slots of each
+//    column chunk from Y, in slot order.
+//  * BWD_CHAIN: one workgroup per block, last block first (x_b = X_b^T
+//    (z_b - sum_{j>b} L_jb^T x_j)), reading x_j from xp as it arrives (xp
+//    is reset to kChainEmpty between the two sweeps).
+// Workgroups take their items in launch order from a ticket counter (one
+// per chain launch, zeroed at the start of every solve), so an item only
+// waits for items of earlier tickets, which are running or done: the chain
+// cannot deadlock however many workgroups are resident.
 constexpr int kSolveSmall = 64;
 constexpr int kSolveThreads = 512;
-// rows per forward workgroup below a block: 128 when the factor is FP32,
-// 64 when FP64 (register budget of the row loads)
-constexpr int solve_fwd_rows(int elem_bytes) { return elem_bytes == 4 ? 128 : 64; }
-constexpr int kSolveCols = 64;      // pivot columns per backward workgroup
+constexpr int kChainBW = 128;
+constexpr int kSolveBW = 256;       // column width of the block inverses (Plan::inv256)
+constexpr int kSolveCols = 64;      // pivot columns per BWD_UPD workgroup
 constexpr int kSolveUpdRows = 1024; // update rows per BWD_UPD workgroup
-// Packed lower-triangular kSolveBW x kSolveBW matrix (a block inverse X for
-// the forward solve, R X^T R with R the order reversal for the backward):
-// row pairs (p, 255 - p), p < 128, of 257 entries each, entry e being
-// (p, e) for e <= p and (255 - p, e - p - 1) otherwise; thread t = 4p + s
-// holds entries [pk_first(s), pk_first(s + 1)) of pair p, entry
-// pk_first(s) + k at [k * kSolveThreads + t] (zero where unused).
-constexpr int kPkSlots = 65;
-constexpr int kPk = kPkSlots * kSolveThreads;
-inline constexpr int pk_first(int s) { return s == 0 ? 0 : 1 + 64 * s; }
-enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_BLK, SV_BWD_SMALL, SV_BWD_UPD, SV_BWD_UPD_SUM, SV_BWD_BLK };
+enum SolveKind { SV_FWD_SMALL = 0, SV_FWD_GATHER, SV_FWD_CHAIN, SV_BWD_SMALL, SV_BWD_UPD, SV_BWD_UPD_SUM, SV_BWD_CHAIN };
 
 struct SolveItem {
     int front;
-    int a;        // FWD_BLK: first row (front-local) of the solve_fwd_rows() chunk;
-                  // BWD_*: first pivot column of the kSolveCols-column chunk
-    int first;    // *_BLK: 1 if this item stores the block's solution
-    int j0;       // *_BLK: first pivot column of the block; BWD_UPD: first update row
+    int a;        // *_CHAIN: first row (FWD) / pivot column (BWD) of the item;
+                  // BWD_UPD*: first pivot column of the kSolveCols-column chunk
+    int first;    // *_CHAIN: 1 for a block of pivots (TRI), 0 for update rows (RECT)
+    int j0;       // BWD_UPD: first update row
     int slot;     // BWD_UPD: partial-sum slot; BWD_UPD_SUM: first slot of the chunk
     int nslot;    // BWD_UPD_SUM: slots of the chunk
     // copies of the front's fields, so that a workgroup reads one item and
@@ -164,7 +157,7 @@ struct SolveItem {
     int n, c, k;
     int64_t qofs;  // offset of the front's row list
     int64_t lofs;  // panel offset in the L store
-    int64_t pk;    // *_BLK: packed inverse of the block (forward form; backward at + kPk)
+    int64_t pk;    // TRI: offset of the block's inverse in Plan::inv256
     int64_t su;    // offset of the front's update vector (solve_uofs)
 };
 
@@ -172,6 +165,7 @@ struct SolveStep {
     int kind;
     int first, count;   // slice of sv_items (or of sv_fronts for *_SMALL / GATHER)
     double bytes;       // algorithmic bytes moved
+    int ctr = -1;       // *_CHAIN: ticket counter
 };
 
 struct Plan {
@@ -190,6 +184,7 @@ struct Plan {
     std::vector<SolveItem> sv_items;
     std::vector<SolveStep> sv_fwd, sv_bwd;
     int64_t sv_upd_partial = 1;        // doubles of BWD_UPD partial sums (one level at a time)
+    int sv_nctr = 1;                   // ticket counters of the chain launches
     std::vector<EaItem> ea;
     std::vector<EaContrib> ea_contrib;
     std::vector<GemmJob> gemm;
@@ -201,14 +196,12 @@ struct Plan {
     // for i, r < b (b = block width, last block may be narrower).
     int64_t dinv = 0;
     // Inverses of the kSolveBW-column diagonal blocks of fronts solved by the
-    // blocked solve path (c > kSolveSmall): block t of front f at
+    // chained solves (c > kSolveSmall): block t of front f at
     // fronts[f].dinv256 + 65536 t, column-major with leading dimension 256,
     // X(r, i) = inv(L_B)(r, i) for r, i < bw, zero elsewhere.  Formed from
-    // the 64x64 inverses after them (STEP_INV256), which also writes the
-    // packed forms: block t of front f at (fronts[f].dinv256 / 65536 + t) * 2
-    // kPk, X then R X^T R (kPk elements each).
+    // the 64x64 inverses after them (STEP_INV256).  Its two diagonal
+    // 128 x 128 blocks are the inverses of the kChainBW-column blocks.
     int64_t inv256 = 1;
-    int64_t invpk = 1;                 // elements of the packed forms
     std::vector<InvJob> inv256_jobs;   // (front, first pivot column of the block)
     std::vector<Step> steps;           // the factorization schedule
     int nevents = 0;                   // events used by the look-ahead

@@ -140,6 +140,41 @@ def test_pivot_growth_fallback_to_fp64(device):
     assert G.info()["fp64_fallbacks"] == 0
 
 
+# plus a case whose large fronts have chains of up to ten 128-column blocks
+SOLVE_CASES = CASES + [("st27_20_rn100000", lambda: gen.stencil27(20, seed=6), "rn", 100000)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("name,make,mode,mz", SOLVE_CASES, ids=[c[0] for c in SOLVE_CASES])
+def test_solve_matches_exported_factor(device, name, make, mode, mz, prec):
+    """The solve (small fronts one workgroup each, large fronts by the chained
+    128-column block launches) against triangular solves in FP64 on the CPU
+    with the factor the GPU computed.  FP64: same L, only the summation order
+    differs (1e-10).  FP32: the solves apply the diagonal-block inverses
+    rounded once to FP32, a perturbation of relative size ~1e-7 (bound 1e-5;
+    a wrong index or a missed block gives O(1))."""
+    import torch
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve_triangular
+    from mdgpu_amd import _lib as L
+    p, i, x = make()
+    S = symbolic.analyze(p, i, True, mode, mz)
+    F = Factorization(S, prec).factorize(torch.from_numpy(x).to(device))
+    Lp, Li, Lx = F.export_L()
+    Lc = sp.csc_matrix((Lx, Li, Lp), shape=(S.n, S.n)).tocsr()
+    perm = S.array(L.ARR_PERM)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(S.n)
+    y = spsolve_triangular(Lc, b[perm], lower=True)
+    z = spsolve_triangular(Lc.T.tocsr(), y, lower=False)
+    ref = np.empty(S.n)
+    ref[perm] = z
+    for rep in range(2):                     # the chain's counters and markers are reset per solve
+        xs = F.solve(torch.from_numpy(b).to(device)).cpu().numpy()
+        assert np.all(np.isfinite(xs)), (name, prec, rep)
+        assert rel_err(xs, ref) <= (1e-10 if prec == "fp64" else 1e-5), (name, prec, rep, rel_err(xs, ref))
+
+
 @pytest.mark.parametrize("nb_outer", ["64", "128"])
 def test_lookahead_outer_blocks(device, monkeypatch, nb_outer):
     """Narrow outer blocks put many trailing updates on the look-ahead side

@@ -59,6 +59,39 @@ __device__ __forceinline__ void keep_in_flight(T (&v)[N]) {
     for (int k = 0; k < N; ++k) asm volatile("" : "+v"(v[k]));
 }
 
+// ------------------------------------------------------------ chained solves
+// The values of Y (forward) and xp (backward) that a chain launch waits for
+// hold kChainEmpty until the workgroup of their block stores them: a
+// signalling NaN, which arithmetic never produces (operations on a NaN give
+// a quiet NaN).
+constexpr unsigned long long kChainEmpty = kChainEmptyBits;
+constexpr int kChainSpin = 1 << 20;   // polls of one value before giving up on it
+
+// *p once it no longer holds kChainEmpty, read with agent-scope atomic
+// loads (another workgroup, possibly on another XCD, stores it with
+// chain_store).  After kChainSpin polls the wait ends with a quiet NaN,
+// which then shows in every result that depends on the value.
+__device__ __forceinline__ double chain_wait(const double* p) {
+    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+    unsigned long long v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int s = 0; v == kChainEmpty; ++s) {
+        if (s >= kChainSpin) return __builtin_nan("");
+        __builtin_amdgcn_s_sleep(2);
+        v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __longlong_as_double((long long)v);
+}
+__device__ __forceinline__ void chain_store(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The workgroup's item index: a ticket from the launch's counter, so every
+// item it waits for belongs to a workgroup that has already started.
+__device__ __forceinline__ int chain_ticket(int* ctr, int* sh) {
+    if (threadIdx.x == 0) *sh = atomicAdd(ctr, 1);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*sh);   // uniform: the item's fields go to scalar registers
+}
 // ---------------------------------------------------------------- fill / A
 template <typename T>
 __global__ void fill_kernel(T* __restrict__ p, int64_t n, T v) {
@@ -1352,39 +1385,6 @@ __global__ __launch_bounds__(kGatherThreads) void fwd_gather_kernel(const int* _
     }
 }
 
-// ------------------------------------------------------------ chained solves
-// The values of Y (forward) and xp (backward) that a chain launch waits for
-// hold kChainEmpty until the workgroup of their block stores them: a
-// signalling NaN, which arithmetic never produces (operations on a NaN give
-// a quiet NaN).
-constexpr unsigned long long kChainEmpty = kChainEmptyBits;
-constexpr int kChainSpin = 1 << 20;   // polls of one value before giving up on it
-
-// *p once it no longer holds kChainEmpty, read with agent-scope atomic
-// loads (another workgroup, possibly on another XCD, stores it with
-// chain_store).  After kChainSpin polls the wait ends with a quiet NaN,
-// which then shows in every result that depends on the value.
-__device__ __forceinline__ double chain_wait(const double* p) {
-    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
-    unsigned long long v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int s = 0; v == kChainEmpty; ++s) {
-        if (s >= kChainSpin) return __builtin_nan("");
-        __builtin_amdgcn_s_sleep(2);
-        v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return __longlong_as_double((long long)v);
-}
-__device__ __forceinline__ void chain_store(double* p, double v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// The workgroup's item index: a ticket from the launch's counter, so every
-// item it waits for belongs to a workgroup that has already started.
-__device__ __forceinline__ int chain_ticket(int* ctr, int* sh) {
-    if (threadIdx.x == 0) *sh = atomicAdd(ctr, 1);
-    __syncthreads();
-    return *sh;
-}
 // Rows r0 and r1 of panel P (n rows, c columns), columns col .. col + 15;
 // columns from c on repeat column c - 1 (in range) and are multiplied by
 // zero by the caller.

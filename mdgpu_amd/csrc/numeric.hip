@@ -71,7 +71,7 @@ struct mdgpu_numeric {
     double* dRowabs = nullptr;
     int64_t* dDiagOff = nullptr;
     int* dInfo = nullptr;
-    unsigned long long* dMax = nullptr;   // one slot per segment
+    unsigned long long* dMax = nullptr;   // three slots per segment (max_abs_n)
     int nseg = 1;
     int* dSeg = nullptr;                  // row -> segment (batches only)
     double* dPartial = nullptr;
@@ -334,20 +334,32 @@ int solve_any(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
     return MDGPU_OK;
 }
 
-// max |v| per segment (synchronizes the stream)
-int max_abs(mdgpu_numeric* N, const double* v, int n, hipStream_t s, std::vector<double>& out) {
-    mdgpu::launch_fill<unsigned long long>(N->dMax, N->nseg, 0ULL, s);
-    if (N->nseg == 1) mdgpu::launch_maxabs(v, n, N->dMax, s);
-    else mdgpu::launch_maxabs_seg(v, N->dSeg, n, N->dMax, s);
-    std::vector<unsigned long long> bits(N->nseg, 0);
+// max |v| per segment of each of the vectors v[0..nv) (nv <= 3), one copy
+// to the host and one synchronization of the stream for all of them
+int max_abs_n(mdgpu_numeric* N, const double* const* v, int nv, int n, hipStream_t s,
+              std::vector<double>* const* out) {
+    const int ns = N->nseg;
+    mdgpu::launch_fill<unsigned long long>(N->dMax, (int64_t)nv * ns, 0ULL, s);
+    for (int t = 0; t < nv; t++) {
+        if (ns == 1) mdgpu::launch_maxabs(v[t], n, N->dMax + t, s);
+        else mdgpu::launch_maxabs_seg(v[t], N->dSeg, n, N->dMax + (int64_t)t * ns, s);
+    }
+    std::vector<unsigned long long> bits((size_t)nv * ns, 0);
     hipError_t e = hipMemcpyAsync(bits.data(), N->dMax, bits.size() * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    out.resize(N->nseg);
-    for (int k = 0; k < N->nseg; k++) memcpy(&out[k], &bits[k], sizeof(double));
+    for (int t = 0; t < nv; t++) {
+        out[t]->resize(ns);
+        for (int k = 0; k < ns; k++) memcpy(&(*out[t])[k], &bits[(size_t)t * ns + k], sizeof(double));
+    }
     return MDGPU_OK;
+}
+// max |v| per segment (synchronizes the stream)
+int max_abs(mdgpu_numeric* N, const double* v, int n, hipStream_t s, std::vector<double>& out) {
+    std::vector<double>* o[1] = {&out};
+    return max_abs_n(N, &v, 1, n, s, o);
 }
 
 }  // namespace
@@ -440,7 +452,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dRowabs)) ||
         (st = upload(N, diag_off, (void**)&N->dDiagOff)) ||
         (st = alloc(N, 2 * sizeof(int), (void**)&N->dInfo)) ||
-        (st = alloc(N, N->nseg * sizeof(unsigned long long), (void**)&N->dMax)) ||
+        (st = alloc(N, 3 * N->nseg * sizeof(unsigned long long), (void**)&N->dMax)) ||
         (N->nseg > 1 && (st = upload(N, seg_of_row, (void**)&N->dSeg))) ||
         (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)) ||
         (st = alloc(N, std::max<size_t>(P.a_dst.size(), 1) * sizeof(double), (void**)&N->dAx)) ||
@@ -627,14 +639,17 @@ extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, 
     const int n = num->sym->S.n;
     TRY(solve_any(num, d_b, d_x, s));
     std::vector<double> bn, rn, xn;
-    TRY(max_abs(num, d_b, n, s, bn));
     double eta = INFINITY;
     int it = 0;
     num->eta_seg.assign(num->nseg, INFINITY);
     for (;; it++) {
         mdgpu::launch_residual(num->dAfp, num->dAfi, num->dAfx, d_x, d_b, num->dR, n, s);
-        TRY(max_abs(num, num->dR, n, s, rn));
-        TRY(max_abs(num, d_x, n, s, xn));
+        {
+            // |b| with the first step's |r| and |x|: one round trip per step
+            const double* vs[3] = {num->dR, d_x, d_b};
+            std::vector<double>* os[3] = {&rn, &xn, &bn};
+            TRY(max_abs_n(num, vs, it == 0 ? 3 : 2, n, s, os));
+        }
         // backward error of the reference's check (mdgpu.cu:1972), per
         // independent matrix of a batch; the worst one decides
         eta = 0;

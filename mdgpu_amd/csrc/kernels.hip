@@ -573,13 +573,16 @@ __global__ __launch_bounds__(kThreads) void inv_kernel(const InvJob* __restrict_
     }
 }
 
-// Inverse X of a kSolveBW-column diagonal block L_B of a front (for the
-// blocked solves), one workgroup per block, from the 64x64 inverses W_s of
-// its diagonal sub-blocks: X_ss = W_s, and below the diagonal, column block
-// by column block, X_ts = -W_t sum_{u=s}^{t-1} L_tu X_us.  Products of
+// Inverses of the two kChainBW-column diagonal blocks inside a
+// kSolveBW-column block L_B of a front (for the chained solves), one
+// workgroup per 256-column block, from the 64x64 inverses W_s of its
+// diagonal sub-blocks: X_ss = W_s, and below the diagonal, X_ts = -W_t
+// sum_{u=s}^{t-1} L_tu X_us for the pairs (s, t) inside one 128-column half
+// (s / 2 == t / 2): the quarter below the halves is not formed.  Products of
 // 64x64 blocks go through LDS 32 columns at a time; thread (i, g) owns
 // column i, rows 16g..16g+15.  X is written column-major with leading
-// dimension 256 and is zero above the diagonal and beyond the block.
+// dimension 256 and is zero above the diagonal, beyond the block and in the
+// quarter below the halves.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restrict__ jobs,
                                                           const FrontDesc* __restrict__ fronts,
@@ -606,7 +609,7 @@ __global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restri
     __syncthreads();
     for (int s = 0; s < nsub; ++s) {
         const int bs = min(64, bw - 64 * s);
-        for (int t = s + 1; t < nsub; ++t) {
+        for (int t = s + 1; t < nsub && t / 2 == s / 2; ++t) {
             const int bt = min(64, bw - 64 * t);
             T acc[16];
 #pragma unroll

@@ -574,8 +574,12 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         if (c256 > 0) {
             Step st{STEP_INV256, b256, c256, c256, P.nlevels - 1, 0};
             for (int t = b256; t < b256 + c256; t++) {
-                const double b = std::min(kSolveBW, P.fronts[P.inv256_jobs[t].front].c - P.inv256_jobs[t].j0);
-                st.flops += b * b * b / 3.0;
+                // the two 128-column halves
+                const int b = std::min(kSolveBW, P.fronts[P.inv256_jobs[t].front].c - P.inv256_jobs[t].j0);
+                for (int h = 0; h < b; h += kChainBW) {
+                    const double bh = std::min(kChainBW, b - h);
+                    st.flops += bh * bh * bh / 3.0;
+                }
                 st.bytes += 1.5 * b * b * o.elem_bytes;
             }
             P.steps.push_back(st);

@@ -397,17 +397,24 @@ __device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T
 // again in FP64 from its assembled values.
 constexpr double kPivotGrowthMin = 1.0 / 4096.0;
 
-// Fused diagonal-block Cholesky + TRSM of one 64-column block step (the
-// potrf + trsm pair of FactorAux_GPU_2, mdgpu.cu:903-1216).  Every
-// workgroup of a job factors the 64x64 diagonal block in LDS (identity
-// beyond the block width b), forms Z = inv(L_D) and multiplies its
-// kBlkRows rows below the block by Z^T on MFMA (X = A Z^T: Z is lower
-// triangular, so block column N of X needs the first N + 1 column blocks
-// of A only).  The rows are loaded at the start, in the operand layout of
-// the products, so their latency hides behind the factorization.  The
-// last workgroup of the job to arrive writes L_D back (counters[jb.cnt]
+// Fused left-looking update + diagonal-block Cholesky + TRSM of one
+// 64-column block step (the potrf + trsm pair of FactorAux_GPU_2,
+// mdgpu.cu:903-1216, with the part of its trailing gemm, :1336, that falls
+// inside the outer block applied here instead of by a separate launch).
+// Workgroups of 256 threads own kBlkRows = 64 rows below the diagonal block
+// (wavefront w: rows 16 w .. 16 w + 15).  Each workgroup
+//  * loads its rows and the diagonal block (64 x 64) of the block's columns;
+//  * subtracts the products with the kprev earlier 64-column blocks of the
+//    outer block (left-looking: C -= L_rows L_diag^T on MFMA, 64 columns of
+//    K at a time, the diagonal block's rows of each K chunk staged in LDS,
+//    the next chunk's loads in flight while the current one is multiplied);
+//  * factors the updated diagonal block in LDS (identity beyond the block
+//    width b), forms Z = inv(L_D) and multiplies its rows by Z^T on MFMA
+//    (X = A Z^T: Z is lower triangular, so block column N of X needs the
+//    first N + 1 column blocks of A only).
+// The last workgroup of the job to arrive writes L_D back (counters[jb.cnt]
 // returns to zero).  float: a pivot-growth check (L(j,j)^2 against the
-// assembled diagonal) refactors the block in FP64 when it fails.
+// updated diagonal) refactors the block in FP64 when it fails.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
                                                        T* __restrict__ L, int* __restrict__ info,
@@ -417,7 +424,9 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
     constexpr int kFb = std::is_same<T, float>::value ? 64 * 65 * 8 : 0;  // FP64 fallback block
     constexpr int kScr = kZ * (int)sizeof(T) > kFb ? kZ * (int)sizeof(T) : kFb;
     __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]
-    __shared__ __align__(16) unsigned char scr[kScr];   // Zs, Ts; or the FP64 fallback block
+    // K chunks of the update, then the rows (for the operand layout), then
+    // Zs, Ts or the FP64 fallback block
+    __shared__ __align__(16) unsigned char scr[kScr];
     __shared__ T rd[64];
     __shared__ T a0[64];
     __shared__ int last;
@@ -427,37 +436,95 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
         if (jobs[mid].tile_base <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
     }
     const BlkJob jb = jobs[lo];
-    const int b = jb.b, ld = jb.ld;
+    const int b = jb.b, ld = jb.ld, K = 64 * jb.kprev;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
     T* __restrict__ g = L + jb.p_off;
-    // diagonal block: 16 elements per thread
-    T vin[16];
+    T* Ws = reinterpret_cast<T*>(scr);
+    // this lane's row below the block (rw < rows) and its diagonal-block row;
+    // rows past the job read in-range duplicates (row b - 1 if rows == 0)
+    const int rw = ((int)blockIdx.x - jb.tile_base) * kBlkRows + 16 * wave + li;
+    const int64_t rrow = b + min(rw, jb.rows - 1);
+    const int drow = min(16 * wave + li, b - 1);
+    // current values in the accumulator layout: R(N, v) = A(row, 16 N + row(lg, v)),
+    // D(N, v) the same for diagonal-block row 16 w + li
+    T Rv[4][4], Dv[4][4];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-        vin[q] = g[min(r, b - 1) + (int64_t)min(c, b - 1) * ld];
+    for (int N = 0; N < 4; ++N)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t col = min(16 * N + MfmaOp<T>::row(lg, v), b - 1);
+            Rv[N][v] = g[rrow + col * ld];
+            Dv[N][v] = g[drow + col * ld];
+        }
+    if (K > 0) {
+        const T* __restrict__ gp = g - (int64_t)K * ld;   // the outer block's earlier columns
+        acc_t aR[4], aD[4];
+#pragma unroll
+        for (int N = 0; N < 4; ++N) {
+            aR[N] = acc_t{T(0), T(0), T(0), T(0)};
+            aD[N] = acc_t{T(0), T(0), T(0), T(0)};
+        }
+        T xr[16], bv[16];
+        auto fetch = [&](int kc) {
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) xr[ks] = gp[rrow + (int64_t)(kc + 4 * ks + lg) * ld];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+                bv[q] = gp[min(r, b - 1) + (int64_t)(kc + c) * ld];
+            }
+        };
+        fetch(0);
+        for (int kc = 0; kc < K; kc += 64) {
+            __syncthreads();   // the previous chunk's readers are done
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
+                Ws[r + 65 * c] = bv[q];
+            }
+            T xc[16];
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) xc[ks] = xr[ks];
+            __syncthreads();
+            if (kc + 64 < K) fetch(kc + 64);
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) {
+                const T xd = Ws[(16 * wave + li) + 65 * (4 * ks + lg)];
+#pragma unroll
+                for (int N = 0; N < 4; ++N) {
+                    const T bf = Ws[(16 * N + li) + 65 * (4 * ks + lg)];
+                    aR[N] = MfmaOp<T>::mma(bf, xc[ks], aR[N]);
+                    aD[N] = MfmaOp<T>::mma(bf, xd, aD[N]);
+                }
+            }
+        }
+#pragma unroll
+        for (int N = 0; N < 4; ++N)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                Rv[N][v] -= aR[N][v];
+                Dv[N][v] -= aD[N][v];
+            }
+        __syncthreads();   // the last chunk's readers are done
     }
-    // this wavefront's 64 rows below the block, xa[M][ks] = A(16 M + li,
-    // 4 ks + lg); rows past the job and columns past b read in-range
-    // duplicates (Z is the identity beyond b, so they never reach a column
-    // < b, and they are never stored)
-    const int rbase = ((int)blockIdx.x - jb.tile_base) * kBlkRows + 64 * wave;
-    T xa[4][16];
+    // the updated diagonal block (identity beyond b) and rows to LDS
 #pragma unroll
-    for (int M = 0; M < 4; ++M) {
-        const T* __restrict__ src = g + b + min(rbase + 16 * M + li, jb.rows - 1);
+    for (int N = 0; N < 4; ++N)
 #pragma unroll
-        for (int ks = 0; ks < 16; ++ks) xa[M][ks] = src[(int64_t)min(4 * ks + lg, b - 1) * ld];
-    }
-    keep_in_flight(vin);
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * wave + li, c = 16 * N + MfmaOp<T>::row(lg, v);
+            As[r + 65 * c] = (r < b && c < b) ? Dv[N][v] : (r == c ? T(1) : T(0));
+            Ws[r + 65 * c] = Rv[N][v];
+        }
+    __syncthreads();
+    // this wavefront's rows in the operand layout, xa[ks] = A(16 w + li, 4 ks + lg)
+    // (columns past b hold finite duplicates; Z is the identity there)
+    T xa[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-        As[r + 65 * c] = (r < b && c < b) ? vin[q] : (r == c ? T(1) : T(0));
-    }
+    for (int ks = 0; ks < 16; ++ks) xa[ks] = Ws[(16 * wave + li) + 65 * (4 * ks + lg)];
     __syncthreads();
     if (tid == 0) last = atomicAdd(&counters[jb.cnt], 1) == jb.ntiles - 1;
-    if (tid < 64) a0[tid] = As[tid * 66];            // assembled diagonal
+    if (tid < 64) a0[tid] = As[tid * 66];            // updated diagonal
     chol64_panels<T>(As, tid);
     bool fell_back = false;
     if constexpr (std::is_same<T, float>::value) {
@@ -468,10 +535,12 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
             fell_back = true;
             double* Ad = reinterpret_cast<double*>(scr);
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int t = tid + kThreads * q, r = t & 63, c = t >> 6;
-                Ad[r + 65 * c] = (r < b && c < b) ? (double)vin[q] : (r == c ? 1.0 : 0.0);
-            }
+            for (int N = 0; N < 4; ++N)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = 16 * wave + li, c = 16 * N + MfmaOp<T>::row(lg, v);
+                    Ad[r + 65 * c] = (r < b && c < b) ? (double)Dv[N][v] : (r == c ? 1.0 : 0.0);
+                }
             __syncthreads();
             chol64_panels<double>(Ad, tid);
             for (int t = tid; t < 64 * 64; t += kThreads) {
@@ -501,35 +570,26 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
         }
     }
     __syncthreads();
-    T* Zs = reinterpret_cast<T*>(scr);
+    T* Zs = Ws;
     inv64_blocked<T>(As, rd, Zs, Zs + 64 * 65, wave, lane);
-    // X = A Z^T: D(p, q) = sum_k Z(16 N + p, k) A(16 M + q, k) = X(16 M + q, 16 N + p)
-    acc_t acc[4][4];
+    // X = A Z^T: D(p, q) = sum_k Z(16 N + p, k) A(row q, k) = X(row q, 16 N + p)
+    acc_t acc[4];
 #pragma unroll
-    for (int M = 0; M < 4; ++M)
-#pragma unroll
-        for (int N = 0; N < 4; ++N) acc[M][N] = acc_t{T(0), T(0), T(0), T(0)};
+    for (int N = 0; N < 4; ++N) acc[N] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int N = 0; N < 4; ++N)
 #pragma unroll
-        for (int ks = 0; ks < 4 * N + 4; ++ks) {
-            const T zb = Zs[(16 * N + li) + 65 * (4 * ks + lg)];
+        for (int ks = 0; ks < 4 * N + 4; ++ks)
+            acc[N] = MfmaOp<T>::mma(Zs[(16 * N + li) + 65 * (4 * ks + lg)], xa[ks], acc[N]);
+    if (rw < jb.rows) {
+        T* __restrict__ dst = g + b + rw;
 #pragma unroll
-            for (int M = 0; M < 4; ++M) acc[M][N] = MfmaOp<T>::mma(zb, xa[M][ks], acc[M][N]);
-        }
+        for (int N = 0; N < 4; ++N)
 #pragma unroll
-    for (int M = 0; M < 4; ++M) {
-        const int row = rbase + 16 * M + li;
-        if (row < jb.rows) {
-            T* __restrict__ dst = g + b + row;
-#pragma unroll
-            for (int N = 0; N < 4; ++N)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int col = 16 * N + MfmaOp<T>::row(lg, v);
-                    if (col < b) dst[(int64_t)col * ld] = acc[M][N][v];
-                }
-        }
+            for (int v = 0; v < 4; ++v) {
+                const int col = 16 * N + MfmaOp<T>::row(lg, v);
+                if (col < b) dst[(int64_t)col * ld] = acc[N][v];
+            }
     }
 }
 

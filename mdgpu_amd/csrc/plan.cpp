@@ -287,11 +287,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
 
     // schedule
-    // GEMM tile size of a launch: o.tile for the deep updates (trailing,
-    // Schur complement); the narrow in-panel updates use 64 when the
-    // MFMA kernels run 128-row tiles (four times the workgroups for the
-    // same work)
-    const int default_narrow = std::min(o.tile, 64);
+    // GEMM tile size of a launch: o.tile unless the autotuner chose
     int gemm_ordinal = 0;   // GEMM launches emitted so far
     // tile of the GEMM launch being assembled (the autotuner's choice if any)
     auto tile_for = [&](int dflt) {
@@ -405,10 +401,11 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 P.steps.push_back(st);
             }
         }
-        // large panels: blocked right-looking in two levels -- nb-column
-        // block columns (BLK + a narrow update of the rest of the outer
-        // block) inside nb_outer-column outer blocks, whose trailing update
-        // is one deep (K = nb_outer) GEMM
+        // large panels: blocked in two levels -- nb-column block columns,
+        // left-looking inside nb_outer-column outer blocks (each BLK first
+        // applies the outer block's earlier block columns to its own), and
+        // right-looking across outer blocks, whose trailing update is one
+        // deep (K = nb_outer) GEMM
         int maxc = 0;
         for (int f : blocked) maxc = std::max(maxc, P.fronts[f].c);
         int side_last = -1, join_ev = -1;
@@ -429,6 +426,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     bj.tile_base = btiles;
                     bj.cnt = (int)P.blk.size();
                     bj.front = f;
+                    bj.kprev = (r0 - R0) / 64;
                     btiles += bj.ntiles;
                     P.blk.push_back(bj);
                 }
@@ -437,35 +435,12 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 {
                     Step st{STEP_BLK, dbegin, dcount, btiles, L, 0};
                     for (int t = dbegin; t < dbegin + dcount; t++) {
-                        const double bb = P.blk[t].b, rows = P.blk[t].rows;
-                        st.flops += bb * bb * bb / 3.0 + rows * bb * bb;
-                        st.bytes += (2.0 * bb * bb / 2 + 2.0 * rows * bb) * o.elem_bytes;
+                        const double bb = P.blk[t].b, rows = P.blk[t].rows, kk = 64.0 * P.blk[t].kprev;
+                        st.flops += bb * bb * bb / 3.0 + rows * bb * bb + 2.0 * (rows + bb) * bb * kk;
+                        st.bytes += (2.0 * bb * bb / 2 + 2.0 * rows * bb + (rows + bb) * kk) * o.elem_bytes;
                     }
                     P.steps.push_back(st);
                 }
-                // update of the outer block's remaining columns [r1, e)
-                std::vector<GemmJob> batch;
-                int tiles = 0;
-                const int narrow_tile = tile_for(default_narrow);
-                for (int t = dbegin; t < dbegin + dcount; t++) {
-                    const BlkJob& dj = P.blk[t];
-                    const FrontDesc& d = P.fronts[dj.front];
-                    const int r1 = r0 + dj.b, e = std::min(R0 + o.nb_outer, d.c);
-                    if (r1 >= e) continue;
-                    GemmJob g{};
-                    g.a_off = d.lofs + (int64_t)r0 * d.n + r1;
-                    g.b_off = g.a_off;
-                    g.c_off = d.lofs + (int64_t)r1 * d.n + r1;
-                    g.lda = g.ldb = g.ldc = d.n;
-                    g.M = d.n - r1;
-                    g.N = e - r1;
-                    g.K = dj.b;
-                    g.c_in_arena = 0;
-                    g.beta0 = 0;
-                    g.lower = 1;
-                    add_gemm(g, batch, tiles, narrow_tile);
-                }
-                flush_gemm(batch, tiles, L, narrow_tile);
             }
             // trailing update of the pivot columns after the outer block, in
             // two parts: (a) the next outer block's columns, on the main
@@ -704,6 +679,16 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         if (bj.p_off < 0 || bj.p_off + (bj.b - 1) + bj.rows + (int64_t)(bj.b - 1) * bj.ld >= P.lstore)
             return bad("block bounds");
         if (bj.b + bj.rows > bj.ld) return bad("block rows exceed panel");
+        if (bj.front < 0 || bj.front >= nf) return bad("block front");
+        {
+            const FrontDesc& d = P.fronts[bj.front];
+            const int r0 = bj.col0 - d.k;
+            if (r0 < 0 || r0 >= d.c || r0 % 64 || bj.ld != d.n || bj.p_off != d.lofs + (int64_t)r0 * d.n + r0)
+                return bad("block position");
+            if (bj.kprev < 0 || 64 * bj.kprev > r0 || (r0 - 64 * bj.kprev) % o.nb_outer ||
+                64 * (bj.kprev + 1) > o.nb_outer)
+                return bad("block left-looking range");
+        }
     }
     auto check_solve = [&](const std::vector<SolveStep>& steps) -> std::string {
         for (const SolveStep& st : steps) {

@@ -79,18 +79,22 @@ struct GemmJob {
     int ntiles;
 };
 
-// Block-column job of the blocked panel path: Cholesky of the b x b
-// diagonal block at p_off and the triangular solve of the `rows` rows
-// below it (L21 <- A21 L_D^{-T}), in ntiles workgroups of kBlkRows rows.
-// Every workgroup factors the diagonal block itself; the last one to have
-// read it (counter `cnt`) writes it back.
+// Block-column job of the blocked panel path, left-looking inside the
+// outer block: the b columns of the block and the rows from its diagonal
+// down first get the update of the kprev 64-column blocks of the same outer
+// block left of it (A -= L_prev L_prev^T), then the b x b diagonal block at
+// p_off is factored (Cholesky) and the `rows` rows below it solved (L21 <-
+// A21 L_D^{-T}), in ntiles workgroups of kBlkRows rows.  Every workgroup
+// updates and factors the diagonal block itself; the last one to have read
+// it (counter `cnt`) writes it back.
 struct BlkJob {
     int64_t p_off;                 // panel offset of the diagonal block
     int ld, b, col0, rows;         // panel ld, block size, global column, rows below
     int tile_base, ntiles;         // first workgroup of this job, workgroups (>= 1)
     int cnt, front;                // arrival counter index, front
+    int kprev;                     // earlier 64-column blocks of the outer block (left-looking update)
 };
-constexpr int kBlkRows = 256;
+constexpr int kBlkRows = 64;
 
 // A launch of the schedule: kind + a slice of the matching job array.
 enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV, STEP_INV256 };

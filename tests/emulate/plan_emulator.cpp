@@ -148,6 +148,15 @@ extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const 
                 const BlkJob& bj = P.blk[t];
                 double* g = &L[bj.p_off];
                 const int b = bj.b, ld = bj.ld;
+                // left-looking: the block's columns, rows from the diagonal
+                // down, minus the outer block's earlier block columns
+                const int K = 64 * bj.kprev;
+                for (int c = 0; c < b; c++)
+                    for (int i = c; i < b + bj.rows; i++) {
+                        double acc = 0;
+                        for (int l = 0; l < K; l++) acc += g[i + (int64_t)(l - K) * ld] * g[c + (int64_t)(l - K) * ld];
+                        g[i + (int64_t)c * ld] -= acc;
+                    }
                 for (int j = 0; j < b; j++) {
                     double rt = std::sqrt(g[j + j * ld]);
                     g[j + j * ld] = rt;

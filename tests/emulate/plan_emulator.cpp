@@ -212,3 +212,72 @@ extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const 
     }
     return 0;
 }
+
+// Plants one mistake in a valid plan and returns what validate_plan says
+// about it (0 when the plan was rejected with a message containing `want`,
+// 1 when it was accepted or rejected for another reason, 2 when the plan
+// has nothing to plant the mistake in).  kinds:
+//  0  two blocks of a front swapped in a forward chain launch
+//  1  a backward chain block moved ahead of the block it waits for
+//  2  a chain launch given a ticket counter past the end
+//  3  a left-looking block job reaching left of its outer block
+//  4  a chain block pointing at another block's inverse
+extern "C" int emulate_validate_mistake(int n, const int* Cp, const int* Ci, int mode, int max_zeros, int kind) {
+    AnalyzeOptions ao;
+    ao.use_amd = 1;
+    ao.partition = (Partition)mode;
+    ao.max_zeros = max_zeros;
+    Symbolic S;
+    if (!analyze(n, Cp, Ci, ao, S).empty()) return 3;
+    PlanOptions po;
+    po.elem_bytes = 4;
+    po.panel_lds_bytes = 61440;
+    Plan P;
+    if (!build_plan(S, po, P).empty() || !validate_plan(P, po).empty()) return 3;
+    const char* want = nullptr;
+    bool done = false;
+    auto chain_pair = [&](std::vector<SolveStep>& steps, int kd) -> int {
+        for (const SolveStep& st : steps) {
+            if (st.kind != kd) continue;
+            for (int t = st.first; t + 1 < st.first + st.count; t++) {
+                const SolveItem &a = P.sv_items[t], &b = P.sv_items[t + 1];
+                if (a.front == b.front && a.first && b.first) return t;
+            }
+        }
+        return -1;
+    };
+    if (kind == 0 || kind == 1) {
+        const int t = chain_pair(kind == 0 ? P.sv_fwd : P.sv_bwd, kind == 0 ? SV_FWD_CHAIN : SV_BWD_CHAIN);
+        if (t < 0) return 2;
+        std::swap(P.sv_items[t], P.sv_items[t + 1]);
+        want = "solve chain order";
+        done = true;
+    } else if (kind == 2) {
+        for (SolveStep& st : P.sv_fwd)
+            if (st.kind == SV_FWD_CHAIN) {
+                st.ctr = P.sv_nctr;
+                done = true;
+                break;
+            }
+        want = "solve chain counter";
+    } else if (kind == 3) {
+        for (BlkJob& bj : P.blk)
+            if (bj.kprev > 0) {
+                bj.kprev += 1;
+                done = true;
+                break;
+            }
+        want = "block left-looking range";
+    } else if (kind == 4) {
+        for (SolveItem& it : P.sv_items)
+            if (it.first && it.c > 2 * kChainBW && it.a == kChainBW) {
+                it.pk += (int64_t)kChainBW * (kSolveBW + 1);   // the next block's inverse
+                done = true;
+                break;
+            }
+        want = "solve chain inverse";
+    }
+    if (!done) return 2;
+    const std::string err = validate_plan(P, po);
+    return err.find(want) != std::string::npos ? 0 : 1;
+}

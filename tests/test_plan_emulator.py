@@ -85,3 +85,16 @@ def test_selinv_schedule_matches_dense_inverse(emulator, name, make, mode, mz):
     cols = np.repeat(np.arange(S.n), np.diff(lp))
     ref = Zd[Li, cols]
     assert rel_err(Zx, ref) < 1e-11
+
+
+@pytest.mark.parametrize("kind,want", [(0, "forward chain order"), (1, "backward chain order"),
+                                       (2, "chain counter"), (3, "left-looking range"),
+                                       (4, "chain block inverse")])
+def test_validator_rejects_planted_mistakes(emulator, kind, want):
+    """A mistake planted in a valid plan (chained-solve order, counters,
+    inverses; left-looking block range) is caught by validate_plan with the
+    matching message, before anything would run on a GPU."""
+    p, i, x = gen.stencil27(16, seed=1)
+    I = C.POINTER(C.c_int)
+    st = emulator.emulate_validate_mistake(len(p) - 1, p.ctypes.data_as(I), i.ctypes.data_as(I), 2, 100000, kind)
+    assert st == 0, (want, st)

@@ -213,6 +213,13 @@ int mdgpu_factorize_steps(mdgpu_numeric* num, const double* d_ax, void* stream, 
  * dead call site mdgpu.cu:1953-1960).  Asynchronous on `stream`. */
 int mdgpu_solve(mdgpu_numeric* num, const double* d_b, double* d_x, void* stream);
 
+/* nrhs right-hand sides at once: column r of B (d_B + r * ld, n entries)
+ * gives column r of X, each as mdgpu_solve (the same launches, so the same
+ * results).  ld >= n.  The solve entry of the interface contract,
+ * mdgpu_solve(num, b, x, nrhs), over cs_lsolve / cs_ltsolve
+ * (cs.cpp:1086-1119) applied per column.  Asynchronous on `stream`. */
+int mdgpu_solve_nrhs(mdgpu_numeric* num, const double* d_B, double* d_X, int nrhs, int64_t ld, void* stream);
+
 /* Iterative refinement in FP64 until the backward error
  * ||b - A x||_inf / (||A||_1 ||x||_inf + ||b||_inf) (mdgpu.cu:1972) is at
  * most tol, or maxit corrections were applied.  The residual uses the

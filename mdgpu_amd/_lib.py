@@ -110,6 +110,7 @@ _SIGNATURES.update({
     "mdgpu_factorize_steps": (C.c_int, [_P, _P, _P, _DP, C.c_int64]),
     "mdgpu_numeric_autotune": (C.c_int, [_P, _P, _P, _I64P]),
     "mdgpu_solve": (C.c_int, [_P, _P, _P, _P]),
+    "mdgpu_solve_nrhs": (C.c_int, [_P, _P, _P, C.c_int, C.c_int64, _P]),
     "mdgpu_refine": (C.c_int, [_P, _P, _P, C.c_double, C.c_int, _DP, C.POINTER(C.c_int), _P]),
     "mdgpu_refine_segment_eta": (C.c_int, [_P, _DP, C.c_int]),
     "mdgpu_logdet": (C.c_int, [_P, _DP, _P]),

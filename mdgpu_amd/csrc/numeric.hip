@@ -630,6 +630,16 @@ extern "C" int mdgpu_solve(mdgpu_numeric* num, const double* d_b, double* d_x, v
     return solve_any(num, d_b, d_x, (hipStream_t)stream);
 }
 
+extern "C" int mdgpu_solve_nrhs(mdgpu_numeric* num, const double* d_B, double* d_X, int nrhs, int64_t ld,
+                                void* stream) {
+    mdgpu::clear_error();
+    if (!num || (nrhs > 0 && (!d_B || !d_X))) return fail(MDGPU_ERR_ARG, "mdgpu_solve_nrhs: null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_solve_nrhs: no valid factorization");
+    if (nrhs < 0 || ld < num->sym->S.n) return fail(MDGPU_ERR_ARG, "mdgpu_solve_nrhs: nrhs < 0 or ld < n");
+    for (int r = 0; r < nrhs; r++) TRY(solve_any(num, d_B + (int64_t)r * ld, d_X + (int64_t)r * ld, (hipStream_t)stream));
+    return MDGPU_OK;
+}
+
 extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol, int maxit,
                             double* eta_out, int* iters_out, void* stream) {
     mdgpu::clear_error();

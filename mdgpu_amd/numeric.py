@@ -94,10 +94,14 @@ class Factorization:
         return out[:6 * nsteps].reshape(nsteps, 6)
 
     def solve(self, b, out=None):
+        """x = A^{-1} b; b of shape (n,) or (nrhs, n) (one right-hand side per row)."""
         torch = _torch()
         b = b.to(device="cuda", dtype=torch.float64).contiguous()
         x = out if out is not None else torch.empty_like(b)
-        L.check(L.lib().mdgpu_solve(self._h, _dptr(b), _dptr(x), _stream()))
+        if b.dim() == 2:
+            L.check(L.lib().mdgpu_solve_nrhs(self._h, _dptr(b), _dptr(x), b.shape[0], b.shape[1], _stream()))
+        else:
+            L.check(L.lib().mdgpu_solve(self._h, _dptr(b), _dptr(x), _stream()))
         return x
 
     def refine(self, b, tol=1e-12, maxit=30, out=None, raise_on_fail=True):

@@ -175,6 +175,25 @@ def test_solve_matches_exported_factor(device, name, make, mode, mz, prec):
         assert rel_err(xs, ref) <= (1e-10 if prec == "fp64" else 1e-5), (name, prec, rep, rel_err(xs, ref))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_solve_nrhs_matches_single_solves(device, prec):
+    """mdgpu_solve_nrhs: each right-hand side of the block gives exactly the
+    single solve's result, and the FP64 block solves to eta <= 1e-14."""
+    import torch
+    p, i, x = gen.stencil27(12, seed=3)
+    S = symbolic.analyze(p, i, True, "rn", 1000)
+    F = Factorization(S, prec).factorize(torch.from_numpy(x).to(device))
+    rng = np.random.default_rng(7)
+    B = torch.from_numpy(rng.standard_normal((3, S.n))).to(device)
+    X = F.solve(B).cpu().numpy()
+    for r in range(3):
+        assert np.array_equal(X[r], F.solve(B[r]).cpu().numpy()), r
+    if prec == "fp64":
+        from oracle import oracle as O
+        for r in range(3):
+            assert O.backward_error(p, i, x, X[r], B[r].cpu().numpy()) <= 1e-14
+
+
 @pytest.mark.parametrize("nb_outer", ["64", "128"])
 def test_lookahead_outer_blocks(device, monkeypatch, nb_outer):
     """Narrow outer blocks put many trailing updates on the look-ahead side

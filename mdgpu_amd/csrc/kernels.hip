@@ -877,7 +877,7 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
     // k-tile depth 16 (register budget of the prefetch, and LDS for four
     // FP32 workgroups per CU); the next k-tile is loaded into registers while
     // the current one is multiplied out of LDS.
-    // BT x BT tiles (128, or 64 for narrow updates); each wavefront owns a
+    // BT x BT tiles (128, or 64 for short launches); each wavefront owns a
     // (BT/2) x (BT/2) quadrant of NB x NB 16x16 MFMA blocks
     constexpr int KT = 16, LDK = KT + 4, NB = BT / 32, KSTEP = kThreads / BT;
     constexpr int PER = (BT * KT) / kThreads;      // staged elements per thread and operand

@@ -446,7 +446,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             // two parts: (a) the next outer block's columns, on the main
             // stream (the next block's factorization needs them), and (b) all
             // later columns, on the side stream, overlapping the next outer
-            // block's BLK/GEMM chain.  (a) of this block waits for (b) of the
+            // block's BLK chain.  (a) of this block waits for (b) of the
             // previous one (both update the same columns); (b) starts when
             // this outer block is factored.
             const int inner_done = P.nevents++;

@@ -223,7 +223,7 @@ struct PlanOptions {
     int tile = 64;               // GEMM tile (rows = cols)
     // per-launch GEMM tile sizes (64 or 128, <= tile) chosen by the
     // autotuner, indexed by the GEMM launch's ordinal in the schedule;
-    // empty: 64 for the narrow in-panel updates, `tile` for the rest
+    // empty: `tile` for every launch
     std::vector<int> gemm_tiles;
     int ea_chunk = 16384;        // target entries per extend-add workgroup
 };

@@ -298,9 +298,10 @@ __device__ __forceinline__ void chol64_panels(U* As, int tid) {
         const int c0 = 16 * p;
         if (wave == 0) {
             const int r = c0 + lane;
+            // lanes past row 63 work on copies of row 63 and store nothing
             U a[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = r < 64 ? As[r + 65 * (c0 + j)] : U(0);
+            for (int j = 0; j < 16; ++j) a[j] = As[min(r, 63) + 65 * (c0 + j)];
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const U djj = bcast(a[j], j);
@@ -315,10 +316,11 @@ __device__ __forceinline__ void chol64_panels(U* As, int tid) {
 #pragma unroll
                 for (int k = j + 1; k < 16; ++k) a[k] -= a[j] * l[k];
             }
+            // above the diagonal the panel holds by-products: the strict
+            // upper triangle of As is never read after this
             if (r < 64) {
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (lane >= j) As[r + 65 * (c0 + j)] = a[j];
+                for (int j = 0; j < 16; ++j) As[r + 65 * (c0 + j)] = a[j];
             }
         } else if (p > 0) {
             // look-ahead: the previous panel's update of the columns right
@@ -333,17 +335,12 @@ __device__ __forceinline__ void chol64_panels(U* As, int tid) {
     }
 }
 
-// Z = inv(L) of the 64x64 lower-triangular factor in As (As[r + 65 c],
-// upper part ignored), rd[i] = 1 / L(i, i), into Zs (same layout, zero
-// above the diagonal of the diagonal blocks) by 16x16 blocks: wavefront w
-// inverts diagonal block w (lane j < 16: column j by forward substitution),
-// then, one block diagonal after the other, Z_IJ = -Z_II sum_{K=J}^{I-1}
-// L_IK Z_KJ on MFMA, one block per wavefront, the sum staged in Ts (16 x 17
-// per wavefront).  Called by all threads; ends with a barrier.
+// Inverses of the four 16x16 diagonal blocks of the 64x64 lower-triangular
+// factor in As (As[r + 65 c]), rd[i] = 1 / L(i, i): wavefront w, lane j < 16,
+// forms column j of inv(L_ww) by forward substitution into Zs (same layout,
+// zero above the diagonal of the block).  No barrier.
 template <typename T>
-__device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T* Ts, int wave, int lane) {
-    typedef typename MfmaOp<T>::acc_t acc_t;
-    const int li = lane & 15, lg = lane >> 4;
+__device__ __forceinline__ void inv16_diag(const T* As, const T* rd, T* Zs, int wave, int lane) {
     if (lane < 16) {
         const int b0 = 16 * wave, j = lane;
         T z[16];
@@ -357,6 +354,20 @@ __device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T
 #pragma unroll
         for (int i = 0; i < 16; ++i) Zs[(b0 + i) + 65 * (b0 + j)] = i >= j ? z[i] : T(0);
     }
+}
+
+// Z = inv(L) of the 64x64 lower-triangular factor in As (As[r + 65 c],
+// upper part ignored), rd[i] = 1 / L(i, i), into Zs (same layout, zero
+// above the diagonal of the diagonal blocks) by 16x16 blocks: wavefront w
+// inverts diagonal block w (lane j < 16: column j by forward substitution),
+// then, one block diagonal after the other, Z_IJ = -Z_II sum_{K=J}^{I-1}
+// L_IK Z_KJ on MFMA, one block per wavefront, the sum staged in Ts (16 x 17
+// per wavefront).  Called by all threads; ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T* Ts, int wave, int lane) {
+    typedef typename MfmaOp<T>::acc_t acc_t;
+    const int li = lane & 15, lg = lane >> 4;
+    inv16_diag<T>(As, rd, Zs, wave, lane);
     __syncthreads();
     T* __restrict__ tw = Ts + wave * (16 * 17);
     for (int d = 1; d < 4; ++d) {
@@ -409,11 +420,10 @@ constexpr double kPivotGrowthMin = 1.0 / 4096.0;
 //    K at a time, the diagonal block's rows of each K chunk staged in LDS,
 //    the next chunk's loads in flight while the current one is multiplied);
 //  * factors the updated diagonal block in LDS (identity beyond the block
-//    width b), forms Z = inv(L_D) and multiplies its rows by Z^T on MFMA
-//    (X = A Z^T: Z is lower triangular, so block column N of X needs the
-//    first N + 1 column blocks of A only).
-// The last workgroup of the job to arrive writes L_D back (counters[jb.cnt]
-// returns to zero).  float: a pivot-growth check (L(j,j)^2 against the
+//    width b), inverts its four 16x16 diagonal blocks and solves its rows
+//    against L_D^T by 16-column blocks on MFMA (X = A inv(L_D)^T).
+// The last workgroup of the job to arrive writes L_D back at its end
+// (counters[jb.cnt] returns to zero).  float: a pivot-growth check (L(j,j)^2 against the
 // updated diagonal) refactors the block in FP64 when it fails.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
@@ -507,21 +517,14 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
             }
         __syncthreads();   // the last chunk's readers are done
     }
-    // the updated diagonal block (identity beyond b) and rows to LDS
+    // the updated diagonal block (identity beyond b) to LDS
 #pragma unroll
     for (int N = 0; N < 4; ++N)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = 16 * wave + li, c = 16 * N + MfmaOp<T>::row(lg, v);
             As[r + 65 * c] = (r < b && c < b) ? Dv[N][v] : (r == c ? T(1) : T(0));
-            Ws[r + 65 * c] = Rv[N][v];
         }
-    __syncthreads();
-    // this wavefront's rows in the operand layout, xa[ks] = A(16 w + li, 4 ks + lg)
-    // (columns past b hold finite duplicates; Z is the identity there)
-    T xa[16];
-#pragma unroll
-    for (int ks = 0; ks < 16; ++ks) xa[ks] = Ws[(16 * wave + li) + 65 * (4 * ks + lg)];
     __syncthreads();
     if (tid == 0) last = atomicAdd(&counters[jb.cnt], 1) == jb.ntiles - 1;
     if (tid < 64) a0[tid] = As[tid * 66];            // updated diagonal
@@ -558,6 +561,42 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
         const unsigned long long nb = __ballot(!(As[lane * 66] > T(0)));
         bad = nb ? __builtin_ctzll(nb) : 64;
     }
+    __syncthreads();   // rd
+    T* Zs = Ws;
+    inv16_diag<T>(As, rd, Zs, wave, lane);
+    __syncthreads();
+    // X = A inv(L_D)^T by 16-column blocks N: T_N = A_N - sum_{M<N} X_M L_NM^T,
+    // X_N = T_N inv(L_NN)^T, all in the accumulator layout (lane (li, lg),
+    // element v: row 16 w + li, column 16 N + row(lg, v)); MFMA step s of a
+    // product contracts over the columns {row(lg, s) : lg = 0..3} of a block,
+    // so an accumulator feeds the next product as its B operand unchanged.
+    // Columns past b hold finite duplicates; L_D is the identity there.
+    acc_t acc[4];
+#pragma unroll
+    for (int N = 0; N < 4; ++N) {
+        acc_t t = {Rv[N][0], Rv[N][1], Rv[N][2], Rv[N][3]};
+#pragma unroll
+        for (int M = 0; M < N; ++M)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                t = MfmaOp<T>::mma(-As[(16 * N + li) + 65 * (16 * M + MfmaOp<T>::row(lg, s))], acc[M][s], t);
+        acc[N] = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            acc[N] = MfmaOp<T>::mma(Zs[(16 * N + li) + 65 * (16 * N + MfmaOp<T>::row(lg, s))], t[s], acc[N]);
+    }
+    if (rw < jb.rows) {
+        T* __restrict__ dst = g + b + rw;
+#pragma unroll
+        for (int N = 0; N < 4; ++N)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = 16 * N + MfmaOp<T>::row(lg, v);
+                if (col < b) dst[(int64_t)col * ld] = acc[N][v];
+            }
+    }
+    // the last workgroup of the job writes L_D back; last, so that no
+    // barrier waits on these stores (As is not written after the Cholesky)
     if (last) {
         for (int t = tid; t < 64 * 64; t += kThreads) {
             const int r = t & 63, c = t >> 6;
@@ -568,28 +607,6 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
             if (fell_back) atomicAdd(fallbacks, 1);
             atomicExch(&counters[jb.cnt], 0);
         }
-    }
-    __syncthreads();
-    T* Zs = Ws;
-    inv64_blocked<T>(As, rd, Zs, Zs + 64 * 65, wave, lane);
-    // X = A Z^T: D(p, q) = sum_k Z(16 N + p, k) A(row q, k) = X(row q, 16 N + p)
-    acc_t acc[4];
-#pragma unroll
-    for (int N = 0; N < 4; ++N) acc[N] = acc_t{T(0), T(0), T(0), T(0)};
-#pragma unroll
-    for (int N = 0; N < 4; ++N)
-#pragma unroll
-        for (int ks = 0; ks < 4 * N + 4; ++ks)
-            acc[N] = MfmaOp<T>::mma(Zs[(16 * N + li) + 65 * (4 * ks + lg)], xa[ks], acc[N]);
-    if (rw < jb.rows) {
-        T* __restrict__ dst = g + b + rw;
-#pragma unroll
-        for (int N = 0; N < 4; ++N)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int col = 16 * N + MfmaOp<T>::row(lg, v);
-                if (col < b) dst[(int64_t)col * ld] = acc[N][v];
-            }
     }
 }
 

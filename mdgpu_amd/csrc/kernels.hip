@@ -430,12 +430,12 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
                                                        T* __restrict__ L, int* __restrict__ info,
                                                        int* __restrict__ counters, int* __restrict__ fallbacks) {
     typedef typename MfmaOp<T>::acc_t acc_t;
-    constexpr int kZ = 64 * 65 + 4 * 16 * 17;                           // Zs + Ts
+    constexpr int kZ = 64 * 65;                                          // one 64 x 64 block
     constexpr int kFb = std::is_same<T, float>::value ? 64 * 65 * 8 : 0;  // FP64 fallback block
     constexpr int kScr = kZ * (int)sizeof(T) > kFb ? kZ * (int)sizeof(T) : kFb;
     __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]
-    // K chunks of the update, then the rows (for the operand layout), then
-    // Zs, Ts or the FP64 fallback block
+    // K chunks of the update, then the FP64 fallback block or the inverses
+    // of the diagonal 16x16 blocks (same layout as As)
     __shared__ __align__(16) unsigned char scr[kScr];
     __shared__ T rd[64];
     __shared__ T a0[64];

@@ -162,7 +162,8 @@ int mdgpu_numeric_set_graph(mdgpu_numeric* num, int enable);
 #define MDGPU_NUM_PANEL_FRONTS 5  /* fronts factored wholly in LDS         */
 #define MDGPU_NUM_DIAG_JOBS 6     /* diagonal blocks of the blocked path   */
 #define MDGPU_NUM_PRECISION 7
-#define MDGPU_NUM_FALLBACKS 8     /* FP32/BF16: diagonal blocks of the last
+#define MDGPU_NUM_FALLBACKS 8     /* FP32/BF16: diagonal blocks and fronts
+                                     factored wholly in LDS of the last
                                      factorization refactored in FP64 after
                                      failing the pivot-growth check
                                      (L_jj^2 < 2^-12 A_jj or L_jj^2 <= 0)  */

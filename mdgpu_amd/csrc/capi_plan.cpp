@@ -12,10 +12,9 @@ PlanOptions plan_options_for(int precision) {
     PlanOptions po;
     po.elem_bytes = precision == MDGPU_FP64 ? 8 : 4;
     po.panel_lds_bytes = 61440;
-    // GEMM tile: 128 selects the MFMA kernel, 64 the FMA kernel
-    if (const char* e = std::getenv("MDGPU_GEMM_TILE")) po.tile = std::atoi(e) == 64 ? 64 : 128;
-    else po.tile = 128;
-    if (precision == MDGPU_BF16 || precision == MDGPU_BF16X3) po.tile = 128;   // bf16 kernels: 128-row tiles only
+    // largest MFMA GEMM tile (rows = columns); the autotuner may pick 64
+    // per launch (not for the bf16 kernels, which have 128-row tiles only)
+    po.tile = 128;
     // outer block width of the blocked panel path (multiple of 64)
     if (const char* e = std::getenv("MDGPU_NB_OUTER")) {
         const int v = std::atoi(e);

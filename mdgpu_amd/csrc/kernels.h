@@ -16,9 +16,12 @@ template <typename T> void launch_scatter_a(T* L, const double* ax, const int64_
 template <typename T>
 void launch_extend_add(const EaItem* items, int count, const EaContrib* contrib, const int* rel, T* L, T* U,
                        hipStream_t s);
+// one workgroup per front of the list; float: fronts that fail the
+// pivot-growth check go on fb_list (counter fb_cnt, zero before the launch)
+// and are refactored in FP64 by a second launch (fallbacks counts them)
 template <typename T>
-void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,
-                  hipStream_t s);
+void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes, int* fb_cnt,
+                  int* fb_list, int* fallbacks, hipStream_t s);
 // fused diagonal-block Cholesky + TRSM of the rows below; counters: one
 // zero-initialised int per job (returned to zero by the kernel); fallbacks:
 // incremented once per diagonal block refactored in FP64 (pivot growth)
@@ -27,10 +30,9 @@ void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int
                 hipStream_t s);
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
-// MFMA kernel with tile x tile tiles (128 or 64, the step's tile size); fma:
-// the FMA kernel with 64x64 tiles instead
+// MFMA kernel with tile x tile tiles (128 or 64, the step's tile size)
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, bool fma, T* L, T* U, hipStream_t s);
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s);
 // same jobs (128-row tiles), FP32 data, bf16 MFMA with FP32 accumulation;
 // nsplit 1: one bf16 product, 3: split hi/lo products (hi*hi + hi*lo + lo*hi)
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s);

@@ -175,45 +175,112 @@ __device__ inline double pivot_rsqrt(double d) {
     return y;
 }
 
+// Pivot-growth threshold of the reduced-precision factorizations: a
+// diagonal block (or a front factored wholly in LDS) whose pivots satisfy
+// L_jj^2 < 2^-12 A_jj (more than 12 of FP32's 24 bits cancelled), or that
+// has a non-positive pivot, is factored again in FP64 from its assembled
+// values.
+constexpr double kPivotGrowthMin = 1.0 / 4096.0;
+
 // ------------------------------------------------------ dense: LDS panels
-// Partial Cholesky of a whole panel held in LDS: for each pivot column,
-// sqrt, scale, rank-1 update of the remaining pivot columns (the unblocked
-// scal+syr loop of FactorAux_CPU_3, mdgpu.cu:1318-1326, applied to the full
-// n_f rows, which also performs the trsm of :1330).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void panel_kernel(const int* __restrict__ list,
-                                                         const FrontDesc* __restrict__ fronts,
-                                                         T* __restrict__ L, int* __restrict__ info) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* P = reinterpret_cast<T*>(smem_raw);
-    __shared__ T pivinv;
-    const FrontDesc d = fronts[list[blockIdx.x]];
-    const int n = d.n, c = d.c;
-    T* __restrict__ g = L + d.lofs;
-    const int tot = n * c;
-    for (int t = threadIdx.x; t < tot; t += kThreads) P[t] = g[t];
-    __syncthreads();
+// Partial Cholesky of a whole panel P (n x c, column-major, leading
+// dimension n) held in LDS: for each pivot column, sqrt, scale, rank-1
+// update of the remaining pivot columns (the unblocked scal+syr loop of
+// FactorAux_CPU_3, mdgpu.cu:1318-1326, applied to the full n_f rows, which
+// also performs the trsm of :1330).  Called by all threads; ends with a
+// barrier.  check: reduced precision, where a pivot that fails the
+// pivot-growth check against the assembled diagonal a0 sets *weak (the
+// caller then falls back to FP64) instead of being reported; otherwise a
+// non-positive pivot is reported in *info as its global column.
+template <typename U>
+__device__ __forceinline__ void panel_chol(U* P, int n, int c, int k, bool check, const U* a0, int* weak, U* pivinv,
+                                           int* info) {
     for (int j = 0; j < c; ++j) {
         if (threadIdx.x == 0) {
-            T dj = P[j + j * n];
-            if (!(dj > T(0))) atomicMin(info, d.k + j);
-            const T is = pivot_rsqrt(dj);
+            const U dj = P[j + j * n];
+            if (check) {
+                if (!(dj > U(0)) || !((double)dj >= kPivotGrowthMin * (double)a0[j])) *weak = 1;
+            } else if (!(dj > U(0))) {
+                atomicMin(info, k + j);
+            }
+            const U is = pivot_rsqrt(dj);
             P[j + j * n] = dj * is;
-            pivinv = is;
+            *pivinv = is;
         }
         __syncthreads();
-        const T s = pivinv;
+        const U s = *pivinv;
         for (int i = j + 1 + threadIdx.x; i < n; i += kThreads) P[i + j * n] *= s;
         __syncthreads();
         for (int i = j + 1 + threadIdx.x; i < n; i += kThreads) {
-            const T lij = P[i + j * n];
+            const U lij = P[i + j * n];
             const int cmax = min(i, c - 1);
 #pragma unroll 4
             for (int col = j + 1; col <= cmax; ++col) P[i + col * n] -= lij * P[col + j * n];
         }
         __syncthreads();
     }
+}
+
+// One workgroup per front of the step's list.  float: a front that fails
+// the pivot-growth check keeps its assembled panel in L and goes on the
+// fallback list (fb_list[0 .. *fb_cnt)) for panel_fb_kernel.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void panel_kernel(const int* __restrict__ list,
+                                                         const FrontDesc* __restrict__ fronts,
+                                                         T* __restrict__ L, int* __restrict__ info,
+                                                         int* __restrict__ fb_cnt, int* __restrict__ fb_list) {
+    constexpr bool kCheck = std::is_same<T, float>::value;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* P = reinterpret_cast<T*>(smem_raw);
+    __shared__ T pivinv;
+    __shared__ T a0[kCheck ? kPanelMaxPivots : 1];
+    __shared__ int weak;
+    const int f = list[blockIdx.x];
+    const FrontDesc d = fronts[f];
+    const int n = d.n, c = d.c;
+    T* __restrict__ g = L + d.lofs;
+    const int tot = n * c;
+    for (int t = threadIdx.x; t < tot; t += kThreads) P[t] = g[t];
+    if (threadIdx.x == 0) weak = 0;
+    __syncthreads();
+    if constexpr (kCheck) {
+        if (threadIdx.x < c) a0[threadIdx.x] = P[threadIdx.x * (n + 1)];   // assembled diagonal
+        __syncthreads();
+    }
+    panel_chol<T>(P, n, c, d.k, kCheck, a0, &weak, &pivinv, info);
+    if (kCheck && weak) {
+        if (threadIdx.x == 0) fb_list[atomicAdd(fb_cnt, 1)] = f;
+        return;
+    }
     for (int t = threadIdx.x; t < tot; t += kThreads) g[t] = P[t];
+}
+
+// FP64 refactorization of the fronts panel_kernel<float> put on the
+// fallback list: each workgroup takes list entries blockIdx.x,
+// blockIdx.x + gridDim.x, ..., loads the assembled FP32 panel into LDS as
+// FP64, factors it (a non-positive pivot is reported in *info) and stores
+// the result rounded to FP32.  With an empty list every workgroup exits at
+// once.
+__global__ __launch_bounds__(kThreads) void panel_fb_kernel(const int* __restrict__ fb_cnt,
+                                                            const int* __restrict__ fb_list,
+                                                            const FrontDesc* __restrict__ fronts,
+                                                            float* __restrict__ L, int* __restrict__ info,
+                                                            int* __restrict__ fallbacks) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* P = reinterpret_cast<double*>(smem_raw);
+    __shared__ double pivinv;
+    const int cnt = *fb_cnt;
+    for (int t = blockIdx.x; t < cnt; t += gridDim.x) {
+        const FrontDesc d = fronts[fb_list[t]];
+        const int n = d.n, c = d.c, tot = n * c;
+        float* __restrict__ g = L + d.lofs;
+        for (int e = threadIdx.x; e < tot; e += kThreads) P[e] = (double)g[e];
+        __syncthreads();
+        panel_chol<double>(P, n, c, d.k, false, nullptr, nullptr, &pivinv, info);
+        for (int e = threadIdx.x; e < tot; e += kThreads) g[e] = (float)P[e];
+        if (threadIdx.x == 0) atomicAdd(fallbacks, 1);
+        __syncthreads();   // P is reused by the next entry
+    }
 }
 
 // MFMA operands of the 16x16x4 products (v_mfma_{f32,f64}_16x16x4):
@@ -401,12 +468,6 @@ __device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T
         __syncthreads();
     }
 }
-
-// Pivot-growth threshold of the reduced-precision factorizations: a
-// diagonal block whose pivots satisfy L_jj^2 < 2^-12 A_jj (more than 12 of
-// FP32's 24 bits cancelled), or that has a non-positive pivot, is factored
-// again in FP64 from its assembled values.
-constexpr double kPivotGrowthMin = 1.0 / 4096.0;
 
 // Fused left-looking update + diagonal-block Cholesky + TRSM of one
 // 64-column block step (the potrf + trsm pair of FactorAux_GPU_2,
@@ -749,83 +810,10 @@ __global__ __launch_bounds__(kThreads) void inv256_kernel(const InvJob* __restri
     }
 }
 
-// C -= A * B^T on 64x64 tiles (beta0: C = -A * B^T), optionally keeping only
-// the lower triangle; tiles of many jobs (fronts) in one launch.  This is the
-// trailing-matrix gemm of FactorAux_CPU_3 / FactorAux_GPU_2 (mdgpu.cu:1336,
-// :1058) restricted to the lower triangle.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void gemm_nt_kernel(const GemmJob* __restrict__ jobs, int njobs,
-                                                           T* __restrict__ L, T* __restrict__ U) {
-    __shared__ T As[16][64 + 1];
-    __shared__ T Bs[16][64 + 1];
-    int lo = 0, hi = njobs - 1;
-    const int bid = blockIdx.x;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
-    }
-    const GemmJob g = jobs[lo];
-    int local = bid - g.tile_base;
-    int ti, tj;
-    if (g.lower) {
-        tj = 0;
-        int cnt = g.tiles_m;
-        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
-        ti = tj + local;
-    } else {
-        tj = local / g.tiles_m;
-        ti = local % g.tiles_m;
-    }
-    const T* __restrict__ A = L + g.a_off;
-    const T* __restrict__ B = L + g.b_off;
-    T* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
-    const int row0 = ti * 64, col0 = tj * 64;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    T acc[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc[r][s] = T(0);
-    const int li = threadIdx.x & 63, lk = threadIdx.x >> 6;
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int kk = lk + 4 * s;
-            const int k = k0 + kk;
-            const int ia = row0 + li, jb = col0 + li;
-            As[kk][li] = (k < g.K && ia < g.M) ? A[ia + (int64_t)k * g.lda] : T(0);
-            Bs[kk][li] = (k < g.K && jb < g.N) ? B[jb + (int64_t)k * g.ldb] : T(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            T a[4], bb[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = As[kk][tx + 16 * r];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) bb[s] = Bs[kk][ty + 16 * s];
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int s = 0; s < 4; ++s) acc[r][s] += a[r] * bb[s];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int j = col0 + ty + 16 * s;
-        if (j >= g.N) continue;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = row0 + tx + 16 * r;
-            if (i >= g.M || (g.lower && i < j)) continue;
-            T* cp = C + i + (int64_t)j * g.ldc;
-            *cp = g.beta0 ? -acc[r][s] : *cp - acc[r][s];
-        }
-    }
-}
-
-// MFMA version of gemm_nt_kernel for gfx950: 128x128 C tiles, four
+// C -= A * B^T (beta0: C = -A * B^T), optionally keeping only the lower
+// triangle, tiles of many jobs (fronts) in one launch: the trailing-matrix
+// gemm of FactorAux_CPU_3 / FactorAux_GPU_2 (mdgpu.cu:1336, :1058)
+// restricted to the lower triangle.  128x128 (or 64x64) C tiles, four
 // wavefronts of 64x64, v_mfma_{f32,f64}_16x16x4.  The contraction index of
 // one 16x16x4 instruction is free to relabel as long as both operands agree,
 // so lane group g = lane/16 supplies k = 4g+t in sub-step t: each lane then
@@ -1873,15 +1861,22 @@ void launch_extend_add(const EaItem* items, int count, const EaContrib* contrib,
     if (count > 0) extend_add_kernel<T><<<(count + 3) / 4, kThreads, 0, s>>>(items, count, contrib, rel, L, U);
 }
 template <typename T>
-void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes,
-                  hipStream_t s) {
+void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int* info, int lds_bytes, int* fb_cnt,
+                  int* fb_list, int* fallbacks, hipStream_t s) {
     if (count <= 0) return;
-    static bool attr_set = false;
-    if (!attr_set) {
+    // dynamic LDS beyond 64 KB must be allowed explicitly (static
+    // initialisation: once, thread-safe)
+    static const bool attr_set = [] {
         (void)hipFuncSetAttribute((const void*)panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-        attr_set = true;
-    }
-    panel_kernel<T><<<count, kThreads, lds_bytes, s>>>(list, fronts, L, info);
+        (void)hipFuncSetAttribute((const void*)panel_fb_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  2 * 65536);
+        return true;
+    }();
+    (void)attr_set;
+    panel_kernel<T><<<count, kThreads, lds_bytes, s>>>(list, fronts, L, info, fb_cnt, fb_list);
+    if constexpr (std::is_same<T, float>::value)
+        panel_fb_kernel<<<count < 64 ? count : 64, kThreads, 2 * lds_bytes, s>>>(fb_cnt, fb_list, fronts, L, info,
+                                                                                 fallbacks);
 }
 template <typename T>
 void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
@@ -1893,10 +1888,9 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
     if (count > 0) inv_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, bool fma, T* L, T* U, hipStream_t s) {
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s) {
     if (nblocks <= 0) return;
-    if (fma) gemm_nt_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
-    else if (tile == 128) gemm_mfma_kernel<T, 128><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    if (tile == 128) gemm_mfma_kernel<T, 128><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
     else gemm_mfma_kernel<T, 64><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
 }
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s) {
@@ -1986,11 +1980,11 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_fill<T>(T*, int64_t, T, hipStream_t);                                                 \
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
     template void launch_extend_add<T>(const EaItem*, int, const EaContrib*, const int*, T*, T*, hipStream_t); \
-    template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, hipStream_t);              \
+    template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, int*, int*, int*, hipStream_t); \
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
-    template void launch_gemm<T>(const GemmJob*, int, int, int, bool, T*, T*, hipStream_t);                         \
+    template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \
     template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, const T*, double*,    \

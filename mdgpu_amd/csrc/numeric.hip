@@ -56,6 +56,9 @@ struct mdgpu_numeric {
     double* dUpdPartial = nullptr;        // BWD_UPD partial sums
     mdgpu::InvJob* dInv256Jobs = nullptr;
     int* dPanel = nullptr;
+    int* dPanelFbList = nullptr;          // FP64 fallback lists of the panel steps (one slot per panel front)
+    int* dPanelFbCnt = nullptr;           // their counters, one per panel step
+    int npanel_steps = 0;
     int* dLevelList = nullptr;
     std::vector<int> level_off;
     double* dSu = nullptr;
@@ -76,7 +79,6 @@ struct mdgpu_numeric {
     int* dSeg = nullptr;                  // row -> segment (batches only)
     double* dPartial = nullptr;
     int nparts = 0;
-    int tile = 64;
     // hipGraph replay of the fixed launch sequences (refactorization and
     // solves on one pattern): captured on cap_stream against the internal
     // buffers dAx / dB / dX, launched on the caller's stream
@@ -215,9 +217,11 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
     mdgpu::launch_fill<T>(L, P.lstore, T(0), s);
     mdgpu::launch_fill<int>(N->dInfo, 1, INT_MAX, s);
     mdgpu::launch_fill<int>(N->dInfo + 1, 1, 0, s);     // FP64 diagonal fallbacks
+    if (N->npanel_steps > 0) mdgpu::launch_fill<int>(N->dPanelFbCnt, N->npanel_steps, 0, s);
     mdgpu::launch_scatter_a<T>(L, ax, N->dAdst, (int)P.a_dst.size(), s);
     if (ev) hipEventRecord(ev[1], s);
     const hipStream_t main_s = s;
+    int panel_step = 0;
     for (size_t t = 0; t < P.steps.size(); t++) {
         const mdgpu::Step& st = P.steps[t];
         // look-ahead (not when timing every step): side-stream launches and
@@ -230,7 +234,9 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_extend_add<T>(N->dEa + st.first, st.count, N->dEaContrib, N->dRel, L, U, s);
                 break;
             case mdgpu::STEP_PANEL:
-                mdgpu::launch_panel<T>(N->dPanel + st.first, st.count, N->dFronts, L, N->dInfo, st.arg, s);
+                mdgpu::launch_panel<T>(N->dPanel + st.first, st.count, N->dFronts, L, N->dInfo, st.arg,
+                                       N->dPanelFbCnt + panel_step, N->dPanelFbList + st.first, N->dInfo + 1, s);
+                panel_step++;
                 break;
             case mdgpu::STEP_BLK:
                 mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt,
@@ -244,7 +250,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                         break;
                     }
                 }
-                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, st.arg, N->tile == 64, L, U, s);
+                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, st.arg, L, U, s);
                 break;
             case mdgpu::STEP_INV:
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
@@ -409,6 +415,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         const mdgpu::FrontDesc& d = P.fronts[f];
         for (int j = 0; j < d.c; j++) diag_off[d.k + j] = d.lofs + (int64_t)j * d.n + j;
     }
+    for (const mdgpu::Step& stp : P.steps) N->npanel_steps += stp.kind == mdgpu::STEP_PANEL;
     N->nparts = kLogParts;
     N->nseg = std::max(1, (int)sym->seg_col.size() - 1);
     std::vector<int> seg_of_row;
@@ -417,7 +424,6 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         for (int k = 0; k < N->nseg; k++)
             for (int r = sym->seg_col[k]; r < sym->seg_col[k + 1]; r++) seg_of_row[r] = k;
     }
-    N->tile = po.tile;
     if (const char* g = std::getenv("MDGPU_GRAPH")) N->use_graph = std::atoi(g) != 0;
     if ((st = alloc(N, P.lstore * es, &N->dL)) ||
         (st = alloc(N, P.arena * es, &N->dU)) ||
@@ -438,6 +444,8 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.blk, (void**)&N->dBlk)) ||
         (st = upload(N, std::vector<int>(P.blk.size(), 0), (void**)&N->dBlkCnt)) ||
         (st = upload(N, P.panel, (void**)&N->dPanel)) ||
+        (st = alloc(N, std::max<size_t>(P.panel.size(), 1) * sizeof(int), (void**)&N->dPanelFbList)) ||
+        (st = alloc(N, std::max(N->npanel_steps, 1) * sizeof(int), (void**)&N->dPanelFbCnt)) ||
         (st = upload(N, level_list, (void**)&N->dLevelList)) ||
         (st = alloc(N, P.solve_u * sizeof(double), (void**)&N->dSu)) ||
         (st = upload(N, P.solve_uofs, (void**)&N->dSuofs)) ||
@@ -887,9 +895,9 @@ int install_gemm_tiles(mdgpu_numeric* N, const std::vector<int>& tiles) {
 extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, void* stream, int64_t* changed) {
     mdgpu::clear_error();
     if (!num || !d_ax) return fail(MDGPU_ERR_ARG, "mdgpu_numeric_autotune: null pointer");
-    // FMA kernels have one tile size; the bf16 variants keep their 128-row
-    // bf16 kernel (64-row launches would run the FP32 kernel instead)
-    if (num->po.tile != 128 || num->precision == MDGPU_BF16 || num->precision == MDGPU_BF16X3) {
+    // the bf16 variants keep their 128-row bf16 kernel (64-row launches
+    // would run the FP32 kernel instead)
+    if (num->precision == MDGPU_BF16 || num->precision == MDGPU_BF16X3) {
         if (changed) *changed = 0;
         return MDGPU_OK;
     }

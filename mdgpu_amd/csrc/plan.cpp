@@ -380,7 +380,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             for (int f : fl) {
                 const FrontDesc& d = P.fronts[f];
                 int64_t bytes = (int64_t)d.n * d.c * o.elem_bytes;
-                if (bytes <= o.panel_lds_bytes) {
+                if (bytes <= o.panel_lds_bytes && d.c <= kPanelMaxPivots) {
                     P.panel.push_back(f);
                     lds = std::max<int>(lds, (int)bytes);
                 } else {
@@ -784,6 +784,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
     for (int f : P.panel) {
         const FrontDesc& d = P.fronts[f];
         if ((int64_t)d.n * d.c * o.elem_bytes > o.panel_lds_bytes) return bad("panel LDS");
+        if (d.c > kPanelMaxPivots) return bad("panel pivots");
     }
     // look-ahead events: recorded once, waited only after being recorded,
     // only GEMMs on the side stream, and every side launch joined back to

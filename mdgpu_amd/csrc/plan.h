@@ -95,6 +95,13 @@ struct BlkJob {
     int kprev;                     // earlier 64-column blocks of the outer block (left-looking update)
 };
 constexpr int kBlkRows = 64;
+// Fronts factored wholly in LDS by one workgroup (STEP_PANEL) have at most
+// this many pivots (the kernel keeps their assembled diagonal for the
+// pivot-growth check).  In reduced precision a front that fails the check
+// is put on the step's fallback list (its panel left as assembled) and
+// factored again in FP64 by a second launch right after the step, with
+// twice the LDS.
+constexpr int kPanelMaxPivots = 128;
 
 // A launch of the schedule: kind + a slice of the matching job array.
 enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV, STEP_INV256 };

@@ -8,9 +8,11 @@ writes
 Only data is committed (inputs are regenerated from mdgpu_amd.gen and
 checked against a stored hash of the input pattern).
 
-usage: python tests/golden/make_symbolic_golden.py [--large | --large-only]
+usage: python tests/golden/make_symbolic_golden.py [--large | --large-only | --cases a,b,...]
 (--large-only rewrites only the large-case hashes; the reference's relaxed
-amalgamation on the 64^3 stencil takes about 20 minutes)
+amalgamation on the 64^3 stencil takes about 20 minutes; --cases computes the
+named large cases only and merges them into the existing hash file, so they
+can run as separate processes)
 """
 from __future__ import annotations
 
@@ -51,7 +53,32 @@ def run_case(name, make, mode, mz):
     return p, i, out
 
 
+def large_hashes(cases):
+    hashes = {}
+    for name, make, mode, mz in cases:
+        p, i, out = run_case(name, make, mode, mz)
+        hashes[name] = {"input": input_hash(p, i)}
+        for k in ARRAYS:
+            if k in out:
+                hashes[name][k] = [int(len(out[k])), sha(out[k])]
+        print(name, "done", flush=True)
+    return hashes
+
+
 def main():
+    path = os.path.join(HERE, "symbolic_hashes.json")
+    if "--cases" in sys.argv:
+        names = sys.argv[sys.argv.index("--cases") + 1].split(",")
+        new = large_hashes([c for c in LARGE_CASES if c[0] in names])
+        # merge under an exclusive lock: several of these may run at once
+        import fcntl
+        with open(path + ".lock", "w") as lk:
+            fcntl.flock(lk, fcntl.LOCK_EX)
+            hashes = json.load(open(path)) if os.path.exists(path) else {}
+            hashes.update(new)
+            with open(path, "w") as f:
+                json.dump(hashes, f, indent=1, sort_keys=True)
+        return
     large_only = "--large-only" in sys.argv
     large = large_only or "--large" in sys.argv
     small = {}
@@ -68,16 +95,8 @@ def main():
         with open(os.path.join(HERE, "symbolic_small_meta.json"), "w") as f:
             json.dump(meta, f, indent=1, sort_keys=True)
     if large:
-        hashes = {}
-        for name, make, mode, mz in LARGE_CASES:
-            p, i, out = run_case(name, make, mode, mz)
-            hashes[name] = {"input": input_hash(p, i)}
-            for k in ARRAYS:
-                if k in out:
-                    hashes[name][k] = [int(len(out[k])), sha(out[k])]
-            print(name, "done", flush=True)
-        with open(os.path.join(HERE, "symbolic_hashes.json"), "w") as f:
-            json.dump(hashes, f, indent=1, sort_keys=True)
+        with open(path, "w") as f:
+            json.dump(large_hashes(LARGE_CASES), f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":

@@ -255,3 +255,65 @@ def test_indefinite_matrix_is_reported(device, prec):
     xt, b = gen.rhs_from_xtrue(p, i, x, seed=9)
     _, eta, _ = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=30)
     assert eta <= 1e-12
+
+
+def test_pivot_growth_fallback_lds_front(device):
+    """A front factored wholly in LDS (one 40 x 40 dense supernode, 6.4 KB in
+    FP32) with heavy cancellation (ones + 1e-4 I: every pivot after the first
+    is ~1e-4 of its assembled diagonal, below the 2^-12 threshold) fails the
+    FP32 pivot-growth check and is refactored in FP64 by the fallback launch.
+    The FP32 factor then equals the FP64 factor rounded to FP32, and
+    refinement reaches 1e-12; the FP64 path never falls back."""
+    import torch
+    from oracle import oracle as O
+    n = 40
+    A = np.ones((n, n)) + 1e-4 * np.eye(n)
+    p, i, x = _dense_lower(A)
+    S = symbolic.analyze(p, i, False, "sn", 0)
+    assert S.nnodes == 1
+    ax = torch.from_numpy(x).to(device)
+    F = Factorization(S, "fp32").factorize(ax)
+    info = F.info()
+    assert info["panel_fronts"] == 1 and info["diag_jobs"] == 0, info
+    assert info["fp64_fallbacks"] == 1, info
+    D = Factorization(S, "fp64").factorize(ax)
+    assert D.info()["fp64_fallbacks"] == 0
+    _, Li, Lx32 = F.export_L()
+    _, Li64, Lx64 = D.export_L()
+    assert np.array_equal(Li, Li64)
+    # the fallback factors the FP32-rounded assembled values in FP64: within
+    # FP32 rounding (relative to the largest entry) of the FP64 factor
+    assert rel_err(Lx32, Lx64) <= 1e-5, rel_err(Lx32, Lx64)
+    b = A @ np.random.default_rng(0).uniform(-1, 1, n)
+    xs, eta, iters = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=80)
+    assert eta <= 1e-12 and O.backward_error(p, i, x, xs.cpu().numpy(), b) <= 1e-12, (eta, iters)
+    # a second factorization on the same handle: the fallback list starts empty again
+    F.factorize(ax)
+    assert F.info()["fp64_fallbacks"] == 1
+    assert np.array_equal(F.export_L()[2], Lx32)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_refine_eta_equals_oracle_backward_error(device, prec):
+    """The backward error mdgpu_refine reports is the reference's
+    ||b - A x||_inf / (||A||_1 ||x||_inf + ||b||_inf) (mdgpu.cu:1972) of the
+    x it returns: with maxit = 0 the x is the unrefined reduced-precision
+    solve (eta ~1e-8, far above rounding), and the oracle's backward_error of
+    that same x must agree to 1e-6 relative.  A dropped ||b|| term or a wrong
+    norm changes eta by O(1)."""
+    import torch
+    from oracle import oracle as O
+    p, i, x = gen.stencil27(14, seed=8)
+    xt, b = gen.rhs_from_xtrue(p, i, x, seed=8)
+    S = symbolic.analyze(p, i, True, "rn", 1000)
+    F = Factorization(S, prec).factorize(torch.from_numpy(x).to(device))
+    xs, eta, iters = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=0, raise_on_fail=False)
+    assert iters == 0
+    eta_cpu = O.backward_error(p, i, x, xs.cpu().numpy(), b)
+    assert eta_cpu > 1e-11, eta_cpu             # far above FP64 rounding
+    assert abs(eta - eta_cpu) <= 1e-6 * eta_cpu, (eta, eta_cpu)
+    # after refinement: the same agreement at the converged x (bound by rounding)
+    xs, eta, iters = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=30)
+    eta_cpu = O.backward_error(p, i, x, xs.cpu().numpy(), b)
+    assert eta <= 1e-12 and eta_cpu <= 1e-12
+    assert abs(eta - eta_cpu) <= 1e-3 * eta_cpu + 1e-17, (eta, eta_cpu)

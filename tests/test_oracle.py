@@ -97,3 +97,25 @@ def test_forest_factorization():
     D = np.linalg.cholesky((B + sp.tril(B, -1).T).toarray())
     Lm = sp.csc_matrix((lx, li, lp), shape=(S.n, S.n)).toarray()
     assert np.abs(Lm - D).max() <= 1e-12 * np.abs(D).max()
+
+
+def test_backward_error_against_scipy_norms():
+    """oracle.backward_error is the reference's eta = ||b - A x||_inf /
+    (||A||_1 ||x||_inf + ||b||_inf) (mdgpu.cu:1972, cs_norm cs.cpp:236-248)
+    on the full symmetric A rebuilt from the lower triangle: checked against
+    scipy's sparse norms at a perturbed x (eta of order 1e-3).  The checks
+    below fail for a dropped ||b|| term, the infinity norm of A in place of
+    the 1-norm on a non-symmetric-looking weighting, or the lower triangle
+    alone in the product."""
+    import scipy.sparse.linalg as sla
+    p, i, x = gen.stencil27(6, seed=2)
+    A = gen.full_from_lower(p, i, x)
+    n = A.shape[0]
+    rng = np.random.default_rng(3)
+    xv = rng.uniform(-1, 1, n)
+    b = A @ xv + 1e-3 * rng.standard_normal(n)
+    eta = O.backward_error(p, i, x, xv, b)
+    r = b - A @ xv
+    ref = np.abs(r).max() / (sla.norm(A, 1) * np.abs(xv).max() + np.abs(b).max())
+    assert abs(eta - ref) <= 1e-12 * ref
+    assert abs(eta - np.abs(r).max() / (sla.norm(A, 1) * np.abs(xv).max())) > 1e-3 * ref

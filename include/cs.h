@@ -14,6 +14,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+MDGPU_DROPIN_BEGIN
+
 typedef struct cs_sparse { /* compressed-column (nz == -1) or triplet form */
     int nzmax;             /* capacity of i and x                        */
     int m;                 /* rows                                       */
@@ -82,6 +84,8 @@ void *cs_free(void *p);
 void *cs_realloc(void *p, int n, size_t size, int *ok);
 cs *cs_done(cs *C, void *w, void *x, int ok);
 int *cs_idone(int *p, cs *C, void *w, int ok);
+
+MDGPU_DROPIN_END
 
 #define CS_MAX(a, b) (((a) > (b)) ? (a) : (b))
 #define CS_MIN(a, b) (((a) < (b)) ? (a) : (b))

@@ -6,6 +6,8 @@
 
 #include "cs.h"
 
+MDGPU_DROPIN_BEGIN
+
 struct ETree {
     int nodo;               /* first column of the node                 */
     int nHijos;             /* number of children                       */
@@ -27,5 +29,7 @@ void GetSuperETree(struct ETree **listaNodos, int cantNodos, struct ETree ***lis
 void GetRelaxedETree(struct ETree **listaNodosSuper, int cantNodosSuper, int max_ceros, int *count, int **perm,
                      struct ETree ***listaNodosRelaxed_out, int *cantNodosRelaxed_out);
 void PrintETree(struct ETree *root, int nivel);
+
+MDGPU_DROPIN_END
 
 #endif

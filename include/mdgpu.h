@@ -12,6 +12,8 @@
 #include "etree.h"
 #include "types.h"
 
+MDGPU_DROPIN_BEGIN
+
 typedef struct {
     int w;          /* leading dimension (rows allocated)  */
     int n;          /* rows = columns of the front          */
@@ -62,5 +64,7 @@ int cs_print(FILE *f, const cs *A, int brief);
  * listaNodos = postordered node list (root(s) last).  On failure spL is left
  * unfilled and mdgpu_last_error() describes why. */
 void Factor(cs *A, cs *spL, struct ETree **listaNodos, int cantNodos);
+
+MDGPU_DROPIN_END
 
 #endif

@@ -8,6 +8,12 @@
  * ordinary memory.  `stream` is a hipStream_t passed as void* (NULL = the
  * null stream).  Each entry point names the reference interface it replaces.
  *
+ * Calls on one numeric handle are ordered on the GPU: each call's stream
+ * waits for the work of the previous call on that handle (an internal
+ * event), so a handle can be used from several streams, one host thread at
+ * a time; its calls then do not overlap on the GPU.  Host copies out of the
+ * handle (mdgpu_export_L, mdgpu_selinv_export) wait for its GPU work.
+ *
  * Life cycle:
  *   mdgpu_analyze   -> symbolic handle (host: ordering, trees, front maps)
  *   mdgpu_numeric_create(sym, precision) -> numeric handle (device arena)
@@ -229,6 +235,16 @@ int mdgpu_solve_nrhs(mdgpu_numeric* num, const double* d_B, double* d_X, int nrh
 int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol, int maxit,
                  double* eta, int* iters, void* stream);
 
+/* Status of the solves run since the last call: MDGPU_SOLVE_TIMED_OUT is set
+ * when a wait inside a chained block solve gave up (about a second of
+ * polling for a value that never came): the solution then holds NaN where
+ * it depends on that value.  mdgpu_solve / mdgpu_solve_nrhs return before
+ * the GPU finishes and cannot report it; mdgpu_refine checks it itself and
+ * returns MDGPU_ERR_DEVICE.  Reading clears the bits.  Synchronizes
+ * `stream`.  No counterpart in the reference. */
+#define MDGPU_SOLVE_TIMED_OUT 1
+int mdgpu_solve_status(mdgpu_numeric* num, int* status, void* stream);
+
 /* Backward error of each independent matrix (MDGPU_SYM_NSEG of them) after
  * the last mdgpu_refine; for a batch, *eta of mdgpu_refine is their max.
  * No counterpart in the reference. */
@@ -264,6 +280,12 @@ int mdgpu_logdet(mdgpu_numeric* num, double* out, void* stream);
  * of its front from j down, diagonal first, ascending (mdgpu.cu:1346-1352,
  * 1859-1873).  Lp has n+1 entries, Li/Lx have lnz; any may be NULL. */
 int mdgpu_export_L(mdgpu_numeric* num, int32_t* Lp, int32_t* Li, double* Lx);
+
+/* mdgpu_export_L for matrix k of a batch (mdgpu_analyze_batch): its own
+ * columns, in its own numbering (rows and column pointers relative to
+ * MDGPU_ARR_SEG_COL[k]); Lp has n_k + 1 entries.  Works when the whole
+ * batch's L exceeds int32 entries.  No counterpart in the reference. */
+int mdgpu_export_L_segment(mdgpu_numeric* num, int seg, int32_t* Lp, int32_t* Li, double* Lx);
 
 #ifdef __cplusplus
 }

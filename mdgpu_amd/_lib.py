@@ -18,6 +18,7 @@ MDGPU_ERR_NOT_SPD = 2
 MDGPU_ERR_DEVICE = 3
 MDGPU_ERR_NOMEM = 4
 MDGPU_ERR_NOT_CONVERGED = 5
+SOLVE_TIMED_OUT = 1        # MDGPU_SOLVE_TIMED_OUT
 
 FP64, FP32, BF16, BF16X3 = 0, 1, 2, 3
 PART_PLAIN, PART_SUPERNODAL, PART_RELAXED = 0, 1, 2
@@ -115,6 +116,8 @@ _SIGNATURES.update({
     "mdgpu_refine_segment_eta": (C.c_int, [_P, _DP, C.c_int]),
     "mdgpu_logdet": (C.c_int, [_P, _DP, _P]),
     "mdgpu_export_L": (C.c_int, [_P, _I32P, _I32P, _DP]),
+    "mdgpu_export_L_segment": (C.c_int, [_P, C.c_int, _I32P, _I32P, _DP]),
+    "mdgpu_solve_status": (C.c_int, [_P, C.POINTER(C.c_int), _P]),
     "mdgpu_selinv": (C.c_int, [_P, _P]),
     "mdgpu_selinv_export": (C.c_int, [_P, _DP]),
     "mdgpu_logdet_grad": (C.c_int, [_P, _P, _P]),

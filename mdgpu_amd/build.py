@@ -41,6 +41,12 @@ def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     jobs = []
     for src in sorted(glob.glob(os.path.join(CSRC, "*.cpp"))):
+        if os.path.basename(src) == "dropin.cpp":
+            # the C++ drop-in once per FLOTANTE (its symbols carry it)
+            for t in ("float", "double"):
+                obj = os.path.join(OBJ, f"dropin.{t}.o")
+                jobs.append(["g++", *HOST_FLAGS, f"-DFLOTANTE={t}", "-c", src, "-o", obj])
+            continue
         obj = os.path.join(OBJ, os.path.basename(src) + ".o")
         jobs.append(["g++", *HOST_FLAGS, "-c", src, "-o", obj])
     for src in sorted(glob.glob(os.path.join(CSRC, "*.hip"))):
@@ -55,7 +61,8 @@ def build(verbose=False):
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
     shutil.move(tmp, OUT)
     # command-line driver over the drop-in C++ interface
-    _run(["g++", "-O2", "-std=c++17", os.path.join(HERE, "cli", "mdgpu_main.cpp"), "-o", CLI,
+    # (FP64 drop-in, like the reference driver built with FLOTANTE = double)
+    _run(["g++", "-O2", "-std=c++17", "-DFLOTANTE=double", os.path.join(HERE, "cli", "mdgpu_main.cpp"), "-o", CLI,
           f"-L{HERE}", "-lmdgpu_amd", "-Wl,-rpath,$ORIGIN"])
     return OUT
 

@@ -181,5 +181,7 @@ int main(int argc, char** argv) {
     cs_gaxpy(Cfull, x2.data(), resid.data());
     const double eta = inf_norm(resid.data(), n) / (cs_norm(Cfull) * inf_norm(x2.data(), n) + inf_norm(b.data(), n));
     printf("resid: %8.2e\n", eta);
-    return eta < 1e-10 ? 0 : 3;
+    // FLOTANTE = float factors and solves in float: its backward error is
+    // of the order of float's rounding
+    return eta < (sizeof(FLOTANTE) == sizeof(float) ? 1e-5 : 1e-10) ? 0 : 3;
 }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -18,6 +19,13 @@
 #include "../../include/mdgpu_c.h"
 #include "common.h"
 #include "symbolic.h"
+
+// Built once per FLOTANTE (float and double): every definition below lives
+// in the inline namespace of that FLOTANTE (include/types.h).
+static_assert(std::is_same<FLOTANTE, float>::value || std::is_same<FLOTANTE, double>::value,
+              "the drop-in is built for FLOTANTE = float or double");
+
+MDGPU_DROPIN_BEGIN
 
 bool GPU = true;
 int block_size = -1;
@@ -115,14 +123,20 @@ int cs_entry(cs* T, int i, int j, FLOTANTE x) {
     return 1;
 }
 
+// one "i j x" line, the value in FLOTANTE's own format (cs.cpp:12-18)
+namespace {
+bool read_triplet(FILE* f, int* i, int* j, float* x) { return fscanf(f, "%d %d %g\n", i, j, x) == 3; }
+bool read_triplet(FILE* f, int* i, int* j, double* x) { return fscanf(f, "%d %d %lg\n", i, j, x) == 3; }
+}  // namespace
+
 // cs_load (cs.cpp:4-27): headerless "i j x" lines, 1-based
 cs* cs_load(FILE* f) {
     if (!f) return NULL;
     cs* T = cs_spalloc(0, 0, 1, 1, 1);
     int i, j;
-    double x;
-    while (T && fscanf(f, "%d %d %lg\n", &i, &j, &x) == 3)
-        if (!cs_entry(T, i - 1, j - 1, (FLOTANTE)x)) return cs_spfree(T);
+    FLOTANTE x;
+    while (T && read_triplet(f, &i, &j, &x))
+        if (!cs_entry(T, i - 1, j - 1, x)) return cs_spfree(T);
     return T;
 }
 
@@ -632,8 +646,9 @@ void MatrizSpy2(Matriz* A, Matriz* L) {
 
 // ------------------------------------------------------------ Factor
 // Numeric factorization of the permuted lower triangle A over the node list
-// (mdgpu.cu:1394-1608), on the GPU, FP64; L is written into spL (layout of
-// mdgpu.cu:1346-1352).
+// (mdgpu.cu:1394-1608), on the GPU in FLOTANTE's precision (MDGPU_FP32 for
+// float, MDGPU_FP64 for double, as the reference computes everything in
+// FLOTANTE); L is written into spL (layout of mdgpu.cu:1346-1352).
 void Factor(cs* A, cs* spL, struct ETree** listaNodos, int cantNodos) {
     auto report = [](const char* what) {
         fprintf(stderr, "Factor: %s: %s\n", what, mdgpu_last_error());
@@ -679,7 +694,8 @@ void Factor(cs* A, cs* spL, struct ETree** listaNodos, int cantNodos) {
     mdgpu_numeric* num = nullptr;
     double* d_ax = nullptr;
     const size_t nz = (size_t)A->p[n];
-    int st = mdgpu_numeric_create(sym, MDGPU_FP64, &num);
+    const int precision = std::is_same<FLOTANTE, float>::value ? MDGPU_FP32 : MDGPU_FP64;
+    int st = mdgpu_numeric_create(sym, precision, &num);
     if (st == MDGPU_OK && hipMalloc((void**)&d_ax, sizeof(double) * std::max<size_t>(nz, 1)) != hipSuccess)
         st = mdgpu::fail(MDGPU_ERR_NOMEM, "Factor: hipMalloc failed");
     if (st == MDGPU_OK) {
@@ -701,3 +717,5 @@ void Factor(cs* A, cs* spL, struct ETree** listaNodos, int cantNodos) {
     mdgpu_numeric_destroy(num);
     mdgpu_symbolic_destroy(sym);
 }
+
+MDGPU_DROPIN_END

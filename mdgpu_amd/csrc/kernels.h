@@ -42,20 +42,30 @@ void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, f
 // backward result to xp
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s);
 void launch_permute_out(const double* xp, const int* perm, double* x, int n, hipStream_t s);
+// Capacity (entries) of the LDS copies of a front vector in the forward
+// solve: small fronts (launch_fwd) and the gather of large fronts
+// (launch_fwd_gather).  vec_cap (at most the capacity) is the longest
+// front vector kept in LDS; longer ones are accumulated in place in global
+// memory.
+constexpr int kSmallVec = 2048, kGatherVec = 16384;
 template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                 const int* q, const T* L, const T* inv, double* xp, double* Y, double* su, const int64_t* suofs,
-                hipStream_t s);
+                int vec_cap, hipStream_t s);
 template <typename T>
 void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
                 double* xp, const double* Y, hipStream_t s);
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
-                       double* xp, double* su, const int64_t* suofs, hipStream_t s);
+                       double* xp, double* su, const int64_t* suofs, int vec_cap, hipStream_t s);
 // forward chain over a level's large fronts (ctr: the launch's ticket
-// counter, zero; Y holds kChainEmpty where the chain stores y)
+// counter, zero; Y holds kChainEmpty where the chain stores y); a wait that
+// gives up after `spin` polls (kChainSpin: about a second) sets
+// kSolveTimedOut in *status
+constexpr int kSolveTimedOut = 1;
+constexpr int kChainSpin = 1 << 20;
 template <typename T>
-void launch_fwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, const double* xp,
-                      double* Y, double* su, hipStream_t s);
+void launch_fwd_chain(const SolveItem* items, int count, int* ctr, int* status, int spin, const T* L, const T* X256,
+                      const double* xp, double* Y, double* su, hipStream_t s);
 // backward: BWD_UPD writes the update rows' dot products to partial-sum
 // slots, BWD_UPD_SUM subtracts them from Y, then the chain solves the blocks
 // into xp (which holds kChainEmpty there)
@@ -65,8 +75,8 @@ void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, 
 void launch_bwd_upd_sum(const SolveItem* items, int count, const FrontDesc* fronts, const double* partial,
                         double* Y, hipStream_t s);
 template <typename T>
-void launch_bwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, double* xp,
-                      const double* Y, hipStream_t s);
+void launch_bwd_chain(const SolveItem* items, int count, int* ctr, int* status, int spin, const T* L, const T* X256,
+                      double* xp, const double* Y, hipStream_t s);
 // inverses of the kSolveBW-column diagonal blocks (Plan::inv256), after
 // launch_inv
 template <typename T>

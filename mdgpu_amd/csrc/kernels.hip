@@ -65,17 +65,21 @@ __device__ __forceinline__ void keep_in_flight(T (&v)[N]) {
 // signalling NaN, which arithmetic never produces (operations on a NaN give
 // a quiet NaN).
 constexpr unsigned long long kChainEmpty = kChainEmptyBits;
-constexpr int kChainSpin = 1 << 20;   // polls of one value before giving up on it
 
 // *p once it no longer holds kChainEmpty, read with agent-scope atomic
 // loads (another workgroup, possibly on another XCD, stores it with
-// chain_store).  After kChainSpin polls the wait ends with a quiet NaN,
-// which then shows in every result that depends on the value.
-__device__ __forceinline__ double chain_wait(const double* p) {
+// chain_store).  After `spin` polls (kChainSpin, about a second) the wait
+// ends with a quiet NaN, which then shows in every result that depends on
+// the value, and sets bit kSolveTimedOut of *status (read by
+// mdgpu_solve_status / mdgpu_refine).
+__device__ __forceinline__ double chain_wait(const double* p, int* status, int spin) {
     const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
     unsigned long long v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int s = 0; v == kChainEmpty; ++s) {
-        if (s >= kChainSpin) return __builtin_nan("");
+        if (s >= spin) {
+            __hip_atomic_fetch_or(status, kSolveTimedOut, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return __builtin_nan("");
+        }
         __builtin_amdgcn_s_sleep(2);
         v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -1248,10 +1252,10 @@ __device__ __forceinline__ double column_sums64(double (&acc)[32], double* redc,
 }
 
 // Fronts with at most kSolveSmall pivots (one diagonal block), one
-// workgroup per front and one launch per level.
-// Front vectors of at most kSmallVec entries live in LDS while the
-// children's update vectors are added.
-constexpr int kSmallVec = 2048;
+// workgroup per front and one launch per level.  Front vectors of at most
+// vec_cap (<= kSmallVec) entries live in LDS while the children's update
+// vectors are added; longer ones are accumulated in place in xp and the
+// update vector.
 
 // Forward substitution L y = x over one level, multifrontal form of
 // cs_lsolve (cs.cpp:1086-1101): the front vector (pivot entries from xp,
@@ -1267,7 +1271,7 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
                                                             const T* __restrict__ L, const T* __restrict__ inv,
                                                             double* __restrict__ xp, double* __restrict__ Y,
                                                             double* __restrict__ su,
-                                                            const int64_t* __restrict__ suofs) {
+                                                            const int64_t* __restrict__ suofs, int vec_cap) {
     __shared__ double fv[kSmallVec];
     __shared__ double yb[64];
     __shared__ double red[4 * 64];
@@ -1276,7 +1280,7 @@ __global__ __launch_bounds__(kThreads) void fwd_kernel(const int* __restrict__ l
     const int n = d.n, c = d.c, k = d.k, tid = threadIdx.x;
     const T* __restrict__ P = L + d.lofs;
     double* __restrict__ u = su + suofs[f];
-    const bool inl = n <= kSmallVec;
+    const bool inl = n <= vec_cap;
     for (int t = tid; t < n; t += kThreads) {
         if (inl) fv[t] = t < c ? xp[k + t] : 0.0;
         else if (t >= c) u[t - c] = 0.0;
@@ -1399,21 +1403,21 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_kernel(const int* __restric
 // Forward, step 0 of a large front: its front vector (pivot entries from
 // xp, update entries zero) gathers the children's update vectors through
 // their relative maps, child by child (the first phase of fwd_kernel).
-// Fronts of at most kGatherVec rows accumulate in LDS; each thread keeps
-// four entries of a child in flight.
-constexpr int kGatherThreads = 1024, kGatherVec = 16384;
+// Fronts of at most vec_cap (<= kGatherVec) rows accumulate in LDS, longer
+// ones in place; each thread keeps four entries of a child in flight.
+constexpr int kGatherThreads = 1024;
 __global__ __launch_bounds__(kGatherThreads) void fwd_gather_kernel(const int* __restrict__ list,
                                                                     const FrontDesc* __restrict__ fronts,
                                                                     const int* __restrict__ kids,
                                                                     const int* __restrict__ rel,
                                                                     double* __restrict__ xp, double* __restrict__ su,
-                                                                    const int64_t* __restrict__ suofs) {
+                                                                    const int64_t* __restrict__ suofs, int vec_cap) {
     __shared__ double fv[kGatherVec];
     const int f = list[blockIdx.x];
     const FrontDesc d = fronts[f];
     const int n = d.n, c = d.c, k = d.k, tid = threadIdx.x;
     double* __restrict__ u = su + suofs[f];
-    const bool inl = n <= kGatherVec;
+    const bool inl = n <= vec_cap;
     for (int t = tid; t < n; t += kGatherThreads) {
         if (inl) fv[t] = t < c ? xp[k + t] : 0.0;
         else if (t >= c) u[t - c] = 0.0;
@@ -1483,7 +1487,8 @@ __device__ __forceinline__ void chain_tile(T (&lo)[16], T (&hi)[16], const T* __
 //   RECT: u -= sum_j L_rj y_j.
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void fwd_chain_kernel(const SolveItem* __restrict__ items,
-                                                                  int* __restrict__ ctr, const T* __restrict__ L,
+                                                                  int* __restrict__ ctr, int* __restrict__ status,
+                                                                  int spin, const T* __restrict__ L,
                                                                   const T* __restrict__ X256,
                                                                   const double* __restrict__ xp,
                                                                   double* __restrict__ Y, double* __restrict__ su) {
@@ -1508,7 +1513,7 @@ __global__ __launch_bounds__(kSolveThreads) void fwd_chain_kernel(const SolveIte
     // first 128 threads
     auto apply = [&](int j, const T (&tl)[16], const T (&th)[16]) {
         double* yb = yl[j & 1];
-        if (tid < BW) yb[tid] = tid < c - BW * j ? chain_wait(Y + it.k + BW * j + tid) : 0.0;
+        if (tid < BW) yb[tid] = tid < c - BW * j ? chain_wait(Y + it.k + BW * j + tid, status, spin) : 0.0;
         __syncthreads();
         double s0 = 0.0, s1 = 0.0;
 #pragma unroll
@@ -1672,7 +1677,8 @@ __device__ __forceinline__ void wave_transpose_sum16(double (&acc)[16], int lane
 // stored in xp (chain_store).
 template <typename T>
 __global__ __launch_bounds__(kSolveThreads) void bwd_chain_kernel(const SolveItem* __restrict__ items,
-                                                                  int* __restrict__ ctr, const T* __restrict__ L,
+                                                                  int* __restrict__ ctr, int* __restrict__ status,
+                                                                  int spin, const T* __restrict__ L,
                                                                   const T* __restrict__ X256, double* __restrict__ xp,
                                                                   const double* __restrict__ Y) {
     constexpr int BW = kChainBW;
@@ -1695,7 +1701,7 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_chain_kernel(const SolveIte
     // first 128 threads
     auto apply = [&](int j, const T (&tl)[16], const T (&th)[16]) {
         double* xb = xl[j & 1];
-        if (tid < BW) xb[tid] = tid < c - BW * j ? chain_wait(xp + it.k + BW * j + tid) : 0.0;
+        if (tid < BW) xb[tid] = tid < c - BW * j ? chain_wait(xp + it.k + BW * j + tid, status, spin) : 0.0;
         __syncthreads();
         const double x0 = xb[lane], x1 = xb[64 + lane];
 #pragma unroll
@@ -1908,8 +1914,10 @@ void launch_permute_out(const double* xp, const int* perm, double* x, int n, hip
 template <typename T>
 void launch_fwd(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
                 const int* q, const T* L, const T* inv, double* xp, double* Y, double* su, const int64_t* suofs,
-                hipStream_t s) {
-    if (count > 0) fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, Y, su, suofs);
+                int vec_cap, hipStream_t s) {
+    if (count > 0)
+        fwd_kernel<T><<<count, kThreads, 0, s>>>(list, fronts, kids, rel, q, L, inv, xp, Y, su, suofs,
+                                                 min(vec_cap, kSmallVec));
 }
 template <typename T>
 void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* q, const T* L, const T* inv,
@@ -1918,13 +1926,15 @@ void launch_bwd(const int* list, int count, const FrontDesc* fronts, const int* 
 }
 
 void launch_fwd_gather(const int* list, int count, const FrontDesc* fronts, const int* kids, const int* rel,
-                       double* xp, double* su, const int64_t* suofs, hipStream_t s) {
-    if (count > 0) fwd_gather_kernel<<<count, kGatherThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs);
+                       double* xp, double* su, const int64_t* suofs, int vec_cap, hipStream_t s) {
+    if (count > 0)
+        fwd_gather_kernel<<<count, kGatherThreads, 0, s>>>(list, fronts, kids, rel, xp, su, suofs,
+                                                           min(vec_cap, kGatherVec));
 }
 template <typename T>
-void launch_fwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, const double* xp,
-                      double* Y, double* su, hipStream_t s) {
-    if (count > 0) fwd_chain_kernel<T><<<count, kSolveThreads, 0, s>>>(items, ctr, L, X256, xp, Y, su);
+void launch_fwd_chain(const SolveItem* items, int count, int* ctr, int* status, int spin, const T* L, const T* X256,
+                      const double* xp, double* Y, double* su, hipStream_t s) {
+    if (count > 0) fwd_chain_kernel<T><<<count, kSolveThreads, 0, s>>>(items, ctr, status, spin, L, X256, xp, Y, su);
 }
 template <typename T>
 void launch_bwd_upd(const SolveItem* items, int count, const FrontDesc* fronts, const int* q, const T* L,
@@ -1936,9 +1946,9 @@ void launch_bwd_upd_sum(const SolveItem* items, int count, const FrontDesc* fron
     if (count > 0) bwd_upd_sum_kernel<<<count, kSolveCols, 0, s>>>(items, partial, Y);
 }
 template <typename T>
-void launch_bwd_chain(const SolveItem* items, int count, int* ctr, const T* L, const T* X256, double* xp,
-                      const double* Y, hipStream_t s) {
-    if (count > 0) bwd_chain_kernel<T><<<count, kSolveThreads, 0, s>>>(items, ctr, L, X256, xp, Y);
+void launch_bwd_chain(const SolveItem* items, int count, int* ctr, int* status, int spin, const T* L, const T* X256,
+                      double* xp, const double* Y, hipStream_t s) {
+    if (count > 0) bwd_chain_kernel<T><<<count, kSolveThreads, 0, s>>>(items, ctr, status, spin, L, X256, xp, Y);
 }
 template <typename T>
 void launch_inv256(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, const T* inv, T* X256,
@@ -1986,15 +1996,16 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
-                                const T*, const T*, double*, double*, double*, const int64_t*, hipStream_t);   \
+                                const T*, const T*, double*, double*, double*, const int64_t*, int,            \
+                                hipStream_t);                                                                  \
     template void launch_bwd<T>(const int*, int, const FrontDesc*, const int*, const T*, const T*, double*,    \
                                 const double*, hipStream_t);                                                   \
     template void launch_logdiag<T>(const int64_t*, const T*, int, double*, int, hipStream_t);               \
-    template void launch_fwd_chain<T>(const SolveItem*, int, int*, const T*, const T*, const double*,        \
+    template void launch_fwd_chain<T>(const SolveItem*, int, int*, int*, int, const T*, const T*, const double*, \
                                       double*, double*, hipStream_t);                                        \
     template void launch_bwd_upd<T>(const SolveItem*, int, const FrontDesc*, const int*, const T*,           \
                                     const double*, double*, hipStream_t);                                    \
-    template void launch_bwd_chain<T>(const SolveItem*, int, int*, const T*, const T*, double*,              \
+    template void launch_bwd_chain<T>(const SolveItem*, int, int*, int*, int, const T*, const T*, double*,   \
                                       const double*, hipStream_t);
 
 MDGPU_INSTANTIATE(double)

@@ -74,7 +74,14 @@ struct mdgpu_numeric {
     double* dRowabs = nullptr;
     int64_t* dDiagOff = nullptr;
     int* dInfo = nullptr;
-    unsigned long long* dMax = nullptr;   // three slots per segment (max_abs_n)
+    unsigned long long* dMax = nullptr;   // three slots per segment (max_abs_n), then the solve status
+    int* dStatus = nullptr;               // solve status bits (kSolveTimedOut), sticky until read
+    int vec_small = mdgpu::kSmallVec;     // longest front vector kept in LDS by the forward solve
+    int vec_gather = mdgpu::kGatherVec;
+    int chain_spin = mdgpu::kChainSpin;   // polls of a chained-solve wait before it gives up
+    // GPU-side order of the calls on this handle (CallOrder)
+    hipEvent_t done = nullptr;
+    bool done_recorded = false;
     int nseg = 1;
     int* dSeg = nullptr;                  // row -> segment (batches only)
     double* dPartial = nullptr;
@@ -161,6 +168,8 @@ void release(mdgpu_numeric* N) {
     N->side = nullptr;
     for (hipEvent_t e : N->events) (void)hipEventDestroy(e);
     N->events.clear();
+    if (N->done) (void)hipEventDestroy(N->done);
+    N->done = nullptr;
     for (auto& b : N->bufs) (void)hipFree(b.p);
     N->bufs.clear();
 }
@@ -189,6 +198,28 @@ int capture(mdgpu_numeric* N, hipGraphExec_t* out, F body) {
         int _st = (x);                \
         if (_st != MDGPU_OK) return _st; \
     } while (0)
+
+// The work of the calls on one handle runs on the GPU in call order,
+// whatever streams the calls use (they share the handle's buffers: the L
+// store, the solve vectors, the chain counters): each call's stream first
+// waits for the event the previous call recorded, and the call records it
+// again on its own stream when it returns.
+struct CallOrder {
+    mdgpu_numeric* N;
+    hipStream_t s;
+    CallOrder(mdgpu_numeric* n, hipStream_t st) : N(n), s(st) {
+        if (N->done_recorded) (void)hipStreamWaitEvent(s, N->done, 0);
+    }
+    ~CallOrder() {
+        if (hipEventRecord(N->done, s) == hipSuccess) N->done_recorded = true;
+    }
+};
+// Host copies out of the handle's buffers wait for all of its GPU work.
+int wait_handle(mdgpu_numeric* N) {
+    if (!N->done_recorded) return MDGPU_OK;
+    hipError_t e = hipEventSynchronize(N->done);
+    return e == hipSuccess ? MDGPU_OK : hip_fail(e, "hipEventSynchronize");
+}
 
 // Kernel classes reported by mdgpu_factorize_profile.
 enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_POTRF_TRSM, KIND_INVERSE, KIND_GEMM, KIND_N };
@@ -283,14 +314,15 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
         switch (st.kind) {
             case mdgpu::SV_FWD_SMALL:
                 mdgpu::launch_fwd<T>(list, st.count, N->dFronts, N->dKids, N->dRel, N->dQ, L, inv, N->dXp, N->dY,
-                                     N->dSu, N->dSuofs, s);
+                                     N->dSu, N->dSuofs, N->vec_small, s);
                 break;
             case mdgpu::SV_FWD_GATHER:
                 mdgpu::launch_fwd_gather(list, st.count, N->dFronts, N->dKids, N->dRel, N->dXp, N->dSu, N->dSuofs,
-                                         s);
+                                         N->vec_gather, s);
                 break;
             case mdgpu::SV_FWD_CHAIN:
-                mdgpu::launch_fwd_chain<T>(items, st.count, N->dChainCtr + st.ctr, L, X256, N->dXp, N->dY, N->dSu, s);
+                mdgpu::launch_fwd_chain<T>(items, st.count, N->dChainCtr + st.ctr, N->dStatus, N->chain_spin, L, X256,
+                                           N->dXp, N->dY, N->dSu, s);
                 break;
         }
     }
@@ -311,7 +343,8 @@ int run_solve(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
                 mdgpu::launch_bwd_upd_sum(items, st.count, N->dFronts, N->dUpdPartial, N->dY, s);
                 break;
             case mdgpu::SV_BWD_CHAIN:
-                mdgpu::launch_bwd_chain<T>(items, st.count, N->dChainCtr + st.ctr, L, X256, N->dXp, N->dY, s);
+                mdgpu::launch_bwd_chain<T>(items, st.count, N->dChainCtr + st.ctr, N->dStatus, N->chain_spin, L, X256,
+                                           N->dXp, N->dY, s);
                 break;
         }
     }
@@ -341,16 +374,17 @@ int solve_any(mdgpu_numeric* N, const double* b, double* x, hipStream_t s) {
 }
 
 // max |v| per segment of each of the vectors v[0..nv) (nv <= 3), one copy
-// to the host and one synchronization of the stream for all of them
+// to the host and one synchronization of the stream for all of them; the
+// same copy brings the solve status bits (*status, if not null)
 int max_abs_n(mdgpu_numeric* N, const double* const* v, int nv, int n, hipStream_t s,
-              std::vector<double>* const* out) {
+              std::vector<double>* const* out, int* status = nullptr) {
     const int ns = N->nseg;
     mdgpu::launch_fill<unsigned long long>(N->dMax, (int64_t)nv * ns, 0ULL, s);
     for (int t = 0; t < nv; t++) {
         if (ns == 1) mdgpu::launch_maxabs(v[t], n, N->dMax + t, s);
         else mdgpu::launch_maxabs_seg(v[t], N->dSeg, n, N->dMax + (int64_t)t * ns, s);
     }
-    std::vector<unsigned long long> bits((size_t)nv * ns, 0);
+    std::vector<unsigned long long> bits((size_t)3 * ns + 1, 0);
     hipError_t e = hipMemcpyAsync(bits.data(), N->dMax, bits.size() * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
@@ -360,6 +394,7 @@ int max_abs_n(mdgpu_numeric* N, const double* const* v, int nv, int n, hipStream
         out[t]->resize(ns);
         for (int k = 0; k < ns; k++) memcpy(&(*out[t])[k], &bits[(size_t)t * ns + k], sizeof(double));
     }
+    if (status) memcpy(status, &bits[(size_t)3 * ns], sizeof(int));
     return MDGPU_OK;
 }
 // max |v| per segment (synchronizes the stream)
@@ -460,7 +495,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dRowabs)) ||
         (st = upload(N, diag_off, (void**)&N->dDiagOff)) ||
         (st = alloc(N, 2 * sizeof(int), (void**)&N->dInfo)) ||
-        (st = alloc(N, 3 * N->nseg * sizeof(unsigned long long), (void**)&N->dMax)) ||
+        (st = alloc(N, (3 * N->nseg + 1) * sizeof(unsigned long long), (void**)&N->dMax)) ||
         (N->nseg > 1 && (st = upload(N, seg_of_row, (void**)&N->dSeg))) ||
         (st = alloc(N, kLogParts * sizeof(double), (void**)&N->dPartial)) ||
         (st = alloc(N, std::max<size_t>(P.a_dst.size(), 1) * sizeof(double), (void**)&N->dAx)) ||
@@ -473,6 +508,21 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&N->side, hipStreamNonBlocking) != hipSuccess)
         return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: hipStreamCreate failed"));
+    N->dStatus = reinterpret_cast<int*>(N->dMax + 3 * N->nseg);
+    if (hipMemset(N->dMax, 0, (3 * N->nseg + 1) * sizeof(unsigned long long)) != hipSuccess ||
+        hipEventCreateWithFlags(&N->done, hipEventDisableTiming) != hipSuccess)
+        return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: status word / event set-up failed"));
+    // longest front vector the forward solve keeps in LDS (default: the
+    // kernels' capacity; a smaller value sends longer fronts through the
+    // in-place global-memory path)
+    if (const char* e = std::getenv("MDGPU_SOLVE_LDS_ROWS")) {
+        const int v = std::max(0, std::atoi(e));
+        N->vec_small = std::min(v, mdgpu::kSmallVec);
+        N->vec_gather = std::min(v, mdgpu::kGatherVec);
+    }
+    // polls before a chained-solve wait gives up (a smaller value makes the
+    // give-up path observable without a stalled producer)
+    if (const char* e = std::getenv("MDGPU_CHAIN_SPIN")) N->chain_spin = std::max(0, std::atoi(e));
     N->events.assign(P.nevents, nullptr);
     for (hipEvent_t& e : N->events)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
@@ -517,6 +567,7 @@ extern "C" int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* str
     mdgpu::clear_error();
     if (!num || !d_ax) return fail(MDGPU_ERR_ARG, "mdgpu_factorize: null pointer");
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(num, s);
     num->factored = false;
     int st;
     if (num->use_graph) {
@@ -568,6 +619,7 @@ int factor_timed(mdgpu_numeric* num, const double* d_ax, hipStream_t s, std::vec
 extern "C" int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream, double* out) {
     mdgpu::clear_error();
     if (!num || !d_ax || !out) return fail(MDGPU_ERR_ARG, "mdgpu_factorize_profile: null pointer");
+    CallOrder order(num, (hipStream_t)stream);
     std::vector<float> ms;
     TRY(factor_timed(num, d_ax, (hipStream_t)stream, ms));
     const mdgpu::Plan& P = num->plan;
@@ -593,6 +645,7 @@ extern "C" int mdgpu_factorize_steps(mdgpu_numeric* num, const double* d_ax, voi
     const mdgpu::Plan& P = num->plan;
     if (cap < (int64_t)P.steps.size() * 6)
         return fail(MDGPU_ERR_ARG, "mdgpu_factorize_steps: capacity below 6 * MDGPU_NUM_STEPS");
+    CallOrder order(num, (hipStream_t)stream);
     std::vector<float> ms;
     TRY(factor_timed(num, d_ax, (hipStream_t)stream, ms));
     for (size_t t = 0; t < P.steps.size(); t++) {
@@ -635,6 +688,7 @@ extern "C" int mdgpu_solve(mdgpu_numeric* num, const double* d_b, double* d_x, v
     mdgpu::clear_error();
     if (!num || !d_b || !d_x) return fail(MDGPU_ERR_ARG, "mdgpu_solve: null pointer");
     if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_solve: no valid factorization");
+    CallOrder order(num, (hipStream_t)stream);
     return solve_any(num, d_b, d_x, (hipStream_t)stream);
 }
 
@@ -644,6 +698,7 @@ extern "C" int mdgpu_solve_nrhs(mdgpu_numeric* num, const double* d_B, double* d
     if (!num || (nrhs > 0 && (!d_B || !d_X))) return fail(MDGPU_ERR_ARG, "mdgpu_solve_nrhs: null pointer");
     if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_solve_nrhs: no valid factorization");
     if (nrhs < 0 || ld < num->sym->S.n) return fail(MDGPU_ERR_ARG, "mdgpu_solve_nrhs: nrhs < 0 or ld < n");
+    CallOrder order(num, (hipStream_t)stream);
     for (int r = 0; r < nrhs; r++) TRY(solve_any(num, d_B + (int64_t)r * ld, d_X + (int64_t)r * ld, (hipStream_t)stream));
     return MDGPU_OK;
 }
@@ -654,6 +709,7 @@ extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, 
     if (!num || !d_b || !d_x) return fail(MDGPU_ERR_ARG, "mdgpu_refine: null pointer");
     if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_refine: no valid factorization");
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(num, s);
     const int n = num->sym->S.n;
     TRY(solve_any(num, d_b, d_x, s));
     std::vector<double> bn, rn, xn;
@@ -662,11 +718,17 @@ extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, 
     num->eta_seg.assign(num->nseg, INFINITY);
     for (;; it++) {
         mdgpu::launch_residual(num->dAfp, num->dAfi, num->dAfx, d_x, d_b, num->dR, n, s);
+        int status = 0;
         {
             // |b| with the first step's |r| and |x|: one round trip per step
             const double* vs[3] = {num->dR, d_x, d_b};
             std::vector<double>* os[3] = {&rn, &xn, &bn};
-            TRY(max_abs_n(num, vs, it == 0 ? 3 : 2, n, s, os));
+            TRY(max_abs_n(num, vs, it == 0 ? 3 : 2, n, s, os, &status));
+        }
+        if (status & mdgpu::kSolveTimedOut) {
+            mdgpu::launch_fill<int>(num->dStatus, 1, 0, s);
+            return fail(MDGPU_ERR_DEVICE, "mdgpu_refine: a chained solve gave up waiting for a block "
+                                          "(x holds NaN)");
         }
         // backward error of the reference's check (mdgpu.cu:1972), per
         // independent matrix of a batch; the worst one decides
@@ -701,11 +763,26 @@ extern "C" int mdgpu_refine_segment_eta(const mdgpu_numeric* num, double* out, i
     return MDGPU_OK;
 }
 
+extern "C" int mdgpu_solve_status(mdgpu_numeric* num, int* status, void* stream) {
+    mdgpu::clear_error();
+    if (!num || !status) return fail(MDGPU_ERR_ARG, "mdgpu_solve_status: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    CallOrder order(num, s);
+    int v = 0;
+    hipError_t e = hipMemcpyAsync(&v, num->dStatus, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "mdgpu_solve_status");
+    if (v) mdgpu::launch_fill<int>(num->dStatus, 1, 0, s);
+    *status = v;
+    return MDGPU_OK;
+}
+
 extern "C" int mdgpu_logdet(mdgpu_numeric* num, double* out, void* stream) {
     mdgpu::clear_error();
     if (!num || !out) return fail(MDGPU_ERR_ARG, "mdgpu_logdet: null pointer");
     if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_logdet: no valid factorization");
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(num, s);
     const int n = num->sym->S.n;
     if (num->precision == MDGPU_FP64)
         mdgpu::launch_logdiag<double>(num->dDiagOff, (const double*)num->dL, n, num->dPartial, num->nparts, s);
@@ -722,38 +799,71 @@ extern "C" int mdgpu_logdet(mdgpu_numeric* num, double* out, void* stream) {
     return MDGPU_OK;
 }
 
-extern "C" int mdgpu_export_L(mdgpu_numeric* num, int32_t* Lp, int32_t* Li, double* Lx) {
-    mdgpu::clear_error();
-    if (!num) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: null pointer");
-    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: no valid factorization");
+namespace {
+// L columns [c0, c1) (one matrix of a batch, or all of L) to host CSC in
+// the reference's spL layout, rows and column pointers relative to c0.
+int export_columns(mdgpu_numeric* num, int c0, int c1, int32_t* Lp, int32_t* Li, double* Lx, const char* who) {
     const mdgpu::Symbolic& S = num->sym->S;
     const mdgpu::Plan& P = num->plan;
-    if (S.lnz > INT32_MAX) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: nnz(L) exceeds int32");
-    std::vector<double> h(P.lstore);
-    hipError_t e;
-    if (num->precision == MDGPU_FP64) {
-        e = hipMemcpy(h.data(), num->dL, P.lstore * sizeof(double), hipMemcpyDeviceToHost);
-    } else {
-        std::vector<float> hf(P.lstore);
-        e = hipMemcpy(hf.data(), num->dL, P.lstore * sizeof(float), hipMemcpyDeviceToHost);
-        for (int64_t t = 0; t < P.lstore; t++) h[t] = hf[t];
-    }
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpy L");
-    // column k+j holds rows q_f[j..n_f) of its front, diagonal first
-    // (the spL write-out of FactorAux_CPU_3, mdgpu.cu:1346-1352)
-    if (Lp)
-        for (int k = 0; k <= S.n; k++) Lp[k] = S.lp[k];
+    std::vector<int64_t> lp(c1 - c0 + 1, 0);
+    for (int k = c0; k < c1; k++) lp[k - c0 + 1] = lp[k - c0] + S.count_final[k];
+    if (lp.back() > INT32_MAX) return fail(MDGPU_ERR_ARG, std::string(who) + ": nnz(L) exceeds int32");
+    // the L-store range of the fronts in [c0, c1)
+    int64_t lo = INT64_MAX, hi = 0;
     for (int f = 0; f < P.nfronts; f++) {
         const mdgpu::FrontDesc& d = P.fronts[f];
+        if (d.k < c0 || d.k >= c1) continue;
+        lo = std::min(lo, d.lofs);
+        hi = std::max(hi, d.lofs + (int64_t)d.n * d.c);
+    }
+    if (lo > hi) lo = hi = 0;
+    std::vector<double> h(Lx ? hi - lo : 0);
+    hipError_t e = hipSuccess;
+    if (Lx && hi > lo) {
+        if (num->precision == MDGPU_FP64) {
+            e = hipMemcpy(h.data(), (const double*)num->dL + lo, (hi - lo) * sizeof(double), hipMemcpyDeviceToHost);
+        } else {
+            std::vector<float> hf(hi - lo);
+            e = hipMemcpy(hf.data(), (const float*)num->dL + lo, (hi - lo) * sizeof(float), hipMemcpyDeviceToHost);
+            for (int64_t t = 0; t < hi - lo; t++) h[t] = hf[t];
+        }
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy L");
+    if (Lp)
+        for (size_t k = 0; k < lp.size(); k++) Lp[k] = (int32_t)lp[k];
+    // column k+j holds rows q_f[j..n_f) of its front, diagonal first
+    // (the spL write-out of FactorAux_CPU_3, mdgpu.cu:1346-1352)
+    for (int f = 0; f < P.nfronts; f++) {
+        const mdgpu::FrontDesc& d = P.fronts[f];
+        if (d.k < c0 || d.k >= c1) continue;
         for (int j = 0; j < d.c; j++) {
-            int64_t dst = S.lp[d.k + j];
+            int64_t dst = lp[d.k + j - c0];
             for (int i = j; i < d.n; i++, dst++) {
-                if (Li) Li[dst] = S.q_all[d.qofs + i];
-                if (Lx) Lx[dst] = h[d.lofs + (int64_t)j * d.n + i];
+                if (Li) Li[dst] = S.q_all[d.qofs + i] - c0;
+                if (Lx) Lx[dst] = h[d.lofs - lo + (int64_t)j * d.n + i];
             }
         }
     }
     return MDGPU_OK;
+}
+}  // namespace
+
+extern "C" int mdgpu_export_L(mdgpu_numeric* num, int32_t* Lp, int32_t* Li, double* Lx) {
+    mdgpu::clear_error();
+    if (!num) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_export_L: no valid factorization");
+    TRY(wait_handle(num));
+    return export_columns(num, 0, num->sym->S.n, Lp, Li, Lx, "mdgpu_export_L");
+}
+
+extern "C" int mdgpu_export_L_segment(mdgpu_numeric* num, int seg, int32_t* Lp, int32_t* Li, double* Lx) {
+    mdgpu::clear_error();
+    if (!num) return fail(MDGPU_ERR_ARG, "mdgpu_export_L_segment: null pointer");
+    if (!num->factored) return fail(MDGPU_ERR_ARG, "mdgpu_export_L_segment: no valid factorization");
+    const std::vector<int>& sc = num->sym->seg_col;
+    if (seg < 0 || seg + 1 >= (int)sc.size()) return fail(MDGPU_ERR_ARG, "mdgpu_export_L_segment: no such matrix");
+    TRY(wait_handle(num));
+    return export_columns(num, sc[seg], sc[seg + 1], Lp, Li, Lx, "mdgpu_export_L_segment");
 }
 
 // ------------------------------------------------------------ reverse mode
@@ -801,6 +911,7 @@ extern "C" int mdgpu_selinv(mdgpu_numeric* num, void* stream) {
     TRY(check_fp64_factor(num, "mdgpu_selinv"));
     TRY(selinv_setup(num));
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(num, s);
     mdgpu::SelBuffers b;
     b.p[mdgpu::SB_L] = (double*)num->dL;
     b.p[mdgpu::SB_Z] = num->dZ;
@@ -826,6 +937,7 @@ extern "C" int mdgpu_selinv_export(mdgpu_numeric* num, double* Zx) {
     TRY(check_fp64_factor(num, "mdgpu_selinv_export"));
     if (!num->z_valid) return fail(MDGPU_ERR_ARG, "mdgpu_selinv_export: run mdgpu_selinv first");
     if (!Zx) return fail(MDGPU_ERR_ARG, "mdgpu_selinv_export: null pointer");
+    TRY(wait_handle(num));
     const mdgpu::Symbolic& S = num->sym->S;
     const mdgpu::Plan& P = num->plan;
     std::vector<double> h(P.lstore);
@@ -846,6 +958,7 @@ extern "C" int mdgpu_logdet_grad(mdgpu_numeric* num, double* d_g, void* stream) 
     TRY(check_fp64_factor(num, "mdgpu_logdet_grad"));
     if (!d_g) return fail(MDGPU_ERR_ARG, "mdgpu_logdet_grad: null pointer");
     if (!num->z_valid) TRY(mdgpu_selinv(num, stream));
+    CallOrder order(num, (hipStream_t)stream);
     mdgpu::launch_logdet_grad(num->dZ, num->dAdst, num->dArow, num->dAcol, d_g, num->sym->S.nnz_in,
                               (hipStream_t)stream);
     hipError_t e = hipGetLastError();
@@ -859,6 +972,7 @@ extern "C" int mdgpu_solve_adjoint(mdgpu_numeric* num, const double* d_x, const 
     if (!d_x || !d_xbar || !d_g) return fail(MDGPU_ERR_ARG, "mdgpu_solve_adjoint: null pointer");
     TRY(selinv_setup(num));
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(num, s);
     double* bb = d_bbar ? d_bbar : num->dBbar;
     TRY(solve_any(num, d_xbar, bb, s));
     mdgpu::launch_solve_adjoint(d_x, bb, num->dArow, num->dAcol, d_g, num->sym->S.nnz_in, s);
@@ -902,6 +1016,7 @@ extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, vo
         return MDGPU_OK;
     }
     hipStream_t s = (hipStream_t)stream;
+    CallOrder order(num, s);
     std::vector<int> ords;          // step index of each GEMM launch, in order
     for (size_t t = 0; t < num->plan.steps.size(); t++)
         if (num->plan.steps[t].kind == mdgpu::STEP_GEMM) ords.push_back((int)t);

@@ -135,6 +135,17 @@ def bbt_plus_ni(n, seed, nnz_per_col=5):
     return lower_csc_from_coo(n, A.row, A.col, A.data)
 
 
+def _last_side(n_target, area, nmin, nmax):
+    """Last grid side k for a grid of `area` points per layer: the nearest
+    to n_target / area with nmin <= area k <= nmax."""
+    k = max(1, int(round(n_target / area)))
+    while area * k < nmin:
+        k += 1
+    while area * k > nmax and k > 1:
+        k -= 1
+    return k
+
+
 def batch_matrices(count=512, seed=0, nmin=500, nmax=20000):
     """Uneven batch: n log-uniform in [nmin, nmax]; even entries are 2D/3D
     Laplacian stencils with diagonal shift U(1e-3, 1e-1), odd ones B B^T + nI."""
@@ -145,14 +156,16 @@ def batch_matrices(count=512, seed=0, nmin=500, nmax=20000):
         sub_seed = int(rng.integers(0, 2**31 - 1))
         if t % 2 == 0:
             shift = float(rng.uniform(1e-3, 1e-1))
+            # grid sides near the target; the last side is rounded so that
+            # n = kx ky kz stays inside [nmin, nmax]
             if rng.uniform() < 0.5:
                 kx = int(np.floor(np.sqrt(n_target)))
-                ky = max(1, n_target // kx)
+                ky = _last_side(n_target, kx, nmin, nmax)
                 out.append(stencil_lower((kx, ky, 1), 5, 4.0 + shift))
             else:
                 kx = max(2, int(np.floor(n_target ** (1.0 / 3.0))))
                 ky = kx
-                kz = max(1, n_target // (kx * ky))
+                kz = _last_side(n_target, kx * ky, nmin, nmax)
                 out.append(stencil_lower((kx, ky, kz), 7, 6.0 + shift))
         else:
             out.append(bbt_plus_ni(n_target, sub_seed))

@@ -115,6 +115,13 @@ class Factorization:
             L.check(st)
         return x, eta.value, it.value
 
+    def solve_status(self):
+        """Status bits of the solves since the last call (1: a chained
+        solve's wait gave up); reading clears them."""
+        v = C.c_int()
+        L.check(L.lib().mdgpu_solve_status(self._h, C.byref(v), _stream()))
+        return v.value
+
     def segment_eta(self):
         """Backward error of each matrix of a batch after the last refine."""
         out = np.empty(max(self.sym.nseg, 1), dtype=np.float64)
@@ -163,6 +170,19 @@ class Factorization:
         Lx = np.empty(lnz, dtype=np.float64)
         L.check(L.lib().mdgpu_export_L(self._h, Lp.ctypes.data_as(L._I32P), Li.ctypes.data_as(L._I32P),
                                        Lx.ctypes.data_as(L._DP)))
+        return Lp, Li, Lx
+
+    def export_L_segment(self, k):
+        """L of matrix k of a batch in its own numbering (Lp, Li, Lx)."""
+        from . import _lib
+        seg = self.sym.array(_lib.ARR_SEG_COL)
+        n = int(seg[k + 1] - seg[k])
+        Lp = np.empty(n + 1, dtype=np.int32)
+        L.check(L.lib().mdgpu_export_L_segment(self._h, k, Lp.ctypes.data_as(L._I32P), None, None))
+        Li = np.empty(int(Lp[-1]), dtype=np.int32)
+        Lx = np.empty(int(Lp[-1]), dtype=np.float64)
+        L.check(L.lib().mdgpu_export_L_segment(self._h, k, Lp.ctypes.data_as(L._I32P), Li.ctypes.data_as(L._I32P),
+                                               Lx.ctypes.data_as(L._DP)))
         return Lp, Li, Lx
 
     def close(self):

@@ -312,8 +312,90 @@ def test_refine_eta_equals_oracle_backward_error(device, prec):
     eta_cpu = O.backward_error(p, i, x, xs.cpu().numpy(), b)
     assert eta_cpu > 1e-11, eta_cpu             # far above FP64 rounding
     assert abs(eta - eta_cpu) <= 1e-6 * eta_cpu, (eta, eta_cpu)
-    # after refinement: the same agreement at the converged x (bound by rounding)
+    # after refinement the residual is rounding (summed in another order on
+    # the GPU): agreement to 5%
     xs, eta, iters = F.refine(torch.from_numpy(b).to(device), tol=1e-12, maxit=30)
     eta_cpu = O.backward_error(p, i, x, xs.cpu().numpy(), b)
     assert eta <= 1e-12 and eta_cpu <= 1e-12
-    assert abs(eta - eta_cpu) <= 1e-3 * eta_cpu + 1e-17, (eta, eta_cpu)
+    assert abs(eta - eta_cpu) <= 5e-2 * eta_cpu + 1e-16, (eta, eta_cpu)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_solve_in_place_vector_path_equals_lds_path(device, monkeypatch, prec):
+    """With MDGPU_SOLVE_LDS_ROWS=0 every front vector of the forward solve is
+    accumulated in place in global memory instead of in LDS (the path fronts
+    longer than the LDS capacity take).  Children are added in the same
+    order either way, so the solution is bit-identical, and it matches
+    triangular solves with the exported factor."""
+    import torch
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve_triangular
+    p, i, x = gen.stencil27(20, seed=6)
+    S = symbolic.analyze(p, i, True, "rn", 100000)
+    ax = torch.from_numpy(x).to(device)
+    b = torch.from_numpy(np.random.default_rng(12).standard_normal(S.n)).to(device)
+    F = Factorization(S, prec).factorize(ax)
+    x_lds = F.solve(b).cpu().numpy()
+    monkeypatch.setenv("MDGPU_SOLVE_LDS_ROWS", "0")
+    G = Factorization(S, prec).factorize(ax)
+    x_glob = G.solve(b).cpu().numpy()
+    assert np.array_equal(x_lds, x_glob)
+    from mdgpu_amd import _lib as L
+    Lp, Li, Lx = G.export_L()
+    Lc = sp.csc_matrix((Lx, Li, Lp), shape=(S.n, S.n)).tocsr()
+    perm = S.array(L.ARR_PERM)
+    bh = b.cpu().numpy()
+    z = spsolve_triangular(Lc.T.tocsr(), spsolve_triangular(Lc, bh[perm], lower=True), lower=False)
+    ref = np.empty(S.n)
+    ref[perm] = z
+    assert rel_err(x_glob, ref) <= (1e-10 if prec == "fp64" else 1e-5)
+
+
+def test_calls_on_two_streams_run_in_call_order(device):
+    """Solves on one handle issued from two streams without synchronizing
+    (they share the handle's solve buffers) run in call order on the GPU:
+    each result equals the solve of its own right-hand side."""
+    import torch
+    p, i, x = gen.stencil27(16, seed=2)
+    S = symbolic.analyze(p, i, True, "rn", 1000)
+    F = Factorization(S, "fp64").factorize(torch.from_numpy(x).to(device))
+    rng = np.random.default_rng(5)
+    bs = [torch.from_numpy(rng.standard_normal(S.n)).to(device) for _ in range(6)]
+    ref = [F.solve(b).cpu().numpy() for b in bs]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = []
+    for k, b in enumerate(bs):
+        with torch.cuda.stream(streams[k % 2]):
+            outs.append(F.solve(b))
+    torch.cuda.synchronize()
+    for k in range(len(bs)):
+        assert np.array_equal(outs[k].cpu().numpy(), ref[k]), k
+
+
+def test_chained_wait_give_up_is_reported(device, monkeypatch):
+    """MDGPU_CHAIN_SPIN=0 makes every wait of the chained block solves give
+    up at its first poll of a value not yet stored (every kernel still
+    exits): the solve status reports it once, mdgpu_refine returns
+    MDGPU_ERR_DEVICE, and a handle with the normal limit solves correctly."""
+    import torch
+    from mdgpu_amd import _lib as L
+    p, i, x = gen.stencil27(20, seed=6)
+    S = symbolic.analyze(p, i, True, "rn", 100000)
+    ax = torch.from_numpy(x).to(device)
+    xt, b = gen.rhs_from_xtrue(p, i, x, seed=3)
+    bd = torch.from_numpy(b).to(device)
+    monkeypatch.setenv("MDGPU_CHAIN_SPIN", "0")
+    F = Factorization(S, "fp64").factorize(ax)
+    assert F.solve_status() == 0
+    xs = F.solve(bd).cpu().numpy()
+    assert F.solve_status() == L.SOLVE_TIMED_OUT
+    assert F.solve_status() == 0                 # reading clears it
+    assert not np.all(np.isfinite(xs))
+    with pytest.raises(L.MdgpuError) as e:
+        F.refine(bd, tol=1e-12, maxit=5)
+    assert e.value.code == L.MDGPU_ERR_DEVICE
+    monkeypatch.delenv("MDGPU_CHAIN_SPIN")
+    G = Factorization(S, "fp64").factorize(ax)
+    xs = G.solve(bd).cpu().numpy()
+    assert G.solve_status() == 0 and np.all(np.isfinite(xs))

@@ -142,6 +142,15 @@ int mdgpu_symbolic_flops(const mdgpu_symbolic* sym, double* out /* [2] */);
 int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_t* dst, int64_t cap);
 
 
+/* Sorted set union of two sorted row lists (VectorMerge, mdgpu.cu:535-583;
+ * out has room for na + nb rows, *nout = rows of the union) and the
+ * relative map of a sorted sublist inside a sorted list (VectorMergeIndices,
+ * mdgpu.cu:585-595): map[t] = position of sub[t] in super.  These are the
+ * operations the analysis builds the front row lists and the extend-add
+ * maps with. */
+int mdgpu_row_union(const int32_t* a, int na, const int32_t* b, int nb, int32_t* out, int* nout);
+int mdgpu_row_map(const int32_t* super, int n, const int32_t* sub, int m, int32_t* map);
+
 /* ------------------------------------------------------------ numeric */
 
 /* Allocate the device arena for a numeric factorization of `sym` (L store,

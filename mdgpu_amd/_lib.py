@@ -118,6 +118,8 @@ _SIGNATURES.update({
     "mdgpu_export_L": (C.c_int, [_P, _I32P, _I32P, _DP]),
     "mdgpu_export_L_segment": (C.c_int, [_P, C.c_int, _I32P, _I32P, _DP]),
     "mdgpu_solve_status": (C.c_int, [_P, C.POINTER(C.c_int), _P]),
+    "mdgpu_row_union": (C.c_int, [_I32P, C.c_int, _I32P, C.c_int, _I32P, C.POINTER(C.c_int)]),
+    "mdgpu_row_map": (C.c_int, [_I32P, C.c_int, _I32P, C.c_int, _I32P]),
     "mdgpu_selinv": (C.c_int, [_P, _P]),
     "mdgpu_selinv_export": (C.c_int, [_P, _DP]),
     "mdgpu_logdet_grad": (C.c_int, [_P, _P, _P]),

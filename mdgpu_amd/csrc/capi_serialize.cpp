@@ -5,6 +5,7 @@
 // mdgpu::Symbolic and the batch segments, each array as an i64 length and
 // its elements, and finally the FNV-1a 64-bit hash of every preceding byte.
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <new>
 #include <string>
@@ -175,6 +176,45 @@ extern "C" int mdgpu_symbolic_load(const char* path, mdgpu_symbolic** sym) {
         ok = S.q_off[f] <= S.q_off[f + 1] && S.front_parent[f] >= -1 && S.front_parent[f] < (int)nn &&
              S.nodes[f].first >= 0 && S.nodes[f].ncols > 0 && S.nodes[f].first + S.nodes[f].ncols <= S.n &&
              S.q_off[f + 1] - S.q_off[f] >= S.nodes[f].ncols;
+    // L's layout: lp is the prefix sum of the column counts (lnz entries;
+    // left empty past int32, as for a large batch), and the columns of each
+    // front hold its rows from the diagonal down: count_final[first + j] =
+    // rows - j, with the pivot columns first in its sorted row list.  The
+    // fronts cover every column once, children before parents.
+    if (ok) {
+        int64_t sum = 0;
+        for (size_t k = 0; ok && k < n; k++) {
+            ok = S.count_final[k] >= 1;
+            sum += S.count_final[k];
+        }
+        ok = ok && sum == S.lnz;
+        if (ok && S.lnz <= INT32_MAX) {
+            ok = S.lp.size() == n + 1 && S.lp[0] == 0;
+            for (size_t k = 0; ok && k < n; k++) ok = S.lp[k + 1] == S.lp[k] + S.count_final[k];
+        } else if (ok) {
+            ok = S.lp.empty();
+        }
+    }
+    std::vector<char> covered(ok ? n : 0, 0);
+    for (size_t f = 0; ok && f < nn; f++) {
+        const mdgpu::PartNode& nd = S.nodes[f];
+        const int64_t q0 = S.q_off[f], rows = S.q_off[f + 1] - S.q_off[f];
+        ok = S.front_parent[f] == -1 || S.front_parent[f] > (int)f;
+        for (int j = 0; ok && j < nd.ncols; j++) {
+            ok = !covered[nd.first + j] && S.count_final[nd.first + j] == rows - j && S.q_all[q0 + j] == nd.first + j;
+            covered[nd.first + j] = 1;
+        }
+        for (int64_t t = q0 + 1; ok && t < q0 + rows; t++) ok = S.q_all[t - 1] < S.q_all[t];
+        for (int k : nd.kids) ok = ok && S.front_parent[k] == (int)f;
+    }
+    for (size_t k = 0; ok && k < n; k++) ok = covered[k];
+    // segments of a batch: columns and caller entries split in order
+    const bool no_seg = h->seg_col.empty() && h->seg_nnz.empty();   // one matrix (filled in below)
+    ok = ok && (no_seg || (h->seg_col.size() >= 2 && h->seg_col.size() == h->seg_nnz.size() &&
+                           h->seg_col[0] == 0 && h->seg_nnz[0] == 0 && h->seg_col.back() == S.n &&
+                           h->seg_nnz.back() == S.nnz_in));
+    for (size_t k = 0; ok && k + 1 < h->seg_col.size(); k++)
+        ok = h->seg_col[k] < h->seg_col[k + 1] && h->seg_nnz[k] <= h->seg_nnz[k + 1];
     if (!ok) {
         delete h;
         return fail(MDGPU_ERR_ARG, "mdgpu_symbolic_load: inconsistent or truncated file");

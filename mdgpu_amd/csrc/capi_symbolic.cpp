@@ -255,3 +255,32 @@ extern "C" int mdgpu_symbolic_array(const mdgpu_symbolic* sym, int which, int32_
         default: return fail(MDGPU_ERR_ARG, "mdgpu_symbolic_array: unknown array id");
     }
 }
+
+extern "C" int mdgpu_row_union(const int32_t* a, int na, const int32_t* b, int nb, int32_t* out, int* nout) {
+    mdgpu::clear_error();
+    if ((na > 0 && !a) || (nb > 0 && !b) || !out || !nout || na < 0 || nb < 0)
+        return fail(MDGPU_ERR_ARG, "mdgpu_row_union: invalid argument");
+    int top = -1;
+    for (int t = 0; t < na; t++) top = std::max(top, a[t]);
+    for (int t = 0; t < nb; t++) top = std::max(top, b[t]);
+    for (int t = 0; t < na; t++)
+        if (a[t] < 0 || (t > 0 && a[t] <= a[t - 1])) return fail(MDGPU_ERR_ARG, "mdgpu_row_union: a not sorted");
+    for (int t = 0; t < nb; t++)
+        if (b[t] < 0 || (t > 0 && b[t] <= b[t - 1])) return fail(MDGPU_ERR_ARG, "mdgpu_row_union: b not sorted");
+    std::vector<int> stamp(top + 1, -1), rows;
+    mdgpu::union_rows(rows, stamp, 0, a, na);
+    mdgpu::union_rows(rows, stamp, 0, b, nb);
+    mdgpu::sort_union_tail(rows, 0);
+    std::copy(rows.begin(), rows.end(), out);
+    *nout = (int)rows.size();
+    return MDGPU_OK;
+}
+
+extern "C" int mdgpu_row_map(const int32_t* super, int n, const int32_t* sub, int m, int32_t* map) {
+    mdgpu::clear_error();
+    if ((n > 0 && !super) || (m > 0 && (!sub || !map)) || n < 0 || m < 0)
+        return fail(MDGPU_ERR_ARG, "mdgpu_row_map: invalid argument");
+    if (!mdgpu::relative_map(super, n, sub, m, map))
+        return fail(MDGPU_ERR_ARG, "mdgpu_row_map: a row of the sublist is not in the list");
+    return MDGPU_OK;
+}

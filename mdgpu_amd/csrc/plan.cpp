@@ -142,14 +142,11 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             continue;
         }
         const FrontDesc& pd = P.fronts[d.parent];
-        const int* qp = S.q_all.data() + pd.qofs;
-        for (int t = 0; t < d.m; t++) {
-            int r = S.q_all[d.qofs + d.c + t];
-            int pos = pos_in(qp, pd.n, r);
-            if (pos < 0) return "update row missing from parent front";
-            P.rel.push_back(pos);
-            if (pos < pd.c) d.split++;
-        }
+        const size_t at = P.rel.size();
+        P.rel.resize(at + d.m);
+        if (!relative_map(S.q_all.data() + pd.qofs, pd.n, S.q_all.data() + d.qofs + d.c, d.m, P.rel.data() + at))
+            return "update row missing from parent front";
+        for (int t = 0; t < d.m; t++) d.split += P.rel[at + t] < pd.c;
     }
 
     // caller entry -> panel offset (MatrizGetFk_3's A scatter, mdgpu.cu:272-282)

@@ -709,6 +709,27 @@ std::string relaxed_partition(const std::vector<PartNode>& plain, int max_zeros,
 
 Csc full_pattern(const Csc& low) { return full_from_lower(low); }
 
+void union_rows(std::vector<int>& rows, std::vector<int>& stamp, int tag, const int* list, int64_t len) {
+    for (int64_t t = 0; t < len; t++) {
+        const int r = list[t];
+        if (stamp[r] != tag) {
+            stamp[r] = tag;
+            rows.push_back(r);
+        }
+    }
+}
+
+void sort_union_tail(std::vector<int>& rows, size_t from) { std::sort(rows.begin() + from, rows.end()); }
+
+bool relative_map(const int* super, int n, const int* sub, int m, int* map) {
+    for (int t = 0; t < m; t++) {
+        const int* it = std::lower_bound(super, super + n, sub[t]);
+        if (it == super + n || *it != sub[t]) return false;
+        map[t] = (int)(it - super);
+    }
+    return true;
+}
+
 // Front row lists: pivot columns, rows of A below them, and the update
 // rows of every child -- the union MatrizGetFk_3 and VectorMerge build per
 // factorization (mdgpu.cu:180-291, 1456-1508).  With check_counts the
@@ -734,18 +755,12 @@ std::string build_fronts(Symbolic& S, bool check_counts) {
         rows.clear();
         for (int j = k; j < k + c; j++) { stamp[j] = f; rows.push_back(j); }
         for (int j = k; j < k + c; j++)
-            for (int p = S.a_low.p[j]; p < S.a_low.p[j + 1]; p++) {
-                int r = S.a_low.i[p];
-                if (stamp[r] != f) { stamp[r] = f; rows.push_back(r); }
-            }
+            union_rows(rows, stamp, f, S.a_low.i.data() + S.a_low.p[j], S.a_low.p[j + 1] - S.a_low.p[j]);
         for (int kid : nd.kids) {
             const int kc = S.nodes[kid].ncols;
-            for (int64_t t = S.q_off[kid] + kc; t < S.q_off[kid + 1]; t++) {
-                int r = S.q_all[t];
-                if (stamp[r] != f) { stamp[r] = f; rows.push_back(r); }
-            }
+            union_rows(rows, stamp, f, S.q_all.data() + S.q_off[kid] + kc, S.q_off[kid + 1] - S.q_off[kid] - kc);
         }
-        std::sort(rows.begin() + c, rows.end());
+        sort_union_tail(rows, c);
         if ((int)rows.size() > c && rows[c] < k + c) return "front rows precede the pivot block";
         const int nf = (int)rows.size();
         if (!check_counts)

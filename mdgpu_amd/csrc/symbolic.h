@@ -101,6 +101,16 @@ Csc full_pattern(const Csc& low);
 std::string analyze_with_nodes(int n, const int* Cp, const int* Ci, const std::vector<PartNode>& nodes,
                                Symbolic& out);
 
+// Sorted set union of row lists (VectorMerge, mdgpu.cu:535-583): append
+// the rows of `list` not yet in `rows` (stamp[r] == tag marks a row already
+// there); after the last list, sort_union_tail sorts rows[from..).  The
+// front row lists of build_fronts are unions of this kind.
+void union_rows(std::vector<int>& rows, std::vector<int>& stamp, int tag, const int* list, int64_t len);
+void sort_union_tail(std::vector<int>& rows, size_t from);
+// Relative map (VectorMergeIndices, mdgpu.cu:585-595): map[t] = position
+// of sub[t] in the sorted list super; false if some sub[t] is missing.
+bool relative_map(const int* super, int n, const int* sub, int m, int* map);
+
 // Block-diagonal union of independent analyses (a batch of matrices): the
 // columns, rows, caller entries and nodes of part k are shifted by the
 // totals of parts 0..k-1, so the result is the analysis of the block

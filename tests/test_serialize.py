@@ -59,3 +59,76 @@ def test_damaged_files_are_rejected(tmp_path):
             symbolic.load(bad)
     with pytest.raises(L.MdgpuError):
         symbolic.load(tmp_path / "missing.sym")
+
+
+def _fnv1a(b):
+    h = 1469598103934665603
+    for c in b:
+        h = ((h ^ c) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _vector_offsets(d):
+    """Byte offsets of the int32 vectors at the head of the file: magic (8),
+    version (4), n, nnz_in, use_amd, partition, max_zeros (4 each), then
+    amd_perm, parent, post, count, relaxed_perm, perm, pinv, count_final,
+    lp, ... each as an int64 length and its entries."""
+    off, out = 32, {}
+    for name in ["amd_perm", "parent", "post", "count", "relaxed_perm", "perm", "pinv", "count_final", "lp"]:
+        ln = int(np.frombuffer(bytes(d[off:off + 8]), dtype="<i8")[0])
+        out[name] = (off + 8, ln)
+        off += 8 + 4 * ln
+    return out
+
+
+@pytest.mark.parametrize("field,delta", [("lp", 1), ("count_final", 1)])
+def test_inconsistent_layout_with_valid_checksum_is_rejected(tmp_path, field, delta):
+    """A file that passes the checksum but whose L layout is inconsistent
+    (spL->p not the prefix sum of the counts, or a front column count that
+    does not match its row list) is rejected: mdgpu_export_L would write
+    outside the caller's buffers with it."""
+    p, i, x = gen.laplace2d(12)
+    S = symbolic.analyze(p, i, True, "rn", 100)
+    path = tmp_path / "c.sym"
+    S.save(path)
+    d = bytearray(path.read_bytes())
+    start, ln = _vector_offsets(d)[field]
+    t = start + 4 * (ln // 2)
+    v = int(np.frombuffer(bytes(d[t:t + 4]), dtype="<i4")[0]) + delta
+    d[t:t + 4] = np.array([v], dtype="<i4").tobytes()
+    d[-8:] = np.array([_fnv1a(d[:-8])], dtype="<u8").tobytes()
+    bad = tmp_path / "bad.sym"
+    bad.write_bytes(bytes(d))
+    with pytest.raises(L.MdgpuError) as e:
+        symbolic.load(bad)
+    assert "inconsistent" in str(e.value)
+    # the untouched file (checksum recomputed the same way) still loads
+    d0 = bytearray(path.read_bytes())
+    d0[-8:] = np.array([_fnv1a(d0[:-8])], dtype="<u8").tobytes()
+    assert bytes(d0) == path.read_bytes()
+    symbolic.load(path)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,mz", [("rn", 1000), ("sn", 0)])
+def test_factor_from_loaded_analysis_matches_oracle(tmp_path, device, mode, mz):
+    """Refactorization without analysis: a numeric handle built on an
+    analysis read back from disk factors the matrix exactly as one built on
+    the original (bit for bit in FP64), and both match the oracle."""
+    import torch
+    from mdgpu_amd.numeric import Factorization
+    from oracle import oracle as O
+    from tests.helpers import oracle_factor, rel_err
+    p, i, x = gen.stencil27(14, seed=9)
+    S = symbolic.analyze(p, i, True, mode, mz)
+    S.save(tmp_path / "s.sym")
+    T = symbolic.load(tmp_path / "s.sym")
+    ax = torch.from_numpy(x).to(device)
+    Lp, Li, Lx = Factorization(T, "fp64").factorize(ax).export_L()
+    Lp0, Li0, Lx0 = Factorization(S, "fp64").factorize(ax).export_L()
+    assert np.array_equal(Lp, Lp0) and np.array_equal(Li, Li0) and np.array_equal(Lx, Lx0)
+    lp, Li_ref, Lx_ref, _, bad = oracle_factor(p, i, x, T)
+    assert bad == -1 and np.array_equal(Li, Li_ref) and rel_err(Lx, Lx_ref) <= 1e-10
+    xt, b = gen.rhs_from_xtrue(p, i, x, seed=4)
+    xs, eta, _ = Factorization(T, "fp32").factorize(ax).refine(torch.from_numpy(b).to(device), tol=1e-12)
+    assert eta <= 1e-12 and O.backward_error(p, i, x, xs.cpu().numpy(), b) <= 1e-12

@@ -51,6 +51,9 @@ int cs_sprealloc(cs *A, int nzmax);
 int cs_entry(cs *T, int i, int j, FLOTANTE x);
 cs *cs_compress(const cs *T);
 cs *cs_load(FILE *f);
+/* Matrix Market coordinate file -> lower-triangle triplets (this library's
+ * addition; cs_load reads the reference's headerless "i j x" lines). */
+cs *cs_load_mm(FILE *f);
 cs *cs_transpose(const cs *A, int values);
 cs *cs_add(const cs *A, const cs *B, FLOTANTE alpha, FLOTANTE beta);
 int cs_scatter(const cs *A, int j, FLOTANTE beta, int *w, FLOTANTE *x, int mark, cs *C, int nz);

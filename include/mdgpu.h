@@ -65,6 +65,28 @@ int cs_print(FILE *f, const cs *A, int brief);
  * unfilled and mdgpu_last_error() describes why. */
 void Factor(cs *A, cs *spL, struct ETree **listaNodos, int cantNodos);
 
+/* Phase times in seconds of the last Factor() call of the calling thread:
+ * the line items of the reference driver's report (mdgpu.cu:1901-1930),
+ * which it kept in global tick counters.  No counterpart as a function in
+ * the reference. */
+typedef struct {
+    double mflops;      /* MFlops: sum over columns of (c_j^2 + 1), padded counts (mdgpu.cu:1354-1359) */
+    double factor;      /* Factor: the whole call                                   */
+    double factor_aux;  /* FactorAux: GPU compute (sum of the kernel classes below) */
+    double blas_gpu;    /* BLAS GPU: dense front kernels                            */
+    double trsm_gpu;    /*   TRSM GPU: potrf + TRSM (LDS panels and blocked path)   */
+    double gemm_gpu;    /*   GEMM GPU: MFMA Schur-complement updates                */
+    double other_gpu;   /*   Otros: diagonal-block inverses used by the solves      */
+    double memcpy_x;    /* MemcpyX: A's values host -> device                       */
+    double memcpy_l;    /* Memcpy2: L device -> host (spL)                          */
+    double merge;       /* Merge: front row lists and relative maps (host)          */
+    double getfk;       /* GetFk: front assembly from A on the GPU                  */
+    double extend_add;  /* ExtendAdd                                                */
+    double malloc_gpu;  /* Malloc GPU: schedule and device arena                    */
+    double free_gpu;    /* Free GPU                                                 */
+} FactorTimes;
+void FactorGetTimes(FactorTimes *out);
+
 MDGPU_DROPIN_END
 
 #endif

@@ -5,9 +5,14 @@
 // followed by the solve and backward-error check the reference left after
 // its exit(1) (mdgpu.cu:1941-1973), run here.
 //
-// usage: mdgpu [-gpu|-cpu] [-amd] [-sn|-rn c] [-b bs] [-pad] [-symbolic] [-dump DIR] matrix
+// usage: mdgpu [-gpu] [-amd] [-sn|-rn c] [-b bs] [-pad] [-symbolic] [-dump DIR] matrix
+//   matrix     the reference's headerless "i j x" triplets (cs_load), or a
+//              Matrix Market coordinate file (read by cs_load_mm)
+//   -cpu       rejected: the numeric factorization runs on the GPU only
 //   -symbolic  stop after the analysis (no GPU needed)
 //   -dump DIR  write perm / parent / post / count / lp as raw int32 files
+// The report prints the reference's line items (mdgpu.cu:1901-1930) from
+// FactorGetTimes.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -54,12 +59,16 @@ int main(int argc, char** argv) {
     int max_ceros = 0;
     std::string dumpdir;
     if (argc < 2) {
-        printf("mdgpu [-gpu|-cpu] [-amd] [-sn|-rn c] [-b bs] [-pad] [-symbolic] [-dump DIR] matriz\n");
+        printf("mdgpu [-gpu] [-amd] [-sn|-rn c] [-b bs] [-pad] [-symbolic] [-dump DIR] matriz\n");
         return 1;
     }
     for (int t = 1; t < argc - 1; t++) {
         if (!strcmp(argv[t], "-gpu")) GPU = true;
-        else if (!strcmp(argv[t], "-cpu")) GPU = false;
+        else if (!strcmp(argv[t], "-cpu")) {
+            fprintf(stderr, "mdgpu: -cpu is not supported: this build factors on the GPU only "
+                            "(the reference's CPU BLAS path, FactorAux_CPU, is not part of it)\n");
+            return 1;
+        }
         else if (!strcmp(argv[t], "-amd")) amd = true;
         else if (!strcmp(argv[t], "-sn")) super = true;
         else if (!strcmp(argv[t], "-rn")) { relaxed = true; max_ceros = atoi(argv[t + 1]); }
@@ -68,15 +77,19 @@ int main(int argc, char** argv) {
         else if (!strcmp(argv[t], "-symbolic")) symbolic_only = true;
         else if (!strcmp(argv[t], "-dump")) dumpdir = argv[t + 1];
         else if (!strcmp(argv[t], "-help")) {
-            printf("mdgpu [-gpu|-cpu] [-amd] [-sn|-rn c] [-b bs] [-pad] [-symbolic] [-dump DIR] matriz\n");
+            printf("mdgpu [-gpu] [-amd] [-sn|-rn c] [-b bs] [-pad] [-symbolic] [-dump DIR] matriz\n");
             return 1;
         }
     }
     double t_load = now();
     FILE* f = fopen(argv[argc - 1], "r");
     if (!f) { perror(argv[argc - 1]); return 1; }
-    cs* T = cs_load(f);
+    char head[16] = {0};
+    const bool mm = fread(head, 1, 14, f) == 14 && !strncmp(head, "%%MatrixMarket", 14);
+    rewind(f);
+    cs* T = mm ? cs_load_mm(f) : cs_load(f);
     fclose(f);
+    if (!T || T->nz <= 0) { fprintf(stderr, "mdgpu: cannot read %s\n", argv[argc - 1]); return 1; }
     cs* C_low = cs_compress(T);
     cs_spfree(T);
     t_load = now() - t_load;
@@ -163,11 +176,28 @@ int main(int argc, char** argv) {
     printf("Symbolic       = %.6f\n", t_sym);
     if (symbolic_only) return 0;
 
-    double t_fac = now();
     Factor(A_low, spL, list, nlist);
-    t_fac = now() - t_fac;
     if (!spL->x) { fprintf(stderr, "Factor failed: %s\n", mdgpu_last_error()); return 2; }
-    printf("Factor         = %.6f\n", t_fac);
+    FactorTimes ft;
+    FactorGetTimes(&ft);
+    // the reference's report (mdgpu.cu:1901-1930); its CPU BLAS items are 0
+    printf("\nMFlops     = %.2f\n", ft.mflops);
+    printf("MFlops/s   = %.3f\n\n", ft.factor_aux > 0 ? ft.mflops / ft.factor_aux : 0.0);
+    printf("Factor         = %.6f Factorizacion numerica total\n", ft.factor);
+    printf("FactorAux      = %.6f Factorizacion numerica solo calculo\n", ft.factor_aux);
+    printf("  BLAS CPU     = %.6f\n", 0.0);
+    printf("  BLAS GPU     = %.6f\n", ft.blas_gpu);
+    printf("    TRSM GPU   = %.6f\n", ft.trsm_gpu);
+    printf("    GEMM GPU   = %.6f\n", ft.gemm_gpu);
+    printf("  Otros        = %.6f\n", ft.other_gpu);
+    printf("Memcpy         = %.6f\n", ft.memcpy_x + ft.memcpy_l);
+    printf("  Memcpy2      = %.6f Copia de L GPU->CPU\n", ft.memcpy_l);
+    printf("  MemcpyX      = %.6f Copia inicial de A CPU->GPU\n", ft.memcpy_x);
+    printf("Merge          = %.6f\n", ft.merge);
+    printf("GetFk          = %.6f Formar el frente\n", ft.getfk);
+    printf("ExtendAdd      = %.6f Suma extendida\n", ft.extend_add);
+    printf("Malloc GPU     = %.6f\n", ft.malloc_gpu);
+    printf("Free GPU       = %.6f\n\n", ft.free_gpu);
 
     // solve + backward error (mdgpu.cu:1941-1973), b_i = 1 + i/n
     std::vector<FLOTANTE> b(n), x(n), x2(n), resid(n);

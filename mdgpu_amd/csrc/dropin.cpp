@@ -7,6 +7,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cctype>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -137,6 +139,44 @@ cs* cs_load(FILE* f) {
     FLOTANTE x;
     while (T && read_triplet(f, &i, &j, &x))
         if (!cs_entry(T, i - 1, j - 1, x)) return cs_spfree(T);
+    return T;
+}
+
+// Matrix Market coordinate file (real, integer or pattern; symmetric or
+// general): the entries on and below the diagonal as a triplet matrix, the
+// input cs_compress and the driver take (the reference's test() reads .mtx
+// files through cs_load, mdgpu.cu:1655, which the header lines stop).  Upper
+// entries of a general file are dropped (the matrix is taken symmetric);
+// pattern entries get the value 1.  NULL on a malformed file.
+cs* cs_load_mm(FILE* f) {
+    if (!f) return NULL;
+    char line[1024];
+    if (!fgets(line, sizeof line, f)) return NULL;
+    std::string head(line);
+    for (char& ch : head) ch = (char)tolower((unsigned char)ch);
+    if (head.rfind("%%matrixmarket", 0) != 0 || head.find("coordinate") == std::string::npos) return NULL;
+    const bool pattern = head.find("pattern") != std::string::npos;
+    if (head.find("complex") != std::string::npos) return NULL;
+    int m = 0, n = 0, nnz = -1;
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '%' || line[0] == '\n') continue;
+        if (sscanf(line, "%d %d %d", &m, &n, &nnz) != 3) return NULL;
+        break;
+    }
+    if (nnz < 0 || m != n || n <= 0) return NULL;
+    cs* T = cs_spalloc(n, n, CS_MAX(nnz, 1), 1, 1);
+    for (int t = 0; T && t < nnz; t++) {
+        int i, j;
+        double v = 1.0;
+        if (!fgets(line, sizeof line, f)) return cs_spfree(T);
+        const int got = pattern ? sscanf(line, "%d %d", &i, &j) : sscanf(line, "%d %d %lg", &i, &j, &v);
+        if (got != (pattern ? 2 : 3) || i < 1 || j < 1 || i > n || j > n) return cs_spfree(T);
+        if (i >= j && !cs_entry(T, i - 1, j - 1, (FLOTANTE)v)) return cs_spfree(T);
+    }
+    if (T) {
+        T->m = n;
+        T->n = n;
+    }
     return T;
 }
 
@@ -645,11 +685,28 @@ void MatrizSpy2(Matriz* A, Matriz* L) {
 }
 
 // ------------------------------------------------------------ Factor
+namespace {
+thread_local FactorTimes g_times;
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+void FactorGetTimes(FactorTimes* out) {
+    if (out) *out = g_times;
+}
+
 // Numeric factorization of the permuted lower triangle A over the node list
 // (mdgpu.cu:1394-1608), on the GPU in FLOTANTE's precision (MDGPU_FP32 for
 // float, MDGPU_FP64 for double, as the reference computes everything in
-// FLOTANTE); L is written into spL (layout of mdgpu.cu:1346-1352).
+// FLOTANTE); L is written into spL (layout of mdgpu.cu:1346-1352).  The
+// phase times of the reference's report (mdgpu.cu:1901-1930) are kept for
+// FactorGetTimes: the kernel classes are timed with HIP events around every
+// launch (mdgpu_factorize_profile).
 void Factor(cs* A, cs* spL, struct ETree** listaNodos, int cantNodos) {
+    const double t_start = now_s();
+    FactorTimes& T = g_times;
+    T = FactorTimes{};
     auto report = [](const char* what) {
         fprintf(stderr, "Factor: %s: %s\n", what, mdgpu_last_error());
     };
@@ -659,6 +716,7 @@ void Factor(cs* A, cs* spL, struct ETree** listaNodos, int cantNodos) {
         return;
     }
     const int n = A->n;
+    double t0 = now_s();
     std::unordered_map<const struct ETree*, int> index;
     for (int f = 0; f < cantNodos; f++) index[listaNodos[f]] = f;
     std::vector<int32_t> first(cantNodos), ncols(cantNodos), kid_ptr(cantNodos + 1, 0), kids;
@@ -682,6 +740,10 @@ void Factor(cs* A, cs* spL, struct ETree** listaNodos, int cantNodos) {
         report("analysis");
         return;
     }
+    T.merge = now_s() - t0;
+    double fl[2] = {0, 0};
+    mdgpu_symbolic_flops(sym, fl);
+    T.mflops = fl[1] / 1e6;
     std::vector<int32_t> lp(n + 1);
     mdgpu_symbolic_array(sym, MDGPU_ARR_LP, lp.data(), n + 1);
     for (int k = 0; k <= n; k++)
@@ -695,27 +757,50 @@ void Factor(cs* A, cs* spL, struct ETree** listaNodos, int cantNodos) {
     double* d_ax = nullptr;
     const size_t nz = (size_t)A->p[n];
     const int precision = std::is_same<FLOTANTE, float>::value ? MDGPU_FP32 : MDGPU_FP64;
+    t0 = now_s();
     int st = mdgpu_numeric_create(sym, precision, &num);
     if (st == MDGPU_OK && hipMalloc((void**)&d_ax, sizeof(double) * std::max<size_t>(nz, 1)) != hipSuccess)
         st = mdgpu::fail(MDGPU_ERR_NOMEM, "Factor: hipMalloc failed");
+    T.malloc_gpu = now_s() - t0;
     if (st == MDGPU_OK) {
+        t0 = now_s();
         std::vector<double> ax(A->x, A->x + nz);
         if (hipMemcpy(d_ax, ax.data(), sizeof(double) * nz, hipMemcpyHostToDevice) != hipSuccess)
             st = mdgpu::fail(MDGPU_ERR_DEVICE, "Factor: hipMemcpy failed");
+        T.memcpy_x = now_s() - t0;
     }
-    if (st == MDGPU_OK) st = mdgpu_factorize(num, d_ax, nullptr);
     if (st == MDGPU_OK) {
+        double prof[4 * MDGPU_KIND_N];
+        st = mdgpu_factorize_profile(num, d_ax, nullptr, prof);
+        if (st == MDGPU_OK) {
+            // classes: 0 assembly, 1 extend-add, 2 LDS panels, 3 blocked
+            // potrf + TRSM, 4 inverses for the solves, 5 GEMM/SYRK (ms)
+            T.getfk = prof[4 * 0] * 1e-3;
+            T.extend_add = prof[4 * 1] * 1e-3;
+            T.trsm_gpu = (prof[4 * 2] + prof[4 * 3]) * 1e-3;
+            T.gemm_gpu = prof[4 * 5] * 1e-3;
+            T.other_gpu = prof[4 * 4] * 1e-3;
+            T.blas_gpu = T.trsm_gpu + T.gemm_gpu;
+            T.factor_aux = T.getfk + T.extend_add + T.blas_gpu + T.other_gpu;
+        }
+    }
+    if (st == MDGPU_OK) {
+        t0 = now_s();
         if (!spL->i) spL->i = (int*)cs_malloc(lp[n], sizeof(int));
         if (!spL->x) spL->x = (FLOTANTE*)cs_malloc(lp[n], sizeof(FLOTANTE));
         std::vector<double> lx(lp[n]);
         st = mdgpu_export_L(num, nullptr, spL->i, lx.data());
         if (st == MDGPU_OK)
             for (int t = 0; t < lp[n]; t++) spL->x[t] = (FLOTANTE)lx[t];
+        T.memcpy_l = now_s() - t0;
     }
     if (st != MDGPU_OK) report("factorization");
+    t0 = now_s();
     if (d_ax) (void)hipFree(d_ax);
     mdgpu_numeric_destroy(num);
     mdgpu_symbolic_destroy(sym);
+    T.free_gpu = now_s() - t0;
+    T.factor = now_s() - t_start;
 }
 
 MDGPU_DROPIN_END

@@ -287,6 +287,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aniso", action="store_true", help="skip the z-anisotropic C3 variant")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile sizes")
+    ap.add_argument("--budgets", default="250,1000,2000,4000,8000,16000",
+                    help="relaxed-amalgamation zero budgets the partition tuner tries per precision "
+                         "(empty: --mode/--max-zeros for every precision)")
     ap.add_argument("--only", default="fp32,fp64,bf16x3,bf16",
                     help="comma-separated precisions to time (fp32 and fp64 are needed for the JSON line)")
     ap.add_argument("--steps-json", default="", help="write per-launch FP32 timings (class, ms, wgs, flops, bytes, level)")
@@ -308,19 +311,40 @@ def main():
     desc, make = WORKLOADS[args.workload]
     p, i, x = make(gen)
     xt, b = gen.rhs_from_xtrue(p, i, x, seed=0)
-    t0 = time.perf_counter()
-    S = symbolic.analyze(p, i, True, args.mode, args.max_zeros)
-    t_an = time.perf_counter() - t0
-    log(f"analyze: n={S.n} fronts={S.nnodes} lnz={S.lnz} max_front={S.max_front} "
-        f"flops={S.flops:.3e} ({t_an:.1f}s)")
     ax = torch.from_numpy(x).cuda()
     bd = torch.from_numpy(b).cuda()
     torch.cuda.synchronize()
+    precs = [q for q in ("fp32", "fp64", "bf16x3", "bf16") if q in args.only.split(",")]
+    # the analysis (and, with --budgets, the partition tuner: the relaxed
+    # zero budget with the fastest factorization + solves, per precision)
+    # is setup that depends on the pattern only: untimed, like the analysis
+    budgets = [int(v) for v in args.budgets.split(",") if v.strip()]
+    t0 = time.perf_counter()
+    tuning = {}
+    if budgets:
+        cands = symbolic.analyze_budgets(p, i, budgets)
+        t_an = time.perf_counter() - t0
+        from mdgpu_amd.numeric import tune_partition
+        sym_for = {}
+        for prec in precs:
+            # solves per time to solution: FP64 one, the refined paths ~3 (bf16: ~7)
+            nsolve = {"fp64": 1, "bf16": 7}.get(prec, 3)
+            best, ms = tune_partition(cands, ax, prec, reps=2, solves=nsolve)
+            sym_for[prec] = cands[best]
+            tuning[prec] = {"budget": budgets[best], "ms_factor_plus_solves": dict(zip(map(str, budgets), ms))}
+            log(f"partition tuner {prec}: rn {budgets[best]}  " +
+                " ".join(f"{bb}:{m:.2f}ms" for bb, m in zip(budgets, ms)))
+    else:
+        S0 = symbolic.analyze(p, i, True, args.mode, args.max_zeros)
+        t_an = time.perf_counter() - t0
+        sym_for = {prec: S0 for prec in precs}
+    S = sym_for.get("fp32", next(iter(sym_for.values())))
+    log(f"analyze: n={S.n} fronts={S.nnodes} lnz={S.lnz} max_front={S.max_front} "
+        f"flops={S.flops:.3e} ({t_an:.1f}s)")
 
     res = {}
-    precs = [q for q in ("fp32", "fp64", "bf16x3", "bf16") if q in args.only.split(",")]
     for prec in precs:
-        F = Factorization(S, prec)
+        F = Factorization(sym_for[prec], prec)
         # per-launch GEMM tile sizes from timed factorizations (untimed setup,
         # like the analysis: it depends on the pattern only)
         tuned = F.autotune(ax) if not args.no_autotune else None
@@ -370,7 +394,7 @@ def main():
         bda = torch.from_numpy(ba).cuda()
         aniso = {}
         for prec in ("fp32", "fp64"):
-            F = Factorization(S, prec)
+            F = Factorization(sym_for[prec], prec)
             if not args.no_autotune:
                 F.autotune(axa)
             r = time_path(F, axa, bda, args.steps, args.warmup, args.tol)
@@ -446,7 +470,9 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": desc, "n": S.n, "nnz_lower": int(len(i)), "ordering": "amd",
-                   "partition": f"{args.mode} {args.max_zeros}", "fronts": S.nnodes, "lnz": S.lnz,
+                   "partition": (f"rn {tuning['fp32']['budget']} (tuned; fp64: rn {tuning['fp64']['budget']})"
+                                 if tuning else f"{args.mode} {args.max_zeros}"),
+                   "fronts": S.nnodes, "lnz": S.lnz,
                    "max_front": S.max_front, "flops_counts": S.flops, "flops_padded": S.flops_padded},
         "autotune_64_tile_launches": {"fp32": r32["autotune_64_tile_launches"],
                                       "fp64": r64["autotune_64_tile_launches"]},
@@ -473,6 +499,7 @@ def main():
         "kernel_classes_fp32": {k: {"ms": v["ms"], "launches": v["launches"]} for k, v in prof.items()},
         "kernel_report": {"fp32": r32["report"], "fp64": r64["report"]},
         "levels": level_report(S),
+        "partition_tuning": tuning or None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "analyze_s": t_an,

@@ -71,6 +71,14 @@ int mdgpu_analyze(int n, const int32_t* Cp, const int32_t* Ci,
                   const mdgpu_analyze_opts* opts, mdgpu_symbolic** sym);
 int mdgpu_symbolic_destroy(mdgpu_symbolic* sym);
 
+/* The relaxed-amalgamation analyses of one matrix for nbudgets zero budgets
+ * (the -rn c flag; the greedy cost of etree.cpp:236-237), sharing the
+ * ordering, elimination tree and column counts: syms[k] equals what
+ * mdgpu_analyze with partition MDGPU_PART_RELAXED and max_zeros =
+ * budgets[k] returns.  The candidates of mdgpu_tune_partition. */
+int mdgpu_analyze_budgets(int n, const int32_t* Cp, const int32_t* Ci, int use_amd, int nbudgets,
+                          const int32_t* budgets, mdgpu_symbolic** syms);
+
 /* Batch of independent matrices analysed as one: each matrix k (order n[k],
  * lower triangle Cp[k]/Ci[k]) gets exactly the analysis mdgpu_analyze
  * would give it (run on `nthreads` host threads, 0 = all cores), and the
@@ -217,6 +225,16 @@ int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream
  * a valid factorization of d_ax.  No counterpart in the reference. */
 int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, void* stream, int64_t* changed);
 
+/* Partition tuner (relaxed-amalgamation zero budget, etree.cpp:236-237):
+ * for each analysis syms[k] of one matrix (e.g. mdgpu_analyze_budgets),
+ * build a numeric handle in `precision`, autotune its GEMM tiles on d_ax,
+ * and time `reps` factorizations plus reps * `solves` solves (wall clock,
+ * synchronized); ms[k] = time of one factorization + `solves` solves, *best
+ * = the fastest k.  One handle exists at a time.  Allocates (a setup
+ * routine, like the analysis).  No counterpart in the reference. */
+int mdgpu_tune_partition(const mdgpu_symbolic* const* syms, int nsym, const double* d_ax, int precision, int reps,
+                         int solves, void* stream, double* ms, int* best);
+
 /* mdgpu_factorize with one record per launch of the schedule:
  * out[6*t + {0..5}] = kernel class (as above), milliseconds, workgroups,
  * algorithmic flops, algorithmic bytes, level.  cap >= 6 * MDGPU_NUM_STEPS.
@@ -243,6 +261,16 @@ int mdgpu_solve_nrhs(mdgpu_numeric* num, const double* d_B, double* d_X, int nrh
  * (with eta/iters filled) if tol was not reached. */
 int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol, int maxit,
                  double* eta, int* iters, void* stream);
+
+/* Backward error of x for the symmetric system A x = b, computed on the
+ * GPU: eta = ||b - A x||_inf / (||A||_1 ||x||_inf + ||b||_inf) -- the check
+ * of the reference driver (cs_gaxpy + cs_norm + norm, mdgpu.cu:1962-1972).
+ * d_rowptr / d_colind / d_val: the FULL symmetric matrix (both triangles)
+ * in CSR form on the device (for a symmetric matrix the same arrays are its
+ * CSC form); d_x, d_b: device vectors.  Allocates a small device workspace
+ * per call; synchronizes `stream`. */
+int mdgpu_backward_error(int n, const int32_t* d_rowptr, const int32_t* d_colind, const double* d_val,
+                         const double* d_x, const double* d_b, double* eta, void* stream);
 
 /* Status of the solves run since the last call: MDGPU_SOLVE_TIMED_OUT is set
  * when a wait inside a chained block solve gave up (about a second of

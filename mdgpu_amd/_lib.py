@@ -120,6 +120,10 @@ _SIGNATURES.update({
     "mdgpu_solve_status": (C.c_int, [_P, C.POINTER(C.c_int), _P]),
     "mdgpu_row_union": (C.c_int, [_I32P, C.c_int, _I32P, C.c_int, _I32P, C.POINTER(C.c_int)]),
     "mdgpu_row_map": (C.c_int, [_I32P, C.c_int, _I32P, C.c_int, _I32P]),
+    "mdgpu_backward_error": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, _DP, _P]),
+    "mdgpu_analyze_budgets": (C.c_int, [C.c_int, _I32P, _I32P, C.c_int, C.c_int, _I32P, C.POINTER(_P)]),
+    "mdgpu_tune_partition": (C.c_int, [C.POINTER(_P), C.c_int, _P, C.c_int, C.c_int, C.c_int, _P, _DP,
+                                       C.POINTER(C.c_int)]),
     "mdgpu_selinv": (C.c_int, [_P, _P]),
     "mdgpu_selinv_export": (C.c_int, [_P, _DP]),
     "mdgpu_logdet_grad": (C.c_int, [_P, _P, _P]),

@@ -284,3 +284,35 @@ extern "C" int mdgpu_row_map(const int32_t* super, int n, const int32_t* sub, in
         return fail(MDGPU_ERR_ARG, "mdgpu_row_map: a row of the sublist is not in the list");
     return MDGPU_OK;
 }
+
+extern "C" int mdgpu_analyze_budgets(int n, const int32_t* Cp, const int32_t* Ci, int use_amd, int nbudgets,
+                                     const int32_t* budgets, mdgpu_symbolic** syms) {
+    mdgpu::clear_error();
+    if (!Cp || !Ci || !budgets || !syms || nbudgets <= 0)
+        return fail(MDGPU_ERR_ARG, "mdgpu_analyze_budgets: invalid argument");
+    for (int k = 0; k < nbudgets; k++) {
+        syms[k] = nullptr;
+        if (budgets[k] < 0) return fail(MDGPU_ERR_ARG, "mdgpu_analyze_budgets: budgets must be >= 0");
+    }
+    mdgpu::AnalyzeOptions o;
+    o.use_amd = use_amd != 0;
+    o.partition = mdgpu::PART_RELAXED;
+    std::vector<mdgpu::Symbolic> out;
+    try {
+        std::string err = mdgpu::analyze_budgets(n, Cp, Ci, o, std::vector<int>(budgets, budgets + nbudgets), out);
+        if (!err.empty()) return fail(MDGPU_ERR_ARG, "mdgpu_analyze_budgets: " + err);
+        for (int k = 0; k < nbudgets; k++) {
+            mdgpu_symbolic* h = new mdgpu_symbolic();
+            h->S = std::move(out[k]);
+            finish_handle(h);
+            syms[k] = h;
+        }
+    } catch (const std::bad_alloc&) {
+        for (int k = 0; k < nbudgets; k++) {
+            delete syms[k];
+            syms[k] = nullptr;
+        }
+        return fail(MDGPU_ERR_NOMEM, "mdgpu_analyze_budgets: out of host memory");
+    }
+    return MDGPU_OK;
+}

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
@@ -1040,4 +1041,55 @@ extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, vo
     if (changed) *changed = n64;
     (void)initial;
     return MDGPU_OK;
+}
+
+// ------------------------------------------------------ partition tuning
+extern "C" int mdgpu_tune_partition(const mdgpu_symbolic* const* syms, int nsym, const double* d_ax, int precision,
+                                    int reps, int solves, void* stream, double* ms, int* best) {
+    mdgpu::clear_error();
+    if (!syms || nsym <= 0 || !d_ax || !ms || !best || reps < 1 || solves < 0)
+        return fail(MDGPU_ERR_ARG, "mdgpu_tune_partition: invalid argument");
+    for (int k = 0; k < nsym; k++)
+        if (!syms[k] || syms[k]->S.n != syms[0]->S.n || syms[k]->S.nnz_in != syms[0]->S.nnz_in)
+            return fail(MDGPU_ERR_ARG, "mdgpu_tune_partition: the analyses are not of one matrix");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = syms[0]->S.n;
+    double* dbx = nullptr;
+    if (hipMalloc((void**)&dbx, 2 * sizeof(double) * (size_t)n) != hipSuccess)
+        return fail(MDGPU_ERR_NOMEM, "mdgpu_tune_partition: hipMalloc failed");
+    mdgpu::launch_fill<double>(dbx, n, 1.0, s);
+    auto wall = [] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+            .count();
+    };
+    int st = MDGPU_OK;
+    *best = 0;
+    for (int k = 0; k < nsym && st == MDGPU_OK; k++) {
+        mdgpu_numeric* N = nullptr;
+        st = mdgpu_numeric_create(syms[k], precision, &N);
+        if (st == MDGPU_OK) st = mdgpu_numeric_autotune(N, d_ax, stream, nullptr);
+        double tf = 0, ts = 0;
+        if (st == MDGPU_OK) st = mdgpu_factorize(N, d_ax, stream);   // graph capture, warm
+        for (int r = 0; r < reps && st == MDGPU_OK; r++) {
+            const double t0 = wall();
+            st = mdgpu_factorize(N, d_ax, stream);                    // ends with a stream sync
+            tf += wall() - t0;
+        }
+        if (st == MDGPU_OK && solves > 0) {
+            st = mdgpu_solve(N, dbx, dbx + n, stream);                // warm
+            if (st == MDGPU_OK && hipStreamSynchronize(s) != hipSuccess)
+                st = fail(MDGPU_ERR_DEVICE, "mdgpu_tune_partition: hipStreamSynchronize failed");
+            const double t0 = wall();
+            for (int r = 0; r < reps * solves && st == MDGPU_OK; r++) st = mdgpu_solve(N, dbx, dbx + n, stream);
+            if (st == MDGPU_OK && hipStreamSynchronize(s) != hipSuccess)
+                st = fail(MDGPU_ERR_DEVICE, "mdgpu_tune_partition: hipStreamSynchronize failed");
+            ts = wall() - t0;
+        }
+        mdgpu_numeric_destroy(N);
+        if (st != MDGPU_OK) break;
+        ms[k] = (tf + ts) / reps;   // factorization + `solves` solves
+        if (ms[k] < ms[*best]) *best = k;
+    }
+    (void)hipFree(dbx);
+    return st;
 }

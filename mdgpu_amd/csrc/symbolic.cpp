@@ -364,19 +364,33 @@ class MinDegreeOrdering {
 void first_min_selection_sort(std::vector<int>& a, const std::vector<int>& key) {
     const int n = (int)a.size();
     if (n < 2) return;
-    std::map<int, std::set<int>> where;
-    for (int t = 0; t < n; t++) where[key[a[t]]].insert(t);
+    // a non-decreasing array is left as it is (every step picks position i)
+    bool sorted = true;
+    for (int t = 1; t < n && sorted; t++) sorted = key[a[t - 1]] <= key[a[t]];
+    if (sorted) return;
+    // The sort itself, step by step: step i swaps a[i] with the first
+    // minimum of a[i..n).  A segment tree over the positions holds (key,
+    // position) with done positions at +inf, so the root is that first
+    // minimum; a swap updates two leaves.
+    int size = 1;
+    while (size < n) size <<= 1;
+    const long long kInf = LLONG_MAX;
+    std::vector<long long> tree(2 * size, kInf);   // key * 2^32 + position
+    auto val = [&](int t) { return ((long long)key[a[t]] << 32) | (long long)t; };
+    for (int t = 0; t < n; t++) tree[size + t] = val(t);
+    for (int v = size - 1; v >= 1; v--) tree[v] = std::min(tree[2 * v], tree[2 * v + 1]);
+    auto set_leaf = [&](int t, long long x) {
+        int v = size + t;
+        tree[v] = x;
+        for (v >>= 1; v >= 1; v >>= 1) tree[v] = std::min(tree[2 * v], tree[2 * v + 1]);
+    };
     for (int i = 0; i < n; i++) {
-        auto it = where.begin();
-        const int x = *it->second.begin();
-        it->second.erase(it->second.begin());
+        const int x = (int)(tree[1] & 0xffffffffLL);
         if (x != i) {
-            auto& s = where[key[a[i]]];
-            s.erase(i);
-            s.insert(x);
             std::swap(a[i], a[x]);
+            set_leaf(x, val(x));
         }
-        if (it->second.empty()) where.erase(it);
+        set_leaf(i, kInf);
     }
 }
 
@@ -803,53 +817,73 @@ std::string read_lower(int n, const int* Cp, const int* Ci, Csc& low) {
 
 }  // namespace
 
-std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts, Symbolic& S) {
-    S = Symbolic();
-    S.n = n;
-    S.opts = opts;
+namespace {
+
+// Steps of the analysis that do not depend on the node partition: the
+// fill-reducing ordering, elimination tree, postorder, column counts and
+// the plain node tree.
+struct AnalysisPrefix {
+    int n = 0, nnz_in = 0;
+    std::vector<int> amd_perm, parent, post, count;
+    Csc a_low;                // AMD-permuted lower triangle
+    std::vector<int> src;     // a_low entry -> caller entry
+    std::vector<PartNode> plain;
+};
+
+std::string analysis_prefix(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts, AnalysisPrefix& A) {
+    A.n = n;
     Csc low;
     std::string verr = read_lower(n, Cp, Ci, low);
     if (!verr.empty()) return verr;
     const int nz = low.p[n];
-    S.nnz_in = nz;
+    A.nnz_in = nz;
     std::vector<int> ident(nz);
     for (int p = 0; p < nz; p++) ident[p] = p;
 
     // 1. fill-reducing ordering on the upper triangle (mdgpu.cu:1745-1773)
-    Csc a_low;
-    std::vector<int> src;
     if (opts.use_amd) {
-        S.amd_perm = amd_order(transpose(low));
+        A.amd_perm = amd_order(transpose(low));
         std::vector<int> pinv(n);
-        for (int k = 0; k < n; k++) pinv[S.amd_perm[k]] = k;
-        a_low = permute_lower(low, pinv, ident, src);
+        for (int k = 0; k < n; k++) pinv[A.amd_perm[k]] = k;
+        A.a_low = permute_lower(low, pinv, ident, A.src);
     } else {
-        S.amd_perm.resize(n);
-        for (int k = 0; k < n; k++) S.amd_perm[k] = k;
-        a_low = low;
-        src = ident;
+        A.amd_perm.resize(n);
+        for (int k = 0; k < n; k++) A.amd_perm[k] = k;
+        A.a_low = low;
+        A.src = ident;
     }
-    Csc full = full_from_lower(a_low);
+    Csc full = full_from_lower(A.a_low);
 
     // 2. etree, postorder, column counts (mdgpu.cu:1783-1785)
-    S.parent = etree(full);
-    S.post = postorder(S.parent);
-    S.count = column_counts(full, S.parent, S.post);
-
-    // 3. node partition (mdgpu.cu:1803-1853)
-    std::vector<PartNode> plain;
+    A.parent = etree(full);
+    A.post = postorder(A.parent);
+    A.count = column_counts(full, A.parent, A.post);
     std::vector<int> node_of_col;
-    std::string err = plain_tree(S.parent, S.post, S.count, opts.single_tree, plain, node_of_col);
-    if (!err.empty()) return err;
+    return plain_tree(A.parent, A.post, A.count, opts.single_tree, A.plain, node_of_col);
+}
+
+// 3. node partition (mdgpu.cu:1803-1853) and the fronts, on a prefix
+std::string analysis_finish(const AnalysisPrefix& A, const AnalyzeOptions& opts, Symbolic& S) {
+    const int n = A.n;
+    S = Symbolic();
+    S.n = n;
+    S.opts = opts;
+    S.nnz_in = A.nnz_in;
+    S.amd_perm = A.amd_perm;
+    S.parent = A.parent;
+    S.post = A.post;
+    S.count = A.count;
+    Csc a_low = A.a_low;
+    std::vector<int> src = A.src;
     S.perm = S.amd_perm;
     S.count_final = S.count;
     if (opts.partition == PART_PLAIN) {
-        S.nodes = std::move(plain);
+        S.nodes = A.plain;
     } else if (opts.partition == PART_SUPERNODAL) {
-        supernodal_tree(plain, S.nodes);
+        supernodal_tree(A.plain, S.nodes);
     } else if (opts.partition == PART_RELAXED) {
         std::vector<int> padded = S.count;
-        err = relaxed_tree(plain, opts.max_zeros, padded, S.relaxed_perm, S.nodes);
+        std::string err = relaxed_tree(A.plain, opts.max_zeros, padded, S.relaxed_perm, S.nodes);
         if (!err.empty()) return err;
         std::vector<int> rpinv(n);
         for (int k = 0; k < n; k++) rpinv[S.relaxed_perm[k]] = k;
@@ -872,8 +906,33 @@ std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& o
         S.flops_counts += (double)S.count[k] * S.count[k];
         S.flops_padded += (double)S.count_final[k] * S.count_final[k] + 1.0;
     }
-
     return build_fronts(S, true);
+}
+
+}  // namespace
+
+std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts, Symbolic& S) {
+    AnalysisPrefix A;
+    S = Symbolic();
+    std::string err = analysis_prefix(n, Cp, Ci, opts, A);
+    if (!err.empty()) return err;
+    return analysis_finish(A, opts, S);
+}
+
+std::string analyze_budgets(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts,
+                            const std::vector<int>& max_zeros, std::vector<Symbolic>& out) {
+    AnalysisPrefix A;
+    std::string err = analysis_prefix(n, Cp, Ci, opts, A);
+    if (!err.empty()) return err;
+    out.assign(max_zeros.size(), Symbolic());
+    for (size_t k = 0; k < max_zeros.size(); k++) {
+        AnalyzeOptions o = opts;
+        o.partition = PART_RELAXED;
+        o.max_zeros = max_zeros[k];
+        err = analysis_finish(A, o, out[k]);
+        if (!err.empty()) return err;
+    }
+    return "";
 }
 
 std::string analyze_with_nodes(int n, const int* Cp, const int* Ci, const std::vector<PartNode>& nodes,

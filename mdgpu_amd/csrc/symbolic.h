@@ -85,6 +85,12 @@ std::vector<int> column_counts(const Csc& full, const std::vector<int>& parent,
 std::string analyze(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts,
                     Symbolic& out);
 
+// The relaxed analyses of one matrix for several zero budgets, sharing the
+// ordering, elimination tree, postorder and column counts: out[k] equals
+// analyze() with partition = PART_RELAXED and max_zeros = max_zeros[k].
+std::string analyze_budgets(int n, const int* Cp, const int* Ci, const AnalyzeOptions& opts,
+                            const std::vector<int>& max_zeros, std::vector<Symbolic>& out);
+
 // Partition builders on a plain (one column per node) postordered tree:
 // fundamental supernodes (GetSuperETree, etree.cpp:50-118) and relaxed
 // supernodes (GetRelaxedETree, etree.cpp:194-342; pads `count`, returns the

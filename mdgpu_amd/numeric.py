@@ -195,3 +195,41 @@ class Factorization:
             self.close()
         except Exception:
             pass
+
+
+def tune_partition(syms, ax, precision="fp32", reps=2, solves=1):
+    """Partition tuner: for each analysis in ``syms`` (one matrix, e.g.
+    ``symbolic.analyze_budgets``) time a factorization plus ``solves``
+    solves on the GPU (GEMM tiles autotuned first).  Returns (index of the
+    fastest, [ms per analysis])."""
+    torch = _torch()
+    ax = ax.to(device="cuda", dtype=torch.float64).contiguous()
+    hs = (C.c_void_p * len(syms))(*[S.handle.value if isinstance(S.handle, C.c_void_p) else S.handle
+                                     for S in syms])
+    ms = np.zeros(len(syms))
+    best = C.c_int()
+    L.check(L.lib().mdgpu_tune_partition(hs, len(syms), _dptr(ax), PRECISIONS[precision], int(reps), int(solves),
+                                         _stream(), ms.ctypes.data_as(L._DP), C.byref(best)))
+    return best.value, ms.tolist()
+
+
+def backward_error(p, i, x, xs, b):
+    """eta = ||b - A x||_inf / (||A||_1 ||x||_inf + ||b||_inf) on the GPU for
+    the symmetric matrix with lower-triangle CSC (p, i, x) (host arrays),
+    xs and b device or host vectors (mdgpu_backward_error)."""
+    torch = _torch()
+    import scipy.sparse as sp
+    n = len(p) - 1
+    Lo = sp.csc_matrix((np.asarray(x, np.float64), np.asarray(i), np.asarray(p)), shape=(n, n))
+    A = (Lo + sp.tril(Lo, -1).T).tocsr()
+    A.sort_indices()
+    dev = dict(device="cuda")
+    rp = torch.from_numpy(A.indptr.astype(np.int32)).to(**dev)
+    ci = torch.from_numpy(A.indices.astype(np.int32)).to(**dev)
+    av = torch.from_numpy(A.data).to(**dev)
+    xs = torch.as_tensor(xs, dtype=torch.float64).to(**dev).contiguous()
+    b = torch.as_tensor(b, dtype=torch.float64).to(**dev).contiguous()
+    eta = C.c_double()
+    L.check(L.lib().mdgpu_backward_error(n, _dptr(rp), _dptr(ci), _dptr(av), _dptr(xs), _dptr(b), C.byref(eta),
+                                         _stream()))
+    return eta.value

@@ -92,6 +92,18 @@ def analyze(p, i, amd=True, mode="plain", max_zeros=0):
     return Symbolic(h, (p, i))
 
 
+def analyze_budgets(p, i, budgets, amd=True):
+    """Relaxed analyses (mode "rn") of one matrix for each zero budget in
+    ``budgets``, sharing the ordering, elimination tree and column counts;
+    element k equals ``analyze(p, i, amd, "rn", budgets[k])``."""
+    p, i = _i32(p), _i32(i)
+    b = _i32(budgets)
+    hs = (C.c_void_p * len(b))()
+    L.check(L.lib().mdgpu_analyze_budgets(len(p) - 1, p.ctypes.data_as(L._I32P), i.ctypes.data_as(L._I32P),
+                                          int(bool(amd)), len(b), b.ctypes.data_as(L._I32P), hs))
+    return [Symbolic(C.c_void_p(h), (p, i)) for h in hs]
+
+
 def analyze_batch(mats, amd=True, mode="plain", max_zeros=0, nthreads=0):
     """One analysis for a batch of independent matrices ``mats`` = [(p, i), ...]
     (lower-triangular CSC patterns): each matrix is analysed exactly as by

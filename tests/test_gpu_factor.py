@@ -399,3 +399,39 @@ def test_chained_wait_give_up_is_reported(device, monkeypatch):
     G = Factorization(S, "fp64").factorize(ax)
     xs = G.solve(bd).cpu().numpy()
     assert G.solve_status() == 0 and np.all(np.isfinite(xs))
+
+
+def test_partition_tuner_picks_a_budget_and_keeps_parity(device):
+    """mdgpu_tune_partition times each relaxed analysis of the budget sweep
+    (factorization + solves) and returns the fastest; the analysis it picks
+    factors to the oracle's L like any other."""
+    import torch
+    from mdgpu_amd.numeric import tune_partition
+    p, i, x = gen.stencil27(16, seed=1)
+    budgets = [250, 1000, 4000, 16000]
+    Ss = symbolic.analyze_budgets(p, i, budgets)
+    ax = torch.from_numpy(x).to(device)
+    best, ms = tune_partition(Ss, ax, "fp32", reps=2, solves=3)
+    assert 0 <= best < len(budgets) and len(ms) == len(budgets)
+    assert all(m > 0 for m in ms) and ms[best] == min(ms)
+    S = Ss[best]
+    lp, Li_ref, Lx_ref, _, bad = oracle_factor(p, i, x, S)
+    _, Li, Lx = Factorization(S, "fp64").factorize(ax).export_L()
+    assert bad == -1 and np.array_equal(Li, Li_ref) and rel_err(Lx, Lx_ref) <= 1e-10
+
+
+def test_backward_error_entry_matches_oracle(device):
+    """mdgpu_backward_error (the reference's eta check on the GPU) against
+    the oracle's backward_error of the same x, at perturbed solutions where
+    eta is far above rounding; matrix orders below, at and above one
+    workgroup's 1024 rows."""
+    from mdgpu_amd.numeric import backward_error
+    from oracle import oracle as O
+    rng = np.random.default_rng(21)
+    for p, i, x in [gen.laplace2d(32), gen.stencil27(12, seed=1), gen.laplace2d(100), gen.bbt_plus_ni(600, seed=2)]:
+        n = len(p) - 1
+        xt, b = gen.rhs_from_xtrue(p, i, x, seed=3)
+        xs = xt + 1e-6 * rng.standard_normal(n)
+        e_gpu = backward_error(p, i, x, xs, b)
+        e_cpu = O.backward_error(p, i, x, xs, b)
+        assert e_cpu > 1e-12 and abs(e_gpu - e_cpu) <= 1e-12 * e_cpu, (n, e_gpu, e_cpu)

@@ -20,7 +20,7 @@ import pytest
 
 from mdgpu_amd import _lib as L
 from mdgpu_amd import symbolic
-from tests.golden.golden_cases import LARGE_CASES, SMALL_CASES, input_hash
+from tests.golden.golden_cases import LARGE_CASES, SMALL_CASES, SWEEP_GRIDS, TUNER_BUDGETS, input_hash
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SMALL = np.load(os.path.join(HERE, "symbolic_small.npz"))
@@ -77,15 +77,12 @@ def _sha(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
 
 
-@pytest.mark.skipif(not HASHES, reason="large golden hashes not generated")
-@pytest.mark.parametrize("name,make,mode,mz", LARGE_CASES, ids=[c[0] for c in LARGE_CASES])
-def test_large_cases_hashes(name, make, mode, mz):
-    if name not in HASHES:
-        pytest.skip("no golden hash for this case")
-    p, i, _ = make()
-    ref = HASHES[name]
-    assert input_hash(p, i) == ref["input"]
-    S = symbolic.analyze(p, i, True, mode, mz)
+# the relaxed cases of the tuner's budgets other than 1000 are checked by the
+# budget sweep below (one ordering for all budgets of a grid)
+SINGLE_CASES = [c for c in LARGE_CASES if not (c[2] == "rn" and c[3] in TUNER_BUDGETS and c[3] != 1000)]
+
+
+def check_hashes(S, ref, mode):
     for k, (length, digest) in ((k, v) for k, v in ref.items() if k != "input"):
         if k == "nodes":
             if mode == "sn":
@@ -99,6 +96,31 @@ def test_large_cases_hashes(name, make, mode, mz):
             mine = S.array(ARR[k])
         assert len(mine) == length, k
         assert _sha(mine) == digest, k
+
+
+@pytest.mark.skipif(not HASHES, reason="large golden hashes not generated")
+@pytest.mark.parametrize("name,make,mode,mz", SINGLE_CASES, ids=[c[0] for c in SINGLE_CASES])
+def test_large_cases_hashes(name, make, mode, mz):
+    if name not in HASHES:
+        pytest.skip("no golden hash for this case")
+    p, i, _ = make()
+    ref = HASHES[name]
+    assert input_hash(p, i) == ref["input"]
+    check_hashes(symbolic.analyze(p, i, True, mode, mz), ref, mode)
+
+
+@pytest.mark.skipif(not HASHES, reason="large golden hashes not generated")
+@pytest.mark.parametrize("grid,make", SWEEP_GRIDS, ids=[g for g, _ in SWEEP_GRIDS])
+def test_budget_sweep_matches_reference(grid, make):
+    """mdgpu_analyze_budgets (one ordering, one relaxed partition per zero
+    budget: the amalgamation tuner's candidates) gives, for every budget,
+    the reference's relaxed analysis of the full-size grid bit for bit."""
+    p, i, _ = make()
+    Ss = symbolic.analyze_budgets(p, i, TUNER_BUDGETS)
+    for mz, S in zip(TUNER_BUDGETS, Ss):
+        ref = HASHES[f"{grid}_rn{mz}"]
+        assert input_hash(p, i) == ref["input"]
+        check_hashes(S, ref, "rn")
 
 
 def test_forest_is_accepted_and_matches_reference_prefix():

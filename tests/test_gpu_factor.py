@@ -434,4 +434,6 @@ def test_backward_error_entry_matches_oracle(device):
         xs = xt + 1e-6 * rng.standard_normal(n)
         e_gpu = backward_error(p, i, x, xs, b)
         e_cpu = O.backward_error(p, i, x, xs, b)
-        assert e_cpu > 1e-12 and abs(e_gpu - e_cpu) <= 1e-12 * e_cpu, (n, e_gpu, e_cpu)
+        # the residual at eta ~1e-6 carries rounding of ~1e-16 ||A|| ||x||
+        # (summation orders differ): agreement to 1e-8 relative
+        assert e_cpu > 1e-12 and abs(e_gpu - e_cpu) <= 1e-8 * e_cpu, (n, e_gpu, e_cpu)

@@ -272,6 +272,16 @@ int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol,
 int mdgpu_backward_error(int n, const int32_t* d_rowptr, const int32_t* d_colind, const double* d_val,
                          const double* d_x, const double* d_b, double* eta, void* stream);
 
+/* mdgpu_backward_error without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The workspace is a static device array of the library (131 KB per
+ * device, held until the process ends), so n is limited to 4,194,304 rows
+ * (MDGPU_ERR_ARG beyond), and calls that overlap on one device race (each
+ * call synchronizes its stream, so calls from one host thread do not). */
+int mdgpu_backward_error_noalloc(int n, const int32_t* d_rowptr, const int32_t* d_colind, const double* d_val,
+                                 const double* d_x, const double* d_b, double* eta, void* stream);
+
 /* Status of the solves run since the last call: MDGPU_SOLVE_TIMED_OUT is set
  * when a wait inside a chained block solve gave up (about a second of
  * polling for a value that never came): the solution then holds NaN where

@@ -121,6 +121,7 @@ _SIGNATURES.update({
     "mdgpu_row_union": (C.c_int, [_I32P, C.c_int, _I32P, C.c_int, _I32P, C.POINTER(C.c_int)]),
     "mdgpu_row_map": (C.c_int, [_I32P, C.c_int, _I32P, C.c_int, _I32P]),
     "mdgpu_backward_error": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, _DP, _P]),
+    "mdgpu_backward_error_noalloc": (C.c_int, [C.c_int, _P, _P, _P, _P, _P, _DP, _P]),
     "mdgpu_analyze_budgets": (C.c_int, [C.c_int, _I32P, _I32P, C.c_int, C.c_int, _I32P, C.POINTER(_P)]),
     "mdgpu_tune_partition": (C.c_int, [C.POINTER(_P), C.c_int, _P, C.c_int, C.c_int, C.c_int, _P, _DP,
                                        C.POINTER(C.c_int)]),

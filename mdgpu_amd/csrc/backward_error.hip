@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -91,14 +92,15 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(MDGPU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Launches into `scratch` (4 nblocks + 4 doubles), copies the four maxima to
-// the host and forms eta.  Synchronizes the stream.
+// Launches into the workspace ws (4 nblocks + 4 doubles of device memory),
+// copies the four maxima to the host and forms eta.  Synchronizes the
+// stream.
 int backward_error_into(int n, const int32_t* rp, const int32_t* ci, const double* av, const double* x,
-                        const double* b, double* scratch, double* eta, hipStream_t s, const char* who) {
+                        const double* b, double* ws, double* eta, hipStream_t s, const char* who) {
     const int nblocks = (n + kBeRowsPerBlock - 1) / kBeRowsPerBlock;
-    double* out = scratch + 4 * (int64_t)nblocks;
-    be_rows_kernel<<<nblocks, kBeThreads, 0, s>>>(n, rp, ci, av, x, b, scratch);
-    be_final_kernel<<<1, kBeThreads, 0, s>>>(scratch, nblocks, out);
+    double* out = ws + 4 * (int64_t)nblocks;
+    be_rows_kernel<<<nblocks, kBeThreads, 0, s>>>(n, rp, ci, av, x, b, ws);
+    be_final_kernel<<<1, kBeThreads, 0, s>>>(ws, nblocks, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, who);
     double h[4];
@@ -126,10 +128,65 @@ extern "C" int mdgpu_backward_error(int n, const int32_t* d_rowptr, const int32_
     int st = check_args(n, d_rowptr, d_colind, d_val, d_x, d_b, eta, who);
     if (st != MDGPU_OK) return st;
     const int nblocks = (n + kBeRowsPerBlock - 1) / kBeRowsPerBlock;
-    double* scratch = nullptr;
-    hipError_t e = hipMalloc((void**)&scratch, sizeof(double) * (4 * (size_t)nblocks + 4));
+    double* ws = nullptr;
+    hipError_t e = hipMalloc((void**)&ws, sizeof(double) * (4 * (size_t)nblocks + 4));
     if (e != hipSuccess) return fail(MDGPU_ERR_NOMEM, std::string(who) + ": hipMalloc failed");
-    st = backward_error_into(n, d_rowptr, d_colind, d_val, d_x, d_b, scratch, eta, (hipStream_t)stream, who);
-    (void)hipFree(scratch);
+    st = backward_error_into(n, d_rowptr, d_colind, d_val, d_x, d_b, ws, eta, (hipStream_t)stream, who);
+    (void)hipFree(ws);
     return st;
+}
+
+// ------------------------------------------------ no-allocation variant
+// mdgpu_backward_error_noalloc: the same kernels with their workspace in a
+// file-scope device array, for callers that cannot allocate (code under
+// graph capture, serving loops).  Every device the library runs on holds
+// one copy until the process ends; all calls on a device share it, so
+// calls that overlap on one device race.  Host code uses the array only
+// through the device address hipGetSymbolAddress gives (its name is a
+// host-side stand-in), looked up once per device.
+constexpr int kNoAllocMaxBlocks = 4096;                                      // n <= 4096 * 1024 rows
+static __device__ __align__(256) double scratch[4 * kNoAllocMaxBlocks + 4];  // 131 KB
+
+namespace {
+
+constexpr int kNoAllocMaxRows = kNoAllocMaxBlocks * kBeRowsPerBlock;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up on the
+// first call there)
+int scratch_ptr(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "mdgpu_backward_error_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(MDGPU_ERR_DEVICE, "mdgpu_backward_error_noalloc: device index");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "mdgpu_backward_error_noalloc: hipGetSymbolAddress");
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MDGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int mdgpu_backward_error_noalloc(int n, const int32_t* d_rowptr, const int32_t* d_colind,
+                                            const double* d_val, const double* d_x, const double* d_b, double* eta,
+                                            void* stream) {
+    mdgpu::clear_error();
+    const char* who = "mdgpu_backward_error_noalloc";
+    int st = check_args(n, d_rowptr, d_colind, d_val, d_x, d_b, eta, who);
+    if (st != MDGPU_OK) return st;
+    if (n > kNoAllocMaxRows)
+        return fail(MDGPU_ERR_ARG, std::string(who) + ": n = " + std::to_string(n) + " exceeds the limit of " +
+                                       std::to_string(kNoAllocMaxRows) + " rows of the static workspace");
+    double* ws = nullptr;
+    st = scratch_ptr(&ws);
+    if (st != MDGPU_OK) return st;
+    // every slot the kernels read is written by them first: nothing to clear
+    return backward_error_into(n, d_rowptr, d_colind, d_val, d_x, d_b, ws, eta, (hipStream_t)stream, who);
 }

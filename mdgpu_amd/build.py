@@ -37,6 +37,35 @@ def _run(cmd):
     return r.stderr
 
 
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized():
+    """Host code (the analysis, plan, drop-in and C ABI: every *.cpp) built
+    with AddressSanitizer + UndefinedBehaviorSanitizer into
+    libmdgpu_amd_san.so; the device code is compiled as usual.  Run the CPU
+    tests against it with MDGPU_AMD_LIB pointing at it and the gcc
+    sanitizer runtimes preloaded."""
+    obj = os.path.join(HERE, "_build_san")
+    os.makedirs(obj, exist_ok=True)
+    jobs = []
+    for src in sorted(glob.glob(os.path.join(CSRC, "*.cpp"))):
+        base = os.path.basename(src)
+        flots = ("float", "double") if base == "dropin.cpp" else (None,)
+        for t in flots:
+            extra = [f"-DFLOTANTE={t}"] if t else []
+            o = os.path.join(obj, base + (f".{t}" if t else "") + ".o")
+            jobs.append(["g++", *HOST_FLAGS, *SAN_FLAGS, *extra, "-c", src, "-o", o])
+    for src in sorted(glob.glob(os.path.join(CSRC, "*.hip"))):
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", os.path.join(obj, os.path.basename(src) + ".o")])
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        list(ex.map(_run, jobs))
+    out = os.path.join(HERE, "libmdgpu_amd_san.so")
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", out, *[j[-1] for j in jobs],
+          "-lasan", "-lubsan"])
+    return out
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     jobs = []
@@ -68,4 +97,7 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--sanitize" in sys.argv:
+        print(build_sanitized())
+    else:
+        print(build(verbose="-v" in sys.argv))

@@ -94,6 +94,15 @@ void launch_maxabs(const double* v, int n, unsigned long long* out, hipStream_t 
 // out[seg[i]] = max |v[i]| per contiguous segment (out cleared by the caller)
 void launch_maxabs_seg(const double* v, const int* seg, int n, unsigned long long* out, hipStream_t s);
 void launch_axpy(double* x, const double* d, int n, hipStream_t s);
+// conjugate-gradient refinement: y = A x; *out = a . c (partial: kDotParts
+// doubles; fixed summation order); p = z (+ (rz / rz_old) p); x += (rz /
+// pq) p, with scal = {rz, rz_old, pq} on the device
+constexpr int kDotParts = 256;
+void launch_spmv(const int* rp, const int* ci, const double* av, const double* x, double* y, int n, hipStream_t s);
+void launch_dot(const double* a, const double* c, int n, double* partial, double* out, hipStream_t s);
+void launch_cg_dir(double* p, const double* z, const double* scal, int n, bool first, hipStream_t s);
+void launch_cg_step(double* x, const double* p, const double* scal, int n, hipStream_t s);
+void launch_copy_scalar(double* dst, const double* src, hipStream_t s);
 template <typename T>
 void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial, int nparts, hipStream_t s);
 

@@ -1830,6 +1830,60 @@ __global__ void axpy_kernel(double* __restrict__ x, const double* __restrict__ d
     if (i < n) x[i] += d[i];
 }
 
+// ------------------------------------------- conjugate-gradient refinement
+// y = A x (FP64 CSR SpMV, 16 lanes per row)
+__global__ __launch_bounds__(kThreads) void spmv_kernel(const int* __restrict__ rp, const int* __restrict__ ci,
+                                                        const double* __restrict__ av, const double* __restrict__ x,
+                                                        double* __restrict__ y, int n) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes, l = threadIdx.x % kRowLanes;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int p = rp[i] + l; p < rp[i + 1]; p += kRowLanes) s += av[p] * x[ci[p]];
+#pragma unroll
+    for (int o = kRowLanes / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o, kRowLanes);
+    if (l == 0) y[i] = s;
+}
+
+// partial[b] = sum of a_i c_i over the entries of workgroup b (grid-stride,
+// fixed order), then dot_final_kernel sums the kDotParts partials in a
+// fixed tree: the dot product is the same on every run
+__global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __restrict__ a,
+                                                               const double* __restrict__ c, int n,
+                                                               double* __restrict__ partial) {
+    __shared__ double red[kThreads / 64];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += a[i] * c[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+__global__ __launch_bounds__(kDotParts) void dot_final_kernel(const double* __restrict__ partial,
+                                                              double* __restrict__ out) {
+    __shared__ double red[kDotParts];
+    red[threadIdx.x] = partial[threadIdx.x];
+    __syncthreads();
+    for (int w = kDotParts / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+// CG direction: p = z (first) or p = z + (rz / rz_old) p; scal = {rz, rz_old, pq}
+__global__ void cg_dir_kernel(double* __restrict__ p, const double* __restrict__ z, const double* __restrict__ scal,
+                              int n, int first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p[i] = first ? z[i] : z[i] + (scal[0] / scal[1]) * p[i];
+}
+// x += (rz / pq) p
+__global__ void cg_step_kernel(double* __restrict__ x, const double* __restrict__ p,
+                               const double* __restrict__ scal, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (scal[0] / scal[2]) * p[i];
+}
+__global__ void copy_scalar_kernel(double* __restrict__ dst, const double* __restrict__ src) { *dst = *src; }
+
 // partial sums of log(L_jj), fixed reduction tree per block
 template <typename T>
 __global__ __launch_bounds__(kThreads) void logdiag_kernel(const int64_t* __restrict__ diag_off,
@@ -1981,6 +2035,20 @@ void launch_maxabs_seg(const double* v, const int* seg, int n, unsigned long lon
 void launch_axpy(double* x, const double* d, int n, hipStream_t s) {
     axpy_kernel<<<blocks_for(n), kThreads, 0, s>>>(x, d, n);
 }
+void launch_spmv(const int* rp, const int* ci, const double* av, const double* x, double* y, int n, hipStream_t s) {
+    spmv_kernel<<<blocks_for((int64_t)n * kRowLanes), kThreads, 0, s>>>(rp, ci, av, x, y, n);
+}
+void launch_dot(const double* a, const double* c, int n, double* partial, double* out, hipStream_t s) {
+    dot_partial_kernel<<<kDotParts, kThreads, 0, s>>>(a, c, n, partial);
+    dot_final_kernel<<<1, kDotParts, 0, s>>>(partial, out);
+}
+void launch_cg_dir(double* p, const double* z, const double* scal, int n, bool first, hipStream_t s) {
+    cg_dir_kernel<<<blocks_for(n), kThreads, 0, s>>>(p, z, scal, n, first ? 1 : 0);
+}
+void launch_cg_step(double* x, const double* p, const double* scal, int n, hipStream_t s) {
+    cg_step_kernel<<<blocks_for(n), kThreads, 0, s>>>(x, p, scal, n);
+}
+void launch_copy_scalar(double* dst, const double* src, hipStream_t s) { copy_scalar_kernel<<<1, 1, 0, s>>>(dst, src); }
 template <typename T>
 void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial, int nparts, hipStream_t s) {
     logdiag_kernel<T><<<nparts, kThreads, 0, s>>>(diag_off, L, n, partial);

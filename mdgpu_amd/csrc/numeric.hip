@@ -72,6 +72,10 @@ struct mdgpu_numeric {
     double* dXp = nullptr;
     double* dR = nullptr;
     double* dD = nullptr;
+    double* dP = nullptr;                 // CG refinement: search direction, A p
+    double* dQ2 = nullptr;
+    double* dScal = nullptr;              // CG scalars {rz, rz_old, pq} + dot partials
+    bool refine_cg = true;                // CG refinement (false: classical refinement)
     double* dRowabs = nullptr;
     int64_t* dDiagOff = nullptr;
     int* dInfo = nullptr;
@@ -493,6 +497,9 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dXp)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dR)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dD)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dP)) ||
+        (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dQ2)) ||
+        (st = alloc(N, (8 + mdgpu::kDotParts) * sizeof(double), (void**)&N->dScal)) ||
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dRowabs)) ||
         (st = upload(N, diag_off, (void**)&N->dDiagOff)) ||
         (st = alloc(N, 2 * sizeof(int), (void**)&N->dInfo)) ||
@@ -524,6 +531,9 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     // polls before a chained-solve wait gives up (a smaller value makes the
     // give-up path observable without a stalled producer)
     if (const char* e = std::getenv("MDGPU_CHAIN_SPIN")) N->chain_spin = std::max(0, std::atoi(e));
+    // refinement method: conjugate gradients preconditioned by the factor
+    // (default) or classical iterative refinement ("ir")
+    if (const char* e = std::getenv("MDGPU_REFINE")) N->refine_cg = std::string(e) != "ir";
     N->events.assign(P.nevents, nullptr);
     for (hipEvent_t& e : N->events)
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
@@ -712,17 +722,44 @@ extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, 
     hipStream_t s = (hipStream_t)stream;
     CallOrder order(num, s);
     const int n = num->sym->S.n;
-    TRY(solve_any(num, d_b, d_x, s));
+    // Solve k (k = 0, 1, ...) applies the factor M = L L^T to the residual
+    // r_k = b - A x_k.  Classical refinement: x_{k+1} = x_k + M^-1 r_k
+    // (x_0 = 0).  Conjugate gradients preconditioned by M: the step along
+    // p_k = M^-1 r_k + beta_k p_{k-1} of optimal length, which minimizes
+    // the A-norm error over the same Krylov space the classical iterates
+    // lie in; the residual is recomputed from x every step (true residual).
+    // The backward error (mdgpu.cu:1972) of x_{k+1} decides after solve k.
+    double* r = num->dR;
+    double* z = num->dD;
+    double* scal = num->dScal;                  // {rz, rz_old, pq}, then the dot partials
+    double* part = num->dScal + 8;
     std::vector<double> bn, rn, xn;
     double eta = INFINITY;
     int it = 0;
     num->eta_seg.assign(num->nseg, INFINITY);
+    hipError_t e = hipMemcpyAsync(r, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "mdgpu_refine: copy");
     for (;; it++) {
-        mdgpu::launch_residual(num->dAfp, num->dAfi, num->dAfx, d_x, d_b, num->dR, n, s);
+        if (num->refine_cg) {
+            TRY(solve_any(num, r, z, s));
+            mdgpu::launch_dot(r, z, n, part, scal + 0, s);
+            mdgpu::launch_cg_dir(num->dP, z, scal, n, it == 0, s);
+            mdgpu::launch_spmv(num->dAfp, num->dAfi, num->dAfx, num->dP, num->dQ2, n, s);
+            mdgpu::launch_dot(num->dP, num->dQ2, n, part, scal + 2, s);
+            if (it == 0) mdgpu::launch_fill<double>(d_x, n, 0.0, s);
+            mdgpu::launch_cg_step(d_x, num->dP, scal, n, s);
+            mdgpu::launch_copy_scalar(scal + 1, scal + 0, s);
+        } else if (it == 0) {
+            TRY(solve_any(num, r, d_x, s));
+        } else {
+            TRY(solve_any(num, r, z, s));
+            mdgpu::launch_axpy(d_x, z, n, s);
+        }
+        mdgpu::launch_residual(num->dAfp, num->dAfi, num->dAfx, d_x, d_b, r, n, s);
         int status = 0;
         {
             // |b| with the first step's |r| and |x|: one round trip per step
-            const double* vs[3] = {num->dR, d_x, d_b};
+            const double* vs[3] = {r, d_x, d_b};
             std::vector<double>* os[3] = {&rn, &xn, &bn};
             TRY(max_abs_n(num, vs, it == 0 ? 3 : 2, n, s, os, &status));
         }
@@ -736,17 +773,15 @@ extern "C" int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, 
         eta = 0;
         for (int k = 0; k < num->nseg; k++) {
             const double den = num->anorm_seg[k] * xn[k] + bn[k];
-            const double e = den > 0 ? rn[k] / den : rn[k];
-            num->eta_seg[k] = e;
-            eta = (e > eta || e != e) ? e : eta;
+            const double ek = den > 0 ? rn[k] / den : rn[k];
+            num->eta_seg[k] = ek;
+            eta = (ek > eta || ek != ek) ? ek : eta;
         }
         if (eta <= tol || it >= maxit || !(eta == eta)) break;
-        TRY(solve_any(num, num->dR, num->dD, s));
-        mdgpu::launch_axpy(d_x, num->dD, n, s);
     }
     if (eta_out) *eta_out = eta;
     if (iters_out) *iters_out = it;
-    hipError_t e = hipGetLastError();
+    e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "refine launch");
     if (!(eta <= tol))
         return fail(MDGPU_ERR_NOT_CONVERGED, "mdgpu_refine: backward error " + std::to_string(eta) +

@@ -11,12 +11,23 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// C = alpha op(A) op(B) + beta C, 64x64 tile per workgroup, 4x4 per thread,
-// K staged through LDS 16 at a time.  Tiles of many jobs in one launch.
-__global__ __launch_bounds__(kThreads) void sel_gemm_kernel(const SelGemm* __restrict__ jobs, int njobs,
-                                                            SelBuffers bufs) {
-    __shared__ double As[16][64 + 1];
-    __shared__ double Bs[16][64 + 1];
+// C = alpha op(A) op(B) + beta C, 64x64 tile per workgroup, tiles of many
+// jobs in one launch, on v_mfma_f64_16x16x4f64: four wavefronts of 32x32
+// (2x2 blocks of 16x16).  K is staged through two LDS buffers 16 at a time
+// (the next k-tile is loaded into registers while this one is multiplied),
+// each operand row-major with k contiguous: op(A)(i, k) and op(B)^T(j, k)
+// for both transpose flags, so one MFMA operand read is one 16-byte LDS
+// read.  The contraction index of one 16x16x4 instruction is free to
+// relabel: lane group lg = lane / 16 supplies k = 4 lg + t in sub-step t.
+// Accumulator register v of lane (16 lg + li) holds the C entry of row li
+// of the 16x16 block and column lg + 4 v (the product is formed as
+// op(B)^T op(A)^T).
+__global__ __launch_bounds__(kThreads, 2) void sel_gemm_kernel(const SelGemm* __restrict__ jobs, int njobs,
+                                                               SelBuffers bufs) {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    constexpr int BT = 64, KT = 16, LDK = KT + 2;
+    __shared__ __align__(16) double As[2][BT * LDK];
+    __shared__ __align__(16) double Bs[2][BT * LDK];
     int lo = 0, hi = njobs - 1;
     const int bid = blockIdx.x;
     while (lo < hi) {
@@ -29,52 +40,83 @@ __global__ __launch_bounds__(kThreads) void sel_gemm_kernel(const SelGemm* __res
     const double* __restrict__ A = bufs.p[g.a_buf] + g.a_off;
     const double* __restrict__ B = bufs.p[g.b_buf] + g.b_off;
     double* __restrict__ C = bufs.p[g.c_buf] + g.c_off;
-    const int row0 = ti * 64, col0 = tj * 64;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4];
+    const int row0 = ti * BT, col0 = tj * BT;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    // staging: each thread stages four k of one row per operand.  Rows
+    // contiguous in memory (op = no transpose for A, transpose for B): row
+    // tid % 64, k 4 (tid / 64) + e (loads 64 rows apart are coalesced);
+    // k contiguous: row tid / 4, k 4 (tid % 4) + e (one 32-byte run).
+    const bool a_rows = !g.ta, b_rows = g.tb;
+    const int ar = a_rows ? (tid & 63) : (tid >> 2), ak = a_rows ? 4 * (tid >> 6) : 4 * (tid & 3);
+    const int br = b_rows ? (tid & 63) : (tid >> 2), bk = b_rows ? 4 * (tid >> 6) : 4 * (tid & 3);
+    const int ia = row0 + ar, jb = col0 + br;
+    const bool a_ok = ia < g.M, b_ok = jb < g.N;
+    double ra[4], rb[4];
+    auto fetch = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc[r][s] = 0.0;
-    const int li = threadIdx.x & 63, lk = threadIdx.x >> 6;
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int kk = lk + 4 * s, k = k0 + kk;
-            const int i = row0 + li, j = col0 + li;
-            double a = 0.0, b = 0.0;
-            if (k < g.K && i < g.M) a = g.ta ? A[k + (int64_t)i * g.lda] : A[i + (int64_t)k * g.lda];
-            if (k < g.K && j < g.N) b = g.tb ? B[j + (int64_t)k * g.ldb] : B[k + (int64_t)j * g.ldb];
-            As[kk][li] = a;
-            Bs[kk][li] = b;
+        for (int e = 0; e < 4; ++e) {
+            const int k = k0 + ak + e, kb = k0 + bk + e;
+            const int kc = min(k, g.K - 1), kbc = min(kb, g.K - 1);
+            const int64_t ai = a_ok ? ia : 0, bj = b_ok ? jb : 0;
+            // op(A)(i, k): A[i + k lda] or, transposed, A[k + i lda]
+            ra[e] = g.ta ? A[kc + ai * g.lda] : A[ai + (int64_t)kc * g.lda];
+            // op(B)(k, j): B[k + j ldb] or, transposed, B[j + k ldb]
+            rb[e] = g.tb ? B[bj + (int64_t)kbc * g.ldb] : B[kbc + bj * g.ldb];
+            if (!(a_ok && k < g.K)) ra[e] = 0.0;
+            if (!(b_ok && kb < g.K)) rb[e] = 0.0;
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        *reinterpret_cast<d2*>(&As[buf][ar * LDK + ak]) = d2{ra[0], ra[1]};
+        *reinterpret_cast<d2*>(&As[buf][ar * LDK + ak + 2]) = d2{ra[2], ra[3]};
+        *reinterpret_cast<d2*>(&Bs[buf][br * LDK + bk]) = d2{rb[0], rb[1]};
+        *reinterpret_cast<d2*>(&Bs[buf][br * LDK + bk + 2]) = d2{rb[2], rb[3]};
+    };
+    acc_t acc[2][2];
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            double a[4], b[4];
+    for (int a2 = 0; a2 < 2; ++a2)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = As[kk][tx + 16 * r];
+        for (int b2 = 0; b2 < 2; ++b2) acc[a2][b2] = acc_t{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    store(0);
+    if (KT < g.K) fetch(KT);
+    __syncthreads();
+    for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+        if (k0 + KT < g.K) {
+            store(cur ^ 1);
+            if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+        }
+        const double* __restrict__ Ac = As[cur];
+        const double* __restrict__ Bc = Bs[cur];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) b[s] = Bs[kk][ty + 16 * s];
+        for (int t = 0; t < 4; ++t) {
+            double av[2], bv[2];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r2 = 0; r2 < 2; ++r2) av[r2] = Ac[(wr + 16 * r2 + li) * LDK + 4 * lg + t];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) acc[r][s] += a[r] * b[s];
+            for (int c2 = 0; c2 < 2; ++c2) bv[c2] = Bc[(wc + 16 * c2 + li) * LDK + 4 * lg + t];
+#pragma unroll
+            for (int r2 = 0; r2 < 2; ++r2)
+#pragma unroll
+                for (int c2 = 0; c2 < 2; ++c2)
+                    acc[r2][c2] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[c2], av[r2], acc[r2][c2], 0, 0, 0);
         }
         __syncthreads();
     }
+    // acc[r2][c2][v] = C(row0 + wr + 16 r2 + li, col0 + wc + 16 c2 + lg + 4 v)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int j = col0 + ty + 16 * s;
-        if (j >= g.N) continue;
+    for (int r2 = 0; r2 < 2; ++r2)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = row0 + tx + 16 * r;
-            if (i >= g.M) continue;
-            double* cp = C + i + (int64_t)j * g.ldc;
-            *cp = g.beta == 0.0 ? g.alpha * acc[r][s] : g.alpha * acc[r][s] + g.beta * *cp;
-        }
-    }
+        for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = row0 + wr + 16 * r2 + li, j = col0 + wc + 16 * c2 + lg + 4 * v;
+                if (i >= g.M || j >= g.N) continue;
+                double* cp = C + i + (int64_t)j * g.ldc;
+                *cp = g.beta == 0.0 ? g.alpha * acc[r2][c2][v] : g.alpha * acc[r2][c2][v] + g.beta * *cp;
+            }
 }
 
 // The copies of the selected inverse (SelCopyKind), one work item per

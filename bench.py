@@ -310,7 +310,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aniso", action="store_true", help="skip the z-anisotropic C3 variant")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile sizes")
-    ap.add_argument("--budgets", default="250,1000,2000,4000,8000,16000",
+    ap.add_argument("--budgets", default="250,1000,2000,4000,8000,16000,64000",
                     help="relaxed-amalgamation zero budgets the partition tuner tries per precision "
                          "(empty: --mode/--max-zeros for every precision)")
     ap.add_argument("--only", default="fp32,fp64,bf16x3,bf16",

@@ -69,9 +69,10 @@ LARGE_CASES = [
     ("c5_256_rn1000", gen.gmrf_precision, "rn", 1000),
 ]
 # zero budgets of the amalgamation tuner's search (max_zeros candidates)
-TUNER_BUDGETS = (250, 1000, 2000, 4000, 8000, 16000)
-SWEEP_GRIDS = [("c2_48", gen.laplace3d_7pt), ("c3_64", gen.stencil27)]
+TUNER_BUDGETS = (250, 1000, 2000, 4000, 8000, 16000, 64000)
+SWEEP_GRIDS = [("c2_48", gen.laplace3d_7pt), ("c3_64", gen.stencil27), ("c1_100", lambda: gen.laplace2d(100))]
 for _mz in TUNER_BUDGETS:
     if _mz != 1000:
         LARGE_CASES.append((f"c2_48_rn{_mz}", gen.laplace3d_7pt, "rn", _mz))
         LARGE_CASES.append((f"c3_64_rn{_mz}", gen.stencil27, "rn", _mz))
+    LARGE_CASES.append((f"c1_100_rn{_mz}", lambda: gen.laplace2d(100), "rn", _mz))

@@ -120,7 +120,9 @@ __global__ __launch_bounds__(kThreads, 2) void sel_gemm_kernel(const SelGemm* __
 }
 
 // The copies of the selected inverse (SelCopyKind), one work item per
-// workgroup: columns j of the item, lanes over rows (coalesced).
+// workgroup: the item's entries flattened (column by column, rows
+// contiguous, so reads and writes are coalesced wherever the layouts allow),
+// four entries per thread in flight.
 __global__ __launch_bounds__(kThreads) void sel_copy_kernel(int kind, const SelCopy* __restrict__ items,
                                                             const FrontDesc* __restrict__ fronts,
                                                             const int* __restrict__ rel,
@@ -129,32 +131,60 @@ __global__ __launch_bounds__(kThreads) void sel_copy_kernel(int kind, const SelC
     const FrontDesc d = fronts[it.front];
     double* __restrict__ ZF = bufs.p[SB_ZF];
     double* __restrict__ U = bufs.p[SB_U];
-    const int n = d.n, c = d.c, m = d.m;
+    const int n = d.n, c = d.c, m = d.m, ncol = it.j1 - it.j0;
     if (kind == SC_INIT) {
         double* __restrict__ z = ZF + zf_ofs[it.front];
-        for (int j = it.j0; j < it.j1; ++j)
-            for (int i = threadIdx.x; i < m; i += kThreads)
-                z[(c + i) + (int64_t)(c + j) * n] = U[d.uofs + i + (int64_t)j * m];
+        const int cnt = ncol * m;   // an item holds at most ~4096 entries or one column
+        for (int e0 = threadIdx.x; e0 < cnt; e0 += 4 * kThreads) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = min(e0 + u * kThreads, cnt - 1);
+                v[u] = U[d.uofs + e % m + (int64_t)(it.j0 + e / m) * m];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * kThreads;
+                if (e < cnt) z[(c + e % m) + (int64_t)(c + it.j0 + e / m) * n] = v[u];
+            }
+        }
     } else if (kind == SC_TRANS) {
         double* __restrict__ z = ZF + zf_ofs[it.front];
         const int r0 = it.b0 + it.b;                 // first row of R
-        for (int t = threadIdx.x; t < (it.j1 - it.j0) * it.b; t += kThreads) {
+        for (int t = threadIdx.x; t < ncol * it.b; t += kThreads) {
             const int jj = t % it.b, i = r0 + it.j0 + t / it.b;
             z[(it.b0 + jj) + (int64_t)i * n] = z[i + (int64_t)(it.b0 + jj) * n];
         }
     } else if (kind == SC_OUT) {
         const double* __restrict__ z = ZF + zf_ofs[it.front];
         double* __restrict__ Z = bufs.p[SB_Z] + d.lofs;
-        for (int j = it.j0; j < it.j1; ++j)
-            for (int i = threadIdx.x; i < n; i += kThreads) Z[i + (int64_t)j * n] = z[i + (int64_t)j * n];
+        const int64_t base = (int64_t)it.j0 * n;   // the item's columns are one contiguous run
+        const int cnt = ncol * n;
+        for (int e0 = threadIdx.x; e0 < cnt; e0 += 4 * kThreads) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = z[base + min(e0 + u * kThreads, cnt - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (e0 + u * kThreads < cnt) Z[base + e0 + u * kThreads] = v[u];
+        }
     } else {   // SC_PUSH: Z22 of child front it.front from its parent's ZF
         const FrontDesc pd = fronts[d.parent];
         const double* __restrict__ zp = ZF + zf_ofs[d.parent];
         const int* __restrict__ r = rel + d.relofs;
-        for (int j = it.j0; j < it.j1; ++j) {
-            const int pj = r[j];
-            for (int i = threadIdx.x; i < m; i += kThreads)
-                U[d.uofs + i + (int64_t)j * m] = zp[r[i] + (int64_t)pj * pd.n];
+        const int cnt = ncol * m;
+        for (int e0 = threadIdx.x; e0 < cnt; e0 += 4 * kThreads) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = min(e0 + u * kThreads, cnt - 1);
+                v[u] = zp[r[e % m] + (int64_t)r[it.j0 + e / m] * pd.n];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * kThreads;
+                if (e < cnt) U[d.uofs + e % m + (int64_t)(it.j0 + e / m) * m] = v[u];
+            }
         }
     }
 }

@@ -8,7 +8,7 @@ namespace mdgpu {
 namespace {
 
 constexpr int kTile = 64;
-constexpr int64_t kCopyChunk = 16384;   // elements per copy work item
+constexpr int64_t kCopyChunk = 4096;    // elements per copy work item
 
 int tiles(int x) { return (x + kTile - 1) / kTile; }
 

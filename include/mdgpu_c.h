@@ -219,10 +219,13 @@ int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream
                             double* out /* [4*MDGPU_KIND_N] */);
 
 /* Autotuner: factorizes d_ax with every GEMM launch of the schedule on 64-
- * and on 128-row MFMA tiles (HIP-event time per launch), keeps the faster
- * tile for each launch and installs that schedule (captured graphs are
- * recaptured on next use).  *changed = launches on 64-row tiles.  Ends with
- * a valid factorization of d_ax.  No counterpart in the reference. */
+ * and on 128-row 16x16x4 MFMA tiles and, in MDGPU_FP32, on the 32x32x2
+ * MFMA kernel (HIP-event time per launch), keeps the fastest kernel for each
+ * launch and installs that schedule (captured graphs are recaptured on next
+ * use).  *changed = launches on 64-row tiles.  Ends with a valid
+ * factorization of d_ax.  The FP32 kernels round differently, so an FP32
+ * factor can change in its last bits with the choice.  No counterpart in the
+ * reference. */
 int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, void* stream, int64_t* changed);
 
 /* Partition tuner (relaxed-amalgamation zero budget, etree.cpp:236-237):

@@ -15,6 +15,9 @@ PlanOptions plan_options_for(int precision) {
     // largest MFMA GEMM tile (rows = columns); the autotuner may pick 64
     // per launch (not for the bf16 kernels, which have 128-row tiles only)
     po.tile = 128;
+    // FP32: the 32x32x2 MFMA kernel by default (the autotuner may pick the
+    // 16x16x4 kernel's 64- or 128-row tiles per launch)
+    po.gemm_default = precision == MDGPU_FP32 ? kGemmMfma32 : 128;
     // outer block width of the blocked panel path (multiple of 64)
     if (const char* e = std::getenv("MDGPU_NB_OUTER")) {
         const int v = std::atoi(e);

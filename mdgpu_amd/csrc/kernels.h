@@ -30,9 +30,11 @@ void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int
                 hipStream_t s);
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
-// MFMA kernel with tile x tile tiles (128 or 64, the step's tile size)
+// MFMA kernel of the step (64, 128 or kGemmMfma32, the step's GEMM kernel
+// code; kGemmMfma32 is FP32 only); nL: elements of the L store (the FP32
+// 32x32 kernel's loads stay inside [L, L + nL + 64))
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s);
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int kernel, T* L, int64_t nL, T* U, hipStream_t s);
 // same jobs (128-row tiles), FP32 data, bf16 MFMA with FP32 accumulation;
 // nsplit 1: one bf16 product, 3: split hi/lo products (hi*hi + hi*lo + lo*hi)
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s);

@@ -1050,6 +1050,171 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
                          [](int lg_, int v_) { return MfmaOp<T>::row(lg_, v_); });
 }
 
+// FP32 form of the same product on v_mfma_f32_32x32x2f32, for the same jobs
+// and 128x128 tiles (four wavefronts of 64x64, 2x2 blocks of 32x32).
+//  * Global loads are buffer loads (raw buffer resource over the L store from
+//    the panel's current column to the store's end): 16 bytes = 4
+//    consecutive rows of one column per load, four per thread and k-tile, and
+//    loads past the store's end return zero without a bounds test.  Rows past
+//    M (or N) are loaded as they are: they only reach C rows (columns) that
+//    are not stored.  Columns past K are zeroed in the last k-tile, on both
+//    operands (0 * NaN would not vanish).
+//  * LDS holds a k-tile of 16 as 8 column pairs, [pair][row][2]: a thread
+//    stores its two columns' 4 rows interleaved as two 16-byte stores; the
+//    MFMA's contraction index is relabelled so that lane half h reads column
+//    pair 2 s + h and sub-step q takes column q of the pair.  B-side reads
+//    take rows 2 j and 2 j + 1 in one 16-byte read (its two 32-column blocks
+//    interleaved), A-side reads rows j and 32 + j (8 bytes each), so that a
+//    wavefront's accumulators map to 32 consecutive C rows per column.
+//  * Each wavefront's LDS reads of a k-tile are issued before its products.
+// Same k order per entry as a sequential sum over k-pairs; the rounding
+// differs from gemm_mfma_kernel's (pairs of k per instruction).
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// raw buffer resource (stride 0) over [base, base + bytes_left), clamped to
+// the 32-bit record count; a load outside returns zero
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, int64_t bytes_left) {
+    const int64_t b = bytes_left < 0 ? 0 : (bytes_left > 0xFFFFFFFFll ? 0xFFFFFFFFll : bytes_left);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(uint32_t)b, 0x00020000);
+}
+__device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+}
+
+__global__ __launch_bounds__(kThreads, 2) void gemm_mfma32_kernel(const GemmJob* __restrict__ jobs, int njobs,
+                                                                 float* __restrict__ L, float* __restrict__ U,
+                                                                 int64_t nL) {
+    constexpr int BT = 128, KT = 16, NP = KT / 2;
+    __shared__ __align__(16) float As[2][NP * BT * 2];
+    __shared__ __align__(16) float Bs[2][NP * BT * 2];
+    int lo = 0, hi = njobs - 1;
+    const int bid = xcd_tile(blockIdx.x, gridDim.x);
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
+    }
+    const GemmJob g = jobs[lo];
+    int local = bid - g.tile_base;
+    int ti, tj;
+    if (g.lower) {
+        tj = 0;
+        int cnt = g.tiles_m;
+        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+        ti = tj + local;
+    } else {
+        tj = local / g.tiles_m;
+        ti = local % g.tiles_m;
+    }
+    float* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+    const int row0 = ti * BT, col0 = tj * BT;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
+    const int j = lane & 31, h = lane >> 5;
+    // staging: rows sr..sr+3 of columns 2 sp and 2 sp + 1 of the k-tile
+    const int sr = 4 * (tid & 31), sp = tid >> 5;
+    const int64_t a0 = g.a_off + row0, b0 = g.b_off + col0;
+    const uint32_t va0 = (uint32_t)(sr + 2 * sp * g.lda) * 4u, va1 = va0 + 4u * (uint32_t)g.lda;
+    const uint32_t vb0 = (uint32_t)(sr + 2 * sp * g.ldb) * 4u, vb1 = vb0 + 4u * (uint32_t)g.ldb;
+    f32x4_t fa0, fa1, fb0, fb1;
+    auto fetch = [&](int k0) {
+        const int64_t oa = a0 + (int64_t)k0 * g.lda, ob = b0 + (int64_t)k0 * g.ldb;
+        const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4), rb = buffer_rsrc(L + ob, (nL - ob) * 4);
+        fa0 = buffer_load4(ra, va0);
+        fa1 = buffer_load4(ra, va1);
+        fb0 = buffer_load4(rb, vb0);
+        fb1 = buffer_load4(rb, vb1);
+    };
+    auto store = [&](int buf, bool tail, int k0) {
+        if (tail) {
+            const int kk = k0 + 2 * sp;
+            const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+            if (kk >= g.K) { fa0 = z; fb0 = z; }
+            if (kk + 1 >= g.K) { fa1 = z; fb1 = z; }
+        }
+        float* pa = &As[buf][(sp * BT + sr) * 2];
+        float* pb = &Bs[buf][(sp * BT + sr) * 2];
+        *reinterpret_cast<f32x4_t*>(pa) = f32x4_t{fa0[0], fa1[0], fa0[1], fa1[1]};
+        *reinterpret_cast<f32x4_t*>(pa + 4) = f32x4_t{fa0[2], fa1[2], fa0[3], fa1[3]};
+        *reinterpret_cast<f32x4_t*>(pb) = f32x4_t{fb0[0], fb1[0], fb0[1], fb1[1]};
+        *reinterpret_cast<f32x4_t*>(pb + 4) = f32x4_t{fb0[2], fb1[2], fb0[3], fb1[3]};
+    };
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+    fetch(0);
+    store(0, KT > g.K, 0);
+    if (KT < g.K) fetch(KT);
+    __syncthreads();
+    for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+        // tile k0 + KT (in registers) to the other buffer, whose last readers
+        // passed the barrier that ended the previous iteration
+        if (k0 + KT < g.K) {
+            store(cur ^ 1, k0 + 2 * KT > g.K, k0 + KT);
+            if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+        }
+        const float* Ac = As[cur];
+        const float* Bc = Bs[cur];
+        // av[s][2 rb + q] = A(wr + 32 rb + j, column q of pair 2 s + h),
+        // bv[s][2 cb + q] = B(wc + 2 j + cb, same column)
+        f32x4_t av[NP / 2], bv[NP / 2];
+#pragma unroll
+        for (int s2 = 0; s2 < NP / 2; ++s2) {
+            const int pr = 2 * s2 + h;
+            const f32x2_t x0 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + j) * 2]);
+            const f32x2_t x1 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + 32 + j) * 2]);
+            av[s2] = f32x4_t{x0[0], x0[1], x1[0], x1[1]};
+            bv[s2] = *reinterpret_cast<const f32x4_t*>(&Bc[(pr * BT + wc + 2 * j) * 2]);
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < NP / 2; ++s2)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][2 * cb + q], av[s2][2 * rb + q],
+                                                                           acc[rb][cb], 0, 0, 0);
+        __syncthreads();
+    }
+    // acc[rb][cb][v] = C(row0 + wr + 32 rb + j, col0 + wc + 2 (8 (v / 4) + 4 h + v % 4) + cb);
+    // C -= acc (beta0: C = -acc), all old values of a column block loaded
+    // first at clamped in-range addresses
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        float old[2][16];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                old[rb][v] = 0.f;
+                if (!g.beta0) {
+                    const int i = min(row0 + wr + 32 * rb + j, g.M - 1);
+                    const int c = min(col0 + wc + 2 * (8 * (v >> 2) + 4 * h + (v & 3)) + cb, g.N - 1);
+                    old[rb][v] = C[i + (int64_t)c * g.ldc];
+                }
+            }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) asm volatile("" : "+v"(old[rb][v]));
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int i = row0 + wr + 32 * rb + j;
+                const int c = col0 + wc + 2 * (8 * (v >> 2) + 4 * h + (v & 3)) + cb;
+                if (i < g.M && c < g.N && !(g.lower && i < c)) C[i + (int64_t)c * g.ldc] = old[rb][v] - acc[rb][cb][v];
+            }
+    }
+}
+
 // BF16 variants of gemm_mfma_kernel: the factor stays FP32 in HBM, the A
 // and B tiles are rounded to bf16 (v_cvt_pk_bf16_f32, round to nearest
 // even) when staged into LDS, and v_mfma_f32_16x16x32_bf16 accumulates in
@@ -1948,9 +2113,15 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
     if (count > 0) inv_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int tile, T* L, T* U, hipStream_t s) {
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int kernel, T* L, int64_t nL, T* U, hipStream_t s) {
     if (nblocks <= 0) return;
-    if (tile == 128) gemm_mfma_kernel<T, 128><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    if constexpr (std::is_same<T, float>::value) {
+        if (kernel == kGemmMfma32) {
+            gemm_mfma32_kernel<<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            return;
+        }
+    }
+    if (gemm_tile_rows(kernel) == 128) gemm_mfma_kernel<T, 128><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
     else gemm_mfma_kernel<T, 64><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
 }
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s) {
@@ -2062,7 +2233,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
-    template void launch_gemm<T>(const GemmJob*, int, int, int, T*, T*, hipStream_t);                               \
+    template void launch_gemm<T>(const GemmJob*, int, int, int, T*, int64_t, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, int,            \
                                 hipStream_t);                                                                  \

@@ -286,7 +286,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                         break;
                     }
                 }
-                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, st.arg, L, U, s);
+                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, st.arg, L, P.lstore, U, s);
                 break;
             case mdgpu::STEP_INV:
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
@@ -465,7 +465,9 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
             for (int r = sym->seg_col[k]; r < sym->seg_col[k + 1]; r++) seg_of_row[r] = k;
     }
     if (const char* g = std::getenv("MDGPU_GRAPH")) N->use_graph = std::atoi(g) != 0;
-    if ((st = alloc(N, P.lstore * es, &N->dL)) ||
+    // 64 elements past the L store: the FP32 GEMM's 16-byte buffer loads
+    // are bounded by the store's end at their first element
+    if ((st = alloc(N, (P.lstore + 64) * es, &N->dL)) ||
         (st = alloc(N, P.arena * es, &N->dU)) ||
         (st = alloc(N, P.dinv * es, &N->dInv)) ||
         (st = upload(N, P.inv, (void**)&N->dInvJobs)) ||
@@ -1058,25 +1060,30 @@ extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, vo
     std::vector<int> ords;          // step index of each GEMM launch, in order
     for (size_t t = 0; t < num->plan.steps.size(); t++)
         if (num->plan.steps[t].kind == mdgpu::STEP_GEMM) ords.push_back((int)t);
-    const std::vector<int> initial = num->po.gemm_tiles;
-    std::vector<float> ms64, ms128, ms;
-    TRY(install_gemm_tiles(num, std::vector<int>(ords.size(), 64)));
-    TRY(factor_timed(num, d_ax, s, ms));        // warm
-    TRY(factor_timed(num, d_ax, s, ms64));
-    TRY(install_gemm_tiles(num, std::vector<int>(ords.size(), 128)));
-    TRY(factor_timed(num, d_ax, s, ms));        // warm
-    TRY(factor_timed(num, d_ax, s, ms128));
+    // candidate kernels: the 16x16x4 kernel's 64- and 128-row tiles, and in
+    // FP32 the 32x32x2 kernel
+    std::vector<int> cands = {64, 128};
+    if (num->precision == MDGPU_FP32) cands.push_back(mdgpu::kGemmMfma32);
+    std::vector<std::vector<float>> msk(cands.size());
+    std::vector<float> ms;
+    for (size_t c = 0; c < cands.size(); c++) {
+        TRY(install_gemm_tiles(num, std::vector<int>(ords.size(), cands[c])));
+        TRY(factor_timed(num, d_ax, s, ms));    // warm
+        TRY(factor_timed(num, d_ax, s, msk[c]));
+    }
     std::vector<int> best(ords.size());
     int64_t n64 = 0;
     for (size_t g = 0; g < ords.size(); g++) {
         const int t = ords[g] + 1;              // ms[0] is the assembly
-        best[g] = ms64[t] < ms128[t] ? 64 : 128;
+        size_t b = 0;
+        for (size_t c = 1; c < cands.size(); c++)
+            if (msk[c][t] < msk[b][t]) b = c;
+        best[g] = cands[b];
         n64 += best[g] == 64;
     }
     TRY(install_gemm_tiles(num, best));
     TRY(factor_timed(num, d_ax, s, ms));        // leaves a valid factorization
     if (changed) *changed = n64;
-    (void)initial;
     return MDGPU_OK;
 }
 

@@ -284,14 +284,15 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
 
     // schedule
-    // GEMM tile size of a launch: o.tile unless the autotuner chose
+    // GEMM kernel code of a launch: o.gemm_default unless the autotuner chose
     int gemm_ordinal = 0;   // GEMM launches emitted so far
-    // tile of the GEMM launch being assembled (the autotuner's choice if any)
+    // kernel of the GEMM launch being assembled (the autotuner's choice if any)
     auto tile_for = [&](int dflt) {
         if (gemm_ordinal < (int)o.gemm_tiles.size()) return o.gemm_tiles[gemm_ordinal];
         return dflt;
     };
-    auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles, int T) {
+    auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles, int kernel) {
+        const int T = gemm_tile_rows(kernel);
         g.tiles_m = (g.M + T - 1) / T;
         g.tiles_n = (g.N + T - 1) / T;
         if (g.lower) {
@@ -451,8 +452,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             std::vector<GemmJob> batch_a, batch_b;
             int tiles_a = 0, tiles_b = 0;
             // ordinals: (a) is the next GEMM launch, (b) the one after it
-            const int tile_a = tile_for(o.tile);
-            const int tile_b = gemm_ordinal + 1 < (int)o.gemm_tiles.size() ? o.gemm_tiles[gemm_ordinal + 1] : o.tile;
+            const int tile_a = tile_for(o.gemm_default);
+            const int tile_b =
+                gemm_ordinal + 1 < (int)o.gemm_tiles.size() ? o.gemm_tiles[gemm_ordinal + 1] : o.gemm_default;
             for (int f : blocked) {
                 const FrontDesc& d = P.fronts[f];
                 if (R0 >= d.c) continue;
@@ -501,7 +503,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         {
             std::vector<GemmJob> batch;
             int tiles = 0;
-            const int tile_s = tile_for(o.tile);
+            const int tile_s = tile_for(o.gemm_default);
             for (int f : fl) {
                 const FrontDesc& d = P.fronts[f];
                 if (d.m == 0) continue;
@@ -811,11 +813,13 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
     for (const Step& s : P.steps) {
         if (s.nblocks <= 0 || s.count <= 0) return bad("empty step");
         if (s.kind == STEP_GEMM) {
-            if ((s.arg != 64 && s.arg != 128) || s.arg > o.tile) return bad("gemm tile size");
+            if (!gemm_kernel_valid(s.arg, o.elem_bytes) || gemm_tile_rows(s.arg) > o.tile)
+                return bad("gemm kernel or tile size");
+            const int T = gemm_tile_rows(s.arg);
             int tiles = 0;
             for (int t = s.first; t < s.first + s.count; t++) {
                 const GemmJob& g = P.gemm[t];
-                if (g.tiles_m != (g.M + s.arg - 1) / s.arg || g.tiles_n != (g.N + s.arg - 1) / s.arg)
+                if (g.tiles_m != (g.M + T - 1) / T || g.tiles_n != (g.N + T - 1) / T)
                     return bad("gemm tiles do not match the launch's tile size");
                 if (P.gemm[t].tile_base != tiles) return bad("gemm tile bases");
                 tiles += P.gemm[t].ntiles;

@@ -222,15 +222,25 @@ struct Plan {
     int max_kids = 0;
 };
 
+// GEMM kernel code of a launch (Step.arg of a STEP_GEMM): 64 or 128, the
+// 16x16x4 MFMA kernel with that tile size, or kGemmMfma32, the FP32
+// 32x32x2 MFMA kernel with 128-row tiles
+constexpr int kGemmMfma32 = 129;
+inline int gemm_tile_rows(int kernel) { return kernel == 64 ? 64 : 128; }
+inline bool gemm_kernel_valid(int kernel, int elem_bytes) {
+    return kernel == 64 || kernel == 128 || (kernel == kGemmMfma32 && elem_bytes == 4);
+}
+
 struct PlanOptions {
     int elem_bytes = 8;          // 8: FP64 store, 4: FP32 store
     int panel_lds_bytes = 65536; // LDS budget of the one-workgroup panel kernel
     int nb = 64;                 // block width of the blocked panel path
     int nb_outer = 256;          // outer block width (trailing updates with K = nb_outer)
-    int tile = 64;               // GEMM tile (rows = cols)
-    // per-launch GEMM tile sizes (64 or 128, <= tile) chosen by the
-    // autotuner, indexed by the GEMM launch's ordinal in the schedule;
-    // empty: `tile` for every launch
+    int tile = 64;               // largest GEMM tile (rows = cols)
+    int gemm_default = 64;       // GEMM kernel code of launches the autotuner did not set
+    // per-launch GEMM kernel codes (64, 128 or kGemmMfma32; tiles <= tile)
+    // chosen by the autotuner, indexed by the GEMM launch's ordinal in the
+    // schedule; empty: gemm_default for every launch
     std::vector<int> gemm_tiles;
     int ea_chunk = 16384;        // target entries per extend-add workgroup
 };

@@ -73,8 +73,9 @@ class Factorization:
                     "bytes": out[4 * t + 3]} for t, k in enumerate(L.KIND_NAMES)}
 
     def autotune(self, ax):
-        """Pick 64- or 128-row GEMM tiles per launch by timing both on `ax`;
-        returns the number of launches on 64-row tiles.  Leaves `ax` factored."""
+        """Pick the GEMM kernel per launch (64- or 128-row 16x16x4 MFMA tiles,
+        and in FP32 the 32x32x2 MFMA kernel) by timing each on `ax`; returns
+        the number of launches on 64-row tiles.  Leaves `ax` factored."""
         torch = _torch()
         ax = ax.to(device="cuda", dtype=torch.float64).contiguous()
         self._ax = ax

@@ -880,9 +880,16 @@ __device__ inline int xcd_tile(int bid, int n) {
 // Register budget: at least 3 wavefronts per SIMD in FP32, 2 in FP64
 // (measured: the 128-row FP32 kernel then runs four workgroups per CU, the
 // FP64 one two instead of one).
-template <typename T, int BT>
+// SPLIT (FP32): two-level accumulation -- the products of each run of
+// kSplitTiles k-tiles (64 k) are summed in a partial accumulator that is
+// then added to the total, so a long K is summed as K/64 partial sums of 64
+// terms instead of one running sum of K terms (a smaller rounding error;
+// long-K Schur complements in FP32 get it).
+constexpr int kSplitTiles = 4;
+template <typename T, int BT, bool SPLIT>
 __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                              T* __restrict__ L, T* __restrict__ U) {
+    static_assert(!SPLIT || sizeof(T) == 4, "two-level accumulation is for FP32");
     // k-tile depth 16 (register budget of the prefetch, and LDS for four
     // FP32 workgroups per CU); the next k-tile is loaded into registers while
     // the current one is multiplied out of LDS.
@@ -1014,6 +1021,13 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
             __syncthreads();
         }
     } else {
+        acc_t part[NB][NB];   // SPLIT: this run's products
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) part[a][b][v] = T(0);
         for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
             // tile k0 + KT (in registers) to the other buffer, whose last
             // readers passed the barrier that ended the previous iteration
@@ -1021,6 +1035,7 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
                 store(cur ^ 1, k0 + KT);
                 if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
             }
+            auto& sum = SPLIT ? part : acc;
             const T* __restrict__ Ac = As[cur];
             const T* __restrict__ Bc = Bs[cur];
     #pragma unroll
@@ -1040,7 +1055,17 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
                     for (int rb2 = 0; rb2 < NB; ++rb2)
     #pragma unroll
                         for (int cb = 0; cb < NB; ++cb)
-                            acc[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], acc[rb2][cb]);
+                            sum[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], sum[rb2][cb]);
+            }
+            if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
+#pragma unroll
+                for (int a = 0; a < NB; ++a)
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        acc[a][b] += part[a][b];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) part[a][b][v] = T(0);
+                    }
             }
             __syncthreads();
         }
@@ -1083,7 +1108,11 @@ __device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32
     return __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
 }
 
-__global__ __launch_bounds__(kThreads, 2) void gemm_mfma32_kernel(const GemmJob* __restrict__ jobs, int njobs,
+// SPLIT: two-level accumulation over runs of kSplitTiles k-tiles, as in
+// gemm_mfma_kernel; its second accumulator set gets a 3-wavefront register
+// budget (measured faster than 2 wavefronts without spills).
+template <bool SPLIT>
+__global__ __launch_bounds__(kThreads, SPLIT ? 3 : 2) void gemm_mfma32_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                                  float* __restrict__ L, float* __restrict__ U,
                                                                  int64_t nL) {
     constexpr int BT = 128, KT = 16, NP = KT / 2;
@@ -1140,13 +1169,14 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma32_kernel(const GemmJob*
         *reinterpret_cast<f32x4_t*>(pb) = f32x4_t{fb0[0], fb1[0], fb0[1], fb1[1]};
         *reinterpret_cast<f32x4_t*>(pb + 4) = f32x4_t{fb0[2], fb1[2], fb0[3], fb1[3]};
     };
-    f32x16_t acc[2][2];
+    f32x16_t acc[2][2], part[2][2];   // part: SPLIT, this run's products
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = part[a][b][v] = 0.f;
+    auto& sum = SPLIT ? part : acc;
     fetch(0);
     store(0, KT > g.K, 0);
     if (KT < g.K) fetch(KT);
@@ -1179,8 +1209,18 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma32_kernel(const GemmJob*
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int cb = 0; cb < 2; ++cb)
-                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][2 * cb + q], av[s2][2 * rb + q],
-                                                                           acc[rb][cb], 0, 0, 0);
+                        sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][2 * cb + q], av[s2][2 * rb + q],
+                                                                           sum[rb][cb], 0, 0, 0);
+        if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    acc[rb][cb] += part[rb][cb];
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) part[rb][cb][v] = 0.f;
+                }
+        }
         __syncthreads();
     }
     // acc[rb][cb][v] = C(row0 + wr + 32 rb + j, col0 + wc + 2 (8 (v / 4) + 4 h + v % 4) + cb);
@@ -2115,14 +2155,24 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int kernel, T* L, int64_t nL, T* U, hipStream_t s) {
     if (nblocks <= 0) return;
+    const bool rows128 = gemm_tile_rows(kernel) == 128;
     if constexpr (std::is_same<T, float>::value) {
-        if (kernel == kGemmMfma32) {
-            gemm_mfma32_kernel<<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+        // two-level accumulation on 128-row tiles always runs the 32x32x2
+        // kernel (same tiles; the 16x16x4 kernel's second accumulator set
+        // spills)
+        const bool split = gemm_split(kernel);
+        if (gemm_kernel_base(kernel) == kGemmMfma32 || (split && rows128)) {
+            if (split) gemm_mfma32_kernel<true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            else gemm_mfma32_kernel<false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            return;
+        }
+        if (split) {
+            gemm_mfma_kernel<T, 64, true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
             return;
         }
     }
-    if (gemm_tile_rows(kernel) == 128) gemm_mfma_kernel<T, 128><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
-    else gemm_mfma_kernel<T, 64><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    if (rows128) gemm_mfma_kernel<T, 128, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    else gemm_mfma_kernel<T, 64, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
 }
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s) {
     if (nblocks <= 0) return;

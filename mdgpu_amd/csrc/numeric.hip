@@ -1062,12 +1062,18 @@ extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, vo
         if (num->plan.steps[t].kind == mdgpu::STEP_GEMM) ords.push_back((int)t);
     // candidate kernels: the 16x16x4 kernel's 64- and 128-row tiles, and in
     // FP32 the 32x32x2 kernel
+    // (each launch keeps its kGemmSplit flag: that is a rounding choice; a
+    // split launch on 128-row tiles runs the 32x32x2 kernel either way)
     std::vector<int> cands = {64, 128};
     if (num->precision == MDGPU_FP32) cands.push_back(mdgpu::kGemmMfma32);
+    std::vector<int> split(ords.size());
+    for (size_t g = 0; g < ords.size(); g++) split[g] = num->plan.steps[ords[g]].arg & mdgpu::kGemmSplit;
     std::vector<std::vector<float>> msk(cands.size());
     std::vector<float> ms;
     for (size_t c = 0; c < cands.size(); c++) {
-        TRY(install_gemm_tiles(num, std::vector<int>(ords.size(), cands[c])));
+        std::vector<int> tiles(ords.size());
+        for (size_t g = 0; g < ords.size(); g++) tiles[g] = cands[c] | split[g];
+        TRY(install_gemm_tiles(num, tiles));
         TRY(factor_timed(num, d_ax, s, ms));    // warm
         TRY(factor_timed(num, d_ax, s, msk[c]));
     }
@@ -1078,8 +1084,8 @@ extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, vo
         size_t b = 0;
         for (size_t c = 1; c < cands.size(); c++)
             if (msk[c][t] < msk[b][t]) b = c;
-        best[g] = cands[b];
-        n64 += best[g] == 64;
+        best[g] = cands[b] | split[g];
+        n64 += cands[b] == 64;
     }
     TRY(install_gemm_tiles(num, best));
     TRY(factor_timed(num, d_ax, s, ms));        // leaves a valid factorization

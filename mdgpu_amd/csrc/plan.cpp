@@ -503,7 +503,11 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         {
             std::vector<GemmJob> batch;
             int tiles = 0;
-            const int tile_s = tile_for(o.gemm_default);
+            int kmax = 0;
+            for (int f : fl)
+                if (P.fronts[f].m > 0) kmax = std::max(kmax, P.fronts[f].c);
+            const bool split = o.split_k_min > 0 && kmax >= o.split_k_min;
+            const int tile_s = tile_for(o.gemm_default | (split ? kGemmSplit : 0));
             for (int f : fl) {
                 const FrontDesc& d = P.fronts[f];
                 if (d.m == 0) continue;

@@ -222,13 +222,19 @@ struct Plan {
     int max_kids = 0;
 };
 
-// GEMM kernel code of a launch (Step.arg of a STEP_GEMM): 64 or 128, the
-// 16x16x4 MFMA kernel with that tile size, or kGemmMfma32, the FP32
-// 32x32x2 MFMA kernel with 128-row tiles
+// GEMM kernel code of a launch (Step.arg of a STEP_GEMM): a base kernel --
+// 64 or 128, the 16x16x4 MFMA kernel with that tile size, or kGemmMfma32,
+// the FP32 32x32x2 MFMA kernel with 128-row tiles -- plus the kGemmSplit
+// flag (FP32): two-level accumulation over runs of 64 k
 constexpr int kGemmMfma32 = 129;
-inline int gemm_tile_rows(int kernel) { return kernel == 64 ? 64 : 128; }
+constexpr int kGemmSplit = 256;
+inline int gemm_kernel_base(int kernel) { return kernel & ~kGemmSplit; }
+inline bool gemm_split(int kernel) { return (kernel & kGemmSplit) != 0; }
+inline int gemm_tile_rows(int kernel) { return gemm_kernel_base(kernel) == 64 ? 64 : 128; }
 inline bool gemm_kernel_valid(int kernel, int elem_bytes) {
-    return kernel == 64 || kernel == 128 || (kernel == kGemmMfma32 && elem_bytes == 4);
+    const int b = gemm_kernel_base(kernel);
+    if (gemm_split(kernel) && elem_bytes != 4) return false;
+    return b == 64 || b == 128 || (b == kGemmMfma32 && elem_bytes == 4);
 }
 
 struct PlanOptions {
@@ -238,6 +244,9 @@ struct PlanOptions {
     int nb_outer = 256;          // outer block width (trailing updates with K = nb_outer)
     int tile = 64;               // largest GEMM tile (rows = cols)
     int gemm_default = 64;       // GEMM kernel code of launches the autotuner did not set
+    // Schur-complement launches whose largest K is at least split_k_min get
+    // the kGemmSplit flag by default (0: never)
+    int split_k_min = 0;
     // per-launch GEMM kernel codes (64, 128 or kGemmMfma32; tiles <= tile)
     // chosen by the autotuner, indexed by the GEMM launch's ordinal in the
     // schedule; empty: gemm_default for every launch

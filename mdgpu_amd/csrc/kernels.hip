@@ -1456,8 +1456,9 @@ __device__ __forceinline__ double column_sums64(double (&acc)[32], double* redc,
     return sum;
 }
 
-// Fronts with at most kSolveSmall pivots (one diagonal block), one
-// workgroup per front and one launch per level.  Front vectors of at most
+// Small fronts (solve_small_front: at most kSolveSmall pivots, one
+// diagonal block, and few rows), one workgroup per front and one launch
+// per level.  Front vectors of at most
 // vec_cap (<= kSmallVec) entries live in LDS while the children's update
 // vectors are added; longer ones are accumulated in place in xp and the
 // update vector.
@@ -1599,7 +1600,7 @@ __global__ __launch_bounds__(kSolveThreads) void bwd_kernel(const int* __restric
 }
 
 // ------------------------------------------------- solves of large fronts
-// Fronts with more than kSolveSmall pivots: the children's update vectors
+// Fronts other than the small ones (solve_small_front): the children's update vectors
 // are gathered first (forward) or the update rows applied first (backward),
 // then the kChainBW-column blocks of all of a level's large fronts are
 // solved in one launch, each block by its own workgroup, which waits inside

@@ -125,7 +125,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         for (int f = 0; f < nf; f++) {
             FrontDesc& d = P.fronts[f];
             d.dinv256 = -1;
-            if (d.c <= kSolveSmall) continue;
+            if (solve_small_front(d.c, d.n)) continue;
             d.dinv256 = off256;
             off256 += (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW;
         }
@@ -195,7 +195,8 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         int nctr = 0;
         for (int L = 0; L < P.nlevels; L++) {
             std::vector<int> small, big;
-            for (int f : P.level_fronts[L]) (P.fronts[f].c > kSolveSmall ? big : small).push_back(f);
+            for (int f : P.level_fronts[L])
+                (solve_small_front(P.fronts[f].c, P.fronts[f].n) ? small : big).push_back(f);
             if (!small.empty()) {
                 SolveStep st{SV_FWD_SMALL, (int)P.sv_fronts.size(), (int)small.size(), 0};
                 for (int f : small) st.bytes += (double)P.fronts[f].n * P.fronts[f].c * o.elem_bytes;
@@ -221,7 +222,8 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         }
         for (int L = P.nlevels - 1; L >= 0; L--) {
             std::vector<int> small, big;
-            for (int f : P.level_fronts[L]) (P.fronts[f].c > kSolveSmall ? big : small).push_back(f);
+            for (int f : P.level_fronts[L])
+                (solve_small_front(P.fronts[f].c, P.fronts[f].n) ? small : big).push_back(f);
             if (!small.empty()) {
                 SolveStep st{SV_BWD_SMALL, (int)P.sv_fronts.size(), (int)small.size(), 0};
                 for (int f : small) st.bytes += (double)P.fronts[f].n * P.fronts[f].c * o.elem_bytes;
@@ -701,7 +703,9 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
             if (uses_fronts) {
                 for (int t = st.first; t < st.first + st.count; t++) {
                     if (P.sv_fronts[t] < 0 || P.sv_fronts[t] >= nf) return "solve front";
-                    if (st.kind != SV_FWD_GATHER && P.fronts[P.sv_fronts[t]].c > kSolveSmall) return "solve small front";
+                    if (st.kind != SV_FWD_GATHER &&
+                        !solve_small_front(P.fronts[P.sv_fronts[t]].c, P.fronts[P.sv_fronts[t]].n))
+                        return "solve small front";
                 }
                 continue;
             }
@@ -712,7 +716,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 if (it.n != d.n || it.c != d.c || it.k != d.k || it.qofs != d.qofs || it.lofs != d.lofs ||
                     it.su != P.solve_uofs[it.front])
                     return "solve item front fields";
-                if (d.c <= kSolveSmall || d.dinv256 < 0) return "solve item front size";
+                if (solve_small_front(d.c, d.n) || d.dinv256 < 0) return "solve item front size";
                 if (st.kind == SV_BWD_UPD || st.kind == SV_BWD_UPD_SUM) {
                     if (d.n <= d.c || it.a < 0 || it.a >= d.c || it.a % kSolveCols) return "solve update columns";
                     if (st.kind == SV_BWD_UPD &&
@@ -772,7 +776,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         const int nblk = (d.c + 63) / 64;
         if (d.dinv < 0 || d.dinv + (int64_t)(nblk - 1) * 4096 + (int64_t)(d.c - (nblk - 1) * 64) * 64 > P.dinv)
             return bad("front inverse store");
-        if ((d.c > kSolveSmall) != (d.dinv256 >= 0)) return bad("front block-inverse presence");
+        if (solve_small_front(d.c, d.n) != (d.dinv256 < 0)) return bad("front block-inverse presence");
         if (d.dinv256 >= 0 &&
             d.dinv256 + (int64_t)((d.c + kSolveBW - 1) / kSolveBW) * kSolveBW * kSolveBW > P.inv256)
             return bad("front block-inverse store");

@@ -126,8 +126,8 @@ struct Step {
     int stream = 0, wait_ev = -1, rec_ev = -1;
 };
 
-// Solve schedule.  Fronts with at most kSolveSmall pivots are solved by one
-// workgroup each (FWD_SMALL / BWD_SMALL).  Larger fronts are cut into
+// Solve schedule.  Small fronts (solve_small_front) are solved by one
+// workgroup each (FWD_SMALL / BWD_SMALL).  The other fronts are cut into
 // kChainBW-column blocks and solved by one launch per level whose
 // workgroups form a chain inside the launch:
 //  * FWD_CHAIN: one workgroup per block (TRI item: y_b = X_b (v_b -
@@ -147,7 +147,12 @@ struct Step {
 // per chain launch, zeroed at the start of every solve), so an item only
 // waits for items of earlier tickets, which are running or done: the chain
 // cannot deadlock however many workgroups are resident.
+// fronts with at most kSolveSmall pivots and kSolveSmallEntries panel
+// entries are solved by one workgroup each (SV_*_SMALL); the others in the
+// chained launches, whose rows spread over many workgroups
 constexpr int kSolveSmall = 64;
+constexpr int64_t kSolveSmallEntries = 32768;
+inline bool solve_small_front(int c, int n) { return c <= kSolveSmall && (int64_t)n * c <= kSolveSmallEntries; }
 constexpr int kSolveThreads = 512;
 constexpr int kChainBW = 128;
 constexpr int kSolveBW = 256;       // column width of the block inverses (Plan::inv256)

@@ -220,7 +220,8 @@ int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream
 
 /* Autotuner: factorizes d_ax with every GEMM launch of the schedule on 64-
  * and on 128-row 16x16x4 MFMA tiles and, in MDGPU_FP32, on the 32x32x2
- * MFMA kernel (HIP-event time per launch), keeps the fastest kernel for each
+ * MFMA kernel's 128- and 64-row tiles (HIP-event time per launch), keeps the
+ * fastest kernel for each
  * launch and installs that schedule (captured graphs are recaptured on next
  * use).  *changed = launches on 64-row tiles.  Ends with a valid
  * factorization of d_ax.  The FP32 kernels round differently, so an FP32

@@ -1076,21 +1076,24 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
 }
 
 // FP32 form of the same product on v_mfma_f32_32x32x2f32, for the same jobs
-// and 128x128 tiles (four wavefronts of 64x64, 2x2 blocks of 32x32).
+// and BT x BT tiles (BT = 128: four wavefronts of 64x64, 2x2 blocks of
+// 32x32; BT = 64: four wavefronts of one 32x32 block).
 //  * Global loads are buffer loads (raw buffer resource over the L store from
 //    the panel's current column to the store's end): 16 bytes = 4
-//    consecutive rows of one column per load, four per thread and k-tile, and
-//    loads past the store's end return zero without a bounds test.  Rows past
-//    M (or N) are loaded as they are: they only reach C rows (columns) that
-//    are not stored.  Columns past K are zeroed in the last k-tile, on both
-//    operands (0 * NaN would not vanish).
+//    consecutive rows of one column per load, and loads past the store's end
+//    return zero without a bounds test.  Rows past M (or N) are loaded as
+//    they are: they only reach C rows (columns) that are not stored.
+//    Columns past K are zeroed in the last k-tile, on both operands (0 * NaN
+//    would not vanish).
 //  * LDS holds a k-tile of 16 as 8 column pairs, [pair][row][2]: a thread
-//    stores its two columns' 4 rows interleaved as two 16-byte stores; the
-//    MFMA's contraction index is relabelled so that lane half h reads column
-//    pair 2 s + h and sub-step q takes column q of the pair.  B-side reads
-//    take rows 2 j and 2 j + 1 in one 16-byte read (its two 32-column blocks
-//    interleaved), A-side reads rows j and 32 + j (8 bytes each), so that a
-//    wavefront's accumulators map to 32 consecutive C rows per column.
+//    loads two columns' 4 rows and stores them interleaved as two 16-byte
+//    stores (BT = 128: every thread stages both operands; BT = 64: waves 0-1
+//    stage A, waves 2-3 B).  The MFMA's contraction index is relabelled so
+//    that lane half h reads column pair 2 s + h and sub-step q takes column q
+//    of the pair.  A-side reads take rows 32 rb + j (8 bytes: both columns of
+//    the pair), so that a wavefront's accumulators map to 32 consecutive C
+//    rows per column; B-side reads (BT = 128) take rows 2 j and 2 j + 1 in
+//    one 16-byte read, its two 32-column blocks interleaved.
 //  * Each wavefront's LDS reads of a k-tile are issued before its products.
 // Same k order per entry as a sequential sum over k-pairs; the rounding
 // differs from gemm_mfma_kernel's (pairs of k per instruction).
@@ -1109,13 +1112,15 @@ __device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32
 }
 
 // SPLIT: two-level accumulation over runs of kSplitTiles k-tiles, as in
-// gemm_mfma_kernel; its second accumulator set gets a 3-wavefront register
-// budget (measured faster than 2 wavefronts without spills).
-template <bool SPLIT>
-__global__ __launch_bounds__(kThreads, SPLIT ? 3 : 2) void gemm_mfma32_kernel(const GemmJob* __restrict__ jobs, int njobs,
-                                                                 float* __restrict__ L, float* __restrict__ U,
-                                                                 int64_t nL) {
-    constexpr int BT = 128, KT = 16, NP = KT / 2;
+// gemm_mfma_kernel; at BT = 128 its second accumulator set gets a
+// 3-wavefront register budget (measured faster than 2 wavefronts without
+// spills).
+template <int BT, bool SPLIT>
+__global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_mfma32_kernel(
+    const GemmJob* __restrict__ jobs, int njobs, float* __restrict__ L, float* __restrict__ U, int64_t nL) {
+    static_assert(BT == 128 || BT == 64, "128- or 64-row tiles");
+    constexpr int KT = 16, NP = KT / 2, NB = BT / 64;   // NB x NB 32x32 blocks per wavefront
+    constexpr bool kBoth = BT == 128;                   // every thread stages both operands
     __shared__ __align__(16) float As[2][NP * BT * 2];
     __shared__ __align__(16) float Bs[2][NP * BT * 2];
     int lo = 0, hi = njobs - 1;
@@ -1141,19 +1146,26 @@ __global__ __launch_bounds__(kThreads, SPLIT ? 3 : 2) void gemm_mfma32_kernel(co
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
     const int j = lane & 31, h = lane >> 5;
-    // staging: rows sr..sr+3 of columns 2 sp and 2 sp + 1 of the k-tile
-    const int sr = 4 * (tid & 31), sp = tid >> 5;
+    // staging: rows sr..sr+3 of columns 2 sp and 2 sp + 1 of the k-tile, of
+    // A and B (kBoth) or of operand `opb` (0: A, 1: B)
+    const int st = kBoth ? tid : (tid & 127), opb = kBoth ? 0 : (tid >> 7);
+    const int sr = 4 * (st % (BT / 4)), sp = st / (BT / 4);
     const int64_t a0 = g.a_off + row0, b0 = g.b_off + col0;
     const uint32_t va0 = (uint32_t)(sr + 2 * sp * g.lda) * 4u, va1 = va0 + 4u * (uint32_t)g.lda;
     const uint32_t vb0 = (uint32_t)(sr + 2 * sp * g.ldb) * 4u, vb1 = vb0 + 4u * (uint32_t)g.ldb;
     f32x4_t fa0, fa1, fb0, fb1;
     auto fetch = [&](int k0) {
         const int64_t oa = a0 + (int64_t)k0 * g.lda, ob = b0 + (int64_t)k0 * g.ldb;
-        const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4), rb = buffer_rsrc(L + ob, (nL - ob) * 4);
-        fa0 = buffer_load4(ra, va0);
-        fa1 = buffer_load4(ra, va1);
-        fb0 = buffer_load4(rb, vb0);
-        fb1 = buffer_load4(rb, vb1);
+        if (kBoth || opb == 0) {
+            const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4);
+            fa0 = buffer_load4(ra, va0);
+            fa1 = buffer_load4(ra, va1);
+        }
+        if (kBoth || opb == 1) {
+            const auto rb = buffer_rsrc(L + ob, (nL - ob) * 4);
+            fb0 = buffer_load4(rb, vb0);
+            fb1 = buffer_load4(rb, vb1);
+        }
     };
     auto store = [&](int buf, bool tail, int k0) {
         if (tail) {
@@ -1162,18 +1174,22 @@ __global__ __launch_bounds__(kThreads, SPLIT ? 3 : 2) void gemm_mfma32_kernel(co
             if (kk >= g.K) { fa0 = z; fb0 = z; }
             if (kk + 1 >= g.K) { fa1 = z; fb1 = z; }
         }
-        float* pa = &As[buf][(sp * BT + sr) * 2];
-        float* pb = &Bs[buf][(sp * BT + sr) * 2];
-        *reinterpret_cast<f32x4_t*>(pa) = f32x4_t{fa0[0], fa1[0], fa0[1], fa1[1]};
-        *reinterpret_cast<f32x4_t*>(pa + 4) = f32x4_t{fa0[2], fa1[2], fa0[3], fa1[3]};
-        *reinterpret_cast<f32x4_t*>(pb) = f32x4_t{fb0[0], fb1[0], fb0[1], fb1[1]};
-        *reinterpret_cast<f32x4_t*>(pb + 4) = f32x4_t{fb0[2], fb1[2], fb0[3], fb1[3]};
+        if (kBoth || opb == 0) {
+            float* pa = &As[buf][(sp * BT + sr) * 2];
+            *reinterpret_cast<f32x4_t*>(pa) = f32x4_t{fa0[0], fa1[0], fa0[1], fa1[1]};
+            *reinterpret_cast<f32x4_t*>(pa + 4) = f32x4_t{fa0[2], fa1[2], fa0[3], fa1[3]};
+        }
+        if (kBoth || opb == 1) {
+            float* pb = &Bs[buf][(sp * BT + sr) * 2];
+            *reinterpret_cast<f32x4_t*>(pb) = f32x4_t{fb0[0], fb1[0], fb0[1], fb1[1]};
+            *reinterpret_cast<f32x4_t*>(pb + 4) = f32x4_t{fb0[2], fb1[2], fb0[3], fb1[3]};
+        }
     };
-    f32x16_t acc[2][2], part[2][2];   // part: SPLIT, this run's products
+    f32x16_t acc[NB][NB], part[NB][NB];   // part: SPLIT, this run's products
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NB; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = part[a][b][v] = 0.f;
     auto& sum = SPLIT ? part : acc;
@@ -1191,31 +1207,37 @@ __global__ __launch_bounds__(kThreads, SPLIT ? 3 : 2) void gemm_mfma32_kernel(co
         const float* Ac = As[cur];
         const float* Bc = Bs[cur];
         // av[s][2 rb + q] = A(wr + 32 rb + j, column q of pair 2 s + h),
-        // bv[s][2 cb + q] = B(wc + 2 j + cb, same column)
+        // bv[s][2 cb + q] = B(wc + NB j + cb, same column)
         f32x4_t av[NP / 2], bv[NP / 2];
 #pragma unroll
         for (int s2 = 0; s2 < NP / 2; ++s2) {
             const int pr = 2 * s2 + h;
             const f32x2_t x0 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + j) * 2]);
-            const f32x2_t x1 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + 32 + j) * 2]);
-            av[s2] = f32x4_t{x0[0], x0[1], x1[0], x1[1]};
-            bv[s2] = *reinterpret_cast<const f32x4_t*>(&Bc[(pr * BT + wc + 2 * j) * 2]);
+            if constexpr (NB == 2) {
+                const f32x2_t x1 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + 32 + j) * 2]);
+                av[s2] = f32x4_t{x0[0], x0[1], x1[0], x1[1]};
+                bv[s2] = *reinterpret_cast<const f32x4_t*>(&Bc[(pr * BT + wc + 2 * j) * 2]);
+            } else {
+                const f32x2_t y0 = *reinterpret_cast<const f32x2_t*>(&Bc[(pr * BT + wc + j) * 2]);
+                av[s2] = f32x4_t{x0[0], x0[1], 0.f, 0.f};
+                bv[s2] = f32x4_t{y0[0], y0[1], 0.f, 0.f};
+            }
         }
 #pragma unroll
         for (int s2 = 0; s2 < NP / 2; ++s2)
 #pragma unroll
             for (int q = 0; q < 2; ++q)
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
+                for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
+                    for (int cb = 0; cb < NB; ++cb)
                         sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][2 * cb + q], av[s2][2 * rb + q],
                                                                            sum[rb][cb], 0, 0, 0);
         if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
+            for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
+                for (int cb = 0; cb < NB; ++cb) {
                     acc[rb][cb] += part[rb][cb];
 #pragma unroll
                     for (int v = 0; v < 16; ++v) part[rb][cb][v] = 0.f;
@@ -1223,33 +1245,33 @@ __global__ __launch_bounds__(kThreads, SPLIT ? 3 : 2) void gemm_mfma32_kernel(co
         }
         __syncthreads();
     }
-    // acc[rb][cb][v] = C(row0 + wr + 32 rb + j, col0 + wc + 2 (8 (v / 4) + 4 h + v % 4) + cb);
+    // acc[rb][cb][v] = C(row0 + wr + 32 rb + j, col0 + wc + NB (8 (v / 4) + 4 h + v % 4) + cb);
     // C -= acc (beta0: C = -acc), all old values of a column block loaded
     // first at clamped in-range addresses
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-        float old[2][16];
+    for (int cb = 0; cb < NB; ++cb) {
+        float old[NB][16];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 old[rb][v] = 0.f;
                 if (!g.beta0) {
                     const int i = min(row0 + wr + 32 * rb + j, g.M - 1);
-                    const int c = min(col0 + wc + 2 * (8 * (v >> 2) + 4 * h + (v & 3)) + cb, g.N - 1);
+                    const int c = min(col0 + wc + NB * (8 * (v >> 2) + 4 * h + (v & 3)) + cb, g.N - 1);
                     old[rb][v] = C[i + (int64_t)c * g.ldc];
                 }
             }
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
             for (int v = 0; v < 16; ++v) asm volatile("" : "+v"(old[rb][v]));
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int i = row0 + wr + 32 * rb + j;
-                const int c = col0 + wc + 2 * (8 * (v >> 2) + 4 * h + (v & 3)) + cb;
+                const int c = col0 + wc + NB * (8 * (v >> 2) + 4 * h + (v & 3)) + cb;
                 if (i < g.M && c < g.N && !(g.lower && i < c)) C[i + (int64_t)c * g.ldc] = old[rb][v] - acc[rb][cb][v];
             }
     }
@@ -2162,9 +2184,15 @@ void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int kernel, T* L, 
         // kernel (same tiles; the 16x16x4 kernel's second accumulator set
         // spills)
         const bool split = gemm_split(kernel);
-        if (gemm_kernel_base(kernel) == kGemmMfma32 || (split && rows128)) {
-            if (split) gemm_mfma32_kernel<true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
-            else gemm_mfma32_kernel<false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+        const int base = gemm_kernel_base(kernel);
+        if (base == kGemmMfma32 || (split && rows128)) {
+            if (split) gemm_mfma32_kernel<128, true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            else gemm_mfma32_kernel<128, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            return;
+        }
+        if (base == kGemmMfma32x64) {
+            if (split) gemm_mfma32_kernel<64, true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            else gemm_mfma32_kernel<64, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
             return;
         }
         if (split) {

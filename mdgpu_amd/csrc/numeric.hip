@@ -1065,7 +1065,10 @@ extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, vo
     // (each launch keeps its kGemmSplit flag: that is a rounding choice; a
     // split launch on 128-row tiles runs the 32x32x2 kernel either way)
     std::vector<int> cands = {64, 128};
-    if (num->precision == MDGPU_FP32) cands.push_back(mdgpu::kGemmMfma32);
+    if (num->precision == MDGPU_FP32) {
+        cands.push_back(mdgpu::kGemmMfma32);
+        cands.push_back(mdgpu::kGemmMfma32x64);
+    }
     std::vector<int> split(ords.size());
     for (size_t g = 0; g < ords.size(); g++) split[g] = num->plan.steps[ords[g]].arg & mdgpu::kGemmSplit;
     std::vector<std::vector<float>> msk(cands.size());
@@ -1085,7 +1088,7 @@ extern "C" int mdgpu_numeric_autotune(mdgpu_numeric* num, const double* d_ax, vo
         for (size_t c = 1; c < cands.size(); c++)
             if (msk[c][t] < msk[b][t]) b = c;
         best[g] = cands[b] | split[g];
-        n64 += cands[b] == 64;
+        n64 += mdgpu::gemm_tile_rows(cands[b]) == 64;
     }
     TRY(install_gemm_tiles(num, best));
     TRY(factor_timed(num, d_ax, s, ms));        // leaves a valid factorization

@@ -228,18 +228,22 @@ struct Plan {
 };
 
 // GEMM kernel code of a launch (Step.arg of a STEP_GEMM): a base kernel --
-// 64 or 128, the 16x16x4 MFMA kernel with that tile size, or kGemmMfma32,
-// the FP32 32x32x2 MFMA kernel with 128-row tiles -- plus the kGemmSplit
-// flag (FP32): two-level accumulation over runs of 64 k
+// 64 or 128, the 16x16x4 MFMA kernel with that tile size, or kGemmMfma32 /
+// kGemmMfma32x64, the FP32 32x32x2 MFMA kernel with 128- / 64-row tiles --
+// plus the kGemmSplit flag (FP32): two-level accumulation over runs of 64 k
 constexpr int kGemmMfma32 = 129;
+constexpr int kGemmMfma32x64 = 65;   // the 32x32x2 kernel with 64-row tiles
 constexpr int kGemmSplit = 256;
 inline int gemm_kernel_base(int kernel) { return kernel & ~kGemmSplit; }
 inline bool gemm_split(int kernel) { return (kernel & kGemmSplit) != 0; }
-inline int gemm_tile_rows(int kernel) { return gemm_kernel_base(kernel) == 64 ? 64 : 128; }
+inline int gemm_tile_rows(int kernel) {
+    const int b = gemm_kernel_base(kernel);
+    return b == 64 || b == kGemmMfma32x64 ? 64 : 128;
+}
 inline bool gemm_kernel_valid(int kernel, int elem_bytes) {
     const int b = gemm_kernel_base(kernel);
     if (gemm_split(kernel) && elem_bytes != 4) return false;
-    return b == 64 || b == 128 || (b == kGemmMfma32 && elem_bytes == 4);
+    return b == 64 || b == 128 || ((b == kGemmMfma32 || b == kGemmMfma32x64) && elem_bytes == 4);
 }
 
 struct PlanOptions {

@@ -74,7 +74,8 @@ class Factorization:
 
     def autotune(self, ax):
         """Pick the GEMM kernel per launch (64- or 128-row 16x16x4 MFMA tiles,
-        and in FP32 the 32x32x2 MFMA kernel) by timing each on `ax`; returns
+        and in FP32 the 32x32x2 MFMA kernel's 128- and 64-row tiles) by timing
+        each on `ax`; returns
         the number of launches on 64-row tiles.  Leaves `ax` factored."""
         torch = _torch()
         ax = ax.to(device="cuda", dtype=torch.float64).contiguous()

@@ -461,7 +461,7 @@ def main():
     # HBM traffic per launch of the dominant kernel, measured by two
     # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled on a copy of
     # known size; the committed summary holds the figure
-    kname = {"gemm": "gemm_mfma_kernel<float>"}.get(dom)
+    kname = {"gemm": "fp32_gemm"}.get(dom)
     tpath = os.path.join(ROOT, "profiles", "gemm_blk_extendadd_traffic_c3_fp32.md")
     if kname and args.workload == "c3" and os.path.exists(tpath):
         for line in open(tpath):

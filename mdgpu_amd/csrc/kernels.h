@@ -31,13 +31,24 @@ void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
 // MFMA kernel of the step (64, 128 or kGemmMfma32, the step's GEMM kernel
-// code; kGemmMfma32 is FP32 only); nL: elements of the L store (the FP32
-// 32x32 kernel's loads stay inside [L, L + nL + 64))
+// code; kGemmMfma32 is FP32 only) over its nblocks tiles; max_grid > 0 (a
+// multiple of 8): at most that many workgroups, each looping over tiles;
+// nL: elements of the L store (the FP32 32x32 kernel's loads stay inside
+// [L, L + nL + 64))
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int kernel, T* L, int64_t nL, T* U, hipStream_t s);
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int kernel, T* L, int64_t nL, T* U,
+                 hipStream_t s);
 // same jobs (128-row tiles), FP32 data, bf16 MFMA with FP32 accumulation;
 // nsplit 1: one bf16 product, 3: split hi/lo products (hi*hi + hi*lo + lo*hi)
-void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s);
+void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit, float* L, float* U,
+                      hipStream_t s);
+// kGemmPlanes launches: the same products with A and B from the bf16 split
+// planes hi / lo (GemmJob::a_pl, b_pl); C in U or L as for the others
+void launch_gemm_bf16_planes(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit,
+                             const __bf16* hi, const __bf16* lo, float* U, float* L, hipStream_t s);
+// split planes of the jobs' blocks (SplitJob): hi = bf16(x), lo = bf16(x - hi)
+void launch_split_planes(const SplitJob* jobs, int njobs, int nblocks, const float* L, __bf16* hi, __bf16* lo,
+                         hipStream_t s);
 
 // solves (compute in FP64, factor stored as T); the forward result goes to
 // Y (which the backward solve of large fronts then updates in place), the

@@ -888,7 +888,7 @@ __device__ inline int xcd_tile(int bid, int n) {
 constexpr int kSplitTiles = 4;
 template <typename T, int BT, bool SPLIT>
 __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_kernel(const GemmJob* __restrict__ jobs, int njobs,
-                                                             T* __restrict__ L, T* __restrict__ U) {
+                                                             T* __restrict__ L, T* __restrict__ U, int ntiles) {
     static_assert(!SPLIT || sizeof(T) == 4, "two-level accumulation is for FP32");
     // k-tile depth 16 (register budget of the prefetch, and LDS for four
     // FP32 workgroups per CU); the next k-tile is loaded into registers while
@@ -903,176 +903,180 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
     // two LDS buffers: the next k-tile is stored while this one is multiplied
     __shared__ __align__(16) T As[2][BT * LDK];
     __shared__ __align__(16) T Bs[2][BT * LDK];
-    int lo = 0, hi = njobs - 1;
-    const int bid = xcd_tile(blockIdx.x, gridDim.x);
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
-    }
-    const GemmJob g = jobs[lo];
-    int local = bid - g.tile_base;
-    int ti, tj;
-    if (g.lower) {
-        tj = 0;
-        int cnt = g.tiles_m;
-        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
-        ti = tj + local;
-    } else {
-        tj = local / g.tiles_m;
-        ti = local % g.tiles_m;
-    }
-    const T* __restrict__ A = L + g.a_off;
-    const T* __restrict__ B = L + g.b_off;
-    T* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
-    const int row0 = ti * BT, col0 = tj * BT;
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
-    const int li = lane & 15, lg = lane >> 4;
-    // staging coordinates: element G j + u of this thread is row r = tid % BT,
-    // k-offset kk = G (tid / BT + KSTEP j) + u: G consecutive k of one row go
-    // to LDS in one 16-byte store (rows 4 banks apart: conflict-free)
-    const int r_st = tid & (BT - 1), kgrp = tid / BT;
-    const int ia = row0 + r_st, jb = col0 + r_st;
-    const bool a_ok = ia < g.M, b_ok = jb < g.N;
-    const T* __restrict__ Ap = A + ia;
-    const T* __restrict__ Bp = B + jb;
-    T ra[PER], rb[PER];
-    // loads at clamped in-range addresses; entries outside the job are
-    // zeroed when stored to LDS, so no wait sits between a fetch and the
-    // multiplications that follow it
-    const T* __restrict__ Apc = a_ok ? Ap : A;
-    const T* __restrict__ Bpc = b_ok ? Bp : B;
-    auto kof = [&](int s) { return G * (kgrp + KSTEP * (s / G)) + s % G; };
-    auto fetch = [&](int k0) {
-#pragma unroll
-        for (int s = 0; s < PER; ++s) {
-            const int k = min(k0 + kof(s), g.K - 1);
-            ra[s] = Apc[(int64_t)k * g.lda];
-            rb[s] = Bpc[(int64_t)k * g.ldb];
+    // persistent over the launch's ntiles tiles when the grid is smaller
+    // (side-stream launches leave CUs to the main stream's launches)
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int lo = 0, hi = njobs - 1;
+        const int bid = xcd_tile(tile, ntiles);
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
         }
-    };
-    acc_t acc[NB][NB];
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
-    auto store = [&](int buf, int k0) {
-#pragma unroll
-        for (int j = 0; j < NG; ++j) {
-            vec_t va, vb;
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-                const bool kin = k0 + kof(G * j + u) < g.K;
-                va[u] = (a_ok && kin) ? ra[G * j + u] : T(0);
-                vb[u] = (b_ok && kin) ? rb[G * j + u] : T(0);
-            }
-            const int kk = G * (kgrp + KSTEP * j);
-            *reinterpret_cast<vec_t*>(&As[buf][r_st * LDK + kk]) = va;
-            *reinterpret_cast<vec_t*>(&Bs[buf][r_st * LDK + kk]) = vb;
+        const GemmJob g = jobs[lo];
+        int local = bid - g.tile_base;
+        int ti, tj;
+        if (g.lower) {
+            tj = 0;
+            int cnt = g.tiles_m;
+            while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+            ti = tj + local;
+        } else {
+            tj = local / g.tiles_m;
+            ti = local % g.tiles_m;
         }
-    };
-    fetch(0);
-    store(0, 0);
-    if (KT < g.K) fetch(KT);
-    __syncthreads();
-    if constexpr (sizeof(T) == 8) {
-        // FP64: each iteration is one basic block (the store and fetch of
-        // the next tiles run unconditionally: past the end they store zeros
-        // to the unused buffer and load clamped addresses), and scheduling
-        // groups interleave them with the multiplications -- measured
-        // faster for FP64, slower for FP32
-        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
-            const T* __restrict__ Ac = As[cur];
-            const T* __restrict__ Bc = Bs[cur];
-            T av[KT / 16][NB][4], bv[KT / 16][NB][4];
-#pragma unroll
-            for (int q = 0; q < KT / 16; ++q) {
-#pragma unroll
-                for (int rb2 = 0; rb2 < NB; ++rb2)
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) av[q][rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + 16 * q + 4 * lg + t];
-#pragma unroll
-                for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) bv[q][cb][t] = Bc[(wc + cb * 16 + li) * LDK + 16 * q + 4 * lg + t];
+        const T* __restrict__ A = L + g.a_off;
+        const T* __restrict__ B = L + g.b_off;
+        T* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+        const int row0 = ti * BT, col0 = tj * BT;
+        const int tid = threadIdx.x;
+        const int wave = tid >> 6, lane = tid & 63;
+        const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
+        const int li = lane & 15, lg = lane >> 4;
+        // staging coordinates: element G j + u of this thread is row r = tid % BT,
+        // k-offset kk = G (tid / BT + KSTEP j) + u: G consecutive k of one row go
+        // to LDS in one 16-byte store (rows 4 banks apart: conflict-free)
+        const int r_st = tid & (BT - 1), kgrp = tid / BT;
+        const int ia = row0 + r_st, jb = col0 + r_st;
+        const bool a_ok = ia < g.M, b_ok = jb < g.N;
+        const T* __restrict__ Ap = A + ia;
+        const T* __restrict__ Bp = B + jb;
+        T ra[PER], rb[PER];
+        // loads at clamped in-range addresses; entries outside the job are
+        // zeroed when stored to LDS, so no wait sits between a fetch and the
+        // multiplications that follow it
+        const T* __restrict__ Apc = a_ok ? Ap : A;
+        const T* __restrict__ Bpc = b_ok ? Bp : B;
+        auto kof = [&](int s) { return G * (kgrp + KSTEP * (s / G)) + s % G; };
+        auto fetch = [&](int k0) {
+    #pragma unroll
+            for (int s = 0; s < PER; ++s) {
+                const int k = min(k0 + kof(s), g.K - 1);
+                ra[s] = Apc[(int64_t)k * g.lda];
+                rb[s] = Bpc[(int64_t)k * g.ldb];
             }
-            store(cur ^ 1, k0 + KT);
-            fetch(k0 + 2 * KT);
-#pragma unroll
-            for (int q = 0; q < KT / 16; ++q)
-#pragma unroll
-                for (int t = 0; t < 4; ++t)
-#pragma unroll
-                    for (int rb2 = 0; rb2 < NB; ++rb2)
-#pragma unroll
-                        for (int cb = 0; cb < NB; ++cb)
-                            acc[rb2][cb] = MfmaOp<T>::mma(bv[q][cb][t], av[q][rb2][t], acc[rb2][cb]);
-            // NG rounds of: a share of the products, the two 16-byte LDS
-            // stores of one group, its 2 G global loads
-#pragma unroll
-            for (int i = 0; i < NG; ++i) {
-                __builtin_amdgcn_sched_group_barrier(0x008, (KT / 4) * NB * NB / NG, 0);   // MFMA
-                __builtin_amdgcn_sched_group_barrier(0x200, 2, 0);                          // DS write
-                __builtin_amdgcn_sched_group_barrier(0x020, 2 * G, 0);                      // VMEM read
-            }
-            __syncthreads();
-        }
-    } else {
-        acc_t part[NB][NB];   // SPLIT: this run's products
-#pragma unroll
+        };
+        acc_t acc[NB][NB];
+    #pragma unroll
         for (int a = 0; a < NB; ++a)
-#pragma unroll
+    #pragma unroll
             for (int b = 0; b < NB; ++b)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) part[a][b][v] = T(0);
-        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
-            // tile k0 + KT (in registers) to the other buffer, whose last
-            // readers passed the barrier that ended the previous iteration
-            if (k0 + KT < g.K) {
-                store(cur ^ 1, k0 + KT);
-                if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+    #pragma unroll
+                for (int v = 0; v < 4; ++v) acc[a][b][v] = T(0);
+        auto store = [&](int buf, int k0) {
+    #pragma unroll
+            for (int j = 0; j < NG; ++j) {
+                vec_t va, vb;
+    #pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    const bool kin = k0 + kof(G * j + u) < g.K;
+                    va[u] = (a_ok && kin) ? ra[G * j + u] : T(0);
+                    vb[u] = (b_ok && kin) ? rb[G * j + u] : T(0);
+                }
+                const int kk = G * (kgrp + KSTEP * j);
+                *reinterpret_cast<vec_t*>(&As[buf][r_st * LDK + kk]) = va;
+                *reinterpret_cast<vec_t*>(&Bs[buf][r_st * LDK + kk]) = vb;
             }
-            auto& sum = SPLIT ? part : acc;
-            const T* __restrict__ Ac = As[cur];
-            const T* __restrict__ Bc = Bs[cur];
+        };
+        fetch(0);
+        store(0, 0);
+        if (KT < g.K) fetch(KT);
+        __syncthreads();
+        if constexpr (sizeof(T) == 8) {
+            // FP64: each iteration is one basic block (the store and fetch of
+            // the next tiles run unconditionally: past the end they store zeros
+            // to the unused buffer and load clamped addresses), and scheduling
+            // groups interleave them with the multiplications -- measured
+            // faster for FP64, slower for FP32
+            for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+                const T* __restrict__ Ac = As[cur];
+                const T* __restrict__ Bc = Bs[cur];
+                T av[KT / 16][NB][4], bv[KT / 16][NB][4];
     #pragma unroll
-            for (int kq = 0; kq < KT; kq += 16) {
-                T av[NB][4], bv[NB][4];   // [block][t]
-    #pragma unroll
-                for (int rb2 = 0; rb2 < NB; ++rb2)
-    #pragma unroll
-                    for (int t = 0; t < 4; ++t) av[rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
-    #pragma unroll
-                for (int cb = 0; cb < NB; ++cb)
-    #pragma unroll
-                    for (int t = 0; t < 4; ++t) bv[cb][t] = Bc[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
-    #pragma unroll
-                for (int t = 0; t < 4; ++t)
+                for (int q = 0; q < KT / 16; ++q) {
     #pragma unroll
                     for (int rb2 = 0; rb2 < NB; ++rb2)
     #pragma unroll
-                        for (int cb = 0; cb < NB; ++cb)
-                            sum[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], sum[rb2][cb]);
+                        for (int t = 0; t < 4; ++t) av[q][rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + 16 * q + 4 * lg + t];
+    #pragma unroll
+                    for (int cb = 0; cb < NB; ++cb)
+    #pragma unroll
+                        for (int t = 0; t < 4; ++t) bv[q][cb][t] = Bc[(wc + cb * 16 + li) * LDK + 16 * q + 4 * lg + t];
+                }
+                store(cur ^ 1, k0 + KT);
+                fetch(k0 + 2 * KT);
+    #pragma unroll
+                for (int q = 0; q < KT / 16; ++q)
+    #pragma unroll
+                    for (int t = 0; t < 4; ++t)
+    #pragma unroll
+                        for (int rb2 = 0; rb2 < NB; ++rb2)
+    #pragma unroll
+                            for (int cb = 0; cb < NB; ++cb)
+                                acc[rb2][cb] = MfmaOp<T>::mma(bv[q][cb][t], av[q][rb2][t], acc[rb2][cb]);
+                // NG rounds of: a share of the products, the two 16-byte LDS
+                // stores of one group, its 2 G global loads
+    #pragma unroll
+                for (int i = 0; i < NG; ++i) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, (KT / 4) * NB * NB / NG, 0);   // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x200, 2, 0);                          // DS write
+                    __builtin_amdgcn_sched_group_barrier(0x020, 2 * G, 0);                      // VMEM read
+                }
+                __syncthreads();
             }
-            if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
-#pragma unroll
-                for (int a = 0; a < NB; ++a)
-#pragma unroll
-                    for (int b = 0; b < NB; ++b) {
-                        acc[a][b] += part[a][b];
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) part[a][b][v] = T(0);
-                    }
+        } else {
+            acc_t part[NB][NB];   // SPLIT: this run's products
+    #pragma unroll
+            for (int a = 0; a < NB; ++a)
+    #pragma unroll
+                for (int b = 0; b < NB; ++b)
+    #pragma unroll
+                    for (int v = 0; v < 4; ++v) part[a][b][v] = T(0);
+            for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+                // tile k0 + KT (in registers) to the other buffer, whose last
+                // readers passed the barrier that ended the previous iteration
+                if (k0 + KT < g.K) {
+                    store(cur ^ 1, k0 + KT);
+                    if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+                }
+                auto& sum = SPLIT ? part : acc;
+                const T* __restrict__ Ac = As[cur];
+                const T* __restrict__ Bc = Bs[cur];
+        #pragma unroll
+                for (int kq = 0; kq < KT; kq += 16) {
+                    T av[NB][4], bv[NB][4];   // [block][t]
+        #pragma unroll
+                    for (int rb2 = 0; rb2 < NB; ++rb2)
+        #pragma unroll
+                        for (int t = 0; t < 4; ++t) av[rb2][t] = Ac[(wr + rb2 * 16 + li) * LDK + kq + 4 * lg + t];
+        #pragma unroll
+                    for (int cb = 0; cb < NB; ++cb)
+        #pragma unroll
+                        for (int t = 0; t < 4; ++t) bv[cb][t] = Bc[(wc + cb * 16 + li) * LDK + kq + 4 * lg + t];
+        #pragma unroll
+                    for (int t = 0; t < 4; ++t)
+        #pragma unroll
+                        for (int rb2 = 0; rb2 < NB; ++rb2)
+        #pragma unroll
+                            for (int cb = 0; cb < NB; ++cb)
+                                sum[rb2][cb] = MfmaOp<T>::mma(bv[cb][t], av[rb2][t], sum[rb2][cb]);
+                }
+                if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
+    #pragma unroll
+                    for (int a = 0; a < NB; ++a)
+    #pragma unroll
+                        for (int b = 0; b < NB; ++b) {
+                            acc[a][b] += part[a][b];
+    #pragma unroll
+                            for (int v = 0; v < 4; ++v) part[a][b][v] = T(0);
+                        }
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
+        // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + row(lg, v))
+        gemm_epilogue<T, NB>(acc, C, g, row0 + wr + li, col0 + wc, lg,
+                             [](int lg_, int v_) { return MfmaOp<T>::row(lg_, v_); });
     }
-    // acc[rb][cb][v] = C(row0 + wr + rb*16 + li, col0 + wc + cb*16 + row(lg, v))
-    gemm_epilogue<T, NB>(acc, C, g, row0 + wr + li, col0 + wc, lg,
-                         [](int lg_, int v_) { return MfmaOp<T>::row(lg_, v_); });
 }
 
 // FP32 form of the same product on v_mfma_f32_32x32x2f32, for the same jobs
@@ -1117,163 +1121,167 @@ __device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32
 // spills).
 template <int BT, bool SPLIT>
 __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_mfma32_kernel(
-    const GemmJob* __restrict__ jobs, int njobs, float* __restrict__ L, float* __restrict__ U, int64_t nL) {
+    const GemmJob* __restrict__ jobs, int njobs, float* __restrict__ L, float* __restrict__ U, int64_t nL, int ntiles) {
     static_assert(BT == 128 || BT == 64, "128- or 64-row tiles");
     constexpr int KT = 16, NP = KT / 2, NB = BT / 64;   // NB x NB 32x32 blocks per wavefront
     constexpr bool kBoth = BT == 128;                   // every thread stages both operands
     __shared__ __align__(16) float As[2][NP * BT * 2];
     __shared__ __align__(16) float Bs[2][NP * BT * 2];
-    int lo = 0, hi = njobs - 1;
-    const int bid = xcd_tile(blockIdx.x, gridDim.x);
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
-    }
-    const GemmJob g = jobs[lo];
-    int local = bid - g.tile_base;
-    int ti, tj;
-    if (g.lower) {
-        tj = 0;
-        int cnt = g.tiles_m;
-        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
-        ti = tj + local;
-    } else {
-        tj = local / g.tiles_m;
-        ti = local % g.tiles_m;
-    }
-    float* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
-    const int row0 = ti * BT, col0 = tj * BT;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
-    const int j = lane & 31, h = lane >> 5;
-    // staging: rows sr..sr+3 of columns 2 sp and 2 sp + 1 of the k-tile, of
-    // A and B (kBoth) or of operand `opb` (0: A, 1: B)
-    const int st = kBoth ? tid : (tid & 127), opb = kBoth ? 0 : (tid >> 7);
-    const int sr = 4 * (st % (BT / 4)), sp = st / (BT / 4);
-    const int64_t a0 = g.a_off + row0, b0 = g.b_off + col0;
-    const uint32_t va0 = (uint32_t)(sr + 2 * sp * g.lda) * 4u, va1 = va0 + 4u * (uint32_t)g.lda;
-    const uint32_t vb0 = (uint32_t)(sr + 2 * sp * g.ldb) * 4u, vb1 = vb0 + 4u * (uint32_t)g.ldb;
-    f32x4_t fa0, fa1, fb0, fb1;
-    auto fetch = [&](int k0) {
-        const int64_t oa = a0 + (int64_t)k0 * g.lda, ob = b0 + (int64_t)k0 * g.ldb;
-        if (kBoth || opb == 0) {
-            const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4);
-            fa0 = buffer_load4(ra, va0);
-            fa1 = buffer_load4(ra, va1);
+    // persistent over the launch's ntiles tiles when the grid is smaller
+    // (side-stream launches leave CUs to the main stream's launches)
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int lo = 0, hi = njobs - 1;
+        const int bid = xcd_tile(tile, ntiles);
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
         }
-        if (kBoth || opb == 1) {
-            const auto rb = buffer_rsrc(L + ob, (nL - ob) * 4);
-            fb0 = buffer_load4(rb, vb0);
-            fb1 = buffer_load4(rb, vb1);
+        const GemmJob g = jobs[lo];
+        int local = bid - g.tile_base;
+        int ti, tj;
+        if (g.lower) {
+            tj = 0;
+            int cnt = g.tiles_m;
+            while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+            ti = tj + local;
+        } else {
+            tj = local / g.tiles_m;
+            ti = local % g.tiles_m;
         }
-    };
-    auto store = [&](int buf, bool tail, int k0) {
-        if (tail) {
-            const int kk = k0 + 2 * sp;
-            const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
-            if (kk >= g.K) { fa0 = z; fb0 = z; }
-            if (kk + 1 >= g.K) { fa1 = z; fb1 = z; }
-        }
-        if (kBoth || opb == 0) {
-            float* pa = &As[buf][(sp * BT + sr) * 2];
-            *reinterpret_cast<f32x4_t*>(pa) = f32x4_t{fa0[0], fa1[0], fa0[1], fa1[1]};
-            *reinterpret_cast<f32x4_t*>(pa + 4) = f32x4_t{fa0[2], fa1[2], fa0[3], fa1[3]};
-        }
-        if (kBoth || opb == 1) {
-            float* pb = &Bs[buf][(sp * BT + sr) * 2];
-            *reinterpret_cast<f32x4_t*>(pb) = f32x4_t{fb0[0], fb1[0], fb0[1], fb1[1]};
-            *reinterpret_cast<f32x4_t*>(pb + 4) = f32x4_t{fb0[2], fb1[2], fb0[3], fb1[3]};
-        }
-    };
-    f32x16_t acc[NB][NB], part[NB][NB];   // part: SPLIT, this run's products
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[a][b][v] = part[a][b][v] = 0.f;
-    auto& sum = SPLIT ? part : acc;
-    fetch(0);
-    store(0, KT > g.K, 0);
-    if (KT < g.K) fetch(KT);
-    __syncthreads();
-    for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
-        // tile k0 + KT (in registers) to the other buffer, whose last readers
-        // passed the barrier that ended the previous iteration
-        if (k0 + KT < g.K) {
-            store(cur ^ 1, k0 + 2 * KT > g.K, k0 + KT);
-            if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
-        }
-        const float* Ac = As[cur];
-        const float* Bc = Bs[cur];
-        // av[s][2 rb + q] = A(wr + 32 rb + j, column q of pair 2 s + h),
-        // bv[s][2 cb + q] = B(wc + NB j + cb, same column)
-        f32x4_t av[NP / 2], bv[NP / 2];
-#pragma unroll
-        for (int s2 = 0; s2 < NP / 2; ++s2) {
-            const int pr = 2 * s2 + h;
-            const f32x2_t x0 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + j) * 2]);
-            if constexpr (NB == 2) {
-                const f32x2_t x1 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + 32 + j) * 2]);
-                av[s2] = f32x4_t{x0[0], x0[1], x1[0], x1[1]};
-                bv[s2] = *reinterpret_cast<const f32x4_t*>(&Bc[(pr * BT + wc + 2 * j) * 2]);
-            } else {
-                const f32x2_t y0 = *reinterpret_cast<const f32x2_t*>(&Bc[(pr * BT + wc + j) * 2]);
-                av[s2] = f32x4_t{x0[0], x0[1], 0.f, 0.f};
-                bv[s2] = f32x4_t{y0[0], y0[1], 0.f, 0.f};
+        float* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+        const int row0 = ti * BT, col0 = tj * BT;
+        const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+        const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
+        const int j = lane & 31, h = lane >> 5;
+        // staging: rows sr..sr+3 of columns 2 sp and 2 sp + 1 of the k-tile, of
+        // A and B (kBoth) or of operand `opb` (0: A, 1: B)
+        const int st = kBoth ? tid : (tid & 127), opb = kBoth ? 0 : (tid >> 7);
+        const int sr = 4 * (st % (BT / 4)), sp = st / (BT / 4);
+        const int64_t a0 = g.a_off + row0, b0 = g.b_off + col0;
+        const uint32_t va0 = (uint32_t)(sr + 2 * sp * g.lda) * 4u, va1 = va0 + 4u * (uint32_t)g.lda;
+        const uint32_t vb0 = (uint32_t)(sr + 2 * sp * g.ldb) * 4u, vb1 = vb0 + 4u * (uint32_t)g.ldb;
+        f32x4_t fa0, fa1, fb0, fb1;
+        auto fetch = [&](int k0) {
+            const int64_t oa = a0 + (int64_t)k0 * g.lda, ob = b0 + (int64_t)k0 * g.ldb;
+            if (kBoth || opb == 0) {
+                const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4);
+                fa0 = buffer_load4(ra, va0);
+                fa1 = buffer_load4(ra, va1);
             }
-        }
-#pragma unroll
-        for (int s2 = 0; s2 < NP / 2; ++s2)
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-                    for (int cb = 0; cb < NB; ++cb)
-                        sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][2 * cb + q], av[s2][2 * rb + q],
-                                                                           sum[rb][cb], 0, 0, 0);
-        if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
-#pragma unroll
-            for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-                for (int cb = 0; cb < NB; ++cb) {
-                    acc[rb][cb] += part[rb][cb];
-#pragma unroll
-                    for (int v = 0; v < 16; ++v) part[rb][cb][v] = 0.f;
-                }
-        }
+            if (kBoth || opb == 1) {
+                const auto rb = buffer_rsrc(L + ob, (nL - ob) * 4);
+                fb0 = buffer_load4(rb, vb0);
+                fb1 = buffer_load4(rb, vb1);
+            }
+        };
+        auto store = [&](int buf, bool tail, int k0) {
+            if (tail) {
+                const int kk = k0 + 2 * sp;
+                const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+                if (kk >= g.K) { fa0 = z; fb0 = z; }
+                if (kk + 1 >= g.K) { fa1 = z; fb1 = z; }
+            }
+            if (kBoth || opb == 0) {
+                float* pa = &As[buf][(sp * BT + sr) * 2];
+                *reinterpret_cast<f32x4_t*>(pa) = f32x4_t{fa0[0], fa1[0], fa0[1], fa1[1]};
+                *reinterpret_cast<f32x4_t*>(pa + 4) = f32x4_t{fa0[2], fa1[2], fa0[3], fa1[3]};
+            }
+            if (kBoth || opb == 1) {
+                float* pb = &Bs[buf][(sp * BT + sr) * 2];
+                *reinterpret_cast<f32x4_t*>(pb) = f32x4_t{fb0[0], fb1[0], fb0[1], fb1[1]};
+                *reinterpret_cast<f32x4_t*>(pb + 4) = f32x4_t{fb0[2], fb1[2], fb0[3], fb1[3]};
+            }
+        };
+        f32x16_t acc[NB][NB], part[NB][NB];   // part: SPLIT, this run's products
+    #pragma unroll
+        for (int a = 0; a < NB; ++a)
+    #pragma unroll
+            for (int b = 0; b < NB; ++b)
+    #pragma unroll
+                for (int v = 0; v < 16; ++v) acc[a][b][v] = part[a][b][v] = 0.f;
+        auto& sum = SPLIT ? part : acc;
+        fetch(0);
+        store(0, KT > g.K, 0);
+        if (KT < g.K) fetch(KT);
         __syncthreads();
-    }
-    // acc[rb][cb][v] = C(row0 + wr + 32 rb + j, col0 + wc + NB (8 (v / 4) + 4 h + v % 4) + cb);
-    // C -= acc (beta0: C = -acc), all old values of a column block loaded
-    // first at clamped in-range addresses
-#pragma unroll
-    for (int cb = 0; cb < NB; ++cb) {
-        float old[NB][16];
-#pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                old[rb][v] = 0.f;
-                if (!g.beta0) {
-                    const int i = min(row0 + wr + 32 * rb + j, g.M - 1);
-                    const int c = min(col0 + wc + NB * (8 * (v >> 2) + 4 * h + (v & 3)) + cb, g.N - 1);
-                    old[rb][v] = C[i + (int64_t)c * g.ldc];
+        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+            // tile k0 + KT (in registers) to the other buffer, whose last readers
+            // passed the barrier that ended the previous iteration
+            if (k0 + KT < g.K) {
+                store(cur ^ 1, k0 + 2 * KT > g.K, k0 + KT);
+                if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+            }
+            const float* Ac = As[cur];
+            const float* Bc = Bs[cur];
+            // av[s][2 rb + q] = A(wr + 32 rb + j, column q of pair 2 s + h),
+            // bv[s][2 cb + q] = B(wc + NB j + cb, same column)
+            f32x4_t av[NP / 2], bv[NP / 2];
+    #pragma unroll
+            for (int s2 = 0; s2 < NP / 2; ++s2) {
+                const int pr = 2 * s2 + h;
+                const f32x2_t x0 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + j) * 2]);
+                if constexpr (NB == 2) {
+                    const f32x2_t x1 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + 32 + j) * 2]);
+                    av[s2] = f32x4_t{x0[0], x0[1], x1[0], x1[1]};
+                    bv[s2] = *reinterpret_cast<const f32x4_t*>(&Bc[(pr * BT + wc + 2 * j) * 2]);
+                } else {
+                    const f32x2_t y0 = *reinterpret_cast<const f32x2_t*>(&Bc[(pr * BT + wc + j) * 2]);
+                    av[s2] = f32x4_t{x0[0], x0[1], 0.f, 0.f};
+                    bv[s2] = f32x4_t{y0[0], y0[1], 0.f, 0.f};
                 }
             }
-#pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) asm volatile("" : "+v"(old[rb][v]));
-#pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int i = row0 + wr + 32 * rb + j;
-                const int c = col0 + wc + NB * (8 * (v >> 2) + 4 * h + (v & 3)) + cb;
-                if (i < g.M && c < g.N && !(g.lower && i < c)) C[i + (int64_t)c * g.ldc] = old[rb][v] - acc[rb][cb][v];
+    #pragma unroll
+            for (int s2 = 0; s2 < NP / 2; ++s2)
+    #pragma unroll
+                for (int q = 0; q < 2; ++q)
+    #pragma unroll
+                    for (int rb = 0; rb < NB; ++rb)
+    #pragma unroll
+                        for (int cb = 0; cb < NB; ++cb)
+                            sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][2 * cb + q], av[s2][2 * rb + q],
+                                                                               sum[rb][cb], 0, 0, 0);
+            if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
+    #pragma unroll
+                for (int rb = 0; rb < NB; ++rb)
+    #pragma unroll
+                    for (int cb = 0; cb < NB; ++cb) {
+                        acc[rb][cb] += part[rb][cb];
+    #pragma unroll
+                        for (int v = 0; v < 16; ++v) part[rb][cb][v] = 0.f;
+                    }
             }
+            __syncthreads();
+        }
+        // acc[rb][cb][v] = C(row0 + wr + 32 rb + j, col0 + wc + NB (8 (v / 4) + 4 h + v % 4) + cb);
+        // C -= acc (beta0: C = -acc), all old values of a column block loaded
+        // first at clamped in-range addresses
+    #pragma unroll
+        for (int cb = 0; cb < NB; ++cb) {
+            float old[NB][16];
+    #pragma unroll
+            for (int rb = 0; rb < NB; ++rb)
+    #pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    old[rb][v] = 0.f;
+                    if (!g.beta0) {
+                        const int i = min(row0 + wr + 32 * rb + j, g.M - 1);
+                        const int c = min(col0 + wc + NB * (8 * (v >> 2) + 4 * h + (v & 3)) + cb, g.N - 1);
+                        old[rb][v] = C[i + (int64_t)c * g.ldc];
+                    }
+                }
+    #pragma unroll
+            for (int rb = 0; rb < NB; ++rb)
+    #pragma unroll
+                for (int v = 0; v < 16; ++v) asm volatile("" : "+v"(old[rb][v]));
+    #pragma unroll
+            for (int rb = 0; rb < NB; ++rb)
+    #pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int i = row0 + wr + 32 * rb + j;
+                    const int c = col0 + wc + NB * (8 * (v >> 2) + 4 * h + (v & 3)) + cb;
+                    if (i < g.M && c < g.N && !(g.lower && i < c)) C[i + (int64_t)c * g.ldc] = old[rb][v] - acc[rb][cb][v];
+                }
+        }
     }
 }
 
@@ -1291,107 +1299,263 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int NSPLIT>
 __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __restrict__ jobs, int njobs,
-                                                             float* __restrict__ L, float* __restrict__ U) {
+                                                             float* __restrict__ L, float* __restrict__ U, int ntiles) {
     constexpr int BT = 128, KT = 32, LDK = KT + 8;                     // rows 16-byte aligned
     constexpr int KS = KT / 2;                                        // staged k per thread
     constexpr int NP = NSPLIT == 1 ? 1 : 2;                           // hi (and lo) planes
     // two LDS buffers: the next k-tile is stored while this one is multiplied
     __shared__ __align__(16) __bf16 As[2][NP][BT * LDK];
     __shared__ __align__(16) __bf16 Bs[2][NP][BT * LDK];
-    int lo = 0, hi = njobs - 1;
-    const int bid = xcd_tile(blockIdx.x, gridDim.x);
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
-    }
-    const GemmJob g = jobs[lo];
-    int local = bid - g.tile_base;
-    int ti, tj;
-    if (g.lower) {
-        tj = 0;
-        int cnt = g.tiles_m;
-        while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
-        ti = tj + local;
-    } else {
-        tj = local / g.tiles_m;
-        ti = local % g.tiles_m;
-    }
-    const float* __restrict__ A = L + g.a_off;
-    const float* __restrict__ B = L + g.b_off;
-    float* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
-    const int row0 = ti * BT, col0 = tj * BT;
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-    const int li = lane & 15, lg = lane >> 4;
-    const int r_st = tid & (BT - 1), kg = (tid >> 7) * KS;   // staged row, first staged k
-    const int ia = row0 + r_st, jb = col0 + r_st;
-    const bool a_ok = ia < g.M, b_ok = jb < g.N;
-    const float* __restrict__ Ap = A + ia;
-    const float* __restrict__ Bp = B + jb;
-    float ra[KS], rb[KS];
-    // loads at clamped in-range addresses; entries outside the job are
-    // zeroed when stored to LDS, so no wait sits between a fetch and the
-    // multiplications that follow it
-    const float* __restrict__ Apc = a_ok ? Ap : A;
-    const float* __restrict__ Bpc = b_ok ? Bp : B;
-    auto fetch = [&](int k0) {
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int k = min(k0 + kg + s, g.K - 1);
-            ra[s] = Apc[(int64_t)k * g.lda];
-            rb[s] = Bpc[(int64_t)k * g.ldb];
+    // persistent over the launch's ntiles tiles when the grid is smaller
+    // (side-stream launches leave CUs to the main stream's launches)
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int lo = 0, hi = njobs - 1;
+        const int bid = xcd_tile(tile, ntiles);
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
         }
-    };
-    auto store = [&](int buf, int k0) {
-#pragma unroll
-        for (int q = 0; q < KS / 8; ++q) {
-            bf16x8 ah, bh, al, bl;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const bool kin = k0 + kg + 8 * q + e < g.K;
-                const float av = (a_ok && kin) ? ra[8 * q + e] : 0.0f;
-                const float bv = (b_ok && kin) ? rb[8 * q + e] : 0.0f;
-                ah[e] = (__bf16)av;
-                bh[e] = (__bf16)bv;
+        const GemmJob g = jobs[lo];
+        int local = bid - g.tile_base;
+        int ti, tj;
+        if (g.lower) {
+            tj = 0;
+            int cnt = g.tiles_m;
+            while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+            ti = tj + local;
+        } else {
+            tj = local / g.tiles_m;
+            ti = local % g.tiles_m;
+        }
+        const float* __restrict__ A = L + g.a_off;
+        const float* __restrict__ B = L + g.b_off;
+        float* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+        const int row0 = ti * BT, col0 = tj * BT;
+        const int tid = threadIdx.x;
+        const int wave = tid >> 6, lane = tid & 63;
+        const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+        const int li = lane & 15, lg = lane >> 4;
+        const int r_st = tid & (BT - 1), kg = (tid >> 7) * KS;   // staged row, first staged k
+        const int ia = row0 + r_st, jb = col0 + r_st;
+        const bool a_ok = ia < g.M, b_ok = jb < g.N;
+        const float* __restrict__ Ap = A + ia;
+        const float* __restrict__ Bp = B + jb;
+        float ra[KS], rb[KS];
+        // loads at clamped in-range addresses; entries outside the job are
+        // zeroed when stored to LDS, so no wait sits between a fetch and the
+        // multiplications that follow it
+        const float* __restrict__ Apc = a_ok ? Ap : A;
+        const float* __restrict__ Bpc = b_ok ? Bp : B;
+        auto fetch = [&](int k0) {
+    #pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int k = min(k0 + kg + s, g.K - 1);
+                ra[s] = Apc[(int64_t)k * g.lda];
+                rb[s] = Bpc[(int64_t)k * g.ldb];
+            }
+        };
+        auto store = [&](int buf, int k0) {
+    #pragma unroll
+            for (int q = 0; q < KS / 8; ++q) {
+                bf16x8 ah, bh, al, bl;
+    #pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const bool kin = k0 + kg + 8 * q + e < g.K;
+                    const float av = (a_ok && kin) ? ra[8 * q + e] : 0.0f;
+                    const float bv = (b_ok && kin) ? rb[8 * q + e] : 0.0f;
+                    ah[e] = (__bf16)av;
+                    bh[e] = (__bf16)bv;
+                    if constexpr (NSPLIT == 3) {
+                        al[e] = (__bf16)(av - (float)ah[e]);
+                        bl[e] = (__bf16)(bv - (float)bh[e]);
+                    }
+                }
+                *reinterpret_cast<bf16x8*>(&As[buf][0][r_st * LDK + kg + 8 * q]) = ah;
+                *reinterpret_cast<bf16x8*>(&Bs[buf][0][r_st * LDK + kg + 8 * q]) = bh;
                 if constexpr (NSPLIT == 3) {
-                    al[e] = (__bf16)(av - (float)ah[e]);
-                    bl[e] = (__bf16)(bv - (float)bh[e]);
+                    *reinterpret_cast<bf16x8*>(&As[buf][NP - 1][r_st * LDK + kg + 8 * q]) = al;
+                    *reinterpret_cast<bf16x8*>(&Bs[buf][NP - 1][r_st * LDK + kg + 8 * q]) = bl;
                 }
             }
-            *reinterpret_cast<bf16x8*>(&As[buf][0][r_st * LDK + kg + 8 * q]) = ah;
-            *reinterpret_cast<bf16x8*>(&Bs[buf][0][r_st * LDK + kg + 8 * q]) = bh;
-            if constexpr (NSPLIT == 3) {
-                *reinterpret_cast<bf16x8*>(&As[buf][NP - 1][r_st * LDK + kg + 8 * q]) = al;
-                *reinterpret_cast<bf16x8*>(&Bs[buf][NP - 1][r_st * LDK + kg + 8 * q]) = bl;
+        };
+        f32x4 acc[4][4];
+    #pragma unroll
+        for (int a = 0; a < 4; ++a)
+    #pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        fetch(0);
+        store(0, 0);
+        if (KT < g.K) fetch(KT);
+        __syncthreads();
+        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+            // tile k0 + KT (in registers) to the other buffer, whose last
+            // readers passed the barrier that ended the previous iteration
+            if (k0 + KT < g.K) {
+                store(cur ^ 1, k0 + KT);
+                if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
             }
+    #pragma unroll
+            for (int kq = 0; kq < KT; kq += 32) {
+                bf16x8 av[NP][4], bv[NP][4];
+    #pragma unroll
+                for (int p = 0; p < NP; ++p)
+    #pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        av[p][r] = *reinterpret_cast<const bf16x8*>(&As[cur][p][(wr + r * 16 + li) * LDK + kq + 8 * lg]);
+                        bv[p][r] = *reinterpret_cast<const bf16x8*>(&Bs[cur][p][(wc + r * 16 + li) * LDK + kq + 8 * lg]);
+                    }
+    #pragma unroll
+                for (int r = 0; r < 4; ++r)
+    #pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        if constexpr (NSPLIT == 3) {
+                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[1][c], av[0][r], acc[r][c], 0, 0, 0);
+                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[1][r], acc[r][c], 0, 0, 0);
+                        }
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[0][r], acc[r][c], 0, 0, 0);
+                    }
+            }
+            __syncthreads();
         }
-    };
-    f32x4 acc[4][4];
+        // acc[r][c][v] = C(row0 + wr + r*16 + li, col0 + wc + c*16 + 4*lg + v)
+        gemm_epilogue<float, 4>(acc, C, g, row0 + wr + li, col0 + wc, lg, [](int lg_, int v_) { return 4 * lg_ + v_; });
+    }
+}
+
+// bf16 split planes (SplitJob, plan.h): one workgroup converts kSplitRows
+// rows of one octet (8 columns) of a job.  Each thread reads 8 entries of
+// its row (column-major: a wavefront's loads of one column are contiguous)
+// and writes one 16-byte hi and one 16-byte lo vector (contiguous over the
+// rows).  hi = bf16(x), lo = bf16(x - hi), round to nearest even, as in
+// gemm_bf16_kernel.
+__global__ __launch_bounds__(kSplitRows) void split_kernel(const SplitJob* __restrict__ jobs, int njobs,
+                                                           const float* __restrict__ L, __bf16* __restrict__ hi,
+                                                           __bf16* __restrict__ lo) {
+    int a = 0, b = njobs - 1;
+    const int bid = blockIdx.x;
+    while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (jobs[mid].tile_base <= bid) a = mid; else b = mid - 1;
+    }
+    const SplitJob j = jobs[a];
+    const int local = bid - j.tile_base;
+    const int o = local % j.octets;
+    const int i = (local / j.octets) * kSplitRows + (int)threadIdx.x;
+    if (i >= j.rows) return;
+    const float* __restrict__ src = L + j.src + (int64_t)(8 * o) * j.ld + i;
+    float v[8];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int e = 0; e < 8; ++e) v[e] = src[(int64_t)e * j.ld];
+    bf16x8 h, l;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    fetch(0);
-    store(0, 0);
-    if (KT < g.K) fetch(KT);
-    __syncthreads();
-    for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
-        // tile k0 + KT (in registers) to the other buffer, whose last
-        // readers passed the barrier that ended the previous iteration
-        if (k0 + KT < g.K) {
-            store(cur ^ 1, k0 + KT);
-            if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+    for (int e = 0; e < 8; ++e) {
+        h[e] = (__bf16)v[e];
+        l[e] = (__bf16)(v[e] - (float)h[e]);
+    }
+    const int64_t d = j.dst + (int64_t)o * 8 * j.ld + 8 * (int64_t)i;
+    *reinterpret_cast<bf16x8*>(hi + d) = h;
+    *reinterpret_cast<bf16x8*>(lo + d) = l;
+}
+
+// The bf16 products of gemm_bf16_kernel with A and B read from the split
+// planes (kGemmPlanes launches): no conversion in the kernel, and every
+// load is a 16-byte vector of 8 consecutive k of one row.  128 x 128 tiles,
+// k-tiles of 32 = 4 octets.  Thread t stages row t % 128, octets
+// 2 (t / 128) and 2 (t / 128) + 1, of each plane of A and B: 8 (NSPLIT = 3)
+// or 4 loads per k-tile, held in registers while the current k-tile is
+// multiplied, then stored to the other LDS buffer, laid out [octet][row]
+// (the 16 lanes of an MFMA operand read 256 contiguous bytes).  Rows past M
+// (N) are read clamped to row M - 1 (N - 1): they reach only C entries that
+// are not stored.  K is a multiple of 32.  Same products, order and
+// accumulator layout as gemm_bf16_kernel.
+template <int NSPLIT>
+__global__ __launch_bounds__(kThreads, 2) void gemm_bf16p_kernel(const GemmJob* __restrict__ jobs, int njobs,
+                                                                 const __bf16* __restrict__ Ph,
+                                                                 const __bf16* __restrict__ Pl, float* __restrict__ U,
+                                                                 float* __restrict__ L, int ntiles) {
+    constexpr int BT = 128, KT = 32, NO = KT / 8;
+    constexpr int NP = NSPLIT == 1 ? 1 : 2;   // hi (and lo) planes
+    __shared__ bf16x8 As[2][NP][NO * BT];
+    __shared__ bf16x8 Bs[2][NP][NO * BT];
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int lo = 0, hi = njobs - 1;
+        const int bid = xcd_tile(tile, ntiles);
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_base <= bid) lo = mid; else hi = mid - 1;
         }
+        const GemmJob g = jobs[lo];
+        int local = bid - g.tile_base;
+        int ti, tj;
+        if (g.lower) {
+            tj = 0;
+            int cnt = g.tiles_m;
+            while (local >= cnt) { local -= cnt; ++tj; cnt = g.tiles_m - tj; }
+            ti = tj + local;
+        } else {
+            tj = local / g.tiles_m;
+            ti = local % g.tiles_m;
+        }
+        float* __restrict__ C = (g.c_in_arena ? U : L) + g.c_off;
+        const int row0 = ti * BT, col0 = tj * BT;
+        const int tid = threadIdx.x;
+        const int wave = tid >> 6, lane = tid & 63;
+        const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+        const int li = lane & 15, lg = lane >> 4;
+        const int r_st = tid & (BT - 1), o_st = 2 * (tid >> 7);
+        const int64_t ia = min(row0 + r_st, g.M - 1), jb = min(col0 + r_st, g.N - 1);
+        // 16-byte vectors: octet o of row i at base + o * ld + i
+        const bf16x8* __restrict__ pa[NP];
+        const bf16x8* __restrict__ pb[NP];
+        pa[0] = reinterpret_cast<const bf16x8*>(Ph + g.a_pl) + ia;
+        pb[0] = reinterpret_cast<const bf16x8*>(Ph + g.b_pl) + jb;
+        if constexpr (NP == 2) {
+            pa[1] = reinterpret_cast<const bf16x8*>(Pl + g.a_pl) + ia;
+            pb[1] = reinterpret_cast<const bf16x8*>(Pl + g.b_pl) + jb;
+        }
+        bf16x8 ra[NP][2], rb[NP][2];
+        auto fetch = [&](int k0) {
+            const int o0 = k0 / 8 + o_st;
 #pragma unroll
-        for (int kq = 0; kq < KT; kq += 32) {
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    ra[p][q] = pa[p][(int64_t)(o0 + q) * g.lda];
+                    rb[p][q] = pb[p][(int64_t)(o0 + q) * g.ldb];
+                }
+        };
+        auto store = [&](int buf) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    As[buf][p][(o_st + q) * BT + r_st] = ra[p][q];
+                    Bs[buf][p][(o_st + q) * BT + r_st] = rb[p][q];
+                }
+        };
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        fetch(0);
+        store(0);
+        if (KT < g.K) fetch(KT);
+        __syncthreads();
+        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
+            // tile k0 + KT (in registers) to the other buffer, whose last
+            // readers passed the barrier that ended the previous iteration
+            if (k0 + KT < g.K) {
+                store(cur ^ 1);
+                if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
+            }
             bf16x8 av[NP][4], bv[NP][4];
 #pragma unroll
             for (int p = 0; p < NP; ++p)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    av[p][r] = *reinterpret_cast<const bf16x8*>(&As[cur][p][(wr + r * 16 + li) * LDK + kq + 8 * lg]);
-                    bv[p][r] = *reinterpret_cast<const bf16x8*>(&Bs[cur][p][(wc + r * 16 + li) * LDK + kq + 8 * lg]);
+                    av[p][r] = As[cur][p][lg * BT + wr + r * 16 + li];
+                    bv[p][r] = Bs[cur][p][lg * BT + wc + r * 16 + li];
                 }
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -1403,11 +1567,12 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
                     }
                     acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[0][r], acc[r][c], 0, 0, 0);
                 }
+            __syncthreads();
         }
-        __syncthreads();
+        // acc[r][c][v] = C(row0 + wr + r*16 + li, col0 + wc + c*16 + 4*lg + v)
+        gemm_epilogue<float, 4>(acc, C, g, row0 + wr + li, col0 + wc, lg,
+                                [](int lg_, int v_) { return 4 * lg_ + v_; });
     }
-    // acc[r][c][v] = C(row0 + wr + r*16 + li, col0 + wc + c*16 + 4*lg + v)
-    gemm_epilogue<float, 4>(acc, C, g, row0 + wr + li, col0 + wc, lg, [](int lg_, int v_) { return 4 * lg_ + v_; });
 }
 
 // ------------------------------------------------------------------ solves
@@ -2176,8 +2341,12 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
     if (count > 0) inv_kernel<T><<<count, kThreads, 0, s>>>(jobs, fronts, L, inv);
 }
 template <typename T>
-void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int kernel, T* L, int64_t nL, T* U, hipStream_t s) {
+void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int kernel, T* L, int64_t nL, T* U,
+                 hipStream_t s) {
     if (nblocks <= 0) return;
+    // a grid of at most max_grid workgroups (a multiple of the 8 XCDs)
+    // loops over the nblocks tiles
+    const int grid = max_grid > 0 && max_grid < nblocks ? max_grid : nblocks;
     const bool rows128 = gemm_tile_rows(kernel) == 128;
     if constexpr (std::is_same<T, float>::value) {
         // two-level accumulation on 128-row tiles always runs the 32x32x2
@@ -2186,27 +2355,41 @@ void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int kernel, T* L, 
         const bool split = gemm_split(kernel);
         const int base = gemm_kernel_base(kernel);
         if (base == kGemmMfma32 || (split && rows128)) {
-            if (split) gemm_mfma32_kernel<128, true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
-            else gemm_mfma32_kernel<128, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            if (split) gemm_mfma32_kernel<128, true><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
+            else gemm_mfma32_kernel<128, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
             return;
         }
         if (base == kGemmMfma32x64) {
-            if (split) gemm_mfma32_kernel<64, true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
-            else gemm_mfma32_kernel<64, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U, nL);
+            if (split) gemm_mfma32_kernel<64, true><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
+            else gemm_mfma32_kernel<64, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
             return;
         }
         if (split) {
-            gemm_mfma_kernel<T, 64, true><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+            gemm_mfma_kernel<T, 64, true><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
             return;
         }
     }
-    if (rows128) gemm_mfma_kernel<T, 128, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
-    else gemm_mfma_kernel<T, 64, false><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    if (rows128) gemm_mfma_kernel<T, 128, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
+    else gemm_mfma_kernel<T, 64, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
 }
-void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int nsplit, float* L, float* U, hipStream_t s) {
+void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit, float* L, float* U,
+                      hipStream_t s) {
     if (nblocks <= 0) return;
-    if (nsplit == 3) gemm_bf16_kernel<3><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
-    else gemm_bf16_kernel<1><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, U);
+    const int grid = max_grid > 0 && max_grid < nblocks ? max_grid : nblocks;
+    if (nsplit == 3) gemm_bf16_kernel<3><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
+    else gemm_bf16_kernel<1><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
+}
+
+void launch_gemm_bf16_planes(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit,
+                             const __bf16* hi, const __bf16* lo, float* U, float* L, hipStream_t s) {
+    if (nblocks <= 0) return;
+    const int grid = max_grid > 0 && max_grid < nblocks ? max_grid : nblocks;
+    if (nsplit == 3) gemm_bf16p_kernel<3><<<grid, kThreads, 0, s>>>(jobs, njobs, hi, lo, U, L, nblocks);
+    else gemm_bf16p_kernel<1><<<grid, kThreads, 0, s>>>(jobs, njobs, hi, lo, U, L, nblocks);
+}
+void launch_split_planes(const SplitJob* jobs, int njobs, int nblocks, const float* L, __bf16* hi, __bf16* lo,
+                         hipStream_t s) {
+    if (nblocks > 0) split_kernel<<<nblocks, kSplitRows, 0, s>>>(jobs, njobs, L, hi, lo);
 }
 
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s) {
@@ -2312,7 +2495,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
-    template void launch_gemm<T>(const GemmJob*, int, int, int, T*, int64_t, T*, hipStream_t);                               \
+    template void launch_gemm<T>(const GemmJob*, int, int, int, int, T*, int64_t, T*, hipStream_t);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, int,            \
                                 hipStream_t);                                                                  \

@@ -49,6 +49,11 @@ struct mdgpu_numeric {
     mdgpu::EaItem* dEa = nullptr;
     mdgpu::EaContrib* dEaContrib = nullptr;
     mdgpu::GemmJob* dGemm = nullptr;
+    mdgpu::SplitJob* dSplit = nullptr;
+    // bf16 split planes (hi, then lo at + plane_stride), over the L store's
+    // index space (PlanOptions::presplit; else unallocated)
+    __bf16* dPlanes = nullptr;
+    int64_t plane_stride = 0;
     mdgpu::BlkJob* dBlk = nullptr;
     int* dBlkCnt = nullptr;          // arrival counters of the block-column jobs
     mdgpu::InvJob* dInvJobs = nullptr;
@@ -96,7 +101,11 @@ struct mdgpu_numeric {
     // buffers dAx / dB / dX, launched on the caller's stream
     bool use_graph = true;
     hipStream_t cap_stream = nullptr;
-    hipStream_t side = nullptr;           // look-ahead stream (default priority)
+    hipStream_t side = nullptr;           // look-ahead streams (default priority): trailing
+    hipStream_t side2 = nullptr;          // updates of pivot columns / of update matrices
+    // workgroups of a side-stream GEMM launch (0: one per tile): a bounded
+    // grid loops over its tiles and leaves CUs to the main stream's chain
+    int side_grid = 512;
     std::vector<hipEvent_t> events;       // look-ahead events of the plan
     hipGraphExec_t g_factor = nullptr;
     mdgpu::PlanOptions po;                // options the plan was built with
@@ -170,7 +179,8 @@ void release(mdgpu_numeric* N) {
     if (N->cap_stream) (void)hipStreamDestroy(N->cap_stream);
     N->cap_stream = nullptr;
     if (N->side) (void)hipStreamDestroy(N->side);
-    N->side = nullptr;
+    if (N->side2) (void)hipStreamDestroy(N->side2);
+    N->side = N->side2 = nullptr;
     for (hipEvent_t e : N->events) (void)hipEventDestroy(e);
     N->events.clear();
     if (N->done) (void)hipEventDestroy(N->done);
@@ -227,7 +237,8 @@ int wait_handle(mdgpu_numeric* N) {
 }
 
 // Kernel classes reported by mdgpu_factorize_profile.
-enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_POTRF_TRSM, KIND_INVERSE, KIND_GEMM, KIND_N };
+enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_POTRF_TRSM, KIND_INVERSE, KIND_GEMM, KIND_SPLIT, KIND_N };
+static_assert(KIND_N == MDGPU_KIND_N, "kernel classes of mdgpu_factorize_profile");
 
 int kind_of(int step_kind) {
     switch (step_kind) {
@@ -237,6 +248,7 @@ int kind_of(int step_kind) {
         case mdgpu::STEP_BLK: return KIND_POTRF_TRSM;
         case mdgpu::STEP_INV:
         case mdgpu::STEP_INV256: return KIND_INVERSE;
+        case mdgpu::STEP_SPLIT: return KIND_SPLIT;
         default: return KIND_GEMM;
     }
 }
@@ -262,8 +274,9 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
         const mdgpu::Step& st = P.steps[t];
         // look-ahead (not when timing every step): side-stream launches and
         // their event waits / records
-        s = (!ev && st.stream == 1) ? N->side : main_s;
+        s = ev ? main_s : st.stream == 1 ? N->side : st.stream == 2 ? N->side2 : main_s;
         if (!ev && st.wait_ev >= 0) hipStreamWaitEvent(s, N->events[st.wait_ev], 0);
+        if (!ev && st.wait_ev2 >= 0) hipStreamWaitEvent(s, N->events[st.wait_ev2], 0);
         switch (st.kind) {
             case mdgpu::STEP_EA1:
             case mdgpu::STEP_EA2:
@@ -278,18 +291,31 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt,
                                      N->dInfo + 1, s);
                 break;
-            case mdgpu::STEP_GEMM:
+            case mdgpu::STEP_GEMM: {
+                // side-stream launches run on a bounded grid (look-ahead only)
+                const int cap = (!ev && st.stream != 0) ? N->side_grid : 0;
                 if constexpr (std::is_same<T, float>::value) {
+                    const int nsplit = N->precision == MDGPU_BF16X3 ? 3 : 1;
+                    if (mdgpu::gemm_planes(st.arg)) {
+                        mdgpu::launch_gemm_bf16_planes(N->dGemm + st.first, st.count, st.nblocks, cap, nsplit,
+                                                       N->dPlanes, N->dPlanes + N->plane_stride, U, L, s);
+                        break;
+                    }
                     if ((N->precision == MDGPU_BF16 || N->precision == MDGPU_BF16X3) && st.arg == 128) {
-                        mdgpu::launch_gemm_bf16(N->dGemm + st.first, st.count, st.nblocks,
-                                                N->precision == MDGPU_BF16X3 ? 3 : 1, L, U, s);
+                        mdgpu::launch_gemm_bf16(N->dGemm + st.first, st.count, st.nblocks, cap, nsplit, L, U, s);
                         break;
                     }
                 }
-                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, st.arg, L, P.lstore, U, s);
+                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, cap, st.arg, L, P.lstore, U, s);
                 break;
+            }
             case mdgpu::STEP_INV:
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
+                break;
+            case mdgpu::STEP_SPLIT:
+                if constexpr (std::is_same<T, float>::value)
+                    mdgpu::launch_split_planes(N->dSplit + st.first, st.count, st.nblocks, L, N->dPlanes,
+                                               N->dPlanes + N->plane_stride, s);
                 break;
             case mdgpu::STEP_INV256:
                 mdgpu::launch_inv256<T>(N->dInv256Jobs + st.first, st.count, N->dFronts, L, inv, (T*)N->dInv256, s);
@@ -465,6 +491,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
             for (int r = sym->seg_col[k]; r < sym->seg_col[k + 1]; r++) seg_of_row[r] = k;
     }
     if (const char* g = std::getenv("MDGPU_GRAPH")) N->use_graph = std::atoi(g) != 0;
+    if (const char* g = std::getenv("MDGPU_SIDE_GRID")) N->side_grid = std::max(0, std::atoi(g)) / 8 * 8;
     // 64 elements past the L store: the FP32 GEMM's 16-byte buffer loads
     // are bounded by the store's end at their first element
     if ((st = alloc(N, (P.lstore + 64) * es, &N->dL)) ||
@@ -483,6 +510,9 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.ea, (void**)&N->dEa)) ||
         (st = upload(N, P.ea_contrib, (void**)&N->dEaContrib)) ||
         (st = upload(N, P.gemm, (void**)&N->dGemm)) ||
+        (st = upload(N, P.split, (void**)&N->dSplit)) ||
+        (!P.split.empty() &&
+         (st = alloc(N, 2 * (size_t)P.planes * sizeof(__bf16), (void**)&N->dPlanes))) ||
         (st = upload(N, P.blk, (void**)&N->dBlk)) ||
         (st = upload(N, std::vector<int>(P.blk.size(), 0), (void**)&N->dBlkCnt)) ||
         (st = upload(N, P.panel, (void**)&N->dPanel)) ||
@@ -516,9 +546,11 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = alloc(N, (size_t)S.n * sizeof(double), (void**)&N->dY)))
         return guard(st);
     if (hipStreamCreateWithFlags(&N->cap_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&N->side, hipStreamNonBlocking) != hipSuccess)
+        hipStreamCreateWithFlags(&N->side, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&N->side2, hipStreamNonBlocking) != hipSuccess)
         return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: hipStreamCreate failed"));
     N->dStatus = reinterpret_cast<int*>(N->dMax + 3 * N->nseg);
+    N->plane_stride = P.planes;
     if (hipMemset(N->dMax, 0, (3 * N->nseg + 1) * sizeof(unsigned long long)) != hipSuccess ||
         hipEventCreateWithFlags(&N->done, hipEventDisableTiming) != hipSuccess)
         return guard(fail(MDGPU_ERR_DEVICE, "mdgpu_numeric_create: status word / event set-up failed"));

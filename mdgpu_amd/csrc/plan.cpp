@@ -112,6 +112,16 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         P.max_kids = std::max(P.max_kids, d.kid_end - d.kid_begin);
     }
     P.lstore = lofs;
+    // split-plane offsets (bf16 variants): 16-byte aligned regions
+    std::vector<int64_t> pofs(nf, 0);
+    if (o.presplit) {
+        int64_t off = 0;
+        for (int f = 0; f < nf; f++) {
+            pofs[f] = off;
+            off += align_up((int64_t)P.fronts[f].n * P.fronts[f].c, 8);
+        }
+        P.planes = off;
+    }
     {
         int64_t off = 0;
         for (int f = 0; f < nf; f++) {
@@ -288,30 +298,28 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
     // schedule
     // GEMM kernel code of a launch: o.gemm_default unless the autotuner chose
     int gemm_ordinal = 0;   // GEMM launches emitted so far
-    // kernel of the GEMM launch being assembled (the autotuner's choice if any)
-    auto tile_for = [&](int dflt) {
-        if (gemm_ordinal < (int)o.gemm_tiles.size()) return o.gemm_tiles[gemm_ordinal];
-        return dflt;
-    };
-    auto add_gemm = [&](GemmJob g, std::vector<GemmJob>& batch, int& tiles, int kernel) {
+    // A launch's jobs are collected first; its kernel (the autotuner's choice
+    // for this launch ordinal, else `dflt`) and with it the tile grid are
+    // fixed when the launch is emitted.  Returns false for an empty batch.
+    auto flush_gemm = [&](std::vector<GemmJob>& batch, int L, int dflt) {
+        if (batch.empty()) return false;
+        const int kernel = gemm_ordinal < (int)o.gemm_tiles.size() ? o.gemm_tiles[gemm_ordinal] : dflt;
         const int T = gemm_tile_rows(kernel);
-        g.tiles_m = (g.M + T - 1) / T;
-        g.tiles_n = (g.N + T - 1) / T;
-        if (g.lower) {
-            g.ntiles = 0;
-            for (int tj = 0; tj < g.tiles_n; tj++) g.ntiles += std::max(0, g.tiles_m - tj);
-        } else {
-            g.ntiles = g.tiles_m * g.tiles_n;
+        int tiles = 0;
+        for (GemmJob& g : batch) {
+            g.tiles_m = (g.M + T - 1) / T;
+            g.tiles_n = (g.N + T - 1) / T;
+            if (g.lower) {
+                g.ntiles = 0;
+                for (int tj = 0; tj < g.tiles_n; tj++) g.ntiles += std::max(0, g.tiles_m - tj);
+            } else {
+                g.ntiles = g.tiles_m * g.tiles_n;
+            }
+            g.tile_base = tiles;
+            tiles += g.ntiles;
         }
-        if (g.ntiles == 0) return;
-        g.tile_base = tiles;
-        tiles += g.ntiles;
-        batch.push_back(g);
-    };
-    auto flush_gemm = [&](std::vector<GemmJob>& batch, int tiles, int L, int T) {
-        if (batch.empty()) return;
         gemm_ordinal++;
-        Step s{STEP_GEMM, (int)P.gemm.size(), (int)batch.size(), tiles, L, T};
+        Step s{STEP_GEMM, (int)P.gemm.size(), (int)batch.size(), tiles, L, kernel};
         for (const GemmJob& g : batch) {
             double entries = g.lower ? (double)g.N * g.M - 0.5 * (double)g.N * (g.N - 1) : (double)g.M * g.N;
             s.flops += 2.0 * g.K * entries;
@@ -320,6 +328,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         P.gemm.insert(P.gemm.end(), batch.begin(), batch.end());
         P.steps.push_back(s);
         batch.clear();
+        return true;
     };
     auto add_ea = [&](int L, bool first_pass) {
         // items of this pass: every column of a level-L front that receives
@@ -408,7 +417,10 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         // deep (K = nb_outer) GEMM
         int maxc = 0;
         for (int f : blocked) maxc = std::max(maxc, P.fronts[f].c);
-        int side_last = -1, join_ev = -1;
+        // columns [0, u_done[f]) of a blocked front are applied to U_f
+        std::vector<int> u_done(P.nfronts, 0);
+        int side_last = -1;   // event of the last (b) launch (side stream 1)
+        int u_last = -1;      // event of the last update-matrix launch (side stream 2)
         for (int R0 = 0; R0 < maxc; R0 += o.nb_outer) {
             for (int r0 = R0; r0 < std::min(R0 + o.nb_outer, maxc); r0 += o.nb) {
                 const int dbegin = (int)P.blk.size();
@@ -442,21 +454,23 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     P.steps.push_back(st);
                 }
             }
-            // trailing update of the pivot columns after the outer block, in
-            // two parts: (a) the next outer block's columns, on the main
-            // stream (the next block's factorization needs them), and (b) all
-            // later columns, on the side stream, overlapping the next outer
-            // block's BLK chain.  (a) of this block waits for (b) of the
-            // previous one (both update the same columns); (b) starts when
-            // this outer block is factored.
+            // trailing update after the outer block, in three parts:
+            //  (a) the next outer block's pivot columns, on the main stream
+            //      (the next block's factorization needs them);
+            //  (b) the later pivot columns, on side stream 1, overlapping the
+            //      next outer block's BLK chain;
+            //  (u) with u_lookahead, the front's update matrix U_f (rows and
+            //      columns past the pivots, lower triangle), on side stream 2:
+            //      U_f (+)= -L(c:n, R0:E) L(c:n, R0:E)^T, the first outer
+            //      block with beta0.
+            // (a) of this block waits for (b) of the previous one (both
+            // update the same columns); (b) and (u) start when this outer
+            // block is factored.  A front's last outer block has no trailing
+            // update: its part of U_f is the level's Schur launch below.
             const int inner_done = P.nevents++;
             if (!P.steps.empty()) P.steps.back().rec_ev = inner_done;
-            std::vector<GemmJob> batch_a, batch_b;
-            int tiles_a = 0, tiles_b = 0;
-            // ordinals: (a) is the next GEMM launch, (b) the one after it
-            const int tile_a = tile_for(o.gemm_default);
-            const int tile_b =
-                gemm_ordinal + 1 < (int)o.gemm_tiles.size() ? o.gemm_tiles[gemm_ordinal + 1] : o.gemm_default;
+            std::vector<GemmJob> batch_a, batch_b, batch_u;
+            std::vector<SplitJob> splits;
             for (int f : blocked) {
                 const FrontDesc& d = P.fronts[f];
                 if (R0 >= d.c) continue;
@@ -474,7 +488,19 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 g.c_in_arena = 0;
                 g.beta0 = 0;
                 g.lower = 1;
-                add_gemm(g, batch_a, tiles_a, tile_a);
+                batch_a.push_back(g);
+                // split planes of the block's columns, rows E..n (the side
+                // launches' operands); R0 and E are multiples of 8
+                const int64_t pl0 = pofs[f] + (int64_t)R0 * d.n;
+                if (o.presplit && (E2 < d.c || (o.u_lookahead && d.m > 0))) {
+                    SplitJob sj{};
+                    sj.src = d.lofs + (int64_t)R0 * d.n + E;
+                    sj.dst = pl0 + 8 * (int64_t)E;
+                    sj.ld = d.n;
+                    sj.rows = d.n - E;
+                    sj.octets = (E - R0) / 8;
+                    splits.push_back(sj);
+                }
                 if (E2 < d.c) {
                     GemmJob h = g;
                     h.a_off = d.lofs + (int64_t)R0 * d.n + E2;
@@ -482,51 +508,92 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     h.c_off = d.lofs + (int64_t)E2 * d.n + E2;
                     h.M = d.n - E2;
                     h.N = d.c - E2;
-                    add_gemm(h, batch_b, tiles_b, tile_b);
+                    if (o.presplit) h.a_pl = h.b_pl = pl0 + 8 * (int64_t)E2;
+                    batch_b.push_back(h);
+                }
+                if (o.u_lookahead && d.m > 0 && E - u_done[f] >= std::max(1, o.u_stride) * o.nb_outer) {
+                    const int k0 = u_done[f];
+                    GemmJob u{};
+                    u.a_off = d.lofs + (int64_t)k0 * d.n + d.c;
+                    u.b_off = u.a_off;
+                    if (o.presplit) u.a_pl = u.b_pl = pofs[f] + (int64_t)k0 * d.n + 8 * (int64_t)d.c;
+                    u.c_off = d.uofs;
+                    u.lda = u.ldb = d.n;
+                    u.ldc = d.m;
+                    u.M = u.N = d.m;
+                    u.K = E - k0;
+                    u.c_in_arena = 1;
+                    u.beta0 = k0 == 0;
+                    u.lower = 1;
+                    batch_u.push_back(u);
+                    u_done[f] = E;
                 }
             }
             const int wait_prev = side_last;
-            flush_gemm(batch_a, tiles_a, L, tile_a);
-            if (tiles_a > 0 && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
-            if (tiles_b > 0) {
-                flush_gemm(batch_b, tiles_b, L, tile_b);
+            if (flush_gemm(batch_a, L, o.gemm_default) && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
+            // presplit: the split launch goes first on side stream 1; (b)
+            // follows it there and (u) waits for it
+            int u_wait = inner_done;
+            if (!splits.empty()) {
+                Step st{STEP_SPLIT, (int)P.split.size(), (int)splits.size(), 0, L, 0};
+                for (SplitJob& sj : splits) {
+                    sj.ntiles = (sj.rows + kSplitRows - 1) / kSplitRows * sj.octets;
+                    sj.tile_base = st.nblocks;
+                    st.nblocks += sj.ntiles;
+                    st.bytes += (double)sj.rows * sj.octets * 8 * (o.elem_bytes + 4);
+                }
+                P.split.insert(P.split.end(), splits.begin(), splits.end());
+                st.stream = 1;
+                st.wait_ev = inner_done;
+                // (the join waits for stream 1's last event: this one when
+                // no (b) follows)
+                st.rec_ev = u_wait = side_last = P.nevents++;
+                P.steps.push_back(st);
+            }
+            const int side_kernel = o.presplit ? (o.gemm_default | kGemmPlanes) : o.gemm_default;
+            if (flush_gemm(batch_b, L, side_kernel)) {
                 Step& sb = P.steps.back();
                 sb.stream = 1;
                 sb.wait_ev = inner_done;
                 sb.rec_ev = side_last = P.nevents++;
             }
+            if (flush_gemm(batch_u, L, side_kernel)) {
+                Step& su = P.steps.back();
+                su.stream = 2;
+                su.wait_ev = u_wait;
+                su.rec_ev = u_last = P.nevents++;
+            }
         }
-        // join: everything on the side stream is done before the Schur complements
-        if (side_last >= 0) join_ev = side_last;
         // Schur complements of every front of the level: U_f = -L21 L21^T
-        // the first main-stream launch after the blocked part waits for the
-        // side stream (see the join pass after the loop)
-        if (join_ev >= 0) joins.push_back({P.steps.size(), join_ev});
+        // (with u_lookahead, for a blocked front only the columns from
+        // u_done[f] on: the earlier ones went to side stream 2).  The first
+        // main-stream launch after the blocked part waits for both side
+        // streams (see the join pass after the loop).
+        if (side_last >= 0) joins.push_back({P.steps.size(), side_last});
+        if (u_last >= 0) joins.push_back({P.steps.size(), u_last});
         {
             std::vector<GemmJob> batch;
-            int tiles = 0;
             int kmax = 0;
-            for (int f : fl)
-                if (P.fronts[f].m > 0) kmax = std::max(kmax, P.fronts[f].c);
-            const bool split = o.split_k_min > 0 && kmax >= o.split_k_min;
-            const int tile_s = tile_for(o.gemm_default | (split ? kGemmSplit : 0));
             for (int f : fl) {
                 const FrontDesc& d = P.fronts[f];
                 if (d.m == 0) continue;
+                const int k0 = u_done[f];
                 GemmJob g{};
-                g.a_off = d.lofs + d.c;
+                g.a_off = d.lofs + (int64_t)k0 * d.n + d.c;
                 g.b_off = g.a_off;
                 g.c_off = d.uofs;
                 g.lda = g.ldb = d.n;
                 g.ldc = d.m;
                 g.M = g.N = d.m;
-                g.K = d.c;
+                g.K = d.c - k0;
                 g.c_in_arena = 1;
-                g.beta0 = 1;
+                g.beta0 = k0 == 0;
                 g.lower = 1;
-                add_gemm(g, batch, tiles, tile_s);
+                kmax = std::max(kmax, g.K);
+                batch.push_back(g);
             }
-            flush_gemm(batch, tiles, L, tile_s);
+            const bool split = o.split_k_min > 0 && kmax >= o.split_k_min;
+            flush_gemm(batch, L, o.gemm_default | (split ? kGemmSplit : 0));
         }
         add_ea(L, false);
     }
@@ -572,8 +639,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         size_t t = jn.first;
         while (t < P.steps.size() && P.steps[t].stream != 0) t++;
         if (t == P.steps.size()) return "look-ahead: no launch to join the side stream";
-        if (P.steps[t].wait_ev >= 0) return "look-ahead: join on a launch that already waits";
-        P.steps[t].wait_ev = jn.second;
+        if (P.steps[t].wait_ev < 0) P.steps[t].wait_ev = jn.second;
+        else if (P.steps[t].wait_ev2 < 0) P.steps[t].wait_ev2 = jn.second;
+        else return "look-ahead: join on a launch that already waits twice";
     }
 
     // full symmetric A in the caller's numbering (CSR rows sorted), for the
@@ -667,6 +735,15 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         std::sort(iv.begin(), iv.end());
         for (size_t t = 1; t < iv.size(); t++)
             if (iv[t].first < iv[t - 1].second) return bad("overlapping update matrices at level " + std::to_string(L));
+    }
+    for (const SplitJob& sj : P.split) {
+        if (sj.rows <= 0 || sj.octets <= 0 || sj.ld < sj.rows) return bad("split shape");
+        if (sj.src < 0 || sj.src + (int64_t)(8 * sj.octets - 1) * sj.ld + sj.rows - 1 >= P.lstore)
+            return bad("split source bounds");
+        if (sj.dst < 0 || sj.dst % 8 ||
+            sj.dst + (int64_t)(sj.octets - 1) * 8 * sj.ld + 8 * (int64_t)(sj.rows - 1) + 7 >= P.planes)
+            return bad("split plane bounds");
+        if (sj.ntiles != (sj.rows + kSplitRows - 1) / kSplitRows * sj.octets) return bad("split tiles");
     }
     for (const GemmJob& g : P.gemm) {
         if (g.M <= 0 || g.N <= 0 || g.K <= 0 || (g.lower && g.M < g.N)) return bad("gemm shape");
@@ -794,27 +871,41 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         if (d.c > kPanelMaxPivots) return bad("panel pivots");
     }
     // look-ahead events: recorded once, waited only after being recorded,
-    // only GEMMs on the side stream, and every side launch joined back to
-    // the main stream by a later wait
+    // only GEMMs on the side streams, and every side launch joined back to
+    // the main stream: a later main-stream launch waits for the event of it
+    // or of a later launch on the same side stream (stream order)
     {
         std::vector<int> rec_at(std::max(P.nevents, 1), -1);
+        std::vector<char> waited(std::max(P.nevents, 1), 0);   // by a main-stream launch
         for (size_t t = 0; t < P.steps.size(); t++) {
             const Step& st = P.steps[t];
-            if (st.stream != 0 && (st.stream != 1 || st.kind != STEP_GEMM)) return bad("side-stream launch kind");
-            if (st.wait_ev >= P.nevents || st.rec_ev >= P.nevents) return bad("event index");
-            if (st.wait_ev >= 0 && rec_at[st.wait_ev] < 0) return bad("wait on an event not yet recorded");
+            if (st.stream < 0 || st.stream > 2 || (st.stream != 0 && st.kind != STEP_GEMM && st.kind != STEP_SPLIT))
+                return bad("side-stream launch kind");
+            if (st.wait_ev >= P.nevents || st.rec_ev >= P.nevents || st.wait_ev2 >= P.nevents)
+                return bad("event index");
+            for (int w : {st.wait_ev, st.wait_ev2}) {
+                if (w < 0) continue;
+                if (rec_at[w] < 0) return bad("wait on an event not yet recorded");
+                if (st.stream == 0) waited[w] = 1;
+            }
             if (st.rec_ev >= 0) {
                 if (rec_at[st.rec_ev] >= 0) return bad("event recorded twice");
                 rec_at[st.rec_ev] = (int)t;
             }
         }
         for (size_t t = 0; t < P.steps.size(); t++) {
-            const Step& st = P.steps[t];
-            if (st.stream != 1) continue;
-            if (st.rec_ev < 0) return bad("side launch records no event");
+            const int s = P.steps[t].stream;
+            if (s == 0) continue;
+            if (P.steps[t].rec_ev < 0) return bad("side launch records no event");
             bool joined = false;
-            for (size_t u = t + 1; u < P.steps.size() && !joined; u++)
-                joined = P.steps[u].stream == 0 && P.steps[u].wait_ev == st.rec_ev;
+            for (size_t u = t; u < P.steps.size() && !joined; u++) {
+                const Step& su = P.steps[u];
+                if (su.stream != s || su.rec_ev < 0 || !waited[su.rec_ev]) continue;
+                // the waiting main-stream launch comes after u
+                for (size_t v = u + 1; v < P.steps.size() && !joined; v++)
+                    joined = P.steps[v].stream == 0 &&
+                             (P.steps[v].wait_ev == su.rec_ev || P.steps[v].wait_ev2 == su.rec_ev);
+            }
             if (!joined) return bad("side launch never joined");
         }
     }
@@ -823,6 +914,16 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         if (s.kind == STEP_GEMM) {
             if (!gemm_kernel_valid(s.arg, o.elem_bytes) || gemm_tile_rows(s.arg) > o.tile)
                 return bad("gemm kernel or tile size");
+            if (gemm_planes(s.arg) && !o.presplit) return bad("split-plane GEMM without planes");
+            for (int t = s.first; gemm_planes(s.arg) && t < s.first + s.count; t++) {
+                // rows past M are read clamped to row M - 1
+                const GemmJob& g = P.gemm[t];
+                if (g.K % 32 || g.a_pl < 0 || g.b_pl < 0 || g.a_pl % 8 || g.b_pl % 8) return bad("split-plane GEMM job");
+                const int64_t kl = (int64_t)(g.K / 8 - 1) * 8;
+                if (g.a_pl + kl * g.lda + 8 * (int64_t)(g.M - 1) + 7 >= P.planes ||
+                    g.b_pl + kl * g.ldb + 8 * (int64_t)(g.N - 1) + 7 >= P.planes)
+                    return bad("split-plane GEMM bounds");
+            }
             const int T = gemm_tile_rows(s.arg);
             int tiles = 0;
             for (int t = s.first; t < s.first + s.count; t++) {
@@ -833,6 +934,15 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 tiles += P.gemm[t].ntiles;
             }
             if (tiles != s.nblocks) return bad("gemm tile count");
+        }
+        if (s.kind == STEP_SPLIT) {
+            if (!o.presplit || s.stream != 1) return bad("split launch");
+            int tiles = 0;
+            for (int t = s.first; t < s.first + s.count; t++) {
+                if (t >= (int)P.split.size() || P.split[t].tile_base != tiles) return bad("split tile bases");
+                tiles += P.split[t].ntiles;
+            }
+            if (tiles != s.nblocks) return bad("split tile count");
         }
         if (s.kind == STEP_BLK) {
             int tiles = 0;

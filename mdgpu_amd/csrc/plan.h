@@ -69,6 +69,10 @@ struct EaContrib {
 // One dense job: C -= A * B^T (beta0: C = -A * B^T) on a tile grid.
 struct GemmJob {
     int64_t a_off, b_off, c_off;   // A, B in the L store; C in L store or arena
+    // launches of kernel code kGemmPlanes: A and B read from the bf16 split
+    // planes (see SplitJob) at these offsets -- element (i, k) of A at
+    // a_pl + (k / 8) * 8 * lda + 8 * i + k % 8 -- instead of from the L store
+    int64_t a_pl = -1, b_pl = -1;
     int lda, ldb, ldc;
     int M, N, K;
     int c_in_arena;                // C lives in the update arena
@@ -103,8 +107,24 @@ constexpr int kBlkRows = 64;
 // twice the LDS.
 constexpr int kPanelMaxPivots = 128;
 
+// bf16 split planes (MDGPU_BF16 / MDGPU_BF16X3 with PlanOptions::presplit):
+// two bf16 arrays of Plan::planes entries, hi = bf16(x) and lo =
+// bf16(x - hi) of the factor's entries, in the "octet" layout of each panel:
+// entry (i, k) of front f (rows n) at pofs_f + (k / 8) * 8 n + 8 i + k % 8
+// (pofs_f: a multiple of 8, one n x c region per front), so that 8
+// consecutive k of one row (one MFMA operand) are 16 contiguous bytes.  A split job converts the rows [rb, rb + rows) of `octets` groups
+// of 8 columns starting at a column that is a multiple of 8:
+// element (i, 8 o + e) read at src + (8 o + e) * ld + i and written at dst +
+// 8 o * ld + 8 i + e; workgroups of kSplitRows rows and one octet.
+struct SplitJob {
+    int64_t src, dst;
+    int ld, rows, octets;
+    int tile_base, ntiles;
+};
+constexpr int kSplitRows = 256;
+
 // A launch of the schedule: kind + a slice of the matching job array.
-enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV, STEP_INV256 };
+enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV, STEP_INV256, STEP_SPLIT };
 
 // Diagonal-block inverse job (fronts factored by the LDS panel kernel):
 // block of 64 pivot columns starting at j0 of front `front`.
@@ -121,9 +141,11 @@ struct Step {
     int arg;               // panel: LDS bytes needed
     double flops = 0;      // algorithmic flops of the launch
     double bytes = 0;      // algorithmic HBM bytes of the launch
-    // look-ahead: stream 0 = main, 1 = side; before the launch the stream
-    // waits for event wait_ev, after it records rec_ev (-1: none)
-    int stream = 0, wait_ev = -1, rec_ev = -1;
+    // look-ahead: stream 0 = main, 1 = side (trailing updates of pivot
+    // columns), 2 = side (update matrices); before the launch the stream
+    // waits for events wait_ev and wait_ev2, after it records rec_ev (-1:
+    // none)
+    int stream = 0, wait_ev = -1, rec_ev = -1, wait_ev2 = -1;
 };
 
 // Solve schedule.  Small fronts (solve_small_front) are solved by one
@@ -204,6 +226,8 @@ struct Plan {
     std::vector<EaItem> ea;
     std::vector<EaContrib> ea_contrib;
     std::vector<GemmJob> gemm;
+    std::vector<SplitJob> split;       // bf16 split planes (PlanOptions::presplit)
+    int64_t planes = 0;                // entries of each split plane
     std::vector<BlkJob> blk;
     std::vector<int> panel;            // fronts factored by the LDS panel kernel
     std::vector<InvJob> inv;           // diagonal-block inverses of every front
@@ -234,8 +258,12 @@ struct Plan {
 constexpr int kGemmMfma32 = 129;
 constexpr int kGemmMfma32x64 = 65;   // the 32x32x2 kernel with 64-row tiles
 constexpr int kGemmSplit = 256;
-inline int gemm_kernel_base(int kernel) { return kernel & ~kGemmSplit; }
+// bf16 variants: operands from the split planes (128-row tiles, K a
+// multiple of 32); the other bf16 launches convert FP32 tiles as they stage
+constexpr int kGemmPlanes = 512;
+inline int gemm_kernel_base(int kernel) { return kernel & ~(kGemmSplit | kGemmPlanes); }
 inline bool gemm_split(int kernel) { return (kernel & kGemmSplit) != 0; }
+inline bool gemm_planes(int kernel) { return (kernel & kGemmPlanes) != 0; }
 inline int gemm_tile_rows(int kernel) {
     const int b = gemm_kernel_base(kernel);
     return b == 64 || b == kGemmMfma32x64 ? 64 : 128;
@@ -243,6 +271,7 @@ inline int gemm_tile_rows(int kernel) {
 inline bool gemm_kernel_valid(int kernel, int elem_bytes) {
     const int b = gemm_kernel_base(kernel);
     if (gemm_split(kernel) && elem_bytes != 4) return false;
+    if (gemm_planes(kernel) && (elem_bytes != 4 || b != 128 || gemm_split(kernel))) return false;
     return b == 64 || b == 128 || ((b == kGemmMfma32 || b == kGemmMfma32x64) && elem_bytes == 4);
 }
 
@@ -261,6 +290,17 @@ struct PlanOptions {
     // schedule; empty: gemm_default for every launch
     std::vector<int> gemm_tiles;
     int ea_chunk = 16384;        // target entries per extend-add workgroup
+    // right-looking update-matrix look-ahead: a blocked front's U_f receives
+    // each outer block's product as soon as the block is factored, on a
+    // second side stream, instead of in one Schur launch after the front
+    bool u_lookahead = true;
+    // ... applied every u_stride outer blocks (K = u_stride * nb_outer per
+    // launch): a deeper K reads and writes U_f fewer times
+    int u_stride = 2;
+    // bf16 variants: each outer block's trailing columns are split into the
+    // bf16 planes once (side stream 1), and the side-stream trailing updates
+    // read them (kGemmPlanes)
+    bool presplit = false;
 };
 
 std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P);

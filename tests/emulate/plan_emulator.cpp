@@ -99,8 +99,35 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
     return emulate_factor_selinv(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, Lx_out, Li_out, nullptr);
 }
 
+static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode, int max_zeros,
+                   int nb_outer, bool presplit, int u_stride, double* Lx_out, int* Li_out, double* Zx_out);
+
 extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                                      int max_zeros, int nb_outer, double* Lx_out, int* Li_out, double* Zx_out) {
+    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, false, PlanOptions().u_stride, Lx_out, Li_out,
+                   Zx_out);
+}
+
+// update-matrix look-ahead every u_stride outer blocks
+extern "C" int emulate_factor_ustride(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
+                                      int max_zeros, int nb_outer, int u_stride, int presplit, double* Lx_out,
+                                      int* Li_out) {
+    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, presplit != 0, u_stride, Lx_out, Li_out,
+                   nullptr);
+}
+
+// The schedule of the bf16 variants (split launches, trailing updates from
+// the split planes), run in FP64 with planes that hold the values unsplit:
+// checks the split jobs' coverage and their order before the GEMMs that
+// read the planes (the planes start as NaN).
+extern "C" int emulate_factor_presplit(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
+                                       int max_zeros, int nb_outer, double* Lx_out, int* Li_out) {
+    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, true, PlanOptions().u_stride, Lx_out, Li_out,
+                   nullptr);
+}
+
+static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode, int max_zeros,
+                   int nb_outer, bool presplit, int u_stride, double* Lx_out, int* Li_out, double* Zx_out) {
     AnalyzeOptions ao;
     ao.use_amd = use_amd;
     ao.partition = (Partition)mode;
@@ -111,13 +138,24 @@ extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const 
     po.elem_bytes = 8;
     po.panel_lds_bytes = 61440;
     po.nb = 64;
-    po.nb_outer = nb_outer;
+    // nb_outer < 0: outer blocks of -nb_outer columns without the
+    // update-matrix look-ahead (one Schur launch per level)
+    po.nb_outer = nb_outer < 0 ? -nb_outer : nb_outer;
+    po.u_lookahead = nb_outer > 0;
+    po.u_stride = u_stride;
     po.tile = 64;
+    if (presplit) {   // the bf16 variants' options (FP32 store, 128-row tiles)
+        po.elem_bytes = 4;
+        po.tile = 128;
+        po.gemm_default = 128;
+        po.presplit = true;
+    }
     Plan P;
     if (!build_plan(S, po, P).empty()) return 2;
     if (!validate_plan(P, po).empty()) return 3;
+    if (presplit && P.split.empty()) return 4;
     std::vector<double> L(P.lstore, 0.0), U(std::max<int64_t>(P.arena, 1), std::nan("")),
-        inv(P.dinv, 0.0);
+        inv(P.dinv, 0.0), planes(std::max<int64_t>(P.planes, 1), std::nan(""));
     for (size_t e = 0; e < P.a_dst.size(); e++) L[P.a_dst[e]] = ax[e];
     for (const Step& st : P.steps) {
         if (st.kind == STEP_EA1 || st.kind == STEP_EA2) {
@@ -180,14 +218,32 @@ extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const 
                 invert_block(&L[d.lofs + j0 + (int64_t)j0 * d.n], d.n, std::min(64, d.c - j0),
                              &inv[d.dinv + (int64_t)(j0 / 64) * 4096]);
             }
+        } else if (st.kind == STEP_SPLIT) {
+            for (int t = st.first; t < st.first + st.count; t++) {
+                const SplitJob& sj = P.split[t];
+                for (int o = 0; o < sj.octets; o++)
+                    for (int e = 0; e < 8; e++)
+                        for (int i = 0; i < sj.rows; i++)
+                            planes[sj.dst + (int64_t)o * 8 * sj.ld + 8 * (int64_t)i + e] =
+                                L[sj.src + (int64_t)(8 * o + e) * sj.ld + i];
+            }
         } else if (st.kind == STEP_GEMM) {
+            const bool pl = gemm_planes(st.arg);
             for (int t = st.first; t < st.first + st.count; t++) {
                 const GemmJob& g = P.gemm[t];
                 double* C = g.c_in_arena ? &U[g.c_off] : &L[g.c_off];
+                auto a = [&](int i, int k) {
+                    return pl ? planes[g.a_pl + (int64_t)(k / 8) * 8 * g.lda + 8 * (int64_t)i + k % 8]
+                              : L[g.a_off + i + (int64_t)k * g.lda];
+                };
+                auto b = [&](int j, int k) {
+                    return pl ? planes[g.b_pl + (int64_t)(k / 8) * 8 * g.ldb + 8 * (int64_t)j + k % 8]
+                              : L[g.b_off + j + (int64_t)k * g.ldb];
+                };
                 for (int j = 0; j < g.N; j++)
                     for (int i = (g.lower ? j : 0); i < g.M; i++) {
                         double acc = 0;
-                        for (int k = 0; k < g.K; k++) acc += L[g.a_off + i + (int64_t)k * g.lda] * L[g.b_off + j + (int64_t)k * g.ldb];
+                        for (int k = 0; k < g.K; k++) acc += a(i, k) * b(j, k);
                         double* cp = &C[i + (int64_t)j * g.ldc];
                         *cp = g.beta0 ? -acc : *cp - acc;
                     }

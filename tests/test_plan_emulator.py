@@ -1,5 +1,6 @@
 """The numeric schedule (extend-add rounds and splits, update-arena offsets,
-panel / blocked potrf+TRSM / GEMM jobs at several outer block widths,
+panel / blocked potrf+TRSM / GEMM jobs at several outer block widths, with
+and without the update-matrix look-ahead,
 diagonal-block inverses),
 executed step by step on the CPU by tests/emulate/plan_emulator.cpp with the
 device kernels' loops, reproduces the oracle's factor.  This checks the plan
@@ -30,6 +31,7 @@ def emulator():
 
 CASES = [
     ("grid2d_7_plain", lambda: gen.laplace2d(7), 0, 0),
+    ("st27_12_rn100000", lambda: gen.stencil27(12, seed=1), 2, 100000),
     ("c1_rn1000", lambda: gen.laplace2d(100), 2, 1000),
     ("grid3d_16_rn4000", lambda: gen.laplace3d_7pt(16), 2, 4000),
     ("st27_10_rn100000", lambda: gen.stencil27(10, seed=2), 2, 100000),
@@ -38,7 +40,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("nb_outer", [64, 128, 256])
+@pytest.mark.parametrize("nb_outer", [64, 128, 256, -256])   # -256: no update-matrix look-ahead
 @pytest.mark.parametrize("name,make,mode,mz", CASES, ids=[c[0] for c in CASES])
 def test_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_outer):
     p, i, x = make()
@@ -54,8 +56,52 @@ def test_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_outer):
     assert rel_err(Lx, lx) < 1e-12
 
 
+@pytest.mark.parametrize("nb_outer", [64, 256])
+@pytest.mark.parametrize("name,make,mode,mz", CASES, ids=[c[0] for c in CASES])
+def test_presplit_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_outer):
+    """The bf16 variants' schedule: split launches on the side stream and
+    trailing updates that read the split planes (kept unsplit here, in FP64,
+    and NaN until a split launch writes them) reproduce the oracle's factor."""
+    p, i, x = make()
+    S = symbolic.analyze(p, i, True, ["plain", "sn", "rn"][mode], mz)
+    lp, li, lx, _, _ = oracle_factor(p, i, x, S)
+    I, D = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    Lx = np.zeros(S.lnz)
+    Li = np.zeros(S.lnz, np.int32)
+    st = emulator.emulate_factor_presplit(len(p) - 1, p.ctypes.data_as(I), i.ctypes.data_as(I),
+                                          x.ctypes.data_as(D), 1, mode, mz, nb_outer, Lx.ctypes.data_as(D),
+                                          Li.ctypes.data_as(I))
+    if st == 4:
+        pytest.skip("no front with a trailing update (no split launch)")
+    assert st == 0
+    assert np.array_equal(Li, li)
+    assert rel_err(Lx, lx) < 1e-12
+
+
+@pytest.mark.parametrize("u_stride,presplit", [(1, 0), (3, 1)])
+@pytest.mark.parametrize("name,make,mode,mz", CASES[1:5], ids=[c[0] for c in CASES[1:5]])
+def test_update_matrix_stride_reproduces_oracle(emulator, name, make, mode, mz, u_stride, presplit):
+    """Update-matrix look-ahead applied every u_stride outer blocks (64
+    columns), with and without the split planes."""
+    p, i, x = make()
+    S = symbolic.analyze(p, i, True, ["plain", "sn", "rn"][mode], mz)
+    lp, li, lx, _, _ = oracle_factor(p, i, x, S)
+    I, D = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    Lx = np.zeros(S.lnz)
+    Li = np.zeros(S.lnz, np.int32)
+    st = emulator.emulate_factor_ustride(len(p) - 1, p.ctypes.data_as(I), i.ctypes.data_as(I),
+                                         x.ctypes.data_as(D), 1, mode, mz, 64, u_stride, presplit,
+                                         Lx.ctypes.data_as(D), Li.ctypes.data_as(I))
+    if st == 4:
+        pytest.skip("no front with a trailing update (no split launch)")
+    assert st == 0
+    assert np.array_equal(Li, li)
+    assert rel_err(Lx, lx) < 1e-12
+
+
 SELINV_CASES = [
     ("grid2d_7_plain", lambda: gen.laplace2d(7), 0, 0),
+    ("st27_12_rn100000", lambda: gen.stencil27(12, seed=1), 2, 100000),
     ("grid2d_30_rn500", lambda: gen.laplace2d(30), 2, 500),
     ("grid3d_9_sn", lambda: gen.laplace3d_7pt(9), 1, 0),
     ("st27_9_rn100000", lambda: gen.stencil27(9, seed=2), 2, 100000),

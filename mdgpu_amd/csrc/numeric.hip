@@ -105,7 +105,7 @@ struct mdgpu_numeric {
     hipStream_t side2 = nullptr;          // updates of pivot columns / of update matrices
     // workgroups of a side-stream GEMM launch (0: one per tile): a bounded
     // grid loops over its tiles and leaves CUs to the main stream's chain
-    int side_grid = 512;
+    int side_grid = 0;
     std::vector<hipEvent_t> events;       // look-ahead events of the plan
     hipGraphExec_t g_factor = nullptr;
     mdgpu::PlanOptions po;                // options the plan was built with

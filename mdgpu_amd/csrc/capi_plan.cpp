@@ -26,6 +26,7 @@ PlanOptions plan_options_for(int precision) {
     // every bf16 GEMM converts its FP32 tiles as it stages them)
     po.presplit = precision == MDGPU_BF16 || precision == MDGPU_BF16X3;
     if (const char* e = std::getenv("MDGPU_PRESPLIT")) po.presplit = po.presplit && std::atoi(e) != 0;
+    if (const char* e = std::getenv("MDGPU_SPLIT_MAIN")) po.split_main = std::atoi(e) != 0;
     // MDGPU_U_LOOKAHEAD=0: update matrices in one Schur launch per level
     // (no second side stream)
     if (const char* e = std::getenv("MDGPU_U_LOOKAHEAD")) po.u_lookahead = std::atoi(e) != 0;

@@ -488,11 +488,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 g.c_in_arena = 0;
                 g.beta0 = 0;
                 g.lower = 1;
-                batch_a.push_back(g);
-                // split planes of the block's columns, rows E..n (the side
-                // launches' operands); R0 and E are multiples of 8
+                // split planes of the block's columns, rows E..n (the
+                // operands of the side launches and, split_main, of (a)); R0
+                // and E are multiples of 8
                 const int64_t pl0 = pofs[f] + (int64_t)R0 * d.n;
-                if (o.presplit && (E2 < d.c || (o.u_lookahead && d.m > 0))) {
+                if (o.presplit && o.split_main) g.a_pl = g.b_pl = pl0 + 8 * (int64_t)E;
+                batch_a.push_back(g);
+                if (o.presplit && (o.split_main || E2 < d.c || (o.u_lookahead && d.m > 0))) {
                     SplitJob sj{};
                     sj.src = d.lofs + (int64_t)R0 * d.n + E;
                     sj.dst = pl0 + 8 * (int64_t)E;
@@ -529,12 +531,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     u_done[f] = E;
                 }
             }
+            // presplit: one split launch per outer block, either on the
+            // main stream right after the block chain (split_main: (a) reads
+            // the planes too, and the side streams wait for the split) or
+            // first on side stream 1, where (b) follows it and (u) waits for it
             const int wait_prev = side_last;
-            if (flush_gemm(batch_a, L, o.gemm_default) && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
-            // presplit: the split launch goes first on side stream 1; (b)
-            // follows it there and (u) waits for it
-            int u_wait = inner_done;
-            if (!splits.empty()) {
+            int u_wait = inner_done, b_wait = inner_done;
+            auto emit_split = [&](int stream) {
                 Step st{STEP_SPLIT, (int)P.split.size(), (int)splits.size(), 0, L, 0};
                 for (SplitJob& sj : splits) {
                     sj.ntiles = (sj.rows + kSplitRows - 1) / kSplitRows * sj.octets;
@@ -543,18 +546,25 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     st.bytes += (double)sj.rows * sj.octets * 8 * (o.elem_bytes + 4);
                 }
                 P.split.insert(P.split.end(), splits.begin(), splits.end());
-                st.stream = 1;
-                st.wait_ev = inner_done;
-                // (the join waits for stream 1's last event: this one when
-                // no (b) follows)
-                st.rec_ev = u_wait = side_last = P.nevents++;
+                st.stream = stream;
+                st.wait_ev = stream ? inner_done : -1;
+                st.rec_ev = u_wait = P.nevents++;
+                // (on stream 1 the join waits for its last event: this one
+                // when no (b) follows)
+                if (stream) side_last = st.rec_ev;
+                else b_wait = st.rec_ev;
                 P.steps.push_back(st);
-            }
+            };
+            const bool planes_a = !splits.empty() && o.split_main;
+            if (planes_a) emit_split(0);
+            if (flush_gemm(batch_a, L, planes_a ? (o.gemm_default | kGemmPlanes) : o.gemm_default) && wait_prev >= 0)
+                P.steps.back().wait_ev = wait_prev;
+            if (!splits.empty() && !o.split_main) emit_split(1);
             const int side_kernel = o.presplit ? (o.gemm_default | kGemmPlanes) : o.gemm_default;
             if (flush_gemm(batch_b, L, side_kernel)) {
                 Step& sb = P.steps.back();
                 sb.stream = 1;
-                sb.wait_ev = inner_done;
+                sb.wait_ev = b_wait;
                 sb.rec_ev = side_last = P.nevents++;
             }
             if (flush_gemm(batch_u, L, side_kernel)) {
@@ -936,7 +946,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
             if (tiles != s.nblocks) return bad("gemm tile count");
         }
         if (s.kind == STEP_SPLIT) {
-            if (!o.presplit || s.stream != 1) return bad("split launch");
+            if (!o.presplit || s.stream != (o.split_main ? 0 : 1)) return bad("split launch");
             int tiles = 0;
             for (int t = s.first; t < s.first + s.count; t++) {
                 if (t >= (int)P.split.size() || P.split[t].tile_base != tiles) return bad("split tile bases");

@@ -301,6 +301,10 @@ struct PlanOptions {
     // bf16 planes once (side stream 1), and the side-stream trailing updates
     // read them (kGemmPlanes)
     bool presplit = false;
+    // ... with the split launch on the main stream, so that the next outer
+    // block's update (a) reads the planes too (measured slower on the 64^3
+    // 27-point stencil: the launch costs the chain more than (a) gains)
+    bool split_main = false;
 };
 
 std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P);

@@ -108,12 +108,17 @@ extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const 
                    Zx_out);
 }
 
-// update-matrix look-ahead every u_stride outer blocks
+// update-matrix look-ahead every u_stride outer blocks; presplit 1: split
+// planes with the split launches on side stream 1, 2: on the main stream
+static bool g_split_main = false;
 extern "C" int emulate_factor_ustride(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                                       int max_zeros, int nb_outer, int u_stride, int presplit, double* Lx_out,
                                       int* Li_out) {
-    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, presplit != 0, u_stride, Lx_out, Li_out,
-                   nullptr);
+    g_split_main = presplit == 2;
+    const int st = emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, presplit != 0, u_stride, Lx_out,
+                           Li_out, nullptr);
+    g_split_main = false;
+    return st;
 }
 
 // The schedule of the bf16 variants (split launches, trailing updates from
@@ -149,6 +154,7 @@ static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int us
         po.tile = 128;
         po.gemm_default = 128;
         po.presplit = true;
+        po.split_main = g_split_main;
     }
     Plan P;
     if (!build_plan(S, po, P).empty()) return 2;

@@ -78,7 +78,7 @@ def test_presplit_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_
     assert rel_err(Lx, lx) < 1e-12
 
 
-@pytest.mark.parametrize("u_stride,presplit", [(1, 0), (3, 1)])
+@pytest.mark.parametrize("u_stride,presplit", [(1, 0), (3, 1), (2, 2)])   # presplit 2: split on the main stream
 @pytest.mark.parametrize("name,make,mode,mz", CASES[1:5], ids=[c[0] for c in CASES[1:5]])
 def test_update_matrix_stride_reproduces_oracle(emulator, name, make, mode, mz, u_stride, presplit):
     """Update-matrix look-ahead applied every u_stride outer blocks (64

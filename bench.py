@@ -315,7 +315,8 @@ def main():
                          "(empty: --mode/--max-zeros for every precision)")
     ap.add_argument("--only", default="fp32,fp64,bf16x3,bf16",
                     help="comma-separated precisions to time (fp32 and fp64 are needed for the JSON line)")
-    ap.add_argument("--steps-json", default="", help="write per-launch FP32 timings (class, ms, wgs, flops, bytes, level)")
+    ap.add_argument("--steps-json", default="",
+                    help="write per-launch FP32 timings (class, ms, wgs, flops, bytes, level, stream, GEMM kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "

@@ -242,8 +242,11 @@ int mdgpu_tune_partition(const mdgpu_symbolic* const* syms, int nsym, const doub
                          int solves, void* stream, double* ms, int* best);
 
 /* mdgpu_factorize with one record per launch of the schedule:
- * out[6*t + {0..5}] = kernel class (as above), milliseconds, workgroups,
- * algorithmic flops, algorithmic bytes, level.  cap >= 6 * MDGPU_NUM_STEPS.
+ * out[8*t + {0..7}] = kernel class (as above), milliseconds, workgroups,
+ * algorithmic flops, algorithmic bytes, level, stream of the look-ahead
+ * schedule (0 main, 1 trailing updates of pivot columns, 2 update
+ * matrices), GEMM kernel code (0 for other launches).  The launches run
+ * one after the other on `stream` here.  cap >= 8 * MDGPU_NUM_STEPS.
  * No counterpart in the reference (its timers are per phase, mdgpu.cu:1905-1930). */
 int mdgpu_factorize_steps(mdgpu_numeric* num, const double* d_ax, void* stream, double* out, int64_t cap);
 

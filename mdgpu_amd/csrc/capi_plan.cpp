@@ -31,6 +31,7 @@ PlanOptions plan_options_for(int precision) {
     // (no second side stream)
     if (const char* e = std::getenv("MDGPU_U_LOOKAHEAD")) po.u_lookahead = std::atoi(e) != 0;
     if (const char* e = std::getenv("MDGPU_U_STRIDE")) po.u_stride = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("MDGPU_U_FLUSH")) po.u_flush = std::atoi(e) != 0;
     // outer block width of the blocked panel path (multiple of 64)
     if (const char* e = std::getenv("MDGPU_NB_OUTER")) {
         const int v = std::atoi(e);

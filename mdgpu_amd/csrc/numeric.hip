@@ -690,20 +690,22 @@ extern "C" int mdgpu_factorize_steps(mdgpu_numeric* num, const double* d_ax, voi
     mdgpu::clear_error();
     if (!num || !d_ax || !out) return fail(MDGPU_ERR_ARG, "mdgpu_factorize_steps: null pointer");
     const mdgpu::Plan& P = num->plan;
-    if (cap < (int64_t)P.steps.size() * 6)
-        return fail(MDGPU_ERR_ARG, "mdgpu_factorize_steps: capacity below 6 * MDGPU_NUM_STEPS");
+    if (cap < (int64_t)P.steps.size() * 8)
+        return fail(MDGPU_ERR_ARG, "mdgpu_factorize_steps: capacity below 8 * MDGPU_NUM_STEPS");
     CallOrder order(num, (hipStream_t)stream);
     std::vector<float> ms;
     TRY(factor_timed(num, d_ax, (hipStream_t)stream, ms));
     for (size_t t = 0; t < P.steps.size(); t++) {
         const mdgpu::Step& st = P.steps[t];
-        double* o = out + 6 * t;
+        double* o = out + 8 * t;
         o[0] = kind_of(st.kind);
         o[1] = ms[t + 1];
         o[2] = st.nblocks;
         o[3] = st.flops;
         o[4] = st.bytes;
         o[5] = st.level;
+        o[6] = st.stream;
+        o[7] = st.kind == mdgpu::STEP_GEMM ? st.arg : 0;
     }
     return MDGPU_OK;
 }

@@ -513,7 +513,12 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     if (o.presplit) h.a_pl = h.b_pl = pl0 + 8 * (int64_t)E2;
                     batch_b.push_back(h);
                 }
-                if (o.u_lookahead && d.m > 0 && E - u_done[f] >= std::max(1, o.u_stride) * o.nb_outer) {
+                // (u) once u_stride blocks are pending, and (u_flush) at the
+                // block before the front's last, so that the Schur launch
+                // after the join is left at most one outer block
+                const bool last_next = E + o.nb_outer >= d.c;
+                if (o.u_lookahead && d.m > 0 &&
+                    (E - u_done[f] >= std::max(1, o.u_stride) * o.nb_outer || (o.u_flush && last_next))) {
                     const int k0 = u_done[f];
                     GemmJob u{};
                     u.a_off = d.lofs + (int64_t)k0 * d.n + d.c;

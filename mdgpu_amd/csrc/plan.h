@@ -297,6 +297,8 @@ struct PlanOptions {
     // ... applied every u_stride outer blocks (K = u_stride * nb_outer per
     // launch): a deeper K reads and writes U_f fewer times
     int u_stride = 2;
+    // ... and at the outer block before a front's last one
+    bool u_flush = true;
     // bf16 variants: each outer block's trailing columns are split into the
     // bf16 planes once (side stream 1), and the side-stream trailing updates
     // read them (kGemmPlanes)

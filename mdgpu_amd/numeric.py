@@ -85,15 +85,16 @@ class Factorization:
         return out.value
 
     def factorize_steps(self, ax):
-        """Factorize with a HIP-event time for every launch of the schedule:
-        array of rows (kernel class, ms, workgroups, flops, bytes, level)."""
+        """Factorize with a HIP-event time for every launch of the schedule
+        (one launch after the other): array of rows (kernel class, ms,
+        workgroups, flops, bytes, level, look-ahead stream, GEMM kernel code)."""
         torch = _torch()
         ax = ax.to(device="cuda", dtype=torch.float64).contiguous()
         self._ax = ax
         nsteps = self.info()["steps"]
-        out = np.zeros(6 * max(nsteps, 1))
+        out = np.zeros(8 * max(nsteps, 1))
         L.check(L.lib().mdgpu_factorize_steps(self._h, _dptr(ax), _stream(), out.ctypes.data_as(L._DP), len(out)))
-        return out[:6 * nsteps].reshape(nsteps, 6)
+        return out[:8 * nsteps].reshape(nsteps, 8)
 
     def solve(self, b, out=None):
         """x = A^{-1} b; b of shape (n,) or (nrhs, n) (one right-hand side per row)."""

@@ -212,11 +212,9 @@ int mdgpu_factorize(mdgpu_numeric* num, const double* d_ax, void* stream);
  * launches, algorithmic flops, algorithmic HBM bytes of kernel class k:
  * 0 assembly (zero fill + A scatter), 1 extend-add, 2 LDS panel
  * factorization, 3 blocked diagonal potrf + TRSM, 4 diagonal-block
- * inverses (used by the solves), 5 tiled GEMM/SYRK, 6 bf16 split planes
- * (MDGPU_BF16 / MDGPU_BF16X3: operands converted once for the trailing
- * updates).  The event gaps add launch latency, so use mdgpu_factorize for
- * end-to-end times. */
-#define MDGPU_KIND_N 7
+ * inverses (used by the solves), 5 tiled GEMM/SYRK.  The event
+ * gaps add launch latency, so use mdgpu_factorize for end-to-end times. */
+#define MDGPU_KIND_N 6
 int mdgpu_factorize_profile(mdgpu_numeric* num, const double* d_ax, void* stream,
                             double* out /* [4*MDGPU_KIND_N] */);
 

@@ -98,7 +98,7 @@ def check(status):
 (NUM_DEVICE_BYTES, NUM_LSTORE, NUM_ARENA, NUM_STEPS, NUM_LEVELS, NUM_PANEL_FRONTS, NUM_DIAG_JOBS, NUM_PRECISION,
  NUM_FALLBACKS) = range(9)
 NUM_NINFO = 9
-KIND_NAMES = ["assembly", "extend_add", "panel", "potrf_trsm", "inverse", "gemm", "split"]
+KIND_NAMES = ["assembly", "extend_add", "panel", "potrf_trsm", "inverse", "gemm"]
 
 _SIGNATURES.update({
     "mdgpu_numeric_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),

@@ -22,11 +22,11 @@ PlanOptions plan_options_for(int precision) {
     // in two levels (runs of 64 k), which cuts the factor's backward error
     // after one refinement step by about 10x on the 64^3 27-point stencil
     if (precision == MDGPU_FP32) po.split_k_min = 257;
-    // bf16 variants: trailing updates from split planes (MDGPU_PRESPLIT=0:
-    // every bf16 GEMM converts its FP32 tiles as it stages them)
+    // bf16 variants: GEMMs of blocked fronts from the split planes the block
+    // steps write (MDGPU_PRESPLIT=0: every bf16 GEMM converts its FP32 tiles
+    // as it stages them)
     po.presplit = precision == MDGPU_BF16 || precision == MDGPU_BF16X3;
     if (const char* e = std::getenv("MDGPU_PRESPLIT")) po.presplit = po.presplit && std::atoi(e) != 0;
-    if (const char* e = std::getenv("MDGPU_SPLIT_MAIN")) po.split_main = std::atoi(e) != 0;
     // MDGPU_U_LOOKAHEAD=0: update matrices in one Schur launch per level
     // (no second side stream)
     if (const char* e = std::getenv("MDGPU_U_LOOKAHEAD")) po.u_lookahead = std::atoi(e) != 0;
@@ -54,7 +54,7 @@ extern "C" int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_
     const int64_t es = po.elem_bytes;
     int64_t bytes = (P.lstore + P.arena + P.dinv) * es + P.solve_u * 8 +
                     (int64_t)P.a_dst.size() * 8 + (int64_t)(P.rel.size() + P.q.size()) * 4 +
-                    (int64_t)P.afull_i.size() * 16 + (P.split.empty() ? 0 : 4 * P.planes);
+                    (int64_t)P.afull_i.size() * 16 + 4 * P.planes;
     out[MDGPU_NUM_DEVICE_BYTES] = bytes;
     out[MDGPU_NUM_LSTORE] = P.lstore;
     out[MDGPU_NUM_ARENA] = P.arena;

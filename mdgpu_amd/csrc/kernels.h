@@ -24,10 +24,12 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
                   int* fb_list, int* fallbacks, hipStream_t s);
 // fused diagonal-block Cholesky + TRSM of the rows below; counters: one
 // zero-initialised int per job (returned to zero by the kernel); fallbacks:
-// incremented once per diagonal block refactored in FP64 (pivot growth)
+// incremented once per diagonal block refactored in FP64 (pivot growth);
+// phi / plo (FP32 store, else null): the bf16 split planes the steps also
+// write (BlkJob::pl)
 template <typename T>
 void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
-                hipStream_t s);
+                __bf16* phi, __bf16* plo, hipStream_t s);
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
 // MFMA kernel of the step (64, 128 or kGemmMfma32, the step's GEMM kernel
@@ -46,9 +48,6 @@ void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int max_grid,
 // planes hi / lo (GemmJob::a_pl, b_pl); C in U or L as for the others
 void launch_gemm_bf16_planes(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit,
                              const __bf16* hi, const __bf16* lo, float* U, float* L, hipStream_t s);
-// split planes of the jobs' blocks (SplitJob): hi = bf16(x), lo = bf16(x - hi)
-void launch_split_planes(const SplitJob* jobs, int njobs, int nblocks, const float* L, __bf16* hi, __bf16* lo,
-                         hipStream_t s);
 
 // solves (compute in FP64, factor stored as T); the forward result goes to
 // Y (which the backward solve of large fronts then updates in place), the

@@ -311,6 +311,9 @@ template <> struct MfmaOp<double> {
     __device__ static int row(int lg, int v) { return lg + 4 * v; }
 };
 
+// 8 bf16 values: one 16-byte MFMA operand (8 consecutive k of one row)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
 // ----------------------------------------------- dense: blocked panel path
 // Wavefront-level dense kernels on a 64x64 diagonal block held in registers:
 // lane i keeps row i (a[0..63]); column entries of other lanes are read with
@@ -493,7 +496,8 @@ __device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T
 template <typename T>
 __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
                                                        T* __restrict__ L, int* __restrict__ info,
-                                                       int* __restrict__ counters, int* __restrict__ fallbacks) {
+                                                       int* __restrict__ counters, int* __restrict__ fallbacks,
+                                                       __bf16* __restrict__ phi, __bf16* __restrict__ plo) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     constexpr int kZ = 64 * 65;                                          // one 64 x 64 block
     constexpr int kFb = std::is_same<T, float>::value ? 64 * 65 * 8 : 0;  // FP64 fallback block
@@ -659,6 +663,42 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
                 const int col = 16 * N + MfmaOp<T>::row(lg, v);
                 if (col < b) dst[(int64_t)col * ld] = acc[N][v];
             }
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        // bf16 split planes of the rows (bf16 variants; see Plan::planes):
+        // lane (li, lg) holds columns 16 N + 4 lg .. + 3, so octet (N, p)
+        // (columns 16 N + 8 p .. + 7) is split over lanes lg = 2 p and
+        // 2 p + 1.  Lane parity e writes the octets of N = 2 e, 2 e + 1
+        // after swapping halves with its partner (lane ^ 16).  Columns past
+        // b inside the last octet hold the duplicates above; octets past b
+        // are not written.
+        if (phi != nullptr) {
+            const int p = lg >> 1, e = lg & 1;
+            float rcv[2][4];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) rcv[j][v] = __shfl_xor(e ? acc[j][v] : acc[2 + j][v], 16, 64);
+            if (rw < jb.rows) {
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int N = 2 * e + j;
+                    if (16 * N + 8 * p >= b) continue;
+                    bf16x8 h, l;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const float x0 = e ? rcv[j][v] : acc[N][v], x1 = e ? acc[N][v] : rcv[j][v];
+                        h[v] = (__bf16)x0;
+                        h[4 + v] = (__bf16)x1;
+                        l[v] = (__bf16)(x0 - (float)h[v]);
+                        l[4 + v] = (__bf16)(x1 - (float)h[4 + v]);
+                    }
+                    const int64_t d = jb.pl + (int64_t)(2 * N + p) * 8 * ld + 8 * (int64_t)rw;
+                    *reinterpret_cast<bf16x8*>(phi + d) = h;
+                    *reinterpret_cast<bf16x8*>(plo + d) = l;
+                }
+            }
+        }
     }
     // the last workgroup of the job writes L_D back; last, so that no
     // barrier waits on these stores (As is not written after the Cholesky)
@@ -1294,7 +1334,6 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
 // row per operand as 16-byte vectors; a lane's MFMA operand is 8
 // consecutive k of one row (one 16-byte LDS read).  Accumulator layout as
 // for FP32.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int NSPLIT>
@@ -1422,41 +1461,6 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
     }
 }
 
-// bf16 split planes (SplitJob, plan.h): one workgroup converts kSplitRows
-// rows of one octet (8 columns) of a job.  Each thread reads 8 entries of
-// its row (column-major: a wavefront's loads of one column are contiguous)
-// and writes one 16-byte hi and one 16-byte lo vector (contiguous over the
-// rows).  hi = bf16(x), lo = bf16(x - hi), round to nearest even, as in
-// gemm_bf16_kernel.
-__global__ __launch_bounds__(kSplitRows) void split_kernel(const SplitJob* __restrict__ jobs, int njobs,
-                                                           const float* __restrict__ L, __bf16* __restrict__ hi,
-                                                           __bf16* __restrict__ lo) {
-    int a = 0, b = njobs - 1;
-    const int bid = blockIdx.x;
-    while (a < b) {
-        const int mid = (a + b + 1) >> 1;
-        if (jobs[mid].tile_base <= bid) a = mid; else b = mid - 1;
-    }
-    const SplitJob j = jobs[a];
-    const int local = bid - j.tile_base;
-    const int o = local % j.octets;
-    const int i = (local / j.octets) * kSplitRows + (int)threadIdx.x;
-    if (i >= j.rows) return;
-    const float* __restrict__ src = L + j.src + (int64_t)(8 * o) * j.ld + i;
-    float v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = src[(int64_t)e * j.ld];
-    bf16x8 h, l;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        h[e] = (__bf16)v[e];
-        l[e] = (__bf16)(v[e] - (float)h[e]);
-    }
-    const int64_t d = j.dst + (int64_t)o * 8 * j.ld + 8 * (int64_t)i;
-    *reinterpret_cast<bf16x8*>(hi + d) = h;
-    *reinterpret_cast<bf16x8*>(lo + d) = l;
-}
-
 // The bf16 products of gemm_bf16_kernel with A and B read from the split
 // planes (kGemmPlanes launches): no conversion in the kernel, and every
 // load is a 16-byte vector of 8 consecutive k of one row.  128 x 128 tiles,
@@ -1466,8 +1470,9 @@ __global__ __launch_bounds__(kSplitRows) void split_kernel(const SplitJob* __res
 // multiplied, then stored to the other LDS buffer, laid out [octet][row]
 // (the 16 lanes of an MFMA operand read 256 contiguous bytes).  Rows past M
 // (N) are read clamped to row M - 1 (N - 1): they reach only C entries that
-// are not stored.  K is a multiple of 32.  Same products, order and
-// accumulator layout as gemm_bf16_kernel.
+// are not stored.  In the last k-tile, octets past K are not read and
+// entries past K are zeroed (the planes hold no factor entries there).
+// Same products, order and accumulator layout as gemm_bf16_kernel.
 template <int NSPLIT>
 __global__ __launch_bounds__(kThreads, 2) void gemm_bf16p_kernel(const GemmJob* __restrict__ jobs, int njobs,
                                                                  const __bf16* __restrict__ Ph,
@@ -1516,13 +1521,37 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_bf16p_kernel(const GemmJob* 
         bf16x8 ra[NP][2], rb[NP][2];
         auto fetch = [&](int k0) {
             const int o0 = k0 / 8 + o_st;
+            if (k0 + KT <= g.K) {
 #pragma unroll
-            for (int p = 0; p < NP; ++p)
+                for (int p = 0; p < NP; ++p)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    ra[p][q] = pa[p][(int64_t)(o0 + q) * g.lda];
-                    rb[p][q] = pb[p][(int64_t)(o0 + q) * g.ldb];
+                    for (int q = 0; q < 2; ++q) {
+                        ra[p][q] = pa[p][(int64_t)(o0 + q) * g.lda];
+                        rb[p][q] = pb[p][(int64_t)(o0 + q) * g.ldb];
+                    }
+                return;
+            }
+            // last k-tile, K not a multiple of 32
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int kk = 8 * (o0 + q);   // first k of the octet
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    bf16x8 x = {}, y = {};
+                    if (kk < g.K) {
+                        x = pa[p][(int64_t)(o0 + q) * g.lda];
+                        y = pb[p][(int64_t)(o0 + q) * g.ldb];
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            if (kk + e >= g.K) {
+                                x[e] = (__bf16)0.0f;
+                                y[e] = (__bf16)0.0f;
+                            }
+                    }
+                    ra[p][q] = x;
+                    rb[p][q] = y;
                 }
+            }
         };
         auto store = [&](int buf) {
 #pragma unroll
@@ -2333,8 +2362,9 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 }
 template <typename T>
 void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
-                hipStream_t s) {
-    if (nblocks > 0) blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters, fallbacks);
+                __bf16* phi, __bf16* plo, hipStream_t s) {
+    if (nblocks > 0)
+        blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters, fallbacks, phi, plo);
 }
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {
@@ -2387,11 +2417,6 @@ void launch_gemm_bf16_planes(const GemmJob* jobs, int njobs, int nblocks, int ma
     if (nsplit == 3) gemm_bf16p_kernel<3><<<grid, kThreads, 0, s>>>(jobs, njobs, hi, lo, U, L, nblocks);
     else gemm_bf16p_kernel<1><<<grid, kThreads, 0, s>>>(jobs, njobs, hi, lo, U, L, nblocks);
 }
-void launch_split_planes(const SplitJob* jobs, int njobs, int nblocks, const float* L, __bf16* hi, __bf16* lo,
-                         hipStream_t s) {
-    if (nblocks > 0) split_kernel<<<nblocks, kSplitRows, 0, s>>>(jobs, njobs, L, hi, lo);
-}
-
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s) {
     permute_in_kernel<<<blocks_for(n), kThreads, 0, s>>>(b, perm, xp, n);
 }
@@ -2492,7 +2517,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
     template void launch_extend_add<T>(const EaItem*, int, const EaContrib*, const int*, T*, T*, hipStream_t); \
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, int*, int*, int*, hipStream_t); \
-    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, hipStream_t);                   \
+    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, __bf16*, __bf16*, hipStream_t); \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmJob*, int, int, int, int, T*, int64_t, T*, hipStream_t);                               \

@@ -49,9 +49,8 @@ struct mdgpu_numeric {
     mdgpu::EaItem* dEa = nullptr;
     mdgpu::EaContrib* dEaContrib = nullptr;
     mdgpu::GemmJob* dGemm = nullptr;
-    mdgpu::SplitJob* dSplit = nullptr;
-    // bf16 split planes (hi, then lo at + plane_stride), over the L store's
-    // index space (PlanOptions::presplit; else unallocated)
+    // bf16 split planes (hi, then lo at + plane_stride; Plan::planes entries
+    // each, PlanOptions::presplit; else unallocated)
     __bf16* dPlanes = nullptr;
     int64_t plane_stride = 0;
     mdgpu::BlkJob* dBlk = nullptr;
@@ -237,7 +236,7 @@ int wait_handle(mdgpu_numeric* N) {
 }
 
 // Kernel classes reported by mdgpu_factorize_profile.
-enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_POTRF_TRSM, KIND_INVERSE, KIND_GEMM, KIND_SPLIT, KIND_N };
+enum { KIND_ASSEMBLY = 0, KIND_EXTEND_ADD, KIND_PANEL, KIND_POTRF_TRSM, KIND_INVERSE, KIND_GEMM, KIND_N };
 static_assert(KIND_N == MDGPU_KIND_N, "kernel classes of mdgpu_factorize_profile");
 
 int kind_of(int step_kind) {
@@ -248,7 +247,6 @@ int kind_of(int step_kind) {
         case mdgpu::STEP_BLK: return KIND_POTRF_TRSM;
         case mdgpu::STEP_INV:
         case mdgpu::STEP_INV256: return KIND_INVERSE;
-        case mdgpu::STEP_SPLIT: return KIND_SPLIT;
         default: return KIND_GEMM;
     }
 }
@@ -289,7 +287,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 break;
             case mdgpu::STEP_BLK:
                 mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt,
-                                     N->dInfo + 1, s);
+                                     N->dInfo + 1, N->dPlanes, N->dPlanes ? N->dPlanes + N->plane_stride : nullptr, s);
                 break;
             case mdgpu::STEP_GEMM: {
                 // side-stream launches run on a bounded grid (look-ahead only)
@@ -311,11 +309,6 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
             }
             case mdgpu::STEP_INV:
                 mdgpu::launch_inv<T>(N->dInvJobs + st.first, st.count, N->dFronts, L, inv, s);
-                break;
-            case mdgpu::STEP_SPLIT:
-                if constexpr (std::is_same<T, float>::value)
-                    mdgpu::launch_split_planes(N->dSplit + st.first, st.count, st.nblocks, L, N->dPlanes,
-                                               N->dPlanes + N->plane_stride, s);
                 break;
             case mdgpu::STEP_INV256:
                 mdgpu::launch_inv256<T>(N->dInv256Jobs + st.first, st.count, N->dFronts, L, inv, (T*)N->dInv256, s);
@@ -510,9 +503,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.ea, (void**)&N->dEa)) ||
         (st = upload(N, P.ea_contrib, (void**)&N->dEaContrib)) ||
         (st = upload(N, P.gemm, (void**)&N->dGemm)) ||
-        (st = upload(N, P.split, (void**)&N->dSplit)) ||
-        (!P.split.empty() &&
-         (st = alloc(N, 2 * (size_t)P.planes * sizeof(__bf16), (void**)&N->dPlanes))) ||
+        (P.planes > 0 && (st = alloc(N, 2 * (size_t)P.planes * sizeof(__bf16), (void**)&N->dPlanes))) ||
         (st = upload(N, P.blk, (void**)&N->dBlk)) ||
         (st = upload(N, std::vector<int>(P.blk.size(), 0), (void**)&N->dBlkCnt)) ||
         (st = upload(N, P.panel, (void**)&N->dPanel)) ||

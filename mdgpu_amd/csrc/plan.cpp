@@ -112,13 +112,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         P.max_kids = std::max(P.max_kids, d.kid_end - d.kid_begin);
     }
     P.lstore = lofs;
-    // split-plane offsets (bf16 variants): 16-byte aligned regions
+    // split-plane offsets (bf16 variants): an n x ceil8(c) region per front
     std::vector<int64_t> pofs(nf, 0);
     if (o.presplit) {
         int64_t off = 0;
         for (int f = 0; f < nf; f++) {
             pofs[f] = off;
-            off += align_up((int64_t)P.fronts[f].n * P.fronts[f].c, 8);
+            off += (int64_t)P.fronts[f].n * align_up(P.fronts[f].c, 8);
         }
         P.planes = off;
     }
@@ -439,6 +439,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     bj.cnt = (int)P.blk.size();
                     bj.front = f;
                     bj.kprev = (r0 - R0) / 64;
+                    if (o.presplit) bj.pl = pofs[f] + (int64_t)r0 * d.n + 8 * (int64_t)(r0 + bj.b);
                     btiles += bj.ntiles;
                     P.blk.push_back(bj);
                 }
@@ -470,17 +471,21 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             const int inner_done = P.nevents++;
             if (!P.steps.empty()) P.steps.back().rec_ev = inner_done;
             std::vector<GemmJob> batch_a, batch_b, batch_u;
-            std::vector<SplitJob> splits;
             for (int f : blocked) {
                 const FrontDesc& d = P.fronts[f];
                 if (R0 >= d.c) continue;
                 const int E = std::min(R0 + o.nb_outer, d.c);
                 if (E >= d.c) continue;
                 const int E2 = std::min(E + o.nb_outer, d.c);
+                // split planes of the outer block's columns (presplit): the
+                // block steps wrote them for the rows below each diagonal
+                // block; R0, E, E2 and u_done are multiples of 8
+                const int64_t pl0 = pofs[f] + (int64_t)R0 * d.n;
                 GemmJob g{};
                 g.a_off = d.lofs + (int64_t)R0 * d.n + E;
                 g.b_off = g.a_off;
                 g.c_off = d.lofs + (int64_t)E * d.n + E;
+                if (o.presplit) g.a_pl = g.b_pl = pl0 + 8 * (int64_t)E;
                 g.lda = g.ldb = g.ldc = d.n;
                 g.M = d.n - E;
                 g.N = E2 - E;
@@ -488,29 +493,15 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 g.c_in_arena = 0;
                 g.beta0 = 0;
                 g.lower = 1;
-                // split planes of the block's columns, rows E..n (the
-                // operands of the side launches and, split_main, of (a)); R0
-                // and E are multiples of 8
-                const int64_t pl0 = pofs[f] + (int64_t)R0 * d.n;
-                if (o.presplit && o.split_main) g.a_pl = g.b_pl = pl0 + 8 * (int64_t)E;
                 batch_a.push_back(g);
-                if (o.presplit && (o.split_main || E2 < d.c || (o.u_lookahead && d.m > 0))) {
-                    SplitJob sj{};
-                    sj.src = d.lofs + (int64_t)R0 * d.n + E;
-                    sj.dst = pl0 + 8 * (int64_t)E;
-                    sj.ld = d.n;
-                    sj.rows = d.n - E;
-                    sj.octets = (E - R0) / 8;
-                    splits.push_back(sj);
-                }
                 if (E2 < d.c) {
                     GemmJob h = g;
                     h.a_off = d.lofs + (int64_t)R0 * d.n + E2;
                     h.b_off = h.a_off;
                     h.c_off = d.lofs + (int64_t)E2 * d.n + E2;
+                    if (o.presplit) h.a_pl = h.b_pl = pl0 + 8 * (int64_t)E2;
                     h.M = d.n - E2;
                     h.N = d.c - E2;
-                    if (o.presplit) h.a_pl = h.b_pl = pl0 + 8 * (int64_t)E2;
                     batch_b.push_back(h);
                 }
                 // (u) once u_stride blocks are pending, and (u_flush) at the
@@ -536,46 +527,19 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     u_done[f] = E;
                 }
             }
-            // presplit: one split launch per outer block, either on the
-            // main stream right after the block chain (split_main: (a) reads
-            // the planes too, and the side streams wait for the split) or
-            // first on side stream 1, where (b) follows it and (u) waits for it
             const int wait_prev = side_last;
-            int u_wait = inner_done, b_wait = inner_done;
-            auto emit_split = [&](int stream) {
-                Step st{STEP_SPLIT, (int)P.split.size(), (int)splits.size(), 0, L, 0};
-                for (SplitJob& sj : splits) {
-                    sj.ntiles = (sj.rows + kSplitRows - 1) / kSplitRows * sj.octets;
-                    sj.tile_base = st.nblocks;
-                    st.nblocks += sj.ntiles;
-                    st.bytes += (double)sj.rows * sj.octets * 8 * (o.elem_bytes + 4);
-                }
-                P.split.insert(P.split.end(), splits.begin(), splits.end());
-                st.stream = stream;
-                st.wait_ev = stream ? inner_done : -1;
-                st.rec_ev = u_wait = P.nevents++;
-                // (on stream 1 the join waits for its last event: this one
-                // when no (b) follows)
-                if (stream) side_last = st.rec_ev;
-                else b_wait = st.rec_ev;
-                P.steps.push_back(st);
-            };
-            const bool planes_a = !splits.empty() && o.split_main;
-            if (planes_a) emit_split(0);
-            if (flush_gemm(batch_a, L, planes_a ? (o.gemm_default | kGemmPlanes) : o.gemm_default) && wait_prev >= 0)
-                P.steps.back().wait_ev = wait_prev;
-            if (!splits.empty() && !o.split_main) emit_split(1);
-            const int side_kernel = o.presplit ? (o.gemm_default | kGemmPlanes) : o.gemm_default;
-            if (flush_gemm(batch_b, L, side_kernel)) {
+            const int gk = o.presplit ? (o.gemm_default | kGemmPlanes) : o.gemm_default;
+            if (flush_gemm(batch_a, L, gk) && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
+            if (flush_gemm(batch_b, L, gk)) {
                 Step& sb = P.steps.back();
                 sb.stream = 1;
-                sb.wait_ev = b_wait;
+                sb.wait_ev = inner_done;
                 sb.rec_ev = side_last = P.nevents++;
             }
-            if (flush_gemm(batch_u, L, side_kernel)) {
+            if (flush_gemm(batch_u, L, gk)) {
                 Step& su = P.steps.back();
                 su.stream = 2;
-                su.wait_ev = u_wait;
+                su.wait_ev = inner_done;
                 su.rec_ev = u_last = P.nevents++;
             }
         }
@@ -587,6 +551,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         if (side_last >= 0) joins.push_back({P.steps.size(), side_last});
         if (u_last >= 0) joins.push_back({P.steps.size(), u_last});
         {
+            // presplit: the launch reads the split planes when every front
+            // of it was blocked (fronts factored in LDS write no planes)
+            std::vector<char> is_blocked(P.nfronts, 0);
+            for (int f : blocked) is_blocked[f] = 1;
+            bool planes = o.presplit;
+            for (int f : fl)
+                if (P.fronts[f].m > 0 && !is_blocked[f]) planes = false;
             std::vector<GemmJob> batch;
             int kmax = 0;
             for (int f : fl) {
@@ -596,6 +567,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 GemmJob g{};
                 g.a_off = d.lofs + (int64_t)k0 * d.n + d.c;
                 g.b_off = g.a_off;
+                if (planes) g.a_pl = g.b_pl = pofs[f] + (int64_t)k0 * d.n + 8 * (int64_t)d.c;
                 g.c_off = d.uofs;
                 g.lda = g.ldb = d.n;
                 g.ldc = d.m;
@@ -608,7 +580,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 batch.push_back(g);
             }
             const bool split = o.split_k_min > 0 && kmax >= o.split_k_min;
-            flush_gemm(batch, L, o.gemm_default | (split ? kGemmSplit : 0));
+            flush_gemm(batch, L, o.gemm_default | (split ? kGemmSplit : 0) | (planes ? kGemmPlanes : 0));
         }
         add_ea(L, false);
     }
@@ -751,15 +723,6 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         for (size_t t = 1; t < iv.size(); t++)
             if (iv[t].first < iv[t - 1].second) return bad("overlapping update matrices at level " + std::to_string(L));
     }
-    for (const SplitJob& sj : P.split) {
-        if (sj.rows <= 0 || sj.octets <= 0 || sj.ld < sj.rows) return bad("split shape");
-        if (sj.src < 0 || sj.src + (int64_t)(8 * sj.octets - 1) * sj.ld + sj.rows - 1 >= P.lstore)
-            return bad("split source bounds");
-        if (sj.dst < 0 || sj.dst % 8 ||
-            sj.dst + (int64_t)(sj.octets - 1) * 8 * sj.ld + 8 * (int64_t)(sj.rows - 1) + 7 >= P.planes)
-            return bad("split plane bounds");
-        if (sj.ntiles != (sj.rows + kSplitRows - 1) / kSplitRows * sj.octets) return bad("split tiles");
-    }
     for (const GemmJob& g : P.gemm) {
         if (g.M <= 0 || g.N <= 0 || g.K <= 0 || (g.lower && g.M < g.N)) return bad("gemm shape");
         int64_t alast = g.a_off + (g.M - 1) + (int64_t)(g.K - 1) * g.lda;
@@ -777,6 +740,10 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
             return bad("block bounds");
         if (bj.b + bj.rows > bj.ld) return bad("block rows exceed panel");
         if (bj.front < 0 || bj.front >= nf) return bad("block front");
+        if (o.presplit != (bj.pl >= 0)) return bad("block split planes");
+        if (bj.pl >= 0 && bj.rows > 0 &&
+            (bj.pl % 8 || bj.pl + (int64_t)((bj.b + 7) / 8 - 1) * 8 * bj.ld + 8 * (int64_t)(bj.rows - 1) + 7 >= P.planes))
+            return bad("block split-plane bounds");
         {
             const FrontDesc& d = P.fronts[bj.front];
             const int r0 = bj.col0 - d.k;
@@ -894,7 +861,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
         std::vector<char> waited(std::max(P.nevents, 1), 0);   // by a main-stream launch
         for (size_t t = 0; t < P.steps.size(); t++) {
             const Step& st = P.steps[t];
-            if (st.stream < 0 || st.stream > 2 || (st.stream != 0 && st.kind != STEP_GEMM && st.kind != STEP_SPLIT))
+            if (st.stream < 0 || st.stream > 2 || (st.stream != 0 && st.kind != STEP_GEMM))
                 return bad("side-stream launch kind");
             if (st.wait_ev >= P.nevents || st.rec_ev >= P.nevents || st.wait_ev2 >= P.nevents)
                 return bad("event index");
@@ -931,10 +898,11 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 return bad("gemm kernel or tile size");
             if (gemm_planes(s.arg) && !o.presplit) return bad("split-plane GEMM without planes");
             for (int t = s.first; gemm_planes(s.arg) && t < s.first + s.count; t++) {
-                // rows past M are read clamped to row M - 1
+                // rows past M are read clamped to row M - 1; octets past K
+                // are not read
                 const GemmJob& g = P.gemm[t];
-                if (g.K % 32 || g.a_pl < 0 || g.b_pl < 0 || g.a_pl % 8 || g.b_pl % 8) return bad("split-plane GEMM job");
-                const int64_t kl = (int64_t)(g.K / 8 - 1) * 8;
+                if (g.a_pl < 0 || g.b_pl < 0 || g.a_pl % 8 || g.b_pl % 8) return bad("split-plane GEMM job");
+                const int64_t kl = (int64_t)((g.K + 7) / 8 - 1) * 8;
                 if (g.a_pl + kl * g.lda + 8 * (int64_t)(g.M - 1) + 7 >= P.planes ||
                     g.b_pl + kl * g.ldb + 8 * (int64_t)(g.N - 1) + 7 >= P.planes)
                     return bad("split-plane GEMM bounds");
@@ -949,15 +917,6 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
                 tiles += P.gemm[t].ntiles;
             }
             if (tiles != s.nblocks) return bad("gemm tile count");
-        }
-        if (s.kind == STEP_SPLIT) {
-            if (!o.presplit || s.stream != (o.split_main ? 0 : 1)) return bad("split launch");
-            int tiles = 0;
-            for (int t = s.first; t < s.first + s.count; t++) {
-                if (t >= (int)P.split.size() || P.split[t].tile_base != tiles) return bad("split tile bases");
-                tiles += P.split[t].ntiles;
-            }
-            if (tiles != s.nblocks) return bad("split tile count");
         }
         if (s.kind == STEP_BLK) {
             int tiles = 0;

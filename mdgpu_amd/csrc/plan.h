@@ -70,7 +70,7 @@ struct EaContrib {
 struct GemmJob {
     int64_t a_off, b_off, c_off;   // A, B in the L store; C in L store or arena
     // launches of kernel code kGemmPlanes: A and B read from the bf16 split
-    // planes (see SplitJob) at these offsets -- element (i, k) of A at
+    // planes (see Plan::planes) at these offsets -- element (i, k) of A at
     // a_pl + (k / 8) * 8 * lda + 8 * i + k % 8 -- instead of from the L store
     int64_t a_pl = -1, b_pl = -1;
     int lda, ldb, ldc;
@@ -97,6 +97,10 @@ struct BlkJob {
     int tile_base, ntiles;         // first workgroup of this job, workgroups (>= 1)
     int cnt, front;                // arrival counter index, front
     int kprev;                     // earlier 64-column blocks of the outer block (left-looking update)
+    // split planes (presplit, else -1): plane offset of octet col0 / 8 at
+    // the first row below the diagonal block; the step writes the octets
+    // of its columns there (row r below the block: + 8 r, octet q: + 8 q ld)
+    int64_t pl = -1;
 };
 constexpr int kBlkRows = 64;
 // Fronts factored wholly in LDS by one workgroup (STEP_PANEL) have at most
@@ -111,20 +115,12 @@ constexpr int kPanelMaxPivots = 128;
 // two bf16 arrays of Plan::planes entries, hi = bf16(x) and lo =
 // bf16(x - hi) of the factor's entries, in the "octet" layout of each panel:
 // entry (i, k) of front f (rows n) at pofs_f + (k / 8) * 8 n + 8 i + k % 8
-// (pofs_f: a multiple of 8, one n x c region per front), so that 8
-// consecutive k of one row (one MFMA operand) are 16 contiguous bytes.  A split job converts the rows [rb, rb + rows) of `octets` groups
-// of 8 columns starting at a column that is a multiple of 8:
-// element (i, 8 o + e) read at src + (8 o + e) * ld + i and written at dst +
-// 8 o * ld + 8 i + e; workgroups of kSplitRows rows and one octet.
-struct SplitJob {
-    int64_t src, dst;
-    int ld, rows, octets;
-    int tile_base, ntiles;
-};
-constexpr int kSplitRows = 256;
+// (pofs_f: one n x ceil8(c) region per front), so that 8 consecutive k of one
+// row (one MFMA operand) are 16 contiguous bytes.  The block steps write
+// them (BlkJob::pl) for the rows below their diagonal block.
 
 // A launch of the schedule: kind + a slice of the matching job array.
-enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV, STEP_INV256, STEP_SPLIT };
+enum StepKind { STEP_EA1 = 0, STEP_EA2, STEP_PANEL, STEP_BLK, STEP_GEMM, STEP_INV, STEP_INV256 };
 
 // Diagonal-block inverse job (fronts factored by the LDS panel kernel):
 // block of 64 pivot columns starting at j0 of front `front`.
@@ -226,8 +222,7 @@ struct Plan {
     std::vector<EaItem> ea;
     std::vector<EaContrib> ea_contrib;
     std::vector<GemmJob> gemm;
-    std::vector<SplitJob> split;       // bf16 split planes (PlanOptions::presplit)
-    int64_t planes = 0;                // entries of each split plane
+    int64_t planes = 0;                // entries of each bf16 split plane (PlanOptions::presplit)
     std::vector<BlkJob> blk;
     std::vector<int> panel;            // fronts factored by the LDS panel kernel
     std::vector<InvJob> inv;           // diagonal-block inverses of every front
@@ -258,8 +253,8 @@ struct Plan {
 constexpr int kGemmMfma32 = 129;
 constexpr int kGemmMfma32x64 = 65;   // the 32x32x2 kernel with 64-row tiles
 constexpr int kGemmSplit = 256;
-// bf16 variants: operands from the split planes (128-row tiles, K a
-// multiple of 32); the other bf16 launches convert FP32 tiles as they stage
+// bf16 variants: operands from the split planes (128-row tiles); the other
+// bf16 launches convert FP32 tiles as they stage them
 constexpr int kGemmPlanes = 512;
 inline int gemm_kernel_base(int kernel) { return kernel & ~(kGemmSplit | kGemmPlanes); }
 inline bool gemm_split(int kernel) { return (kernel & kGemmSplit) != 0; }
@@ -299,14 +294,9 @@ struct PlanOptions {
     int u_stride = 2;
     // ... and at the outer block before a front's last one
     bool u_flush = true;
-    // bf16 variants: each outer block's trailing columns are split into the
-    // bf16 planes once (side stream 1), and the side-stream trailing updates
-    // read them (kGemmPlanes)
+    // bf16 variants: the block steps also write their columns as bf16 split
+    // planes, and the GEMMs of the blocked fronts read them (kGemmPlanes)
     bool presplit = false;
-    // ... with the split launch on the main stream, so that the next outer
-    // block's update (a) reads the planes too (measured slower on the 64^3
-    // 27-point stencil: the launch costs the chain more than (a) gains)
-    bool split_main = false;
 };
 
 std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P);

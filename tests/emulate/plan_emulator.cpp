@@ -3,6 +3,7 @@
 // (extend-add rounds, splits, arena offsets, blocked steps) can be checked
 // against the oracle without a GPU.  Built and loaded by the tests only.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -108,23 +109,20 @@ extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const 
                    Zx_out);
 }
 
-// update-matrix look-ahead every u_stride outer blocks; presplit 1: split
-// planes with the split launches on side stream 1, 2: on the main stream
-static bool g_split_main = false;
+// update-matrix look-ahead every u_stride outer blocks, with (presplit) or
+// without the split planes
 extern "C" int emulate_factor_ustride(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                                       int max_zeros, int nb_outer, int u_stride, int presplit, double* Lx_out,
                                       int* Li_out) {
-    g_split_main = presplit == 2;
-    const int st = emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, presplit != 0, u_stride, Lx_out,
-                           Li_out, nullptr);
-    g_split_main = false;
-    return st;
+    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, presplit != 0, u_stride, Lx_out, Li_out,
+                   nullptr);
 }
 
-// The schedule of the bf16 variants (split launches, trailing updates from
-// the split planes), run in FP64 with planes that hold the values unsplit:
-// checks the split jobs' coverage and their order before the GEMMs that
-// read the planes (the planes start as NaN).
+// The schedule of the bf16 variants (block steps that also write the split
+// planes, GEMMs that read them), run in FP64 with planes that hold the
+// values unsplit: checks that every plane entry a GEMM reads was written
+// before it (the planes start as NaN, and the padding past a front's last
+// column is written as NaN).
 extern "C" int emulate_factor_presplit(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                                        int max_zeros, int nb_outer, double* Lx_out, int* Li_out) {
     return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, true, PlanOptions().u_stride, Lx_out, Li_out,
@@ -154,12 +152,19 @@ static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int us
         po.tile = 128;
         po.gemm_default = 128;
         po.presplit = true;
-        po.split_main = g_split_main;
     }
     Plan P;
-    if (!build_plan(S, po, P).empty()) return 2;
-    if (!validate_plan(P, po).empty()) return 3;
-    if (presplit && P.split.empty()) return 4;
+    std::string err = build_plan(S, po, P);
+    if (!err.empty()) {
+        fprintf(stderr, "emulate: %s\n", err.c_str());
+        return 2;
+    }
+    err = validate_plan(P, po);
+    if (!err.empty()) {
+        fprintf(stderr, "emulate: %s\n", err.c_str());
+        return 3;
+    }
+    if (presplit && P.blk.empty()) return 4;
     std::vector<double> L(P.lstore, 0.0), U(std::max<int64_t>(P.arena, 1), std::nan("")),
         inv(P.dinv, 0.0), planes(std::max<int64_t>(P.planes, 1), std::nan(""));
     for (size_t e = 0; e < P.a_dst.size(); e++) L[P.a_dst[e]] = ax[e];
@@ -216,6 +221,12 @@ static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int us
                         row[(int64_t)j * ld] = acc / g[j + (int64_t)j * ld];
                     }
                 }
+                // split planes of the rows below the block (octet layout)
+                if (bj.pl >= 0)
+                    for (int i = 0; i < bj.rows; i++)
+                        for (int j = 0; j < (b + 7) / 8 * 8; j++)
+                            planes[bj.pl + (int64_t)(j / 8) * 8 * ld + 8 * (int64_t)i + j % 8] =
+                                j < b ? g[b + i + (int64_t)j * ld] : std::nan("");
             }
         } else if (st.kind == STEP_INV) {
             for (int t = st.first; t < st.first + st.count; t++) {
@@ -223,15 +234,6 @@ static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int us
                 const int j0 = P.inv[t].j0;
                 invert_block(&L[d.lofs + j0 + (int64_t)j0 * d.n], d.n, std::min(64, d.c - j0),
                              &inv[d.dinv + (int64_t)(j0 / 64) * 4096]);
-            }
-        } else if (st.kind == STEP_SPLIT) {
-            for (int t = st.first; t < st.first + st.count; t++) {
-                const SplitJob& sj = P.split[t];
-                for (int o = 0; o < sj.octets; o++)
-                    for (int e = 0; e < 8; e++)
-                        for (int i = 0; i < sj.rows; i++)
-                            planes[sj.dst + (int64_t)o * 8 * sj.ld + 8 * (int64_t)i + e] =
-                                L[sj.src + (int64_t)(8 * o + e) * sj.ld + i];
             }
         } else if (st.kind == STEP_GEMM) {
             const bool pl = gemm_planes(st.arg);

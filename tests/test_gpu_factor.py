@@ -65,6 +65,30 @@ def test_bf16x3_refined_to_1e12(device, name, make, mode, mz):
     assert eta <= 1e-12 and eta_cpu <= 1e-12, (eta, eta_cpu, iters)
 
 
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16"])
+@pytest.mark.parametrize("nb_outer", ["64", "256"])
+def test_split_planes_equal_staged_conversion(device, monkeypatch, prec, nb_outer):
+    """The bf16 GEMMs that read the split planes the block steps wrote give
+    the factor the staged kernels give (they convert the FP32 tiles as they
+    load them), bit for bit: same hi/lo values, same k order and products.
+    Fronts of several outer blocks with pivot counts that are not multiples
+    of 32 (the planes kernel's last k-tile is masked)."""
+    import torch
+    p, i, x = gen.stencil27(14, seed=3)
+    S = symbolic.analyze(p, i, True, "rn", 100000)
+    ax = torch.from_numpy(x).to(device)
+    monkeypatch.setenv("MDGPU_NB_OUTER", nb_outer)
+    out = {}
+    for presplit in ("0", "1"):
+        monkeypatch.setenv("MDGPU_PRESPLIT", presplit)
+        F = Factorization(S, prec).factorize(ax)
+        out[presplit] = F.export_L()[2]
+    cs = [nd[1] for nd in S.nodes()]
+    assert max(cs) > 2 * int(nb_outer) and any(c % 32 for c in cs if c > int(nb_outer))
+    assert np.isfinite(out["1"]).all()
+    assert np.array_equal(out["0"], out["1"])
+
+
 BF16_CASES = [c for c in CASES if c[0] in ("grid2d_7_plain", "bbt_600_rn1000")]
 
 

@@ -59,9 +59,10 @@ def test_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_outer):
 @pytest.mark.parametrize("nb_outer", [64, 256])
 @pytest.mark.parametrize("name,make,mode,mz", CASES, ids=[c[0] for c in CASES])
 def test_presplit_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_outer):
-    """The bf16 variants' schedule: split launches on the side stream and
-    trailing updates that read the split planes (kept unsplit here, in FP64,
-    and NaN until a split launch writes them) reproduce the oracle's factor."""
+    """The bf16 variants' schedule: block steps that also write the split
+    planes, GEMMs that read them (kept unsplit here, in FP64, NaN until a
+    block step writes them and past a front's last column) reproduce the
+    oracle's factor."""
     p, i, x = make()
     S = symbolic.analyze(p, i, True, ["plain", "sn", "rn"][mode], mz)
     lp, li, lx, _, _ = oracle_factor(p, i, x, S)
@@ -72,13 +73,13 @@ def test_presplit_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_
                                           x.ctypes.data_as(D), 1, mode, mz, nb_outer, Lx.ctypes.data_as(D),
                                           Li.ctypes.data_as(I))
     if st == 4:
-        pytest.skip("no front with a trailing update (no split launch)")
+        pytest.skip("no blocked front (no split planes)")
     assert st == 0
     assert np.array_equal(Li, li)
     assert rel_err(Lx, lx) < 1e-12
 
 
-@pytest.mark.parametrize("u_stride,presplit", [(1, 0), (3, 1), (2, 2)])   # presplit 2: split on the main stream
+@pytest.mark.parametrize("u_stride,presplit", [(1, 0), (3, 1)])
 @pytest.mark.parametrize("name,make,mode,mz", CASES[1:5], ids=[c[0] for c in CASES[1:5]])
 def test_update_matrix_stride_reproduces_oracle(emulator, name, make, mode, mz, u_stride, presplit):
     """Update-matrix look-ahead applied every u_stride outer blocks (64
@@ -93,7 +94,7 @@ def test_update_matrix_stride_reproduces_oracle(emulator, name, make, mode, mz, 
                                          x.ctypes.data_as(D), 1, mode, mz, 64, u_stride, presplit,
                                          Lx.ctypes.data_as(D), Li.ctypes.data_as(I))
     if st == 4:
-        pytest.skip("no front with a trailing update (no split launch)")
+        pytest.skip("no blocked front (no split planes)")
     assert st == 0
     assert np.array_equal(Li, li)
     assert rel_err(Lx, lx) < 1e-12

@@ -536,7 +536,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 sb.wait_ev = inner_done;
                 sb.rec_ev = side_last = P.nevents++;
             }
-            if (flush_gemm(batch_u, L, gk)) {
+            if (flush_gemm(batch_u, L, o.schur_kernel ? o.schur_kernel : gk)) {
                 Step& su = P.steps.back();
                 su.stream = 2;
                 su.wait_ev = inner_done;
@@ -580,7 +580,9 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 batch.push_back(g);
             }
             const bool split = o.split_k_min > 0 && kmax >= o.split_k_min;
-            flush_gemm(batch, L, o.gemm_default | (split ? kGemmSplit : 0) | (planes ? kGemmPlanes : 0));
+            flush_gemm(batch, L,
+                       o.schur_kernel ? o.schur_kernel
+                                      : o.gemm_default | (split ? kGemmSplit : 0) | (planes ? kGemmPlanes : 0));
         }
         add_ea(L, false);
     }

@@ -297,6 +297,9 @@ struct PlanOptions {
     // bf16 variants: the block steps also write their columns as bf16 split
     // planes, and the GEMMs of the blocked fronts read them (kGemmPlanes)
     bool presplit = false;
+    // GEMM kernel code of the update-matrix launches ((u) and the Schur
+    // launch) when nonzero, else the same kernel as the other GEMMs
+    int schur_kernel = 0;
 };
 
 std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P);

@@ -340,6 +340,22 @@ extern "C" int emulate_validate_mistake(int n, const int* Cp, const int* Ci, int
                 break;
             }
         want = "solve chain inverse";
+    } else if (kind == 5) {
+        // the last join of side stream 2 dropped: its launches of that level
+        // are never waited for
+        for (size_t t = P.steps.size(); t-- > 0 && !done;) {
+            Step& st = P.steps[t];
+            if (st.stream != 0) continue;
+            for (int* w : {&st.wait_ev2, &st.wait_ev})
+                if (!done && *w >= 0)
+                    for (const Step& q : P.steps)
+                        if (q.stream == 2 && q.rec_ev == *w) {
+                            *w = -1;
+                            done = true;
+                            break;
+                        }
+        }
+        want = "side launch never joined";
     }
     if (!done) return 2;
     const std::string err = validate_plan(P, po);

@@ -136,11 +136,12 @@ def test_selinv_schedule_matches_dense_inverse(emulator, name, make, mode, mz):
 
 @pytest.mark.parametrize("kind,want", [(0, "forward chain order"), (1, "backward chain order"),
                                        (2, "chain counter"), (3, "left-looking range"),
-                                       (4, "chain block inverse")])
+                                       (4, "chain block inverse"), (5, "side stream never joined")])
 def test_validator_rejects_planted_mistakes(emulator, kind, want):
     """A mistake planted in a valid plan (chained-solve order, counters,
-    inverses; left-looking block range) is caught by validate_plan with the
-    matching message, before anything would run on a GPU."""
+    inverses; left-looking block range; a side-stream launch never joined)
+    is caught by validate_plan with the matching message, before anything
+    would run on a GPU."""
     p, i, x = gen.stencil27(16, seed=1)
     I = C.POINTER(C.c_int)
     st = emulator.emulate_validate_mistake(len(p) - 1, p.ctypes.data_as(I), i.ctypes.data_as(I), 2, 100000, kind)

@@ -111,7 +111,9 @@ def permuted_lower(p, i, x, perm):
 
 def cpu_sample(S, p, i, x, budget_flops):
     """Time the CPU oracle (single thread) on complete subtrees of the front
-    tree, smallest-level first, until ~budget_flops of factorization work."""
+    tree, about budget_flops of factorization work: the largest subtrees
+    that fit, so that the sample holds fronts as large as the budget allows
+    (the whole tree when it fits)."""
     from mdgpu_amd import _lib as L
     from oracle import oracle as O
     nodes = S.nodes()
@@ -127,19 +129,28 @@ def cpu_sample(S, p, i, x, budget_flops):
         sub[f] += float(np.sum(count[c0:c0 + c] ** 2))
         if parent[f] >= 0:
             sub[parent[f]] += sub[f]
-    # maximal subtrees under the budget, taken in postorder
+    # disjoint subtrees, largest first, while they fit in the budget
+    inside = np.zeros(nn, bool)             # in a chosen subtree
     chosen, total = [], 0.0
-    for f in range(nn):
-        fits = sub[f] <= budget_flops - total
-        par_fits = parent[f] >= 0 and sub[parent[f]] <= budget_flops - total
-        if fits and not par_fits and sub[f] > 0:
-            chosen.append(f)
-            total += sub[f]
-    keep = np.zeros(nn, bool)
-    for f in reversed(range(nn)):
-        if f in set(chosen) or (parent[f] >= 0 and keep[parent[f]]):
-            keep[f] = True
-    keep_idx = np.nonzero(keep)[0]
+    for f in sorted(range(nn), key=lambda t: -sub[t]):
+        if sub[f] <= 0 or sub[f] > budget_flops - total:
+            continue
+        a, up = f, False
+        while a >= 0:                       # skip subtrees of chosen ones
+            if inside[a]:
+                up = True
+                break
+            a = parent[a]
+        if up:
+            continue
+        chosen.append(f)
+        total += sub[f]
+        stack = [f]
+        while stack:
+            t = stack.pop()
+            inside[t] = True
+            stack.extend(nodes[t][4])
+    keep_idx = np.nonzero(inside)[0]
     remap = {int(f): t for t, f in enumerate(keep_idx)}
     first = [nodes[f][0] for f in keep_idx]
     ncols = [nodes[f][1] for f in keep_idx]
@@ -306,7 +317,7 @@ def main():
     ap.add_argument("--mode", default="rn")
     ap.add_argument("--max-zeros", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-12)
-    ap.add_argument("--cpu-budget-gflop", type=float, default=60.0)
+    ap.add_argument("--cpu-budget-gflop", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aniso", action="store_true", help="skip the z-anisotropic C3 variant")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile sizes")
@@ -475,9 +486,10 @@ def main():
     if not args.no_cpu:
         cs = cpu_sample(S, p, i, x, args.cpu_budget_gflop * 1e9)
         cpu = {"value": cs["value"], "unit": "GFLOP/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle (FP64 restatement of the reference CPU path) on {cs['subtrees']} "
-                         f"complete subtrees ({cs['fronts']} fronts, {cs['flops']:.3e} flops) of the "
-                         f"same front tree, {cs['seconds']:.1f} s, 1 thread"}
+               "sample": f"CPU oracle (FP64 restatement of the reference CPU path) on the {cs['subtrees']} "
+                         f"largest complete subtrees of the same front tree that fit in "
+                         f"{args.cpu_budget_gflop:g} GFLOP ({cs['fronts']} fronts, {cs['flops']:.3e} flops), "
+                         f"{cs['seconds']:.1f} s, 1 thread"}
         log(f"cpu sample: {cpu['value']:.2f} GFLOP/s over {cs['seconds']:.1f}s")
 
     out = {

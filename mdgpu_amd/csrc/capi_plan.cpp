@@ -27,6 +27,7 @@ PlanOptions plan_options_for(int precision) {
     // as it stages them)
     po.presplit = precision == MDGPU_BF16 || precision == MDGPU_BF16X3;
     if (const char* e = std::getenv("MDGPU_PRESPLIT")) po.presplit = po.presplit && std::atoi(e) != 0;
+    if (const char* e = std::getenv("MDGPU_PLANES")) po.nplanes = std::atoi(e) == 3 ? 3 : 2;
     // MDGPU_U_LOOKAHEAD=0: update matrices in one Schur launch per level
     // (no second side stream)
     if (const char* e = std::getenv("MDGPU_U_LOOKAHEAD")) po.u_lookahead = std::atoi(e) != 0;
@@ -55,7 +56,7 @@ extern "C" int mdgpu_plan_check(const mdgpu_symbolic* sym, int precision, int64_
     const int64_t es = po.elem_bytes;
     int64_t bytes = (P.lstore + P.arena + P.dinv) * es + P.solve_u * 8 +
                     (int64_t)P.a_dst.size() * 8 + (int64_t)(P.rel.size() + P.q.size()) * 4 +
-                    (int64_t)P.afull_i.size() * 16 + 4 * P.planes;
+                    (int64_t)P.afull_i.size() * 16 + 2 * po.nplanes * P.planes;
     out[MDGPU_NUM_DEVICE_BYTES] = bytes;
     out[MDGPU_NUM_LSTORE] = P.lstore;
     out[MDGPU_NUM_ARENA] = P.arena;

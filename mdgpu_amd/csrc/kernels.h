@@ -25,11 +25,11 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 // fused diagonal-block Cholesky + TRSM of the rows below; counters: one
 // zero-initialised int per job (returned to zero by the kernel); fallbacks:
 // incremented once per diagonal block refactored in FP64 (pivot growth);
-// phi / plo (FP32 store, else null): the bf16 split planes the steps also
-// write (BlkJob::pl)
+// nplanes > 0 (FP32 store): the steps also write that many bf16 split
+// planes (BlkJob::pl), plane p at planes + p * pstride
 template <typename T>
 void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
-                __bf16* phi, __bf16* plo, hipStream_t s);
+                __bf16* planes, int64_t pstride, int nplanes, hipStream_t s);
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
 // MFMA kernel of the step (64, 128 or kGemmMfma32, the step's GEMM kernel
@@ -45,9 +45,11 @@ void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int 
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit, float* L, float* U,
                       hipStream_t s);
 // kGemmPlanes launches: the same products with A and B from the bf16 split
-// planes hi / lo (GemmJob::a_pl, b_pl); C in U or L as for the others
+// planes (GemmJob::a_pl, b_pl; plane p at planes + p * pstride); nsplit 6:
+// hi, mid and lo planes, products hi*lo + lo*hi + mid*mid + hi*mid + mid*hi
+// + hi*hi; C in U or L as for the others
 void launch_gemm_bf16_planes(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit,
-                             const __bf16* hi, const __bf16* lo, float* U, float* L, hipStream_t s);
+                             const __bf16* planes, int64_t pstride, float* U, float* L, hipStream_t s);
 
 // solves (compute in FP64, factor stored as T); the forward result goes to
 // Y (which the backward solve of large fronts then updates in place), the

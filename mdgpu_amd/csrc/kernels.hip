@@ -497,7 +497,7 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
                                                        T* __restrict__ L, int* __restrict__ info,
                                                        int* __restrict__ counters, int* __restrict__ fallbacks,
-                                                       __bf16* __restrict__ phi, __bf16* __restrict__ plo) {
+                                                       __bf16* __restrict__ planes, int64_t pstride, int nplanes) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     constexpr int kZ = 64 * 65;                                          // one 64 x 64 block
     constexpr int kFb = std::is_same<T, float>::value ? 64 * 65 * 8 : 0;  // FP64 fallback block
@@ -665,14 +665,15 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
             }
     }
     if constexpr (std::is_same<T, float>::value) {
-        // bf16 split planes of the rows (bf16 variants; see Plan::planes):
+        // bf16 split planes of the rows (bf16 variants; see Plan::planes;
+        // nplanes = 2: hi and lo = bf16(x - hi), 3: hi, mid and lo):
         // lane (li, lg) holds columns 16 N + 4 lg .. + 3, so octet (N, p)
         // (columns 16 N + 8 p .. + 7) is split over lanes lg = 2 p and
         // 2 p + 1.  Lane parity e writes the octets of N = 2 e, 2 e + 1
         // after swapping halves with its partner (lane ^ 16).  Columns past
         // b inside the last octet hold the duplicates above; octets past b
         // are not written.
-        if (phi != nullptr) {
+        if (nplanes > 0) {
             const int p = lg >> 1, e = lg & 1;
             float rcv[2][4];
 #pragma unroll
@@ -684,18 +685,21 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
                 for (int j = 0; j < 2; ++j) {
                     const int N = 2 * e + j;
                     if (16 * N + 8 * p >= b) continue;
-                    bf16x8 h, l;
+                    // x = h + m + l: h = bf16(x), m = bf16(x - h), l = bf16(x - h - m)
+                    // (each difference exact in FP32)
+                    bf16x8 h, m, l;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const float x0 = e ? rcv[j][v] : acc[N][v], x1 = e ? acc[N][v] : rcv[j][v];
-                        h[v] = (__bf16)x0;
-                        h[4 + v] = (__bf16)x1;
-                        l[v] = (__bf16)(x0 - (float)h[v]);
-                        l[4 + v] = (__bf16)(x1 - (float)h[4 + v]);
+                    for (int v = 0; v < 8; ++v) {
+                        const float x = v < 4 ? (e ? rcv[j][v] : acc[N][v]) : (e ? acc[N][v - 4] : rcv[j][v - 4]);
+                        h[v] = (__bf16)x;
+                        const float r = x - (float)h[v];
+                        m[v] = (__bf16)r;
+                        l[v] = (__bf16)(r - (float)m[v]);
                     }
                     const int64_t d = jb.pl + (int64_t)(2 * N + p) * 8 * ld + 8 * (int64_t)rw;
-                    *reinterpret_cast<bf16x8*>(phi + d) = h;
-                    *reinterpret_cast<bf16x8*>(plo + d) = l;
+                    *reinterpret_cast<bf16x8*>(planes + d) = h;
+                    *reinterpret_cast<bf16x8*>(planes + pstride + d) = m;
+                    if (nplanes == 3) *reinterpret_cast<bf16x8*>(planes + 2 * pstride + d) = l;
                 }
             }
         }
@@ -1474,12 +1478,13 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16_kernel(const GemmJob* __re
 // entries past K are zeroed (the planes hold no factor entries there).
 // Same products, order and accumulator layout as gemm_bf16_kernel.
 template <int NSPLIT>
-__global__ __launch_bounds__(kThreads, 2) void gemm_bf16p_kernel(const GemmJob* __restrict__ jobs, int njobs,
-                                                                 const __bf16* __restrict__ Ph,
-                                                                 const __bf16* __restrict__ Pl, float* __restrict__ U,
-                                                                 float* __restrict__ L, int ntiles) {
+__global__ __launch_bounds__(kThreads, NSPLIT == 6 ? 1 : 2) void gemm_bf16p_kernel(
+    const GemmJob* __restrict__ jobs, int njobs, const __bf16* __restrict__ Pbase, int64_t pstride,
+    float* __restrict__ U, float* __restrict__ L, int ntiles) {
+    static_assert(NSPLIT == 1 || NSPLIT == 3 || NSPLIT == 6, "1, 3 or 6 bf16 products");
     constexpr int BT = 128, KT = 32, NO = KT / 8;
-    constexpr int NP = NSPLIT == 1 ? 1 : 2;   // hi (and lo) planes
+    // planes read: hi (1 product); hi, mid (3: mid is BF16X3's lo); hi, mid, lo (6)
+    constexpr int NP = NSPLIT == 1 ? 1 : NSPLIT == 3 ? 2 : 3;
     __shared__ bf16x8 As[2][NP][NO * BT];
     __shared__ bf16x8 Bs[2][NP][NO * BT];
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -1512,11 +1517,10 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_bf16p_kernel(const GemmJob* 
         // 16-byte vectors: octet o of row i at base + o * ld + i
         const bf16x8* __restrict__ pa[NP];
         const bf16x8* __restrict__ pb[NP];
-        pa[0] = reinterpret_cast<const bf16x8*>(Ph + g.a_pl) + ia;
-        pb[0] = reinterpret_cast<const bf16x8*>(Ph + g.b_pl) + jb;
-        if constexpr (NP == 2) {
-            pa[1] = reinterpret_cast<const bf16x8*>(Pl + g.a_pl) + ia;
-            pb[1] = reinterpret_cast<const bf16x8*>(Pl + g.b_pl) + jb;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            pa[p] = reinterpret_cast<const bf16x8*>(Pbase + p * pstride + g.a_pl) + ia;
+            pb[p] = reinterpret_cast<const bf16x8*>(Pbase + p * pstride + g.b_pl) + jb;
         }
         bf16x8 ra[NP][2], rb[NP][2];
         auto fetch = [&](int k0) {
@@ -1590,7 +1594,13 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_bf16p_kernel(const GemmJob* 
             for (int r = 0; r < 4; ++r)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    if constexpr (NSPLIT == 3) {
+                    // small terms first
+                    if constexpr (NSPLIT == 6) {
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[2][c], av[0][r], acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[2][r], acc[r][c], 0, 0, 0);
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[1][c], av[1][r], acc[r][c], 0, 0, 0);
+                    }
+                    if constexpr (NSPLIT >= 3) {
                         acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[1][c], av[0][r], acc[r][c], 0, 0, 0);
                         acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[1][r], acc[r][c], 0, 0, 0);
                     }
@@ -2362,9 +2372,10 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 }
 template <typename T>
 void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
-                __bf16* phi, __bf16* plo, hipStream_t s) {
+                __bf16* planes, int64_t pstride, int nplanes, hipStream_t s) {
     if (nblocks > 0)
-        blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters, fallbacks, phi, plo);
+        blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters, fallbacks, planes, pstride,
+                                                   nplanes);
 }
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s) {
@@ -2411,11 +2422,12 @@ void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int max_grid,
 }
 
 void launch_gemm_bf16_planes(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit,
-                             const __bf16* hi, const __bf16* lo, float* U, float* L, hipStream_t s) {
+                             const __bf16* planes, int64_t pstride, float* U, float* L, hipStream_t s) {
     if (nblocks <= 0) return;
     const int grid = max_grid > 0 && max_grid < nblocks ? max_grid : nblocks;
-    if (nsplit == 3) gemm_bf16p_kernel<3><<<grid, kThreads, 0, s>>>(jobs, njobs, hi, lo, U, L, nblocks);
-    else gemm_bf16p_kernel<1><<<grid, kThreads, 0, s>>>(jobs, njobs, hi, lo, U, L, nblocks);
+    if (nsplit == 6) gemm_bf16p_kernel<6><<<grid, kThreads, 0, s>>>(jobs, njobs, planes, pstride, U, L, nblocks);
+    else if (nsplit == 3) gemm_bf16p_kernel<3><<<grid, kThreads, 0, s>>>(jobs, njobs, planes, pstride, U, L, nblocks);
+    else gemm_bf16p_kernel<1><<<grid, kThreads, 0, s>>>(jobs, njobs, planes, pstride, U, L, nblocks);
 }
 void launch_permute_in(const double* b, const int* perm, double* xp, int n, hipStream_t s) {
     permute_in_kernel<<<blocks_for(n), kThreads, 0, s>>>(b, perm, xp, n);
@@ -2517,7 +2529,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
     template void launch_extend_add<T>(const EaItem*, int, const EaContrib*, const int*, T*, T*, hipStream_t); \
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, int*, int*, int*, hipStream_t); \
-    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, __bf16*, __bf16*, hipStream_t); \
+    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, __bf16*, int64_t, int, hipStream_t); \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmJob*, int, int, int, int, T*, int64_t, T*, hipStream_t);                               \

@@ -49,8 +49,8 @@ struct mdgpu_numeric {
     mdgpu::EaItem* dEa = nullptr;
     mdgpu::EaContrib* dEaContrib = nullptr;
     mdgpu::GemmJob* dGemm = nullptr;
-    // bf16 split planes (hi, then lo at + plane_stride; Plan::planes entries
-    // each, PlanOptions::presplit; else unallocated)
+    // bf16 split planes (PlanOptions::nplanes of Plan::planes entries each,
+    // plane p at + p * plane_stride; PlanOptions::presplit, else unallocated)
     __bf16* dPlanes = nullptr;
     int64_t plane_stride = 0;
     mdgpu::BlkJob* dBlk = nullptr;
@@ -287,7 +287,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 break;
             case mdgpu::STEP_BLK:
                 mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt,
-                                     N->dInfo + 1, N->dPlanes, N->dPlanes ? N->dPlanes + N->plane_stride : nullptr, s);
+                                     N->dInfo + 1, N->dPlanes, N->plane_stride, N->dPlanes ? N->po.nplanes : 0, s);
                 break;
             case mdgpu::STEP_GEMM: {
                 // side-stream launches run on a bounded grid (look-ahead only)
@@ -295,8 +295,9 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 if constexpr (std::is_same<T, float>::value) {
                     const int nsplit = N->precision == MDGPU_BF16X3 ? 3 : 1;
                     if (mdgpu::gemm_planes(st.arg)) {
-                        mdgpu::launch_gemm_bf16_planes(N->dGemm + st.first, st.count, st.nblocks, cap, nsplit,
-                                                       N->dPlanes, N->dPlanes + N->plane_stride, U, L, s);
+                        mdgpu::launch_gemm_bf16_planes(N->dGemm + st.first, st.count, st.nblocks, cap,
+                                                       mdgpu::gemm_six(st.arg) ? 6 : nsplit, N->dPlanes,
+                                                       N->plane_stride, U, L, s);
                         break;
                     }
                     if ((N->precision == MDGPU_BF16 || N->precision == MDGPU_BF16X3) && st.arg == 128) {
@@ -503,7 +504,7 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
         (st = upload(N, P.ea, (void**)&N->dEa)) ||
         (st = upload(N, P.ea_contrib, (void**)&N->dEaContrib)) ||
         (st = upload(N, P.gemm, (void**)&N->dGemm)) ||
-        (P.planes > 0 && (st = alloc(N, 2 * (size_t)P.planes * sizeof(__bf16), (void**)&N->dPlanes))) ||
+        (P.planes > 0 && (st = alloc(N, po.nplanes * (size_t)P.planes * sizeof(__bf16), (void**)&N->dPlanes))) ||
         (st = upload(N, P.blk, (void**)&N->dBlk)) ||
         (st = upload(N, std::vector<int>(P.blk.size(), 0), (void**)&N->dBlkCnt)) ||
         (st = upload(N, P.panel, (void**)&N->dPanel)) ||

@@ -536,7 +536,8 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                 sb.wait_ev = inner_done;
                 sb.rec_ev = side_last = P.nevents++;
             }
-            if (flush_gemm(batch_u, L, o.schur_kernel ? o.schur_kernel : gk)) {
+            const int uk = o.schur_kernel ? o.schur_kernel : (o.presplit && o.nplanes == 3) ? (gk | kGemmSix) : gk;
+            if (flush_gemm(batch_u, L, uk)) {
                 Step& su = P.steps.back();
                 su.stream = 2;
                 su.wait_ev = inner_done;
@@ -582,7 +583,8 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             const bool split = o.split_k_min > 0 && kmax >= o.split_k_min;
             flush_gemm(batch, L,
                        o.schur_kernel ? o.schur_kernel
-                                      : o.gemm_default | (split ? kGemmSplit : 0) | (planes ? kGemmPlanes : 0));
+                                      : o.gemm_default | (split ? kGemmSplit : 0) | (planes ? kGemmPlanes : 0) |
+                                            (planes && o.nplanes == 3 ? kGemmSix : 0));
         }
         add_ea(L, false);
     }
@@ -899,6 +901,7 @@ std::string validate_plan(const Plan& P, const PlanOptions& o) {
             if (!gemm_kernel_valid(s.arg, o.elem_bytes) || gemm_tile_rows(s.arg) > o.tile)
                 return bad("gemm kernel or tile size");
             if (gemm_planes(s.arg) && !o.presplit) return bad("split-plane GEMM without planes");
+            if (gemm_six(s.arg) && o.nplanes != 3) return bad("six-product GEMM without three planes");
             for (int t = s.first; gemm_planes(s.arg) && t < s.first + s.count; t++) {
                 // rows past M are read clamped to row M - 1; octets past K
                 // are not read

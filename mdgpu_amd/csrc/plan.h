@@ -256,9 +256,12 @@ constexpr int kGemmSplit = 256;
 // bf16 variants: operands from the split planes (128-row tiles); the other
 // bf16 launches convert FP32 tiles as they stage them
 constexpr int kGemmPlanes = 512;
-inline int gemm_kernel_base(int kernel) { return kernel & ~(kGemmSplit | kGemmPlanes); }
+// ... with six products from three planes (hi, mid, lo): FP32-level products
+constexpr int kGemmSix = 1024;
+inline int gemm_kernel_base(int kernel) { return kernel & ~(kGemmSplit | kGemmPlanes | kGemmSix); }
 inline bool gemm_split(int kernel) { return (kernel & kGemmSplit) != 0; }
 inline bool gemm_planes(int kernel) { return (kernel & kGemmPlanes) != 0; }
+inline bool gemm_six(int kernel) { return (kernel & kGemmSix) != 0; }
 inline int gemm_tile_rows(int kernel) {
     const int b = gemm_kernel_base(kernel);
     return b == 64 || b == kGemmMfma32x64 ? 64 : 128;
@@ -267,6 +270,7 @@ inline bool gemm_kernel_valid(int kernel, int elem_bytes) {
     const int b = gemm_kernel_base(kernel);
     if (gemm_split(kernel) && elem_bytes != 4) return false;
     if (gemm_planes(kernel) && (elem_bytes != 4 || b != 128 || gemm_split(kernel))) return false;
+    if (gemm_six(kernel) && !gemm_planes(kernel)) return false;
     return b == 64 || b == 128 || ((b == kGemmMfma32 || b == kGemmMfma32x64) && elem_bytes == 4);
 }
 
@@ -297,6 +301,9 @@ struct PlanOptions {
     // bf16 variants: the block steps also write their columns as bf16 split
     // planes, and the GEMMs of the blocked fronts read them (kGemmPlanes)
     bool presplit = false;
+    // split planes written by the block steps: 2 (hi, lo) or 3 (hi, mid, lo;
+    // the update-matrix launches then make six products, kGemmSix)
+    int nplanes = 2;
     // GEMM kernel code of the update-matrix launches ((u) and the Schur
     // launch) when nonzero, else the same kernel as the other GEMMs
     int schur_kernel = 0;

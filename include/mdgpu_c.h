@@ -264,8 +264,11 @@ int mdgpu_solve_nrhs(mdgpu_numeric* num, const double* d_B, double* d_X, int nrh
 /* Iterative refinement in FP64 until the backward error
  * ||b - A x||_inf / (||A||_1 ||x||_inf + ||b||_inf) (mdgpu.cu:1972) is at
  * most tol, or maxit corrections were applied.  The residual uses the
- * values last passed to mdgpu_factorize.  Returns MDGPU_ERR_NOT_CONVERGED
- * (with eta/iters filled) if tol was not reached. */
+ * values last passed to mdgpu_factorize.  Classical refinement for FP64,
+ * FP32 and BF16X3 factors; conjugate gradients preconditioned by the factor
+ * for MDGPU_BF16 (environment MDGPU_REFINE=cg or =ir overrides either).
+ * Returns MDGPU_ERR_NOT_CONVERGED (with eta/iters filled) if tol was not
+ * reached. */
 int mdgpu_refine(mdgpu_numeric* num, const double* d_b, double* d_x, double tol, int maxit,
                  double* eta, int* iters, void* stream);
 

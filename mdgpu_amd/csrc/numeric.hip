@@ -79,7 +79,7 @@ struct mdgpu_numeric {
     double* dP = nullptr;                 // CG refinement: search direction, A p
     double* dQ2 = nullptr;
     double* dScal = nullptr;              // CG scalars {rz, rz_old, pq} + dot partials
-    bool refine_cg = false;               // CG refinement (default: classical refinement)
+    bool refine_cg = false;               // CG refinement instead of classical refinement
     double* dRowabs = nullptr;
     int64_t* dDiagOff = nullptr;
     int* dInfo = nullptr;
@@ -557,10 +557,13 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     // polls before a chained-solve wait gives up (a smaller value makes the
     // give-up path observable without a stalled producer)
     if (const char* e = std::getenv("MDGPU_CHAIN_SPIN")) N->chain_spin = std::max(0, std::atoi(e));
-    // refinement method: classical iterative refinement (default) or
-    // conjugate gradients preconditioned by the factor ("cg"; fewer solves
-    // when the factor is poor, e.g. ill-conditioned matrices in FP32 or
-    // bf16; on the 64^3 stencil both need the same three solves)
+    // refinement method: classical iterative refinement or conjugate
+    // gradients preconditioned by the factor ("cg"; fewer solves when the
+    // factor is poor).  Single bf16 factors default to CG (on the 64^3
+    // 27-point stencil it reaches 1e-12 after 5 solves, classical after 6);
+    // FP32 and BF16X3 factors need the same solves either way and default
+    // to the classical steps, which cost one SpMV and two dots less each.
+    N->refine_cg = precision == MDGPU_BF16;
     if (const char* e = std::getenv("MDGPU_REFINE")) N->refine_cg = std::string(e) == "cg";
     N->events.assign(P.nevents, nullptr);
     for (hipEvent_t& e : N->events)

@@ -39,7 +39,10 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
 // [L, L + nL + 64))
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int kernel, T* L, int64_t nL, T* U,
-                 hipStream_t s);
+                 hipStream_t s, int lds_min = 0);
+// LDS bytes of one FP32 block-step workgroup (blk_kernel<float>), 0 if the
+// query fails
+int blk_lds_bytes_f32();
 // same jobs (128-row tiles), FP32 data, bf16 MFMA with FP32 accumulation;
 // nsplit 1: one bf16 product, 3: split hi/lo products (hi*hi + hi*lo + lo*hi)
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit, float* L, float* U,

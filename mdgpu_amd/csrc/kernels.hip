@@ -2383,35 +2383,54 @@ void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T*
 }
 template <typename T>
 void launch_gemm(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int kernel, T* L, int64_t nL, T* U,
-                 hipStream_t s) {
+                 hipStream_t s, int lds_min) {
     if (nblocks <= 0) return;
     // a grid of at most max_grid workgroups (a multiple of the 8 XCDs)
     // loops over the nblocks tiles
     const int grid = max_grid > 0 && max_grid < nblocks ? max_grid : nblocks;
     const bool rows128 = gemm_tile_rows(kernel) == 128;
+    // lds_min > 0: dynamic LDS that brings a workgroup's LDS to lds_min bytes
+    // (fewer workgroups per CU)
+    auto pad = [lds_min](const void* fn) -> size_t {
+        if (lds_min <= 0) return 0;
+        hipFuncAttributes a{};
+        if (hipFuncGetAttributes(&a, fn) != hipSuccess) return 0;
+        return (size_t)lds_min > a.sharedSizeBytes ? (size_t)lds_min - a.sharedSizeBytes : 0;
+    };
     if constexpr (std::is_same<T, float>::value) {
         // two-level accumulation on 128-row tiles always runs the 32x32x2
         // kernel (same tiles; the 16x16x4 kernel's second accumulator set
         // spills)
         const bool split = gemm_split(kernel);
         const int base = gemm_kernel_base(kernel);
+        auto mfma32 = [&](auto fn) {
+            fn<<<grid, kThreads, pad((const void*)fn), s>>>(jobs, njobs, L, U, nL, nblocks);
+        };
         if (base == kGemmMfma32 || (split && rows128)) {
-            if (split) gemm_mfma32_kernel<128, true><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
-            else gemm_mfma32_kernel<128, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
+            if (split) mfma32(gemm_mfma32_kernel<128, true>);
+            else mfma32(gemm_mfma32_kernel<128, false>);
             return;
         }
         if (base == kGemmMfma32x64) {
-            if (split) gemm_mfma32_kernel<64, true><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
-            else gemm_mfma32_kernel<64, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nL, nblocks);
-            return;
-        }
-        if (split) {
-            gemm_mfma_kernel<T, 64, true><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
+            if (split) mfma32(gemm_mfma32_kernel<64, true>);
+            else mfma32(gemm_mfma32_kernel<64, false>);
             return;
         }
     }
-    if (rows128) gemm_mfma_kernel<T, 128, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
-    else gemm_mfma_kernel<T, 64, false><<<grid, kThreads, 0, s>>>(jobs, njobs, L, U, nblocks);
+    auto mfma = [&](auto fn) { fn<<<grid, kThreads, pad((const void*)fn), s>>>(jobs, njobs, L, U, nblocks); };
+    if constexpr (std::is_same<T, float>::value) {
+        if (gemm_split(kernel)) {
+            mfma(gemm_mfma_kernel<T, 64, true>);
+            return;
+        }
+    }
+    if (rows128) mfma(gemm_mfma_kernel<T, 128, false>);
+    else mfma(gemm_mfma_kernel<T, 64, false>);
+}
+int blk_lds_bytes_f32() {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, (const void*)blk_kernel<float>) != hipSuccess) return 0;
+    return (int)a.sharedSizeBytes;
 }
 void launch_gemm_bf16(const GemmJob* jobs, int njobs, int nblocks, int max_grid, int nsplit, float* L, float* U,
                       hipStream_t s) {
@@ -2532,7 +2551,7 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, __bf16*, int64_t, int, hipStream_t); \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
-    template void launch_gemm<T>(const GemmJob*, int, int, int, int, T*, int64_t, T*, hipStream_t);                               \
+    template void launch_gemm<T>(const GemmJob*, int, int, int, int, T*, int64_t, T*, hipStream_t, int);                               \
     template void launch_fwd<T>(const int*, int, const FrontDesc*, const int*, const int*, const int*,         \
                                 const T*, const T*, double*, double*, double*, const int64_t*, int,            \
                                 hipStream_t);                                                                  \

@@ -105,6 +105,7 @@ struct mdgpu_numeric {
     // workgroups of a side-stream GEMM launch (0: one per tile): a bounded
     // grid loops over its tiles and leaves CUs to the main stream's chain
     int side_grid = 0;
+    int side_lds = 0;                     // LDS floor of side-stream GEMM workgroups (0: none)
     std::vector<hipEvent_t> events;       // look-ahead events of the plan
     hipGraphExec_t g_factor = nullptr;
     mdgpu::PlanOptions po;                // options the plan was built with
@@ -305,7 +306,8 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                         break;
                     }
                 }
-                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, cap, st.arg, L, P.lstore, U, s);
+                mdgpu::launch_gemm<T>(N->dGemm + st.first, st.count, st.nblocks, cap, st.arg, L, P.lstore, U, s,
+                                      (!ev && st.stream != 0) ? N->side_lds : 0);
                 break;
             }
             case mdgpu::STEP_INV:
@@ -486,6 +488,20 @@ extern "C" int mdgpu_numeric_create(const mdgpu_symbolic* sym, int precision, md
     }
     if (const char* g = std::getenv("MDGPU_GRAPH")) N->use_graph = std::atoi(g) != 0;
     if (const char* g = std::getenv("MDGPU_SIDE_GRID")) N->side_grid = std::max(0, std::atoi(g)) / 8 * 8;
+    // FP32 side-stream GEMM workgroups get an LDS floor just above a third
+    // of the CU's LDS: at most two share a CU, and a block-step workgroup of
+    // the main stream always fits beside them instead of waiting for a GEMM
+    // tile to finish (both GEMM kernels' registers leave room for it)
+    if (precision == MDGPU_FP32) {
+        int cu_lds = 0;
+        if (hipDeviceGetAttribute(&cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, N->device) ==
+                hipSuccess && cu_lds > 0) {
+            const int floor_lds = (cu_lds / 3 + 256) / 256 * 256;
+            const int blk = mdgpu::blk_lds_bytes_f32();
+            if (blk > 0 && 2 * floor_lds + blk <= cu_lds) N->side_lds = floor_lds;
+        }
+    }
+    if (const char* g = std::getenv("MDGPU_SIDE_LDS")) N->side_lds = std::max(0, std::atoi(g));
     // 64 elements past the L store: the FP32 GEMM's 16-byte buffer loads
     // are bounded by the store's end at their first element
     if ((st = alloc(N, (P.lstore + 64) * es, &N->dL)) ||

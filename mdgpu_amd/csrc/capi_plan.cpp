@@ -34,6 +34,8 @@ PlanOptions plan_options_for(int precision) {
     if (const char* e = std::getenv("MDGPU_U_STRIDE")) po.u_stride = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("MDGPU_U_FLUSH")) po.u_flush = std::atoi(e) != 0;
     if (const char* e = std::getenv("MDGPU_SCHUR_KERNEL")) po.schur_kernel = std::atoi(e);
+    // most pivots of a front factored in LDS by one workgroup (at most 128)
+    if (const char* e = std::getenv("MDGPU_PANEL_MAXC")) po.panel_max_pivots = std::max(0, std::atoi(e));
     // outer block width of the blocked panel path (multiple of 64)
     if (const char* e = std::getenv("MDGPU_NB_OUTER")) {
         const int v = std::atoi(e);

@@ -389,7 +389,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             for (int f : fl) {
                 const FrontDesc& d = P.fronts[f];
                 int64_t bytes = (int64_t)d.n * d.c * o.elem_bytes;
-                if (bytes <= o.panel_lds_bytes && d.c <= kPanelMaxPivots) {
+                if (bytes <= o.panel_lds_bytes && d.c <= std::min(o.panel_max_pivots, kPanelMaxPivots)) {
                     P.panel.push_back(f);
                     lds = std::max<int>(lds, (int)bytes);
                 } else {

@@ -277,6 +277,11 @@ inline bool gemm_kernel_valid(int kernel, int elem_bytes) {
 struct PlanOptions {
     int elem_bytes = 8;          // 8: FP64 store, 4: FP32 store
     int panel_lds_bytes = 65536; // LDS budget of the one-workgroup panel kernel
+    // most pivots of a front factored wholly in LDS: the panel kernel's
+    // column-by-column loop beats the blocked path only on narrow fronts
+    // (measured with 16, 32, 64 and 128 as the bound: 16 is fastest on
+    // every workload that has such fronts)
+    int panel_max_pivots = 16;
     int nb = 64;                 // block width of the blocked panel path
     int nb_outer = 256;          // outer block width (trailing updates with K = nb_outer)
     int tile = 64;               // largest GEMM tile (rows = cols)

@@ -146,6 +146,9 @@ static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int us
     po.nb_outer = nb_outer < 0 ? -nb_outer : nb_outer;
     po.u_lookahead = nb_outer > 0;
     po.u_stride = u_stride;
+    // outer width 128 also sends fronts of up to 128 pivots to the LDS panel
+    // kernel (its upper bound; the default keeps it to the narrow fronts)
+    if (nb_outer == 128) po.panel_max_pivots = kPanelMaxPivots;
     po.tile = 64;
     if (presplit) {   // the bf16 variants' options (FP32 store, 128-row tiles)
         po.elem_bytes = 4;

@@ -282,7 +282,7 @@ def test_indefinite_matrix_is_reported(device, prec):
 
 
 def test_pivot_growth_fallback_lds_front(device):
-    """A front factored wholly in LDS (one 40 x 40 dense supernode, 6.4 KB in
+    """A front factored wholly in LDS (one 16 x 16 dense supernode, 1 KB in
     FP32) with heavy cancellation (ones + 1e-4 I: every pivot after the first
     is ~1e-4 of its assembled diagonal, below the 2^-12 threshold) fails the
     FP32 pivot-growth check and is refactored in FP64 by the fallback launch.
@@ -290,7 +290,7 @@ def test_pivot_growth_fallback_lds_front(device):
     refinement reaches 1e-12; the FP64 path never falls back."""
     import torch
     from oracle import oracle as O
-    n = 40
+    n = 16
     A = np.ones((n, n)) + 1e-4 * np.eye(n)
     p, i, x = _dense_lower(A)
     S = symbolic.analyze(p, i, False, "sn", 0)

@@ -1131,23 +1131,28 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
 //    consecutive rows of one column per load, and loads past the store's end
 //    return zero without a bounds test.  Rows past M (or N) are loaded as
 //    they are: they only reach C rows (columns) that are not stored.
-//    Columns past K are zeroed in the last k-tile, on both operands (0 * NaN
-//    would not vanish).
-//  * LDS holds a k-tile of 16 as 8 column pairs, [pair][row][2]: a thread
-//    loads two columns' 4 rows and stores them interleaved as two 16-byte
-//    stores (BT = 128: every thread stages both operands; BT = 64: waves 0-1
-//    stage A, waves 2-3 B).  The MFMA's contraction index is relabelled so
-//    that lane half h reads column pair 2 s + h and sub-step q takes column q
-//    of the pair.  A-side reads take rows 32 rb + j (8 bytes: both columns of
-//    the pair), so that a wavefront's accumulators map to 32 consecutive C
-//    rows per column; B-side reads (BT = 128) take rows 2 j and 2 j + 1 in
-//    one 16-byte read, its two 32-column blocks interleaved.
+//  * Every full k-tile (16 columns) goes from global memory straight into
+//    LDS (buffer_load_dwordx4 ... lds): one wavefront load fills 1 KB of LDS
+//    in lane order, so a k-tile is laid out [k][row], BT values per column,
+//    each lane taking 4 rows of one column.  No registers hold the staged
+//    tile and no LDS store instructions run.  The next k-tile's loads go
+//    into the other LDS buffer while the current one is multiplied; a
+//    vmcnt wait and one barrier per k-tile make it visible.
+//  * A partial last k-tile is staged through registers instead, its columns
+//    past K zeroed on both operands (0 * NaN would not vanish).
+//  * The MFMA's contraction index is relabelled: for sub-step (s, q) lane
+//    half h takes column k = 4 s + 2 h + q of the k-tile.  A-side reads take
+//    rows 32 rb + j (two 4-byte reads), so that a wavefront's accumulators
+//    map to 32 consecutive C rows per column; B-side reads (BT = 128) take
+//    rows 2 j and 2 j + 1 in one 8-byte read, its two 32-column blocks
+//    interleaved.
 //  * Each wavefront's LDS reads of a k-tile are issued before its products.
 // Same k order per entry as a sequential sum over k-pairs; the rounding
 // differs from gemm_mfma_kernel's (pairs of k per instruction).
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
 // raw buffer resource (stride 0) over [base, base + bytes_left), clamped to
 // the 32-bit record count; a load outside returns zero
@@ -1158,6 +1163,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, 
 __device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
     return __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
 }
+// s_waitcnt immediate that waits for all vector-memory operations (vmcnt 0)
+// and leaves the LDS/scalar and export counters alone
+constexpr int kWaitVmcnt0 = (0xF << 8) | (0x7 << 4);
 
 // SPLIT: two-level accumulation over runs of kSplitTiles k-tiles, as in
 // gemm_mfma_kernel; at BT = 128 its second accumulator set gets a
@@ -1167,10 +1175,13 @@ template <int BT, bool SPLIT>
 __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_mfma32_kernel(
     const GemmJob* __restrict__ jobs, int njobs, float* __restrict__ L, float* __restrict__ U, int64_t nL, int ntiles) {
     static_assert(BT == 128 || BT == 64, "128- or 64-row tiles");
-    constexpr int KT = 16, NP = KT / 2, NB = BT / 64;   // NB x NB 32x32 blocks per wavefront
-    constexpr bool kBoth = BT == 128;                   // every thread stages both operands
-    __shared__ __align__(16) float As[2][NP * BT * 2];
-    __shared__ __align__(16) float Bs[2][NP * BT * 2];
+    constexpr int KT = 16, NB = BT / 64;                // NB x NB 32x32 blocks per wavefront
+    constexpr int kLanesPerCol = BT / 4;                // lanes loading one column (4 rows each)
+    constexpr int kColsPerLoad = 64 / kLanesPerCol;     // columns of one wavefront's 1 KB load
+    constexpr int kLoads = KT / (4 * kColsPerLoad);     // loads per operand and wavefront per k-tile
+    static_assert(kLoads >= 1 && kLoads * 4 * kColsPerLoad == KT, "a k-tile is whole wavefront loads");
+    __shared__ __align__(16) float As[2][KT * BT];
+    __shared__ __align__(16) float Bs[2][KT * BT];
     // persistent over the launch's ntiles tiles when the grid is smaller
     // (side-stream launches leave CUs to the main stream's launches)
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -1197,43 +1208,40 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
         const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
         const int wr = (wave >> 1) * (BT / 2), wc = (wave & 1) * (BT / 2);
         const int j = lane & 31, h = lane >> 5;
-        // staging: rows sr..sr+3 of columns 2 sp and 2 sp + 1 of the k-tile, of
-        // A and B (kBoth) or of operand `opb` (0: A, 1: B)
-        const int st = kBoth ? tid : (tid & 127), opb = kBoth ? 0 : (tid >> 7);
-        const int sr = 4 * (st % (BT / 4)), sp = st / (BT / 4);
         const int64_t a0 = g.a_off + row0, b0 = g.b_off + col0;
-        const uint32_t va0 = (uint32_t)(sr + 2 * sp * g.lda) * 4u, va1 = va0 + 4u * (uint32_t)g.lda;
-        const uint32_t vb0 = (uint32_t)(sr + 2 * sp * g.ldb) * 4u, vb1 = vb0 + 4u * (uint32_t)g.ldb;
-        f32x4_t fa0, fa1, fb0, fb1;
-        auto fetch = [&](int k0) {
-            const int64_t oa = a0 + (int64_t)k0 * g.lda, ob = b0 + (int64_t)k0 * g.ldb;
-            if (kBoth || opb == 0) {
-                const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4);
-                fa0 = buffer_load4(ra, va0);
-                fa1 = buffer_load4(ra, va1);
-            }
-            if (kBoth || opb == 1) {
-                const auto rb = buffer_rsrc(L + ob, (nL - ob) * 4);
-                fb0 = buffer_load4(rb, vb0);
-                fb1 = buffer_load4(rb, vb1);
+        // this lane's part of load i of its wavefront: rows lr .. lr + 3 of
+        // column (kLoads wave + i) kColsPerLoad + lc of the k-tile
+        const int lr = 4 * (lane % kLanesPerCol), lc = lane / kLanesPerCol;
+        const int nfull = g.K / KT, nt = (g.K + KT - 1) / KT;
+        auto col_of = [&](int i) { return (kLoads * wave + i) * kColsPerLoad; };
+        // full k-tile t into LDS buffer buf
+        auto issue = [&](int buf, int t) {
+            const int64_t oa = a0 + (int64_t)t * KT * g.lda, ob = b0 + (int64_t)t * KT * g.ldb;
+            const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4);
+            const auto rb = buffer_rsrc(L + ob, (nL - ob) * 4);
+#pragma unroll
+            for (int i = 0; i < kLoads; ++i) {
+                const int c0 = col_of(i), c = c0 + lc;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void_ptr)&As[buf][c0 * BT], 16,
+                                                         (uint32_t)(lr + c * g.lda) * 4u, 0, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void_ptr)&Bs[buf][c0 * BT], 16,
+                                                         (uint32_t)(lr + c * g.ldb) * 4u, 0, 0, 0);
             }
         };
-        auto store = [&](int buf, bool tail, int k0) {
-            if (tail) {
-                const int kk = k0 + 2 * sp;
-                const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
-                if (kk >= g.K) { fa0 = z; fb0 = z; }
-                if (kk + 1 >= g.K) { fa1 = z; fb1 = z; }
-            }
-            if (kBoth || opb == 0) {
-                float* pa = &As[buf][(sp * BT + sr) * 2];
-                *reinterpret_cast<f32x4_t*>(pa) = f32x4_t{fa0[0], fa1[0], fa0[1], fa1[1]};
-                *reinterpret_cast<f32x4_t*>(pa + 4) = f32x4_t{fa0[2], fa1[2], fa0[3], fa1[3]};
-            }
-            if (kBoth || opb == 1) {
-                float* pb = &Bs[buf][(sp * BT + sr) * 2];
-                *reinterpret_cast<f32x4_t*>(pb) = f32x4_t{fb0[0], fb1[0], fb0[1], fb1[1]};
-                *reinterpret_cast<f32x4_t*>(pb + 4) = f32x4_t{fb0[2], fb1[2], fb0[3], fb1[3]};
+        // partial k-tile t through registers, columns past K zeroed
+        auto stage_tail = [&](int buf, int t) {
+            const int64_t oa = a0 + (int64_t)t * KT * g.lda, ob = b0 + (int64_t)t * KT * g.ldb;
+            const auto ra = buffer_rsrc(L + oa, (nL - oa) * 4);
+            const auto rb = buffer_rsrc(L + ob, (nL - ob) * 4);
+            const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int i = 0; i < kLoads; ++i) {
+                const int c = col_of(i) + lc;
+                f32x4_t va = buffer_load4(ra, (uint32_t)(lr + c * g.lda) * 4u);
+                f32x4_t vb = buffer_load4(rb, (uint32_t)(lr + c * g.ldb) * 4u);
+                if (t * KT + c >= g.K) { va = z; vb = z; }
+                *reinterpret_cast<f32x4_t*>(&As[buf][c * BT + lr]) = va;
+                *reinterpret_cast<f32x4_t*>(&Bs[buf][c * BT + lr]) = vb;
             }
         };
         f32x16_t acc[NB][NB], part[NB][NB];   // part: SPLIT, this run's products
@@ -1244,47 +1252,49 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
     #pragma unroll
                 for (int v = 0; v < 16; ++v) acc[a][b][v] = part[a][b][v] = 0.f;
         auto& sum = SPLIT ? part : acc;
-        fetch(0);
-        store(0, KT > g.K, 0);
-        if (KT < g.K) fetch(KT);
-        __syncthreads();
-        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
-            // tile k0 + KT (in registers) to the other buffer, whose last readers
-            // passed the barrier that ended the previous iteration
-            if (k0 + KT < g.K) {
-                store(cur ^ 1, k0 + 2 * KT > g.K, k0 + KT);
-                if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
-            }
+        if (nfull > 0) issue(0, 0);
+        else stage_tail(0, 0);
+        for (int t = 0; t < nt; ++t) {
+            const int cur = t & 1;
+            // k-tile t's loads are the only ones outstanding: wait for them,
+            // then the barrier makes every wavefront's part visible (and ends
+            // the reads of the other buffer, k-tile t - 1)
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+            __syncthreads();
+            if (t + 1 < nfull) issue(cur ^ 1, t + 1);
+            else if (t + 1 < nt) stage_tail(cur ^ 1, t + 1);
             const float* Ac = As[cur];
             const float* Bc = Bs[cur];
-            // av[s][2 rb + q] = A(wr + 32 rb + j, column q of pair 2 s + h),
-            // bv[s][2 cb + q] = B(wc + NB j + cb, same column)
-            f32x4_t av[NP / 2], bv[NP / 2];
+            // av[s][q][rb] = A(wr + 32 rb + j, k), bv[s][q][cb] = B(wc + NB j + cb, k),
+            // k = 4 s + 2 h + q
+            float av[4][2][NB];
+            float bv[4][2][NB];
     #pragma unroll
-            for (int s2 = 0; s2 < NP / 2; ++s2) {
-                const int pr = 2 * s2 + h;
-                const f32x2_t x0 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + j) * 2]);
-                if constexpr (NB == 2) {
-                    const f32x2_t x1 = *reinterpret_cast<const f32x2_t*>(&Ac[(pr * BT + wr + 32 + j) * 2]);
-                    av[s2] = f32x4_t{x0[0], x0[1], x1[0], x1[1]};
-                    bv[s2] = *reinterpret_cast<const f32x4_t*>(&Bc[(pr * BT + wc + 2 * j) * 2]);
-                } else {
-                    const f32x2_t y0 = *reinterpret_cast<const f32x2_t*>(&Bc[(pr * BT + wc + j) * 2]);
-                    av[s2] = f32x4_t{x0[0], x0[1], 0.f, 0.f};
-                    bv[s2] = f32x4_t{y0[0], y0[1], 0.f, 0.f};
+            for (int s2 = 0; s2 < 4; ++s2)
+    #pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int k = 4 * s2 + 2 * h + q;
+    #pragma unroll
+                    for (int rb = 0; rb < NB; ++rb) av[s2][q][rb] = Ac[k * BT + wr + 32 * rb + j];
+                    if constexpr (NB == 2) {
+                        const f32x2_t y = *reinterpret_cast<const f32x2_t*>(&Bc[k * BT + wc + 2 * j]);
+                        bv[s2][q][0] = y[0];
+                        bv[s2][q][1] = y[1];
+                    } else {
+                        bv[s2][q][0] = Bc[k * BT + wc + j];
+                    }
                 }
-            }
     #pragma unroll
-            for (int s2 = 0; s2 < NP / 2; ++s2)
+            for (int s2 = 0; s2 < 4; ++s2)
     #pragma unroll
                 for (int q = 0; q < 2; ++q)
     #pragma unroll
                     for (int rb = 0; rb < NB; ++rb)
     #pragma unroll
                         for (int cb = 0; cb < NB; ++cb)
-                            sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][2 * cb + q], av[s2][2 * rb + q],
+                            sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][q][cb], av[s2][q][rb],
                                                                                sum[rb][cb], 0, 0, 0);
-            if (SPLIT && ((k0 / KT) % kSplitTiles == kSplitTiles - 1 || k0 + KT >= g.K)) {
+            if (SPLIT && (t % kSplitTiles == kSplitTiles - 1 || t + 1 >= nt)) {
     #pragma unroll
                 for (int rb = 0; rb < NB; ++rb)
     #pragma unroll
@@ -1294,8 +1304,9 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
                         for (int v = 0; v < 16; ++v) part[rb][cb][v] = 0.f;
                     }
             }
-            __syncthreads();
         }
+        // the next tile's first loads reuse the buffers
+        __syncthreads();
         // acc[rb][cb][v] = C(row0 + wr + 32 rb + j, col0 + wc + NB (8 (v / 4) + 4 h + v % 4) + cb);
         // C -= acc (beta0: C = -acc), all old values of a column block loaded
         // first at clamped in-range addresses

@@ -1167,7 +1167,13 @@ __device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32
 // and leaves the LDS/scalar and export counters alone
 constexpr int kWaitVmcnt0 = (0xF << 8) | (0x7 << 4);
 
-// SPLIT: two-level accumulation over runs of kSplitTiles k-tiles, as in
+#ifndef MDGPU_KT64
+#define MDGPU_KT64 32
+#endif
+#ifndef MDGPU_KT128
+#define MDGPU_KT128 16
+#endif
+// SPLIT: two-level accumulation over runs of 64 k, as in
 // gemm_mfma_kernel; at BT = 128 its second accumulator set gets a
 // 3-wavefront register budget (measured faster than 2 wavefronts without
 // spills).
@@ -1175,7 +1181,8 @@ template <int BT, bool SPLIT>
 __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_mfma32_kernel(
     const GemmJob* __restrict__ jobs, int njobs, float* __restrict__ L, float* __restrict__ U, int64_t nL, int ntiles) {
     static_assert(BT == 128 || BT == 64, "128- or 64-row tiles");
-    constexpr int KT = 16, NB = BT / 64;                // NB x NB 32x32 blocks per wavefront
+    constexpr int KT = BT == 64 ? MDGPU_KT64 : MDGPU_KT128, NB = BT / 64;   // NB x NB 32x32 blocks per wavefront
+    constexpr int kRun = 64 / KT;                       // k-tiles per two-level accumulation run
     constexpr int kLanesPerCol = BT / 4;                // lanes loading one column (4 rows each)
     constexpr int kColsPerLoad = 64 / kLanesPerCol;     // columns of one wavefront's 1 KB load
     constexpr int kLoads = KT / (4 * kColsPerLoad);     // loads per operand and wavefront per k-tile
@@ -1267,10 +1274,10 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
             const float* Bc = Bs[cur];
             // av[s][q][rb] = A(wr + 32 rb + j, k), bv[s][q][cb] = B(wc + NB j + cb, k),
             // k = 4 s + 2 h + q
-            float av[4][2][NB];
-            float bv[4][2][NB];
+            float av[KT / 4][2][NB];
+            float bv[KT / 4][2][NB];
     #pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2)
+            for (int s2 = 0; s2 < KT / 4; ++s2)
     #pragma unroll
                 for (int q = 0; q < 2; ++q) {
                     const int k = 4 * s2 + 2 * h + q;
@@ -1285,7 +1292,7 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
                     }
                 }
     #pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2)
+            for (int s2 = 0; s2 < KT / 4; ++s2)
     #pragma unroll
                 for (int q = 0; q < 2; ++q)
     #pragma unroll
@@ -1294,7 +1301,7 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
                         for (int cb = 0; cb < NB; ++cb)
                             sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][q][cb], av[s2][q][rb],
                                                                                sum[rb][cb], 0, 0, 0);
-            if (SPLIT && (t % kSplitTiles == kSplitTiles - 1 || t + 1 >= nt)) {
+            if (SPLIT && (t % kRun == kRun - 1 || t + 1 >= nt)) {
     #pragma unroll
                 for (int rb = 0; rb < NB; ++rb)
     #pragma unroll

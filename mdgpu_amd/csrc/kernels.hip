@@ -1532,6 +1532,36 @@ __global__ __launch_bounds__(kThreads, NSPLIT == 6 ? 1 : 2) void gemm_bf16p_kern
         const int li = lane & 15, lg = lane >> 4;
         const int r_st = tid & (BT - 1), o_st = 2 * (tid >> 7);
         const int64_t ia = min(row0 + r_st, g.M - 1), jb = min(col0 + r_st, g.N - 1);
+        // full k-tiles go straight into LDS (buffer_load_dwordx4 ... lds):
+        // wavefront load i covers octet (2 wave + i) / 2 of the k-tile and rows
+        // 64 ((2 wave + i) % 2) + lane, 1 KB of consecutive rows (clamped to
+        // the job's last row, as the staged loads below)
+        const int nfull = g.K / KT, nt = (g.K + KT - 1) / KT;
+        int dra[2], drb[2];   // clamped row of this lane in load i
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int idx = 2 * wave + i;
+            dra[i] = min(row0 + 64 * (idx & 1) + lane, g.M - 1);
+            drb[i] = min(col0 + 64 * (idx & 1) + lane, g.N - 1);
+        }
+        auto issue = [&](int buf, int t) {
+            const int o0 = t * NO;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const int64_t ba = p * pstride + g.a_pl + (int64_t)o0 * 8 * g.lda;
+                const int64_t bb = p * pstride + g.b_pl + (int64_t)o0 * 8 * g.ldb;
+                const auto ra_ = buffer_rsrc(Pbase + ba, ((p + 1) * pstride - ba) * 2);
+                const auto rb_ = buffer_rsrc(Pbase + bb, ((p + 1) * pstride - bb) * 2);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int idx = 2 * wave + i, ol = idx >> 1, rbase = 64 * (idx & 1);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(ra_, (lds_void_ptr)&As[buf][p][ol * BT + rbase], 16,
+                                                             (uint32_t)(ol * g.lda + dra[i]) * 16u, 0, 0, 0);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rb_, (lds_void_ptr)&Bs[buf][p][ol * BT + rbase], 16,
+                                                             (uint32_t)(ol * g.ldb + drb[i]) * 16u, 0, 0, 0);
+                }
+            }
+        };
         // 16-byte vectors: octet o of row i at base + o * ld + i
         const bf16x8* __restrict__ pa[NP];
         const bf16x8* __restrict__ pb[NP];
@@ -1540,20 +1570,10 @@ __global__ __launch_bounds__(kThreads, NSPLIT == 6 ? 1 : 2) void gemm_bf16p_kern
             pa[p] = reinterpret_cast<const bf16x8*>(Pbase + p * pstride + g.a_pl) + ia;
             pb[p] = reinterpret_cast<const bf16x8*>(Pbase + p * pstride + g.b_pl) + jb;
         }
-        bf16x8 ra[NP][2], rb[NP][2];
-        auto fetch = [&](int k0) {
+        // partial last k-tile (K not a multiple of 32) through registers,
+        // entries past K zeroed
+        auto stage_tail = [&](int buf, int k0) {
             const int o0 = k0 / 8 + o_st;
-            if (k0 + KT <= g.K) {
-#pragma unroll
-                for (int p = 0; p < NP; ++p)
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        ra[p][q] = pa[p][(int64_t)(o0 + q) * g.lda];
-                        rb[p][q] = pb[p][(int64_t)(o0 + q) * g.ldb];
-                    }
-                return;
-            }
-            // last k-tile, K not a multiple of 32
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const int kk = 8 * (o0 + q);   // first k of the octet
@@ -1570,36 +1590,26 @@ __global__ __launch_bounds__(kThreads, NSPLIT == 6 ? 1 : 2) void gemm_bf16p_kern
                                 y[e] = (__bf16)0.0f;
                             }
                     }
-                    ra[p][q] = x;
-                    rb[p][q] = y;
+                    As[buf][p][(o_st + q) * BT + r_st] = x;
+                    Bs[buf][p][(o_st + q) * BT + r_st] = y;
                 }
             }
-        };
-        auto store = [&](int buf) {
-#pragma unroll
-            for (int p = 0; p < NP; ++p)
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    As[buf][p][(o_st + q) * BT + r_st] = ra[p][q];
-                    Bs[buf][p][(o_st + q) * BT + r_st] = rb[p][q];
-                }
         };
         f32x4 acc[4][4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        fetch(0);
-        store(0);
-        if (KT < g.K) fetch(KT);
-        __syncthreads();
-        for (int k0 = 0, cur = 0; k0 < g.K; k0 += KT, cur ^= 1) {
-            // tile k0 + KT (in registers) to the other buffer, whose last
-            // readers passed the barrier that ended the previous iteration
-            if (k0 + KT < g.K) {
-                store(cur ^ 1);
-                if (k0 + 2 * KT < g.K) fetch(k0 + 2 * KT);
-            }
+        if (nfull > 0) issue(0, 0);
+        else stage_tail(0, 0);
+        for (int t = 0; t < nt; ++t) {
+            const int cur = t & 1;
+            // k-tile t's loads are the only ones outstanding: wait for them;
+            // the barrier makes them visible and ends the reads of k-tile t - 1
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+            __syncthreads();
+            if (t + 1 < nfull) issue(cur ^ 1, t + 1);
+            else if (t + 1 < nt) stage_tail(cur ^ 1, (t + 1) * KT);
             bf16x8 av[NP][4], bv[NP][4];
 #pragma unroll
             for (int p = 0; p < NP; ++p)
@@ -1624,8 +1634,9 @@ __global__ __launch_bounds__(kThreads, NSPLIT == 6 ? 1 : 2) void gemm_bf16p_kern
                     }
                     acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0][c], av[0][r], acc[r][c], 0, 0, 0);
                 }
-            __syncthreads();
         }
+        // the next tile's first loads reuse the buffers
+        __syncthreads();
         // acc[r][c][v] = C(row0 + wr + r*16 + li, col0 + wc + c*16 + 4*lg + v)
         gemm_epilogue<float, 4>(acc, C, g, row0 + wr + li, col0 + wc, lg,
                                 [](int lg_, int v_) { return 4 * lg_ + v_; });

@@ -1175,8 +1175,8 @@ constexpr int kWaitVmcnt0 = (0xF << 8) | (0x7 << 4);
 #endif
 // SPLIT: two-level accumulation over runs of 64 k, as in
 // gemm_mfma_kernel; at BT = 128 its second accumulator set gets a
-// 3-wavefront register budget (measured faster than 2 wavefronts without
-// spills).
+// 3-wavefront register budget (a few registers spill; measured faster on
+// the long-K Schur complements than 2 wavefronts without spills).
 template <int BT, bool SPLIT>
 __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_mfma32_kernel(
     const GemmJob* __restrict__ jobs, int njobs, float* __restrict__ L, float* __restrict__ U, int64_t nL, int ntiles) {
@@ -1273,34 +1273,40 @@ __global__ __launch_bounds__(kThreads, (BT == 128 && SPLIT) ? 3 : 2) void gemm_m
             const float* Ac = As[cur];
             const float* Bc = Bs[cur];
             // av[s][q][rb] = A(wr + 32 rb + j, k), bv[s][q][cb] = B(wc + NB j + cb, k),
-            // k = 4 s + 2 h + q
-            float av[KT / 4][2][NB];
-            float bv[KT / 4][2][NB];
+            // k = 4 s + 2 h + q; all of them read before the products, except
+            // with the second accumulator set (SPLIT at BT = 128), whose
+            // register budget takes one group of 4 k at a time
+            constexpr int kGroups = (SPLIT && NB == 2) ? 1 : KT / 4;
     #pragma unroll
-            for (int s2 = 0; s2 < KT / 4; ++s2)
+            for (int s0 = 0; s0 < KT / 4; s0 += kGroups) {
+                float av[kGroups][2][NB];
+                float bv[kGroups][2][NB];
     #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const int k = 4 * s2 + 2 * h + q;
+                for (int s2 = 0; s2 < kGroups; ++s2)
     #pragma unroll
-                    for (int rb = 0; rb < NB; ++rb) av[s2][q][rb] = Ac[k * BT + wr + 32 * rb + j];
-                    if constexpr (NB == 2) {
-                        const f32x2_t y = *reinterpret_cast<const f32x2_t*>(&Bc[k * BT + wc + 2 * j]);
-                        bv[s2][q][0] = y[0];
-                        bv[s2][q][1] = y[1];
-                    } else {
-                        bv[s2][q][0] = Bc[k * BT + wc + j];
+                    for (int q = 0; q < 2; ++q) {
+                        const int k = 4 * (s0 + s2) + 2 * h + q;
+    #pragma unroll
+                        for (int rb = 0; rb < NB; ++rb) av[s2][q][rb] = Ac[k * BT + wr + 32 * rb + j];
+                        if constexpr (NB == 2) {
+                            const f32x2_t y = *reinterpret_cast<const f32x2_t*>(&Bc[k * BT + wc + 2 * j]);
+                            bv[s2][q][0] = y[0];
+                            bv[s2][q][1] = y[1];
+                        } else {
+                            bv[s2][q][0] = Bc[k * BT + wc + j];
+                        }
                     }
-                }
     #pragma unroll
-            for (int s2 = 0; s2 < KT / 4; ++s2)
+                for (int s2 = 0; s2 < kGroups; ++s2)
     #pragma unroll
-                for (int q = 0; q < 2; ++q)
+                    for (int q = 0; q < 2; ++q)
     #pragma unroll
-                    for (int rb = 0; rb < NB; ++rb)
+                        for (int rb = 0; rb < NB; ++rb)
     #pragma unroll
-                        for (int cb = 0; cb < NB; ++cb)
-                            sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][q][cb], av[s2][q][rb],
-                                                                               sum[rb][cb], 0, 0, 0);
+                            for (int cb = 0; cb < NB; ++cb)
+                                sum[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(bv[s2][q][cb], av[s2][q][rb],
+                                                                                   sum[rb][cb], 0, 0, 0);
+            }
             if (SPLIT && (t % kRun == kRun - 1 || t + 1 >= nt)) {
     #pragma unroll
                 for (int rb = 0; rb < NB; ++rb)

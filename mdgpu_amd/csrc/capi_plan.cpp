@@ -22,6 +22,15 @@ PlanOptions plan_options_for(int precision) {
     // in two levels (runs of 64 k), which cuts the factor's backward error
     // after one refinement step by about 10x on the 64^3 27-point stencil
     if (precision == MDGPU_FP32) po.split_k_min = 257;
+    // MDGPU_GEMM_KERNEL: one GEMM kernel code for every launch (a kernel code
+    // of plan.h valid for the precision; tests pin each kernel with it)
+    if (const char* e = std::getenv("MDGPU_GEMM_KERNEL")) {
+        const int v = std::atoi(e);
+        if (precision != MDGPU_BF16 && precision != MDGPU_BF16X3 && gemm_kernel_valid(v, po.elem_bytes)) {
+            po.gemm_default = v;
+            po.split_k_min = 0;
+        }
+    }
     // bf16 variants: GEMMs of blocked fronts from the split planes the block
     // steps write (MDGPU_PRESPLIT=0: every bf16 GEMM converts its FP32 tiles
     // as it stages them)

@@ -239,6 +239,32 @@ def test_lookahead_outer_blocks(device, monkeypatch, nb_outer):
             assert np.array_equal(G.export_L()[2], Lx), (name, rep)
 
 
+def test_fp32_gemm_kernels_agree(device, monkeypatch):
+    """Every FP32 GEMM kernel, forced on every launch (MDGPU_GEMM_KERNEL):
+    the factor stays within FP32 rounding of the oracle's FP64 factor, and the
+    32x32x2 kernel's 128- and 64-row tiles give the same factor bit for bit
+    (the same products in the same order per entry; their k-tiles are 16 and
+    32 deep, so the partial last k-tile of the fronts whose pivot counts are
+    not multiples of 32 takes both register-staged paths), with and without
+    two-level accumulation."""
+    import torch
+    p, i, x = gen.stencil27(12, seed=1)
+    S = symbolic.analyze(p, i, True, "rn", 100000)
+    lp, Li_ref, Lx_ref, _, bad = oracle_factor(p, i, x, S)
+    ax = torch.from_numpy(x).to(device)
+    out = {}
+    for code in (129, 65, 129 | 256, 65 | 256, 128, 64, 64 | 256):
+        monkeypatch.setenv("MDGPU_GEMM_KERNEL", str(code))
+        F = Factorization(S, "fp32").factorize(ax)
+        _, Li, Lx = F.export_L()
+        assert np.array_equal(Li, Li_ref), code
+        assert rel_err(Lx, Lx_ref) <= 2e-5, (code, rel_err(Lx, Lx_ref))
+        out[code] = Lx
+    assert np.array_equal(out[129], out[65])
+    assert np.array_equal(out[129 | 256], out[65 | 256])
+    assert not np.array_equal(out[129], out[128])   # the 16x16x4 kernel rounds differently
+
+
 def test_autotune_keeps_the_factor(device):
     """The autotuner only changes GEMM tile sizes: the FP64 factor still
     matches the oracle, and graph replays after tuning reproduce it."""

@@ -474,13 +474,16 @@ def main():
     # HBM traffic per launch of the dominant kernel, measured by two
     # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled on a copy of
     # known size; the committed summary holds the figure
+    # (the summary holds one figure per FP32 partition it was measured at)
     kname = {"gemm": "fp32_gemm"}.get(dom)
     tpath = os.path.join(ROOT, "profiles", "gemm_blk_extendadd_traffic_c3_fp32.md")
+    part32 = tuning["fp32"]["budget"] if "fp32" in tuning else args.max_zeros
     if kname and args.workload == "c3" and os.path.exists(tpath):
         for line in open(tpath):
-            if line.startswith(f"traffic_bytes_per_launch {kname} ="):
+            if line.startswith(f"traffic_bytes_per_launch {kname} rn{part32} ="):
                 roof["traffic"] = float(line.split("=")[1])
-                roof["traffic_unit"] = "bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                roof["traffic_unit"] = (f"bytes per launch (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, copy-calibrated, "
+                                        f"partition rn {part32})")
                 roof["algorithmic_bytes_per_launch"] = d["bytes"] / max(d["launches"], 1)
 
     cpu = None

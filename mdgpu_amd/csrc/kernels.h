@@ -28,8 +28,8 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
 // nplanes > 0 (FP32 store): the steps also write that many bf16 split
 // planes (BlkJob::pl), plane p at planes + p * pstride
 template <typename T>
-void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
-                __bf16* planes, int64_t pstride, int nplanes, hipStream_t s);
+void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int64_t nL, int* info, int* counters,
+                int* fallbacks, __bf16* planes, int64_t pstride, int nplanes, hipStream_t s);
 template <typename T>
 void launch_inv(const InvJob* jobs, int count, const FrontDesc* fronts, const T* L, T* inv, hipStream_t s);
 // MFMA kernel of the step (64, 128 or kGemmMfma32, the step's GEMM kernel

@@ -19,6 +19,30 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// ---------------------------------------------- vector types, buffer loads
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// raw buffer resource (stride 0) over [base, base + bytes_left), clamped to
+// the 32-bit record count; a load outside returns zero
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, int64_t bytes_left) {
+    const int64_t b = bytes_left < 0 ? 0 : (bytes_left > 0xFFFFFFFFll ? 0xFFFFFFFFll : bytes_left);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(uint32_t)b, 0x00020000);
+}
+__device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    return __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+}
+// 16 bytes per lane from a buffer straight into LDS: lane l's bytes land at
+// dst + 16 l (dst is the same for the whole wavefront)
+__device__ __forceinline__ void buffer_load_lds16(__amdgpu_buffer_rsrc_t r, lds_void_ptr dst, uint32_t byte_off) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, dst, 16, byte_off, 0, 0, 0);
+}
+// s_waitcnt immediate that waits for all vector-memory operations (vmcnt 0)
+// and leaves the LDS/scalar and export counters alone
+constexpr int kWaitVmcnt0 = (0xF << 8) | (0x7 << 4);
+
 __device__ inline double to_d(double v) { return v; }
 __device__ inline double to_d(float v) { return (double)v; }
 
@@ -495,13 +519,19 @@ __device__ __forceinline__ void inv64_blocked(const T* As, const T* rd, T* Zs, T
 // updated diagonal) refactors the block in FP64 when it fails.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict__ jobs, int njobs,
-                                                       T* __restrict__ L, int* __restrict__ info,
+                                                       T* __restrict__ L, int64_t nL, int* __restrict__ info,
                                                        int* __restrict__ counters, int* __restrict__ fallbacks,
                                                        __bf16* __restrict__ planes, int64_t pstride, int nplanes) {
     typedef typename MfmaOp<T>::acc_t acc_t;
+    constexpr bool kF32 = std::is_same<T, float>::value;
     constexpr int kZ = 64 * 65;                                          // one 64 x 64 block
-    constexpr int kFb = std::is_same<T, float>::value ? 64 * 65 * 8 : 0;  // FP64 fallback block
-    constexpr int kScr = kZ * (int)sizeof(T) > kFb ? kZ * (int)sizeof(T) : kFb;
+    constexpr int kFb = kF32 ? 64 * 65 * 8 : 0;                          // FP64 fallback block
+    // FP32 K chunks: two buffers of 16 blocks of 4 columns (64 rows each,
+    // 16 floats of padding per block), column c in block c % 16, slot c / 16
+    constexpr int kChBlk = 4 * 64 + 16, kChunk = 16 * kChBlk;
+    constexpr int kCh = kF32 ? 2 * kChunk * 4 : 0;
+    constexpr int kScr0 = kZ * (int)sizeof(T) > kFb ? kZ * (int)sizeof(T) : kFb;
+    constexpr int kScr = kScr0 > kCh ? kScr0 : kCh;
     __shared__ __align__(16) T As[64 * 65];   // column-major A(r, c) = As[r + 65 c]
     // K chunks of the update, then the FP64 fallback block or the inverses
     // of the diagonal 16x16 blocks (same layout as As)
@@ -535,7 +565,67 @@ __global__ __launch_bounds__(kThreads) void blk_kernel(const BlkJob* __restrict_
             Rv[N][v] = g[rrow + col * ld];
             Dv[N][v] = g[drow + col * ld];
         }
-    if (K > 0) {
+    if (kF32 && K > 0) {
+        // the chunk's diagonal-block rows go from global memory straight into
+        // LDS (buffer_load_dwordx4 ... lds, double-buffered); wavefront w
+        // loads blocks 4 w .. 4 w + 3, lane l taking rows 4 (l % 16) .. + 3
+        // of column j + 16 (l / 16) of block j.  Rows past the block width b
+        // read the panel's next rows (or zeros past the L store's end): they
+        // reach only entries the step does not keep.  The padding puts the
+        // four columns a k-step reads in different banks.
+        const T* __restrict__ gp = g - (int64_t)K * ld;   // the outer block's earlier columns
+        acc_t aR[4], aD[4];
+#pragma unroll
+        for (int N = 0; N < 4; ++N) {
+            aR[N] = acc_t{T(0), T(0), T(0), T(0)};
+            aD[N] = acc_t{T(0), T(0), T(0), T(0)};
+        }
+        T* __restrict__ Wc = reinterpret_cast<T*>(scr);
+        auto wpos = [&](int r, int c) { return (c & 15) * kChBlk + (c >> 4) * 64 + r; };
+        T xr[16];
+        auto issue = [&](int buf, int kc) {
+            const T* base = gp + (int64_t)kc * ld;
+            const auto rs = buffer_rsrc(base, (nL - (base - L)) * (int64_t)sizeof(T));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int j = 4 * wave + i, c = j + 16 * (lane >> 4);
+                buffer_load_lds16(rs, (lds_void_ptr)&scr[(buf * kChunk + j * kChBlk) * 4],
+                                  (uint32_t)(4 * (lane & 15) + (int64_t)c * ld) * 4u);
+            }
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) xr[ks] = gp[rrow + (int64_t)(kc + 4 * ks + lg) * ld];
+        };
+        issue(0, 0);
+        for (int kc = 0, buf = 0; kc < K; kc += 64, buf ^= 1) {
+            // this chunk's loads are the only ones outstanding; the barrier
+            // makes them visible and ends the previous chunk's reads
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+            __syncthreads();
+            T xc[16];
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) xc[ks] = xr[ks];
+            if (kc + 64 < K) issue(buf ^ 1, kc + 64);
+            const T* __restrict__ W = Wc + buf * kChunk;
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) {
+                const T xd = W[wpos(16 * wave + li, 4 * ks + lg)];
+#pragma unroll
+                for (int N = 0; N < 4; ++N) {
+                    const T bf = W[wpos(16 * N + li, 4 * ks + lg)];
+                    aR[N] = MfmaOp<T>::mma(bf, xc[ks], aR[N]);
+                    aD[N] = MfmaOp<T>::mma(bf, xd, aD[N]);
+                }
+            }
+        }
+#pragma unroll
+        for (int N = 0; N < 4; ++N)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                Rv[N][v] -= aR[N][v];
+                Dv[N][v] -= aD[N][v];
+            }
+        __syncthreads();   // the last chunk's readers are done
+    } else if (K > 0) {
         const T* __restrict__ gp = g - (int64_t)K * ld;   // the outer block's earlier columns
         acc_t aR[4], aD[4];
 #pragma unroll
@@ -1150,23 +1240,6 @@ __global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 3 : 2) void gemm_mfma_ke
 //    (with the second accumulator set, one group of 4 k at a time).
 // Same k order per entry as a sequential sum over k-pairs; the rounding
 // differs from gemm_mfma_kernel's (pairs of k per instruction).
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
-typedef __attribute__((address_space(3))) void* lds_void_ptr;
-
-// raw buffer resource (stride 0) over [base, base + bytes_left), clamped to
-// the 32-bit record count; a load outside returns zero
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, int64_t bytes_left) {
-    const int64_t b = bytes_left < 0 ? 0 : (bytes_left > 0xFFFFFFFFll ? 0xFFFFFFFFll : bytes_left);
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(uint32_t)b, 0x00020000);
-}
-__device__ __forceinline__ f32x4_t buffer_load4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-    return __builtin_bit_cast(f32x4_t, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
-}
-// s_waitcnt immediate that waits for all vector-memory operations (vmcnt 0)
-// and leaves the LDS/scalar and export counters alone
-constexpr int kWaitVmcnt0 = (0xF << 8) | (0x7 << 4);
 
 // k-tile depth by tile size: 16 for 128-row tiles (32 was measured 2-3 %
 // slower on the large updates), 32 for 64-row tiles (1-10 % faster than 16:
@@ -2405,10 +2478,10 @@ void launch_panel(const int* list, int count, const FrontDesc* fronts, T* L, int
                                                                                  fallbacks);
 }
 template <typename T>
-void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int* info, int* counters, int* fallbacks,
-                __bf16* planes, int64_t pstride, int nplanes, hipStream_t s) {
+void launch_blk(const BlkJob* jobs, int njobs, int nblocks, T* L, int64_t nL, int* info, int* counters,
+                int* fallbacks, __bf16* planes, int64_t pstride, int nplanes, hipStream_t s) {
     if (nblocks > 0)
-        blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, info, counters, fallbacks, planes, pstride,
+        blk_kernel<T><<<nblocks, kThreads, 0, s>>>(jobs, njobs, L, nL, info, counters, fallbacks, planes, pstride,
                                                    nplanes);
 }
 template <typename T>
@@ -2582,7 +2655,8 @@ void launch_logdiag(const int64_t* diag_off, const T* L, int n, double* partial,
     template void launch_scatter_a<T>(T*, const double*, const int64_t*, int, hipStream_t);                    \
     template void launch_extend_add<T>(const EaItem*, int, const EaContrib*, const int*, T*, T*, hipStream_t); \
     template void launch_panel<T>(const int*, int, const FrontDesc*, T*, int*, int, int*, int*, int*, hipStream_t); \
-    template void launch_blk<T>(const BlkJob*, int, int, T*, int*, int*, int*, __bf16*, int64_t, int, hipStream_t); \
+    template void launch_blk<T>(const BlkJob*, int, int, T*, int64_t, int*, int*, int*, __bf16*, int64_t, int,       \
+                                hipStream_t);                                                                      \
     template void launch_inv<T>(const InvJob*, int, const FrontDesc*, const T*, T*, hipStream_t);             \
     template void launch_inv256<T>(const InvJob*, int, const FrontDesc*, const T*, const T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmJob*, int, int, int, int, T*, int64_t, T*, hipStream_t, int);                               \

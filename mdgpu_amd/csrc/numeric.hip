@@ -287,7 +287,7 @@ int run_factor(mdgpu_numeric* N, const double* ax, hipStream_t s, hipEvent_t* ev
                 panel_step++;
                 break;
             case mdgpu::STEP_BLK:
-                mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, N->dInfo, N->dBlkCnt,
+                mdgpu::launch_blk<T>(N->dBlk + st.first, st.count, st.nblocks, L, P.lstore, N->dInfo, N->dBlkCnt,
                                      N->dInfo + 1, N->dPlanes, N->plane_stride, N->dPlanes ? N->po.nplanes : 0, s);
                 break;
             case mdgpu::STEP_GEMM: {

@@ -420,6 +420,7 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         // columns [0, u_done[f]) of a blocked front are applied to U_f
         std::vector<int> u_done(P.nfronts, 0);
         int side_last = -1;   // event of the last (b) launch (side stream 1)
+        int side_waited = -1; // the latest side_last a main-stream (a) launch waited for
         int u_last = -1;      // event of the last update-matrix launch (side stream 2)
         for (int R0 = 0; R0 < maxc; R0 += o.nb_outer) {
             for (int r0 = R0; r0 < std::min(R0 + o.nb_outer, maxc); r0 += o.nb) {
@@ -529,7 +530,10 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
             }
             const int wait_prev = side_last;
             const int gk = o.presplit ? (o.gemm_default | kGemmPlanes) : o.gemm_default;
-            if (flush_gemm(batch_a, L, gk) && wait_prev >= 0) P.steps.back().wait_ev = wait_prev;
+            if (flush_gemm(batch_a, L, gk) && wait_prev >= 0) {
+                P.steps.back().wait_ev = wait_prev;
+                side_waited = wait_prev;
+            }
             if (flush_gemm(batch_b, L, gk)) {
                 Step& sb = P.steps.back();
                 sb.stream = 1;
@@ -548,8 +552,10 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
         // (with u_lookahead, for a blocked front only the columns from
         // u_done[f] on: the earlier ones went to side stream 2).  The first
         // main-stream launch after the blocked part waits for both side
-        // streams (see the join pass after the loop).
-        if (side_last >= 0) joins.push_back({P.steps.size(), side_last});
+        // streams (see the join pass after the loop); side stream 1 only if
+        // its last launch is not already waited for by an (a) launch (each
+        // front's last non-empty (b) is followed by an (a) that waits for it)
+        if (side_last >= 0 && side_last != side_waited) joins.push_back({P.steps.size(), side_last});
         if (u_last >= 0) joins.push_back({P.steps.size(), u_last});
         {
             // presplit: the launch reads the split planes when every front

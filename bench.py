@@ -317,8 +317,8 @@ def main():
     ap.add_argument("--mode", default="rn")
     ap.add_argument("--max-zeros", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-12)
-    ap.add_argument("--cpu-budget-gflop", type=float, default=60.0,
-                    help="flops of the CPU baseline sample (about 12 s of one core at 5 GFLOP/s)")
+    ap.add_argument("--cpu-budget-gflop", type=float, default=75.0,
+                    help="flops of the CPU baseline sample (about 11-15 s of one core at 5-7 GFLOP/s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aniso", action="store_true", help="skip the z-anisotropic C3 variant")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile sizes")

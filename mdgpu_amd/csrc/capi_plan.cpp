@@ -41,7 +41,7 @@ PlanOptions plan_options_for(int precision) {
     // (no second side stream)
     if (const char* e = std::getenv("MDGPU_U_LOOKAHEAD")) po.u_lookahead = std::atoi(e) != 0;
     if (const char* e = std::getenv("MDGPU_U_STRIDE")) po.u_stride = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("MDGPU_U_FLUSH")) po.u_flush = std::atoi(e) != 0;
+    if (const char* e = std::getenv("MDGPU_U_FLUSH")) po.u_flush = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("MDGPU_SCHUR_KERNEL")) po.schur_kernel = std::atoi(e);
     // most pivots of a front factored in LDS by one workgroup (at most 128)
     if (const char* e = std::getenv("MDGPU_PANEL_MAXC")) po.panel_max_pivots = std::max(0, std::atoi(e));

@@ -505,12 +505,13 @@ std::string build_plan(const Symbolic& S, const PlanOptions& o, Plan& P) {
                     h.N = d.c - E2;
                     batch_b.push_back(h);
                 }
-                // (u) once u_stride blocks are pending, and (u_flush) at the
-                // block before the front's last, so that the Schur launch
-                // after the join is left at most one outer block
-                const bool last_next = E + o.nb_outer >= d.c;
+                // (u) once u_stride blocks are pending, and (u_flush) at each
+                // of the last u_flush blocks before the front's last, so that
+                // the Schur launch after the join is left at most one outer
+                // block and the (u) launch it waits for at most one more
+                const int blocks_left = (d.c - E + o.nb_outer - 1) / o.nb_outer;
                 if (o.u_lookahead && d.m > 0 &&
-                    (E - u_done[f] >= std::max(1, o.u_stride) * o.nb_outer || (o.u_flush && last_next))) {
+                    (E - u_done[f] >= std::max(1, o.u_stride) * o.nb_outer || blocks_left <= o.u_flush)) {
                     const int k0 = u_done[f];
                     GemmJob u{};
                     u.a_off = d.lofs + (int64_t)k0 * d.n + d.c;

@@ -301,8 +301,10 @@ struct PlanOptions {
     // ... applied every u_stride outer blocks (K = u_stride * nb_outer per
     // launch): a deeper K reads and writes U_f fewer times
     int u_stride = 2;
-    // ... and at the outer block before a front's last one
-    bool u_flush = true;
+    // ... and at each of the u_flush outer blocks before a front's last one
+    // (0: none), so that the launch after the next-to-last block, which the
+    // Schur launch has to wait for, holds one block's columns
+    int u_flush = 1;
     // bf16 variants: the block steps also write their columns as bf16 split
     // planes, and the GEMMs of the blocked fronts read them (kGemmPlanes)
     bool presplit = false;

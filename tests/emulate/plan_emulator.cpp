@@ -101,21 +101,23 @@ extern "C" int emulate_factor(int n, const int* Cp, const int* Ci, const double*
 }
 
 static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode, int max_zeros,
-                   int nb_outer, bool presplit, int u_stride, double* Lx_out, int* Li_out, double* Zx_out);
+                   int nb_outer, bool presplit, int u_stride, int u_flush, double* Lx_out, int* Li_out,
+                   double* Zx_out);
 
 extern "C" int emulate_factor_selinv(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                                      int max_zeros, int nb_outer, double* Lx_out, int* Li_out, double* Zx_out) {
-    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, false, PlanOptions().u_stride, Lx_out, Li_out,
-                   Zx_out);
+    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, false, PlanOptions().u_stride,
+                   PlanOptions().u_flush, Lx_out, Li_out, Zx_out);
 }
 
-// update-matrix look-ahead every u_stride outer blocks, with (presplit) or
-// without the split planes
+// update-matrix look-ahead every u_stride outer blocks and at the last
+// u_flush blocks before a front's last, with (presplit) or without the
+// split planes
 extern "C" int emulate_factor_ustride(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
-                                      int max_zeros, int nb_outer, int u_stride, int presplit, double* Lx_out,
-                                      int* Li_out) {
-    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, presplit != 0, u_stride, Lx_out, Li_out,
-                   nullptr);
+                                      int max_zeros, int nb_outer, int u_stride, int u_flush, int presplit,
+                                      double* Lx_out, int* Li_out) {
+    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, presplit != 0, u_stride, u_flush, Lx_out,
+                   Li_out, nullptr);
 }
 
 // The schedule of the bf16 variants (block steps that also write the split
@@ -125,12 +127,13 @@ extern "C" int emulate_factor_ustride(int n, const int* Cp, const int* Ci, const
 // column is written as NaN).
 extern "C" int emulate_factor_presplit(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode,
                                        int max_zeros, int nb_outer, double* Lx_out, int* Li_out) {
-    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, true, PlanOptions().u_stride, Lx_out, Li_out,
-                   nullptr);
+    return emulate(n, Cp, Ci, ax, use_amd, mode, max_zeros, nb_outer, true, PlanOptions().u_stride,
+                   PlanOptions().u_flush, Lx_out, Li_out, nullptr);
 }
 
 static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int use_amd, int mode, int max_zeros,
-                   int nb_outer, bool presplit, int u_stride, double* Lx_out, int* Li_out, double* Zx_out) {
+                   int nb_outer, bool presplit, int u_stride, int u_flush, double* Lx_out, int* Li_out,
+                   double* Zx_out) {
     AnalyzeOptions ao;
     ao.use_amd = use_amd;
     ao.partition = (Partition)mode;
@@ -146,6 +149,7 @@ static int emulate(int n, const int* Cp, const int* Ci, const double* ax, int us
     po.nb_outer = nb_outer < 0 ? -nb_outer : nb_outer;
     po.u_lookahead = nb_outer > 0;
     po.u_stride = u_stride;
+    po.u_flush = u_flush;
     // outer width 128 also sends fronts of up to 128 pivots to the LDS panel
     // kernel (its upper bound; the default keeps it to the narrow fronts)
     if (nb_outer == 128) po.panel_max_pivots = kPanelMaxPivots;

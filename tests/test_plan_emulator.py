@@ -79,11 +79,12 @@ def test_presplit_schedule_reproduces_oracle(emulator, name, make, mode, mz, nb_
     assert rel_err(Lx, lx) < 1e-12
 
 
-@pytest.mark.parametrize("u_stride,presplit", [(1, 0), (3, 1)])
+@pytest.mark.parametrize("u_stride,u_flush,presplit", [(1, 1, 0), (3, 1, 1), (2, 2, 0), (3, 3, 1), (2, 0, 0)])
 @pytest.mark.parametrize("name,make,mode,mz", CASES[1:5], ids=[c[0] for c in CASES[1:5]])
-def test_update_matrix_stride_reproduces_oracle(emulator, name, make, mode, mz, u_stride, presplit):
+def test_update_matrix_stride_reproduces_oracle(emulator, name, make, mode, mz, u_stride, u_flush, presplit):
     """Update-matrix look-ahead applied every u_stride outer blocks (64
-    columns), with and without the split planes."""
+    columns) and at the last u_flush blocks before a front's last, with and
+    without the split planes."""
     p, i, x = make()
     S = symbolic.analyze(p, i, True, ["plain", "sn", "rn"][mode], mz)
     lp, li, lx, _, _ = oracle_factor(p, i, x, S)
@@ -91,7 +92,7 @@ def test_update_matrix_stride_reproduces_oracle(emulator, name, make, mode, mz, 
     Lx = np.zeros(S.lnz)
     Li = np.zeros(S.lnz, np.int32)
     st = emulator.emulate_factor_ustride(len(p) - 1, p.ctypes.data_as(I), i.ctypes.data_as(I),
-                                         x.ctypes.data_as(D), 1, mode, mz, 64, u_stride, presplit,
+                                         x.ctypes.data_as(D), 1, mode, mz, 64, u_stride, u_flush, presplit,
                                          Lx.ctypes.data_as(D), Li.ctypes.data_as(I))
     if st == 4:
         pytest.skip("no blocked front (no split planes)")
